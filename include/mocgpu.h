@@ -1,0 +1,166 @@
+/*
+ * mocgpu.h -- C ABI of libmocgpu, the MI355X (gfx950) implementation of one
+ * MOCASSIN Monte Carlo iteration: packet emission, packet transport with the
+ * Lucy path-length J_nu estimator, the per-cell opacity, the per-cell diffuse
+ * emission CDF and the per-cell ionization/thermal balance.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns
+ * MOC_OK (0) on success and a negative MOC_ERR_* code on failure; the message
+ * of the last failure is returned by moc_last_error().  The library owns all
+ * device memory; host pointers are read or written only during the call.
+ * Calls on one context are blocking at return and not reentrant.
+ *
+ * Indexing follows MOCASSIN: cells are numbered 1..nCells by the `active`
+ * map (row 0 is the inactive dummy cell) and frequency bins 1..nbins in the
+ * Fortran arrays; in this ABI the bin index is 0-based (bin b here is bin b+1
+ * of the Fortran arrays).
+ *
+ * Reference interfaces replaced (rwesson/mocassin_xray, source/):
+ *   moc_transport   <- energyPacketDriver(iStar,n,grid,plot,gpLoc,cellLoc)
+ *                      photon_mod.f90:28 (with energyPacketRun :418,
+ *                      newPhotonPacket :976, pathSegment :1758,
+ *                      comptonScatter :6805 as device functions)
+ *   moc_opacity     <- ionizationDriver(grid,ix,iy,iz) ionization_mod.f90:26,
+ *                      called per cell from iterateMC iteration_mod.f90:148
+ *   moc_emission    <- emissionDriver(grids,ix,iy,iz,ig) emission_mod.f90:36
+ *                      (setDiffusePDF :1528 -> recPDF, totalLines)
+ *   moc_update      <- updateCell(grid,xP,yP,zP) update_mod.f90:15
+ *   moc_normalize_j <- the J reduction/normalisation of iterateMC
+ *                      iteration_mod.f90:944-1031
+ *   moc_upload/moc_download <- direct access to grid_type components
+ *                      common_mod.f90:308-371
+ */
+#ifndef MOCGPU_H
+#define MOCGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MOC_OK 0
+#define MOC_ERR_ARG (-1)    /* invalid argument or size */
+#define MOC_ERR_HIP (-2)    /* HIP runtime failure */
+#define MOC_ERR_STATE (-3)  /* call out of order (e.g. no grid attached) */
+#define MOC_ERR_KERNEL (-4) /* a kernel reported an insane packet/cell state */
+#define MOC_ERR_NOMEM (-5)  /* device allocation failed */
+
+/* arithmetic of the transport path (chosen at moc_create) */
+#define MOC_PREC_FP64 0  /* positions, opacity, CDFs and path sums in FP64 */
+#define MOC_PREC_MIXED 1 /* FP32 positions/opacity/CDFs, FP64 J accumulation */
+
+/* element type of a host buffer passed to moc_upload/moc_download */
+#define MOC_DT_F32 0
+#define MOC_DT_F64 1
+#define MOC_DT_I32 2
+
+/* layout of a (cell, bin) host buffer */
+#define MOC_LAYOUT_FORTRAN 0 /* (0:nCells, 1:nbins), cell index fastest */
+#define MOC_LAYOUT_CELLROW 1 /* (1:nbins, 0:nCells), bin index fastest  */
+
+/* per-cell fields (shape (nCells+1) unless noted) */
+#define MOC_F_OPACITY 1    /* kappa_nu [cm^-1], (cell, bin) */
+#define MOC_F_JSTE 2       /* J estimator, (cell, bin), FP64 on device */
+#define MOC_F_RECPDF 3     /* diffuse emission CDF, (cell, bin) */
+#define MOC_F_TOTALLINES 4 /* fraction of diffuse energy in lines */
+#define MOC_F_NE 5         /* electron density [cm^-3] */
+#define MOC_F_TE 6         /* electron temperature [K] */
+#define MOC_F_HDEN 7       /* hydrogen density [cm^-3] */
+#define MOC_F_ESCAPED 8    /* escapedPackets(origin cell, bin), angle bin 0 */
+#define MOC_F_IONDEN 9     /* ion fractions (cell, elem, stage): ionDen */
+#define MOC_F_CONVERGED 10 /* lgConverged (int32) */
+#define MOC_F_BLACK 11     /* lgBlack (int32) */
+#define MOC_F_EMISSION 12  /* diffuse continuum emissivity (cell, bin) */
+
+typedef struct moc_ctx moc_ctx;
+
+/* Frequency mesh and source tables (photon_mod.f90 globals, xSec_mod). */
+typedef struct moc_tables {
+    int32_t nbins;             /* B */
+    int32_t precision;         /* MOC_PREC_* */
+    const double* nuArray;     /* [B] bin frequencies [Ryd] */
+    const double* widFlx;      /* [B] bin widths [Ryd] */
+    double ionEdge1;           /* lowest ionisation edge [Ryd] (ionEdge(1)) */
+    int32_t lgCompton;         /* Compton scattering of packets on free e- */
+    int32_t nAngles;           /* angular bins of the KN tables (180) */
+    const double* KNsigmaT;    /* [B] total Klein-Nishina sigma [cm^2] */
+    const double* KNsigmaCDF;  /* [B][nAngles] scattering-angle CDF */
+    const double* Pcomp;       /* [B][nAngles] nu_out/nu_in */
+    int32_t nStars;            /* point sources (>=1) */
+    const double* sourceCDF;   /* [nStars][B] inSpectrumProbDen */
+    const double* Lstar;       /* [nStars] luminosity [1e36 erg/s] */
+    const double* nPhotons;    /* [nStars] packets per iteration (deltaE=L/N) */
+    const double* starPosition;/* [nStars][3] [cm] */
+    const int32_t* starIndex;  /* [nStars][3] 1-based cell indices (xP,yP,zP) */
+} moc_tables;
+
+/* One Cartesian grid (grid_type, common_mod.f90:308-371). */
+typedef struct moc_grid_desc {
+    int32_t nx, ny, nz;
+    const double* xAxis;      /* [nx] cell centres [cm] */
+    const double* yAxis;      /* [ny] */
+    const double* zAxis;      /* [nz] */
+    const int32_t* active;    /* [nx*ny*nz] column-major (x fastest); 1..nCells or 0 */
+    int32_t nCells;
+    int32_t lgSymmetricXYZ;   /* octant model with mirror walls at the lower faces */
+    double R_out;             /* outer radius [cm]; <= 0 disables the test */
+} moc_grid_desc;
+
+/* Counters of one moc_transport call (photon_mod.f90 Qphot, trapped, ...). */
+typedef struct moc_transport_stats {
+    double Qphot;            /* ionising photons emitted [1e36/(erg) units as reference] */
+    double segments;         /* cell-crossing segments walked */
+    double emissions;        /* packet emission events (incl. re-emissions) */
+    double absorbed;         /* absorption events */
+    double scattered;        /* Compton scattering events */
+    double escaped;          /* packets that left the grid (energy tallied) */
+    double lines;            /* packets converted to line photons (end of life) */
+    double lost;             /* packets dropped by a wall snap or safe limit */
+    double trapped;          /* packets that hit the re-emission limit */
+    double kernel_ms;        /* device time of the transport kernel(s) [ms] */
+} moc_transport_stats;
+
+/* Create a context: uploads the mesh and source tables. */
+int moc_create(moc_ctx** ctx, const moc_tables* tables);
+void moc_destroy(moc_ctx* ctx);
+const char* moc_last_error(const moc_ctx* ctx);
+
+/* Attach the (single) Cartesian grid; allocates all per-cell device arrays
+ * and zeroes them. */
+int moc_grid_attach(moc_ctx* ctx, int ig, const moc_grid_desc* grid);
+
+/* Copy a per-cell field between host and device.  `count` is the number of
+ * elements of the host buffer; (cell, bin) fields take `layout`. */
+int moc_upload(moc_ctx* ctx, int field, int ig, const void* host, int64_t count,
+               int dtype, int layout);
+int moc_download(moc_ctx* ctx, int field, int ig, void* host, int64_t count,
+                 int dtype, int layout);
+
+/* Zero the J estimator and the escape tallies (iteration_mod.f90:626-717). */
+int moc_reset_tallies(moc_ctx* ctx, int ig);
+
+/* Transport n packets of source iStar (>=1) through the grid, accumulating
+ * J and escape tallies on the device.  The counter-based RNG stream of
+ * packet p is Philox4x32-10 with key = seed and counter = (draw, p_lo, p_hi,
+ * ctr_base): results do not depend on the launch shape. */
+int moc_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed,
+                  uint32_t ctr_base, moc_transport_stats* stats);
+
+/* J <- J * 1e-9 (and /8 for symmetric grids), escape tallies /8 likewise
+ * (iteration_mod.f90:1002-1031). */
+int moc_normalize_j(moc_ctx* ctx, int ig);
+
+/* Device pointer of a field, for callers that keep data in HBM. */
+int moc_device_ptr(moc_ctx* ctx, int field, int ig, void** dptr, int64_t* count,
+                   int* dtype);
+
+/* Library self-description: "gfx950 mocgpu <version>". */
+const char* moc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOCGPU_H */

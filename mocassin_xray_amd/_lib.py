@@ -1,0 +1,115 @@
+"""ctypes binding of libmocgpu (include/mocgpu.h).
+
+The library is loaded from the package directory; a missing or unloadable
+library raises at once -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libmocgpu.so"
+
+MOC_OK = 0
+MOC_PREC_FP64 = 0
+MOC_PREC_MIXED = 1
+MOC_DT_F32, MOC_DT_F64, MOC_DT_I32 = 0, 1, 2
+MOC_LAYOUT_FORTRAN, MOC_LAYOUT_CELLROW = 0, 1
+
+FIELDS = {
+    "opacity": 1, "Jste": 2, "recPDF": 3, "totalLines": 4, "Ne": 5, "Te": 6,
+    "Hden": 7, "escaped": 8, "ionDen": 9, "lgConverged": 10, "lgBlack": 11,
+    "emission": 12,
+}
+CELLBIN_FIELDS = {"opacity", "Jste", "recPDF", "escaped", "emission"}
+
+# every symbol include/mocgpu.h declares
+EXPORTS = [
+    "moc_create", "moc_destroy", "moc_last_error", "moc_grid_attach", "moc_upload",
+    "moc_download", "moc_reset_tallies", "moc_transport", "moc_normalize_j",
+    "moc_device_ptr", "moc_version",
+]
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class MocTables(C.Structure):
+    _fields_ = [
+        ("nbins", C.c_int32), ("precision", C.c_int32),
+        ("nuArray", _dp), ("widFlx", _dp), ("ionEdge1", C.c_double),
+        ("lgCompton", C.c_int32), ("nAngles", C.c_int32),
+        ("KNsigmaT", _dp), ("KNsigmaCDF", _dp), ("Pcomp", _dp),
+        ("nStars", C.c_int32), ("sourceCDF", _dp), ("Lstar", _dp), ("nPhotons", _dp),
+        ("starPosition", _dp), ("starIndex", _ip),
+    ]
+
+
+class MocGridDesc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("xAxis", _dp), ("yAxis", _dp), ("zAxis", _dp), ("active", _ip),
+        ("nCells", C.c_int32), ("lgSymmetricXYZ", C.c_int32), ("R_out", C.c_double),
+    ]
+
+
+class MocTransportStats(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "Qphot", "segments", "emissions", "absorbed", "scattered", "escaped", "lines",
+        "lost", "trapped", "kernel_ms")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+_LIB = None
+
+
+def lib() -> C.CDLL:
+    """Load libmocgpu.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    L = C.CDLL(str(LIB_PATH))
+    vp = C.c_void_p
+    L.moc_create.argtypes = [C.POINTER(vp), C.POINTER(MocTables)]
+    L.moc_create.restype = C.c_int
+    L.moc_destroy.argtypes = [vp]
+    L.moc_destroy.restype = None
+    L.moc_last_error.argtypes = [vp]
+    L.moc_last_error.restype = C.c_char_p
+    L.moc_grid_attach.argtypes = [vp, C.c_int, C.POINTER(MocGridDesc)]
+    L.moc_grid_attach.restype = C.c_int
+    L.moc_upload.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int64, C.c_int, C.c_int]
+    L.moc_upload.restype = C.c_int
+    L.moc_download.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int64, C.c_int, C.c_int]
+    L.moc_download.restype = C.c_int
+    L.moc_reset_tallies.argtypes = [vp, C.c_int]
+    L.moc_reset_tallies.restype = C.c_int
+    L.moc_transport.argtypes = [vp, C.c_int, C.c_int64, C.c_uint64, C.c_uint32,
+                                C.POINTER(MocTransportStats)]
+    L.moc_transport.restype = C.c_int
+    L.moc_normalize_j.argtypes = [vp, C.c_int]
+    L.moc_normalize_j.restype = C.c_int
+    L.moc_device_ptr.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp), C.POINTER(C.c_int64),
+                                 C.POINTER(C.c_int)]
+    L.moc_device_ptr.restype = C.c_int
+    L.moc_version.argtypes = []
+    L.moc_version.restype = C.c_char_p
+    _LIB = L
+    return L
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)

@@ -1,0 +1,122 @@
+// Internal state of a libmocgpu context (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mocgpu.h"
+
+#define MOC_RECURSION_LIMIT 100000   // re-emissions per packet (constants_mod.f90:70)
+#define MOC_SAFE_LIMIT 500000        // segments per pathSegment call (photon_mod.f90:1873-1878)
+
+// Device counters of one transport launch (doubles accumulated as u64).
+enum {
+    MOC_CNT_SEGMENTS = 0,
+    MOC_CNT_EMISSIONS,
+    MOC_CNT_ABSORBED,
+    MOC_CNT_SCATTERED,
+    MOC_CNT_ESCAPED,
+    MOC_CNT_LINES,
+    MOC_CNT_LOST,
+    MOC_CNT_TRAPPED,
+    MOC_CNT_ERROR,      // insane index / state (Fortran `stop` in the reference)
+    MOC_CNT_NEXT,       // work-queue head
+    MOC_CNT_N
+};
+
+// Transport view of a grid in arithmetic type T (see transport.hip).
+template <typename T>
+struct MocTransportArgs {
+    // grid
+    int nx, ny, nz, nCells, sym;
+    const T* ax;        // [nx] axes
+    const T* ay;
+    const T* az;
+    const T* vx;        // [nx] cell widths / 1e15 (volume factors)
+    const T* vy;
+    const T* vz;
+    const int* active;  // [nx*ny*nz], x fastest
+    T R_out;
+    // mesh & tables
+    int nbins, nAngles, lgCompton;
+    T ionEdge1;
+    const T* nuArray;     // [B]
+    const T* KNsigmaT;    // [B]
+    const T* KNcdf;       // [B][nAngles]
+    const T* Pcomp;       // [B][nAngles]
+    const T* sourceCDF;   // [B] of this source
+    // per-cell
+    const T* opacity;     // [B][C+1]   (cell fastest)
+    const T* Ne;          // [C+1]
+    const T* recPDF;      // [C+1][B]   (bin fastest)
+    const T* totalLines;  // [C+1]
+    // tallies
+    double* J;            // [B][C+1]
+    double* escaped;      // [B][C+1]
+    double* Qphot;        // [1]
+    unsigned long long* counters;  // [MOC_CNT_N]
+    // source
+    T star[3];
+    int starIdx[3];       // 0-based
+    T deltaE;
+    // work
+    int64_t n;
+    uint64_t seed;
+    uint32_t ctr_base;
+};
+
+struct MocGrid {
+    bool attached = false;
+    int nx = 0, ny = 0, nz = 0, nCells = 0, sym = 0;
+    double R_out = 0.0;
+    std::vector<double> ax, ay, az;   // host copies
+    std::vector<int> active;
+    // device arrays (FP64 master copies)
+    double *d_ax = nullptr, *d_ay = nullptr, *d_az = nullptr;
+    double *d_vx = nullptr, *d_vy = nullptr, *d_vz = nullptr;
+    int* d_active = nullptr;
+    double* d_opacity = nullptr;     // [B][C+1]
+    double* d_Ne = nullptr;          // [C+1]
+    double* d_Te = nullptr;          // [C+1]
+    double* d_Hden = nullptr;        // [C+1]
+    double* d_recPDF = nullptr;      // [C+1][B]
+    double* d_totalLines = nullptr;  // [C+1]
+    double* d_J = nullptr;           // [B][C+1]
+    double* d_escaped = nullptr;     // [B][C+1]
+    // FP32 shadow copies for the mixed path (refreshed when dirty)
+    float *f_ax = nullptr, *f_ay = nullptr, *f_az = nullptr;
+    float *f_vx = nullptr, *f_vy = nullptr, *f_vz = nullptr;
+    float* f_opacity = nullptr;
+    float* f_Ne = nullptr;
+    float* f_recPDF = nullptr;
+    float* f_totalLines = nullptr;
+    bool f32_dirty = true;
+};
+
+struct moc_ctx {
+    std::string err;
+    int precision = MOC_PREC_FP64;
+    int nbins = 0, nAngles = 180, lgCompton = 1, nStars = 0;
+    double ionEdge1 = 0.0;
+    std::vector<double> nuArray, widFlx, Lstar, nPhotons, starPos;
+    std::vector<int> starIdx;
+    // device tables
+    double *d_nu = nullptr, *d_KNsigmaT = nullptr, *d_KNcdf = nullptr, *d_Pcomp = nullptr;
+    double* d_sourceCDF = nullptr;  // [nStars][B]
+    float *f_nu = nullptr, *f_KNsigmaT = nullptr, *f_KNcdf = nullptr, *f_Pcomp = nullptr;
+    float* f_sourceCDF = nullptr;
+    double* d_Qphot = nullptr;
+    unsigned long long* d_counters = nullptr;
+    MocGrid grid;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+// launchers (transport.hip)
+int moc_launch_transport_f64(moc_ctx* ctx, const MocTransportArgs<double>& a, float* ms);
+int moc_launch_transport_f32(moc_ctx* ctx, const MocTransportArgs<float>& a, float* ms);
+void moc_convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);
+void moc_scale_f64(double* p, int64_t n, double f, hipStream_t s);

@@ -1,0 +1,462 @@
+// C ABI of libmocgpu: context, grid attachment, field upload/download and the
+// transport entry point.  See include/mocgpu.h for the contract.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "moc_internal.h"
+
+namespace {
+
+#define MOC_TRY(call, what)                                                              \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            ctx->err = std::string(what) + ": " + hipGetErrorString(e_);                 \
+            return e_ == hipErrorOutOfMemory ? MOC_ERR_NOMEM : MOC_ERR_HIP;              \
+        }                                                                                \
+    } while (0)
+
+template <typename P>
+void dfree(P*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+int dalloc(moc_ctx* ctx, void** p, size_t bytes, const char* what) {
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        ctx->err = std::string("hipMalloc ") + what + ": " + hipGetErrorString(e);
+        *p = nullptr;
+        return MOC_ERR_NOMEM;
+    }
+    return MOC_OK;
+}
+
+template <typename P>
+int dalloc_n(moc_ctx* ctx, P** p, size_t n, const char* what) {
+    return dalloc(ctx, (void**)p, n * sizeof(P), what);
+}
+
+int h2d(moc_ctx* ctx, void* d, const void* h, size_t bytes, const char* what) {
+    MOC_TRY(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream), what);
+    MOC_TRY(hipStreamSynchronize(ctx->stream), what);
+    return MOC_OK;
+}
+
+void free_grid(MocGrid& g) {
+    dfree(g.d_ax); dfree(g.d_ay); dfree(g.d_az);
+    dfree(g.d_vx); dfree(g.d_vy); dfree(g.d_vz);
+    dfree(g.d_active);
+    dfree(g.d_opacity); dfree(g.d_Ne); dfree(g.d_Te); dfree(g.d_Hden);
+    dfree(g.d_recPDF); dfree(g.d_totalLines); dfree(g.d_J); dfree(g.d_escaped);
+    dfree(g.f_ax); dfree(g.f_ay); dfree(g.f_az);
+    dfree(g.f_vx); dfree(g.f_vy); dfree(g.f_vz);
+    dfree(g.f_opacity); dfree(g.f_Ne); dfree(g.f_recPDF); dfree(g.f_totalLines);
+    g = MocGrid();
+}
+
+// cell widths of the volume element (photon_mod.f90:2169-2223), in 1e15 cm:
+// interior cells span the two half-gaps; the first cell is half-width in a
+// symmetric (octant) grid, the last cell full-width, as the reference has it.
+std::vector<double> widths(const std::vector<double>& a, bool sym) {
+    const int n = (int)a.size();
+    std::vector<double> w(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        double d;
+        if (n == 1) d = 0.0;
+        else if (i > 0 && i < n - 1) d = fabs(a[i + 1] - a[i - 1]) / 2.0;
+        else if (i == 0) d = sym ? fabs(a[1] - a[0]) / 2.0 : fabs(a[1] - a[0]);
+        else d = fabs(a[i] - a[i - 1]);
+        w[i] = d / 1.e15;
+    }
+    return w;
+}
+
+bool is_cellbin(int f) {
+    return f == MOC_F_OPACITY || f == MOC_F_JSTE || f == MOC_F_RECPDF || f == MOC_F_ESCAPED;
+}
+
+double* field_ptr(moc_ctx* ctx, int f) {
+    MocGrid& g = ctx->grid;
+    switch (f) {
+        case MOC_F_OPACITY: return g.d_opacity;
+        case MOC_F_JSTE: return g.d_J;
+        case MOC_F_RECPDF: return g.d_recPDF;
+        case MOC_F_TOTALLINES: return g.d_totalLines;
+        case MOC_F_NE: return g.d_Ne;
+        case MOC_F_TE: return g.d_Te;
+        case MOC_F_HDEN: return g.d_Hden;
+        case MOC_F_ESCAPED: return g.d_escaped;
+        default: return nullptr;
+    }
+}
+
+// device storage order of a (cell, bin) field: recPDF is bin-fastest (rows
+// searched per cell), the others cell-fastest (planes walked per bin)
+int device_layout(int f) { return f == MOC_F_RECPDF ? MOC_LAYOUT_CELLROW : MOC_LAYOUT_FORTRAN; }
+
+int refresh_f32(moc_ctx* ctx) {
+    MocGrid& g = ctx->grid;
+    if (!g.f32_dirty) return MOC_OK;
+    const int64_t C1 = g.nCells + 1, B = ctx->nbins;
+    moc_convert_f64_to_f32(g.d_opacity, g.f_opacity, C1 * B, ctx->stream);
+    moc_convert_f64_to_f32(g.d_recPDF, g.f_recPDF, C1 * B, ctx->stream);
+    moc_convert_f64_to_f32(g.d_Ne, g.f_Ne, C1, ctx->stream);
+    moc_convert_f64_to_f32(g.d_totalLines, g.f_totalLines, C1, ctx->stream);
+    MOC_TRY(hipGetLastError(), "f32 refresh");
+    g.f32_dirty = false;
+    return MOC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* moc_version(void) { return "gfx950 mocgpu 0.1"; }
+
+const char* moc_last_error(const moc_ctx* ctx) {
+    if (!ctx) return "null context";
+    return ctx->err.c_str();
+}
+
+int moc_create(moc_ctx** out, const moc_tables* t) {
+    if (!out || !t) return MOC_ERR_ARG;
+    *out = nullptr;
+    moc_ctx* ctx = new moc_ctx();
+    *out = ctx;
+    const int B = t->nbins;
+    if (B < 2 || !t->nuArray || !t->widFlx || t->nStars < 1 || !t->sourceCDF || !t->Lstar ||
+        !t->nPhotons || !t->starPosition || !t->starIndex || t->nAngles < 2 || !t->KNsigmaT ||
+        !t->KNsigmaCDF || !t->Pcomp ||
+        (t->precision != MOC_PREC_FP64 && t->precision != MOC_PREC_MIXED)) {
+        ctx->err = "moc_create: invalid table set";
+        return MOC_ERR_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        ctx->err = "moc_create: no HIP device";
+        return MOC_ERR_HIP;
+    }
+    ctx->precision = t->precision;
+    ctx->nbins = B;
+    ctx->nAngles = t->nAngles;
+    ctx->lgCompton = t->lgCompton;
+    ctx->nStars = t->nStars;
+    ctx->ionEdge1 = t->ionEdge1;
+    ctx->nuArray.assign(t->nuArray, t->nuArray + B);
+    ctx->widFlx.assign(t->widFlx, t->widFlx + B);
+    ctx->Lstar.assign(t->Lstar, t->Lstar + t->nStars);
+    ctx->nPhotons.assign(t->nPhotons, t->nPhotons + t->nStars);
+    ctx->starPos.assign(t->starPosition, t->starPosition + 3 * t->nStars);
+    ctx->starIdx.assign(t->starIndex, t->starIndex + 3 * t->nStars);
+    for (int i = 1; i < B; ++i)
+        if (!(ctx->nuArray[i] > ctx->nuArray[i - 1])) {
+            ctx->err = "moc_create: nuArray must be strictly increasing";
+            return MOC_ERR_ARG;
+        }
+    MOC_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "stream");
+    MOC_TRY(hipEventCreate(&ctx->ev0), "event");
+    MOC_TRY(hipEventCreate(&ctx->ev1), "event");
+    const size_t nA = (size_t)B * (size_t)t->nAngles, nS = (size_t)B * (size_t)t->nStars;
+    int rc;
+    if ((rc = dalloc_n(ctx, &ctx->d_nu, B, "nu")) || (rc = dalloc_n(ctx, &ctx->d_KNsigmaT, B, "KN")) ||
+        (rc = dalloc_n(ctx, &ctx->d_KNcdf, nA, "KNcdf")) || (rc = dalloc_n(ctx, &ctx->d_Pcomp, nA, "Pcomp")) ||
+        (rc = dalloc_n(ctx, &ctx->d_sourceCDF, nS, "srcCDF")) ||
+        (rc = dalloc_n(ctx, &ctx->f_nu, B, "nu32")) || (rc = dalloc_n(ctx, &ctx->f_KNsigmaT, B, "KN32")) ||
+        (rc = dalloc_n(ctx, &ctx->f_KNcdf, nA, "KNcdf32")) || (rc = dalloc_n(ctx, &ctx->f_Pcomp, nA, "Pcomp32")) ||
+        (rc = dalloc_n(ctx, &ctx->f_sourceCDF, nS, "srcCDF32")) ||
+        (rc = dalloc_n(ctx, &ctx->d_Qphot, 1, "Qphot")) ||
+        (rc = dalloc_n(ctx, &ctx->d_counters, MOC_CNT_N, "counters")))
+        return rc;
+    if ((rc = h2d(ctx, ctx->d_nu, t->nuArray, B * 8, "nu")) ||
+        (rc = h2d(ctx, ctx->d_KNsigmaT, t->KNsigmaT, B * 8, "KN")) ||
+        (rc = h2d(ctx, ctx->d_KNcdf, t->KNsigmaCDF, nA * 8, "KNcdf")) ||
+        (rc = h2d(ctx, ctx->d_Pcomp, t->Pcomp, nA * 8, "Pcomp")) ||
+        (rc = h2d(ctx, ctx->d_sourceCDF, t->sourceCDF, nS * 8, "srcCDF")))
+        return rc;
+    moc_convert_f64_to_f32(ctx->d_nu, ctx->f_nu, B, ctx->stream);
+    moc_convert_f64_to_f32(ctx->d_KNsigmaT, ctx->f_KNsigmaT, B, ctx->stream);
+    moc_convert_f64_to_f32(ctx->d_KNcdf, ctx->f_KNcdf, (int64_t)nA, ctx->stream);
+    moc_convert_f64_to_f32(ctx->d_Pcomp, ctx->f_Pcomp, (int64_t)nA, ctx->stream);
+    moc_convert_f64_to_f32(ctx->d_sourceCDF, ctx->f_sourceCDF, (int64_t)nS, ctx->stream);
+    MOC_TRY(hipStreamSynchronize(ctx->stream), "table conversion");
+    return MOC_OK;
+}
+
+void moc_destroy(moc_ctx* ctx) {
+    if (!ctx) return;
+    free_grid(ctx->grid);
+    dfree(ctx->d_nu); dfree(ctx->d_KNsigmaT); dfree(ctx->d_KNcdf); dfree(ctx->d_Pcomp);
+    dfree(ctx->d_sourceCDF);
+    dfree(ctx->f_nu); dfree(ctx->f_KNsigmaT); dfree(ctx->f_KNcdf); dfree(ctx->f_Pcomp);
+    dfree(ctx->f_sourceCDF);
+    dfree(ctx->d_Qphot); dfree(ctx->d_counters);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int moc_grid_attach(moc_ctx* ctx, int ig, const moc_grid_desc* gd) {
+    if (!ctx) return MOC_ERR_ARG;
+    if (ig != 1 || !gd || gd->nx < 2 || gd->ny < 2 || gd->nz < 2 || !gd->xAxis || !gd->yAxis ||
+        !gd->zAxis || !gd->active || gd->nCells < 1) {
+        ctx->err = "moc_grid_attach: invalid grid (one grid, ig=1, n>=2 per axis)";
+        return MOC_ERR_ARG;
+    }
+    free_grid(ctx->grid);
+    MocGrid& g = ctx->grid;
+    g.nx = gd->nx; g.ny = gd->ny; g.nz = gd->nz;
+    g.nCells = gd->nCells;
+    g.sym = gd->lgSymmetricXYZ ? 1 : 0;
+    g.R_out = gd->R_out;
+    g.ax.assign(gd->xAxis, gd->xAxis + g.nx);
+    g.ay.assign(gd->yAxis, gd->yAxis + g.ny);
+    g.az.assign(gd->zAxis, gd->zAxis + g.nz);
+    const size_t ncell3 = (size_t)g.nx * g.ny * g.nz;
+    g.active.assign(gd->active, gd->active + ncell3);
+    for (size_t i = 0; i < ncell3; ++i)
+        if (g.active[i] < 0 || g.active[i] > g.nCells) {
+            ctx->err = "moc_grid_attach: active ids must lie in [0, nCells] (no sub-grids)";
+            return MOC_ERR_ARG;
+        }
+    for (int s = 0; s < ctx->nStars; ++s) {
+        const int* si = &ctx->starIdx[3 * s];
+        if (si[0] < 1 || si[0] > g.nx || si[1] < 1 || si[1] > g.ny || si[2] < 1 || si[2] > g.nz) {
+            ctx->err = "moc_grid_attach: star cell index outside the grid";
+            return MOC_ERR_ARG;
+        }
+    }
+    const std::vector<double> vx = widths(g.ax, g.sym), vy = widths(g.ay, g.sym),
+                              vz = widths(g.az, g.sym);
+    const size_t C1 = (size_t)g.nCells + 1, B = (size_t)ctx->nbins;
+    int rc;
+    if ((rc = dalloc_n(ctx, &g.d_ax, g.nx, "ax")) || (rc = dalloc_n(ctx, &g.d_ay, g.ny, "ay")) ||
+        (rc = dalloc_n(ctx, &g.d_az, g.nz, "az")) || (rc = dalloc_n(ctx, &g.d_vx, g.nx, "vx")) ||
+        (rc = dalloc_n(ctx, &g.d_vy, g.ny, "vy")) || (rc = dalloc_n(ctx, &g.d_vz, g.nz, "vz")) ||
+        (rc = dalloc_n(ctx, &g.d_active, ncell3, "active")) ||
+        (rc = dalloc_n(ctx, &g.d_opacity, C1 * B, "opacity")) ||
+        (rc = dalloc_n(ctx, &g.d_Ne, C1, "Ne")) || (rc = dalloc_n(ctx, &g.d_Te, C1, "Te")) ||
+        (rc = dalloc_n(ctx, &g.d_Hden, C1, "Hden")) ||
+        (rc = dalloc_n(ctx, &g.d_recPDF, C1 * B, "recPDF")) ||
+        (rc = dalloc_n(ctx, &g.d_totalLines, C1, "totalLines")) ||
+        (rc = dalloc_n(ctx, &g.d_J, C1 * B, "J")) ||
+        (rc = dalloc_n(ctx, &g.d_escaped, C1 * B, "escaped")) ||
+        (rc = dalloc_n(ctx, &g.f_ax, g.nx, "ax32")) || (rc = dalloc_n(ctx, &g.f_ay, g.ny, "ay32")) ||
+        (rc = dalloc_n(ctx, &g.f_az, g.nz, "az32")) || (rc = dalloc_n(ctx, &g.f_vx, g.nx, "vx32")) ||
+        (rc = dalloc_n(ctx, &g.f_vy, g.ny, "vy32")) || (rc = dalloc_n(ctx, &g.f_vz, g.nz, "vz32")) ||
+        (rc = dalloc_n(ctx, &g.f_opacity, C1 * B, "opacity32")) ||
+        (rc = dalloc_n(ctx, &g.f_Ne, C1, "Ne32")) ||
+        (rc = dalloc_n(ctx, &g.f_recPDF, C1 * B, "recPDF32")) ||
+        (rc = dalloc_n(ctx, &g.f_totalLines, C1, "totalLines32"))) {
+        free_grid(g);
+        return rc;
+    }
+    if ((rc = h2d(ctx, g.d_ax, g.ax.data(), g.nx * 8, "ax")) ||
+        (rc = h2d(ctx, g.d_ay, g.ay.data(), g.ny * 8, "ay")) ||
+        (rc = h2d(ctx, g.d_az, g.az.data(), g.nz * 8, "az")) ||
+        (rc = h2d(ctx, g.d_vx, vx.data(), g.nx * 8, "vx")) ||
+        (rc = h2d(ctx, g.d_vy, vy.data(), g.ny * 8, "vy")) ||
+        (rc = h2d(ctx, g.d_vz, vz.data(), g.nz * 8, "vz")) ||
+        (rc = h2d(ctx, g.d_active, g.active.data(), ncell3 * 4, "active"))) {
+        free_grid(g);
+        return rc;
+    }
+    MOC_TRY(hipMemsetAsync(g.d_opacity, 0, C1 * B * 8, ctx->stream), "zero");
+    MOC_TRY(hipMemsetAsync(g.d_recPDF, 0, C1 * B * 8, ctx->stream), "zero");
+    MOC_TRY(hipMemsetAsync(g.d_J, 0, C1 * B * 8, ctx->stream), "zero");
+    MOC_TRY(hipMemsetAsync(g.d_escaped, 0, C1 * B * 8, ctx->stream), "zero");
+    MOC_TRY(hipMemsetAsync(g.d_Ne, 0, C1 * 8, ctx->stream), "zero");
+    MOC_TRY(hipMemsetAsync(g.d_Te, 0, C1 * 8, ctx->stream), "zero");
+    MOC_TRY(hipMemsetAsync(g.d_Hden, 0, C1 * 8, ctx->stream), "zero");
+    MOC_TRY(hipMemsetAsync(g.d_totalLines, 0, C1 * 8, ctx->stream), "zero");
+    moc_convert_f64_to_f32(g.d_ax, g.f_ax, g.nx, ctx->stream);
+    moc_convert_f64_to_f32(g.d_ay, g.f_ay, g.ny, ctx->stream);
+    moc_convert_f64_to_f32(g.d_az, g.f_az, g.nz, ctx->stream);
+    moc_convert_f64_to_f32(g.d_vx, g.f_vx, g.nx, ctx->stream);
+    moc_convert_f64_to_f32(g.d_vy, g.f_vy, g.ny, ctx->stream);
+    moc_convert_f64_to_f32(g.d_vz, g.f_vz, g.nz, ctx->stream);
+    MOC_TRY(hipStreamSynchronize(ctx->stream), "grid attach");
+    g.f32_dirty = true;
+    g.attached = true;
+    return MOC_OK;
+}
+
+int moc_upload(moc_ctx* ctx, int field, int ig, const void* host, int64_t count, int dtype,
+               int layout) {
+    if (!ctx) return MOC_ERR_ARG;
+    MocGrid& g = ctx->grid;
+    if (!g.attached || ig != 1) { ctx->err = "moc_upload: no grid attached"; return MOC_ERR_STATE; }
+    double* d = field_ptr(ctx, field);
+    if (!d || !host) { ctx->err = "moc_upload: unknown field"; return MOC_ERR_ARG; }
+    const int64_t C1 = g.nCells + 1, B = ctx->nbins;
+    const bool cb = is_cellbin(field);
+    const int64_t need = cb ? C1 * B : C1;
+    if (count != need) { ctx->err = "moc_upload: element count does not match the field"; return MOC_ERR_ARG; }
+    if (dtype != MOC_DT_F32 && dtype != MOC_DT_F64) { ctx->err = "moc_upload: dtype"; return MOC_ERR_ARG; }
+    if (cb && layout != MOC_LAYOUT_FORTRAN && layout != MOC_LAYOUT_CELLROW) { ctx->err = "moc_upload: layout"; return MOC_ERR_ARG; }
+    std::vector<double> buf((size_t)need);
+    for (int64_t i = 0; i < need; ++i)
+        buf[i] = dtype == MOC_DT_F64 ? ((const double*)host)[i] : (double)((const float*)host)[i];
+    if (cb && layout != device_layout(field)) {  // transpose (cell,bin) <-> (bin,cell)
+        std::vector<double> t((size_t)need);
+        if (layout == MOC_LAYOUT_FORTRAN)   // host cell-fastest -> device bin-fastest
+            for (int64_t b = 0; b < B; ++b)
+                for (int64_t c = 0; c < C1; ++c) t[c * B + b] = buf[b * C1 + c];
+        else
+            for (int64_t c = 0; c < C1; ++c)
+                for (int64_t b = 0; b < B; ++b) t[b * C1 + c] = buf[c * B + b];
+        buf.swap(t);
+    }
+    int rc = h2d(ctx, d, buf.data(), (size_t)need * 8, "upload");
+    if (rc) return rc;
+    g.f32_dirty = true;
+    return MOC_OK;
+}
+
+int moc_download(moc_ctx* ctx, int field, int ig, void* host, int64_t count, int dtype,
+                 int layout) {
+    if (!ctx) return MOC_ERR_ARG;
+    MocGrid& g = ctx->grid;
+    if (!g.attached || ig != 1) { ctx->err = "moc_download: no grid attached"; return MOC_ERR_STATE; }
+    double* d = field_ptr(ctx, field);
+    if (!d || !host) { ctx->err = "moc_download: unknown field"; return MOC_ERR_ARG; }
+    const int64_t C1 = g.nCells + 1, B = ctx->nbins;
+    const bool cb = is_cellbin(field);
+    const int64_t need = cb ? C1 * B : C1;
+    if (count != need) { ctx->err = "moc_download: element count does not match the field"; return MOC_ERR_ARG; }
+    if (dtype != MOC_DT_F32 && dtype != MOC_DT_F64) { ctx->err = "moc_download: dtype"; return MOC_ERR_ARG; }
+    std::vector<double> buf((size_t)need);
+    MOC_TRY(hipMemcpyAsync(buf.data(), d, (size_t)need * 8, hipMemcpyDeviceToHost, ctx->stream), "download");
+    MOC_TRY(hipStreamSynchronize(ctx->stream), "download");
+    if (cb && layout != device_layout(field)) {
+        std::vector<double> t((size_t)need);
+        if (layout == MOC_LAYOUT_FORTRAN)   // device bin-fastest -> host cell-fastest
+            for (int64_t c = 0; c < C1; ++c)
+                for (int64_t b = 0; b < B; ++b) t[b * C1 + c] = buf[c * B + b];
+        else
+            for (int64_t b = 0; b < B; ++b)
+                for (int64_t c = 0; c < C1; ++c) t[c * B + b] = buf[b * C1 + c];
+        buf.swap(t);
+    }
+    for (int64_t i = 0; i < need; ++i) {
+        if (dtype == MOC_DT_F64) ((double*)host)[i] = buf[i];
+        else ((float*)host)[i] = (float)buf[i];
+    }
+    return MOC_OK;
+}
+
+int moc_device_ptr(moc_ctx* ctx, int field, int ig, void** dptr, int64_t* count, int* dtype) {
+    if (!ctx || !dptr) return MOC_ERR_ARG;
+    MocGrid& g = ctx->grid;
+    if (!g.attached || ig != 1) { ctx->err = "moc_device_ptr: no grid attached"; return MOC_ERR_STATE; }
+    double* d = field_ptr(ctx, field);
+    if (!d) { ctx->err = "moc_device_ptr: unknown field"; return MOC_ERR_ARG; }
+    *dptr = d;
+    if (count) *count = is_cellbin(field) ? (int64_t)(g.nCells + 1) * ctx->nbins : g.nCells + 1;
+    if (dtype) *dtype = MOC_DT_F64;
+    g.f32_dirty = true;  // the caller may write through the pointer
+    return MOC_OK;
+}
+
+int moc_reset_tallies(moc_ctx* ctx, int ig) {
+    if (!ctx) return MOC_ERR_ARG;
+    MocGrid& g = ctx->grid;
+    if (!g.attached || ig != 1) { ctx->err = "moc_reset_tallies: no grid attached"; return MOC_ERR_STATE; }
+    const size_t n = ((size_t)g.nCells + 1) * ctx->nbins;
+    MOC_TRY(hipMemsetAsync(g.d_J, 0, n * 8, ctx->stream), "reset J");
+    MOC_TRY(hipMemsetAsync(g.d_escaped, 0, n * 8, ctx->stream), "reset escaped");
+    MOC_TRY(hipStreamSynchronize(ctx->stream), "reset");
+    return MOC_OK;
+}
+
+int moc_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ctr_base,
+                  moc_transport_stats* stats) {
+    if (!ctx) return MOC_ERR_ARG;
+    MocGrid& g = ctx->grid;
+    if (!g.attached) { ctx->err = "moc_transport: no grid attached"; return MOC_ERR_STATE; }
+    if (iStar < 1 || iStar > ctx->nStars || n < 0) { ctx->err = "moc_transport: iStar/n"; return MOC_ERR_ARG; }
+    const int s = iStar - 1;
+    const double deltaE = ctx->Lstar[s] / ctx->nPhotons[s];  // photon_mod.f90:88
+    if (!(deltaE >= 0.0)) { ctx->err = "moc_transport: deltaEUsed < 0"; return MOC_ERR_ARG; }
+    MOC_TRY(hipMemsetAsync(ctx->d_counters, 0, MOC_CNT_N * sizeof(unsigned long long), ctx->stream), "counters");
+    MOC_TRY(hipMemsetAsync(ctx->d_Qphot, 0, sizeof(double), ctx->stream), "Qphot");
+    float ms = 0.f;
+    int rc;
+    if (ctx->precision == MOC_PREC_FP64) {
+        MocTransportArgs<double> a;
+        a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nCells = g.nCells; a.sym = g.sym;
+        a.ax = g.d_ax; a.ay = g.d_ay; a.az = g.d_az; a.vx = g.d_vx; a.vy = g.d_vy; a.vz = g.d_vz;
+        a.active = g.d_active; a.R_out = g.R_out;
+        a.nbins = ctx->nbins; a.nAngles = ctx->nAngles; a.lgCompton = ctx->lgCompton;
+        a.ionEdge1 = ctx->ionEdge1; a.nuArray = ctx->d_nu; a.KNsigmaT = ctx->d_KNsigmaT;
+        a.KNcdf = ctx->d_KNcdf; a.Pcomp = ctx->d_Pcomp;
+        a.sourceCDF = ctx->d_sourceCDF + (size_t)s * ctx->nbins;
+        a.opacity = g.d_opacity; a.Ne = g.d_Ne; a.recPDF = g.d_recPDF; a.totalLines = g.d_totalLines;
+        a.J = g.d_J; a.escaped = g.d_escaped; a.Qphot = ctx->d_Qphot; a.counters = ctx->d_counters;
+        for (int k = 0; k < 3; ++k) { a.star[k] = ctx->starPos[3 * s + k]; a.starIdx[k] = ctx->starIdx[3 * s + k] - 1; }
+        a.deltaE = deltaE; a.n = n; a.seed = seed; a.ctr_base = ctr_base;
+        rc = moc_launch_transport_f64(ctx, a, &ms);
+    } else {
+        if ((rc = refresh_f32(ctx))) return rc;
+        MocTransportArgs<float> a;
+        a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nCells = g.nCells; a.sym = g.sym;
+        a.ax = g.f_ax; a.ay = g.f_ay; a.az = g.f_az; a.vx = g.f_vx; a.vy = g.f_vy; a.vz = g.f_vz;
+        a.active = g.d_active; a.R_out = (float)g.R_out;
+        a.nbins = ctx->nbins; a.nAngles = ctx->nAngles; a.lgCompton = ctx->lgCompton;
+        a.ionEdge1 = (float)ctx->ionEdge1; a.nuArray = ctx->f_nu; a.KNsigmaT = ctx->f_KNsigmaT;
+        a.KNcdf = ctx->f_KNcdf; a.Pcomp = ctx->f_Pcomp;
+        a.sourceCDF = ctx->f_sourceCDF + (size_t)s * ctx->nbins;
+        a.opacity = g.f_opacity; a.Ne = g.f_Ne; a.recPDF = g.f_recPDF; a.totalLines = g.f_totalLines;
+        a.J = g.d_J; a.escaped = g.d_escaped; a.Qphot = ctx->d_Qphot; a.counters = ctx->d_counters;
+        for (int k = 0; k < 3; ++k) { a.star[k] = (float)ctx->starPos[3 * s + k]; a.starIdx[k] = ctx->starIdx[3 * s + k] - 1; }
+        a.deltaE = (float)deltaE; a.n = n; a.seed = seed; a.ctr_base = ctr_base;
+        rc = moc_launch_transport_f32(ctx, a, &ms);
+    }
+    if (rc) return rc;
+    unsigned long long cnt[MOC_CNT_N];
+    double q = 0.0;
+    MOC_TRY(hipMemcpyAsync(cnt, ctx->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream), "counters");
+    MOC_TRY(hipMemcpyAsync(&q, ctx->d_Qphot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "Qphot");
+    MOC_TRY(hipStreamSynchronize(ctx->stream), "counters");
+    if (stats) {
+        stats->Qphot = q;
+        stats->segments = (double)cnt[MOC_CNT_SEGMENTS];
+        stats->emissions = (double)cnt[MOC_CNT_EMISSIONS];
+        stats->absorbed = (double)cnt[MOC_CNT_ABSORBED];
+        stats->scattered = (double)cnt[MOC_CNT_SCATTERED];
+        stats->escaped = (double)cnt[MOC_CNT_ESCAPED];
+        stats->lines = (double)cnt[MOC_CNT_LINES];
+        stats->lost = (double)cnt[MOC_CNT_LOST];
+        stats->trapped = (double)cnt[MOC_CNT_TRAPPED];
+        stats->kernel_ms = ms;
+    }
+    if (cnt[MOC_CNT_ERROR]) {
+        ctx->err = "moc_transport: " + std::to_string(cnt[MOC_CNT_ERROR]) +
+                   " packets reached an insane state (index outside the grid or emission "
+                   "from an inactive cell)";
+        return MOC_ERR_KERNEL;
+    }
+    return MOC_OK;
+}
+
+int moc_normalize_j(moc_ctx* ctx, int ig) {
+    if (!ctx) return MOC_ERR_ARG;
+    MocGrid& g = ctx->grid;
+    if (!g.attached || ig != 1) { ctx->err = "moc_normalize_j: no grid attached"; return MOC_ERR_STATE; }
+    const int64_t n = ((int64_t)g.nCells + 1) * ctx->nbins;
+    // units: deltaE [1e36 erg/s] * l [cm] / V [1e45 cm^3] -> erg/s/cm^2 per bin
+    moc_scale_f64(g.d_J, n, g.sym ? 1.e-9 / 8.0 : 1.e-9, ctx->stream);
+    if (g.sym) moc_scale_f64(g.d_escaped, n, 1.0 / 8.0, ctx->stream);
+    MOC_TRY(hipGetLastError(), "normalize");
+    MOC_TRY(hipStreamSynchronize(ctx->stream), "normalize");
+    return MOC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,180 @@
+"""Python host mirror of the reference's transport interface over libmocgpu.
+
+`MocGPU` owns one libmocgpu context (one Cartesian grid).  Its methods carry
+the names of the MOCASSIN routines they replace:
+
+    energyPacketDriver(iStar, n, ...)  photon_mod.f90:28
+    reset_tallies()                     iteration_mod.f90:626-717 (Jste=0, escapedPackets=0)
+    normalize_j()                       iteration_mod.f90:1002-1031
+
+Per-cell arrays are exchanged in MOCASSIN's shapes: (nCells+1,) for cell
+fields and (nCells+1, nbins) for (cell, bin) fields, row 0 being the inactive
+dummy cell.  Errors raise MocError with the library's message (the reference
+prints and stops).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import (CELLBIN_FIELDS, FIELDS, MOC_DT_F64, MOC_LAYOUT_FORTRAN, MOC_PREC_FP64,
+                   MOC_PREC_MIXED, MocGridDesc, MocTables, MocTransportStats, dptr, iptr)
+
+
+class MocError(RuntimeError):
+    pass
+
+
+@dataclass
+class SourceTables:
+    """Mesh and point-source tables (moc_tables)."""
+    nuArray: np.ndarray
+    widFlx: np.ndarray
+    ionEdge1: float
+    KNsigmaT: np.ndarray
+    KNcdf: np.ndarray          # [B, nAngles]
+    Pcomp: np.ndarray          # [B, nAngles]
+    sourceCDF: np.ndarray      # [nStars, B]
+    Lstar: np.ndarray          # [nStars] 1e36 erg/s
+    nPhotons: np.ndarray       # [nStars]
+    starPosition: np.ndarray   # [nStars, 3] cm
+    starIndex: np.ndarray      # [nStars, 3] 1-based
+    lgCompton: bool = True
+
+    @property
+    def nbins(self) -> int:
+        return int(self.nuArray.size)
+
+
+@dataclass
+class CartesianGrid:
+    """One grid_type (common_mod.f90:308-371) restricted to what the path needs."""
+    xAxis: np.ndarray
+    yAxis: np.ndarray
+    zAxis: np.ndarray
+    active: np.ndarray         # [nx, ny, nz] int32, 1..nCells or 0
+    nCells: int
+    lgSymmetricXYZ: bool
+    R_out: float = 0.0
+    extra: dict = field(default_factory=dict)
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class MocGPU:
+    def __init__(self, tables: SourceTables, precision: str = "fp64"):
+        self._L = _lib.lib()
+        self.tables = tables
+        self.precision = precision
+        prec = {"fp64": MOC_PREC_FP64, "mixed": MOC_PREC_MIXED}[precision]
+        B = tables.nbins
+        self._keep = [_c(tables.nuArray), _c(tables.widFlx), _c(tables.KNsigmaT),
+                      _c(tables.KNcdf), _c(tables.Pcomp), _c(tables.sourceCDF),
+                      _c(tables.Lstar), _c(tables.nPhotons), _c(tables.starPosition),
+                      _c(tables.starIndex, np.int32)]
+        k = self._keep
+        nst = int(np.atleast_1d(tables.Lstar).size)
+        assert k[5].size == nst * B and k[3].size == k[4].size and k[3].size % B == 0
+        t = MocTables(nbins=B, precision=prec, nuArray=dptr(k[0]), widFlx=dptr(k[1]),
+                      ionEdge1=float(tables.ionEdge1), lgCompton=int(bool(tables.lgCompton)),
+                      nAngles=k[3].size // B, KNsigmaT=dptr(k[2]), KNsigmaCDF=dptr(k[3]),
+                      Pcomp=dptr(k[4]), nStars=nst, sourceCDF=dptr(k[5]), Lstar=dptr(k[6]),
+                      nPhotons=dptr(k[7]), starPosition=dptr(k[8]), starIndex=iptr(k[9]))
+        self._ctx = C.c_void_p()
+        rc = self._L.moc_create(C.byref(self._ctx), C.byref(t))
+        if rc != 0:
+            msg = self._L.moc_last_error(self._ctx).decode()
+            self._L.moc_destroy(self._ctx)
+            self._ctx = None
+            raise MocError(f"moc_create: {msg} (rc={rc})")
+        self.grid: CartesianGrid | None = None
+
+    # ------------------------------------------------------------------ utils
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            raise MocError(f"{what}: {self._L.moc_last_error(self._ctx).decode()} (rc={rc})")
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._L.moc_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def nbins(self) -> int:
+        return self.tables.nbins
+
+    # ------------------------------------------------------------------- grid
+    def grid_attach(self, g: CartesianGrid):
+        act = np.asfortranarray(g.active, dtype=np.int32).reshape(-1, order="F")
+        act = np.ascontiguousarray(act)
+        xa, ya, za = _c(g.xAxis), _c(g.yAxis), _c(g.zAxis)
+        d = MocGridDesc(nx=xa.size, ny=ya.size, nz=za.size, xAxis=dptr(xa), yAxis=dptr(ya),
+                        zAxis=dptr(za), active=iptr(act), nCells=int(g.nCells),
+                        lgSymmetricXYZ=int(bool(g.lgSymmetricXYZ)), R_out=float(g.R_out))
+        self._check(self._L.moc_grid_attach(self._ctx, 1, C.byref(d)), "moc_grid_attach")
+        self.grid = g
+
+    def _count(self, name: str) -> int:
+        c1 = self.grid.nCells + 1
+        return c1 * self.nbins if name in CELLBIN_FIELDS else c1
+
+    def upload(self, name: str, arr: np.ndarray):
+        """Upload a field given in MOCASSIN shape ((C+1,) or (C+1, B))."""
+        c1 = self.grid.nCells + 1
+        a = np.asarray(arr, dtype=np.float64)
+        if name in CELLBIN_FIELDS:
+            assert a.shape == (c1, self.nbins), (name, a.shape)
+            buf = np.ascontiguousarray(a.T)      # Fortran order: cell fastest
+        else:
+            assert a.shape == (c1,), (name, a.shape)
+            buf = np.ascontiguousarray(a)
+        self._check(self._L.moc_upload(self._ctx, FIELDS[name], 1, buf.ctypes.data, buf.size,
+                                       MOC_DT_F64, MOC_LAYOUT_FORTRAN), f"moc_upload({name})")
+
+    def download(self, name: str) -> np.ndarray:
+        c1 = self.grid.nCells + 1
+        n = self._count(name)
+        buf = np.empty(n, dtype=np.float64)
+        self._check(self._L.moc_download(self._ctx, FIELDS[name], 1, buf.ctypes.data, n,
+                                         MOC_DT_F64, MOC_LAYOUT_FORTRAN), f"moc_download({name})")
+        if name in CELLBIN_FIELDS:
+            return np.ascontiguousarray(buf.reshape(self.nbins, c1).T)
+        return buf
+
+    # -------------------------------------------------------------- transport
+    def reset_tallies(self):
+        self._check(self._L.moc_reset_tallies(self._ctx, 1), "moc_reset_tallies")
+
+    def energyPacketDriver(self, iStar: int, n: int, seed: int = 20261203,
+                           ctr_base: int = 0) -> dict:
+        """Transport n packets of star iStar (photon_mod.f90:28); J and escape
+        tallies accumulate on the device.  Returns the packet counters."""
+        st = MocTransportStats()
+        self._check(self._L.moc_transport(self._ctx, int(iStar), int(n), int(seed) & (2**64 - 1),
+                                          int(ctr_base) & 0xffffffff, C.byref(st)),
+                    "energyPacketDriver")
+        return st.as_dict()
+
+    def normalize_j(self):
+        self._check(self._L.moc_normalize_j(self._ctx, 1), "moc_normalize_j")
+
+
+def version() -> str:
+    return _lib.lib().moc_version().decode()

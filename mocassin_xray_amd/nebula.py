@@ -1,0 +1,125 @@
+"""Synthetic model set-ups of the benchmark workloads.
+
+Each builder returns the grid, the mesh/source tables and the per-cell fields
+the transport consumes.  `frozen_*` builders fill the per-cell fields from a
+closed-form ionisation structure (a neutral-hydrogen profile with a
+hydrogenic cross-section and a recombination-like diffuse spectrum) so the
+transport can be exercised and timed on its own at the real shapes.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import tables as T
+from .mocgpu import CartesianGrid, SourceTables
+
+A0 = 6.30e-18          # H0 photoionisation cross-section at threshold [cm^2] (constants_mod.f90:10)
+H_EDGE = 0.999434      # ionEdge of H I [Ryd] (grid_mod.f90:194)
+
+
+@dataclass
+class FrozenModel:
+    name: str
+    tables: SourceTables
+    grid: CartesianGrid
+    fields: dict           # name -> array in MOCASSIN shape
+    n_packets: int
+
+
+def _star_at_origin(sym: bool, nx, ny, nz):
+    if sym:
+        return np.zeros((1, 3)), np.array([[1, 1, 1]], dtype=np.int32)
+    return np.zeros((1, 3)), np.array([[(nx + 1) // 2, (ny + 1) // 2, (nz + 1) // 2]],
+                                      dtype=np.int32)
+
+
+def source_tables(nu, T_star, L36, n_packets, sym, nx, ny, nz, lgCompton=True,
+                  ion_edge1=H_EDGE, flux=None) -> SourceTables:
+    wid = T.widths_of(nu)
+    f = T.blackbody_flux(nu, T_star) if flux is None else flux
+    cdf = T.source_cdf(f, wid)
+    kn_t, kn_cdf, pc = T.compton_tables(nu)
+    pos, idx = _star_at_origin(sym, nx, ny, nz)
+    return SourceTables(nuArray=nu, widFlx=wid, ionEdge1=ion_edge1, KNsigmaT=kn_t, KNcdf=kn_cdf,
+                        Pcomp=pc, sourceCDF=cdf[None, :], Lstar=np.array([L36]),
+                        nPhotons=np.array([float(n_packets)]), starPosition=pos,
+                        starIndex=idx, lgCompton=lgCompton)
+
+
+def frozen_fields(grid: CartesianGrid, nu: np.ndarray, hden_cell: np.ndarray, R_s: float,
+                  x_in: float = 1.e-3, T_e: float = 1.e4, lines: float = 0.3,
+                  kn_sigma: np.ndarray | None = None, rng_seed: int = 7) -> dict:
+    """Per-cell opacity / Ne / recPDF / totalLines of a Stromgren-like sphere.
+
+    Neutral fraction x_in inside R_s and 1 outside (smoothed over a shell);
+    kappa = n_H0 * A0 (nu/nu0)^-3 above the H edge (+ n_e sigma_KN);
+    diffuse emission ~ exp(-(nu - nu0) Te1Ryd/T_e) above the edge plus a flat
+    sub-edge continuum; totalLines = `lines` everywhere.
+    """
+    C1 = grid.nCells + 1
+    B = nu.size
+    xa, ya, za = grid.xAxis, grid.yAxis, grid.zAxis
+    X, Y, Z = np.meshgrid(xa, ya, za, indexing="ij")
+    r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    on = grid.active > 0
+    rcell = np.zeros(C1)
+    rcell[grid.active[on]] = r[on]
+    width = 0.05 * R_s
+    xh0 = x_in + (1.0 - x_in) / (1.0 + np.exp(-(rcell - R_s) / width))
+    xh0[0] = 0.0
+    nH0 = xh0 * hden_cell
+    ne = (1.0 - xh0) * hden_cell
+    ne[0] = 0.0
+    sig = np.where(nu >= H_EDGE, A0 * (nu / H_EDGE) ** -3.0, 0.0)
+    kap = nH0[:, None] * sig[None, :]
+    if kn_sigma is not None:
+        kap = kap + ne[:, None] * kn_sigma[None, :]
+    kap[0, :] = 0.0
+    wid = T.widths_of(nu)
+    Te1Ryd = 1.578866e5
+    em = np.where(nu >= H_EDGE, np.exp(-(nu - H_EDGE) * Te1Ryd / T_e), 0.02)
+    cdf = np.cumsum(em * wid)
+    cdf /= cdf[-1]
+    rec = np.tile(cdf, (C1, 1))
+    rec[0, :] = 0.0
+    tl = np.full(C1, lines)
+    tl[0] = 0.0
+    return {"opacity": kap, "Ne": ne, "recPDF": rec, "totalLines": tl}
+
+
+def frozen_hsphere(n: int = 16, nbins: int = 400, n_packets: int = 1_000_000,
+                   mesh: str = "reference", R: float = 3.e18, hden: float = 100.,
+                   T_star: float = 4.e4, L36: float = 10.0, R_s: float = 2.4e18,
+                   lgCompton: bool = True) -> FrozenModel:
+    """Pure-hydrogen homogeneous sphere, one octant (16^3, edge 3e18 cm,
+    n_H = 100, 40 kK blackbody of 1e37 erg/s, 400 bins over 1-25 Ryd)."""
+    ax = T.symmetric_axes(n, R)
+    hd = np.full((n, n, n), hden)
+    act, ncells, hcell = T.active_map(ax, ax, ax, hd)
+    grid = CartesianGrid(ax, ax.copy(), ax.copy(), act, ncells, True, 0.0)
+    if mesh == "reference":
+        nu = T.reference_gas_mesh(nbins, 1.0, 25.0, np.array([H_EDGE]))
+    else:
+        nu = T.log_mesh(nbins, 1.0, 25.0)
+    tabs = source_tables(nu, T_star, L36, n_packets, True, n, n, n, lgCompton)
+    f = frozen_fields(grid, nu, hcell, R_s, kn_sigma=tabs.KNsigmaT if lgCompton else None)
+    f["Hden"] = hcell
+    return FrozenModel("hsphere%d" % n, tabs, grid, f, n_packets)
+
+
+def frozen_nebula(n: int = 32, nbins: int = 1000, n_packets: int = 10_000_000,
+                  R: float = 1.e18, hden: float = 1000., T_star: float = 1.e5,
+                  L36: float = 10 ** 1.5, R_s: float = 0.8e18) -> FrozenModel:
+    """Multi-element nebula shape: 32^3 octant, n_H = 1000, 100 kK blackbody of
+    10^37.5 erg/s, 1000 log bins over 1-100 Ryd."""
+    ax = T.symmetric_axes(n, R)
+    hd = np.full((n, n, n), hden)
+    act, ncells, hcell = T.active_map(ax, ax, ax, hd)
+    grid = CartesianGrid(ax, ax.copy(), ax.copy(), act, ncells, True, 0.0)
+    nu = T.log_mesh(nbins, 1.0, 100.0)
+    tabs = source_tables(nu, T_star, L36, n_packets, True, n, n, n, True)
+    f = frozen_fields(grid, nu, hcell, R_s, kn_sigma=tabs.KNsigmaT)
+    f["Hden"] = hcell
+    return FrozenModel("nebula%d" % n, tabs, grid, f, n_packets)

@@ -1,0 +1,71 @@
+/*
+ * moc_oracle.h -- CPU restatement (plain C, FP64) of MOCASSIN's Monte Carlo
+ * iteration, used ONLY as test infrastructure: the parity tests, the smoke
+ * check and the cpu_baseline leg of bench.py call it as the checker.  The
+ * product (libmocgpu) never links or calls it.
+ */
+#ifndef MOC_ORACLE_H
+#define MOC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Philox4x32-10 block (Salmon et al. SC'11), for known-answer tests. */
+void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* getNu2 (photon_mod.f90:928-972): 1-based bin from a CDF of n entries and a
+ * uniform r in (0,1), by the reference's linear scan and bin-1 fold. */
+int oracle_getnu2(const double* cdf, int n, double r);
+
+/* locate (interpolation_mod.f90:49-106): 1-based bracket index, 0 or n out
+ * of range, exact hits return the hit. */
+int oracle_locate(const double* xa, int n, double x);
+
+typedef struct oracle_transport_in {
+    int nx, ny, nz, nCells, sym;
+    const double *xAxis, *yAxis, *zAxis; /* [n] cm */
+    const int32_t* active;               /* [nx*ny*nz], x fastest */
+    double R_out;
+    int nbins, nAngles, lgCompton;
+    double ionEdge1;
+    const double* nuArray;    /* [B] */
+    const double* KNsigmaT;   /* [B] */
+    const double* KNcdf;      /* [B][nAngles] */
+    const double* Pcomp;      /* [B][nAngles] */
+    const double* sourceCDF;  /* [B] */
+    const double* opacity;    /* [B][C+1] cell fastest */
+    const double* Ne;         /* [C+1] */
+    const double* recPDF;     /* [C+1][B] bin fastest */
+    const double* totalLines; /* [C+1] */
+    double star[3];
+    int32_t starIdx[3];       /* 1-based */
+    double deltaE;
+    int64_t n;
+    uint64_t seed;
+    uint32_t ctr_base;
+    int nthreads;             /* OpenMP threads (<=1: serial) */
+} oracle_transport_in;
+
+/* counters[] slots */
+enum {
+    ORC_SEGMENTS = 0, ORC_EMISSIONS, ORC_ABSORBED, ORC_SCATTERED, ORC_ESCAPED,
+    ORC_LINES, ORC_LOST, ORC_TRAPPED, ORC_ERRORS, ORC_N
+};
+
+typedef struct oracle_transport_out {
+    double* J;        /* [B][C+1], accumulated (not zeroed here) */
+    double* escaped;  /* [B][C+1], accumulated */
+    double counters[ORC_N];
+    double Qphot;
+} oracle_transport_out;
+
+/* energyPacketDriver (photon_mod.f90:28-414) for one stellar source. */
+int oracle_transport(const oracle_transport_in* in, oracle_transport_out* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,120 @@
+"""ctypes front end of the ORACLE (liboracle.so) -- test infrastructure only.
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+where it is the checker; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class TransportIn(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nCells", C.c_int), ("sym", C.c_int),
+        ("xAxis", _dp), ("yAxis", _dp), ("zAxis", _dp), ("active", _ip), ("R_out", C.c_double),
+        ("nbins", C.c_int), ("nAngles", C.c_int), ("lgCompton", C.c_int), ("ionEdge1", C.c_double),
+        ("nuArray", _dp), ("KNsigmaT", _dp), ("KNcdf", _dp), ("Pcomp", _dp), ("sourceCDF", _dp),
+        ("opacity", _dp), ("Ne", _dp), ("recPDF", _dp), ("totalLines", _dp),
+        ("star", C.c_double * 3), ("starIdx", C.c_int32 * 3), ("deltaE", C.c_double),
+        ("n", C.c_int64), ("seed", C.c_uint64), ("ctr_base", C.c_uint32), ("nthreads", C.c_int),
+    ]
+
+
+class TransportOut(C.Structure):
+    _fields_ = [("J", _dp), ("escaped", _dp), ("counters", C.c_double * 9), ("Qphot", C.c_double)]
+
+
+COUNTER_NAMES = ["segments", "emissions", "absorbed", "scattered", "escaped", "lines", "lost",
+                 "trapped", "errors"]
+
+_L = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib() -> C.CDLL:
+    global _L
+    if _L is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        L.oracle_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                           C.POINTER(C.c_uint32)]
+        L.oracle_getnu2.argtypes = [_dp, C.c_int, C.c_double]
+        L.oracle_getnu2.restype = C.c_int
+        L.oracle_locate.argtypes = [_dp, C.c_int, C.c_double]
+        L.oracle_locate.restype = C.c_int
+        L.oracle_transport.argtypes = [C.POINTER(TransportIn), C.POINTER(TransportOut)]
+        L.oracle_transport.restype = C.c_int
+        _L = L
+    return _L
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().oracle_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def getnu2(cdf, r) -> int:
+    a = np.ascontiguousarray(cdf, dtype=np.float64)
+    return lib().oracle_getnu2(a.ctypes.data_as(_dp), a.size, float(r))
+
+
+def locate(xa, x) -> int:
+    a = np.ascontiguousarray(xa, dtype=np.float64)
+    return lib().oracle_locate(a.ctypes.data_as(_dp), a.size, float(x))
+
+
+def transport(model_tables, grid, fields, n, seed, ctr_base=0, nthreads=1, istar=0):
+    """Run the oracle transport.  Returns (J[C+1,B], escaped[C+1,B], counters, Qphot)."""
+    t = model_tables
+    B = t.nbins
+    C1 = grid.nCells + 1
+    keep = {}
+
+    def d(name, a):
+        keep[name] = np.ascontiguousarray(a, dtype=np.float64)
+        return keep[name].ctypes.data_as(_dp)
+
+    act = np.ascontiguousarray(np.asarray(grid.active).reshape(-1, order="F"), dtype=np.int32)
+    keep["act"] = act
+    src = np.asarray(t.sourceCDF).reshape(-1, B)[istar]
+    deltaE = float(np.atleast_1d(t.Lstar)[istar] / np.atleast_1d(t.nPhotons)[istar])
+    inp = TransportIn(
+        nx=len(grid.xAxis), ny=len(grid.yAxis), nz=len(grid.zAxis), nCells=grid.nCells,
+        sym=int(bool(grid.lgSymmetricXYZ)), xAxis=d("x", grid.xAxis), yAxis=d("y", grid.yAxis),
+        zAxis=d("z", grid.zAxis), active=act.ctypes.data_as(_ip), R_out=float(grid.R_out),
+        nbins=B, nAngles=np.asarray(t.KNcdf).shape[1], lgCompton=int(bool(t.lgCompton)),
+        ionEdge1=float(t.ionEdge1), nuArray=d("nu", t.nuArray), KNsigmaT=d("kn", t.KNsigmaT),
+        KNcdf=d("kncdf", t.KNcdf), Pcomp=d("pc", t.Pcomp), sourceCDF=d("src", src),
+        opacity=d("op", np.asarray(fields["opacity"]).T), Ne=d("ne", fields["Ne"]),
+        recPDF=d("rec", fields["recPDF"]), totalLines=d("tl", fields["totalLines"]),
+        deltaE=deltaE, n=int(n), seed=int(seed), ctr_base=int(ctr_base), nthreads=int(nthreads))
+    pos = np.asarray(t.starPosition).reshape(-1, 3)[istar]
+    idx = np.asarray(t.starIndex).reshape(-1, 3)[istar]
+    for k in range(3):
+        inp.star[k] = float(pos[k])
+        inp.starIdx[k] = int(idx[k])
+    J = np.zeros(B * C1)
+    E = np.zeros(B * C1)
+    out = TransportOut(J=J.ctypes.data_as(_dp), escaped=E.ctypes.data_as(_dp))
+    rc = lib().oracle_transport(C.byref(inp), C.byref(out))
+    cnt = {n_: out.counters[i] for i, n_ in enumerate(COUNTER_NAMES)}
+    cnt["rc"] = rc
+    return J.reshape(B, C1).T.copy(), E.reshape(B, C1).T.copy(), cnt, out.Qphot
