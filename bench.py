@@ -41,6 +41,18 @@ def transport_bytes(st: dict, nbins: int, real_bytes: int, n: int) -> float:
     return st["segments"] * seg + st["emissions"] * emit + st["escaped"] * 16.0
 
 
+def profiled_traffic(workload: str):
+    """HBM bytes per launch of k_transport<float> measured by the FETCH_SIZE and
+    WRITE_SIZE counter runs recorded in profiles/ (None if not profiled)."""
+    f = ROOT / "profiles" / f"transport_{workload}.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def time_path(m, prec: str, steps: int, warmup: int, n: int):
     import torch
     from mocassin_xray_amd import MocGPU
@@ -61,6 +73,18 @@ def time_path(m, prec: str, steps: int, warmup: int, n: int):
     dt = time.perf_counter() - t0
     g.close()
     return dt, stats
+
+
+def copy_calibration(nbytes: int = 1 << 30, reps: int = 3):
+    """Plain device copy of known size: FETCH_SIZE/WRITE_SIZE of this kernel
+    calibrate the counters (bytes per KiB reported)."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copy_calibration: {reps} copies of {nbytes} bytes", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(m, target_s: float = 12.0):
@@ -86,6 +110,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="nebula32")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
+                    help="time one path only (for profiler runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a plain 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the transport shards by packets; "
@@ -93,14 +121,23 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.init()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     m = build_model(args.workload)
     n = m.n_packets
     print(f"bench: workload {m.name} cells={m.grid.nCells} bins={m.tables.nbins} packets={n}",
           file=sys.stderr, flush=True)
-    dt_mix, st_mix = time_path(m, "mixed", args.steps, args.warmup, n)
-    print(f"bench: mixed {args.steps * n / dt_mix:.3e} packets/s", file=sys.stderr, flush=True)
-    dt_64, st_64 = time_path(m, "fp64", args.steps, args.warmup, n)
-    print(f"bench: fp64  {args.steps * n / dt_64:.3e} packets/s", file=sys.stderr, flush=True)
+    if args.only != "fp64":
+        dt_mix, st_mix = time_path(m, "mixed", args.steps, args.warmup, n)
+        print(f"bench: mixed {args.steps * n / dt_mix:.3e} packets/s", file=sys.stderr,
+              flush=True)
+    if args.only != "mixed":
+        dt_64, st_64 = time_path(m, "fp64", args.steps, args.warmup, n)
+        print(f"bench: fp64  {args.steps * n / dt_64:.3e} packets/s", file=sys.stderr,
+              flush=True)
+    if args.only is not None:
+        return
     value = args.steps * n / dt_mix
     kms = sum(s["kernel_ms"] for s in st_mix) / len(st_mix)
     bytes_mix = sum(transport_bytes(s, m.tables.nbins, 4, n) for s in st_mix) / len(st_mix)
@@ -128,7 +165,8 @@ def main():
         "segments_per_packet": seg_per_pkt,
         "emissions_per_packet": emit_per_pkt,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(args.workload),
                      "kernel": "k_transport<float>", "kernel_ms": kms},
     }
     if not args.no_cpu:

@@ -69,3 +69,41 @@ int oracle_transport(const oracle_transport_in* in, oracle_transport_out* out);
 }
 #endif
 #endif
+
+#ifndef MOC_ORACLE_OPACITY_H
+#define MOC_ORACLE_OPACITY_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ionizationDriver / BoltGaunt / getGauntFF / eDenSum / addOpacity
+ * (ionization_mod.f90:26-477) for every active cell.  The cross-sections
+ * come as the reference's per-shell stack (elementP ranges). */
+typedef struct oracle_opacity_in {
+    int nbins, nCells, nstages, nElemUsed;
+    const double* nuArray;      /* [B] */
+    const double* brems;        /* [B] xSecArray(bremsXSecP ...) */
+    const double* KNsigmaT;     /* [B] */
+    int lgCompton;
+    int nShells;
+    const int32_t* shZ;         /* [nShells] element Z */
+    const int32_t* shIon;       /* [nShells] stage */
+    const int32_t* shLo;        /* [nShells] 1-based first bin */
+    const int32_t* shHi;        /* [nShells] 1-based last bin */
+    const int64_t* shOff;       /* [nShells] offset into xs */
+    const double* xs;           /* stacked cross-sections [cm^2] */
+    const int32_t* elemZ;       /* [nElemUsed] */
+    const double* abund;        /* [nElemUsed] */
+    const double* ionDen;       /* [nstages][nElemUsed][C+1] (Fortran ionDen) */
+    const double* Ne;           /* [C+1] */
+    const double* Te;           /* [C+1] */
+    const double* Hden;         /* [C+1] */
+} oracle_opacity_in;
+
+void oracle_gaunt_ff(double z, double log10Te, const double* log10nu, int n, double* g);
+int oracle_opacity(const oracle_opacity_in* in, double* opacity /* [B][C+1] */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
