@@ -122,6 +122,21 @@ typedef struct moc_transport_stats {
     double kernel_ms;        /* device time of the transport kernel(s) [ms] */
 } moc_transport_stats;
 
+/* Elements, abundances and the photoionisation cross-section stack
+ * (xSec_mod: initXSecArray ph_mod.f90:204-517, setPointers
+ * hydro_mod.f90:422-677), summed per species (element, stage). */
+typedef struct moc_atomic {
+    int32_t nElemUsed;           /* elements switched on (lgElementOn) */
+    const int32_t* elemZ;        /* [nElemUsed] atomic numbers */
+    const double* abund;         /* [nElemUsed] n(X)/n(H) (elemAbun) */
+    int32_t nstages;             /* ionisation stages kept (ionDen extent) */
+    int32_t nSpecies;            /* photoionisable species */
+    const int32_t* speciesElem;  /* [nSpecies] 0-based index into elemZ */
+    const int32_t* speciesIon;   /* [nSpecies] stage, 1 = neutral */
+    const double* sigma;         /* [nSpecies][nbins] cross-section [cm^2] */
+    const double* brems;         /* [nbins] free-free factor 1.0368e-18/nu^3 */
+} moc_atomic;
+
 /* Create a context: uploads the mesh and source tables. */
 int moc_create(moc_ctx** ctx, const moc_tables* tables);
 void moc_destroy(moc_ctx* ctx);
@@ -137,6 +152,15 @@ int moc_upload(moc_ctx* ctx, int field, int ig, const void* host, int64_t count,
                int dtype, int layout);
 int moc_download(moc_ctx* ctx, int field, int ig, void* host, int64_t count,
                  int dtype, int layout);
+
+/* Upload the element set and cross-section stack.  The per-cell ionDen
+ * field (MOC_F_IONDEN, (0:nCells, nElemUsed, nstages)) exists once both the
+ * grid and the atomic set are known. */
+int moc_set_atomic(moc_ctx* ctx, const moc_atomic* atomic);
+
+/* kappa_nu of every cell from ionDen, Ne, Te, Hden (ionizationDriver,
+ * ionization_mod.f90:26-477; free-free Gaunt factors at each cell's T_e). */
+int moc_opacity(moc_ctx* ctx, int ig);
 
 /* Zero the J estimator and the escape tallies (iteration_mod.f90:626-717). */
 int moc_reset_tallies(moc_ctx* ctx, int ig);

@@ -30,7 +30,7 @@ CELLBIN_FIELDS = {"opacity", "Jste", "recPDF", "escaped", "emission"}
 EXPORTS = [
     "moc_create", "moc_destroy", "moc_last_error", "moc_grid_attach", "moc_upload",
     "moc_download", "moc_reset_tallies", "moc_transport", "moc_normalize_j",
-    "moc_device_ptr", "moc_version",
+    "moc_device_ptr", "moc_version", "moc_set_atomic", "moc_opacity",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -53,6 +53,14 @@ class MocGridDesc(C.Structure):
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
         ("xAxis", _dp), ("yAxis", _dp), ("zAxis", _dp), ("active", _ip),
         ("nCells", C.c_int32), ("lgSymmetricXYZ", C.c_int32), ("R_out", C.c_double),
+    ]
+
+
+class MocAtomic(C.Structure):
+    _fields_ = [
+        ("nElemUsed", C.c_int32), ("elemZ", _ip), ("abund", _dp), ("nstages", C.c_int32),
+        ("nSpecies", C.c_int32), ("speciesElem", _ip), ("speciesIon", _ip), ("sigma", _dp),
+        ("brems", _dp),
     ]
 
 
@@ -99,6 +107,10 @@ def lib() -> C.CDLL:
     L.moc_device_ptr.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int)]
     L.moc_device_ptr.restype = C.c_int
+    L.moc_set_atomic.argtypes = [vp, C.POINTER(MocAtomic)]
+    L.moc_set_atomic.restype = C.c_int
+    L.moc_opacity.argtypes = [vp, C.c_int]
+    L.moc_opacity.restype = C.c_int
     L.moc_version.argtypes = []
     L.moc_version.restype = C.c_char_p
     _LIB = L

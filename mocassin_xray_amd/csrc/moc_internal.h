@@ -68,6 +68,25 @@ struct MocTransportArgs {
     uint32_t ctr_base;
 };
 
+// Opacity kernel view (opacity.hip)
+struct MocOpacityArgs {
+    int C1, nbins, nSpecies, nElemUsed, nstages, lgCompton;
+    const int* specElem;   // [S] index into elemZ
+    const int* specIon;    // [S] stage 1..nstages
+    const int* elemZ;      // [NE]
+    const double* abund;   // [NE]
+    const double* sigma;   // [S][B]
+    const double* brems;   // [B]
+    const double* nuArray; // [B]
+    const double* log10nu; // [B]
+    const double* KNsigmaT;// [B]
+    const double* ionDen;  // [NS][NE][C1]
+    const double* Ne;
+    const double* Te;
+    const double* Hden;
+    double* opacity;       // [B][C1]
+};
+
 struct MocGrid {
     bool attached = false;
     int nx = 0, ny = 0, nz = 0, nCells = 0, sym = 0;
@@ -94,6 +113,16 @@ struct MocGrid {
     float* f_recPDF = nullptr;
     float* f_totalLines = nullptr;
     bool f32_dirty = true;
+    double* d_ionDen = nullptr;      // [NS][NE][C+1]
+};
+
+struct MocAtomic {
+    bool set = false;
+    int nElemUsed = 0, nstages = 0, nSpecies = 0;
+    std::vector<int> elemZ, specElem, specIon;
+    std::vector<double> abund;
+    int *d_elemZ = nullptr, *d_specElem = nullptr, *d_specIon = nullptr;
+    double *d_abund = nullptr, *d_sigma = nullptr, *d_brems = nullptr, *d_log10nu = nullptr;
 };
 
 struct moc_ctx {
@@ -111,6 +140,7 @@ struct moc_ctx {
     double* d_Qphot = nullptr;
     unsigned long long* d_counters = nullptr;
     MocGrid grid;
+    MocAtomic atomic;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -120,3 +150,4 @@ int moc_launch_transport_f64(moc_ctx* ctx, const MocTransportArgs<double>& a, fl
 int moc_launch_transport_f32(moc_ctx* ctx, const MocTransportArgs<float>& a, float* ms);
 void moc_convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);
 void moc_scale_f64(double* p, int64_t n, double f, hipStream_t s);
+int moc_launch_opacity(moc_ctx* ctx, const MocOpacityArgs& a);

@@ -58,7 +58,30 @@ void free_grid(MocGrid& g) {
     dfree(g.f_ax); dfree(g.f_ay); dfree(g.f_az);
     dfree(g.f_vx); dfree(g.f_vy); dfree(g.f_vz);
     dfree(g.f_opacity); dfree(g.f_Ne); dfree(g.f_recPDF); dfree(g.f_totalLines);
+    dfree(g.d_ionDen);
     g = MocGrid();
+}
+
+void free_atomic(MocAtomic& a) {
+    dfree(a.d_elemZ); dfree(a.d_specElem); dfree(a.d_specIon);
+    dfree(a.d_abund); dfree(a.d_sigma); dfree(a.d_brems); dfree(a.d_log10nu);
+    a = MocAtomic();
+}
+
+int64_t ionden_count(moc_ctx* ctx) {
+    return ((int64_t)ctx->grid.nCells + 1) * ctx->atomic.nElemUsed * ctx->atomic.nstages;
+}
+
+// the ionDen plane set exists once both the grid and the element set are known
+int ensure_ionden(moc_ctx* ctx) {
+    MocGrid& g = ctx->grid;
+    if (!g.attached || !ctx->atomic.set || g.d_ionDen) return MOC_OK;
+    const size_t n = (size_t)ionden_count(ctx);
+    int rc = dalloc_n(ctx, &g.d_ionDen, n, "ionDen");
+    if (rc) return rc;
+    MOC_TRY(hipMemsetAsync(g.d_ionDen, 0, n * 8, ctx->stream), "zero ionDen");
+    MOC_TRY(hipStreamSynchronize(ctx->stream), "zero ionDen");
+    return MOC_OK;
 }
 
 // cell widths of the volume element (photon_mod.f90:2169-2223), in 1e15 cm:
@@ -93,6 +116,7 @@ double* field_ptr(moc_ctx* ctx, int f) {
         case MOC_F_TE: return g.d_Te;
         case MOC_F_HDEN: return g.d_Hden;
         case MOC_F_ESCAPED: return g.d_escaped;
+        case MOC_F_IONDEN: return g.d_ionDen;
         default: return nullptr;
     }
 }
@@ -100,6 +124,13 @@ double* field_ptr(moc_ctx* ctx, int f) {
 // device storage order of a (cell, bin) field: recPDF is bin-fastest (rows
 // searched per cell), the others cell-fastest (planes walked per bin)
 int device_layout(int f) { return f == MOC_F_RECPDF ? MOC_LAYOUT_CELLROW : MOC_LAYOUT_FORTRAN; }
+
+int64_t field_count(moc_ctx* ctx, int f) {
+    const int64_t C1 = ctx->grid.nCells + 1;
+    if (is_cellbin(f)) return C1 * ctx->nbins;
+    if (f == MOC_F_IONDEN) return ionden_count(ctx);
+    return C1;
+}
 
 int refresh_f32(moc_ctx* ctx) {
     MocGrid& g = ctx->grid;
@@ -192,6 +223,7 @@ int moc_create(moc_ctx** out, const moc_tables* t) {
 void moc_destroy(moc_ctx* ctx) {
     if (!ctx) return;
     free_grid(ctx->grid);
+    free_atomic(ctx->atomic);
     dfree(ctx->d_nu); dfree(ctx->d_KNsigmaT); dfree(ctx->d_KNcdf); dfree(ctx->d_Pcomp);
     dfree(ctx->d_sourceCDF);
     dfree(ctx->f_nu); dfree(ctx->f_KNsigmaT); dfree(ctx->f_KNcdf); dfree(ctx->f_Pcomp);
@@ -285,6 +317,78 @@ int moc_grid_attach(moc_ctx* ctx, int ig, const moc_grid_desc* gd) {
     MOC_TRY(hipStreamSynchronize(ctx->stream), "grid attach");
     g.f32_dirty = true;
     g.attached = true;
+    return ensure_ionden(ctx);
+}
+
+int moc_set_atomic(moc_ctx* ctx, const moc_atomic* at) {
+    if (!ctx) return MOC_ERR_ARG;
+    if (!at || at->nElemUsed < 1 || at->nElemUsed > 30 || at->nstages < 1 || at->nstages > 31 ||
+        at->nSpecies < 0 || !at->elemZ || !at->abund || (at->nSpecies > 0 && (!at->speciesElem ||
+        !at->speciesIon || !at->sigma)) || !at->brems) {
+        ctx->err = "moc_set_atomic: invalid element set";
+        return MOC_ERR_ARG;
+    }
+    for (int s = 0; s < at->nSpecies; ++s)
+        if (at->speciesElem[s] < 0 || at->speciesElem[s] >= at->nElemUsed || at->speciesIon[s] < 1 ||
+            at->speciesIon[s] > at->nstages) {
+            ctx->err = "moc_set_atomic: species index outside the element/stage range";
+            return MOC_ERR_ARG;
+        }
+    const bool same_shape = ctx->atomic.set && ctx->atomic.nElemUsed == at->nElemUsed &&
+                            ctx->atomic.nstages == at->nstages;
+    free_atomic(ctx->atomic);
+    if (!same_shape) dfree(ctx->grid.d_ionDen);
+    MocAtomic& a = ctx->atomic;
+    a.nElemUsed = at->nElemUsed;
+    a.nstages = at->nstages;
+    a.nSpecies = at->nSpecies;
+    a.elemZ.assign(at->elemZ, at->elemZ + a.nElemUsed);
+    a.abund.assign(at->abund, at->abund + a.nElemUsed);
+    a.specElem.assign(at->speciesElem, at->speciesElem + a.nSpecies);
+    a.specIon.assign(at->speciesIon, at->speciesIon + a.nSpecies);
+    const size_t B = ctx->nbins, S = a.nSpecies;
+    std::vector<double> lnu(B);
+    for (size_t b = 0; b < B; ++b) lnu[b] = log10(ctx->nuArray[b]);
+    int rc;
+    if ((rc = dalloc_n(ctx, &a.d_elemZ, a.nElemUsed, "elemZ")) ||
+        (rc = dalloc_n(ctx, &a.d_abund, a.nElemUsed, "abund")) ||
+        (rc = dalloc_n(ctx, &a.d_specElem, S, "specElem")) ||
+        (rc = dalloc_n(ctx, &a.d_specIon, S, "specIon")) ||
+        (rc = dalloc_n(ctx, &a.d_sigma, S * B, "sigma")) ||
+        (rc = dalloc_n(ctx, &a.d_brems, B, "brems")) ||
+        (rc = dalloc_n(ctx, &a.d_log10nu, B, "log10nu")))
+        return rc;
+    if ((rc = h2d(ctx, a.d_elemZ, at->elemZ, a.nElemUsed * 4, "elemZ")) ||
+        (rc = h2d(ctx, a.d_abund, at->abund, a.nElemUsed * 8, "abund")) ||
+        (S && (rc = h2d(ctx, a.d_specElem, at->speciesElem, S * 4, "specElem"))) ||
+        (S && (rc = h2d(ctx, a.d_specIon, at->speciesIon, S * 4, "specIon"))) ||
+        (S && (rc = h2d(ctx, a.d_sigma, at->sigma, S * B * 8, "sigma"))) ||
+        (rc = h2d(ctx, a.d_brems, at->brems, B * 8, "brems")) ||
+        (rc = h2d(ctx, a.d_log10nu, lnu.data(), B * 8, "log10nu")))
+        return rc;
+    a.set = true;
+    return ensure_ionden(ctx);
+}
+
+int moc_opacity(moc_ctx* ctx, int ig) {
+    if (!ctx) return MOC_ERR_ARG;
+    MocGrid& g = ctx->grid;
+    if (!g.attached || ig != 1 || !ctx->atomic.set || !g.d_ionDen) {
+        ctx->err = "moc_opacity: needs an attached grid and moc_set_atomic";
+        return MOC_ERR_STATE;
+    }
+    MocAtomic& at = ctx->atomic;
+    MocOpacityArgs a;
+    a.C1 = g.nCells + 1; a.nbins = ctx->nbins; a.nSpecies = at.nSpecies;
+    a.nElemUsed = at.nElemUsed; a.nstages = at.nstages; a.lgCompton = ctx->lgCompton;
+    a.specElem = at.d_specElem; a.specIon = at.d_specIon; a.elemZ = at.d_elemZ;
+    a.abund = at.d_abund; a.sigma = at.d_sigma; a.brems = at.d_brems; a.nuArray = ctx->d_nu;
+    a.log10nu = at.d_log10nu; a.KNsigmaT = ctx->d_KNsigmaT; a.ionDen = g.d_ionDen;
+    a.Ne = g.d_Ne; a.Te = g.d_Te; a.Hden = g.d_Hden; a.opacity = g.d_opacity;
+    int rc = moc_launch_opacity(ctx, a);
+    if (rc) return rc;
+    MOC_TRY(hipStreamSynchronize(ctx->stream), "opacity");
+    g.f32_dirty = true;
     return MOC_OK;
 }
 
@@ -297,7 +401,7 @@ int moc_upload(moc_ctx* ctx, int field, int ig, const void* host, int64_t count,
     if (!d || !host) { ctx->err = "moc_upload: unknown field"; return MOC_ERR_ARG; }
     const int64_t C1 = g.nCells + 1, B = ctx->nbins;
     const bool cb = is_cellbin(field);
-    const int64_t need = cb ? C1 * B : C1;
+    const int64_t need = field_count(ctx, field);
     if (count != need) { ctx->err = "moc_upload: element count does not match the field"; return MOC_ERR_ARG; }
     if (dtype != MOC_DT_F32 && dtype != MOC_DT_F64) { ctx->err = "moc_upload: dtype"; return MOC_ERR_ARG; }
     if (cb && layout != MOC_LAYOUT_FORTRAN && layout != MOC_LAYOUT_CELLROW) { ctx->err = "moc_upload: layout"; return MOC_ERR_ARG; }
@@ -329,7 +433,7 @@ int moc_download(moc_ctx* ctx, int field, int ig, void* host, int64_t count, int
     if (!d || !host) { ctx->err = "moc_download: unknown field"; return MOC_ERR_ARG; }
     const int64_t C1 = g.nCells + 1, B = ctx->nbins;
     const bool cb = is_cellbin(field);
-    const int64_t need = cb ? C1 * B : C1;
+    const int64_t need = field_count(ctx, field);
     if (count != need) { ctx->err = "moc_download: element count does not match the field"; return MOC_ERR_ARG; }
     if (dtype != MOC_DT_F32 && dtype != MOC_DT_F64) { ctx->err = "moc_download: dtype"; return MOC_ERR_ARG; }
     std::vector<double> buf((size_t)need);
@@ -359,7 +463,7 @@ int moc_device_ptr(moc_ctx* ctx, int field, int ig, void** dptr, int64_t* count,
     double* d = field_ptr(ctx, field);
     if (!d) { ctx->err = "moc_device_ptr: unknown field"; return MOC_ERR_ARG; }
     *dptr = d;
-    if (count) *count = is_cellbin(field) ? (int64_t)(g.nCells + 1) * ctx->nbins : g.nCells + 1;
+    if (count) *count = field_count(ctx, field);
     if (dtype) *dtype = MOC_DT_F64;
     g.f32_dirty = true;  // the caller may write through the pointer
     return MOC_OK;

@@ -21,7 +21,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import (CELLBIN_FIELDS, FIELDS, MOC_DT_F64, MOC_LAYOUT_FORTRAN, MOC_PREC_FP64,
-                   MOC_PREC_MIXED, MocGridDesc, MocTables, MocTransportStats, dptr, iptr)
+                   MOC_PREC_MIXED, MocAtomic, MocGridDesc, MocTables, MocTransportStats, dptr,
+                   iptr)
 
 
 class MocError(RuntimeError):
@@ -133,13 +134,38 @@ class MocGPU:
 
     def _count(self, name: str) -> int:
         c1 = self.grid.nCells + 1
+        if name == "ionDen":
+            return c1 * self.n_elem * self.nstages
         return c1 * self.nbins if name in CELLBIN_FIELDS else c1
+
+    # ----------------------------------------------------------------- atomic
+    def set_atomic(self, elem_z, abund, nstages: int, xsec) -> None:
+        """Element set, abundances and the cross-section stack (an
+        atomic.XSecStack built on the same mesh)."""
+        z = _c(elem_z, np.int32)
+        zi = {int(v): i for i, v in enumerate(z)}
+        spec_e = _c([zi[e] for e, _ in xsec.species], np.int32)
+        spec_i = _c([i for _, i in xsec.species], np.int32)
+        self._atomic_keep = [z, _c(abund), spec_e, spec_i, _c(xsec.sigma), _c(xsec.brems)]
+        k = self._atomic_keep
+        a = MocAtomic(nElemUsed=z.size, elemZ=iptr(k[0]), abund=dptr(k[1]), nstages=int(nstages),
+                      nSpecies=spec_e.size, speciesElem=iptr(k[2]), speciesIon=iptr(k[3]),
+                      sigma=dptr(k[4]), brems=dptr(k[5]))
+        self._check(self._L.moc_set_atomic(self._ctx, C.byref(a)), "moc_set_atomic")
+        self.n_elem, self.nstages = int(z.size), int(nstages)
+
+    def ionizationDriver(self):
+        """Opacity of every cell from ionDen, Ne, Te, Hden (ionization_mod.f90:26)."""
+        self._check(self._L.moc_opacity(self._ctx, 1), "moc_opacity")
 
     def upload(self, name: str, arr: np.ndarray):
         """Upload a field given in MOCASSIN shape ((C+1,) or (C+1, B))."""
         c1 = self.grid.nCells + 1
         a = np.asarray(arr, dtype=np.float64)
-        if name in CELLBIN_FIELDS:
+        if name == "ionDen":             # (C+1, nElemUsed, nstages) -> Fortran order
+            assert a.shape == (c1, self.n_elem, self.nstages), a.shape
+            buf = np.ascontiguousarray(a.transpose(2, 1, 0))
+        elif name in CELLBIN_FIELDS:
             assert a.shape == (c1, self.nbins), (name, a.shape)
             buf = np.ascontiguousarray(a.T)      # Fortran order: cell fastest
         else:
@@ -156,6 +182,8 @@ class MocGPU:
                                          MOC_DT_F64, MOC_LAYOUT_FORTRAN), f"moc_download({name})")
         if name in CELLBIN_FIELDS:
             return np.ascontiguousarray(buf.reshape(self.nbins, c1).T)
+        if name == "ionDen":
+            return np.ascontiguousarray(buf.reshape(self.nstages, self.n_elem, c1).transpose(2, 1, 0))
         return buf
 
     # -------------------------------------------------------------- transport

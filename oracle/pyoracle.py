@@ -118,3 +118,63 @@ def transport(model_tables, grid, fields, n, seed, ctr_base=0, nthreads=1, istar
     cnt = {n_: out.counters[i] for i, n_ in enumerate(COUNTER_NAMES)}
     cnt["rc"] = rc
     return J.reshape(B, C1).T.copy(), E.reshape(B, C1).T.copy(), cnt, out.Qphot
+
+
+class OpacityIn(C.Structure):
+    _fields_ = [
+        ("nbins", C.c_int), ("nCells", C.c_int), ("nstages", C.c_int), ("nElemUsed", C.c_int),
+        ("nuArray", _dp), ("brems", _dp), ("KNsigmaT", _dp), ("lgCompton", C.c_int),
+        ("nShells", C.c_int), ("shZ", _ip), ("shIon", _ip), ("shLo", _ip), ("shHi", _ip),
+        ("shOff", C.POINTER(C.c_int64)), ("xs", _dp), ("elemZ", _ip), ("abund", _dp),
+        ("ionDen", _dp), ("Ne", _dp), ("Te", _dp), ("Hden", _dp),
+    ]
+
+
+def gaunt_ff(z, log10Te, log10nu):
+    L = lib()
+    L.oracle_gaunt_ff.argtypes = [C.c_double, C.c_double, _dp, C.c_int, _dp]
+    x = np.ascontiguousarray(log10nu, dtype=np.float64)
+    g = np.zeros_like(x)
+    L.oracle_gaunt_ff(float(z), float(log10Te), x.ctypes.data_as(_dp), x.size,
+                      g.ctypes.data_as(_dp))
+    return g
+
+
+def opacity(xsec, kn_sigma, lgCompton, elem_z, abund, nstages, ionDen, Ne, Te, Hden):
+    """Oracle opacity (C+1, B) from the per-shell stack of an atomic.XSecStack.
+    ionDen has shape (C+1, nElemUsed, nstages)."""
+    L = lib()
+    L.oracle_opacity.argtypes = [C.POINTER(OpacityIn), _dp]
+    L.oracle_opacity.restype = C.c_int
+    B = xsec.nu.size
+    C1 = ionDen.shape[0]
+    sh = xsec.shells
+    keep = {
+        "nu": np.ascontiguousarray(xsec.nu, dtype=np.float64),
+        "br": np.ascontiguousarray(xsec.brems, dtype=np.float64),
+        "kn": np.ascontiguousarray(kn_sigma, dtype=np.float64),
+        "z": np.array([s.elem for s in sh], dtype=np.int32),
+        "ion": np.array([s.ion for s in sh], dtype=np.int32),
+        "lo": np.array([s.lo for s in sh], dtype=np.int32),
+        "hi": np.array([s.hi for s in sh], dtype=np.int32),
+        "off": np.cumsum([0] + [s.sigma.size for s in sh[:-1]]).astype(np.int64),
+        "xs": np.ascontiguousarray(np.concatenate([s.sigma for s in sh]) if sh else np.zeros(1)),
+        "ez": np.ascontiguousarray(elem_z, dtype=np.int32),
+        "ab": np.ascontiguousarray(abund, dtype=np.float64),
+        "id": np.ascontiguousarray(np.asarray(ionDen, dtype=np.float64).transpose(2, 1, 0)),
+        "ne": np.ascontiguousarray(Ne, dtype=np.float64),
+        "te": np.ascontiguousarray(Te, dtype=np.float64),
+        "h": np.ascontiguousarray(Hden, dtype=np.float64),
+    }
+    ip = lambda k: keep[k].ctypes.data_as(_ip)
+    dp = lambda k: keep[k].ctypes.data_as(_dp)
+    inp = OpacityIn(nbins=B, nCells=C1 - 1, nstages=int(nstages), nElemUsed=keep["ez"].size,
+                    nuArray=dp("nu"), brems=dp("br"), KNsigmaT=dp("kn"),
+                    lgCompton=int(bool(lgCompton)), nShells=len(sh), shZ=ip("z"), shIon=ip("ion"),
+                    shLo=ip("lo"), shHi=ip("hi"),
+                    shOff=keep["off"].ctypes.data_as(C.POINTER(C.c_int64)), xs=dp("xs"),
+                    elemZ=ip("ez"), abund=dp("ab"), ionDen=dp("id"), Ne=dp("ne"), Te=dp("te"),
+                    Hden=dp("h"))
+    out = np.zeros(B * C1)
+    L.oracle_opacity(C.byref(inp), out.ctypes.data_as(_dp))
+    return out.reshape(B, C1).T.copy()
