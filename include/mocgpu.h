@@ -137,6 +137,60 @@ typedef struct moc_atomic {
     const double* brems;         /* [nbins] free-free factor 1.0368e-18/nu^3 */
 } moc_atomic;
 
+/* Tables of the ionisation/thermal balance (update_mod.f90 and the data it
+ * reads).  Index conventions: Z and stages 1-based; [31][32] tables are
+ * indexed z*32+stage (or z*32+nElec).  The frequency integrals of the balance
+ * (photoionisation and heating per shell, getPhotoRates; the Compton sums)
+ * are given as weight columns W[nbins][ncol], ncol = 3*nShells + 5:
+ * per shell (gamma, heat below 7.353 Ryd, heat above -- scaled by the
+ * secondary-ionisation efficiency), then sum J, sum J/nu, H recoil, He
+ * recoil, and a hit column (1 in every bin). */
+typedef struct moc_balance_tables {
+    int32_t maxLev;              /* n-level truncation of the cooling ions */
+    double XHILimit;             /* convergence limit on X(H0) (XHILimit) */
+    int32_t nShells;
+    const int32_t* shellNum;     /* [nShells] shell number, grouped by species */
+    const int32_t* speciesShell; /* [nSpecies+1] shell range of each species */
+    const double* W;             /* [nbins][3*nShells+5] */
+    const int32_t* outShell;     /* [31*32] (Z, nElec) */
+    const int32_t* statW0;       /* [31*32] g0 (getOuterShell) */
+    const int32_t* statW1;       /* [31*32] g1 */
+    const double* rrb;           /* [6][31][31] Badnell RR (A,B,T0,T1,C,T2) by (Z, N) */
+    const double* drb_c;         /* [9][31][31] Badnell DR c */
+    const double* drb_e;         /* [9][31][31] Badnell DR E */
+    const double* rrec;          /* [2][31][31] radRecFit power laws */
+    const double* rnew;          /* [4][31][31] radRecFit Verner-Ferland fits */
+    int32_t nDiel;
+    const double* diel;          /* [nDiel][8] elem n a b c d f g */
+    int32_t nAldro;
+    const double* aldro;         /* [nAldro][6] elem n a b t0 t1 */
+    const double* cionCF;        /* [5][31][31] Voronov by (Z, nElec) */
+    const double* cionTherm;     /* [31][32] */
+    const int32_t* nAuger;       /* [31][32][11] */
+    const double* auger;         /* [31][32][11][11] */
+    int32_t nChex;
+    const int32_t* chexZ;        /* [nChex] */
+    const int32_t* chexIon;      /* [nChex] */
+    const double* chexFit;       /* [nChex][4] a[1e-9 cm^3/s] b c d */
+    int32_t nAtoms;
+    const int32_t* atomIdx;      /* [31*32] n-level ion or -1 */
+    const int64_t* atomMeta;     /* [nAtoms][8] nLev nT offT offLev offA offQ iup2p 0 */
+    const double* atomAtp;       /* [nAtoms] */
+    int64_t nLogT, nLev, nA, nQ;
+    const double* atomLogT;
+    const double* atomG;
+    const double* atomE;
+    const double* atomA;
+    const double* atomQom;       /* Upsilon [nT][nLev][nLev] per ion */
+    const double* atomQ2;        /* natural-spline second derivatives in log T */
+} moc_balance_tables;
+
+typedef struct moc_update_stats {
+    double noHit, noIonBal, noTeBal; /* grid%noHit, noIonBal, noTeBal */
+    double converged;                /* cells with lgConverged = 1 */
+    double kernel_ms;
+} moc_update_stats;
+
 /* Create a context: uploads the mesh and source tables. */
 int moc_create(moc_ctx** ctx, const moc_tables* tables);
 void moc_destroy(moc_ctx* ctx);
@@ -161,6 +215,14 @@ int moc_set_atomic(moc_ctx* ctx, const moc_atomic* atomic);
 /* kappa_nu of every cell from ionDen, Ne, Te, Hden (ionizationDriver,
  * ionization_mod.f90:26-477; free-free Gaunt factors at each cell's T_e). */
 int moc_opacity(moc_ctx* ctx, int ig);
+
+/* Upload the balance tables (after moc_set_atomic). */
+int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* tables);
+
+/* updateCell for every active cell (update_mod.f90:15): from the normalised
+ * J (moc_normalize_j) and Te, Ne, Hden, ionDen, writes Te, Ne, ionDen,
+ * lgConverged, lgBlack. */
+int moc_update(moc_ctx* ctx, int ig, moc_update_stats* stats);
 
 /* Zero the J estimator and the escape tallies (iteration_mod.f90:626-717). */
 int moc_reset_tallies(moc_ctx* ctx, int ig);

@@ -30,7 +30,8 @@ CELLBIN_FIELDS = {"opacity", "Jste", "recPDF", "escaped", "emission"}
 EXPORTS = [
     "moc_create", "moc_destroy", "moc_last_error", "moc_grid_attach", "moc_upload",
     "moc_download", "moc_reset_tallies", "moc_transport", "moc_normalize_j",
-    "moc_device_ptr", "moc_version", "moc_set_atomic", "moc_opacity",
+    "moc_device_ptr", "moc_version", "moc_set_atomic", "moc_opacity", "moc_set_balance",
+    "moc_update",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -62,6 +63,32 @@ class MocAtomic(C.Structure):
         ("nSpecies", C.c_int32), ("speciesElem", _ip), ("speciesIon", _ip), ("sigma", _dp),
         ("brems", _dp),
     ]
+
+
+_lp = C.POINTER(C.c_int64)
+
+
+class MocBalanceTables(C.Structure):
+    _fields_ = [
+        ("maxLev", C.c_int32), ("XHILimit", C.c_double), ("nShells", C.c_int32),
+        ("shellNum", _ip), ("speciesShell", _ip), ("W", _dp), ("outShell", _ip),
+        ("statW0", _ip), ("statW1", _ip), ("rrb", _dp), ("drb_c", _dp), ("drb_e", _dp),
+        ("rrec", _dp), ("rnew", _dp), ("nDiel", C.c_int32), ("diel", _dp),
+        ("nAldro", C.c_int32), ("aldro", _dp), ("cionCF", _dp), ("cionTherm", _dp),
+        ("nAuger", _ip), ("auger", _dp), ("nChex", C.c_int32), ("chexZ", _ip),
+        ("chexIon", _ip), ("chexFit", _dp), ("nAtoms", C.c_int32), ("atomIdx", _ip),
+        ("atomMeta", _lp), ("atomAtp", _dp), ("nLogT", C.c_int64), ("nLev", C.c_int64),
+        ("nA", C.c_int64), ("nQ", C.c_int64), ("atomLogT", _dp), ("atomG", _dp),
+        ("atomE", _dp), ("atomA", _dp), ("atomQom", _dp), ("atomQ2", _dp),
+    ]
+
+
+class MocUpdateStats(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("noHit", "noIonBal", "noTeBal", "converged",
+                                           "kernel_ms")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 class MocTransportStats(C.Structure):
@@ -111,6 +138,10 @@ def lib() -> C.CDLL:
     L.moc_set_atomic.restype = C.c_int
     L.moc_opacity.argtypes = [vp, C.c_int]
     L.moc_opacity.restype = C.c_int
+    L.moc_set_balance.argtypes = [vp, C.POINTER(MocBalanceTables)]
+    L.moc_set_balance.restype = C.c_int
+    L.moc_update.argtypes = [vp, C.c_int, C.POINTER(MocUpdateStats)]
+    L.moc_update.restype = C.c_int
     L.moc_version.argtypes = []
     L.moc_version.restype = C.c_char_p
     _LIB = L

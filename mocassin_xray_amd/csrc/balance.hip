@@ -1,0 +1,588 @@
+// Per-cell ionisation and thermal balance on gfx950 (updateCell,
+// update_mod.f90:15-482 and the routines it calls).
+//
+// Two kernels:
+//  k_rates   -- every frequency integral of the balance is linear in J_nu:
+//               per-shell photoionisation and heating rates (photoionisation /
+//               getPhotoRates, :3584-3832, :3971-4040) and the Compton sums
+//               (compton :3834-3897).  They are one contraction
+//               R[cell, col] = sum_nu J[nu, cell] W[nu, col] with weight columns
+//               built once on the host; tiles of 64 cells x 32 columns stage J
+//               and W through LDS, J planes are read cell-fastest (coalesced).
+//  k_balance -- one lane per cell: the T_e root-find (iterateT :277-482)
+//               around the ion-fraction chain (ionBalance :1575-1992) and the
+//               heating/cooling integrals (thermBalance :887-1215) with the
+//               n-level forbidden-line cooling (equilibrium, emission_mod.f90:
+//               2505-3106).  Everything in FP64.  Recombination coefficients
+//               (calcAlpha :2298-2413) and collisional ionisation (:3552-3582)
+//               are evaluated per iteration from tables resident in HBM/L2;
+//               the Upsilon(T) splines use second derivatives computed once on
+//               the host.
+#include <hip/hip_runtime.h>
+
+#include "moc_internal.h"
+
+namespace {
+
+constexpr int NEL = 30;
+constexpr double kHcRyd = 2.1799153e-11;  // update_mod.f90:64
+constexpr double kRyd2erg = 2.1799e-11;   // constants_mod.f90:20
+constexpr double kBoltz = 1.380658e-16;   // constants_mod.f90:17
+constexpr double kXMax = 0.99999;         // constants_mod.f90:55
+
+// ---------------------------------------------------------------- k_rates
+constexpr int RC = 64;  // cells per tile
+constexpr int RK = 32;  // bins per LDS stage
+constexpr int RN = 32;  // columns per tile
+
+__global__ void __launch_bounds__(256) k_rates(const double* __restrict__ J, int C1, int B,
+                                               const double* __restrict__ W, int ncol,
+                                               double* __restrict__ R) {
+    __shared__ double sJ[RK][RC];
+    __shared__ double sW[RK][RN];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 4 waves x 8 columns
+    const int c0 = blockIdx.x * RC, n0 = blockIdx.y * RN;
+    double acc[RN / 4];
+#pragma unroll
+    for (int j = 0; j < RN / 4; ++j) acc[j] = 0.0;
+    for (int b0 = 0; b0 < B; b0 += RK) {
+        for (int k = threadIdx.x; k < RK * RC; k += 256) {
+            const int kb = k / RC, kc = k % RC, b = b0 + kb, c = c0 + kc;
+            sJ[kb][kc] = (b < B && c < C1) ? J[(size_t)b * C1 + c] : 0.0;
+        }
+        for (int k = threadIdx.x; k < RK * RN; k += 256) {
+            const int kb = k / RN, kn = k % RN, b = b0 + kb, n = n0 + kn;
+            sW[kb][kn] = (b < B && n < ncol) ? W[(size_t)b * ncol + n] : 0.0;
+        }
+        __syncthreads();
+        for (int kb = 0; kb < RK; ++kb) {
+            const double j = sJ[kb][tx];
+#pragma unroll
+            for (int q = 0; q < RN / 4; ++q) acc[q] += j * sW[kb][ty * (RN / 4) + q];
+        }
+        __syncthreads();
+    }
+    const int c = c0 + tx;
+    if (c >= C1) return;
+#pragma unroll
+    for (int q = 0; q < RN / 4; ++q) {
+        const int n = n0 + ty * (RN / 4) + q;
+        if (n < ncol) R[(size_t)n * C1 + c] = acc[q];
+    }
+}
+
+// ---------------------------------------------------------------- k_balance
+struct CellState {
+    double Te, Ne, H;
+    double X[MOC_MAXE][MOC_MAXS + 1];      // ion fractions by (element slot, stage)
+    double alpha[MOC_MAXE][MOC_MAXS + 1];  // recombination coefficients
+    double cI1[MOC_MAXE][MOC_MAXS + 1];    // collisional ionisation rate
+    double cI2[MOC_MAXE][MOC_MAXS + 1];    // collisional ionisation cooling
+    double heatef;
+    int nIterX;
+};
+
+__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
+
+__device__ __forceinline__ double t3(const double* a, int i, int z, int n) {
+    return a[((size_t)i * 31 + z) * 31 + n];
+}
+
+// recombination coefficients (calcAlpha with Badnell fits, :2298-2362)
+__device__ void recomb(const MocBalanceArgs& a, CellState& s) {
+    const double T = s.Te;
+    const double tt = T > 1000. ? T / 10000. : 0.1;
+    // dielectronic fits that are not Badnell's (dielectronic :3204-3305)
+    double diO[MOC_MAXS + 1];  // oxygen's, borrowed by ions without their own
+    const int eO = a.elemSlot[8];
+    for (int i = 0; i <= MOC_MAXS; ++i) diO[i] = 0.0;
+    for (int e = 0; e < a.NE; ++e)
+        for (int i = 0; i <= MOC_MAXS; ++i) s.alpha[e][i] = 0.0;
+    for (int e = 0; e < a.NE; ++e) {
+        const int z = a.elemZ[e];
+        for (int ion = 1; ion <= imin(a.NS - 1, z); ++ion) {
+            const int nel = z - ion;
+            if (nel > 11) continue;
+            const double bb = t3(a.rrb, 1, z, nel) + t3(a.rrb, 4, z, nel) * exp(-t3(a.rrb, 5, z, nel) / T);
+            const double s0 = sqrt(T / t3(a.rrb, 2, z, nel)), s1 = sqrt(T / t3(a.rrb, 3, z, nel));
+            double dr = 0.0;
+            for (int k = 0; k < 9; ++k) {
+                const double E = t3(a.drb_e, k, z, nel);
+                if (E / T < 87.) dr += t3(a.drb_c, k, z, nel) * exp(-E / T);
+            }
+            s.alpha[e][ion] = t3(a.rrb, 0, z, nel) / (s0 * pow(1. + s0, 1. - bb) * pow(1. + s1, 1. + bb)) +
+                              dr * pow(T, -1.5);
+        }
+    }
+    // diRec for the ions with more than 11 electrons, and oxygen's row
+    auto direc = [&](int z, int ion) -> double {
+        double v = 0.0;
+        if (T > 60000.) {
+            for (int i = 0; i < a.nAldro; ++i) {
+                const double* r = a.aldro + 6 * i;
+                if ((int)r[0] == z && (int)r[0] - (int)r[1] + 1 == ion)
+                    v = r[2] * pow(T, -1.5) * exp(-r[4] / T) * (1. + r[3] * exp(-r[5] / T));
+            }
+            return v;
+        }
+        for (int i = 0; i < a.nDiel; ++i) {
+            const double* r = a.diel + 8 * i;
+            const int g = (int)r[7];
+            if ((int)r[0] != z || (int)r[0] + 1 - (int)r[1] != ion) continue;
+            if (g == 0 || (g == 1 && T < 20000.) || (g == 2 && T >= 20000.))
+                v = 1.e-12 * (r[2] / tt + r[3] + r[4] * tt + r[5] * tt * tt) * pow(tt, -1.5) * exp(-r[6] / tt);
+        }
+        return v;
+    };
+    if (eO >= 0)
+        for (int ion = 1; ion < 7 && ion <= a.NS; ++ion) diO[ion] = direc(8, ion);
+    for (int e = 0; e < a.NE; ++e) {
+        const int z = a.elemZ[e];
+        for (int ion = 1; ion <= imin(a.NS - 1, z); ++ion) {
+            if (z - ion <= 11) continue;
+            const int n = z - ion + 1;
+            double rr;
+            if (n <= 3 || n == 11 || (z > 5 && z < 9) || z == 10 || (z == 26 && n > 11)) {
+                const double q = sqrt(T / t3(a.rnew, 2, z, n)), p = t3(a.rnew, 1, z, n);
+                rr = t3(a.rnew, 0, z, n) / (q * pow(q + 1., 1. - p) * pow(1. + sqrt(T / t3(a.rnew, 3, z, n)), 1. + p));
+            } else {
+                const double q = T * 1.e-4;
+                if (z == 26 && n <= 13) {
+                    const double fe0[10] = {4.33e-10, 3.91e-10, 3.49e-10, 3.16e-10, 2.96e-10,
+                                            2.59e-10, 2.24e-10, 1.91e-10, 1.68e-10, 1.46e-10};
+                    const double fe1[10] = {0.531, 0.523, 0.521, 0.534, 0.557, 0.567, 0.579, 0.601, 0.602, 0.597};
+                    const double fe2[10] = {5.77e-02, 6.15e-02, 6.22e-02, 6.02e-02, 5.79e-02,
+                                            5.65e-02, 5.49e-02, 5.10e-02, 5.07e-02, 5.22e-02};
+                    rr = fe0[n - 4] / pow(q, fe1[n - 4] + fe2[n - 4] * log10(q));
+                } else {
+                    rr = t3(a.rrec, 0, z, n) / pow(q, t3(a.rrec, 1, z, n));
+                }
+            }
+            double di = direc(z, ion);
+            if (di == 0. && ion < 7 && eO >= 0) di = diO[ion];
+            s.alpha[e][ion] = rr + (di > 0. ? di : 0.);
+        }
+    }
+}
+
+// Voronov collisional ionisation with the reference's early exit (:3552-3582)
+__device__ void collion(const MocBalanceArgs& a, CellState& s) {
+    const double te = s.Te * 8.617385e-05;
+    for (int e = 0; e < a.NE; ++e)
+        for (int i = 0; i <= MOC_MAXS; ++i) { s.cI1[e][i] = 0.0; s.cI2[e][i] = 0.0; }
+    for (int e = 0; e < a.NE; ++e) {
+        const int z = a.elemZ[e];
+        for (int ion = 1; ion <= imin(a.NS - 1, z - 1); ++ion) {
+            const int el = z - ion + 1;
+            const double u = t3(a.cion_cf, 0, z, el) / te;
+            if (u > 80.0) return;
+            s.cI1[e][ion] = t3(a.cion_cf, 2, z, el) * (1.0 + t3(a.cion_cf, 1, z, el) * sqrt(u)) /
+                            (t3(a.cion_cf, 3, z, el) + u) * pow(u, t3(a.cion_cf, 4, z, el)) * exp(-u);
+            s.cI2[e][ion] = a.cion_th[z * 32 + ion] * s.cI1[e][ion] * kBoltz;
+        }
+    }
+}
+
+// charge-exchange coefficient [cm^3/s]; T-scaled unless raw (ionBalance keeps the
+// heavy elements' coefficients unscaled)
+__device__ double chex(const MocBalanceArgs& a, int z, int ion, double T, bool scaled) {
+    const int k = a.chexIdx[z * 32 + ion];
+    if (k < 0) return 0.0;
+    const double* c = a.chexTab + 4 * k;
+    const double v0 = c[0] * 1.e-9;
+    if (!scaled) return v0;
+    if (T < 6000. || T > 5.e4) return 0.0;
+    const double t4 = T / 10000.;
+    const double v = v0 * pow(t4, c[1]) * (1. + c[2] * exp(c[3] * t4));
+    return v < 0. ? 0. : v;
+}
+
+__device__ __forceinline__ double dEk(int z, int ion) {
+    return (z == 7 && ion == 1) ? 10863. : ((z == 8 && ion == 1) ? 2205. : 0.);
+}
+
+// photoionisation rate of species (e, ion): shells rates from k_rates, the
+// reverse charge exchange added to the outer shell (:3715-3830)
+struct Rates {
+    const double* R;
+    int C1, cell, nShells;
+    __device__ double at(int col) const { return R[(size_t)col * C1 + cell]; }
+    __device__ double gamma(int s) const { return at(3 * s); }
+    __device__ double heat(int s, double heatef) const { return at(3 * s + 1) + heatef * at(3 * s + 2); }
+};
+
+__device__ double rev_chex(const MocBalanceArgs& a, const CellState& s, int z, int ion) {
+    if (!(dEk(z, ion) > 1.)) return 0.0;
+    const int nel = z - ion + 1;
+    const double g0g1 = (double)a.statW[z * 32 + nel] / (double)a.statW1[z * 32 + nel];
+    return chex(a, z, ion, s.Te, true) * (1. - s.X[a.elemSlot[1]][1]) * s.H * 2. * exp(-dEk(z, ion) / s.Te) / g0g1;
+}
+
+// H I / He I / He II have one shell; out = photoionisations of the ion
+__device__ double gamma_out(const MocBalanceArgs& a, const Rates& r, int spec) {
+    double g = 0.0;
+    for (int sh = a.specShell[spec]; sh < a.specShell[spec + 1]; ++sh) g += r.gamma(sh);
+    return g;
+}
+
+__device__ bool ion_balance(const MocBalanceArgs& a, const Rates& r, CellState& s,
+                            const double revFrozen[2]) {
+    const int NS = a.NS;
+    const int eH = a.elemSlot[1], eHe = a.elemSlot[2];
+    bool conv = true;
+    for (;;) {
+        double expFact = 157893.94 / s.Te;
+        if (expFact > 75.) expFact = 75.;
+        double collIonH = 2.75E-16 * s.Te * sqrt(s.Te) * (157893.94 / s.Te + 2.) * exp(-expFact);
+        const double HIOld = s.X[eH][1];
+        const double HeIOld = eHe >= 0 ? s.X[eHe][1] : 0.0;
+        const double HeIIOld = eHe >= 0 ? s.X[eHe][2] : 0.0;
+        const double H0 = s.X[eH][1];
+        // H and He ratio chains (:1715-1790)
+        for (int zz = 1; zz <= 2; ++zz) {
+            const int e = a.elemSlot[zz];
+            if (e < 0) continue;
+            double ratio[3] = {0., 0., 0.};
+            for (int ion = 1; ion <= imin(zz, NS - 1); ++ion) {
+                const int spec = a.specOf[e * 32 + ion];
+                const double cx = chex(a, zz, ion, s.Te, true);
+                if (zz > 1 || ion > 1) collIonH = 0.;
+                const double g = (spec >= 0 ? gamma_out(a, r, spec) : 0.0) + 1.e-20;
+                ratio[ion] = (g + collIonH * s.Ne) / (s.Ne * s.alpha[e][ion] + cx * s.H * H0);
+            }
+            double prod[3] = {1., 1., 1.}, den = 1.;
+            for (int ion = 1; ion <= imin(zz, NS - 1); ++ion) {
+                for (int i = 1; i <= ion; ++i) prod[ion] *= ratio[i];
+                den += prod[ion];
+            }
+            for (int ion = 1; ion <= imin(zz + 1, NS); ++ion) {
+                double v = ion == 1 ? 1. / den : prod[ion - 1] / den;
+                v = v > kXMax ? kXMax : v;
+                v = v < 1.e-20 ? 1.e-20 : v;
+                s.X[e][ion] = v;
+            }
+        }
+        // heavy elements (:1826-1929)
+        for (int e = 0; e < a.NE; ++e) {
+            const int z = a.elemZ[e];
+            if (z < 3) continue;
+            double out[MOC_MAXS + 1], in[MOC_MAXS + 1];
+            for (int i = 0; i <= MOC_MAXS; ++i) { out[i] = 0.; in[i] = 0.; }
+            const int top = imin(NS, z + 1);
+            for (int ion = 1; ion <= imin(z, NS - 1); ++ion) {
+                out[ion] += s.cI1[e][ion] * s.Ne;
+                const int spec = a.specOf[e * 32 + ion];
+                if (spec < 0) continue;
+                for (int sh = a.specShell[spec]; sh < a.specShell[spec + 1]; ++sh) {
+                    const int shn = a.shellNum[sh];
+                    double g = r.gamma(sh);
+                    if (shn == a.outShell[z * 32 + (z - ion + 1)] && ion == 1 && (z == 7 || z == 8))
+                        g += revFrozen[z - 7];
+                    out[ion] += g;
+                    const int na = a.nauger[(z * 32 + ion) * 11 + shn];
+                    for (int ne = 2; ne <= na; ++ne) {
+                        const int mx = imin(ion + ne - 1, imin(NS - 1, z));
+                        const double ratio = s.X[e][mx] > 1e-30 ? s.X[e][ion] / s.X[e][mx] : 1.;
+                        out[mx] += g * a.auger[((z * 32 + ion) * 11 + shn) * 11 + ne] * ratio;
+                    }
+                }
+            }
+            for (int ion = 1; ion <= imin(z, NS - 1); ++ion)
+                in[ion] = s.Ne * s.alpha[e][ion] + chex(a, z, ion, s.Te, false) * s.H * s.X[eH][1];
+            double mat[MOC_MAXS + 2];
+            mat[1] = 1.;
+            double sum = 1.;
+            for (int ion = 2; ion <= top; ++ion) {
+                mat[ion] = mat[ion - 1] * out[ion - 1] / in[ion - 1];
+                sum += mat[ion];
+            }
+            sum = 1. / sum;
+            s.X[e][top] = mat[top] * sum;
+            for (int ion = 1; ion <= imin(z, NS - 1); ++ion) {
+                double v;
+                if (s.nIterX < 2 && mat[ion] * sum > 0.) {
+                    const double old = s.X[e][ion] <= 0. ? 1.e-5 : s.X[e][ion];
+                    v = pow(10., 0.5 * (log10(old) + log10(mat[ion] * sum)));
+                } else {
+                    v = mat[ion] * sum;
+                }
+                v = v > kXMax ? kXMax : v;
+                v = v < 1.e-20 ? 1.e-20 : v;
+                s.X[e][ion] = v;
+            }
+        }
+        double Ne = 0.;
+        for (int e = 0; e < a.NE; ++e) {
+            const int z = a.elemZ[e];
+            for (int ion = 2; ion <= imin(z + 1, NS); ++ion)
+                if (s.X[e][ion] >= 1.e-10) Ne += (ion - 1) * a.abund[e] * s.X[e][ion];
+        }
+        Ne *= s.H;
+        s.Ne = Ne == 0. ? 1. : Ne;
+        const double dHI = (s.X[eH][1] - HIOld) / HIOld;
+        const double dHeI = eHe >= 0 ? (s.X[eHe][1] - HeIOld) / HeIOld : 0.;
+        const double dHeII = eHe >= 0 ? (s.X[eHe][2] - HeIIOld) / HeIIOld : 0.;
+        const bool big = fabs(dHI) > 0.01 || fabs(dHeI) > 0.01 || fabs(dHeII) > 0.01;
+        if (big && s.nIterX < 15) { ++s.nIterX; continue; }
+        if (big && s.nIterX == 15) conv = false;
+        return conv;
+    }
+}
+
+// forbidden-line cooling of one n-level ion [erg/s per n_H]
+__device__ double nlevel_cool(const MocBalanceArgs& a, int atom, double Te, double Ne, double scale) {
+    const int64_t* m = a.atomMeta + 8 * atom;
+    const int nf = (int)m[0], nT = (int)m[1];
+    const double* lt = a.atomLogT + m[2];
+    const double* g = a.atomG + m[3];
+    const double* en = a.atomE + m[3];
+    const double* A = a.atomA + m[4];
+    const double* q = a.atomQom + m[5];
+    const double* q2 = a.atomQ2 + m[5];
+    const int iup2p = (int)m[6];
+    const double atp = a.atomAtp[atom];
+    const int n = nf < a.maxLev ? nf : a.maxLev;
+    double x[MOC_MAXLEV][MOC_MAXLEV], y[MOC_MAXLEV], qe[MOC_MAXLEV][MOC_MAXLEV];
+    const double sqrTe = sqrt(Te), lT = log10(Te);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) { x[i][j] = 0.; qe[i][j] = 0.; }
+    // temperature bracket shared by every transition of this ion
+    int klo = 0, khi = nT - 1;
+    double sa = 0., sb = 0., h = 1.;
+    const bool below = lT <= lt[0], above = lT >= lt[nT - 1];
+    if (nT > 2 && !below && !above) {
+        while (khi - klo > 1) {
+            const int k = (khi + klo) >> 1;
+            if (lt[k] > lT) khi = k; else klo = k;
+        }
+        h = lt[khi] - lt[klo];
+        sa = (lt[khi] - lT) / h;
+        sb = (lT - lt[klo]) / h;
+    }
+    for (int i = 0; i < n - 1; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            // Upsilon of the pair (i, j) at T (spline in log T, clamped at the ends)
+            const size_t p = (size_t)i * nf + j;
+            double ups;
+            if (nT == 1) ups = q[p];
+            else if (nT == 2) ups = q[p] + (q[(size_t)nf * nf + p] - q[p]) / (lt[1] - lt[0]) * (lT - lt[0]);
+            else if (below) ups = q[p];
+            else if (above) ups = q[(size_t)(nT - 1) * nf * nf + p];
+            else {
+                const double ylo = q[(size_t)klo * nf * nf + p], yhi = q[(size_t)khi * nf * nf + p];
+                const double dlo = q2[(size_t)klo * nf * nf + p], dhi = q2[(size_t)khi * nf * nf + p];
+                ups = sa * ylo + sb * yhi + ((sa * sa * sa - sa) * dlo + (sb * sb * sb - sb) * dhi) * (h * h) / 6.;
+            }
+            const double c = 8.63e-6 * ups / sqrTe;
+            if (en[i] <= en[j]) {
+                qe[i][j] = c * exp((en[i] - en[j]) * 1.4388463 / Te) / g[i];
+                qe[j][i] = c / g[j];
+            } else {
+                qe[j][i] = c * exp((en[j] - en[i]) * 1.4388463 / Te) / g[i];
+                qe[i][j] = c / g[j];
+            }
+        }
+    for (int j = 0; j < n; ++j) x[0][j] = 1.;
+    y[0] = 1.;
+    for (int i = 1; i < n; ++i) {
+        for (int j = 0; j < n; ++j) {
+            if (j == i) continue;
+            x[i][j] += Ne * qe[j][i];
+            x[i][i] -= Ne * qe[i][j];
+            if (j > i) x[i][j] += A[(size_t)j * nf + i];
+            else x[i][i] -= A[(size_t)i * nf + j];
+        }
+        y[i] = 0.;
+    }
+    // elimination without pivoting, as luSlv
+    for (int i = 0; i < n - 1; ++i)
+        for (int k = i + 1; k < n; ++k) {
+            const double f = x[k][i] / x[i][i];
+            for (int j = i + 1; j < n; ++j) x[k][j] -= x[i][j] * f;
+            y[k] -= y[i] * f;
+        }
+    y[n - 1] /= x[n - 1][n - 1];
+    for (int k = n - 2; k >= 0; --k) {
+        for (int j = k + 1; j < n; ++j) y[k] -= y[j] * x[k][j];
+        y[k] /= x[k][k];
+    }
+    double sumN = 0.;
+    for (int i = 0; i < n; ++i) sumN += y[i];
+    double cool = 0.;
+    for (int i = 0; i < n - 1; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            double av = A[(size_t)j * nf + i];
+            if (av == 0.) continue;
+            if (j + 1 == iup2p && i == 0 && atp > 0.) av -= atp;
+            const double em = av * (en[j] - en[i]) * (y[j] / sumN) * scale * 1.9865e-16;
+            if (em > 1.e-35) cool += em;
+        }
+    return cool;
+}
+
+__device__ void therm(const MocBalanceArgs& a, const Rates& r, const CellState& s, double comp[4],
+                      double* heat, double* cool) {
+    const double T = s.Te, Ne = s.Ne;
+    double coolFF = 0., coolRec = 0., coolColl, coolCollH = 0., heatSte = 0., Te4 = T / 1.e4;
+    const double sq = kBoltz * T / sqrt(T);
+    for (int e = 0; e < a.NE; ++e) {  // H-like recombination, elements in ascending Z
+        const int z = a.elemZ[e];
+        if (!(a.NS > z)) continue;
+        const double Np = s.X[e][z + 1] * a.abund[e];
+        const double lt = log10(T / (double)(z * z));
+        Te4 = T / (z * z * 1.e4);
+        const double bFF = z * (1.0108464E-11 + 9.7930778E-13 * lt - 6.6433144E-13 * lt * lt +
+                                2.4793747E-13 * lt * lt * lt - 2.3938215E-14 * lt * lt * lt * lt);
+        const double bRec = z * (9.4255985E-11 - 4.04794384E-12 * lt - 1.0055237E-11 * lt * lt +
+                                 1.99266862E-12 * lt * lt * lt - 1.06681387E-13 * lt * lt * lt * lt);
+        coolFF += Np * Ne * bFF * sq;
+        coolRec += Np * Ne * bRec * sq;
+    }
+    Te4 = T / 1.e4;
+    for (int e = 0; e < a.NE; ++e) {  // He-like
+        const int z = a.elemZ[e];
+        if (z < 2 || !(a.NS > z)) continue;
+        const double Np = s.X[e][z] * a.abund[e];
+        const double lt = log10(4. * T / (double)(z * z));
+        Te4 = 4. * T / (z * z * 1.e4);
+        const double bFF = (z / 2.) * (1.070073e-11 - 2.5730207e-13 * lt + 2.109134e-13 * lt * lt);
+        const double bRec = (z / 2.) * (1.06926e-10 - 2.41756e-11 * lt + 1.14542e-12 * lt * lt -
+                                        5.06535e-13 * lt * lt * lt + 9.16745e-14 * lt * lt * lt * lt);
+        coolFF += Np * Ne * bFF * sq;
+        coolRec += Np * Ne * bRec * sq;
+    }
+    const int eH = a.elemSlot[1];
+    // H Ly-alpha/beta excitation, with Te4 as the loops above left it
+    coolColl = T > 5000. ? (2.47e-8 * exp(-118338. / T) * pow(Te4, -0.228) +
+                            1.32e-8 * exp(-140252. / T) * pow(Te4, -0.460)) * kHcRyd * s.X[eH][1] * Ne
+                         : 0.;
+    for (int e = 0; e < a.NE; ++e) {
+        const int z = a.elemZ[e];
+        for (int ion = 1; ion <= imin(z, a.NS - 1); ++ion) coolCollH += a.abund[e] * s.X[e][ion] * s.cI2[e][ion] * Ne;
+    }
+    for (int e = 0; e < a.NE; ++e) {
+        const int z = a.elemZ[e];
+        if (z < 3) continue;
+        for (int ion = 1; ion <= imin(z + 1, a.NS); ++ion) {
+            const int at = a.atomIdx[z * 32 + ion];
+            if (at >= 0) coolColl += nlevel_cool(a, at, T, Ne, a.abund[e] * s.X[e][ion]);
+        }
+    }
+    for (int sp = 0; sp < a.nSpec; ++sp) {
+        const int e = a.specElem[sp], ion = a.specIon[sp];
+        const int z = a.elemZ[e];
+        if (ion > imin(z, a.NS - 1)) continue;
+        for (int sh = a.specShell[sp]; sh < a.specShell[sp + 1]; ++sh)
+            heatSte += r.heat(sh, z < 3 ? 1.0 : s.heatef) * s.X[e][ion] * a.abund[e];
+    }
+    *heat = heatSte;
+    *cool = coolFF + coolRec + coolColl + coolCollH;
+    if (a.lgCompton) {
+        const double cHeat = (6.65e-25 * comp[0] * kRyd2erg / 8.184e-7) / s.H;
+        const double cCool = (6.65e-25 * 6.75e-10 * T * comp[1]) / s.H;
+        *heat += cHeat * Ne + comp[2];
+        *cool += cCool * Ne;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_balance(MocBalanceArgs a) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (cell > a.nCells) return;
+    const int C1 = a.nCells + 1;
+    a.lgBlack[cell] = 0;
+    Rates r{a.R, C1, cell, a.nShells};
+    if (!(r.at(a.colHit) > 0.)) {  // no packet reached the cell (lgHit, :127-171)
+        a.lgBlack[cell] = 1;
+        atomicAdd(&a.counters[0], 1ull);
+        a.nIterT[cell] = 0;
+        return;
+    }
+    CellState s;
+    s.Te = a.Te[cell];
+    s.Ne = a.Ne[cell];
+    s.H = a.Hden[cell];
+    for (int e = 0; e < a.NE; ++e) {
+        s.X[e][0] = 0.;
+        for (int ion = 1; ion <= a.NS; ++ion) s.X[e][ion] = a.ionDen[((size_t)(ion - 1) * a.NE + e) * C1 + cell];
+    }
+    const int eH = a.elemSlot[1], eHe = a.elemSlot[2];
+    const double XOld = s.X[eH][1];
+    double Thigh = 0., Tlow = 0., rHigh = 0., rLow = 0.;
+    int nT = 0;
+    bool ib = true;
+    for (;;) {
+        ++nT;
+        recomb(a, s);
+        // secondary-ionisation heating efficiency (photoionisation :3605-3627)
+        double xe = 0.;
+        for (int e = 0; e < a.NE; ++e) xe += s.X[e][1] * a.abund[e] * s.H;
+        xe = s.Ne / xe;
+        if (xe > 0.95) s.heatef = 1.;
+        else {
+            xe = xe < 1.e-4 ? 1.e-4 : xe;
+            s.heatef = 0.9971 * (1. - pow(1. - pow(xe, 0.2663), 1.3163));
+        }
+        collion(a, s);
+        s.nIterX = 1;
+        double comp[4] = {0., 0., 0., 0.};
+        if (a.lgCompton) {
+            comp[0] = r.at(a.colComp + 0);
+            comp[1] = r.at(a.colComp + 1);
+            comp[2] = (r.at(a.colComp + 2) * s.X[eH][1] * a.abund[eH] +
+                       (eHe >= 0 ? r.at(a.colComp + 3) * s.X[eHe][1] * a.abund[eHe] * 2. : 0.)) * kRyd2erg;
+        }
+        double revFrozen[2] = {0., 0.};
+        if (a.elemSlot[7] >= 0) revFrozen[0] = rev_chex(a, s, 7, 1);
+        if (a.elemSlot[8] >= 0) revFrozen[1] = rev_chex(a, s, 8, 1);
+        ib = ion_balance(a, r, s, revFrozen);
+        double heat, cool;
+        therm(a, r, s, comp, &heat, &cool);
+        const double res = heat - cool;
+        if (fabs(res) >= 0.02 * (fabs(cool) + fabs(heat)) / 2.) {
+            if (nT < 20) {
+                if (res < 0.) {
+                    rLow = res;
+                    Tlow = s.Te;
+                    s.Te = Thigh != 0. ? Tlow - (Thigh - Tlow) * rLow / (rHigh - rLow) : s.Te / 1.2;
+                } else {
+                    rHigh = res;
+                    Thigh = s.Te;
+                    s.Te = Tlow != 0. ? Tlow - (Thigh - Tlow) * rLow / (rHigh - rLow) : s.Te * 1.2;
+                }
+                if (nT >= 6 && fabs(Thigh - Tlow) <= 0.002 * (Thigh + Tlow)) {
+                    if (res < 0.) { s.Te = Thigh - 100.; Thigh = 0.; }
+                    else { s.Te = Tlow + 100.; Tlow = 0.; }
+                }
+                if (s.Te <= 0.) s.Te = 1.;
+                continue;
+            }
+            s.Te = Thigh == 0. ? Tlow : (Tlow == 0. ? Thigh : (Thigh + Tlow) * 0.5);
+            atomicAdd(&a.counters[2], 1ull);
+        }
+        break;
+    }
+    if (!ib) atomicAdd(&a.counters[1], 1ull);
+    a.Te[cell] = s.Te;
+    a.Ne[cell] = s.Ne;
+    for (int e = 0; e < a.NE; ++e)
+        for (int ion = 1; ion <= a.NS; ++ion) a.ionDen[((size_t)(ion - 1) * a.NE + e) * C1 + cell] = s.X[e][ion];
+    a.lgConverged[cell] = fabs((s.X[eH][1] - XOld) / XOld) <= a.XHILimit ? 1 : 0;
+    a.nIterT[cell] = nT;
+}
+
+}  // namespace
+
+int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a, const double* J, const double* W,
+                       int ncol, double* R) {
+    const int C1 = a.nCells + 1;
+    dim3 g1((unsigned)((C1 + RC - 1) / RC), (unsigned)((ncol + RN - 1) / RN));
+    hipLaunchKernelGGL(k_rates, g1, dim3(256), 0, ctx->stream, J, C1, ctx->nbins, W, ncol, R);
+    hipLaunchKernelGGL(k_balance, dim3((unsigned)((a.nCells + 63) / 64)), dim3(64), 0, ctx->stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        ctx->err = std::string("balance launch: ") + hipGetErrorString(e);
+        return MOC_ERR_HIP;
+    }
+    return MOC_OK;
+}

@@ -87,6 +87,59 @@ struct MocOpacityArgs {
     double* opacity;       // [B][C1]
 };
 
+// Balance kernel view (balance.hip)
+#define MOC_MAXE 30     // elements
+#define MOC_MAXS 31     // ionisation stages
+#define MOC_MAXLEV 27   // levels of an n-level ion
+struct MocBalanceArgs {
+    int nCells, NE, NS, maxLev, lgCompton, nShells, nSpec;
+    double XHILimit;
+    const int* elemZ;      // [NE]
+    const double* abund;   // [NE]
+    const int* elemSlot;   // [31] Z -> slot or -1
+    const int* specElem;   // [S] slot
+    const int* specIon;    // [S]
+    const int* specOf;     // [NE*32] (slot, stage) -> species or -1
+    const int* specShell;  // [S+1] shell range of each species
+    const int* shellNum;   // [nShells] shell number (1 = K)
+    const int* outShell;   // [31*32] (Z, nElec) -> outer shell
+    const int* statW;      // [31*32] g0 (Z, nElec)
+    const int* statW1;     // [31*32] g1
+    const double* rrb;     // [6][31][31]
+    const double* drb_c;   // [9][31][31]
+    const double* drb_e;   // [9][31][31]
+    const double* rrec;    // [2][31][31]
+    const double* rnew;    // [4][31][31]
+    int nDiel, nAldro;
+    const double* diel;    // [nDiel][8]
+    const double* aldro;   // [nAldro][6]
+    const double* cion_cf; // [5][31][31]
+    const double* cion_th; // [31][32]
+    const int* nauger;     // [31][32][11]
+    const double* auger;   // [31][32][11][11]
+    const int* chexIdx;    // [31*32] -> row of chexTab or -1
+    const double* chexTab; // [n][4]
+    const int* atomIdx;    // [31*32]
+    const int64_t* atomMeta;  // [nAtoms][8]
+    const double* atomAtp;
+    const double* atomLogT;
+    const double* atomG;
+    const double* atomE;
+    const double* atomA;
+    const double* atomQom;    // [nT][nLev][nLev] per atom
+    const double* atomQ2;     // spline second derivatives, same layout
+    int colHit, colComp;      // rate columns of the hit test and the Compton sums
+    const double* R;          // [ncol][C1] rates
+    double* Te;
+    double* Ne;
+    const double* Hden;
+    double* ionDen;
+    int* lgConverged;
+    int* lgBlack;
+    int* nIterT;
+    unsigned long long* counters;  // noHit, noIonBal, noTeBal
+};
+
 struct MocGrid {
     bool attached = false;
     int nx = 0, ny = 0, nz = 0, nCells = 0, sym = 0;
@@ -114,6 +167,20 @@ struct MocGrid {
     float* f_totalLines = nullptr;
     bool f32_dirty = true;
     double* d_ionDen = nullptr;      // [NS][NE][C+1]
+    int* d_conv = nullptr;           // [C+1] lgConverged
+    int* d_black = nullptr;          // [C+1] lgBlack
+    int* d_nIterT = nullptr;         // [C+1]
+    double* d_rates = nullptr;       // [ncol][C+1]
+    int rates_cols = 0;
+};
+
+struct MocBalance {
+    bool set = false;
+    MocBalanceArgs a{};           // device pointers (grid-independent part)
+    int ncol = 0;
+    double* d_W = nullptr;        // [B][ncol]
+    std::vector<void*> owned;     // device allocations of this table set
+    unsigned long long* d_counters = nullptr;
 };
 
 struct MocAtomic {
@@ -141,6 +208,7 @@ struct moc_ctx {
     unsigned long long* d_counters = nullptr;
     MocGrid grid;
     MocAtomic atomic;
+    MocBalance balance;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -151,3 +219,5 @@ int moc_launch_transport_f32(moc_ctx* ctx, const MocTransportArgs<float>& a, flo
 void moc_convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);
 void moc_scale_f64(double* p, int64_t n, double f, hipStream_t s);
 int moc_launch_opacity(moc_ctx* ctx, const MocOpacityArgs& a);
+int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a, const double* J, const double* W,
+                       int ncol, double* R);

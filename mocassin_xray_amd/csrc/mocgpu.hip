@@ -59,7 +59,53 @@ void free_grid(MocGrid& g) {
     dfree(g.f_vx); dfree(g.f_vy); dfree(g.f_vz);
     dfree(g.f_opacity); dfree(g.f_Ne); dfree(g.f_recPDF); dfree(g.f_totalLines);
     dfree(g.d_ionDen);
+    dfree(g.d_conv); dfree(g.d_black); dfree(g.d_nIterT); dfree(g.d_rates);
     g = MocGrid();
+}
+
+void free_balance(MocBalance& b) {
+    for (void* p : b.owned) (void)hipFree(p);
+    b.owned.clear();
+    dfree(b.d_W);
+    dfree(b.d_counters);
+    b = MocBalance();
+}
+
+// copy a host table into a new device buffer owned by the balance set
+template <typename T>
+int bal_put(moc_ctx* ctx, const T* h, size_t n, const T** out, const char* what) {
+    T* d = nullptr;
+    int rc = dalloc_n(ctx, &d, n ? n : 1, what);
+    if (rc) return rc;
+    ctx->balance.owned.push_back((void*)d);
+    if (n) {
+        if (!h) { ctx->err = std::string("moc_set_balance: missing table ") + what; return MOC_ERR_ARG; }
+        rc = h2d(ctx, d, h, n * sizeof(T), what);
+        if (rc) return rc;
+    }
+    *out = d;
+    return MOC_OK;
+}
+
+// per-grid balance buffers (flags, iteration counts, rate matrix)
+int ensure_balance_grid(moc_ctx* ctx) {
+    MocGrid& g = ctx->grid;
+    if (!g.attached || !ctx->balance.set) return MOC_OK;
+    const size_t C1 = (size_t)g.nCells + 1;
+    int rc;
+    if (!g.d_conv && ((rc = dalloc_n(ctx, &g.d_conv, C1, "lgConverged")) ||
+                      (rc = dalloc_n(ctx, &g.d_black, C1, "lgBlack")) ||
+                      (rc = dalloc_n(ctx, &g.d_nIterT, C1, "nIterT"))))
+        return rc;
+    if (g.rates_cols != ctx->balance.ncol) {
+        dfree(g.d_rates);
+        if ((rc = dalloc_n(ctx, &g.d_rates, C1 * ctx->balance.ncol, "rates"))) return rc;
+        g.rates_cols = ctx->balance.ncol;
+    }
+    MOC_TRY(hipMemsetAsync(g.d_conv, 0, C1 * 4, ctx->stream), "zero flags");
+    MOC_TRY(hipMemsetAsync(g.d_black, 0, C1 * 4, ctx->stream), "zero flags");
+    MOC_TRY(hipStreamSynchronize(ctx->stream), "zero flags");
+    return MOC_OK;
 }
 
 void free_atomic(MocAtomic& a) {
@@ -224,6 +270,7 @@ void moc_destroy(moc_ctx* ctx) {
     if (!ctx) return;
     free_grid(ctx->grid);
     free_atomic(ctx->atomic);
+    free_balance(ctx->balance);
     dfree(ctx->d_nu); dfree(ctx->d_KNsigmaT); dfree(ctx->d_KNcdf); dfree(ctx->d_Pcomp);
     dfree(ctx->d_sourceCDF);
     dfree(ctx->f_nu); dfree(ctx->f_KNsigmaT); dfree(ctx->f_KNcdf); dfree(ctx->f_Pcomp);
@@ -317,7 +364,9 @@ int moc_grid_attach(moc_ctx* ctx, int ig, const moc_grid_desc* gd) {
     MOC_TRY(hipStreamSynchronize(ctx->stream), "grid attach");
     g.f32_dirty = true;
     g.attached = true;
-    return ensure_ionden(ctx);
+    int rc2 = ensure_ionden(ctx);
+    if (rc2) return rc2;
+    return ensure_balance_grid(ctx);
 }
 
 int moc_set_atomic(moc_ctx* ctx, const moc_atomic* at) {
@@ -429,6 +478,21 @@ int moc_download(moc_ctx* ctx, int field, int ig, void* host, int64_t count, int
     if (!ctx) return MOC_ERR_ARG;
     MocGrid& g = ctx->grid;
     if (!g.attached || ig != 1) { ctx->err = "moc_download: no grid attached"; return MOC_ERR_STATE; }
+    if (field == MOC_F_CONVERGED || field == MOC_F_BLACK) {
+        const int* d = field == MOC_F_CONVERGED ? g.d_conv : g.d_black;
+        const int64_t C1 = g.nCells + 1;
+        if (!d) { ctx->err = "moc_download: no balance run yet"; return MOC_ERR_STATE; }
+        if (count != C1 || !host) { ctx->err = "moc_download: element count does not match the field"; return MOC_ERR_ARG; }
+        std::vector<int> buf((size_t)C1);
+        MOC_TRY(hipMemcpyAsync(buf.data(), d, (size_t)C1 * 4, hipMemcpyDeviceToHost, ctx->stream), "download");
+        MOC_TRY(hipStreamSynchronize(ctx->stream), "download");
+        for (int64_t i = 0; i < C1; ++i) {
+            if (dtype == MOC_DT_I32) ((int*)host)[i] = buf[i];
+            else if (dtype == MOC_DT_F64) ((double*)host)[i] = buf[i];
+            else ((float*)host)[i] = (float)buf[i];
+        }
+        return MOC_OK;
+    }
     double* d = field_ptr(ctx, field);
     if (!d || !host) { ctx->err = "moc_download: unknown field"; return MOC_ERR_ARG; }
     const int64_t C1 = g.nCells + 1, B = ctx->nbins;
@@ -546,6 +610,144 @@ int moc_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ct
                    " packets reached an insane state (index outside the grid or emission "
                    "from an inactive cell)";
         return MOC_ERR_KERNEL;
+    }
+    return MOC_OK;
+}
+
+int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* t) {
+    if (!ctx) return MOC_ERR_ARG;
+    if (!ctx->atomic.set) { ctx->err = "moc_set_balance: call moc_set_atomic first"; return MOC_ERR_STATE; }
+    MocAtomic& at = ctx->atomic;
+    if (!t || t->maxLev < 1 || t->maxLev > MOC_MAXLEV || t->nShells < 0 || !t->speciesShell ||
+        at.nElemUsed > MOC_MAXE || at.nstages > MOC_MAXS || t->nAtoms < 0) {
+        ctx->err = "moc_set_balance: invalid tables (maxLev <= 27, <= 30 elements, <= 31 stages)";
+        return MOC_ERR_ARG;
+    }
+    bool hasH = false;
+    for (int e = 0; e < at.nElemUsed; ++e) hasH |= at.elemZ[e] == 1;
+    if (!hasH) { ctx->err = "moc_set_balance: hydrogen must be switched on"; return MOC_ERR_ARG; }
+    for (int sp = 0; sp < at.nSpecies; ++sp)
+        if (t->speciesShell[sp] > t->speciesShell[sp + 1] || t->speciesShell[sp + 1] > t->nShells) {
+            ctx->err = "moc_set_balance: speciesShell ranges";
+            return MOC_ERR_ARG;
+        }
+    for (int k = 0; k < t->nAtoms; ++k)
+        if (t->atomMeta[8 * k] < 1 || t->atomMeta[8 * k] > 64 || t->atomMeta[8 * k + 1] < 1 ||
+            t->atomMeta[8 * k + 1] > 64) {
+            ctx->err = "moc_set_balance: n-level ion sizes";
+            return MOC_ERR_ARG;
+        }
+    free_balance(ctx->balance);
+    MocBalance& b = ctx->balance;
+    MocBalanceArgs& a = b.a;
+    const int B = ctx->nbins, S = at.nSpecies;
+    b.ncol = 3 * t->nShells + 5;
+    // host-side index tables
+    std::vector<int> slot(31, -1), specOf((size_t)at.nElemUsed * 32, -1), chexIdx(31 * 32, -1);
+    for (int e = 0; e < at.nElemUsed; ++e) slot[at.elemZ[e]] = e;
+    for (int sp = 0; sp < S; ++sp) specOf[(size_t)at.specElem[sp] * 32 + at.specIon[sp]] = sp;
+    for (int k = 0; k < t->nChex; ++k) chexIdx[t->chexZ[k] * 32 + t->chexIon[k]] = k;  // last row wins
+    int rc;
+    const int* p_i;
+    const double* p_d;
+    const int64_t* p_l;
+#define PUT(dst, src, n, what) do { if ((rc = bal_put(ctx, src, (size_t)(n), &dst, what))) return rc; } while (0)
+    PUT(a.elemSlot, slot.data(), 31, "elemSlot");
+    PUT(a.specOf, specOf.data(), specOf.size(), "specOf");
+    PUT(a.specShell, t->speciesShell, S + 1, "specShell");
+    PUT(a.shellNum, t->shellNum, t->nShells, "shellNum");
+    PUT(a.outShell, t->outShell, 31 * 32, "outShell");
+    PUT(a.statW, t->statW0, 31 * 32, "statW0");
+    PUT(a.statW1, t->statW1, 31 * 32, "statW1");
+    PUT(a.rrb, t->rrb, 6 * 31 * 31, "rrb");
+    PUT(a.drb_c, t->drb_c, 9 * 31 * 31, "drb_c");
+    PUT(a.drb_e, t->drb_e, 9 * 31 * 31, "drb_e");
+    PUT(a.rrec, t->rrec, 2 * 31 * 31, "rrec");
+    PUT(a.rnew, t->rnew, 4 * 31 * 31, "rnew");
+    PUT(a.diel, t->diel, 8 * (size_t)t->nDiel, "diel");
+    PUT(a.aldro, t->aldro, 6 * (size_t)t->nAldro, "aldro");
+    PUT(a.cion_cf, t->cionCF, 5 * 31 * 31, "cionCF");
+    PUT(a.cion_th, t->cionTherm, 31 * 32, "cionTherm");
+    PUT(a.nauger, t->nAuger, 31 * 32 * 11, "nAuger");
+    PUT(a.auger, t->auger, 31 * 32 * 11 * 11, "auger");
+    PUT(a.chexIdx, chexIdx.data(), 31 * 32, "chexIdx");
+    PUT(a.chexTab, t->chexFit, 4 * (size_t)t->nChex, "chexFit");
+    PUT(a.atomIdx, t->atomIdx, 31 * 32, "atomIdx");
+    PUT(a.atomMeta, t->atomMeta, 8 * (size_t)t->nAtoms, "atomMeta");
+    PUT(a.atomAtp, t->atomAtp, t->nAtoms, "atomAtp");
+    PUT(a.atomLogT, t->atomLogT, t->nLogT, "atomLogT");
+    PUT(a.atomG, t->atomG, t->nLev, "atomG");
+    PUT(a.atomE, t->atomE, t->nLev, "atomE");
+    PUT(a.atomA, t->atomA, t->nA, "atomA");
+    PUT(a.atomQom, t->atomQom, t->nQ, "atomQom");
+    PUT(a.atomQ2, t->atomQ2, t->nQ, "atomQ2");
+    PUT(p_d, t->W, (size_t)B * b.ncol, "W");
+    b.d_W = nullptr;  // W lives in b.owned
+#undef PUT
+    (void)p_i; (void)p_l;
+    if ((rc = dalloc_n(ctx, &b.d_counters, 4, "balance counters"))) return rc;
+    a.elemZ = at.d_elemZ;
+    a.abund = at.d_abund;
+    a.specElem = at.d_specElem;
+    a.specIon = at.d_specIon;
+    a.NE = at.nElemUsed;
+    a.NS = at.nstages;
+    a.nSpec = S;
+    a.nShells = t->nShells;
+    a.nDiel = t->nDiel;
+    a.nAldro = t->nAldro;
+    a.maxLev = t->maxLev;
+    a.XHILimit = t->XHILimit;
+    a.lgCompton = ctx->lgCompton;
+    a.colComp = 3 * t->nShells;
+    a.colHit = 3 * t->nShells + 4;
+    a.R = p_d;  // placeholder: W pointer kept in R until the launch (set per grid)
+    b.set = true;
+    return ensure_balance_grid(ctx);
+}
+
+int moc_update(moc_ctx* ctx, int ig, moc_update_stats* stats) {
+    if (!ctx) return MOC_ERR_ARG;
+    MocGrid& g = ctx->grid;
+    if (!g.attached || ig != 1 || !ctx->balance.set || !g.d_ionDen) {
+        ctx->err = "moc_update: needs an attached grid, moc_set_atomic and moc_set_balance";
+        return MOC_ERR_STATE;
+    }
+    int rc = ensure_balance_grid(ctx);
+    if (rc) return rc;
+    MocBalance& b = ctx->balance;
+    MocBalanceArgs a = b.a;
+    const double* W = a.R;
+    a.nCells = g.nCells;
+    a.R = g.d_rates;
+    a.Te = g.d_Te;
+    a.Ne = g.d_Ne;
+    a.Hden = g.d_Hden;
+    a.ionDen = g.d_ionDen;
+    a.lgConverged = g.d_conv;
+    a.lgBlack = g.d_black;
+    a.nIterT = g.d_nIterT;
+    a.counters = b.d_counters;
+    MOC_TRY(hipMemsetAsync(b.d_counters, 0, 4 * sizeof(unsigned long long), ctx->stream), "counters");
+    MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
+    if ((rc = moc_launch_balance(ctx, a, g.d_J, W, b.ncol, g.d_rates))) return rc;
+    MOC_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
+    MOC_TRY(hipEventSynchronize(ctx->ev1), "balance kernel");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    unsigned long long cnt[4];
+    MOC_TRY(hipMemcpy(cnt, b.d_counters, sizeof(cnt), hipMemcpyDeviceToHost), "counters");
+    g.f32_dirty = true;
+    if (stats) {
+        std::vector<int> conv((size_t)g.nCells + 1);
+        MOC_TRY(hipMemcpy(conv.data(), g.d_conv, conv.size() * 4, hipMemcpyDeviceToHost), "conv");
+        double nc = 0;
+        for (int i = 1; i <= g.nCells; ++i) nc += conv[i];
+        stats->noHit = (double)cnt[0];
+        stats->noIonBal = (double)cnt[1];
+        stats->noTeBal = (double)cnt[2];
+        stats->converged = nc;
+        stats->kernel_ms = ms;
     }
     return MOC_OK;
 }

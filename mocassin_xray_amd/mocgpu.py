@@ -20,9 +20,9 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import (CELLBIN_FIELDS, FIELDS, MOC_DT_F64, MOC_LAYOUT_FORTRAN, MOC_PREC_FP64,
-                   MOC_PREC_MIXED, MocAtomic, MocGridDesc, MocTables, MocTransportStats, dptr,
-                   iptr)
+from ._lib import (CELLBIN_FIELDS, FIELDS, MOC_DT_F64, MOC_DT_I32, MOC_LAYOUT_FORTRAN,
+                   MOC_PREC_FP64, MOC_PREC_MIXED, MocAtomic, MocBalanceTables, MocGridDesc,
+                   MocTables, MocTransportStats, MocUpdateStats, dptr, iptr)
 
 
 class MocError(RuntimeError):
@@ -157,6 +157,55 @@ class MocGPU:
     def ionizationDriver(self):
         """Opacity of every cell from ionDen, Ne, Te, Hden (ionization_mod.f90:26)."""
         self._check(self._L.moc_opacity(self._ctx, 1), "moc_opacity")
+
+    def set_balance(self, bt) -> None:
+        """Balance tables (a balance.BalanceTables built on the same mesh)."""
+        d = bt.data
+        lp = C.POINTER(C.c_int64)
+        k = {
+            "sn": _c(bt.shellNum, np.int32), "ss": _c(bt.speciesShell, np.int32), "W": _c(bt.W),
+            "os": _c(bt.outShell, np.int32), "g0": _c(bt.statW0, np.int32),
+            "g1": _c(bt.statW1, np.int32), "rrb": _c(d["rrb"]), "dc": _c(d["drb_c"]),
+            "de": _c(d["drb_e"]), "rrec": _c(d["rrec"]), "rnew": _c(d["rnew"]),
+            "diel": _c(d["diel"]), "aldro": _c(d["aldro"]), "cf": _c(d["cion_cf"]),
+            "th": _c(d["cion_th"]), "na": _c(d["nauger"], np.int32), "au": _c(d["auger"]),
+            "cz": _c(bt.chex[:, 0], np.int32), "ci": _c(bt.chex[:, 1], np.int32),
+            "cfit": _c(bt.chex[:, 2:6]), "ai": _c(d["atom_idx"], np.int32),
+            "am": _c(bt.atomMeta8, np.int64), "atp": _c(d["atom_atp"]), "lt": _c(d["atom_logt"]),
+            "ag": _c(d["atom_g"]), "ae": _c(d["atom_e"]), "aA": _c(d["atom_A"]),
+            "aq": _c(bt.atomQom), "aq2": _c(bt.atomQ2),
+        }
+        self._balance_keep = k
+        t = MocBalanceTables(
+            maxLev=int(bt.maxLev), XHILimit=float(bt.XHILimit), nShells=k["sn"].size,
+            shellNum=iptr(k["sn"]), speciesShell=iptr(k["ss"]), W=dptr(k["W"]),
+            outShell=iptr(k["os"]), statW0=iptr(k["g0"]), statW1=iptr(k["g1"]),
+            rrb=dptr(k["rrb"]), drb_c=dptr(k["dc"]), drb_e=dptr(k["de"]), rrec=dptr(k["rrec"]),
+            rnew=dptr(k["rnew"]), nDiel=k["diel"].shape[0], diel=dptr(k["diel"]),
+            nAldro=k["aldro"].shape[0], aldro=dptr(k["aldro"]), cionCF=dptr(k["cf"]),
+            cionTherm=dptr(k["th"]), nAuger=iptr(k["na"]), auger=dptr(k["au"]),
+            nChex=k["cz"].size, chexZ=iptr(k["cz"]), chexIon=iptr(k["ci"]),
+            chexFit=dptr(k["cfit"]), nAtoms=k["am"].shape[0], atomIdx=iptr(k["ai"]),
+            atomMeta=k["am"].ctypes.data_as(lp), atomAtp=dptr(k["atp"]), nLogT=k["lt"].size,
+            nLev=k["ag"].size, nA=k["aA"].size, nQ=k["aq"].size, atomLogT=dptr(k["lt"]),
+            atomG=dptr(k["ag"]), atomE=dptr(k["ae"]), atomA=dptr(k["aA"]),
+            atomQom=dptr(k["aq"]), atomQ2=dptr(k["aq2"]))
+        self._check(self._L.moc_set_balance(self._ctx, C.byref(t)), "moc_set_balance")
+
+    def updateCell(self) -> dict:
+        """updateCell for every active cell (update_mod.f90:15) from the
+        normalised J on the device."""
+        st = MocUpdateStats()
+        self._check(self._L.moc_update(self._ctx, 1, C.byref(st)), "moc_update")
+        return st.as_dict()
+
+    def download_flags(self, name: str) -> np.ndarray:
+        """lgConverged / lgBlack as int32 (nCells+1,)."""
+        n = self.grid.nCells + 1
+        buf = np.zeros(n, dtype=np.int32)
+        self._check(self._L.moc_download(self._ctx, FIELDS[name], 1, buf.ctypes.data, n,
+                                         MOC_DT_I32, MOC_LAYOUT_FORTRAN), f"moc_download({name})")
+        return buf
 
     def upload(self, name: str, arr: np.ndarray):
         """Upload a field given in MOCASSIN shape ((C+1,) or (C+1, B))."""

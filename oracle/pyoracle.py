@@ -178,3 +178,116 @@ def opacity(xsec, kn_sigma, lgCompton, elem_z, abund, nstages, ionDen, Ne, Te, H
     out = np.zeros(B * C1)
     L.oracle_opacity(C.byref(inp), out.ctypes.data_as(_dp))
     return out.reshape(B, C1).T.copy()
+
+
+class BalanceTablesC(C.Structure):
+    _fields_ = [
+        ("nbins", C.c_int), ("nstages", C.c_int), ("nElemUsed", C.c_int), ("maxLev", C.c_int),
+        ("lgCompton", C.c_int), ("elemZ", _ip), ("abund", _dp), ("nuArray", _dp),
+        ("nShells", C.c_int), ("shZ", _ip), ("shIon", _ip), ("shNum", _ip), ("shLo", _ip),
+        ("shHi", _ip), ("shOff", C.POINTER(C.c_int64)), ("xs", _dp), ("secIonP", C.c_int),
+        ("cRecoilHP", C.c_int), ("cRecoilHeP", C.c_int), ("xSecRecoil", _dp), ("rrb", _dp),
+        ("drb_c", _dp), ("drb_e", _dp), ("rrec", _dp), ("rnew", _dp), ("nDiel", C.c_int),
+        ("diel", _dp), ("nAldro", C.c_int), ("aldro", _dp), ("cion_cf", _dp), ("cion_th", _dp),
+        ("nauger", _ip), ("auger", _dp), ("atom_idx", _ip), ("atom_meta", C.POINTER(C.c_int64)),
+        ("atom_atp", _dp), ("atom_logt", _dp), ("atom_g", _dp), ("atom_e", _dp), ("atom_A", _dp),
+        ("atom_qom", C.POINTER(C.c_float)),
+    ]
+
+
+class BalanceStateC(C.Structure):
+    _fields_ = [
+        ("nCells", C.c_int), ("J", _dp), ("Te", _dp), ("Ne", _dp), ("Hden", _dp), ("ionDen", _dp),
+        ("lgConverged", _ip), ("lgBlack", _ip), ("nIterT", _ip), ("XHILimit", C.c_double),
+        ("noHit", C.c_double), ("noIonBal", C.c_double), ("noTeBal", C.c_double),
+    ]
+
+
+def balance_tables(bt, elem_z, abund, nstages, lgCompton):
+    """ctypes view of the oracle's balance tables; returns (struct, keep-alive dict)."""
+    d = bt.data
+    sh = bt.xsec.shells
+    k = {
+        "ez": np.ascontiguousarray(elem_z, dtype=np.int32),
+        "ab": np.ascontiguousarray(abund, dtype=np.float64),
+        "nu": np.ascontiguousarray(bt.nu, dtype=np.float64),
+        "z": np.array([s.elem for s in sh], dtype=np.int32),
+        "ion": np.array([s.ion for s in sh], dtype=np.int32),
+        "num": np.array([s.shell for s in sh], dtype=np.int32),
+        "lo": np.array([s.lo for s in sh], dtype=np.int32),
+        "hi": np.array([s.hi for s in sh], dtype=np.int32),
+        "off": np.cumsum([0] + [s.sigma.size for s in sh[:-1]]).astype(np.int64),
+        "xs": np.ascontiguousarray(np.concatenate([s.sigma for s in sh])),
+        "rec": np.ascontiguousarray(bt.xSecRecoil, dtype=np.float64),
+        "am": np.ascontiguousarray(d["atom_meta"], dtype=np.int64),
+        "aq": np.ascontiguousarray(d["atom_qom"], dtype=np.float32),
+    }
+    for name in ("rrb", "drb_c", "drb_e", "rrec", "rnew", "diel", "aldro", "cion_cf", "cion_th",
+                 "auger", "atom_atp", "atom_logt", "atom_g", "atom_e", "atom_A"):
+        k[name] = np.ascontiguousarray(d[name], dtype=np.float64)
+    for name in ("nauger", "atom_idx"):
+        k[name] = np.ascontiguousarray(d[name], dtype=np.int32)
+    dp = lambda n: k[n].ctypes.data_as(_dp)
+    ip = lambda n: k[n].ctypes.data_as(_ip)
+    t = BalanceTablesC(
+        nbins=bt.nu.size, nstages=int(nstages), nElemUsed=k["ez"].size, maxLev=int(bt.maxLev),
+        lgCompton=int(bool(lgCompton)), elemZ=ip("ez"), abund=dp("ab"), nuArray=dp("nu"),
+        nShells=len(sh), shZ=ip("z"), shIon=ip("ion"), shNum=ip("num"), shLo=ip("lo"),
+        shHi=ip("hi"), shOff=k["off"].ctypes.data_as(C.POINTER(C.c_int64)), xs=dp("xs"),
+        secIonP=bt.secIonP, cRecoilHP=bt.cRecoilHP, cRecoilHeP=bt.cRecoilHeP,
+        xSecRecoil=dp("rec"), rrb=dp("rrb"), drb_c=dp("drb_c"), drb_e=dp("drb_e"),
+        rrec=dp("rrec"), rnew=dp("rnew"), nDiel=k["diel"].shape[0], diel=dp("diel"),
+        nAldro=k["aldro"].shape[0], aldro=dp("aldro"), cion_cf=dp("cion_cf"),
+        cion_th=dp("cion_th"), nauger=ip("nauger"), auger=dp("auger"), atom_idx=ip("atom_idx"),
+        atom_meta=k["am"].ctypes.data_as(C.POINTER(C.c_int64)), atom_atp=dp("atom_atp"),
+        atom_logt=dp("atom_logt"), atom_g=dp("atom_g"), atom_e=dp("atom_e"), atom_A=dp("atom_A"),
+        atom_qom=k["aq"].ctypes.data_as(C.POINTER(C.c_float)))
+    return t, k
+
+
+def alpha(bt, elem_z, nstages, Te):
+    """alphaTot(Z, stage) [31, 32] of the oracle's calcAlpha at Te."""
+    L = lib()
+    L.oracle_alpha.argtypes = [C.POINTER(BalanceTablesC), C.c_double, _dp]
+    t, k = balance_tables(bt, elem_z, np.ones(len(elem_z)), nstages, True)
+    out = np.zeros(31 * 32)
+    L.oracle_alpha(C.byref(t), float(Te), out.ctypes.data_as(_dp))
+    return out.reshape(31, 32)
+
+
+def nlevel_cooling(bt, atom, Te, Ne, scale=1.0):
+    L = lib()
+    L.oracle_nlevel_cooling.argtypes = [C.POINTER(BalanceTablesC), C.c_int, C.c_double,
+                                        C.c_double, C.c_double]
+    L.oracle_nlevel_cooling.restype = C.c_double
+    t, k = balance_tables(bt, [1], np.ones(1), 2, True)
+    return L.oracle_nlevel_cooling(C.byref(t), int(atom), float(Te), float(Ne), float(scale))
+
+
+def update(bt, elem_z, abund, nstages, lgCompton, J, Te, Ne, Hden, ionDen, nthreads=1):
+    """Oracle updateCell over all cells.  J is (C+1, B) normalised; ionDen is
+    (C+1, nElemUsed, nstages).  Returns a dict with the new Te, Ne, ionDen,
+    flags, iteration counts and counters."""
+    L = lib()
+    L.oracle_update.argtypes = [C.POINTER(BalanceTablesC), C.POINTER(BalanceStateC), C.c_int]
+    L.oracle_update.restype = C.c_int
+    t, k = balance_tables(bt, elem_z, abund, nstages, lgCompton)
+    C1 = J.shape[0]
+    s = {
+        "J": np.ascontiguousarray(np.asarray(J, dtype=np.float64).T),
+        "Te": np.array(Te, dtype=np.float64), "Ne": np.array(Ne, dtype=np.float64),
+        "H": np.ascontiguousarray(Hden, dtype=np.float64),
+        "X": np.ascontiguousarray(np.asarray(ionDen, dtype=np.float64).transpose(2, 1, 0)),
+        "conv": np.zeros(C1, dtype=np.int32), "black": np.zeros(C1, dtype=np.int32),
+        "nit": np.zeros(C1, dtype=np.int32),
+    }
+    st = BalanceStateC(nCells=C1 - 1, J=s["J"].ctypes.data_as(_dp), Te=s["Te"].ctypes.data_as(_dp),
+                       Ne=s["Ne"].ctypes.data_as(_dp), Hden=s["H"].ctypes.data_as(_dp),
+                       ionDen=s["X"].ctypes.data_as(_dp), lgConverged=s["conv"].ctypes.data_as(_ip),
+                       lgBlack=s["black"].ctypes.data_as(_ip), nIterT=s["nit"].ctypes.data_as(_ip),
+                       XHILimit=float(bt.XHILimit))
+    L.oracle_update(C.byref(t), C.byref(st), int(nthreads))
+    return {"Te": s["Te"], "Ne": s["Ne"],
+            "ionDen": s["X"].transpose(2, 1, 0).copy(), "lgConverged": s["conv"],
+            "lgBlack": s["black"], "nIterT": s["nit"], "noHit": st.noHit,
+            "noIonBal": st.noIonBal, "noTeBal": st.noTeBal}
