@@ -43,6 +43,6 @@ def test_create_without_gpu_fails_cleanly():
             pytest.skip("a GPU is present")
     except ImportError:
         pass
-    m = nebula.frozen_hsphere(n=4, nbins=20, n_packets=10)
+    m = nebula.frozen_hsphere(n=4, nbins=40, n_packets=10, mesh="log")
     with pytest.raises(MocError):
         MocGPU(m.tables, "fp64")
