@@ -183,6 +183,11 @@ typedef struct moc_balance_tables {
     const double* atomA;
     const double* atomQom;       /* Upsilon [nT][nLev][nLev] per ion */
     const double* atomQ2;        /* natural-spline second derivatives in log T */
+    /* recombination feed of the upper levels (equilibrium rec=.true.,
+     * emission_mod.f90:2828-2877), used by the diffuse emission */
+    const int32_t* atomRecKind;  /* [nAtoms] 0 none, 1 (br,z,a_r(4),a_d(5)) fit, 2 log-T cubic */
+    const double* atomRec;       /* [nLev][11] per level, same offsets as atomG */
+    const int32_t* atomRecValid; /* [nLev] 1 where the level has a rate */
 } moc_balance_tables;
 
 typedef struct moc_update_stats {
@@ -190,6 +195,48 @@ typedef struct moc_update_stats {
     double converged;                /* cells with lgConverged = 1 */
     double kernel_ms;
 } moc_update_stats;
+
+/* Tables of the diffuse emission (emissionDriver, emission_mod.f90:36-252):
+ * the H I / He I / He II free-bound fits with their edge pointers
+ * (initGammaCont, ph_mod.f90:31-202), the ground-state cross-sections above
+ * the H / He thresholds, the heavy-ion outer-shell free-bound entries, the
+ * He I two-photon shape, the hydrogenic and He I recombination-line tables
+ * and the He I 2^3S / He II Lyman line bins.  Built once per mesh and
+ * element set (the package's emission.build_emission). */
+typedef struct moc_emission_tables {
+    int32_t ntk;
+    const double* tkGamma;       /* [ntk] */
+    int32_t nlim[3];             /* points of the H I, He I, He II fits (even, <= 64) */
+    const int32_t* edgeP[3];     /* [nlim] 1-based bin pointers (HINuEdgeP, ...) */
+    const double* logGamma[3];   /* [ntk][nlim] log10 gamma, 0 where gamma <= 0 */
+    int32_t HIPnuP, HeIPnuP, HeIIPnuP; /* ground-state threshold bins (He: 0 when off) */
+    const double* xsHI;          /* [nbins-HIPnuP+1] cm^2 */
+    const double* xsHeI;         /* NULL when He is off */
+    const double* xsHeII;
+    int32_t nFb;                 /* heavy-ion (Z = 3..19) free-bound entries */
+    const int32_t* fbZ;          /* [nFb] Z ascending, then ion */
+    const int32_t* fbIon;
+    const int32_t* fbLo;         /* [nFb] outer-shell bin range [lo, hi), 1-based */
+    const int32_t* fbHi;
+    const int64_t* fbOff;        /* [nFb] offsets into fbXs */
+    int64_t nFbXs;
+    const double* fbXs;
+    const double* fbGratio;      /* [nFb] g0/g1 (getOuterShell) */
+    const int32_t* he2Row;       /* [nbins] He I two-photon table row per bin, 0 = none */
+    const double* hei2phot;      /* [41][2] y, A(y) */
+    int32_t nHlFiles;
+    const int32_t* hlMap;        /* [9][12] table file of (Z, T bin) */
+    const double* hlDens;        /* [nHlFiles][14] */
+    const double* hlLines;       /* [nHlFiles][14][16][9] relative to Hbeta */
+    const double* hbA;           /* [9][4] Hbeta fits */
+    const double* hbB;           /* [9][4] */
+    const double* rbEdge;        /* [2][3][26] */
+    const double* r2aEdge;       /* [3][18] */
+    const double* heiRec;        /* [34][3][4] He I line fits (first coefficient x1e25) */
+    const double* heiiLyman;     /* [4][2] He II Lyman ratio, nu */
+    int32_t j2TsP;               /* bin of He I 2^3S (1.45673 Ryd) */
+    int32_t lymP[4];             /* bins of the He II Lyman lines, 0 = below the mesh */
+} moc_emission_tables;
 
 /* Create a context: uploads the mesh and source tables. */
 int moc_create(moc_ctx** ctx, const moc_tables* tables);
@@ -218,6 +265,15 @@ int moc_opacity(moc_ctx* ctx, int ig);
 
 /* Upload the balance tables (after moc_set_atomic). */
 int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* tables);
+
+/* Upload the emission tables (after moc_set_balance, whose n-level atoms
+ * the forbidden lines use). */
+int moc_set_emission(moc_ctx* ctx, const moc_emission_tables* tables);
+
+/* emissionDriver for every active cell (emission_mod.f90:36; setDiffusePDF
+ * :1528): from Te, Ne, ionDen writes recPDF (the normalised diffuse CDF)
+ * and totalLines.  kernel_ms (may be NULL) gets the kernel time. */
+int moc_emission(moc_ctx* ctx, int ig, float* kernel_ms);
 
 /* updateCell for every active cell (update_mod.f90:15): from the normalised
  * J (moc_normalize_j) and Te, Ne, Hden, ionDen, writes Te, Ne, ionDen,

@@ -31,7 +31,7 @@ EXPORTS = [
     "moc_create", "moc_destroy", "moc_last_error", "moc_grid_attach", "moc_upload",
     "moc_download", "moc_reset_tallies", "moc_transport", "moc_normalize_j",
     "moc_device_ptr", "moc_version", "moc_set_atomic", "moc_opacity", "moc_set_balance",
-    "moc_update",
+    "moc_update", "moc_set_emission", "moc_emission",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -80,6 +80,21 @@ class MocBalanceTables(C.Structure):
         ("atomMeta", _lp), ("atomAtp", _dp), ("nLogT", C.c_int64), ("nLev", C.c_int64),
         ("nA", C.c_int64), ("nQ", C.c_int64), ("atomLogT", _dp), ("atomG", _dp),
         ("atomE", _dp), ("atomA", _dp), ("atomQom", _dp), ("atomQ2", _dp),
+        ("atomRecKind", _ip), ("atomRec", _dp), ("atomRecValid", _ip),
+    ]
+
+
+class MocEmissionTables(C.Structure):
+    _fields_ = [
+        ("ntk", C.c_int32), ("tkGamma", _dp), ("nlim", C.c_int32 * 3), ("edgeP", _ip * 3),
+        ("logGamma", _dp * 3), ("HIPnuP", C.c_int32), ("HeIPnuP", C.c_int32),
+        ("HeIIPnuP", C.c_int32), ("xsHI", _dp), ("xsHeI", _dp), ("xsHeII", _dp),
+        ("nFb", C.c_int32), ("fbZ", _ip), ("fbIon", _ip), ("fbLo", _ip), ("fbHi", _ip),
+        ("fbOff", _lp), ("nFbXs", C.c_int64), ("fbXs", _dp), ("fbGratio", _dp),
+        ("he2Row", _ip), ("hei2phot", _dp), ("nHlFiles", C.c_int32), ("hlMap", _ip),
+        ("hlDens", _dp), ("hlLines", _dp), ("hbA", _dp), ("hbB", _dp), ("rbEdge", _dp),
+        ("r2aEdge", _dp), ("heiRec", _dp), ("heiiLyman", _dp), ("j2TsP", C.c_int32),
+        ("lymP", C.c_int32 * 4),
     ]
 
 
@@ -142,6 +157,10 @@ def lib() -> C.CDLL:
     L.moc_set_balance.restype = C.c_int
     L.moc_update.argtypes = [vp, C.c_int, C.POINTER(MocUpdateStats)]
     L.moc_update.restype = C.c_int
+    L.moc_set_emission.argtypes = [vp, C.POINTER(MocEmissionTables)]
+    L.moc_set_emission.restype = C.c_int
+    L.moc_emission.argtypes = [vp, C.c_int, C.POINTER(C.c_float)]
+    L.moc_emission.restype = C.c_int
     L.moc_version.argtypes = []
     L.moc_version.restype = C.c_char_p
     _LIB = L

@@ -32,8 +32,14 @@ SHELL_ORDER = [6, 8, 7, 10, 11, 12, 13, 14, 15, 16, 17, 26, 18, 19, 20, 21, 22, 
 
 
 @lru_cache(maxsize=1)
+def packed_data() -> dict:
+    """Every table of data/atomic.npz (written by data/make_atomic.py)."""
+    with np.load(Path(__file__).resolve().parent / "data" / "atomic.npz") as d:
+        return {k: d[k] for k in d.files}
+
+
 def verner():
-    d = np.load(Path(__file__).resolve().parent / "data" / "atomic.npz")
+    d = packed_data()
     return d["ph1"], d["ph2"]
 
 
@@ -98,6 +104,27 @@ def outer_shell(z: int, nelec: int) -> int:
     if z == nelec + 1 and z in (20, 21, 22, 25, 26):
         out = 7
     return out
+
+
+def ion_edges(elem_on: list, nstages: int, nu_max: float) -> np.ndarray:
+    """Ionisation edges [Ryd] of the switched-on ions up to nuMax that the
+    gas frequency mesh resolves (initCartesianGrid, grid_mod.f90:170-209);
+    each is held in single precision there."""
+    ph1, _ = verner()
+    out = []
+    for z in sorted(set(elem_on)):
+        for ion in range(1, min(z, nstages - 1) + 1):
+            if z > 2:
+                nel = z - ion + 1
+                e = ph1[0, z - 1, nel - 1, outer_shell(z, nel) - 1] / RYD_TO_EV
+            elif z == 1:
+                e = 0.999434
+            else:
+                e = 1.80804 if ion == 1 else 4.0
+            e = float(np.float32(e))
+            if e <= nu_max:
+                out.append(e)
+    return np.sort(np.array(out, dtype=np.float64))
 
 
 @dataclass

@@ -18,7 +18,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .atomic import XSecStack, locate, outer_shell, verner
+from .atomic import XSecStack, locate, outer_shell, packed_data, verner
 
 HCRYD = 2.1799153e-11  # update_mod.f90:64
 
@@ -128,10 +128,9 @@ class BalanceTables:
 
 
 def build_balance(xsec: XSecStack, maxLev: int = 27, XHILimit: float = 0.05) -> BalanceTables:
-    from pathlib import Path
     nu = xsec.nu
     B = nu.size
-    d = dict(np.load(Path(__file__).resolve().parent / "data" / "atomic.npz"))
+    d = dict(packed_data())
     # shells ordered by species; each species' shells contiguous
     spec_index = {s: i for i, s in enumerate(xsec.species)}
     order = sorted(range(len(xsec.shells)),

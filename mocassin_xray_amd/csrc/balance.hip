@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include "moc_internal.h"
+#include "nlevel.h"
 
 namespace {
 
@@ -331,96 +332,6 @@ __device__ bool ion_balance(const MocBalanceArgs& a, const Rates& r, CellState& 
 }
 
 // forbidden-line cooling of one n-level ion [erg/s per n_H]
-__device__ double nlevel_cool(const MocBalanceArgs& a, int atom, double Te, double Ne, double scale) {
-    const int64_t* m = a.atomMeta + 8 * atom;
-    const int nf = (int)m[0], nT = (int)m[1];
-    const double* lt = a.atomLogT + m[2];
-    const double* g = a.atomG + m[3];
-    const double* en = a.atomE + m[3];
-    const double* A = a.atomA + m[4];
-    const double* q = a.atomQom + m[5];
-    const double* q2 = a.atomQ2 + m[5];
-    const int iup2p = (int)m[6];
-    const double atp = a.atomAtp[atom];
-    const int n = nf < a.maxLev ? nf : a.maxLev;
-    double x[MOC_MAXLEV][MOC_MAXLEV], y[MOC_MAXLEV], qe[MOC_MAXLEV][MOC_MAXLEV];
-    const double sqrTe = sqrt(Te), lT = log10(Te);
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) { x[i][j] = 0.; qe[i][j] = 0.; }
-    // temperature bracket shared by every transition of this ion
-    int klo = 0, khi = nT - 1;
-    double sa = 0., sb = 0., h = 1.;
-    const bool below = lT <= lt[0], above = lT >= lt[nT - 1];
-    if (nT > 2 && !below && !above) {
-        while (khi - klo > 1) {
-            const int k = (khi + klo) >> 1;
-            if (lt[k] > lT) khi = k; else klo = k;
-        }
-        h = lt[khi] - lt[klo];
-        sa = (lt[khi] - lT) / h;
-        sb = (lT - lt[klo]) / h;
-    }
-    for (int i = 0; i < n - 1; ++i)
-        for (int j = i + 1; j < n; ++j) {
-            // Upsilon of the pair (i, j) at T (spline in log T, clamped at the ends)
-            const size_t p = (size_t)i * nf + j;
-            double ups;
-            if (nT == 1) ups = q[p];
-            else if (nT == 2) ups = q[p] + (q[(size_t)nf * nf + p] - q[p]) / (lt[1] - lt[0]) * (lT - lt[0]);
-            else if (below) ups = q[p];
-            else if (above) ups = q[(size_t)(nT - 1) * nf * nf + p];
-            else {
-                const double ylo = q[(size_t)klo * nf * nf + p], yhi = q[(size_t)khi * nf * nf + p];
-                const double dlo = q2[(size_t)klo * nf * nf + p], dhi = q2[(size_t)khi * nf * nf + p];
-                ups = sa * ylo + sb * yhi + ((sa * sa * sa - sa) * dlo + (sb * sb * sb - sb) * dhi) * (h * h) / 6.;
-            }
-            const double c = 8.63e-6 * ups / sqrTe;
-            if (en[i] <= en[j]) {
-                qe[i][j] = c * exp((en[i] - en[j]) * 1.4388463 / Te) / g[i];
-                qe[j][i] = c / g[j];
-            } else {
-                qe[j][i] = c * exp((en[j] - en[i]) * 1.4388463 / Te) / g[i];
-                qe[i][j] = c / g[j];
-            }
-        }
-    for (int j = 0; j < n; ++j) x[0][j] = 1.;
-    y[0] = 1.;
-    for (int i = 1; i < n; ++i) {
-        for (int j = 0; j < n; ++j) {
-            if (j == i) continue;
-            x[i][j] += Ne * qe[j][i];
-            x[i][i] -= Ne * qe[i][j];
-            if (j > i) x[i][j] += A[(size_t)j * nf + i];
-            else x[i][i] -= A[(size_t)i * nf + j];
-        }
-        y[i] = 0.;
-    }
-    // elimination without pivoting, as luSlv
-    for (int i = 0; i < n - 1; ++i)
-        for (int k = i + 1; k < n; ++k) {
-            const double f = x[k][i] / x[i][i];
-            for (int j = i + 1; j < n; ++j) x[k][j] -= x[i][j] * f;
-            y[k] -= y[i] * f;
-        }
-    y[n - 1] /= x[n - 1][n - 1];
-    for (int k = n - 2; k >= 0; --k) {
-        for (int j = k + 1; j < n; ++j) y[k] -= y[j] * x[k][j];
-        y[k] /= x[k][k];
-    }
-    double sumN = 0.;
-    for (int i = 0; i < n; ++i) sumN += y[i];
-    double cool = 0.;
-    for (int i = 0; i < n - 1; ++i)
-        for (int j = i + 1; j < n; ++j) {
-            double av = A[(size_t)j * nf + i];
-            if (av == 0.) continue;
-            if (j + 1 == iup2p && i == 0 && atp > 0.) av -= atp;
-            const double em = av * (en[j] - en[i]) * (y[j] / sumN) * scale * 1.9865e-16;
-            if (em > 1.e-35) cool += em;
-        }
-    return cool;
-}
-
 __device__ void therm(const MocBalanceArgs& a, const Rates& r, const CellState& s, double comp[4],
                       double* heat, double* cool) {
     const double T = s.Te, Ne = s.Ne;
@@ -466,7 +377,7 @@ __device__ void therm(const MocBalanceArgs& a, const Rates& r, const CellState& 
         if (z < 3) continue;
         for (int ion = 1; ion <= imin(z + 1, a.NS); ++ion) {
             const int at = a.atomIdx[z * 32 + ion];
-            if (at >= 0) coolColl += nlevel_cool(a, at, T, Ne, a.abund[e] * s.X[e][ion]);
+            if (at >= 0) coolColl += nlevel_lines(a, at, T, Ne, 0., false, a.abund[e] * s.X[e][ion], 1.9865e-16);
         }
     }
     for (int sp = 0; sp < a.nSpec; ++sp) {

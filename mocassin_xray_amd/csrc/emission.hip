@@ -1,0 +1,423 @@
+// Per-cell diffuse emission on gfx950 (emissionDriver, emission_mod.f90:36-252).
+//
+// One workgroup of four waves per cell.  The continuum part is data-parallel
+// over frequency: each lane evaluates whole bins -- the H I / He I / He II
+// free-bound fits between their edge pointers (fb_ff :389-507), the Milne
+// relation above the edges (:515-550), the heavy-ion free-bound sum
+// (:552-616), free-free with the Hummer Gaunt factor at the cell's T_e
+// (free_free :818-855) and the two-photon continua (:620-817) -- and the
+// workgroup turns the bin energies c*dnu*(e_HI+e_HeI+e_HeII) into the
+// normalised CDF recPDF(cell, :) with a wave-shuffle scan (setDiffusePDF
+// :1593-1835), 256 bins per step, the running total carried across steps.
+// The line part -- hydrogenic recombination lines per element
+// (RecLinesEmission :856-1108), He I recombination lines (:1110-1152) and the
+// n-level forbidden lines with the recombination feed (forLines :1481-1526)
+// -- is a list of tasks spread over the lanes and reduced across the
+// workgroup; its sum against the continuum gives totalLines (:1904-1910).
+// All arithmetic is FP64.  The row of recPDF is written once unnormalised and
+// rescaled in place by the lanes that wrote it.
+#include <hip/hip_runtime.h>
+
+#include "gaunt.h"
+#include "moc_internal.h"
+#include "nlevel.h"
+
+namespace {
+
+constexpr double kPi = 3.141592654;        // constants_mod.f90
+constexpr double kHcRyd = 2.1799153e-11;   // continuum_mod.f90:15
+constexpr double kHcRydK = 157893.94;      // continuum_mod.f90:14
+constexpr double kCRyd = 3.2898423e15;     // continuum_mod.f90:16
+constexpr double kHPlanck = 6.6262e-27;    // constants_mod.f90:14
+constexpr double kMilne = 4.9874105e-6;    // fb_ff :515
+constexpr int kThreads = 256;
+
+// hydroLinesTemps (grid_mod.f90:76-77)
+__constant__ double c_hlT[12] = {0.05, 0.10, 0.30, 0.50, 0.75, 1.0, 1.25, 1.50, 2.0, 3.0, 5.0, 10.0};
+
+// hydro2phot (:640-760): two-photon emission of H I (Z=1) and He II (Z=2)
+__device__ double hydro2phot(double nu, int Z, double Te, double Ne) {
+    const double nu0 = Z == 1 ? 0.7496 : 3.00;
+    if (nu >= nu0) return 0.;
+    const double y = nu / nu0;
+    const double shape = y * (1 - y) * (1 - pow(4 * y * (1 - y), 0.8)) +
+                         0.88 * pow(y * (1 - y), 1.53) * pow(4 * y * (1 - y), 0.8);
+    if (Z == 1) {
+        const double aEff = 0.8368 * pow(Te * 1.e-4, -0.723);
+        const double Ay = 202.0 * shape;
+        const double gNu = kHPlanck * y * Ay / 8.2249 * 1e27;
+        double Q;
+        if (Te <= 10000.) Q = 5.31e-4;
+        else if (Te >= 20000.) Q = 4.71e-4;
+        else Q = pow(10., log10(5.31e-4) + log10(4.71e-4 / 5.31e-4) / log10(2.) * log10(Te / 10000.));
+        return aEff * gNu / (1. + (Ne * Q) / 8.23);
+    }
+    double aEff;
+    if (Te <= 5000.) aEff = 6.161;
+    else if (Te >= 30000.) aEff = 2.035;
+    else if (Te <= 10000.) aEff = pow(10., log10(6.161) + log10(4.091 / 6.161) / log10(2.) * log10(Te / 5000.));
+    else if (Te <= 15000.) aEff = pow(10., log10(4.091) + log10(3.189 / 4.091) / log10(1.5) * log10(Te / 10000.));
+    else aEff = pow(10., log10(3.189) + log10(2.035 / 3.189) / log10(2.) * log10(Te / 15000.));
+    const double Ay = 64. * 0.9994667 * 202.0 * shape;
+    const double gNu = kHPlanck * y * Ay / (8.226 * 64.) * 1e27;
+    return aEff * gNu;
+}
+
+// nearest temperature bin of the hydrogenic line tables (:877-888)
+__device__ int hl_itemp(double T4) {
+    int itemp = 1;
+    for (int i = 1; i <= 12; ++i)
+        if (T4 > c_hlT[i - 1]) itemp = i;
+    if (itemp < 12 && T4 > (c_hlT[itemp] + c_hlT[itemp - 1]) / 2.) ++itemp;
+    return itemp;
+}
+
+__device__ int hl_idenp(const MocEmissionArgs& a, int f, double Ne) {
+    const double* dens = a.hlDens + (size_t)f * 14;
+    int idenp = 1;
+    for (int iden = 1; iden <= 13; ++iden)
+        if (Ne > dens[iden - 1] && dens[iden - 1] > 0.) idenp = iden;
+    return idenp;
+}
+
+// the lines of table file f interpolated in density, each times Hb (:915-938)
+__device__ double hl_lines(const MocEmissionArgs& a, int f, double Ne, double Hb) {
+    const double* dens = a.hlDens + (size_t)f * 14;
+    const double* L = a.hlLines + (size_t)f * 14 * 16 * 9;
+    const int idenp = hl_idenp(a, f, Ne);
+    const double* L0 = L + (size_t)(idenp - 1) * 16 * 9;
+    double acc = 0.;
+    if ((dens[12] > 0. && idenp < 13) || (dens[12] == 0. && idenp < 9)) {
+        const double* L1 = L0 + 16 * 9;
+        const double w = (Ne - dens[idenp - 1]) / (dens[idenp] - dens[idenp - 1]);
+        for (int iup = 2; iup <= 15; ++iup)
+            for (int ilow = 1; ilow <= (iup - 1 < 8 ? iup - 1 : 8); ++ilow) {
+                const double l0 = L0[iup * 9 + ilow];
+                acc += (l0 + (L1[iup * 9 + ilow] - l0) * w) * Hb;
+            }
+    } else {
+        for (int iup = 2; iup <= 15; ++iup)
+            for (int ilow = 1; ilow <= (iup - 1 < 8 ? iup - 1 : 8); ++ilow) acc += L0[iup * 9 + ilow] * Hb;
+    }
+    return acc;
+}
+
+// hydrogenic recombination lines of element izp (RecLinesEmission :890-1099)
+__device__ double hydro_element(const MocEmissionArgs& a, int izp, double Xup, double ab, double Te,
+                                double Ne, double log10Ne) {
+    const double T4 = Te / 10000.;
+    const int elUp = T4 > c_hlT[11] ? 2 : 8;
+    double Hb;
+    if (izp <= elUp) {
+        const int f = a.hlMap[(izp - 1) * 12 + hl_itemp(T4) - 1];
+        if ((izp == 1 && T4 <= 3.) || (izp > 1 && T4 <= c_hlT[11])) {
+            const double* A = a.hbA + 4 * (izp - 1);
+            const double* Bc = a.hbB + 4 * (izp - 1);
+            double zf = (log10Ne - A[1]) / A[2];
+            const double Af = A[0] * exp(-zf * zf / 2.) + A[3];
+            zf = (log10Ne - Bc[1]) / Bc[2];
+            const double Bf = Bc[0] * exp(-zf * zf / 2.) + Bc[3];
+            Hb = pow(10., Af + Bf * log10(Te));
+        } else {
+            // H above 3e4 K, He above 1e5 K: the edge tables in density and T
+            const int idenp = hl_idenp(a, f, Ne);
+            const double* dens = a.hlDens + (size_t)f * 14;
+            const double xx = log10(T4 * 1.e4);
+            double yy1, yy2, x1, x2;
+            if (izp == 1) {
+                const double* r = a.rbEdge + 2 * 26;
+                if (idenp < 13) {
+                    const double w = (Ne - dens[idenp - 1]) / (dens[idenp] - dens[idenp - 1]);
+                    yy1 = r[idenp - 1] + (r[idenp - 1] - r[idenp]) * w;
+                    yy2 = r[13 + idenp - 1] + (r[13 + idenp - 1] - r[13 + idenp]) * w;
+                } else {
+                    yy1 = r[idenp - 1];
+                    yy2 = r[13 + idenp - 1];
+                }
+                x1 = a.rbEdge[0];
+                x2 = a.rbEdge[13];
+            } else {
+                // column 9+idenp(+1) of r2aEdge clamped to its 18 entries
+                const double* r = a.r2aEdge + 2 * 18;
+                const int a9 = 9 + idenp < 18 ? 9 + idenp : 18, b9 = 10 + idenp < 18 ? 10 + idenp : 18;
+                if (idenp < 13) {
+                    const double w = (Ne - dens[idenp - 1]) / (dens[idenp] - dens[idenp - 1]);
+                    yy1 = r[idenp - 1] + (r[idenp - 1] - r[idenp]) * w;
+                    yy2 = r[a9 - 1] + (r[a9 - 1] - r[b9 - 1]) * w;
+                } else {
+                    yy1 = r[idenp - 1];
+                    yy2 = r[a9 - 1];
+                }
+                x1 = a.r2aEdge[0];
+                x2 = a.r2aEdge[9];
+            }
+            Hb = pow(10., yy1 - (yy1 - yy2) * (xx - x1) / (x2 - x1));
+        }
+        Hb = Hb * Ne * Xup * ab;
+        return hl_lines(a, f, Ne, Hb);
+    }
+    // heavier hydrogenic ions: He II tables at the Z-scaled T and n_e
+    const double z2 = (double)(izp * izp);
+    const double T4z = T4 * 4. / z2, lTeZ = log10(Te * 4. / z2);
+    const double Nez = Ne * 4. / z2, lNez = log10(Ne * 4. / z2);
+    const int f = a.hlMap[1 * 12 + hl_itemp(T4z) - 1];
+    double zf = (lNez - a.hbA[5]) / a.hbA[6];
+    const double Af = a.hbA[4] * exp(-zf * zf / 2.) + a.hbA[7];
+    zf = (lNez - a.hbB[5]) / a.hbB[6];
+    const double Bf = a.hbB[4] * exp(-zf * zf / 2.) + a.hbB[7];
+    double Hb2 = ((double)(izp * izp * izp) / 8.) * pow(10., Af + Bf * lTeZ);
+    Hb2 = Hb2 * Ne * Xup * ab;
+    return hl_lines(a, f, Nez, Hb2);
+}
+
+// He I recombination lines (:1110-1152); the T4 clamps are those written there
+__device__ double hei_lines(const MocEmissionArgs& a, double Te, double Ne, double NeGrid, double X2,
+                            double ab) {
+    double T4 = Te / 10000.;
+    if (T4 < 0.5) T4 = 0.5;
+    if (T4 < 2.0) T4 = 2.0;
+    int d;
+    if (NeGrid <= 100.) d = 0;
+    else if (NeGrid <= 1.e4) d = 1;
+    else if (NeGrid <= 1.e6) d = 2;
+    else d = 3;
+    double acc = 0.;
+    for (int i = 0; i < 34; ++i) {
+        const double* c = a.heiRec + (size_t)i * 12;
+        double v;
+        if (d > 0 && d < 3) {
+            const double* c1 = c + (d - 1) * 4;
+            const double* c2 = c + d * 4;
+            const double x1 = c1[0] * pow(T4, c1[1]) * exp(c1[2] / T4);
+            const double x2 = c2[0] * pow(T4, c2[1]) * exp(c2[2] / T4);
+            v = x1 + ((x2 - x1) * (Ne - pow(100., d)) / (pow(100., d + 1) - pow(100., d)));
+        } else {
+            const double* c1 = c + (d == 0 ? 0 : 8);
+            v = c1[0] * pow(T4, c1[1]) * exp(c1[2] / T4);
+        }
+        acc += v * Ne * ab * X2;
+    }
+    return acc;
+}
+
+struct CellCtx {
+    double Te, Ne, sqrTe;
+    double abH, XH2, abHe, XHe2, XHe3;
+    bool heOn;
+    double aEffHeI;    // He I 2^1S effective recombination (HeI2photSub :781)
+    GauntT g1, g2;     // Z = 1 and Z = 2 Gaunt fits at Te
+    double corr;       // He I 2^3S line energy (setDiffusePDF :1611-1622)
+    double ly[4];      // He II Lyman line energies (:1646-1673)
+};
+
+// c * dnu * (e_HI + e_HeI + e_HeII) of bin i (1-based), with the He I / He II
+// line energies of setDiffusePDF added in their bins
+__device__ double bin_energy(const MocEmissionArgs& a, const CellCtx& c, const double (*sLoc)[MOC_EM_MAXLIM],
+                             const double (*sFit)[MOC_EM_MAXLIM / 2], const double (*sX)[MOC_MAXS + 1],
+                             int i) {
+    const double nu = a.nu[i - 1];
+    const double Te = c.Te, Ne = c.Ne;
+    double gam[3] = {0., 0., 0.};
+    // free-bound fits between the edge pointers; a later pair overwrites
+    for (int k = 0; k < 3; ++k) {
+        const int* e = a.edge[k];
+        for (int p = 1; p <= a.nlim[k] / 2; ++p) {
+            const int ilow = e[2 * p - 2], iup = e[2 * p - 1];
+            if (i >= ilow && i <= iup - 1)
+                gam[k] = pow(10., sLoc[k][2 * p - 1] + sFit[k][p - 1] * (nu - a.nu[iup - 1]));
+        }
+    }
+    const double xlf = a.log10nu[i - 1];
+    const double ex0 = exp(-nu * kHcRydK / Te);
+    const double fourPi = 4. * kPi;
+    const double ff1 = fourPi * 54.43 * 1. * gaunt_eval(c.g1, xlf) * ex0 / c.sqrTe;
+    const double ff2 = fourPi * 54.43 * 4. * gaunt_eval(c.g2, xlf) * ex0 / c.sqrTe;
+    const double factor = nu * nu * nu / (Te * c.sqrTe);
+    if (i >= a.HIP) {
+        const double ex = exp((-nu + a.nu[a.HIP - 1]) * kHcRydK / Te);
+        gam[0] = fourPi * a.xsHI[i - a.HIP] * 2. * kHcRyd * kMilne * factor * ex * 1.e20 * 1.e20 + ff1;
+        if (c.heOn && i >= a.edge[1][a.nlim[1] - 1]) {
+            const double sI = i >= a.HeIP ? a.xsHeI[i - a.HeIP] : 0.;
+            const double exI = exp((-nu + a.nu[a.HeIP - 1]) * kHcRydK / Te);
+            gam[1] = fourPi * sI * 0.5 * kHcRyd * kMilne * factor * exI * 1.e20 * 1.e20 + ff1;
+            if (i >= a.edge[2][a.nlim[2] - 1]) {
+                const double sII = i >= a.HeIIP ? a.xsHeII[i - a.HeIIP] : 0.;
+                const double exII = exp((-nu + a.nu[a.HeIIP - 1]) * kHcRydK / Te);
+                gam[2] = fourPi * sII * 2. * kHcRyd * kMilne * factor * exII * 1.e20 * 1.e20 + ff2;
+            }
+        }
+    }
+    // heavy-ion free-bound (Z = 3..19 entries in Z, ion order)
+    double gHv = 0.;
+    for (int n = 0; n < a.nFb; ++n) {
+        const int lo = a.fbLo[n];
+        if (i >= lo && i < a.fbHi[n]) {
+            const int s = a.fbSlot[n];
+            const double ex = exp((-nu + a.nu[lo - 1]) * kHcRydK / Te);
+            gHv += ex * a.fbXs[a.fbOff[n] + (i - lo)] * a.fbG[n] * sX[s][a.fbIon[n] + 1] * a.abund[s];
+        }
+    }
+    gHv = gHv * fourPi * kMilne * kHcRyd * factor * 1.e20 * 1.e20;
+    // two-photon continua
+    const double t2HI = hydro2phot(nu, 1, Te, Ne);
+    const double t2HeII = hydro2phot(nu, 2, Te, Ne);
+    double t2HeI = 0.;
+    const int j = a.he2Row[i - 1];
+    if (j > 0) {
+        const double* yd = a.hei2phot;
+        const double y = nu / 1.514;
+        const double f1 = (yd[2 * j + 1] - yd[2 * (j - 1) + 1]) / (yd[2 * j] - yd[2 * (j - 1)]);
+        const double f2 = yd[2 * (j - 1) + 1] - yd[2 * (j - 1)] * f1;
+        double Ay = f1 * y + f2;
+        if (Ay < 9.2163086E-03) Ay = 0.;
+        t2HeI = c.aEffHeI * Ay * 0.66262 * y / 51.3;
+    }
+    double eHI = (gam[0] + t2HI) * c.abH * c.XH2 * Ne + gHv * Ne;
+    double eHeI = c.heOn ? (gam[1] + t2HeI) * c.abHe * c.XHe2 * Ne : 0.;
+    double eHeII = c.heOn ? (gam[2] + t2HeII) * c.abHe * c.XHe3 * Ne : 0.;
+    if (eHI < 1.e-15) eHI = 0.;
+    if (eHeI < 1.e-15) eHeI = 0.;
+    if (eHeII < 1.e-15) eHeII = 0.;
+    const double w = a.widFlx[i - 1];
+    const double sHI = kCRyd * w * eHI / 1.e15;
+    double sHeI = kCRyd * w * eHeI / 1.e15;
+    double sHeII = kCRyd * w * eHeII / 1.e15;
+    if (c.heOn) {
+        if (i == a.j2TsP) sHeI += c.corr;
+        for (int k = 0; k < 4; ++k)
+            if (i == a.lymP[k]) sHeII += c.ly[k];
+    }
+    return sHI + sHeI + sHeII;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(kThreads) k_emission(MocEmissionArgs a) {
+    __shared__ double sX[MOC_MAXE][MOC_MAXS + 1];
+    __shared__ double sLoc[3][MOC_EM_MAXLIM];
+    __shared__ double sFit[3][MOC_EM_MAXLIM / 2];
+    __shared__ double sWave[kThreads / 64];
+    __shared__ double sLines[kThreads / 64];
+    const int cell = blockIdx.x + 1;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int C1 = a.C1, B = a.B;
+    const double NeGrid = a.Ne[cell], Te = a.Te[cell];
+    const double log10Ne = log10(NeGrid);
+    const double Ne = NeGrid < 1.e-5 ? 1.e-5 : NeGrid;
+    for (int k = tid; k < a.NE * a.NS; k += kThreads) {
+        const int e = k % a.NE, ion = k / a.NE + 1;
+        sX[e][ion] = a.ionDen[(size_t)k * C1 + cell];
+    }
+    // gamma fits at Te: log-log interpolation in T (fb_ff :389-432)
+    for (int k = 0; k < 3; ++k) {
+        const int nl = a.nlim[k];
+        const double* lg = a.lg[k];
+        for (int i = tid; i < nl; i += kThreads) {
+            double v;
+            if (Te <= a.tk[0]) v = lg[i];
+            else if (Te >= a.tk[a.ntk - 1]) v = lg[(size_t)(a.ntk - 1) * nl + i];
+            else {
+                int lo = 0, hi = a.ntk - 1;   // tk[lo] <= Te < tk[hi]
+                while (hi - lo > 1) {
+                    const int m = (lo + hi) >> 1;
+                    if (a.tk[m] <= Te) lo = m; else hi = m;
+                }
+                const double factor = log10(Te / a.tk[lo]);
+                const double s = (lg[(size_t)hi * nl + i] - lg[(size_t)lo * nl + i]) / log10(a.tk[hi] / a.tk[lo]);
+                v = lg[(size_t)lo * nl + i] + s * factor;
+            }
+            sLoc[k][i] = v;
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < 3; ++k)
+        for (int p = tid + 1; p <= a.nlim[k] / 2; p += kThreads) {
+            const int ilow = a.edge[k][2 * p - 2], iup = a.edge[k][2 * p - 1];
+            sFit[k][p - 1] = (sLoc[k][2 * p - 1] - sLoc[k][2 * p - 2]) / (a.nu[iup - 1] - a.nu[ilow - 1]);
+        }
+    CellCtx c;
+    c.Te = Te;
+    c.Ne = Ne;
+    c.sqrTe = sqrt(Te);
+    c.abH = a.abund[a.hSlot];
+    c.XH2 = sX[a.hSlot][2];
+    c.heOn = a.heSlot >= 0 && a.xsHeI && a.xsHeII;
+    c.abHe = c.heOn ? a.abund[a.heSlot] : 0.;
+    c.XHe2 = c.heOn ? sX[a.heSlot][2] : 0.;
+    c.XHe3 = c.heOn ? sX[a.heSlot][3] : 0.;
+    c.aEffHeI = 6.23 * pow(Te / 10000., -0.827);
+    c.g1 = gaunt_setup(0., log10(Te));
+    c.g2 = gaunt_setup(0.30103, log10(Te));
+    c.corr = 0.;
+    for (int k = 0; k < 4; ++k) c.ly[k] = 0.;
+    if (c.heOn) {
+        const double T4 = Te * 1.e-4;
+        c.corr = 27.2 * pow(T4, -0.678) * Ne * c.abHe * c.XHe2 * 1.45673 * kHcRyd * 1e11;
+        c.ly[3] = pow(10., -0.792 * log10(Te) + 6.01) * Ne * c.XHe3 * c.abHe;
+        for (int k = 0; k < 3; ++k) c.ly[k] = a.heiiLyman[2 * k] * c.ly[3];
+    }
+    // ---- lines: hydrogenic elements, He I, n-level ions
+    double lines = 0.;
+    const int nTasks = a.nHyd + 1 + a.nFor;
+    for (int t = tid; t < nTasks; t += kThreads) {
+        if (t < a.nHyd) {
+            const int s = a.hydSlot[t], z = a.hydZ[t];
+            lines += hydro_element(a, z, sX[s][z + 1], a.abund[s], Te, Ne, log10Ne);
+        } else if (t == a.nHyd) {
+            if (c.heOn) lines += hei_lines(a, Te, Ne, NeGrid, c.XHe2, c.abHe);
+        } else {
+            const int k = t - a.nHyd - 1;
+            const int s = a.forSlot[k], ion = a.forIon[k];
+            const double X = sX[s][ion];
+            const double up = (ion < a.forTop[k] && X > 0.) ? sX[s][ion + 1] / X : 0.;
+            lines += nlevel_lines(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9);
+        }
+    }
+    lines = wave_sum(lines);
+    if (lane == 0) sLines[wv] = lines;
+    __syncthreads();
+    // ---- continuum CDF: scan of the bin energies, 256 bins per step
+    double* row = a.recPDF + (size_t)cell * B;
+    double carry = 0.;
+    for (int b0 = 0; b0 < B; b0 += kThreads) {
+        const int b = b0 + tid;
+        const double v = b < B ? bin_energy(a, c, sLoc, sFit, sX, b + 1) : 0.;
+        double sc = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const double t = __shfl_up(sc, off, 64);
+            if (lane >= off) sc += t;
+        }
+        if (lane == 63) sWave[wv] = sc;
+        __syncthreads();
+        double pre = carry;
+        for (int k = 0; k < wv; ++k) pre += sWave[k];
+        if (b < B) row[b] = pre + sc;
+        double next = carry;
+        for (int k = 0; k < kThreads / 64; ++k) next += sWave[k];
+        __syncthreads();
+        carry = next;
+    }
+    const double total = carry;
+    for (int b = tid; b < B; b += kThreads) row[b] = b == B - 1 ? 1. : row[b] / total;
+    if (tid == 0) {
+        double L = 0.;
+        for (int k = 0; k < kThreads / 64; ++k) L += sLines[k];
+        a.totalLines[cell] = L / (total + L);
+    }
+}
+
+}  // namespace
+
+int moc_launch_emission(moc_ctx* ctx, const MocEmissionArgs& a) {
+    if (a.nCells < 1) return MOC_OK;
+    hipLaunchKernelGGL(k_emission, dim3(a.nCells), dim3(kThreads), 0, ctx->stream, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        ctx->err = std::string("emission launch: ") + hipGetErrorString(e);
+        return MOC_ERR_HIP;
+    }
+    return MOC_OK;
+}

@@ -128,6 +128,9 @@ struct MocBalanceArgs {
     const double* atomA;
     const double* atomQom;    // [nT][nLev][nLev] per atom
     const double* atomQ2;     // spline second derivatives, same layout
+    const int* atomRecKind;   // [nAtoms] recombination feed: 0 none, 1 fit per level, 2 log-T cubic
+    const double* atomRec;    // [levels][11] per-level recombination coefficients
+    const int* atomRecValid;  // [levels]
     int colHit, colComp;      // rate columns of the hit test and the Compton sums
     const double* R;          // [ncol][C1] rates
     double* Te;
@@ -138,6 +141,67 @@ struct MocBalanceArgs {
     int* lgBlack;
     int* nIterT;
     unsigned long long* counters;  // noHit, noIonBal, noTeBal
+};
+
+// Emission kernel view (emission.hip)
+#define MOC_EM_MAXLIM 64   // points of a gamma fit table
+struct MocEmissionArgs {
+    int nCells, C1, B, NE, NS;
+    const double* nu;        // [B]
+    const double* widFlx;    // [B]
+    const double* log10nu;   // [B]
+    const double* abund;     // [NE]
+    int ntk;
+    const double* tk;        // [ntk]
+    int nlim[3];             // H I, He I, He II
+    const int* edge[3];      // [nlim] 1-based bin pointers
+    const double* lg[3];     // [ntk][nlim] log10 gamma
+    int HIP, HeIP, HeIIP;    // threshold bins (He: 0 when off)
+    const double* xsHI;
+    const double* xsHeI;
+    const double* xsHeII;
+    int hSlot, heSlot;       // element slots of H and He (-1 when off)
+    int nFb;                 // heavy-ion free-bound entries (Z, then ion, ascending)
+    const int* fbSlot;
+    const int* fbIon;
+    const int* fbLo;
+    const int* fbHi;
+    const int64_t* fbOff;
+    const double* fbXs;
+    const double* fbG;       // g0/g1
+    const int* he2Row;       // [B]
+    const double* hei2phot;  // [41][2]
+    int nHyd;                // hydrogenic line emitters, Z ascending
+    const int* hydZ;
+    const int* hydSlot;
+    int nFor;                // ions with n-level data
+    const int* forSlot;
+    const int* forIon;
+    const int* forAtom;
+    const int* forTop;       // min(Z+1, nstages)
+    const int* hlMap;        // [9][12]
+    const double* hlDens;    // [files][14]
+    const double* hlLines;   // [files][14][16][9]
+    const double* hbA;       // [9][4]
+    const double* hbB;
+    const double* rbEdge;    // [2][3][26]
+    const double* r2aEdge;   // [3][18]
+    const double* heiRec;    // [34][3][4]
+    const double* heiiLyman; // [4][2]
+    int j2TsP;
+    int lymP[4];
+    const double* Te;        // [C1]
+    const double* Ne;
+    const double* ionDen;    // [NS][NE][C1]
+    double* recPDF;          // [C1][B]
+    double* totalLines;      // [C1]
+    MocBalanceArgs bal;      // n-level atoms
+};
+
+struct MocEmission {
+    bool set = false;
+    MocEmissionArgs a{};          // device pointers (grid-independent part)
+    std::vector<void*> owned;
 };
 
 struct MocGrid {
@@ -181,6 +245,7 @@ struct MocBalance {
     double* d_W = nullptr;        // [B][ncol]
     std::vector<void*> owned;     // device allocations of this table set
     unsigned long long* d_counters = nullptr;
+    std::vector<int> atomIdx;     // host copy of (Z, stage) -> n-level ion
 };
 
 struct MocAtomic {
@@ -209,6 +274,7 @@ struct moc_ctx {
     MocGrid grid;
     MocAtomic atomic;
     MocBalance balance;
+    MocEmission emission;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -219,5 +285,6 @@ int moc_launch_transport_f32(moc_ctx* ctx, const MocTransportArgs<float>& a, flo
 void moc_convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);
 void moc_scale_f64(double* p, int64_t n, double f, hipStream_t s);
 int moc_launch_opacity(moc_ctx* ctx, const MocOpacityArgs& a);
+int moc_launch_emission(moc_ctx* ctx, const MocEmissionArgs& a);
 int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a, const double* J, const double* W,
                        int ncol, double* R);

@@ -71,6 +71,28 @@ void free_balance(MocBalance& b) {
     b = MocBalance();
 }
 
+void free_emission(MocEmission& e) {
+    for (void* p : e.owned) (void)hipFree(p);
+    e.owned.clear();
+    e = MocEmission();
+}
+
+// copy a host table into a new device buffer owned by the emission set
+template <typename T>
+int em_put(moc_ctx* ctx, const T* h, size_t n, const T** out, const char* what) {
+    T* d = nullptr;
+    int rc = dalloc_n(ctx, &d, n ? n : 1, what);
+    if (rc) return rc;
+    ctx->emission.owned.push_back((void*)d);
+    if (n) {
+        if (!h) { ctx->err = std::string("moc_set_emission: missing table ") + what; return MOC_ERR_ARG; }
+        rc = h2d(ctx, d, h, n * sizeof(T), what);
+        if (rc) return rc;
+    }
+    *out = d;
+    return MOC_OK;
+}
+
 // copy a host table into a new device buffer owned by the balance set
 template <typename T>
 int bal_put(moc_ctx* ctx, const T* h, size_t n, const T** out, const char* what) {
@@ -271,6 +293,7 @@ void moc_destroy(moc_ctx* ctx) {
     free_grid(ctx->grid);
     free_atomic(ctx->atomic);
     free_balance(ctx->balance);
+    free_emission(ctx->emission);
     dfree(ctx->d_nu); dfree(ctx->d_KNsigmaT); dfree(ctx->d_KNcdf); dfree(ctx->d_Pcomp);
     dfree(ctx->d_sourceCDF);
     dfree(ctx->f_nu); dfree(ctx->f_KNsigmaT); dfree(ctx->f_KNcdf); dfree(ctx->f_Pcomp);
@@ -386,6 +409,9 @@ int moc_set_atomic(moc_ctx* ctx, const moc_atomic* at) {
     const bool same_shape = ctx->atomic.set && ctx->atomic.nElemUsed == at->nElemUsed &&
                             ctx->atomic.nstages == at->nstages;
     free_atomic(ctx->atomic);
+    // the balance and emission sets point into the element set: both must be set again
+    free_balance(ctx->balance);
+    free_emission(ctx->emission);
     if (!same_shape) dfree(ctx->grid.d_ionDen);
     MocAtomic& a = ctx->atomic;
     a.nElemUsed = at->nElemUsed;
@@ -638,6 +664,7 @@ int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* t) {
             return MOC_ERR_ARG;
         }
     free_balance(ctx->balance);
+    free_emission(ctx->emission);  // its n-level ion list refers to these atoms
     MocBalance& b = ctx->balance;
     MocBalanceArgs& a = b.a;
     const int B = ctx->nbins, S = at.nSpecies;
@@ -673,6 +700,7 @@ int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* t) {
     PUT(a.chexIdx, chexIdx.data(), 31 * 32, "chexIdx");
     PUT(a.chexTab, t->chexFit, 4 * (size_t)t->nChex, "chexFit");
     PUT(a.atomIdx, t->atomIdx, 31 * 32, "atomIdx");
+    b.atomIdx.assign(t->atomIdx, t->atomIdx + 31 * 32);
     PUT(a.atomMeta, t->atomMeta, 8 * (size_t)t->nAtoms, "atomMeta");
     PUT(a.atomAtp, t->atomAtp, t->nAtoms, "atomAtp");
     PUT(a.atomLogT, t->atomLogT, t->nLogT, "atomLogT");
@@ -681,6 +709,16 @@ int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* t) {
     PUT(a.atomA, t->atomA, t->nA, "atomA");
     PUT(a.atomQom, t->atomQom, t->nQ, "atomQom");
     PUT(a.atomQ2, t->atomQ2, t->nQ, "atomQ2");
+    if (t->atomRecKind) {
+        PUT(a.atomRecKind, t->atomRecKind, t->nAtoms, "atomRecKind");
+        PUT(a.atomRec, t->atomRec, 11 * (size_t)t->nLev, "atomRec");
+        PUT(a.atomRecValid, t->atomRecValid, t->nLev, "atomRecValid");
+    } else {
+        std::vector<int> none((size_t)t->nAtoms, 0);
+        PUT(a.atomRecKind, none.data(), none.size(), "atomRecKind");
+        a.atomRec = nullptr;
+        a.atomRecValid = nullptr;
+    }
     PUT(p_d, t->W, (size_t)B * b.ncol, "W");
     b.d_W = nullptr;  // W lives in b.owned
 #undef PUT
@@ -749,6 +787,164 @@ int moc_update(moc_ctx* ctx, int ig, moc_update_stats* stats) {
         stats->converged = nc;
         stats->kernel_ms = ms;
     }
+    return MOC_OK;
+}
+
+int moc_set_emission(moc_ctx* ctx, const moc_emission_tables* t) {
+    if (!ctx) return MOC_ERR_ARG;
+    if (!ctx->balance.set) { ctx->err = "moc_set_emission: call moc_set_balance first"; return MOC_ERR_STATE; }
+    MocAtomic& at = ctx->atomic;
+    const int B = ctx->nbins;
+    if (!t || t->ntk < 2 || !t->tkGamma) { ctx->err = "moc_set_emission: invalid gamma tables"; return MOC_ERR_ARG; }
+    for (int k = 0; k < 3; ++k) {
+        if (t->nlim[k] < 2 || t->nlim[k] > MOC_EM_MAXLIM || (t->nlim[k] & 1) || !t->edgeP[k] || !t->logGamma[k]) {
+            ctx->err = "moc_set_emission: gamma fit sizes (even, <= 64)";
+            return MOC_ERR_ARG;
+        }
+        for (int i = 0; i < t->nlim[k]; ++i)
+            if (t->edgeP[k][i] < 1 || t->edgeP[k][i] > B) { ctx->err = "moc_set_emission: edge pointer outside the mesh"; return MOC_ERR_ARG; }
+        for (int p = 0; p < t->nlim[k] / 2; ++p)
+            if (t->edgeP[k][2 * p] == t->edgeP[k][2 * p + 1]) {
+                ctx->err = "moc_set_emission: frequency grid too coarse for the free-bound edges";
+                return MOC_ERR_ARG;
+            }
+    }
+    std::vector<int> slot(31, -1);
+    for (int e = 0; e < at.nElemUsed; ++e) slot[at.elemZ[e]] = e;
+    const bool heOn = slot[2] >= 0;
+    if (t->HIPnuP < 1 || t->HIPnuP > B || !t->xsHI ||
+        (heOn && (!t->xsHeI || !t->xsHeII || t->HeIPnuP < 1 || t->HeIPnuP > B || t->HeIIPnuP < 1 || t->HeIIPnuP > B))) {
+        ctx->err = "moc_set_emission: H / He ground-state cross-sections";
+        return MOC_ERR_ARG;
+    }
+    if (t->j2TsP < 1 || t->j2TsP > B || t->nFb < 0 || t->nHlFiles < 1) {
+        ctx->err = "moc_set_emission: invalid line pointers or table sizes";
+        return MOC_ERR_ARG;
+    }
+    for (int k = 0; k < 4; ++k)
+        if (t->lymP[k] < 0 || t->lymP[k] > B) { ctx->err = "moc_set_emission: He II Lyman bins"; return MOC_ERR_ARG; }
+    for (int i = 0; i < 9 * 12; ++i)
+        if (t->hlMap[i] < 0 || t->hlMap[i] >= t->nHlFiles) { ctx->err = "moc_set_emission: hlMap"; return MOC_ERR_ARG; }
+    for (int i = 0; i < B; ++i)
+        if (t->he2Row[i] < 0 || t->he2Row[i] > 40) { ctx->err = "moc_set_emission: he2Row"; return MOC_ERR_ARG; }
+    std::vector<int> fbSlot((size_t)t->nFb);
+    for (int n = 0; n < t->nFb; ++n) {
+        const int z = t->fbZ[n], ion = t->fbIon[n];
+        if (z < 3 || z > 19 || slot[z] < 0 || ion < 1 || ion >= at.nstages || t->fbLo[n] < 1 ||
+            t->fbHi[n] > B || t->fbOff[n] < 0 ||
+            (t->fbHi[n] > t->fbLo[n] && t->fbOff[n] + (t->fbHi[n] - t->fbLo[n]) > t->nFbXs)) {
+            ctx->err = "moc_set_emission: heavy-ion free-bound entry";
+            return MOC_ERR_ARG;
+        }
+        fbSlot[n] = slot[z];
+    }
+    // line emitters: hydrogenic ions (Z on, nstages > Z), n-level ions with data
+    std::vector<int> hydZ, hydSlot, forSlot, forIon, forAtom, forTop;
+    for (int z = 1; z <= 30; ++z)
+        if (slot[z] >= 0 && at.nstages > z) { hydZ.push_back(z); hydSlot.push_back(slot[z]); }
+    const std::vector<int>& aidx = ctx->balance.atomIdx;
+    for (int z = 3; z <= 30; ++z) {
+        if (slot[z] < 0) continue;
+        const int top = z + 1 < at.nstages ? z + 1 : at.nstages;
+        for (int ion = 1; ion <= top; ++ion) {
+            const int atom = aidx[(size_t)z * 32 + ion];
+            if (atom < 0) continue;
+            forSlot.push_back(slot[z]); forIon.push_back(ion); forAtom.push_back(atom); forTop.push_back(top);
+        }
+    }
+    free_emission(ctx->emission);
+    MocEmission& em = ctx->emission;
+    MocEmissionArgs& a = em.a;
+    int rc;
+    const double* pd;
+#define EPUT(dst, src, n, what) do { if ((rc = em_put(ctx, src, (size_t)(n), &dst, what))) return rc; } while (0)
+    EPUT(pd, ctx->widFlx.data(), B, "widFlx");
+    a.widFlx = pd;
+    EPUT(a.tk, t->tkGamma, t->ntk, "tkGamma");
+    for (int k = 0; k < 3; ++k) {
+        a.nlim[k] = t->nlim[k];
+        EPUT(a.edge[k], t->edgeP[k], t->nlim[k], "edgeP");
+        EPUT(a.lg[k], t->logGamma[k], (size_t)t->ntk * t->nlim[k], "logGamma");
+    }
+    EPUT(a.xsHI, t->xsHI, B - t->HIPnuP + 1, "xsHI");
+    if (heOn) {
+        EPUT(a.xsHeI, t->xsHeI, B - t->HeIPnuP + 1, "xsHeI");
+        EPUT(a.xsHeII, t->xsHeII, B - t->HeIIPnuP + 1, "xsHeII");
+    } else {
+        a.xsHeI = nullptr;
+        a.xsHeII = nullptr;
+    }
+    EPUT(a.fbSlot, fbSlot.data(), fbSlot.size(), "fbSlot");
+    EPUT(a.fbIon, t->fbIon, t->nFb, "fbIon");
+    EPUT(a.fbLo, t->fbLo, t->nFb, "fbLo");
+    EPUT(a.fbHi, t->fbHi, t->nFb, "fbHi");
+    EPUT(a.fbOff, t->fbOff, t->nFb, "fbOff");
+    EPUT(a.fbXs, t->fbXs, t->nFbXs, "fbXs");
+    EPUT(a.fbG, t->fbGratio, t->nFb, "fbGratio");
+    EPUT(a.he2Row, t->he2Row, B, "he2Row");
+    EPUT(a.hei2phot, t->hei2phot, 82, "hei2phot");
+    EPUT(a.hydZ, hydZ.data(), hydZ.size(), "hydZ");
+    EPUT(a.hydSlot, hydSlot.data(), hydSlot.size(), "hydSlot");
+    EPUT(a.forSlot, forSlot.data(), forSlot.size(), "forSlot");
+    EPUT(a.forIon, forIon.data(), forIon.size(), "forIon");
+    EPUT(a.forAtom, forAtom.data(), forAtom.size(), "forAtom");
+    EPUT(a.forTop, forTop.data(), forTop.size(), "forTop");
+    EPUT(a.hlMap, t->hlMap, 9 * 12, "hlMap");
+    EPUT(a.hlDens, t->hlDens, 14 * (size_t)t->nHlFiles, "hlDens");
+    EPUT(a.hlLines, t->hlLines, 14 * 16 * 9 * (size_t)t->nHlFiles, "hlLines");
+    EPUT(a.hbA, t->hbA, 36, "hbA");
+    EPUT(a.hbB, t->hbB, 36, "hbB");
+    EPUT(a.rbEdge, t->rbEdge, 2 * 3 * 26, "rbEdge");
+    EPUT(a.r2aEdge, t->r2aEdge, 3 * 18, "r2aEdge");
+    EPUT(a.heiRec, t->heiRec, 34 * 3 * 4, "heiRec");
+    EPUT(a.heiiLyman, t->heiiLyman, 8, "heiiLyman");
+#undef EPUT
+    a.B = B;
+    a.NE = at.nElemUsed;
+    a.NS = at.nstages;
+    a.nu = ctx->d_nu;
+    a.log10nu = at.d_log10nu;
+    a.abund = at.d_abund;
+    a.ntk = t->ntk;
+    a.HIP = t->HIPnuP;
+    a.HeIP = heOn ? t->HeIPnuP : 0;
+    a.HeIIP = heOn ? t->HeIIPnuP : 0;
+    a.hSlot = slot[1];
+    a.heSlot = slot[2];
+    a.nFb = t->nFb;
+    a.nHyd = (int)hydZ.size();
+    a.nFor = (int)forAtom.size();
+    a.j2TsP = t->j2TsP;
+    for (int k = 0; k < 4; ++k) a.lymP[k] = t->lymP[k];
+    em.set = true;
+    return MOC_OK;
+}
+
+int moc_emission(moc_ctx* ctx, int ig, float* kernel_ms) {
+    if (!ctx) return MOC_ERR_ARG;
+    MocGrid& g = ctx->grid;
+    if (!g.attached || ig != 1 || !ctx->emission.set || !ctx->balance.set || !g.d_ionDen) {
+        ctx->err = "moc_emission: needs an attached grid, moc_set_balance and moc_set_emission";
+        return MOC_ERR_STATE;
+    }
+    MocEmissionArgs a = ctx->emission.a;
+    a.bal = ctx->balance.a;
+    a.nCells = g.nCells;
+    a.C1 = g.nCells + 1;
+    a.Te = g.d_Te;
+    a.Ne = g.d_Ne;
+    a.ionDen = g.d_ionDen;
+    a.recPDF = g.d_recPDF;
+    a.totalLines = g.d_totalLines;
+    MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
+    int rc = moc_launch_emission(ctx, a);
+    if (rc) return rc;
+    MOC_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
+    MOC_TRY(hipEventSynchronize(ctx->ev1), "emission kernel");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    if (kernel_ms) *kernel_ms = ms;
+    g.f32_dirty = true;
     return MOC_OK;
 }
 

@@ -1,0 +1,128 @@
+// n-level statistical equilibrium of one ion (equilibrium, emission_mod.f90:
+// 2505-3028, with luSlv :3031-3106), shared by the thermal balance (cooling,
+// rec = false) and the diffuse emission (line luminosity, rec = true).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "moc_internal.h"
+
+namespace {
+
+// Sum of the line emissivities A_ji dE_ij n_j * scale * unit of ion `atom`
+// that exceed 1e-35.  With rec, levels 2.. are fed by recombination from the
+// next stage (ionDenUp = X(i+1)/X(i)) at the rates of the ion's data file
+// (:2828-2877; kind 1: power-law radiative plus dielectronic fit per level,
+// kind 2: cubic in log T).  Upsilon(T) is the natural cubic spline in log T
+// with second derivatives computed on the host, clamped at the table ends.
+__device__ double nlevel_lines(const MocBalanceArgs& a, int atom, double Te, double Ne,
+                               double ionDenUp, bool rec, double scale, double unit) {
+    const int64_t* m = a.atomMeta + 8 * atom;
+    const int nf = (int)m[0], nT = (int)m[1];
+    const double* lt = a.atomLogT + m[2];
+    const double* g = a.atomG + m[3];
+    const double* en = a.atomE + m[3];
+    const double* A = a.atomA + m[4];
+    const double* q = a.atomQom + m[5];
+    const double* q2 = a.atomQ2 + m[5];
+    const int iup2p = (int)m[6];
+    const double atp = a.atomAtp[atom];
+    const int n = nf < a.maxLev ? nf : a.maxLev;
+    double x[MOC_MAXLEV][MOC_MAXLEV], y[MOC_MAXLEV], qe[MOC_MAXLEV][MOC_MAXLEV];
+    const double sqrTe = sqrt(Te), lT = log10(Te);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) { x[i][j] = 0.; qe[i][j] = 0.; }
+    // temperature bracket shared by every transition of this ion
+    int klo = 0, khi = nT - 1;
+    double sa = 0., sb = 0., h = 1.;
+    const bool below = lT <= lt[0], above = lT >= lt[nT - 1];
+    if (nT > 2 && !below && !above) {
+        while (khi - klo > 1) {
+            const int k = (khi + klo) >> 1;
+            if (lt[k] > lT) khi = k; else klo = k;
+        }
+        h = lt[khi] - lt[klo];
+        sa = (lt[khi] - lT) / h;
+        sb = (lT - lt[klo]) / h;
+    }
+    for (int i = 0; i < n - 1; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            const size_t p = (size_t)i * nf + j;
+            double ups;
+            if (nT == 1) ups = q[p];
+            else if (nT == 2) ups = q[p] + (q[(size_t)nf * nf + p] - q[p]) / (lt[1] - lt[0]) * (lT - lt[0]);
+            else if (below) ups = q[p];
+            else if (above) ups = q[(size_t)(nT - 1) * nf * nf + p];
+            else {
+                const double ylo = q[(size_t)klo * nf * nf + p], yhi = q[(size_t)khi * nf * nf + p];
+                const double dlo = q2[(size_t)klo * nf * nf + p], dhi = q2[(size_t)khi * nf * nf + p];
+                ups = sa * ylo + sb * yhi + ((sa * sa * sa - sa) * dlo + (sb * sb * sb - sb) * dhi) * (h * h) / 6.;
+            }
+            const double c = 8.63e-6 * ups / sqrTe;
+            if (en[i] <= en[j]) {
+                qe[i][j] = c * exp((en[i] - en[j]) * 1.4388463 / Te) / g[i];
+                qe[j][i] = c / g[j];
+            } else {
+                qe[j][i] = c * exp((en[j] - en[i]) * 1.4388463 / Te) / g[i];
+                qe[i][j] = c / g[j];
+            }
+        }
+    for (int j = 0; j < n; ++j) x[0][j] = 1.;
+    y[0] = 1.;
+    for (int i = 1; i < n; ++i) {
+        for (int j = 0; j < n; ++j) {
+            if (j == i) continue;
+            x[i][j] += Ne * qe[j][i];
+            x[i][i] -= Ne * qe[i][j];
+            if (j > i) x[i][j] += A[(size_t)j * nf + i];
+            else x[i][i] -= A[(size_t)i * nf + j];
+        }
+        y[i] = 0.;
+    }
+    const int kind = rec ? a.atomRecKind[atom] : 0;
+    if (kind != 0) {
+        const double* rd = a.atomRec + 11 * m[3];
+        const int* rv = a.atomRecValid + m[3];
+        const double t4 = Te * 1.e-4;
+        for (int i = 1; i < n; ++i) {
+            double al = 0.;
+            if (rv[i]) {
+                const double* r = rd + 11 * i;
+                if (kind == 1) {
+                    const double z = r[1], tz = t4 / (z * z);
+                    al = 1.e-13 * r[0] * z * r[2] * pow(tz, r[3]) / (1. + r[4] * pow(tz, r[5])) +
+                         1.0e-12 * r[0] * (r[6] / t4 + r[7] + r[8] * t4 + r[9] * t4 * t4) * pow(t4, -1.5) *
+                             exp(-r[10] * 1.e4 / Te);
+                } else {
+                    al = pow(10., r[0] + r[1] * lT + r[2] * lT * lT + r[3] * lT * lT * lT);
+                }
+            }
+            y[i] = -Ne * ionDenUp * al;
+        }
+    }
+    // elimination without pivoting, as luSlv
+    for (int i = 0; i < n - 1; ++i)
+        for (int k = i + 1; k < n; ++k) {
+            const double f = x[k][i] / x[i][i];
+            for (int j = i + 1; j < n; ++j) x[k][j] -= x[i][j] * f;
+            y[k] -= y[i] * f;
+        }
+    y[n - 1] /= x[n - 1][n - 1];
+    for (int k = n - 2; k >= 0; --k) {
+        for (int j = k + 1; j < n; ++j) y[k] -= y[j] * x[k][j];
+        y[k] /= x[k][k];
+    }
+    double sumN = 0.;
+    for (int i = 0; i < n; ++i) sumN += y[i];
+    double tot = 0.;
+    for (int i = 0; i < n - 1; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            double av = A[(size_t)j * nf + i];
+            if (av == 0.) continue;
+            if (j + 1 == iup2p && i == 0 && atp > 0.) av -= atp;
+            const double em = av * (en[j] - en[i]) * (y[j] / sumN) * scale * unit;
+            if (em > 1.e-35) tot += em;
+        }
+    return tot;
+}
+
+}  // namespace

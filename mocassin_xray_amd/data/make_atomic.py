@@ -245,13 +245,88 @@ def read_atom(path: Path):
         e[i] = fnum(t[2])
         if i > 1 and e[i] == e[i - 1]:
             e[i] = e[i] + 0.1
-    return nlev, ntemp, logt, qom, A, g, e, atp, iup2p
+    return nlev, ntemp, logt, qom, A, g, e, atp, iup2p, k
+
+
+class ListReader:
+    """Fortran list-directed input over a file's records: each read starts on
+    a new record, spans as many records as it needs (blank ones skipped) and
+    drops what is left of its last record."""
+
+    def __init__(self, path: Path):
+        self.rec = records(path)
+        self.k = 0
+
+    def eof(self) -> bool:
+        j = self.k
+        while j < len(self.rec) and not toks(self.rec[j].split("!")[0]):
+            j += 1
+        return j >= len(self.rec)
+
+    def read(self, n: int):
+        out = []
+        while len(out) < n:
+            if self.k >= len(self.rec):
+                raise EOFError
+            t = toks(self.rec[self.k].split("!")[0])
+            self.k += 1
+            out.extend(t)
+        return out[:n]
+
+    def line(self) -> str:
+        r = self.rec[self.k]
+        self.k += 1
+        return r
+
+
+REC_SPECIAL = {"ci.dat", "ciii.dat", "ni.dat", "neiv.dat", "oii.dat", "oiii.dat", "nii.dat",
+               "nev.dat", "niv.dat"}
+
+
+def read_rec_data(path: Path, nlev: int, lr: "ListReader"):
+    """Recombination data after the level block (equilibrium rec=.true.,
+    emission_mod.f90:2828-2877): returns (kind, table[nlev+1, 11], valid[nlev+1])."""
+    tab = np.zeros((nlev + 1, 11))
+    valid = np.zeros(nlev + 1, dtype=np.int32)
+    special = path.name in REC_SPECIAL
+    kind = 1 if special else 2
+    any_read = False
+    for j in range(2, nlev + 1):
+        if lr.eof():
+            break
+        if special:
+            v = [fnum(x) for x in lr.read(11)]
+            any_read = True
+            if v[1] > 0.0:
+                tab[j] = v
+                valid[j] = 1
+        else:
+            save = lr.k
+            t = lr.read(1)
+            jj = int(float(t[0]))
+            if jj == -9999:
+                break
+            lr.k = save
+            try:
+                v = lr.read(5)
+            except EOFError:        # a truncated last record ends the table
+                break
+            jj = int(float(v[0]))
+            if jj > nlev:
+                break
+            any_read = True
+            tab[jj, :4] = [fnum(x) for x in v[1:5]]
+            valid[jj] = 1
+    if any_read:
+        valid[1] = 1
+    return (kind if any_read else 0), tab, valid
 
 
 def read_atoms(data: Path):
     names = [l.strip() for l in records(data / "fileNames.dat")]
     idx = np.full((31, 32), -1, dtype=np.int32)
     meta, logt, g, e, A, qom = [], [], [], [], [], []
+    rkind, rtab, rvalid = [], [], []
     k = 0
     for elem in range(3, 31):
         for ion in range(1, min(elem + 1, 31) + 1):
@@ -260,7 +335,13 @@ def read_atoms(data: Path):
             p = data / Path(name).name
             if not name or not p.exists():
                 continue
-            nlev, ntemp, lt, q, a, gg, ee, atp, iup2p = read_atom(p)
+            nlev, ntemp, lt, q, a, gg, ee, atp, iup2p, kend = read_atom(p)
+            lr = ListReader(p)
+            lr.k = kend
+            kind, tab, valid = read_rec_data(p, nlev, lr)
+            rkind.append(kind)
+            rtab.extend(tab[1:].reshape(-1))
+            rvalid.extend(valid[1:])
             idx[elem, ion] = len(meta)
             meta.append([nlev, ntemp, len(logt), len(g), len(A), len(qom), iup2p])
             logt.extend(lt)
@@ -272,7 +353,72 @@ def read_atoms(data: Path):
     meta_i = np.array([m[:7] for m in meta], dtype=np.int64)
     atp = np.array([m[7] for m in meta])
     return (idx, meta_i, atp, np.array(logt), np.array(g), np.array(e), np.array(A),
-            np.array(qom, dtype=np.float32))
+            np.array(qom, dtype=np.float32), np.array(rkind, dtype=np.int32), np.array(rtab),
+            np.array(rvalid, dtype=np.int32))
+
+
+def read_emission_tables(data: Path) -> dict:
+    """Tables of the diffuse emission (emissionDriver and its callers)."""
+    out = {}
+    # H, He I, He II free-bound fits (initGammaCont ph_mod.f90:31-202)
+    for tag, fname in (("HI", "gammaHI.dat"), ("HeI", "gammaHeI.dat"), ("HeII", "gammaHeII.dat")):
+        lr = ListReader(data / fname)
+        ntk, nlim = (int(float(x)) for x in lr.read(2))
+        tk = np.array([fnum(x) for x in lr.read(ntk)])
+        nu = np.zeros(nlim)
+        val = np.zeros((ntk, nlim))
+        for i in range(nlim):
+            v = [fnum(x) for x in lr.read(ntk + 1)]
+            nu[i] = v[0]
+            val[:, i] = v[1:]
+        out[f"gam_tk_{tag}"] = tk
+        out[f"gam_nu_{tag}"] = nu
+        out[f"gam_val_{tag}"] = val
+    lr = ListReader(data / "HeI2phot.dat")
+    he2 = np.array([[fnum(x) for x in lr.read(2)] for _ in range(41)])
+    out["hei2phot"] = he2
+    # hydrogenic recombination-line tables (RecLinesEmission emission_mod.f90:856-1153)
+    names = [l.strip().strip("'\"").strip() for l in records(data / "hydroLinesFiles.dat")
+             if l.strip()][:9 * 12]
+    uniq = sorted(set(names))
+    fmap = np.array([uniq.index(n) for n in names], dtype=np.int32).reshape(9, 12)
+    dens = np.zeros((len(uniq), 14))
+    lines = np.zeros((len(uniq), 14, 16, 9))
+    for f, name in enumerate(uniq):
+        lr = ListReader(data / Path(name).name)
+        for iden in range(14):
+            if lr.eof():
+                break
+            dens[f, iden] = fnum(lr.read(1)[0])
+            for iup in range(15, 1, -1):
+                v = lr.read(min(8, iup - 1))
+                lines[f, iden, iup, 1:1 + len(v)] = [fnum(x) for x in v]
+    out["hl_map"] = fmap
+    out["hl_dens"] = dens
+    out["hl_lines"] = lines
+    lr = ListReader(data / "gaussfitHb.dat")
+    hb = np.array([[fnum(x) for x in lr.read(8)] for _ in range(9)])
+    out["hbA"] = hb[:, :4]
+    out["hbB"] = hb[:, 4:8]
+    rb = np.zeros((2, 3, 26))
+    for z, fname in ((0, "r1bEdge.dat"), (1, "r2bEdge.dat")):
+        lr = ListReader(data / fname)
+        for i in range(26):
+            rb[z, :, i] = [fnum(x) for x in lr.read(3)]
+    out["rbEdge"] = rb
+    lr = ListReader(data / "r2aEdge.dat")
+    out["r2aEdge"] = np.array([[fnum(x) for x in lr.read(3)] for _ in range(18)]).T.copy()
+    lr = ListReader(data / "HeIrecLines.dat")
+    he = np.zeros((34, 3, 4))
+    for iden in range(3):
+        for il in range(34):
+            v = [fnum(x) for x in lr.read(4)]
+            he[il, iden] = v
+            he[il, iden, 0] *= 1.e25
+    out["heiRec"] = he
+    lr = ListReader(data / "r2a0100old.dat")
+    out["heiiLyman"] = np.array([[fnum(x) for x in lr.read(2)] for _ in range(4)])
+    return out
 
 
 def main(data_dir: str = "/root/reference/data"):
@@ -284,12 +430,14 @@ def main(data_dir: str = "/root/reference/data"):
     aldro = read_rows(data / "aldrovandi.dat", 6)
     cf, th = read_cion(data)
     nauger, auger = read_auger(data)
-    aidx, ameta, aatp, alogt, ag, ae, aA, aq = read_atoms(data)
+    aidx, ameta, aatp, alogt, ag, ae, aA, aq, rk, rt, rv = read_atoms(data)
+    em = read_emission_tables(data)
     out = Path(__file__).resolve().parent / "atomic.npz"
     np.savez_compressed(out, ph1=ph1, ph2=ph2, rrb=rr, drb_c=dc, drb_e=de, rrec=rrec, rnew=rnew,
                         diel=diel, aldro=aldro, cion_cf=cf, cion_th=th, nauger=nauger,
                         auger=auger, atom_idx=aidx, atom_meta=ameta, atom_atp=aatp,
-                        atom_logt=alogt, atom_g=ag, atom_e=ae, atom_A=aA, atom_qom=aq)
+                        atom_logt=alogt, atom_g=ag, atom_e=ae, atom_A=aA, atom_qom=aq,
+                        atom_rec_kind=rk, atom_rec=rt, atom_rec_valid=rv, **em)
     print(out, out.stat().st_size)
 
 

@@ -21,8 +21,9 @@ import numpy as np
 
 from . import _lib
 from ._lib import (CELLBIN_FIELDS, FIELDS, MOC_DT_F64, MOC_DT_I32, MOC_LAYOUT_FORTRAN,
-                   MOC_PREC_FP64, MOC_PREC_MIXED, MocAtomic, MocBalanceTables, MocGridDesc,
-                   MocTables, MocTransportStats, MocUpdateStats, dptr, iptr)
+                   MOC_PREC_FP64, MOC_PREC_MIXED, MocAtomic, MocBalanceTables, MocEmissionTables,
+                   MocGridDesc, MocTables, MocTransportStats, MocUpdateStats, _dp, _ip, dptr,
+                   iptr)
 
 
 class MocError(RuntimeError):
@@ -173,7 +174,8 @@ class MocGPU:
             "cfit": _c(bt.chex[:, 2:6]), "ai": _c(d["atom_idx"], np.int32),
             "am": _c(bt.atomMeta8, np.int64), "atp": _c(d["atom_atp"]), "lt": _c(d["atom_logt"]),
             "ag": _c(d["atom_g"]), "ae": _c(d["atom_e"]), "aA": _c(d["atom_A"]),
-            "aq": _c(bt.atomQom), "aq2": _c(bt.atomQ2),
+            "aq": _c(bt.atomQom), "aq2": _c(bt.atomQ2), "rk": _c(d["atom_rec_kind"], np.int32),
+            "rt": _c(d["atom_rec"]), "rv": _c(d["atom_rec_valid"], np.int32),
         }
         self._balance_keep = k
         t = MocBalanceTables(
@@ -189,8 +191,48 @@ class MocGPU:
             atomMeta=k["am"].ctypes.data_as(lp), atomAtp=dptr(k["atp"]), nLogT=k["lt"].size,
             nLev=k["ag"].size, nA=k["aA"].size, nQ=k["aq"].size, atomLogT=dptr(k["lt"]),
             atomG=dptr(k["ag"]), atomE=dptr(k["ae"]), atomA=dptr(k["aA"]),
-            atomQom=dptr(k["aq"]), atomQ2=dptr(k["aq2"]))
+            atomQom=dptr(k["aq"]), atomQ2=dptr(k["aq2"]), atomRecKind=iptr(k["rk"]),
+            atomRec=dptr(k["rt"]), atomRecValid=iptr(k["rv"]))
         self._check(self._L.moc_set_balance(self._ctx, C.byref(t)), "moc_set_balance")
+
+    def set_emission(self, et) -> None:
+        """Emission tables (an emission.EmissionTables built on the same mesh
+        and element set); after set_balance."""
+        lp = C.POINTER(C.c_int64)
+        k = {"tk": _c(et.tk), "edge": [_c(e, np.int32) for e in et.edge],
+             "lg": [_c(v) for v in et.loggam], "xsHI": _c(et.xsHI),
+             "xsHeI": None if et.xsHeI is None else _c(et.xsHeI),
+             "xsHeII": None if et.xsHeII is None else _c(et.xsHeII),
+             "fbZ": _c(et.fbZ, np.int32), "fbIon": _c(et.fbIon, np.int32),
+             "fbLo": _c(et.fbLo, np.int32), "fbHi": _c(et.fbHi, np.int32),
+             "fbOff": _c(et.fbOff, np.int64), "fbXs": _c(et.fbXs), "fbG": _c(et.fbGratio),
+             "he2j": _c(et.he2j, np.int32), "he2": _c(et.hei2phot), "map": _c(et.hlMap, np.int32),
+             "dens": _c(et.hlDens), "lines": _c(et.hlLines), "hbA": _c(et.hbA), "hbB": _c(et.hbB),
+             "rb": _c(et.rbEdge), "r2a": _c(et.r2aEdge), "heiRec": _c(et.heiRec),
+             "lym": _c(et.heiiLyman)}
+        self._emission_keep = k
+        opt = lambda a: dptr(a) if a is not None else None
+        t = MocEmissionTables(
+            ntk=k["tk"].size, tkGamma=dptr(k["tk"]), nlim=(C.c_int32 * 3)(*[e.size for e in k["edge"]]),
+            edgeP=(_ip * 3)(*[iptr(e) for e in k["edge"]]), logGamma=(_dp * 3)(*[dptr(v) for v in k["lg"]]),
+            HIPnuP=int(et.HIPnuP), HeIPnuP=int(et.HeIPnuP), HeIIPnuP=int(et.HeIIPnuP),
+            xsHI=dptr(k["xsHI"]), xsHeI=opt(k["xsHeI"]), xsHeII=opt(k["xsHeII"]),
+            nFb=k["fbZ"].size, fbZ=iptr(k["fbZ"]), fbIon=iptr(k["fbIon"]), fbLo=iptr(k["fbLo"]),
+            fbHi=iptr(k["fbHi"]), fbOff=k["fbOff"].ctypes.data_as(lp), nFbXs=k["fbXs"].size,
+            fbXs=dptr(k["fbXs"]), fbGratio=dptr(k["fbG"]), he2Row=iptr(k["he2j"]),
+            hei2phot=dptr(k["he2"]), nHlFiles=k["dens"].shape[0], hlMap=iptr(k["map"]),
+            hlDens=dptr(k["dens"]), hlLines=dptr(k["lines"]), hbA=dptr(k["hbA"]),
+            hbB=dptr(k["hbB"]), rbEdge=dptr(k["rb"]), r2aEdge=dptr(k["r2a"]),
+            heiRec=dptr(k["heiRec"]), heiiLyman=dptr(k["lym"]), j2TsP=int(et.j2TsP),
+            lymP=(C.c_int32 * 4)(*[int(v) for v in et.lymP]))
+        self._check(self._L.moc_set_emission(self._ctx, C.byref(t)), "moc_set_emission")
+
+    def emissionDriver(self) -> float:
+        """Diffuse emission of every active cell (emission_mod.f90:36): recPDF
+        and totalLines from Te, Ne, ionDen on the device.  Returns kernel ms."""
+        ms = C.c_float(0.0)
+        self._check(self._L.moc_emission(self._ctx, 1, C.byref(ms)), "moc_emission")
+        return float(ms.value)
 
     def updateCell(self) -> dict:
         """updateCell for every active cell (update_mod.f90:15) from the

@@ -122,8 +122,8 @@ static double splint_(const double* xa, const double* ya, const double* y2, int 
 /* equilibrium(rec=.false.) (emission_mod.f90:2505-3028) for one ion, summed
  * as forLines/thermBalance do: every line emissivity A_ji dE_ij n_j, scaled
  * by abundance * ionDen * 1.9865e-16, that exceeds 1e-35 [erg/s per n_H] */
-static double nlevel_cooling(const oracle_balance_tables* t, int atom, double Te, double Ne,
-                             double scale) {
+static double nlevel_lines(const oracle_balance_tables* t, int atom, double Te, double Ne,
+                           double ionDenUp, int rec, double scale, double unit) {
     const int64_t* m = t->atom_meta + 7 * atom;
     const int nfile = (int)m[0], ntemp = (int)m[1];
     const double* lt = t->atom_logt + m[2];
@@ -177,6 +177,28 @@ static double nlevel_cooling(const oracle_balance_tables* t, int atom, double Te
         }
         y[i] = 0.;
     }
+    /* recombination feed of the levels (:2828-2877, :2961) */
+    if (rec && t->atom_rec_kind && t->atom_rec_kind[atom] != 0) {
+        const int kind = t->atom_rec_kind[atom];
+        const double* rd = t->atom_rec + 11 * m[3];
+        const int32_t* rv = t->atom_rec_valid + m[3];
+        for (i = 2; i <= n; i++) {
+            double al = 0.;
+            const double* r = rd + 11 * (i - 1);
+            if (rv[i - 1]) {
+                if (kind == 1) {
+                    double z = r[1], tz = Te * 1.e-4 / (z * z), t4 = Te * 1.e-4;
+                    al = 1.e-13 * r[0] * z * r[2] * pow(tz, r[3]) / (1. + r[4] * pow(tz, r[5])) +
+                         1.0e-12 * r[0] * (r[6] / t4 + r[7] + r[8] * t4 + r[9] * t4 * t4) * pow(t4, -1.5) *
+                             exp(-r[10] * 1.e4 / Te);
+                } else {
+                    al = pow(10., r[0] + r[1] * log10Te + r[2] * log10Te * log10Te +
+                                      r[3] * log10Te * log10Te * log10Te);
+                }
+            }
+            y[i] = -Ne * ionDenUp * al;
+        }
+    }
     /* luSlv: Gaussian elimination without pivoting (:3031-3106) */
     for (i = 1; i <= n - 1; i++)
         for (k = i + 1; k <= n; k++) {
@@ -199,12 +221,22 @@ static double nlevel_cooling(const oracle_balance_tables* t, int atom, double Te
             if (a != 0.) {
                 double em;
                 if (j == iup2p && i == 1 && atp > 0.) a -= atp;
-                em = a * (e[j - 1] - e[i - 1]) * nn[j] * scale * 1.9865e-16;
+                em = a * (e[j - 1] - e[i - 1]) * nn[j] * scale * unit;
                 if (em > 1.e-35) cool += em;
             }
         }
 #undef AA
     return cool;
+}
+
+static double nlevel_cooling(const oracle_balance_tables* t, int atom, double Te, double Ne,
+                             double scale) {
+    return nlevel_lines(t, atom, Te, Ne, 0., 0, scale, 1.9865e-16);
+}
+
+double oracle_nlevel_lines(const oracle_balance_tables* t, int atom, double Te, double Ne,
+                           double ionDenUp, int rec, double scale, double unit) {
+    return nlevel_lines(t, atom, Te, Ne, ionDenUp, rec, scale, unit);
 }
 
 /* ---- per-cell state ------------------------------------------------------ */

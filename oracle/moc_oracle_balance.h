@@ -43,6 +43,9 @@ typedef struct oracle_balance_tables {
     const double* atom_e;
     const double* atom_A;
     const float* atom_qom;      /* [nTemp][nLev][nLev] per atom */
+    const int32_t* atom_rec_kind;  /* [nAtoms] 0 none, 1 (br,z,a_r,a_d), 2 log-T cubic */
+    const double* atom_rec;        /* [levels][11] recombination data per level */
+    const int32_t* atom_rec_valid; /* [levels] */
 } oracle_balance_tables;
 
 typedef struct oracle_balance_state {
@@ -74,6 +77,11 @@ extern "C" {
 void oracle_alpha(const oracle_balance_tables* t, double Te, double* out);
 double oracle_nlevel_cooling(const oracle_balance_tables* t, int atom, double Te, double Ne,
                              double scale);
+/* equilibrium (emission_mod.f90:2505-3028): sum of the line emissivities of
+ * one ion times scale*unit that exceed 1e-35; with rec != 0 the upper levels
+ * are fed by recombination from the next stage (ionDenUp = X(i+1)/X(i)). */
+double oracle_nlevel_lines(const oracle_balance_tables* t, int atom, double Te, double Ne,
+                           double ionDenUp, int rec, double scale, double unit);
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,73 @@
+/*
+ * moc_oracle_emission.h -- ORACLE (test infrastructure only): CPU, FP64
+ * restatement of MOCASSIN's per-cell diffuse emission (emission_mod.f90:
+ * emissionDriver :36-252 and the routines it calls, setDiffusePDF
+ * :1528-1919).  Never linked by the product library.
+ */
+#ifndef MOC_ORACLE_EMISSION_H
+#define MOC_ORACLE_EMISSION_H
+
+#include <stdint.h>
+
+#include "moc_oracle_balance.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Raw tables as the reference reads them; the oracle derives the edge
+ * pointers (initGammaCont ph_mod.f90:31-202), the log10 gamma values and the
+ * line-bin pointers itself. */
+typedef struct oracle_emission_tables {
+    const oracle_balance_tables* bal; /* mesh, elements, n-level atoms */
+    const double* widFlx;             /* [B] */
+    int ntk;
+    const double* tk;                 /* [ntk] tkGamma */
+    int nlim[3];                      /* H I, He I, He II */
+    const double* gnu[3];             /* [nlim] nuGamma */
+    const double* gval[3];            /* [ntk][nlim] gamma (linear) */
+    /* ground-state cross sections from their threshold pointer to nbins */
+    int HIPnuP, HeIPnuP, HeIIPnuP;    /* 1-based; He pointers unused when He is off */
+    const double *xsHI, *xsHeI, *xsHeII;
+    /* heavy-ion free-bound, indexed z*32+ion: outer-shell bin range [lo, hi)
+     * (lo = 0: none), offset of its cross sections in fbXs, g0, g1 */
+    const int32_t *fbLo, *fbHi, *fbG0, *fbG1;
+    const int64_t* fbOff;
+    const double* fbXs;
+    const double* hei2phot;           /* [41][2] y, A(y) */
+    const int32_t* hlMap;             /* [9][12] file of (Z, T bin) */
+    const double* hlDens;             /* [files][14] */
+    const double* hlLines;            /* [files][14][16][9] */
+    const double *hbA, *hbB;          /* [9][4] */
+    const double* rbEdge;             /* [2][3][26] */
+    const double* r2aEdge;            /* [3][18] */
+    const double* heiRec;             /* [34][3][4] (first coefficient x1e25) */
+    const double* heiiLyman;          /* [4][2] ratio, nu */
+} oracle_emission_tables;
+
+typedef struct oracle_emission_state {
+    int nCells;
+    const double* Te;       /* [C+1] */
+    const double* Ne;       /* [C+1] */
+    const double* ionDen;   /* [NS][NE][C+1] */
+    double* recPDF;         /* [C+1][B] out (row 0 untouched) */
+    double* totalLines;     /* [C+1] out */
+    double* normCont;       /* [C+1] out, may be NULL: HI+HeI+HeII continuum */
+    double* normRec;        /* [C+1] out, may be NULL: recombination lines */
+    double* normFor;        /* [C+1] out, may be NULL: collisional lines */
+} oracle_emission_state;
+
+/* returns 0, or -1 when two gamma edges share a bin (fb_ff stops there) */
+int oracle_emission(const oracle_emission_tables* t, oracle_emission_state* s, int nthreads);
+
+/* the per-bin emission coefficients of one cell [1e-40 erg cm^3/s/Hz per n_H]:
+ * out[0..B) HI (fb+ff+2q + heavy fb), [B..2B) HeI, [2B..3B) HeII, and the
+ * pieces [3B..) gammaHI, gammaHeI, gammaHeII, gammaHeavies, ffCoeff1,
+ * ffCoeff2, twoPhotHI, twoPhotHeI, twoPhotHeII (9 x B) */
+int oracle_emission_coeffs(const oracle_emission_tables* t, double Te, double Ne,
+                           const double* X /* [NS][NE] */, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

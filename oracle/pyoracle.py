@@ -191,7 +191,8 @@ class BalanceTablesC(C.Structure):
         ("diel", _dp), ("nAldro", C.c_int), ("aldro", _dp), ("cion_cf", _dp), ("cion_th", _dp),
         ("nauger", _ip), ("auger", _dp), ("atom_idx", _ip), ("atom_meta", C.POINTER(C.c_int64)),
         ("atom_atp", _dp), ("atom_logt", _dp), ("atom_g", _dp), ("atom_e", _dp), ("atom_A", _dp),
-        ("atom_qom", C.POINTER(C.c_float)),
+        ("atom_qom", C.POINTER(C.c_float)), ("atom_rec_kind", _ip), ("atom_rec", _dp),
+        ("atom_rec_valid", _ip),
     ]
 
 
@@ -225,8 +226,9 @@ def balance_tables(bt, elem_z, abund, nstages, lgCompton):
     for name in ("rrb", "drb_c", "drb_e", "rrec", "rnew", "diel", "aldro", "cion_cf", "cion_th",
                  "auger", "atom_atp", "atom_logt", "atom_g", "atom_e", "atom_A"):
         k[name] = np.ascontiguousarray(d[name], dtype=np.float64)
-    for name in ("nauger", "atom_idx"):
+    for name in ("nauger", "atom_idx", "atom_rec_kind", "atom_rec_valid"):
         k[name] = np.ascontiguousarray(d[name], dtype=np.int32)
+    k["atom_rec"] = np.ascontiguousarray(d["atom_rec"], dtype=np.float64)
     dp = lambda n: k[n].ctypes.data_as(_dp)
     ip = lambda n: k[n].ctypes.data_as(_ip)
     t = BalanceTablesC(
@@ -241,7 +243,8 @@ def balance_tables(bt, elem_z, abund, nstages, lgCompton):
         cion_th=dp("cion_th"), nauger=ip("nauger"), auger=dp("auger"), atom_idx=ip("atom_idx"),
         atom_meta=k["am"].ctypes.data_as(C.POINTER(C.c_int64)), atom_atp=dp("atom_atp"),
         atom_logt=dp("atom_logt"), atom_g=dp("atom_g"), atom_e=dp("atom_e"), atom_A=dp("atom_A"),
-        atom_qom=k["aq"].ctypes.data_as(C.POINTER(C.c_float)))
+        atom_qom=k["aq"].ctypes.data_as(C.POINTER(C.c_float)), atom_rec_kind=ip("atom_rec_kind"),
+        atom_rec=dp("atom_rec"), atom_rec_valid=ip("atom_rec_valid"))
     return t, k
 
 
@@ -291,3 +294,112 @@ def update(bt, elem_z, abund, nstages, lgCompton, J, Te, Ne, Hden, ionDen, nthre
             "ionDen": s["X"].transpose(2, 1, 0).copy(), "lgConverged": s["conv"],
             "lgBlack": s["black"], "nIterT": s["nit"], "noHit": st.noHit,
             "noIonBal": st.noIonBal, "noTeBal": st.noTeBal}
+
+
+class EmissionTablesC(C.Structure):
+    _fields_ = [
+        ("bal", C.POINTER(BalanceTablesC)), ("widFlx", _dp), ("ntk", C.c_int), ("tk", _dp),
+        ("nlim", C.c_int * 3), ("gnu", _dp * 3), ("gval", _dp * 3), ("HIPnuP", C.c_int),
+        ("HeIPnuP", C.c_int), ("HeIIPnuP", C.c_int), ("xsHI", _dp), ("xsHeI", _dp),
+        ("xsHeII", _dp), ("fbLo", _ip), ("fbHi", _ip), ("fbG0", _ip), ("fbG1", _ip),
+        ("fbOff", C.POINTER(C.c_int64)), ("fbXs", _dp), ("hei2phot", _dp), ("hlMap", _ip),
+        ("hlDens", _dp), ("hlLines", _dp), ("hbA", _dp), ("hbB", _dp), ("rbEdge", _dp),
+        ("r2aEdge", _dp), ("heiRec", _dp), ("heiiLyman", _dp),
+    ]
+
+
+class EmissionStateC(C.Structure):
+    _fields_ = [
+        ("nCells", C.c_int), ("Te", _dp), ("Ne", _dp), ("ionDen", _dp), ("recPDF", _dp),
+        ("totalLines", _dp), ("normCont", _dp), ("normRec", _dp), ("normFor", _dp),
+    ]
+
+
+def emission_tables(et, bt, elem_z, abund, nstages):
+    """ctypes view of the oracle's emission tables built from the RAW data of
+    `et` (the oracle derives pointers and logarithms itself)."""
+    t, k = balance_tables(bt, elem_z, abund, nstages, True)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    fbLo, fbHi, g0, g1 = (np.zeros(31 * 32, dtype=np.int32) for _ in range(4))
+    fbOff = np.zeros(31 * 32, dtype=np.int64)
+    for n in range(et.fbZ.size):
+        idx = int(et.fbZ[n]) * 32 + int(et.fbIon[n])
+        fbLo[idx], fbHi[idx] = et.fbLo[n], et.fbHi[n]
+        g0[idx], g1[idx], fbOff[idx] = et.fbG0[n], et.fbG1[n], et.fbOff[n]
+    g1[g1 == 0] = 1
+    k.update(e_wid=f64(et.widFlx), e_tk=f64(et.tk), e_gnu=[f64(a) for a in et.gnu],
+             e_gval=[f64(a) for a in et.gval], e_xsHI=f64(et.xsHI),
+             e_xsHeI=None if et.xsHeI is None else f64(et.xsHeI),
+             e_xsHeII=None if et.xsHeII is None else f64(et.xsHeII),
+             e_fbLo=fbLo, e_fbHi=fbHi, e_g0=g0, e_g1=g1, e_off=fbOff, e_fbXs=f64(et.fbXs),
+             e_he2=f64(et.hei2phot), e_map=i32(et.hlMap), e_dens=f64(et.hlDens),
+             e_lines=f64(et.hlLines), e_hbA=f64(et.hbA), e_hbB=f64(et.hbB),
+             e_rb=f64(et.rbEdge), e_r2a=f64(et.r2aEdge), e_heiRec=f64(et.heiRec),
+             e_lym=f64(et.heiiLyman), e_bal=t)
+    dp = lambda a: a.ctypes.data_as(_dp) if a is not None else None
+    ip = lambda a: a.ctypes.data_as(_ip)
+    e = EmissionTablesC(
+        bal=C.pointer(t), widFlx=dp(k["e_wid"]), ntk=et.tk.size, tk=dp(k["e_tk"]),
+        nlim=(C.c_int * 3)(*[a.size for a in et.gnu]), gnu=(_dp * 3)(*[dp(a) for a in k["e_gnu"]]),
+        gval=(_dp * 3)(*[dp(a) for a in k["e_gval"]]), HIPnuP=et.HIPnuP, HeIPnuP=et.HeIPnuP,
+        HeIIPnuP=et.HeIIPnuP, xsHI=dp(k["e_xsHI"]), xsHeI=dp(k["e_xsHeI"]),
+        xsHeII=dp(k["e_xsHeII"]), fbLo=ip(fbLo), fbHi=ip(fbHi), fbG0=ip(g0), fbG1=ip(g1),
+        fbOff=fbOff.ctypes.data_as(C.POINTER(C.c_int64)), fbXs=dp(k["e_fbXs"]),
+        hei2phot=dp(k["e_he2"]), hlMap=ip(k["e_map"]), hlDens=dp(k["e_dens"]),
+        hlLines=dp(k["e_lines"]), hbA=dp(k["e_hbA"]), hbB=dp(k["e_hbB"]), rbEdge=dp(k["e_rb"]),
+        r2aEdge=dp(k["e_r2a"]), heiRec=dp(k["e_heiRec"]), heiiLyman=dp(k["e_lym"]))
+    return e, k
+
+
+def emission(et, bt, elem_z, abund, nstages, Te, Ne, ionDen, nthreads=1):
+    """Oracle emissionDriver over all cells.  ionDen is (C+1, nElemUsed,
+    nstages).  Returns recPDF (C+1, B), totalLines and the per-cell sums."""
+    L = lib()
+    L.oracle_emission.argtypes = [C.POINTER(EmissionTablesC), C.POINTER(EmissionStateC), C.c_int]
+    L.oracle_emission.restype = C.c_int
+    t, k = emission_tables(et, bt, elem_z, abund, nstages)
+    C1 = len(Te)
+    B = et.nu.size
+    s = {"Te": np.ascontiguousarray(Te, dtype=np.float64),
+         "Ne": np.ascontiguousarray(Ne, dtype=np.float64),
+         "X": np.ascontiguousarray(np.asarray(ionDen, dtype=np.float64).transpose(2, 1, 0)),
+         "pdf": np.zeros((C1, B)), "tl": np.zeros(C1), "nc": np.zeros(C1), "nr": np.zeros(C1),
+         "nf": np.zeros(C1)}
+    dp = lambda n: s[n].ctypes.data_as(_dp)
+    st = EmissionStateC(nCells=C1 - 1, Te=dp("Te"), Ne=dp("Ne"), ionDen=dp("X"), recPDF=dp("pdf"),
+                        totalLines=dp("tl"), normCont=dp("nc"), normRec=dp("nr"), normFor=dp("nf"))
+    if L.oracle_emission(C.byref(t), C.byref(st), int(nthreads)) != 0:
+        raise ValueError("frequency grid too coarse for the free-bound edges")
+    return {"recPDF": s["pdf"], "totalLines": s["tl"], "normCont": s["nc"], "normRec": s["nr"],
+            "normFor": s["nf"]}
+
+
+EMISSION_PARTS = ["eHI", "eHeI", "eHeII", "gammaHI", "gammaHeI", "gammaHeII", "gammaHeavies",
+                  "ffCoeff1", "ffCoeff2", "twoPhotHI", "twoPhotHeI", "twoPhotHeII"]
+
+
+def emission_coeffs(et, bt, elem_z, abund, nstages, Te, Ne, X):
+    """Per-bin emission coefficients of one cell (X is (nElemUsed, nstages))."""
+    L = lib()
+    L.oracle_emission_coeffs.argtypes = [C.POINTER(EmissionTablesC), C.c_double, C.c_double, _dp,
+                                         _dp]
+    L.oracle_emission_coeffs.restype = C.c_int
+    t, k = emission_tables(et, bt, elem_z, abund, nstages)
+    x = np.ascontiguousarray(np.asarray(X, dtype=np.float64).T)
+    B = et.nu.size
+    out = np.zeros(12 * B)
+    if L.oracle_emission_coeffs(C.byref(t), float(Te), float(Ne), x.ctypes.data_as(_dp),
+                                out.ctypes.data_as(_dp)) != 0:
+        raise ValueError("frequency grid too coarse for the free-bound edges")
+    return {n: out[i * B:(i + 1) * B] for i, n in enumerate(EMISSION_PARTS)}
+
+
+def nlevel_lines(bt, atom, Te, Ne, ionDenUp, rec, scale=1.0, unit=1.9865e9):
+    L = lib()
+    L.oracle_nlevel_lines.argtypes = [C.POINTER(BalanceTablesC), C.c_int, C.c_double, C.c_double,
+                                      C.c_double, C.c_int, C.c_double, C.c_double]
+    L.oracle_nlevel_lines.restype = C.c_double
+    t, k = balance_tables(bt, [1], np.ones(1), 2, True)
+    return L.oracle_nlevel_lines(C.byref(t), int(atom), float(Te), float(Ne), float(ionDenUp),
+                                 int(rec), float(scale), float(unit))
