@@ -1,10 +1,15 @@
 """Benchmark of the MOCASSIN Monte Carlo iteration on one MI355X.
 
-A step is one transport pass of the workload's packets (energyPacketDriver)
-over the grid with inputs resident in HBM.  The timed region brackets K
-steps with device synchronisation; `value` is packets/s of the mixed-precision
-path, the FP64 path is timed beside it, and the oracle (plain C, FP64) is
-timed on a bounded sample of the same workload on the host cores.
+A step is one Monte Carlo iteration of the workload (iterateMC): opacity and
+diffuse emission of every cell, the workload's energy packets through the
+grid, normalisation of J and the ionisation/thermal balance of every cell,
+with all state resident in HBM.  W untimed iterations run first from the
+reference's starting guess, then exactly K iterations are timed between
+device synchronisations.  `value` is packets/s of the whole iteration on the
+mixed-precision path (FP32 transport, FP64 balance); the all-FP64 path runs
+the same iterations beside it, and the difference of the two states after the
+same iterations is reported.  The oracle (plain C, FP64, OpenMP) is timed on
+a bounded sample of the same workload on the host cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload nebula32]
 """
@@ -22,27 +27,30 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+SEED = 20261203
 
 
 def build_model(name: str):
     from mocassin_xray_amd import nebula
     if name == "nebula32":
-        return nebula.frozen_nebula(n=32, nbins=1000, n_packets=10_000_000)
+        return nebula.gas_nebula(n=32, nbins=1000, n_packets=10_000_000)
     if name == "hsphere16":
-        return nebula.frozen_hsphere(n=16, nbins=400, n_packets=1_000_000)
+        return nebula.gas_hsphere(n=16, nbins=400, n_packets=1_000_000)
     raise SystemExit(f"unknown workload {name}")
 
 
-def transport_bytes(st: dict, nbins: int, real_bytes: int, n: int) -> float:
-    """Algorithmic HBM bytes of n packets: per segment kappa + cell id + J RMW;
-    per emission totalLines + CDF bisection; per escape the tally RMW."""
+def transport_bytes(segments: float, emissions: float, escaped: float, nbins: int,
+                    real_bytes: int) -> float:
+    """Algorithmic HBM bytes of a transport launch: per segment kappa + cell
+    id + J read-modify-write; per emission totalLines + the CDF bisection; per
+    escape the tally read-modify-write."""
     seg = real_bytes + 4 + 16
     emit = 4 + real_bytes * math.ceil(math.log2(nbins))
-    return st["segments"] * seg + st["emissions"] * emit + st["escaped"] * 16.0
+    return segments * seg + emissions * emit + escaped * 16.0
 
 
 def profiled_traffic(workload: str):
-    """HBM bytes per launch of k_transport<float> measured by the FETCH_SIZE and
+    """HBM bytes per launch of k_transport<float> from the FETCH_SIZE and
     WRITE_SIZE counter runs recorded in profiles/ (None if not profiled)."""
     f = ROOT / "profiles" / f"transport_{workload}.txt"
     if not f.exists():
@@ -53,26 +61,45 @@ def profiled_traffic(workload: str):
     return None
 
 
-def time_path(m, prec: str, steps: int, warmup: int, n: int):
+def run_path(m, prec: str, steps: int, warmup: int):
     import torch
     from mocassin_xray_amd import MocGPU
-    g = MocGPU(m.tables, prec)
-    g.grid_attach(m.grid)
-    for k in ("opacity", "Ne", "recPDF", "totalLines"):
-        g.upload(k, m.fields[k])
-    for w in range(warmup):
-        g.reset_tallies()
-        g.energyPacketDriver(1, n, seed=1000 + w, ctr_base=w)
-    torch.cuda.synchronize()
-    stats = []
-    t0 = time.perf_counter()
-    for s in range(steps):
-        g.reset_tallies()
-        stats.append(g.energyPacketDriver(1, n, seed=2000 + s, ctr_base=s))
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    g.close()
-    return dt, stats
+    with MocGPU(m.tables, prec) as g:
+        m.load(g)
+        for w in range(1, warmup + 1):
+            g.iterateMC(w, SEED)
+            print(f"bench: {prec} warmup iteration {w} done", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        stats = []
+        t0 = time.perf_counter()
+        for s in range(warmup + 1, warmup + steps + 1):
+            stats.append(g.iterateMC(s, SEED))
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        state = {k: g.download(k) for k in ("Te", "Ne", "ionDen", "Jste", "opacity", "recPDF",
+                                            "totalLines")}
+    return dt, stats, state
+
+
+def state_error(m, a: dict, b: dict) -> dict:
+    """Difference of two model states: volume-averaged T_e and ion fractions
+    (weights n_H dV), and a chi2-like J distance over cells."""
+    import numpy as np
+    from mocassin_xray_amd import tables as T
+    dV = T.cell_volumes(m.grid.xAxis, m.grid.yAxis, m.grid.zAxis, m.grid.active, True)[1:]
+    w = dV * m.state["Hden"][1:]
+    Ta, Tb = a["Te"][1:], b["Te"][1:]
+    te = abs(np.sum(w * Ta) - np.sum(w * Tb)) / np.sum(w * Tb)
+    Xa, Xb = a["ionDen"][1:], b["ionDen"][1:]
+    xa = np.einsum("c,ces->es", w, Xa) / w.sum()
+    xb = np.einsum("c,ces->es", w, Xb) / w.sum()
+    big = xb > 1e-3
+    xrel = float(np.max(np.abs(xa[big] - xb[big]) / xb[big]))
+    Ja, Jb = a["Jste"][1:].sum(axis=1), b["Jste"][1:].sum(axis=1)
+    hit = (Ja + Jb) > 0
+    chi = float(np.mean((Ja[hit] - Jb[hit]) ** 2 / (0.5 * (Ja[hit] + Jb[hit])) ** 2))
+    return {"Te_volavg_rel": float(te), "ion_frac_volavg_max_rel": xrel,
+            "J_cell_rel_rms": math.sqrt(chi)}
 
 
 def copy_calibration(nbytes: int = 1 << 30, reps: int = 3):
@@ -87,27 +114,63 @@ def copy_calibration(nbytes: int = 1 << 30, reps: int = 3):
     print(f"copy_calibration: {reps} copies of {nbytes} bytes", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(m, target_s: float = 12.0):
+def cpu_baseline(m, gs: dict, target_s: float = 20.0):
+    """The oracle's iteration on the host cores, timed on a bounded sample:
+    opacity, emission and balance on a subset of the cells and the transport
+    of a subset of the packets over the full grid (with the device's last
+    fields), each scaled to the whole iteration; reported as packets/s of the
+    whole iteration."""
+    import numpy as np
     sys.path.insert(0, str(ROOT / "oracle"))
-    import pyoracle
+    import pyoracle as O
     threads = min(os.cpu_count() or 1, 16)
+    C1 = m.grid.nCells + 1
+    E, Z, NS = m.elems, m.abund, m.nstages
+    k = max(C1 // 1024, 1)
+    sub = np.r_[0, np.arange(1, C1, k)]
+    scale = (C1 - 1) / (sub.size - 1)
+    part = lambda a: np.ascontiguousarray(a[sub])
+    H = m.state["Hden"]
+    t0 = time.perf_counter()
+    O.opacity(m.xsec, m.tables.KNsigmaT, True, E, Z, NS, part(gs["ionDen"]), part(gs["Ne"]),
+              part(gs["Te"]), part(H))
+    t_op = (time.perf_counter() - t0) * scale
+    t0 = time.perf_counter()
+    O.emission(m.et, m.bt, E, Z, NS, part(gs["Te"]), part(gs["Ne"]), part(gs["ionDen"]),
+               nthreads=threads)
+    t_em = (time.perf_counter() - t0) * scale
+    t0 = time.perf_counter()
+    O.update(m.bt, E, Z, NS, True, part(gs["Jste"]), part(gs["Te"]), part(gs["Ne"]), part(H),
+             part(gs["ionDen"]), nthreads=threads)
+    t_up = (time.perf_counter() - t0) * scale
+    fields = {"opacity": gs["opacity"], "Ne": gs["Ne"], "recPDF": gs["recPDF"],
+              "totalLines": gs["totalLines"]}
     n = 20000
     t0 = time.perf_counter()
-    pyoracle.transport(m.tables, m.grid, m.fields, n, 5, nthreads=threads)
+    O.transport(m.tables, m.grid, fields, n, 5, nthreads=threads)
     dt = time.perf_counter() - t0
     n2 = int(min(max(n * target_s / max(dt, 1e-3), n), 5_000_000))
     t0 = time.perf_counter()
-    pyoracle.transport(m.tables, m.grid, m.fields, n2, 6, nthreads=threads)
-    dt2 = time.perf_counter() - t0
-    return {"value": n2 / dt2, "unit": "packets/s", "cores": threads, "kind": "port",
-            "sample": f"{n2} packets of the same workload, oracle transport (C, FP64, OpenMP)"}
+    O.transport(m.tables, m.grid, fields, n2, 6, nthreads=threads)
+    t_tr = (time.perf_counter() - t0) * m.n_packets / n2
+    t_iter = t_op + t_em + t_tr + t_up
+    return {"value": m.n_packets / t_iter, "unit": "packets/s", "cores": threads, "kind": "port",
+            "sample": (f"one iteration of {m.name} extrapolated from the oracle (C, FP64, OpenMP) "
+                       f"on {sub.size - 1} of {C1 - 1} cells (opacity, emission, balance) and "
+                       f"{n2} of {m.n_packets} packets (transport)"),
+            "phase_s": {"opacity": t_op, "emission": t_em, "transport": t_tr, "balance": t_up},
+            "balance_cells_per_s": (C1 - 1) / t_up}
+
+
+def mean(stats, key):
+    return sum(s[key] for s in stats) / len(stats)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="nebula32")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
@@ -116,8 +179,9 @@ def main():
                     help="run a plain 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the transport shards by packets; "
-              "see the sharding design); run with --gpus 1", file=sys.stderr)
+        print("bench.py: this build measures one GPU only (the transport shards by packets "
+              "and the balance by cells, but no multi-GPU path is built); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     torch.cuda.init()
@@ -126,27 +190,28 @@ def main():
         return
     m = build_model(args.workload)
     n = m.n_packets
-    print(f"bench: workload {m.name} cells={m.grid.nCells} bins={m.tables.nbins} packets={n}",
+    C = m.grid.nCells
+    print(f"bench: workload {m.name} cells={C} bins={m.tables.nbins} packets={n}",
           file=sys.stderr, flush=True)
-    if args.only != "fp64":
-        dt_mix, st_mix = time_path(m, "mixed", args.steps, args.warmup, n)
-        print(f"bench: mixed {args.steps * n / dt_mix:.3e} packets/s", file=sys.stderr,
-              flush=True)
-    if args.only != "mixed":
-        dt_64, st_64 = time_path(m, "fp64", args.steps, args.warmup, n)
-        print(f"bench: fp64  {args.steps * n / dt_64:.3e} packets/s", file=sys.stderr,
-              flush=True)
+    res = {}
+    for prec in ("mixed", "fp64"):
+        if args.only not in (None, prec):
+            continue
+        dt, st, state = run_path(m, prec, args.steps, args.warmup)
+        res[prec] = (dt, st, state)
+        print(f"bench: {prec} {args.steps * n / dt:.3e} packets/s, "
+              f"{dt / args.steps * 1e3:.1f} ms/iteration", file=sys.stderr, flush=True)
     if args.only is not None:
         return
-    value = args.steps * n / dt_mix
-    kms = sum(s["kernel_ms"] for s in st_mix) / len(st_mix)
-    bytes_mix = sum(transport_bytes(s, m.tables.nbins, 4, n) for s in st_mix) / len(st_mix)
+    dt_mix, st_mix, s_mix = res["mixed"]
+    dt_64, st_64, s_64 = res["fp64"]
+    kms = mean(st_mix, "transport_ms")
+    bytes_mix = transport_bytes(mean(st_mix, "segments"), mean(st_mix, "emissions"),
+                                mean(st_mix, "escaped"), m.tables.nbins, 4)
     achieved = bytes_mix / (kms * 1e-3) / 1e9
-    seg_per_pkt = sum(s["segments"] for s in st_mix) / (len(st_mix) * n)
-    emit_per_pkt = sum(s["emissions"] for s in st_mix) / (len(st_mix) * n)
     out = {
-        "metric": "packets/s (transport, mixed precision)",
-        "value": value,
+        "metric": "packets/s of the Monte Carlo iteration (mixed precision) + balance cells/s",
+        "value": args.steps * n / dt_mix,
         "unit": "packets/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -155,22 +220,33 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64acc",
+        "dtype": "f32 transport + f64 balance",
         "data": "synthetic",
-        "config": {"workload": m.name, "cells": m.grid.nCells, "nbins": m.tables.nbins,
-                   "packets": n, "fields": "frozen Stromgren-sphere opacity/emission",
+        "config": {"workload": m.name, "cells": C, "nbins": m.tables.nbins, "packets": n,
+                   "elements": m.elems, "nstages": m.nstages,
+                   "step": "one MC iteration: opacity, emission, transport, balance",
                    "parallelism": "single GPU"},
-        "fp64_packets_per_s": args.steps * n / dt_64,
+        "balance_cells_per_s": C / (mean(st_mix, "update_ms") * 1e-3),
+        "transport_packets_per_s": n / (kms * 1e-3),
+        "phase_ms": {k: mean(st_mix, k + "_ms") for k in ("opacity", "emission", "transport",
+                                                         "update")},
+        "fp64": {"packets_per_s": args.steps * n / dt_64,
+                 "transport_packets_per_s": n / (mean(st_64, "transport_ms") * 1e-3),
+                 "balance_cells_per_s": C / (mean(st_64, "update_ms") * 1e-3),
+                 "phase_ms": {k: mean(st_64, k + "_ms") for k in ("opacity", "emission",
+                                                                  "transport", "update")}},
         "mixed_over_fp64": dt_64 / dt_mix,
-        "segments_per_packet": seg_per_pkt,
-        "emissions_per_packet": emit_per_pkt,
+        "mixed_vs_fp64_error": state_error(m, s_mix, s_64),
+        "converged_percent": st_mix[-1]["convPercent"],
+        "segments_per_packet": mean(st_mix, "segments") / n,
+        "emissions_per_packet": mean(st_mix, "emissions") / n,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profiled_traffic(args.workload),
                      "kernel": "k_transport<float>", "kernel_ms": kms},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(m)
+        out["cpu_baseline"] = cpu_baseline(m, s_mix)
     print(json.dumps(out))
 
 

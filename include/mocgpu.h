@@ -275,6 +275,27 @@ int moc_set_emission(moc_ctx* ctx, const moc_emission_tables* tables);
  * and totalLines.  kernel_ms (may be NULL) gets the kernel time. */
 int moc_emission(moc_ctx* ctx, int ig, float* kernel_ms);
 
+typedef struct moc_iteration_stats {
+    double convPercent;              /* % of active cells with lgConverged */
+    double noHitPercent, noIonBalPercent, noTeBalPercent;
+    double packets;                  /* energy packets transported, all stars */
+    double opacity_ms, emission_ms, transport_ms, update_ms; /* kernel times */
+    double segments, emissions, escaped; /* transport counters, all stars */
+} moc_iteration_stats;
+
+/* Packets of star iStar per iteration (nPhotons(iStar)); the packet energy
+ * is Lstar/nPhotons (photon_mod.f90:86). */
+int moc_set_packets(moc_ctx* ctx, int iStar, int64_t nPhotons);
+
+/* One Monte Carlo iteration on the device (iterateMC, iteration_mod.f90:
+ * 32-1497, without its file output and MPI reductions): opacity and diffuse
+ * emission of every cell -- from iteration 2 on, cells whose Te and Ne moved
+ * by less than 0.5% since the last balance keep theirs -- then the packets
+ * of every star, the normalisation of J and updateCell of every cell.
+ * Needs moc_set_atomic, moc_set_balance and moc_set_emission.  Packet
+ * streams are keyed by (seed, nIterateMC, iStar). */
+int moc_iterate(moc_ctx* ctx, int ig, int nIterateMC, uint64_t seed, moc_iteration_stats* stats);
+
 /* updateCell for every active cell (update_mod.f90:15): from the normalised
  * J (moc_normalize_j) and Te, Ne, Hden, ionDen, writes Te, Ne, ionDen,
  * lgConverged, lgBlack. */

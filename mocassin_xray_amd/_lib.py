@@ -31,7 +31,7 @@ EXPORTS = [
     "moc_create", "moc_destroy", "moc_last_error", "moc_grid_attach", "moc_upload",
     "moc_download", "moc_reset_tallies", "moc_transport", "moc_normalize_j",
     "moc_device_ptr", "moc_version", "moc_set_atomic", "moc_opacity", "moc_set_balance",
-    "moc_update", "moc_set_emission", "moc_emission",
+    "moc_update", "moc_set_emission", "moc_emission", "moc_set_packets", "moc_iterate",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -106,6 +106,16 @@ class MocUpdateStats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class MocIterationStats(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "convPercent", "noHitPercent", "noIonBalPercent", "noTeBalPercent", "packets",
+        "opacity_ms", "emission_ms", "transport_ms", "update_ms", "segments", "emissions",
+        "escaped")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class MocTransportStats(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "Qphot", "segments", "emissions", "absorbed", "scattered", "escaped", "lines",
@@ -161,6 +171,10 @@ def lib() -> C.CDLL:
     L.moc_set_emission.restype = C.c_int
     L.moc_emission.argtypes = [vp, C.c_int, C.POINTER(C.c_float)]
     L.moc_emission.restype = C.c_int
+    L.moc_set_packets.argtypes = [vp, C.c_int, C.c_int64]
+    L.moc_set_packets.restype = C.c_int
+    L.moc_iterate.argtypes = [vp, C.c_int, C.c_int, C.c_uint64, C.POINTER(MocIterationStats)]
+    L.moc_iterate.restype = C.c_int
     L.moc_version.argtypes = []
     L.moc_version.restype = C.c_char_p
     _LIB = L

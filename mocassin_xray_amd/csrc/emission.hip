@@ -302,6 +302,7 @@ __global__ void __launch_bounds__(kThreads) k_emission(MocEmissionArgs a) {
     __shared__ double sWave[kThreads / 64];
     __shared__ double sLines[kThreads / 64];
     const int cell = blockIdx.x + 1;
+    if (moc_unchanged(a.TeOld, a.NeOld, a.Te, a.Ne, cell)) return;  // uniform per workgroup
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int C1 = a.C1, B = a.B;
     const double NeGrid = a.Ne[cell], Te = a.Te[cell];

@@ -85,6 +85,10 @@ struct MocOpacityArgs {
     const double* Te;
     const double* Hden;
     double* opacity;       // [B][C1]
+    // cells whose Te and Ne moved by < 0.5% since the last balance keep their
+    // opacity (iterateMC, iteration_mod.f90:126-153); NULL: every cell
+    const double* TeOld;
+    const double* NeOld;
 };
 
 // Balance kernel view (balance.hip)
@@ -145,6 +149,14 @@ struct MocBalanceArgs {
 
 // Emission kernel view (emission.hip)
 #define MOC_EM_MAXLIM 64   // points of a gamma fit table
+// the skip test of iterateMC for one cell
+__device__ __forceinline__ bool moc_unchanged(const double* TeOld, const double* NeOld,
+                                              const double* Te, const double* Ne, int cell) {
+    if (!TeOld) return false;
+    return fabs(Te[cell] - TeOld[cell]) / TeOld[cell] < 0.005 &&
+           fabs(Ne[cell] - NeOld[cell]) / NeOld[cell] < 0.005;
+}
+
 struct MocEmissionArgs {
     int nCells, C1, B, NE, NS;
     const double* nu;        // [B]
@@ -195,6 +207,8 @@ struct MocEmissionArgs {
     const double* ionDen;    // [NS][NE][C1]
     double* recPDF;          // [C1][B]
     double* totalLines;      // [C1]
+    const double* TeOld;     // skip test as for the opacity (iteration_mod.f90:318-349); NULL: none
+    const double* NeOld;
     MocBalanceArgs bal;      // n-level atoms
 };
 
@@ -218,6 +232,8 @@ struct MocGrid {
     double* d_Ne = nullptr;          // [C+1]
     double* d_Te = nullptr;          // [C+1]
     double* d_Hden = nullptr;        // [C+1]
+    double* d_TeOld = nullptr;       // [C+1] Te, Ne before the last balance
+    double* d_NeOld = nullptr;
     double* d_recPDF = nullptr;      // [C+1][B]
     double* d_totalLines = nullptr;  // [C+1]
     double* d_J = nullptr;           // [B][C+1]
@@ -262,7 +278,7 @@ struct moc_ctx {
     int precision = MOC_PREC_FP64;
     int nbins = 0, nAngles = 180, lgCompton = 1, nStars = 0;
     double ionEdge1 = 0.0;
-    std::vector<double> nuArray, widFlx, Lstar, nPhotons, starPos;
+    std::vector<double> nuArray, widFlx, Lstar, nPhotons, starPos;  // nPhotons: current packets per star
     std::vector<int> starIdx;
     // device tables
     double *d_nu = nullptr, *d_KNsigmaT = nullptr, *d_KNcdf = nullptr, *d_Pcomp = nullptr;

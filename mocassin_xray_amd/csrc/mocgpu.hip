@@ -54,6 +54,7 @@ void free_grid(MocGrid& g) {
     dfree(g.d_vx); dfree(g.d_vy); dfree(g.d_vz);
     dfree(g.d_active);
     dfree(g.d_opacity); dfree(g.d_Ne); dfree(g.d_Te); dfree(g.d_Hden);
+    dfree(g.d_TeOld); dfree(g.d_NeOld);
     dfree(g.d_recPDF); dfree(g.d_totalLines); dfree(g.d_J); dfree(g.d_escaped);
     dfree(g.f_ax); dfree(g.f_ay); dfree(g.f_az);
     dfree(g.f_vx); dfree(g.f_vy); dfree(g.f_vz);
@@ -346,6 +347,7 @@ int moc_grid_attach(moc_ctx* ctx, int ig, const moc_grid_desc* gd) {
         (rc = dalloc_n(ctx, &g.d_opacity, C1 * B, "opacity")) ||
         (rc = dalloc_n(ctx, &g.d_Ne, C1, "Ne")) || (rc = dalloc_n(ctx, &g.d_Te, C1, "Te")) ||
         (rc = dalloc_n(ctx, &g.d_Hden, C1, "Hden")) ||
+        (rc = dalloc_n(ctx, &g.d_TeOld, C1, "TeOld")) || (rc = dalloc_n(ctx, &g.d_NeOld, C1, "NeOld")) ||
         (rc = dalloc_n(ctx, &g.d_recPDF, C1 * B, "recPDF")) ||
         (rc = dalloc_n(ctx, &g.d_totalLines, C1, "totalLines")) ||
         (rc = dalloc_n(ctx, &g.d_J, C1 * B, "J")) ||
@@ -445,6 +447,9 @@ int moc_set_atomic(moc_ctx* ctx, const moc_atomic* at) {
     return ensure_ionden(ctx);
 }
 
+int run_opacity(moc_ctx* ctx, bool skipUnchanged, float* ms);
+int run_emission(moc_ctx* ctx, bool skipUnchanged, float* ms);
+
 int moc_opacity(moc_ctx* ctx, int ig) {
     if (!ctx) return MOC_ERR_ARG;
     MocGrid& g = ctx->grid;
@@ -452,6 +457,11 @@ int moc_opacity(moc_ctx* ctx, int ig) {
         ctx->err = "moc_opacity: needs an attached grid and moc_set_atomic";
         return MOC_ERR_STATE;
     }
+    return run_opacity(ctx, false, nullptr);
+}
+
+int run_opacity(moc_ctx* ctx, bool skipUnchanged, float* ms) {
+    MocGrid& g = ctx->grid;
     MocAtomic& at = ctx->atomic;
     MocOpacityArgs a;
     a.C1 = g.nCells + 1; a.nbins = ctx->nbins; a.nSpecies = at.nSpecies;
@@ -460,9 +470,14 @@ int moc_opacity(moc_ctx* ctx, int ig) {
     a.abund = at.d_abund; a.sigma = at.d_sigma; a.brems = at.d_brems; a.nuArray = ctx->d_nu;
     a.log10nu = at.d_log10nu; a.KNsigmaT = ctx->d_KNsigmaT; a.ionDen = g.d_ionDen;
     a.Ne = g.d_Ne; a.Te = g.d_Te; a.Hden = g.d_Hden; a.opacity = g.d_opacity;
+    a.TeOld = skipUnchanged ? g.d_TeOld : nullptr;
+    a.NeOld = skipUnchanged ? g.d_NeOld : nullptr;
+    MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
     int rc = moc_launch_opacity(ctx, a);
     if (rc) return rc;
-    MOC_TRY(hipStreamSynchronize(ctx->stream), "opacity");
+    MOC_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
+    MOC_TRY(hipEventSynchronize(ctx->ev1), "opacity");
+    if (ms) (void)hipEventElapsedTime(ms, ctx->ev0, ctx->ev1);
     g.f32_dirty = true;
     return MOC_OK;
 }
@@ -767,6 +782,10 @@ int moc_update(moc_ctx* ctx, int ig, moc_update_stats* stats) {
     a.nIterT = g.d_nIterT;
     a.counters = b.d_counters;
     MOC_TRY(hipMemsetAsync(b.d_counters, 0, 4 * sizeof(unsigned long long), ctx->stream), "counters");
+    // TeOld, NeOld = Te, Ne before the balance (iteration_mod.f90:1139-1142)
+    const size_t C1b = ((size_t)g.nCells + 1) * sizeof(double);
+    MOC_TRY(hipMemcpyAsync(g.d_TeOld, g.d_Te, C1b, hipMemcpyDeviceToDevice, ctx->stream), "TeOld");
+    MOC_TRY(hipMemcpyAsync(g.d_NeOld, g.d_Ne, C1b, hipMemcpyDeviceToDevice, ctx->stream), "NeOld");
     MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
     if ((rc = moc_launch_balance(ctx, a, g.d_J, W, b.ncol, g.d_rates))) return rc;
     MOC_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
@@ -927,6 +946,11 @@ int moc_emission(moc_ctx* ctx, int ig, float* kernel_ms) {
         ctx->err = "moc_emission: needs an attached grid, moc_set_balance and moc_set_emission";
         return MOC_ERR_STATE;
     }
+    return run_emission(ctx, false, kernel_ms);
+}
+
+int run_emission(moc_ctx* ctx, bool skipUnchanged, float* kernel_ms) {
+    MocGrid& g = ctx->grid;
     MocEmissionArgs a = ctx->emission.a;
     a.bal = ctx->balance.a;
     a.nCells = g.nCells;
@@ -936,6 +960,8 @@ int moc_emission(moc_ctx* ctx, int ig, float* kernel_ms) {
     a.ionDen = g.d_ionDen;
     a.recPDF = g.d_recPDF;
     a.totalLines = g.d_totalLines;
+    a.TeOld = skipUnchanged ? g.d_TeOld : nullptr;
+    a.NeOld = skipUnchanged ? g.d_NeOld : nullptr;
     MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
     int rc = moc_launch_emission(ctx, a);
     if (rc) return rc;
@@ -945,6 +971,60 @@ int moc_emission(moc_ctx* ctx, int ig, float* kernel_ms) {
     (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     if (kernel_ms) *kernel_ms = ms;
     g.f32_dirty = true;
+    return MOC_OK;
+}
+
+int moc_set_packets(moc_ctx* ctx, int iStar, int64_t n) {
+    if (!ctx) return MOC_ERR_ARG;
+    if (iStar < 1 || iStar > ctx->nStars || n < 1) { ctx->err = "moc_set_packets: iStar/n"; return MOC_ERR_ARG; }
+    ctx->nPhotons[iStar - 1] = (double)n;
+    return MOC_OK;
+}
+
+int moc_iterate(moc_ctx* ctx, int ig, int nIterateMC, uint64_t seed, moc_iteration_stats* stats) {
+    if (!ctx) return MOC_ERR_ARG;
+    MocGrid& g = ctx->grid;
+    if (!g.attached || ig != 1 || !ctx->emission.set || !ctx->balance.set || !g.d_ionDen) {
+        ctx->err = "moc_iterate: needs an attached grid, moc_set_atomic, moc_set_balance and moc_set_emission";
+        return MOC_ERR_STATE;
+    }
+    if (nIterateMC < 1 || nIterateMC > (1 << 23)) { ctx->err = "moc_iterate: nIterateMC"; return MOC_ERR_ARG; }
+    const bool skip = nIterateMC > 1;
+    float tOp = 0.f, tEm = 0.f;
+    int rc;
+    if ((rc = run_opacity(ctx, skip, &tOp))) return rc;
+    if ((rc = run_emission(ctx, skip, &tEm))) return rc;
+    if ((rc = moc_reset_tallies(ctx, ig))) return rc;
+    double packets = 0.0, tTr = 0.0, seg = 0.0, emi = 0.0, esc = 0.0;
+    for (int s = 1; s <= ctx->nStars; ++s) {
+        const int64_t n = (int64_t)ctx->nPhotons[s - 1];
+        moc_transport_stats ts{};
+        const uint32_t ctr = (uint32_t)nIterateMC * 256u + (uint32_t)s;
+        if ((rc = moc_transport(ctx, s, n, seed, ctr, &ts))) return rc;
+        packets += (double)n;
+        tTr += ts.kernel_ms;
+        seg += ts.segments;
+        emi += ts.emissions;
+        esc += ts.escaped;
+    }
+    if ((rc = moc_normalize_j(ctx, ig))) return rc;
+    moc_update_stats us{};
+    if ((rc = moc_update(ctx, ig, &us))) return rc;
+    if (stats) {
+        const double nc = g.nCells > 0 ? (double)g.nCells : 1.0;
+        stats->convPercent = 100.0 * us.converged / nc;
+        stats->noHitPercent = 100.0 * us.noHit / nc;
+        stats->noIonBalPercent = 100.0 * us.noIonBal / nc;
+        stats->noTeBalPercent = 100.0 * us.noTeBal / nc;
+        stats->packets = packets;
+        stats->opacity_ms = tOp;
+        stats->emission_ms = tEm;
+        stats->transport_ms = tTr;
+        stats->update_ms = us.kernel_ms;
+        stats->segments = seg;
+        stats->emissions = emi;
+        stats->escaped = esc;
+    }
     return MOC_OK;
 }
 

@@ -69,6 +69,7 @@ __global__ void __launch_bounds__(256) k_opacity(MocOpacityArgs a) {
         }
         return;
     }
+    if (moc_unchanged(a.TeOld, a.NeOld, a.Te, a.Ne, cell)) return;
     // free-free prefactor (addOpacity :367-368) with density(n,i), i <= min(n, nstages):
     // the bare nuclei never enter, so only He+ and the heavy ions count
     const double Te = a.Te[cell], Ne = a.Ne[cell], H = a.Hden[cell];

@@ -15,6 +15,7 @@ prints and stops).
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -22,7 +23,7 @@ import numpy as np
 from . import _lib
 from ._lib import (CELLBIN_FIELDS, FIELDS, MOC_DT_F64, MOC_DT_I32, MOC_LAYOUT_FORTRAN,
                    MOC_PREC_FP64, MOC_PREC_MIXED, MocAtomic, MocBalanceTables, MocEmissionTables,
-                   MocGridDesc, MocTables, MocTransportStats, MocUpdateStats, _dp, _ip, dptr,
+                   MocGridDesc, MocIterationStats, MocTables, MocTransportStats, MocUpdateStats, _dp, _ip, dptr,
                    iptr)
 
 
@@ -71,7 +72,8 @@ def _c(a, dtype=np.float64):
 class MocGPU:
     def __init__(self, tables: SourceTables, precision: str = "fp64"):
         self._L = _lib.lib()
-        self.tables = tables
+        # own copy: set_packets changes nPhotons
+        self.tables = dataclasses.replace(tables, nPhotons=np.array(tables.nPhotons, dtype=np.float64))
         self.precision = precision
         prec = {"fp64": MOC_PREC_FP64, "mixed": MOC_PREC_MIXED}[precision]
         B = tables.nbins
@@ -293,6 +295,49 @@ class MocGPU:
 
     def normalize_j(self):
         self._check(self._L.moc_normalize_j(self._ctx, 1), "moc_normalize_j")
+
+    # -------------------------------------------------------------- iteration
+    def set_packets(self, iStar: int, n: int) -> None:
+        """nPhotons(iStar) for the following iterations (packet energy L/n)."""
+        self._check(self._L.moc_set_packets(self._ctx, int(iStar), int(n)), "moc_set_packets")
+        self.tables.nPhotons[int(iStar) - 1] = int(n)
+
+    def iterateMC(self, nIterateMC: int, seed: int = 20261203) -> dict:
+        """One Monte Carlo iteration on the device (iteration_mod.f90:32):
+        opacity, emission, packets of every star, J normalisation, updateCell."""
+        st = MocIterationStats()
+        self._check(self._L.moc_iterate(self._ctx, 1, int(nIterateMC), int(seed) & (2**64 - 1),
+                                        C.byref(st)), "moc_iterate")
+        return st.as_dict()
+
+    def MCIterationDriver(self, maxIterateMC: int, minConvergence: float = 100.0,
+                          seed: int = 20261203, lgAutoPackets: bool = False,
+                          convIncPercent: float = 0.0, nPhotIncrease: float = 2.0,
+                          maxPhotons: float = 0.0, callback=None) -> list:
+        """The iteration loop of MCIterationDriver (iteration_mod.f90:17-1497):
+        iterate until the converged-cell percentage reaches minConvergence or
+        maxIterateMC iterations have run; with lgAutoPackets the packets of
+        every star grow by nPhotIncrease when convergence improved by no more
+        than convIncPercent (:1412-1421).  Returns the per-iteration stats."""
+        history = []
+        old = 0.0
+        n = 1
+        while True:
+            st = self.iterateMC(n, seed)
+            st["nIterateMC"] = n
+            history.append(st)
+            if callback is not None:
+                callback(st)
+            tot = st["convPercent"]
+            n_tot = sum(self.tables.nPhotons) + self.tables.nPhotons[0]
+            if (lgAutoPackets and n > 1 and tot < 95.0 and n_tot < maxPhotons and old > 0.0
+                    and (tot - old) / old <= convIncPercent):
+                for s, np_ in enumerate(list(self.tables.nPhotons)):
+                    self.set_packets(s + 1, int(np_ * nPhotIncrease))
+            old = tot
+            if tot >= minConvergence or n >= maxIterateMC:
+                return history
+            n += 1
 
 
 def version() -> str:

@@ -123,3 +123,94 @@ def frozen_nebula(n: int = 32, nbins: int = 1000, n_packets: int = 10_000_000,
     f = frozen_fields(grid, nu, hcell, R_s, kn_sigma=tabs.KNsigmaT)
     f["Hden"] = hcell
     return FrozenModel("nebula%d" % n, tabs, grid, f, n_packets)
+
+
+# ------------------------------------------------------------- full gas models
+@dataclass
+class GasModel:
+    """A gas model ready for the Monte Carlo iteration: grid, mesh/source
+    tables, atomic tables (cross-section stack, balance, emission) and the
+    starting per-cell state."""
+    name: str
+    tables: SourceTables
+    grid: CartesianGrid
+    elems: list
+    abund: np.ndarray
+    nstages: int
+    xsec: object
+    bt: object
+    et: object
+    state: dict            # Te, Ne, Hden (C+1,), ionDen (C+1, NE, NS)
+    n_packets: int
+
+    def load(self, g) -> None:
+        """Upload everything into a MocGPU context."""
+        g.grid_attach(self.grid)
+        g.set_atomic(self.elems, self.abund, self.nstages, self.xsec)
+        g.set_balance(self.bt)
+        g.set_emission(self.et)
+        for k in ("ionDen", "Te", "Ne", "Hden"):
+            g.upload(k, self.state[k])
+
+
+def initial_state(elems: list, nstages: int, hden_cells: np.ndarray, Te: float = 1.e4,
+                  H0in: float = 1.e-5) -> dict:
+    """Starting guess of setGrid (grid_mod.f90:1583-1645): X(H0) = H0in, He
+    split evenly between He+ and He++, heavy elements neutral at H0in and the
+    rest spread evenly over the ionised stages, Te = TeStart, Ne = n_H."""
+    C1 = hden_cells.size
+    X = np.zeros((C1, len(elems), nstages))
+    for e, z in enumerate(elems):
+        if z == 1:
+            X[:, e, 0], X[:, e, 1] = H0in, 1.0 - H0in
+        elif z == 2:
+            X[:, e, 0] = H0in
+            X[:, e, 1] = X[:, e, 2] = (1.0 - H0in) / 2.0
+        else:
+            top = min(z + 1, nstages)
+            X[:, e, 0] = H0in
+            X[:, e, 1:top] = (1.0 - H0in) / min(z, nstages - 1)
+    X[0] = 0.0
+    Te_a = np.full(C1, Te)
+    Te_a[0] = 0.0
+    return {"Te": Te_a, "Ne": hden_cells.copy(), "Hden": hden_cells.copy(), "ionDen": X}
+
+
+def gas_model(name: str, n: int, R: float, hden: float, elems: list, abund: np.ndarray,
+              nstages: int, nbins: int, nu_max: float, T_star: float, L36: float,
+              n_packets: int, maxLev: int = 5, lgCompton: bool = True) -> GasModel:
+    from . import atomic, balance, emission
+    ax = T.symmetric_axes(n, R)
+    act, ncells, hcell = T.active_map(ax, ax, ax, np.full((n, n, n), hden))
+    grid = CartesianGrid(ax, ax.copy(), ax.copy(), act, ncells, True, 0.0)
+    nu = T.reference_gas_mesh(nbins, 1.0, nu_max, atomic.ion_edges(elems, nstages, nu_max))
+    tabs = source_tables(nu, T_star, L36, n_packets, True, n, n, n, lgCompton)
+    xs = atomic.build_xsec(nu, elems, nstages)
+    bt = balance.build_balance(xs, maxLev=maxLev)
+    et = emission.build_emission(xs, tabs.widFlx)
+    st = initial_state(elems, nstages, hcell)
+    return GasModel(name, tabs, grid, list(elems), np.asarray(abund, dtype=np.float64), nstages,
+                    xs, bt, et, st, n_packets)
+
+
+def gas_hsphere(n: int = 16, nbins: int = 400, n_packets: int = 1_000_000,
+                lgCompton: bool = True) -> GasModel:
+    """Pure-hydrogen homogeneous sphere, one octant: 16^3, edge 3e18 cm,
+    n_H = 100, 40 kK blackbody of 1e37 erg/s, 400 points over 1-25 Ryd."""
+    return gas_model("hsphere%d" % n, n, 3.e18, 100., [1], np.ones(1), 2, nbins, 25.0, 4.e4,
+                     10.0, n_packets, lgCompton=lgCompton)
+
+
+NEBULA_ELEMS = [1, 2, 6, 7, 8, 10, 16]
+NEBULA_LOG_ABUND = [0.0, -1.0, -3.6, -4.1, -3.3, -4.1, -5.0]
+
+
+def gas_nebula(n: int = 32, nbins: int = 1000, n_packets: int = 10_000_000,
+               nstages: int = 7) -> GasModel:
+    """Multi-element nebula, one octant: 32^3, edge 1e18 cm, n_H = 1000,
+    H He C N O Ne S at log abundances 0 -1 -3.6 -4.1 -3.3 -4.1 -5, 100 kK
+    blackbody of 10^37.5 erg/s, 1000 points over 1-100 Ryd, 5-level
+    forbidden-line cooling."""
+    return gas_model("nebula%d" % n, n, 1.e18, 1000., NEBULA_ELEMS,
+                     10.0 ** np.array(NEBULA_LOG_ABUND), nstages, nbins, 100.0, 1.e5,
+                     10 ** 1.5, n_packets, maxLev=5)
