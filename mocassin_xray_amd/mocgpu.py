@@ -23,8 +23,7 @@ import numpy as np
 from . import _lib
 from ._lib import (CELLBIN_FIELDS, FIELDS, MOC_DT_F64, MOC_DT_I32, MOC_LAYOUT_FORTRAN,
                    MOC_PREC_FP64, MOC_PREC_MIXED, MocAtomic, MocBalanceTables, MocEmissionTables,
-                   MocGridDesc, MocIterationStats, MocTables, MocTransportStats, MocUpdateStats, _dp, _ip, dptr,
-                   iptr)
+                   MocGridDesc, MocIterationStats, MocTables, MocTransportStats, MocUpdateStats)
 
 
 class MocError(RuntimeError):
@@ -69,26 +68,117 @@ def _c(a, dtype=np.float64):
     return np.ascontiguousarray(a, dtype=dtype)
 
 
+# ------------------------------------------------------------------------
+# The contents of every C struct of the ABI as a dict of field name ->
+# numpy array (pointer fields) or Python scalar, in the struct's own order.
+# MocGPU builds the ctypes structs from these; fortran_io writes the same
+# dicts for the Fortran driver.
+
+def tables_arrays(tables: SourceTables, precision: str) -> dict:
+    B = tables.nbins
+    kn = _c(tables.KNcdf)
+    return {
+        "nbins": B, "precision": {"fp64": MOC_PREC_FP64, "mixed": MOC_PREC_MIXED}[precision],
+        "nuArray": _c(tables.nuArray), "widFlx": _c(tables.widFlx),
+        "ionEdge1": float(tables.ionEdge1), "lgCompton": int(bool(tables.lgCompton)),
+        "nAngles": kn.size // B, "KNsigmaT": _c(tables.KNsigmaT), "KNsigmaCDF": kn,
+        "Pcomp": _c(tables.Pcomp), "nStars": int(np.atleast_1d(tables.Lstar).size),
+        "sourceCDF": _c(tables.sourceCDF), "Lstar": _c(tables.Lstar),
+        "nPhotons": _c(tables.nPhotons), "starPosition": _c(tables.starPosition),
+        "starIndex": _c(tables.starIndex, np.int32),
+    }
+
+
+def grid_arrays(g: CartesianGrid) -> dict:
+    act = np.ascontiguousarray(np.asarray(g.active, dtype=np.int32).reshape(-1, order="F"))
+    xa, ya, za = _c(g.xAxis), _c(g.yAxis), _c(g.zAxis)
+    return {"nx": xa.size, "ny": ya.size, "nz": za.size, "xAxis": xa, "yAxis": ya, "zAxis": za,
+            "active": act, "nCells": int(g.nCells), "lgSymmetricXYZ": int(bool(g.lgSymmetricXYZ)),
+            "R_out": float(g.R_out)}
+
+
+def atomic_arrays(elem_z, abund, nstages: int, xsec) -> dict:
+    z = _c(elem_z, np.int32)
+    zi = {int(v): i for i, v in enumerate(z)}
+    spec_e = _c([zi[e] for e, _ in xsec.species], np.int32)
+    spec_i = _c([i for _, i in xsec.species], np.int32)
+    return {"nElemUsed": z.size, "elemZ": z, "abund": _c(abund), "nstages": int(nstages),
+            "nSpecies": spec_e.size, "speciesElem": spec_e, "speciesIon": spec_i,
+            "sigma": _c(xsec.sigma), "brems": _c(xsec.brems)}
+
+
+def balance_arrays(bt) -> dict:
+    d = bt.data
+    diel, aldro, chex = _c(d["diel"]), _c(d["aldro"]), bt.chex
+    am = _c(bt.atomMeta8, np.int64)
+    ag, aA, aq = _c(d["atom_g"]), _c(d["atom_A"]), _c(bt.atomQom)
+    lt = _c(d["atom_logt"])
+    return {
+        "maxLev": int(bt.maxLev), "XHILimit": float(bt.XHILimit),
+        "nShells": int(np.asarray(bt.shellNum).size), "shellNum": _c(bt.shellNum, np.int32),
+        "speciesShell": _c(bt.speciesShell, np.int32), "W": _c(bt.W),
+        "outShell": _c(bt.outShell, np.int32), "statW0": _c(bt.statW0, np.int32),
+        "statW1": _c(bt.statW1, np.int32), "rrb": _c(d["rrb"]), "drb_c": _c(d["drb_c"]),
+        "drb_e": _c(d["drb_e"]), "rrec": _c(d["rrec"]), "rnew": _c(d["rnew"]),
+        "nDiel": diel.shape[0], "diel": diel, "nAldro": aldro.shape[0], "aldro": aldro,
+        "cionCF": _c(d["cion_cf"]), "cionTherm": _c(d["cion_th"]),
+        "nAuger": _c(d["nauger"], np.int32), "auger": _c(d["auger"]),
+        "nChex": int(chex.shape[0]), "chexZ": _c(chex[:, 0], np.int32),
+        "chexIon": _c(chex[:, 1], np.int32), "chexFit": _c(chex[:, 2:6]),
+        "nAtoms": am.shape[0], "atomIdx": _c(d["atom_idx"], np.int32), "atomMeta": am,
+        "atomAtp": _c(d["atom_atp"]), "nLogT": lt.size, "nLev": ag.size, "nA": aA.size,
+        "nQ": aq.size, "atomLogT": lt, "atomG": ag, "atomE": _c(d["atom_e"]), "atomA": aA,
+        "atomQom": aq, "atomQ2": _c(bt.atomQ2), "atomRecKind": _c(d["atom_rec_kind"], np.int32),
+        "atomRec": _c(d["atom_rec"]), "atomRecValid": _c(d["atom_rec_valid"], np.int32),
+    }
+
+
+def emission_arrays(et) -> dict:
+    fbXs = _c(et.fbXs)
+    return {
+        "ntk": et.tk.size, "tkGamma": _c(et.tk), "nlim": [int(e.size) for e in et.edge],
+        "edgeP": [_c(e, np.int32) for e in et.edge], "logGamma": [_c(v) for v in et.loggam],
+        "HIPnuP": int(et.HIPnuP), "HeIPnuP": int(et.HeIPnuP), "HeIIPnuP": int(et.HeIIPnuP),
+        "xsHI": _c(et.xsHI), "xsHeI": None if et.xsHeI is None else _c(et.xsHeI),
+        "xsHeII": None if et.xsHeII is None else _c(et.xsHeII), "nFb": int(et.fbZ.size),
+        "fbZ": _c(et.fbZ, np.int32), "fbIon": _c(et.fbIon, np.int32),
+        "fbLo": _c(et.fbLo, np.int32), "fbHi": _c(et.fbHi, np.int32),
+        "fbOff": _c(et.fbOff, np.int64), "nFbXs": fbXs.size, "fbXs": fbXs,
+        "fbGratio": _c(et.fbGratio), "he2Row": _c(et.he2j, np.int32),
+        "hei2phot": _c(et.hei2phot), "nHlFiles": int(et.hlDens.shape[0]),
+        "hlMap": _c(et.hlMap, np.int32), "hlDens": _c(et.hlDens), "hlLines": _c(et.hlLines),
+        "hbA": _c(et.hbA), "hbB": _c(et.hbB), "rbEdge": _c(et.rbEdge), "r2aEdge": _c(et.r2aEdge),
+        "heiRec": _c(et.heiRec), "heiiLyman": _c(et.heiiLyman), "j2TsP": int(et.j2TsP),
+        "lymP": [int(v) for v in et.lymP],
+    }
+
+
+def to_struct(cls, d: dict):
+    """The ctypes struct `cls` filled from a field dict (arrays stay owned by d)."""
+    kw = {}
+    for name, ct in cls._fields_:
+        v = d[name]
+        if isinstance(ct, type) and issubclass(ct, C._Pointer):
+            kw[name] = None if v is None else v.ctypes.data_as(ct)
+        elif isinstance(ct, type) and issubclass(ct, C.Array):
+            el = ct._type_
+            if issubclass(el, C._Pointer):
+                kw[name] = ct(*[None if a is None else a.ctypes.data_as(el) for a in v])
+            else:
+                kw[name] = ct(*v)
+        else:
+            kw[name] = v
+    return cls(**kw)
+
+
 class MocGPU:
     def __init__(self, tables: SourceTables, precision: str = "fp64"):
         self._L = _lib.lib()
         # own copy: set_packets changes nPhotons
         self.tables = dataclasses.replace(tables, nPhotons=np.array(tables.nPhotons, dtype=np.float64))
         self.precision = precision
-        prec = {"fp64": MOC_PREC_FP64, "mixed": MOC_PREC_MIXED}[precision]
-        B = tables.nbins
-        self._keep = [_c(tables.nuArray), _c(tables.widFlx), _c(tables.KNsigmaT),
-                      _c(tables.KNcdf), _c(tables.Pcomp), _c(tables.sourceCDF),
-                      _c(tables.Lstar), _c(tables.nPhotons), _c(tables.starPosition),
-                      _c(tables.starIndex, np.int32)]
-        k = self._keep
-        nst = int(np.atleast_1d(tables.Lstar).size)
-        assert k[5].size == nst * B and k[3].size == k[4].size and k[3].size % B == 0
-        t = MocTables(nbins=B, precision=prec, nuArray=dptr(k[0]), widFlx=dptr(k[1]),
-                      ionEdge1=float(tables.ionEdge1), lgCompton=int(bool(tables.lgCompton)),
-                      nAngles=k[3].size // B, KNsigmaT=dptr(k[2]), KNsigmaCDF=dptr(k[3]),
-                      Pcomp=dptr(k[4]), nStars=nst, sourceCDF=dptr(k[5]), Lstar=dptr(k[6]),
-                      nPhotons=dptr(k[7]), starPosition=dptr(k[8]), starIndex=iptr(k[9]))
+        self._keep = tables_arrays(self.tables, precision)
+        t = to_struct(MocTables, self._keep)
         self._ctx = C.c_void_p()
         rc = self._L.moc_create(C.byref(self._ctx), C.byref(t))
         if rc != 0:
@@ -126,12 +216,8 @@ class MocGPU:
 
     # ------------------------------------------------------------------- grid
     def grid_attach(self, g: CartesianGrid):
-        act = np.asfortranarray(g.active, dtype=np.int32).reshape(-1, order="F")
-        act = np.ascontiguousarray(act)
-        xa, ya, za = _c(g.xAxis), _c(g.yAxis), _c(g.zAxis)
-        d = MocGridDesc(nx=xa.size, ny=ya.size, nz=za.size, xAxis=dptr(xa), yAxis=dptr(ya),
-                        zAxis=dptr(za), active=iptr(act), nCells=int(g.nCells),
-                        lgSymmetricXYZ=int(bool(g.lgSymmetricXYZ)), R_out=float(g.R_out))
+        self._grid_keep = grid_arrays(g)
+        d = to_struct(MocGridDesc, self._grid_keep)
         self._check(self._L.moc_grid_attach(self._ctx, 1, C.byref(d)), "moc_grid_attach")
         self.grid = g
 
@@ -145,17 +231,10 @@ class MocGPU:
     def set_atomic(self, elem_z, abund, nstages: int, xsec) -> None:
         """Element set, abundances and the cross-section stack (an
         atomic.XSecStack built on the same mesh)."""
-        z = _c(elem_z, np.int32)
-        zi = {int(v): i for i, v in enumerate(z)}
-        spec_e = _c([zi[e] for e, _ in xsec.species], np.int32)
-        spec_i = _c([i for _, i in xsec.species], np.int32)
-        self._atomic_keep = [z, _c(abund), spec_e, spec_i, _c(xsec.sigma), _c(xsec.brems)]
-        k = self._atomic_keep
-        a = MocAtomic(nElemUsed=z.size, elemZ=iptr(k[0]), abund=dptr(k[1]), nstages=int(nstages),
-                      nSpecies=spec_e.size, speciesElem=iptr(k[2]), speciesIon=iptr(k[3]),
-                      sigma=dptr(k[4]), brems=dptr(k[5]))
+        self._atomic_keep = atomic_arrays(elem_z, abund, nstages, xsec)
+        a = to_struct(MocAtomic, self._atomic_keep)
         self._check(self._L.moc_set_atomic(self._ctx, C.byref(a)), "moc_set_atomic")
-        self.n_elem, self.nstages = int(z.size), int(nstages)
+        self.n_elem, self.nstages = int(self._atomic_keep["nElemUsed"]), int(nstages)
 
     def ionizationDriver(self):
         """Opacity of every cell from ionDen, Ne, Te, Hden (ionization_mod.f90:26)."""
@@ -163,70 +242,15 @@ class MocGPU:
 
     def set_balance(self, bt) -> None:
         """Balance tables (a balance.BalanceTables built on the same mesh)."""
-        d = bt.data
-        lp = C.POINTER(C.c_int64)
-        k = {
-            "sn": _c(bt.shellNum, np.int32), "ss": _c(bt.speciesShell, np.int32), "W": _c(bt.W),
-            "os": _c(bt.outShell, np.int32), "g0": _c(bt.statW0, np.int32),
-            "g1": _c(bt.statW1, np.int32), "rrb": _c(d["rrb"]), "dc": _c(d["drb_c"]),
-            "de": _c(d["drb_e"]), "rrec": _c(d["rrec"]), "rnew": _c(d["rnew"]),
-            "diel": _c(d["diel"]), "aldro": _c(d["aldro"]), "cf": _c(d["cion_cf"]),
-            "th": _c(d["cion_th"]), "na": _c(d["nauger"], np.int32), "au": _c(d["auger"]),
-            "cz": _c(bt.chex[:, 0], np.int32), "ci": _c(bt.chex[:, 1], np.int32),
-            "cfit": _c(bt.chex[:, 2:6]), "ai": _c(d["atom_idx"], np.int32),
-            "am": _c(bt.atomMeta8, np.int64), "atp": _c(d["atom_atp"]), "lt": _c(d["atom_logt"]),
-            "ag": _c(d["atom_g"]), "ae": _c(d["atom_e"]), "aA": _c(d["atom_A"]),
-            "aq": _c(bt.atomQom), "aq2": _c(bt.atomQ2), "rk": _c(d["atom_rec_kind"], np.int32),
-            "rt": _c(d["atom_rec"]), "rv": _c(d["atom_rec_valid"], np.int32),
-        }
-        self._balance_keep = k
-        t = MocBalanceTables(
-            maxLev=int(bt.maxLev), XHILimit=float(bt.XHILimit), nShells=k["sn"].size,
-            shellNum=iptr(k["sn"]), speciesShell=iptr(k["ss"]), W=dptr(k["W"]),
-            outShell=iptr(k["os"]), statW0=iptr(k["g0"]), statW1=iptr(k["g1"]),
-            rrb=dptr(k["rrb"]), drb_c=dptr(k["dc"]), drb_e=dptr(k["de"]), rrec=dptr(k["rrec"]),
-            rnew=dptr(k["rnew"]), nDiel=k["diel"].shape[0], diel=dptr(k["diel"]),
-            nAldro=k["aldro"].shape[0], aldro=dptr(k["aldro"]), cionCF=dptr(k["cf"]),
-            cionTherm=dptr(k["th"]), nAuger=iptr(k["na"]), auger=dptr(k["au"]),
-            nChex=k["cz"].size, chexZ=iptr(k["cz"]), chexIon=iptr(k["ci"]),
-            chexFit=dptr(k["cfit"]), nAtoms=k["am"].shape[0], atomIdx=iptr(k["ai"]),
-            atomMeta=k["am"].ctypes.data_as(lp), atomAtp=dptr(k["atp"]), nLogT=k["lt"].size,
-            nLev=k["ag"].size, nA=k["aA"].size, nQ=k["aq"].size, atomLogT=dptr(k["lt"]),
-            atomG=dptr(k["ag"]), atomE=dptr(k["ae"]), atomA=dptr(k["aA"]),
-            atomQom=dptr(k["aq"]), atomQ2=dptr(k["aq2"]), atomRecKind=iptr(k["rk"]),
-            atomRec=dptr(k["rt"]), atomRecValid=iptr(k["rv"]))
+        self._balance_keep = balance_arrays(bt)
+        t = to_struct(MocBalanceTables, self._balance_keep)
         self._check(self._L.moc_set_balance(self._ctx, C.byref(t)), "moc_set_balance")
 
     def set_emission(self, et) -> None:
         """Emission tables (an emission.EmissionTables built on the same mesh
         and element set); after set_balance."""
-        lp = C.POINTER(C.c_int64)
-        k = {"tk": _c(et.tk), "edge": [_c(e, np.int32) for e in et.edge],
-             "lg": [_c(v) for v in et.loggam], "xsHI": _c(et.xsHI),
-             "xsHeI": None if et.xsHeI is None else _c(et.xsHeI),
-             "xsHeII": None if et.xsHeII is None else _c(et.xsHeII),
-             "fbZ": _c(et.fbZ, np.int32), "fbIon": _c(et.fbIon, np.int32),
-             "fbLo": _c(et.fbLo, np.int32), "fbHi": _c(et.fbHi, np.int32),
-             "fbOff": _c(et.fbOff, np.int64), "fbXs": _c(et.fbXs), "fbG": _c(et.fbGratio),
-             "he2j": _c(et.he2j, np.int32), "he2": _c(et.hei2phot), "map": _c(et.hlMap, np.int32),
-             "dens": _c(et.hlDens), "lines": _c(et.hlLines), "hbA": _c(et.hbA), "hbB": _c(et.hbB),
-             "rb": _c(et.rbEdge), "r2a": _c(et.r2aEdge), "heiRec": _c(et.heiRec),
-             "lym": _c(et.heiiLyman)}
-        self._emission_keep = k
-        opt = lambda a: dptr(a) if a is not None else None
-        t = MocEmissionTables(
-            ntk=k["tk"].size, tkGamma=dptr(k["tk"]), nlim=(C.c_int32 * 3)(*[e.size for e in k["edge"]]),
-            edgeP=(_ip * 3)(*[iptr(e) for e in k["edge"]]), logGamma=(_dp * 3)(*[dptr(v) for v in k["lg"]]),
-            HIPnuP=int(et.HIPnuP), HeIPnuP=int(et.HeIPnuP), HeIIPnuP=int(et.HeIIPnuP),
-            xsHI=dptr(k["xsHI"]), xsHeI=opt(k["xsHeI"]), xsHeII=opt(k["xsHeII"]),
-            nFb=k["fbZ"].size, fbZ=iptr(k["fbZ"]), fbIon=iptr(k["fbIon"]), fbLo=iptr(k["fbLo"]),
-            fbHi=iptr(k["fbHi"]), fbOff=k["fbOff"].ctypes.data_as(lp), nFbXs=k["fbXs"].size,
-            fbXs=dptr(k["fbXs"]), fbGratio=dptr(k["fbG"]), he2Row=iptr(k["he2j"]),
-            hei2phot=dptr(k["he2"]), nHlFiles=k["dens"].shape[0], hlMap=iptr(k["map"]),
-            hlDens=dptr(k["dens"]), hlLines=dptr(k["lines"]), hbA=dptr(k["hbA"]),
-            hbB=dptr(k["hbB"]), rbEdge=dptr(k["rb"]), r2aEdge=dptr(k["r2a"]),
-            heiRec=dptr(k["heiRec"]), heiiLyman=dptr(k["lym"]), j2TsP=int(et.j2TsP),
-            lymP=(C.c_int32 * 4)(*[int(v) for v in et.lymP]))
+        self._emission_keep = emission_arrays(et)
+        t = to_struct(MocEmissionTables, self._emission_keep)
         self._check(self._L.moc_set_emission(self._ctx, C.byref(t)), "moc_set_emission")
 
     def emissionDriver(self) -> float:
