@@ -70,7 +70,7 @@ def build_fortran_shim() -> Path | None:
         return None
     out = FORTRAN / "mocgpu_driver"
     cmd = [fc, "-O2", "-module-dir", str(FORTRAN), "-o", str(out), str(mod_src), str(drv_src),
-           f"-L{PKG}", "-lmocgpu", f"-Wl,-rpath,{PKG}"]
+           f"-L{PKG}", "-lmocgpu", "-Wl,-rpath,$ORIGIN/.."]
     res = subprocess.run(cmd, capture_output=True, text=True, cwd=str(FORTRAN))
     if res.returncode != 0:
         raise RuntimeError("flang failed:\n" + " ".join(cmd) + "\n" + res.stderr[-4000:])

@@ -1,0 +1,435 @@
+! mocgpu_mod -- the Fortran side of libmocgpu: ISO_C_BINDING types and
+! interfaces for every entry point of include/mocgpu.h, and the MOCASSIN
+! drivers rebuilt on them with the reference's names and control flow:
+!
+!   ionizationDriver   <- ionization_mod.f90:26   (every cell at once)
+!   emissionDriver     <- emission_mod.f90:36     (every cell at once)
+!   energyPacketDriver <- photon_mod.f90:28       (pathSegment runs on the device)
+!   updateCell         <- update_mod.f90:15       (every cell at once)
+!   MCIterationDriver  <- iteration_mod.f90:17    (the iterateMC loop: convergence
+!                                                  percentage, minConvergence,
+!                                                  maxIterateMC, autoPackets)
+!
+! The grid arrays stay in the reference's layout on the Fortran side:
+! (0:nCells) per cell, (0:nCells, 1:nbins) per cell and frequency (cell index
+! fastest) and (0:nCells, nElementsUsed, nstages) for ionDen; put_field /
+! get_field move them through moc_upload / moc_download, which transpose on
+! the host.  Every nonzero return code stops the run with the library's
+! message, as the reference's print* + stop does.
+module mocgpu_mod
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: MOC_OK = 0
+  integer(c_int), parameter, public :: MOC_PREC_FP64 = 0, MOC_PREC_MIXED = 1
+  integer(c_int), parameter, public :: MOC_DT_F32 = 0, MOC_DT_F64 = 1, MOC_DT_I32 = 2
+  integer(c_int), parameter, public :: MOC_LAYOUT_FORTRAN = 0, MOC_LAYOUT_CELLROW = 1
+  integer(c_int), parameter, public :: MOC_F_OPACITY = 1, MOC_F_JSTE = 2, MOC_F_RECPDF = 3, &
+       MOC_F_TOTALLINES = 4, MOC_F_NE = 5, MOC_F_TE = 6, MOC_F_HDEN = 7, MOC_F_ESCAPED = 8, &
+       MOC_F_IONDEN = 9, MOC_F_CONVERGED = 10, MOC_F_BLACK = 11
+
+  type, bind(C), public :: moc_tables
+     integer(c_int32_t) :: nbins, precision
+     type(c_ptr) :: nuArray, widFlx
+     real(c_double) :: ionEdge1
+     integer(c_int32_t) :: lgCompton, nAngles
+     type(c_ptr) :: KNsigmaT, KNsigmaCDF, Pcomp
+     integer(c_int32_t) :: nStars
+     type(c_ptr) :: sourceCDF, Lstar, nPhotons, starPosition, starIndex
+  end type moc_tables
+
+  type, bind(C), public :: moc_grid_desc
+     integer(c_int32_t) :: nx, ny, nz
+     type(c_ptr) :: xAxis, yAxis, zAxis, active
+     integer(c_int32_t) :: nCells, lgSymmetricXYZ
+     real(c_double) :: R_out
+  end type moc_grid_desc
+
+  type, bind(C), public :: moc_atomic
+     integer(c_int32_t) :: nElemUsed
+     type(c_ptr) :: elemZ, abund
+     integer(c_int32_t) :: nstages, nSpecies
+     type(c_ptr) :: speciesElem, speciesIon, sigma, brems
+  end type moc_atomic
+
+  type, bind(C), public :: moc_balance_tables
+     integer(c_int32_t) :: maxLev
+     real(c_double) :: XHILimit
+     integer(c_int32_t) :: nShells
+     type(c_ptr) :: shellNum, speciesShell, W, outShell, statW0, statW1, rrb, drb_c, drb_e, &
+          rrec, rnew
+     integer(c_int32_t) :: nDiel
+     type(c_ptr) :: diel
+     integer(c_int32_t) :: nAldro
+     type(c_ptr) :: aldro, cionCF, cionTherm, nAuger, auger
+     integer(c_int32_t) :: nChex
+     type(c_ptr) :: chexZ, chexIon, chexFit
+     integer(c_int32_t) :: nAtoms
+     type(c_ptr) :: atomIdx, atomMeta, atomAtp
+     integer(c_int64_t) :: nLogT, nLev, nA, nQ
+     type(c_ptr) :: atomLogT, atomG, atomE, atomA, atomQom, atomQ2, atomRecKind, atomRec, &
+          atomRecValid
+  end type moc_balance_tables
+
+  type, bind(C), public :: moc_emission_tables
+     integer(c_int32_t) :: ntk
+     type(c_ptr) :: tkGamma
+     integer(c_int32_t) :: nlim(3)
+     type(c_ptr) :: edgeP(3), logGamma(3)
+     integer(c_int32_t) :: HIPnuP, HeIPnuP, HeIIPnuP
+     type(c_ptr) :: xsHI, xsHeI, xsHeII
+     integer(c_int32_t) :: nFb
+     type(c_ptr) :: fbZ, fbIon, fbLo, fbHi, fbOff
+     integer(c_int64_t) :: nFbXs
+     type(c_ptr) :: fbXs, fbGratio, he2Row, hei2phot
+     integer(c_int32_t) :: nHlFiles
+     type(c_ptr) :: hlMap, hlDens, hlLines, hbA, hbB, rbEdge, r2aEdge, heiRec, heiiLyman
+     integer(c_int32_t) :: j2TsP, lymP(4)
+  end type moc_emission_tables
+
+  type, bind(C), public :: moc_transport_stats
+     real(c_double) :: Qphot, segments, emissions, absorbed, scattered, escaped, lines, lost, &
+          trapped, kernel_ms
+  end type moc_transport_stats
+
+  type, bind(C), public :: moc_update_stats
+     real(c_double) :: noHit, noIonBal, noTeBal, converged, kernel_ms
+  end type moc_update_stats
+
+  type, bind(C), public :: moc_iteration_stats
+     real(c_double) :: convPercent, noHitPercent, noIonBalPercent, noTeBalPercent, packets, &
+          opacity_ms, emission_ms, transport_ms, update_ms, segments, emissions, escaped
+  end type moc_iteration_stats
+
+  interface
+     integer(c_int) function moc_create(ctx, tables) bind(C, name="moc_create")
+       import :: c_int, c_ptr, moc_tables
+       type(c_ptr), intent(out) :: ctx
+       type(moc_tables), intent(in) :: tables
+     end function moc_create
+     subroutine moc_destroy(ctx) bind(C, name="moc_destroy")
+       import :: c_ptr
+       type(c_ptr), value :: ctx
+     end subroutine moc_destroy
+     type(c_ptr) function moc_last_error(ctx) bind(C, name="moc_last_error")
+       import :: c_ptr
+       type(c_ptr), value :: ctx
+     end function moc_last_error
+     type(c_ptr) function moc_version() bind(C, name="moc_version")
+       import :: c_ptr
+     end function moc_version
+     integer(c_int) function moc_grid_attach(ctx, ig, grid) bind(C, name="moc_grid_attach")
+       import :: c_int, c_ptr, moc_grid_desc
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ig
+       type(moc_grid_desc), intent(in) :: grid
+     end function moc_grid_attach
+     integer(c_int) function moc_upload(ctx, field, ig, host, count, dtype, layout) &
+          bind(C, name="moc_upload")
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx, host
+       integer(c_int), value :: field, ig, dtype, layout
+       integer(c_int64_t), value :: count
+     end function moc_upload
+     integer(c_int) function moc_download(ctx, field, ig, host, count, dtype, layout) &
+          bind(C, name="moc_download")
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx, host
+       integer(c_int), value :: field, ig, dtype, layout
+       integer(c_int64_t), value :: count
+     end function moc_download
+     integer(c_int) function moc_set_atomic(ctx, atomic) bind(C, name="moc_set_atomic")
+       import :: c_int, c_ptr, moc_atomic
+       type(c_ptr), value :: ctx
+       type(moc_atomic), intent(in) :: atomic
+     end function moc_set_atomic
+     integer(c_int) function moc_opacity(ctx, ig) bind(C, name="moc_opacity")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ig
+     end function moc_opacity
+     integer(c_int) function moc_set_balance(ctx, tables) bind(C, name="moc_set_balance")
+       import :: c_int, c_ptr, moc_balance_tables
+       type(c_ptr), value :: ctx
+       type(moc_balance_tables), intent(in) :: tables
+     end function moc_set_balance
+     integer(c_int) function moc_set_emission(ctx, tables) bind(C, name="moc_set_emission")
+       import :: c_int, c_ptr, moc_emission_tables
+       type(c_ptr), value :: ctx
+       type(moc_emission_tables), intent(in) :: tables
+     end function moc_set_emission
+     integer(c_int) function moc_emission(ctx, ig, kernel_ms) bind(C, name="moc_emission")
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ig
+       real(c_float), intent(out) :: kernel_ms
+     end function moc_emission
+     integer(c_int) function moc_set_packets(ctx, iStar, n) bind(C, name="moc_set_packets")
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: iStar
+       integer(c_int64_t), value :: n
+     end function moc_set_packets
+     integer(c_int) function moc_iterate(ctx, ig, nIterateMC, seed, stats) &
+          bind(C, name="moc_iterate")
+       import :: c_int, c_ptr, c_int64_t, moc_iteration_stats
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ig, nIterateMC
+       integer(c_int64_t), value :: seed
+       type(moc_iteration_stats), intent(out) :: stats
+     end function moc_iterate
+     integer(c_int) function moc_update(ctx, ig, stats) bind(C, name="moc_update")
+       import :: c_int, c_ptr, moc_update_stats
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ig
+       type(moc_update_stats), intent(out) :: stats
+     end function moc_update
+     integer(c_int) function moc_reset_tallies(ctx, ig) bind(C, name="moc_reset_tallies")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ig
+     end function moc_reset_tallies
+     integer(c_int) function moc_transport(ctx, iStar, n, seed, ctr_base, stats) &
+          bind(C, name="moc_transport")
+       import :: c_int, c_ptr, c_int64_t, c_int32_t, moc_transport_stats
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: iStar
+       integer(c_int64_t), value :: n, seed
+       integer(c_int32_t), value :: ctr_base
+       type(moc_transport_stats), intent(out) :: stats
+     end function moc_transport
+     integer(c_int) function moc_normalize_j(ctx, ig) bind(C, name="moc_normalize_j")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ig
+     end function moc_normalize_j
+  end interface
+
+  public :: moc_create, moc_destroy, moc_last_error, moc_version, moc_grid_attach, moc_upload, &
+       moc_download, moc_set_atomic, moc_opacity, moc_set_balance, moc_set_emission, &
+       moc_emission, moc_set_packets, moc_iterate, moc_update, moc_reset_tallies, moc_transport, &
+       moc_normalize_j
+  public :: moc_check, c_string, put_field, get_field, ionizationDriver, emissionDriver, &
+       energyPacketDriver, updateCell, MCIterationDriver
+
+  ! real(8) arrays, or default-real ones as the reference's grid_type holds
+  interface put_field
+     module procedure put_field_1d, put_field_2d, put_field_3d, put_field_1s, put_field_2s, &
+          put_field_3s
+  end interface put_field
+  interface get_field
+     module procedure get_field_1d, get_field_2d, get_field_3d, get_field_1s, get_field_2s, &
+          get_field_3s
+  end interface get_field
+
+contains
+
+  ! a C string as a Fortran character variable
+  function c_string(p) result(s)
+    type(c_ptr), intent(in) :: p
+    character(len=:), allocatable :: s
+    character(kind=c_char), pointer :: a(:)
+    integer :: n
+    interface
+       integer(c_size_t) function strlen(q) bind(C, name="strlen")
+         import :: c_size_t, c_ptr
+         type(c_ptr), value :: q
+       end function strlen
+    end interface
+    if (.not. c_associated(p)) then
+       s = ""
+       return
+    end if
+    n = int(strlen(p))
+    call c_f_pointer(p, a, [n])
+    allocate(character(len=n) :: s)
+    s = transfer(a, s)
+  end function c_string
+
+  ! the reference's error convention: report and stop
+  subroutine moc_check(ctx, rc, where)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    if (rc /= MOC_OK) then
+       print*, "! ", where, ": ", c_string(moc_last_error(ctx)), " (rc=", rc, ")"
+       stop
+    end if
+  end subroutine moc_check
+
+  subroutine put_field_1d(ctx, field, a)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: field
+    real(c_double), intent(in), target, contiguous :: a(:)
+    call moc_check(ctx, moc_upload(ctx, field, 1, c_loc(a), int(size(a), c_int64_t), &
+         MOC_DT_F64, MOC_LAYOUT_FORTRAN), "put_field")
+  end subroutine put_field_1d
+
+  subroutine put_field_2d(ctx, field, a)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: field
+    real(c_double), intent(in), target, contiguous :: a(:, :)
+    call moc_check(ctx, moc_upload(ctx, field, 1, c_loc(a), int(size(a), c_int64_t), &
+         MOC_DT_F64, MOC_LAYOUT_FORTRAN), "put_field")
+  end subroutine put_field_2d
+
+  subroutine put_field_3d(ctx, field, a)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: field
+    real(c_double), intent(in), target, contiguous :: a(:, :, :)
+    call moc_check(ctx, moc_upload(ctx, field, 1, c_loc(a), int(size(a), c_int64_t), &
+         MOC_DT_F64, MOC_LAYOUT_FORTRAN), "put_field")
+  end subroutine put_field_3d
+
+  subroutine get_field_1d(ctx, field, a)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: field
+    real(c_double), intent(inout), target, contiguous :: a(:)
+    call moc_check(ctx, moc_download(ctx, field, 1, c_loc(a), int(size(a), c_int64_t), &
+         MOC_DT_F64, MOC_LAYOUT_FORTRAN), "get_field")
+  end subroutine get_field_1d
+
+  subroutine get_field_2d(ctx, field, a)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: field
+    real(c_double), intent(inout), target, contiguous :: a(:, :)
+    call moc_check(ctx, moc_download(ctx, field, 1, c_loc(a), int(size(a), c_int64_t), &
+         MOC_DT_F64, MOC_LAYOUT_FORTRAN), "get_field")
+  end subroutine get_field_2d
+
+  subroutine get_field_3d(ctx, field, a)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: field
+    real(c_double), intent(inout), target, contiguous :: a(:, :, :)
+    call moc_check(ctx, moc_download(ctx, field, 1, c_loc(a), int(size(a), c_int64_t), &
+         MOC_DT_F64, MOC_LAYOUT_FORTRAN), "get_field")
+  end subroutine get_field_3d
+
+  subroutine put_field_1s(ctx, field, a)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: field
+    real(c_float), intent(in), target, contiguous :: a(:)
+    call moc_check(ctx, moc_upload(ctx, field, 1, c_loc(a), int(size(a), c_int64_t), &
+         MOC_DT_F32, MOC_LAYOUT_FORTRAN), "put_field")
+  end subroutine put_field_1s
+
+  subroutine put_field_2s(ctx, field, a)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: field
+    real(c_float), intent(in), target, contiguous :: a(:, :)
+    call moc_check(ctx, moc_upload(ctx, field, 1, c_loc(a), int(size(a), c_int64_t), &
+         MOC_DT_F32, MOC_LAYOUT_FORTRAN), "put_field")
+  end subroutine put_field_2s
+
+  subroutine put_field_3s(ctx, field, a)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: field
+    real(c_float), intent(in), target, contiguous :: a(:, :, :)
+    call moc_check(ctx, moc_upload(ctx, field, 1, c_loc(a), int(size(a), c_int64_t), &
+         MOC_DT_F32, MOC_LAYOUT_FORTRAN), "put_field")
+  end subroutine put_field_3s
+
+  subroutine get_field_1s(ctx, field, a)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: field
+    real(c_float), intent(inout), target, contiguous :: a(:)
+    call moc_check(ctx, moc_download(ctx, field, 1, c_loc(a), int(size(a), c_int64_t), &
+         MOC_DT_F32, MOC_LAYOUT_FORTRAN), "get_field")
+  end subroutine get_field_1s
+
+  subroutine get_field_2s(ctx, field, a)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: field
+    real(c_float), intent(inout), target, contiguous :: a(:, :)
+    call moc_check(ctx, moc_download(ctx, field, 1, c_loc(a), int(size(a), c_int64_t), &
+         MOC_DT_F32, MOC_LAYOUT_FORTRAN), "get_field")
+  end subroutine get_field_2s
+
+  subroutine get_field_3s(ctx, field, a)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: field
+    real(c_float), intent(inout), target, contiguous :: a(:, :, :)
+    call moc_check(ctx, moc_download(ctx, field, 1, c_loc(a), int(size(a), c_int64_t), &
+         MOC_DT_F32, MOC_LAYOUT_FORTRAN), "get_field")
+  end subroutine get_field_3s
+
+  ! opacities of every cell (ionization_mod.f90:26, called per cell from
+  ! iteration_mod.f90:148)
+  subroutine ionizationDriver(ctx)
+    type(c_ptr), intent(in) :: ctx
+    call moc_check(ctx, moc_opacity(ctx, 1), "ionizationDriver")
+  end subroutine ionizationDriver
+
+  ! diffuse emission CDFs and line fractions of every cell (emission_mod.f90:36)
+  subroutine emissionDriver(ctx)
+    type(c_ptr), intent(in) :: ctx
+    real(c_float) :: ms
+    call moc_check(ctx, moc_emission(ctx, 1, ms), "emissionDriver")
+  end subroutine emissionDriver
+
+  ! n packets of star iStar through the grid (photon_mod.f90:28); the J and
+  ! escape tallies accumulate on the device until moc_reset_tallies
+  subroutine energyPacketDriver(ctx, iStar, n, seed, nIterateMC, stats)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: iStar, nIterateMC
+    integer(c_int64_t), intent(in) :: n, seed
+    type(moc_transport_stats), intent(out) :: stats
+    call moc_check(ctx, moc_transport(ctx, iStar, n, seed, &
+         int(nIterateMC * 256 + iStar, c_int32_t), stats), "energyPacketDriver")
+  end subroutine energyPacketDriver
+
+  ! the balance of every cell (update_mod.f90:15) from the normalised J
+  subroutine updateCell(ctx, stats)
+    type(c_ptr), intent(in) :: ctx
+    type(moc_update_stats), intent(out) :: stats
+    call moc_check(ctx, moc_update(ctx, 1, stats), "updateCell")
+  end subroutine updateCell
+
+  ! The iterateMC loop (iteration_mod.f90:32-1497 without its output files):
+  ! one device iteration per pass, the converged-cell percentage against
+  ! minConvergence and maxIterateMC, and with lgAutoPackets the packet count
+  ! of every star raised by nPhotIncrease when the percentage grew by no more
+  ! than convIncPercent (:1412-1421).  nPhotons is updated in place.
+  subroutine MCIterationDriver(ctx, nPhotons, maxIterateMC, minConvergence, seed, &
+       lgAutoPackets, convIncPercent, nPhotIncrease, maxPhotons, nIterateMC, totPercent)
+    type(c_ptr), intent(in) :: ctx
+    real(c_double), intent(inout) :: nPhotons(:)
+    integer, intent(in) :: maxIterateMC
+    real(c_double), intent(in) :: minConvergence, convIncPercent, nPhotIncrease, maxPhotons
+    integer(c_int64_t), intent(in) :: seed
+    logical, intent(in) :: lgAutoPackets
+    integer, intent(out) :: nIterateMC
+    real(c_double), intent(out) :: totPercent
+    type(moc_iteration_stats) :: st
+    real(c_double) :: totPercentOld, nPhotonsTot
+    integer :: iStar
+    totPercentOld = 0.d0
+    nIterateMC = 1
+    do
+       call moc_check(ctx, moc_iterate(ctx, 1, int(nIterateMC, c_int), seed, st), &
+            "MCIterationDriver")
+       totPercent = st%convPercent
+       print '(A,I4,A,F8.3,A,F8.3,A,F8.3,A,F8.3,A)', " ! iterateMC: iteration ", nIterateMC, &
+            ": ", totPercent, "% converged cells, ", st%noHitPercent, "% no hit, ", &
+            st%noIonBalPercent, "% ion bal not reached, ", st%noTeBalPercent, &
+            "% Te bal not reached"
+       nPhotonsTot = nPhotons(1) + sum(nPhotons)
+       if (lgAutoPackets .and. nIterateMC > 1 .and. totPercent < 95.d0 .and. &
+            nPhotonsTot < maxPhotons .and. totPercentOld > 0.d0) then
+          if ((totPercent - totPercentOld) / totPercentOld <= convIncPercent) then
+             nPhotons = nPhotons * nPhotIncrease
+             do iStar = 1, size(nPhotons)
+                call moc_check(ctx, moc_set_packets(ctx, int(iStar, c_int), &
+                     int(nPhotons(iStar), c_int64_t)), "MCIterationDriver")
+             end do
+             print*, "! iterateMC: nPhotons increased to ", nPhotons
+          end if
+       end if
+       totPercentOld = totPercent
+       if (totPercent >= minConvergence .or. nIterateMC >= maxIterateMC) exit
+       nIterateMC = nIterateMC + 1
+    end do
+  end subroutine MCIterationDriver
+
+end module mocgpu_mod
