@@ -13,6 +13,15 @@
  * path): an element that is switched off has no He terms in the H-only
  * convergence test (the reference reads ionDen(elementXref(2)=0, ...)), and a
  * shell whose threshold lies below the first mesh point starts at bin 1.
+ *
+ * Pinned by tests/test_balance.py: calcAlpha for H and He II (known values,
+ * hydrogenic scaling, T slope), the n-level solve (closed-form two-level
+ * atom, 5-level linear solve, low-density limit), and the H photoionisation
+ * equilibrium of ionBalance.  Parity unpinned (no independent check here):
+ * the heavy-element ionBalance chain with Auger spreading and charge
+ * exchange, the recombination fits for ions with more than 11 electrons
+ * (radRecFit, dielectronic, Aldrovandi), the thermBalance cooling fits and
+ * the iterateT root-find path.
  */
 #include <math.h>
 #include <stdlib.h>

@@ -23,6 +23,14 @@
  *    reference indexes xSecArray before the start of that ion's block there,
  *    reading the preceding table's entries);
  *  - all arithmetic is FP64 (the reference mixes single-precision locals).
+ *
+ * Pinned by tests/test_emission.py: the Milne free-bound of H (recombination
+ * to 1s), its hydrogenic scaling to He II and C VI, the two-photon energy per
+ * decay, the free-free constant, the Hbeta emissivity, the gamma-fit
+ * interpolation at a tabulated T, the recombination feed of the n-level
+ * solve and the CDF assembly.  Parity unpinned: the He I recombination-line
+ * fits, the hydrogenic line tables above 3e4 K (rbEdge / r2aEdge branches)
+ * and the free-bound of non-hydrogenic heavy ions.
  */
 #include <math.h>
 #include <stdlib.h>
