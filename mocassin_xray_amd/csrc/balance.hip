@@ -73,116 +73,142 @@ __global__ void __launch_bounds__(256) k_rates(const double* __restrict__ J, int
 }
 
 // ---------------------------------------------------------------- k_balance
-struct CellState {
-    double Te, Ne, H;
-    double X[MOC_MAXE][MOC_MAXS + 1];      // ion fractions by (element slot, stage)
-    double alpha[MOC_MAXE][MOC_MAXS + 1];  // recombination coefficients
-    double cI1[MOC_MAXE][MOC_MAXS + 1];    // collisional ionisation rate
-    double cI2[MOC_MAXE][MOC_MAXS + 1];    // collisional ionisation cooling
-    double heatef;
-    int nIterX;
-};
+// One wave per cell.  The cell's state (ion fractions, recombination and
+// collisional-ionisation coefficients, the heavy-element chain work rows and
+// the per-ion cooling terms) lives in LDS; per-ion work -- recombination
+// coefficients, collisional ionisation, the heavy-element chains of
+// ionBalance, the n-level cooling of every ion with data -- is spread over the
+// lanes, and every sum is then taken by all lanes in the reference's order, so
+// each lane holds the same (wave-uniform) Te, Ne and control state and the
+// results equal a sequential evaluation operation for operation.
 
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
-__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 
 __device__ __forceinline__ double t3(const double* a, int i, int z, int n) {
     return a[((size_t)i * 31 + z) * 31 + n];
 }
 
-// recombination coefficients (calcAlpha with Badnell fits, :2298-2362)
-__device__ void recomb(const MocBalanceArgs& a, CellState& s) {
-    const double T = s.Te;
+// LDS rows of one cell, each [NE][S2] with S2 = nstages + 2 (stage index 0..NS+1)
+struct Lds {
+    double *X, *alpha, *cI1, *cI2, *out, *in, *mat, *cool, *diO;
+    int* hot;
+    int S2;
+    __device__ double& x(int e, int i) const { return X[e * S2 + i]; }
+    __device__ double& al(int e, int i) const { return alpha[e * S2 + i]; }
+    __device__ double& c1(int e, int i) const { return cI1[e * S2 + i]; }
+    __device__ double& c2(int e, int i) const { return cI2[e * S2 + i]; }
+};
+
+__host__ __device__ inline size_t balance_lds_bytes(int NE, int NS, int nNlv) {
+    const size_t rows = (size_t)7 * NE * (NS + 2);
+    return (rows + (size_t)(nNlv > 0 ? nNlv : 1) + MOC_MAXS + 1) * sizeof(double) + 16;
+}
+
+// dielectronic recombination of the ions with more than 11 electrons
+// (Aldrovandi & Pequignot above 6e4 K, Nussbaumer & Storey below;
+// dielectronic :3204-3305): the last matching row wins
+__device__ double direc(const MocBalanceArgs& a, int z, int ion, double T) {
     const double tt = T > 1000. ? T / 10000. : 0.1;
-    // dielectronic fits that are not Badnell's (dielectronic :3204-3305)
-    double diO[MOC_MAXS + 1];  // oxygen's, borrowed by ions without their own
+    double v = 0.0;
+    if (T > 60000.) {
+        for (int i = 0; i < a.nAldro; ++i) {
+            const double* r = a.aldro + 6 * i;
+            if ((int)r[0] == z && (int)r[0] - (int)r[1] + 1 == ion)
+                v = r[2] * pow(T, -1.5) * exp(-r[4] / T) * (1. + r[3] * exp(-r[5] / T));
+        }
+        return v;
+    }
+    for (int i = 0; i < a.nDiel; ++i) {
+        const double* r = a.diel + 8 * i;
+        const int g = (int)r[7];
+        if ((int)r[0] != z || (int)r[0] + 1 - (int)r[1] != ion) continue;
+        if (g == 0 || (g == 1 && T < 20000.) || (g == 2 && T >= 20000.))
+            v = 1.e-12 * (r[2] / tt + r[3] + r[4] * tt + r[5] * tt * tt) * pow(tt, -1.5) * exp(-r[6] / tt);
+    }
+    return v;
+}
+
+// recombination coefficients of every ion (calcAlpha :2298-2413), one
+// (element, stage) per lane
+__device__ void recomb(const MocBalanceArgs& a, const Lds& L, double T, int tid) {
+    const int NE = a.NE, NS = a.NS, S2 = L.S2;
     const int eO = a.elemSlot[8];
-    for (int i = 0; i <= MOC_MAXS; ++i) diO[i] = 0.0;
-    for (int e = 0; e < a.NE; ++e)
-        for (int i = 0; i <= MOC_MAXS; ++i) s.alpha[e][i] = 0.0;
-    for (int e = 0; e < a.NE; ++e) {
+    for (int k = tid; k < NE * S2; k += 64) L.alpha[k] = 0.0;
+    for (int i = tid; i <= MOC_MAXS; i += 64)
+        L.diO[i] = (eO >= 0 && i >= 1 && i < 7 && i <= NS) ? direc(a, 8, i, T) : 0.0;
+    __syncthreads();
+    for (int k = tid; k < NE * S2; k += 64) {
+        const int e = k / S2, ion = k % S2;
         const int z = a.elemZ[e];
-        for (int ion = 1; ion <= imin(a.NS - 1, z); ++ion) {
-            const int nel = z - ion;
-            if (nel > 11) continue;
+        if (ion < 1 || ion > imin(NS - 1, z)) continue;
+        const int nel = z - ion;
+        if (nel <= 11) {  // Badnell radiative + dielectronic fits (:2314-2330)
             const double bb = t3(a.rrb, 1, z, nel) + t3(a.rrb, 4, z, nel) * exp(-t3(a.rrb, 5, z, nel) / T);
             const double s0 = sqrt(T / t3(a.rrb, 2, z, nel)), s1 = sqrt(T / t3(a.rrb, 3, z, nel));
             double dr = 0.0;
-            for (int k = 0; k < 9; ++k) {
-                const double E = t3(a.drb_e, k, z, nel);
-                if (E / T < 87.) dr += t3(a.drb_c, k, z, nel) * exp(-E / T);
+            for (int q = 0; q < 9; ++q) {
+                const double E = t3(a.drb_e, q, z, nel);
+                if (E / T < 87.) dr += t3(a.drb_c, q, z, nel) * exp(-E / T);
             }
-            s.alpha[e][ion] = t3(a.rrb, 0, z, nel) / (s0 * pow(1. + s0, 1. - bb) * pow(1. + s1, 1. + bb)) +
-                              dr * pow(T, -1.5);
+            L.al(e, ion) = t3(a.rrb, 0, z, nel) / (s0 * pow(1. + s0, 1. - bb) * pow(1. + s1, 1. + bb)) +
+                           dr * pow(T, -1.5);
+            continue;
         }
-    }
-    // diRec for the ions with more than 11 electrons, and oxygen's row
-    auto direc = [&](int z, int ion) -> double {
-        double v = 0.0;
-        if (T > 60000.) {
-            for (int i = 0; i < a.nAldro; ++i) {
-                const double* r = a.aldro + 6 * i;
-                if ((int)r[0] == z && (int)r[0] - (int)r[1] + 1 == ion)
-                    v = r[2] * pow(T, -1.5) * exp(-r[4] / T) * (1. + r[3] * exp(-r[5] / T));
-            }
-            return v;
-        }
-        for (int i = 0; i < a.nDiel; ++i) {
-            const double* r = a.diel + 8 * i;
-            const int g = (int)r[7];
-            if ((int)r[0] != z || (int)r[0] + 1 - (int)r[1] != ion) continue;
-            if (g == 0 || (g == 1 && T < 20000.) || (g == 2 && T >= 20000.))
-                v = 1.e-12 * (r[2] / tt + r[3] + r[4] * tt + r[5] * tt * tt) * pow(tt, -1.5) * exp(-r[6] / tt);
-        }
-        return v;
-    };
-    if (eO >= 0)
-        for (int ion = 1; ion < 7 && ion <= a.NS; ++ion) diO[ion] = direc(8, ion);
-    for (int e = 0; e < a.NE; ++e) {
-        const int z = a.elemZ[e];
-        for (int ion = 1; ion <= imin(a.NS - 1, z); ++ion) {
-            if (z - ion <= 11) continue;
-            const int n = z - ion + 1;
-            double rr;
-            if (n <= 3 || n == 11 || (z > 5 && z < 9) || z == 10 || (z == 26 && n > 11)) {
-                const double q = sqrt(T / t3(a.rnew, 2, z, n)), p = t3(a.rnew, 1, z, n);
-                rr = t3(a.rnew, 0, z, n) / (q * pow(q + 1., 1. - p) * pow(1. + sqrt(T / t3(a.rnew, 3, z, n)), 1. + p));
+        // radRecFit + dielectronic, oxygen's dielectronic borrowed when zero (:2334-2362)
+        const int n = z - ion + 1;
+        double rr;
+        if (n <= 3 || n == 11 || (z > 5 && z < 9) || z == 10 || (z == 26 && n > 11)) {
+            const double q = sqrt(T / t3(a.rnew, 2, z, n)), p = t3(a.rnew, 1, z, n);
+            rr = t3(a.rnew, 0, z, n) / (q * pow(q + 1., 1. - p) * pow(1. + sqrt(T / t3(a.rnew, 3, z, n)), 1. + p));
+        } else {
+            const double q = T * 1.e-4;
+            if (z == 26 && n <= 13) {
+                const double fe0[10] = {4.33e-10, 3.91e-10, 3.49e-10, 3.16e-10, 2.96e-10,
+                                        2.59e-10, 2.24e-10, 1.91e-10, 1.68e-10, 1.46e-10};
+                const double fe1[10] = {0.531, 0.523, 0.521, 0.534, 0.557, 0.567, 0.579, 0.601, 0.602, 0.597};
+                const double fe2[10] = {5.77e-02, 6.15e-02, 6.22e-02, 6.02e-02, 5.79e-02,
+                                        5.65e-02, 5.49e-02, 5.10e-02, 5.07e-02, 5.22e-02};
+                rr = fe0[n - 4] / pow(q, fe1[n - 4] + fe2[n - 4] * log10(q));
             } else {
-                const double q = T * 1.e-4;
-                if (z == 26 && n <= 13) {
-                    const double fe0[10] = {4.33e-10, 3.91e-10, 3.49e-10, 3.16e-10, 2.96e-10,
-                                            2.59e-10, 2.24e-10, 1.91e-10, 1.68e-10, 1.46e-10};
-                    const double fe1[10] = {0.531, 0.523, 0.521, 0.534, 0.557, 0.567, 0.579, 0.601, 0.602, 0.597};
-                    const double fe2[10] = {5.77e-02, 6.15e-02, 6.22e-02, 6.02e-02, 5.79e-02,
-                                            5.65e-02, 5.49e-02, 5.10e-02, 5.07e-02, 5.22e-02};
-                    rr = fe0[n - 4] / pow(q, fe1[n - 4] + fe2[n - 4] * log10(q));
-                } else {
-                    rr = t3(a.rrec, 0, z, n) / pow(q, t3(a.rrec, 1, z, n));
-                }
+                rr = t3(a.rrec, 0, z, n) / pow(q, t3(a.rrec, 1, z, n));
             }
-            double di = direc(z, ion);
-            if (di == 0. && ion < 7 && eO >= 0) di = diO[ion];
-            s.alpha[e][ion] = rr + (di > 0. ? di : 0.);
         }
+        double di = direc(a, z, ion, T);
+        if (di == 0. && ion < 7 && eO >= 0) di = L.diO[ion];
+        L.al(e, ion) = rr + (di > 0. ? di : 0.);
     }
+    __syncthreads();
 }
 
-// Voronov collisional ionisation with the reference's early exit (:3552-3582)
-__device__ void collion(const MocBalanceArgs& a, CellState& s) {
-    const double te = s.Te * 8.617385e-05;
-    for (int e = 0; e < a.NE; ++e)
-        for (int i = 0; i <= MOC_MAXS; ++i) { s.cI1[e][i] = 0.0; s.cI2[e][i] = 0.0; }
-    for (int e = 0; e < a.NE; ++e) {
+// Voronov collisional ionisation (:3552-3582), one (element, stage) per lane;
+// the reference leaves the loops at the first u > 80, so that entry and every
+// later one (in its loop order) stay zero
+__device__ void collion(const MocBalanceArgs& a, const Lds& L, double Te, int tid) {
+    const int NE = a.NE, NS = a.NS, S2 = L.S2;
+    const double te = Te * 8.617385e-05;
+    for (int k = tid; k < NE * S2; k += 64) { L.cI1[k] = 0.0; L.cI2[k] = 0.0; }
+    if (tid == 0) *L.hot = NE * S2;
+    __syncthreads();
+    for (int k = tid; k < NE * S2; k += 64) {
+        const int e = k / S2, ion = k % S2;
         const int z = a.elemZ[e];
-        for (int ion = 1; ion <= imin(a.NS - 1, z - 1); ++ion) {
-            const int el = z - ion + 1;
-            const double u = t3(a.cion_cf, 0, z, el) / te;
-            if (u > 80.0) return;
-            s.cI1[e][ion] = t3(a.cion_cf, 2, z, el) * (1.0 + t3(a.cion_cf, 1, z, el) * sqrt(u)) /
-                            (t3(a.cion_cf, 3, z, el) + u) * pow(u, t3(a.cion_cf, 4, z, el)) * exp(-u);
-            s.cI2[e][ion] = a.cion_th[z * 32 + ion] * s.cI1[e][ion] * kBoltz;
+        if (ion < 1 || ion > imin(NS - 1, z - 1)) continue;
+        const int el = z - ion + 1;
+        const double u = t3(a.cion_cf, 0, z, el) / te;
+        if (u > 80.0) {
+            atomicMin(L.hot, k);
+            continue;
         }
+        const double c = t3(a.cion_cf, 2, z, el) * (1.0 + t3(a.cion_cf, 1, z, el) * sqrt(u)) /
+                         (t3(a.cion_cf, 3, z, el) + u) * pow(u, t3(a.cion_cf, 4, z, el)) * exp(-u);
+        L.cI1[k] = c;
+        L.cI2[k] = a.cion_th[z * 32 + ion] * c * kBoltz;
     }
+    __syncthreads();
+    const int hot = *L.hot;
+    for (int k = tid; k < NE * S2; k += 64)
+        if (k >= hot) { L.cI1[k] = 0.0; L.cI2[k] = 0.0; }
+    __syncthreads();
 }
 
 // charge-exchange coefficient [cm^3/s]; T-scaled unless raw (ionBalance keeps the
@@ -203,7 +229,7 @@ __device__ __forceinline__ double dEk(int z, int ion) {
     return (z == 7 && ion == 1) ? 10863. : ((z == 8 && ion == 1) ? 2205. : 0.);
 }
 
-// photoionisation rate of species (e, ion): shells rates from k_rates, the
+// photoionisation rates of the cell from k_rates (per shell), with the
 // reverse charge exchange added to the outer shell (:3715-3830)
 struct Rates {
     const double* R;
@@ -213,134 +239,160 @@ struct Rates {
     __device__ double heat(int s, double heatef) const { return at(3 * s + 1) + heatef * at(3 * s + 2); }
 };
 
-__device__ double rev_chex(const MocBalanceArgs& a, const CellState& s, int z, int ion) {
+__device__ double rev_chex(const MocBalanceArgs& a, const Lds& L, double Te, double H, int z, int ion) {
     if (!(dEk(z, ion) > 1.)) return 0.0;
     const int nel = z - ion + 1;
     const double g0g1 = (double)a.statW[z * 32 + nel] / (double)a.statW1[z * 32 + nel];
-    return chex(a, z, ion, s.Te, true) * (1. - s.X[a.elemSlot[1]][1]) * s.H * 2. * exp(-dEk(z, ion) / s.Te) / g0g1;
+    return chex(a, z, ion, Te, true) * (1. - L.x(a.elemSlot[1], 1)) * H * 2. * exp(-dEk(z, ion) / Te) / g0g1;
 }
 
-// H I / He I / He II have one shell; out = photoionisations of the ion
 __device__ double gamma_out(const MocBalanceArgs& a, const Rates& r, int spec) {
     double g = 0.0;
     for (int sh = a.specShell[spec]; sh < a.specShell[spec + 1]; ++sh) g += r.gamma(sh);
     return g;
 }
 
-__device__ bool ion_balance(const MocBalanceArgs& a, const Rates& r, CellState& s,
-                            const double revFrozen[2]) {
-    const int NS = a.NS;
+// H or He ratio chain (:1715-1790) on the lane that owns element zz
+__device__ void hhe_chain(const MocBalanceArgs& a, const Rates& r, const Lds& L, int zz, double Te,
+                          double Ne, double H, double H0, double collIonH) {
+    const int NS = a.NS, e = a.elemSlot[zz];
+    double ratio[3] = {0., 0., 0.};
+    for (int ion = 1; ion <= imin(zz, NS - 1); ++ion) {
+        const int spec = a.specOf[e * 32 + ion];
+        const double cx = chex(a, zz, ion, Te, true);
+        if (zz > 1 || ion > 1) collIonH = 0.;
+        const double g = (spec >= 0 ? gamma_out(a, r, spec) : 0.0) + 1.e-20;
+        ratio[ion] = (g + collIonH * Ne) / (Ne * L.al(e, ion) + cx * H * H0);
+    }
+    double prod[3] = {1., 1., 1.}, den = 1.;
+    for (int ion = 1; ion <= imin(zz, NS - 1); ++ion) {
+        for (int i = 1; i <= ion; ++i) prod[ion] *= ratio[i];
+        den += prod[ion];
+    }
+    for (int ion = 1; ion <= imin(zz + 1, NS); ++ion) {
+        double v = ion == 1 ? 1. / den : prod[ion - 1] / den;
+        v = v > kXMax ? kXMax : v;
+        v = v < 1.e-20 ? 1.e-20 : v;
+        L.x(e, ion) = v;
+    }
+}
+
+// bidiagonal chain of one heavy element with Auger spreading (:1826-1929)
+__device__ void heavy_chain(const MocBalanceArgs& a, const Rates& r, const Lds& L, int e, double Te,
+                            double Ne, double H, int nIterX, const double revFrozen[2]) {
+    const int NS = a.NS, z = a.elemZ[e];
+    double* out = L.out + e * L.S2;
+    double* in = L.in + e * L.S2;
+    double* mat = L.mat + e * L.S2;
+    for (int i = 0; i < L.S2; ++i) { out[i] = 0.; in[i] = 0.; }
+    const int top = imin(NS, z + 1);
+    for (int ion = 1; ion <= imin(z, NS - 1); ++ion) {
+        out[ion] += L.c1(e, ion) * Ne;
+        const int spec = a.specOf[e * 32 + ion];
+        if (spec < 0) continue;
+        for (int sh = a.specShell[spec]; sh < a.specShell[spec + 1]; ++sh) {
+            const int shn = a.shellNum[sh];
+            double g = r.gamma(sh);
+            if (shn == a.outShell[z * 32 + (z - ion + 1)] && ion == 1 && (z == 7 || z == 8))
+                g += revFrozen[z - 7];
+            out[ion] += g;
+            const int na = a.nauger[(z * 32 + ion) * 11 + shn];
+            for (int ne = 2; ne <= na; ++ne) {
+                const int mx = imin(ion + ne - 1, imin(NS - 1, z));
+                const double ratio = L.x(e, mx) > 1e-30 ? L.x(e, ion) / L.x(e, mx) : 1.;
+                out[mx] += g * a.auger[((z * 32 + ion) * 11 + shn) * 11 + ne] * ratio;
+            }
+        }
+    }
+    const int eH = a.elemSlot[1];
+    for (int ion = 1; ion <= imin(z, NS - 1); ++ion)
+        in[ion] = Ne * L.al(e, ion) + chex(a, z, ion, Te, false) * H * L.x(eH, 1);
+    mat[1] = 1.;
+    double sum = 1.;
+    for (int ion = 2; ion <= top; ++ion) {
+        mat[ion] = mat[ion - 1] * out[ion - 1] / in[ion - 1];
+        sum += mat[ion];
+    }
+    sum = 1. / sum;
+    L.x(e, top) = mat[top] * sum;
+    for (int ion = 1; ion <= imin(z, NS - 1); ++ion) {
+        double v;
+        if (nIterX < 2 && mat[ion] * sum > 0.) {
+            const double old = L.x(e, ion) <= 0. ? 1.e-5 : L.x(e, ion);
+            v = pow(10., 0.5 * (log10(old) + log10(mat[ion] * sum)));
+        } else {
+            v = mat[ion] * sum;
+        }
+        v = v > kXMax ? kXMax : v;
+        v = v < 1.e-20 ? 1.e-20 : v;
+        L.x(e, ion) = v;
+    }
+}
+
+// ionBalance (:1575-1992); returns false when 15 passes did not settle H/He
+__device__ bool ion_balance(const MocBalanceArgs& a, const Rates& r, const Lds& L, double Te,
+                            double& Ne, double H, const double revFrozen[2], int tid) {
+    const int NS = a.NS, NE = a.NE;
     const int eH = a.elemSlot[1], eHe = a.elemSlot[2];
+    int nIterX = 1;
     bool conv = true;
     for (;;) {
-        double expFact = 157893.94 / s.Te;
+        double expFact = 157893.94 / Te;
         if (expFact > 75.) expFact = 75.;
-        double collIonH = 2.75E-16 * s.Te * sqrt(s.Te) * (157893.94 / s.Te + 2.) * exp(-expFact);
-        const double HIOld = s.X[eH][1];
-        const double HeIOld = eHe >= 0 ? s.X[eHe][1] : 0.0;
-        const double HeIIOld = eHe >= 0 ? s.X[eHe][2] : 0.0;
-        const double H0 = s.X[eH][1];
-        // H and He ratio chains (:1715-1790)
-        for (int zz = 1; zz <= 2; ++zz) {
-            const int e = a.elemSlot[zz];
-            if (e < 0) continue;
-            double ratio[3] = {0., 0., 0.};
-            for (int ion = 1; ion <= imin(zz, NS - 1); ++ion) {
-                const int spec = a.specOf[e * 32 + ion];
-                const double cx = chex(a, zz, ion, s.Te, true);
-                if (zz > 1 || ion > 1) collIonH = 0.;
-                const double g = (spec >= 0 ? gamma_out(a, r, spec) : 0.0) + 1.e-20;
-                ratio[ion] = (g + collIonH * s.Ne) / (s.Ne * s.alpha[e][ion] + cx * s.H * H0);
-            }
-            double prod[3] = {1., 1., 1.}, den = 1.;
-            for (int ion = 1; ion <= imin(zz, NS - 1); ++ion) {
-                for (int i = 1; i <= ion; ++i) prod[ion] *= ratio[i];
-                den += prod[ion];
-            }
-            for (int ion = 1; ion <= imin(zz + 1, NS); ++ion) {
-                double v = ion == 1 ? 1. / den : prod[ion - 1] / den;
-                v = v > kXMax ? kXMax : v;
-                v = v < 1.e-20 ? 1.e-20 : v;
-                s.X[e][ion] = v;
-            }
-        }
-        // heavy elements (:1826-1929)
-        for (int e = 0; e < a.NE; ++e) {
-            const int z = a.elemZ[e];
-            if (z < 3) continue;
-            double out[MOC_MAXS + 1], in[MOC_MAXS + 1];
-            for (int i = 0; i <= MOC_MAXS; ++i) { out[i] = 0.; in[i] = 0.; }
-            const int top = imin(NS, z + 1);
-            for (int ion = 1; ion <= imin(z, NS - 1); ++ion) {
-                out[ion] += s.cI1[e][ion] * s.Ne;
-                const int spec = a.specOf[e * 32 + ion];
-                if (spec < 0) continue;
-                for (int sh = a.specShell[spec]; sh < a.specShell[spec + 1]; ++sh) {
-                    const int shn = a.shellNum[sh];
-                    double g = r.gamma(sh);
-                    if (shn == a.outShell[z * 32 + (z - ion + 1)] && ion == 1 && (z == 7 || z == 8))
-                        g += revFrozen[z - 7];
-                    out[ion] += g;
-                    const int na = a.nauger[(z * 32 + ion) * 11 + shn];
-                    for (int ne = 2; ne <= na; ++ne) {
-                        const int mx = imin(ion + ne - 1, imin(NS - 1, z));
-                        const double ratio = s.X[e][mx] > 1e-30 ? s.X[e][ion] / s.X[e][mx] : 1.;
-                        out[mx] += g * a.auger[((z * 32 + ion) * 11 + shn) * 11 + ne] * ratio;
-                    }
-                }
-            }
-            for (int ion = 1; ion <= imin(z, NS - 1); ++ion)
-                in[ion] = s.Ne * s.alpha[e][ion] + chex(a, z, ion, s.Te, false) * s.H * s.X[eH][1];
-            double mat[MOC_MAXS + 2];
-            mat[1] = 1.;
-            double sum = 1.;
-            for (int ion = 2; ion <= top; ++ion) {
-                mat[ion] = mat[ion - 1] * out[ion - 1] / in[ion - 1];
-                sum += mat[ion];
-            }
-            sum = 1. / sum;
-            s.X[e][top] = mat[top] * sum;
-            for (int ion = 1; ion <= imin(z, NS - 1); ++ion) {
-                double v;
-                if (s.nIterX < 2 && mat[ion] * sum > 0.) {
-                    const double old = s.X[e][ion] <= 0. ? 1.e-5 : s.X[e][ion];
-                    v = pow(10., 0.5 * (log10(old) + log10(mat[ion] * sum)));
-                } else {
-                    v = mat[ion] * sum;
-                }
-                v = v > kXMax ? kXMax : v;
-                v = v < 1.e-20 ? 1.e-20 : v;
-                s.X[e][ion] = v;
-            }
-        }
-        double Ne = 0.;
-        for (int e = 0; e < a.NE; ++e) {
+        const double collIonH = 2.75E-16 * Te * sqrt(Te) * (157893.94 / Te + 2.) * exp(-expFact);
+        const double HIOld = L.x(eH, 1);
+        const double HeIOld = eHe >= 0 ? L.x(eHe, 1) : 0.0;
+        const double HeIIOld = eHe >= 0 ? L.x(eHe, 2) : 0.0;
+        const double H0 = L.x(eH, 1);
+        __syncthreads();
+        if (tid == 0) hhe_chain(a, r, L, 1, Te, Ne, H, H0, collIonH);
+        if (tid == 1 && eHe >= 0) hhe_chain(a, r, L, 2, Te, Ne, H, H0, collIonH);
+        __syncthreads();
+        for (int e = tid; e < NE; e += 64)
+            if (a.elemZ[e] >= 3) heavy_chain(a, r, L, e, Te, Ne, H, nIterX, revFrozen);
+        __syncthreads();
+        double ne = 0.;
+        for (int e = 0; e < NE; ++e) {
             const int z = a.elemZ[e];
             for (int ion = 2; ion <= imin(z + 1, NS); ++ion)
-                if (s.X[e][ion] >= 1.e-10) Ne += (ion - 1) * a.abund[e] * s.X[e][ion];
+                if (L.x(e, ion) >= 1.e-10) ne += (ion - 1) * a.abund[e] * L.x(e, ion);
         }
-        Ne *= s.H;
-        s.Ne = Ne == 0. ? 1. : Ne;
-        const double dHI = (s.X[eH][1] - HIOld) / HIOld;
-        const double dHeI = eHe >= 0 ? (s.X[eHe][1] - HeIOld) / HeIOld : 0.;
-        const double dHeII = eHe >= 0 ? (s.X[eHe][2] - HeIIOld) / HeIIOld : 0.;
+        ne *= H;
+        Ne = ne == 0. ? 1. : ne;
+        const double dHI = (L.x(eH, 1) - HIOld) / HIOld;
+        const double dHeI = eHe >= 0 ? (L.x(eHe, 1) - HeIOld) / HeIOld : 0.;
+        const double dHeII = eHe >= 0 ? (L.x(eHe, 2) - HeIIOld) / HeIIOld : 0.;
         const bool big = fabs(dHI) > 0.01 || fabs(dHeI) > 0.01 || fabs(dHeII) > 0.01;
-        if (big && s.nIterX < 15) { ++s.nIterX; continue; }
-        if (big && s.nIterX == 15) conv = false;
+        if (big && nIterX < 15) { ++nIterX; continue; }
+        if (big && nIterX == 15) conv = false;
         return conv;
     }
 }
 
-// forbidden-line cooling of one n-level ion [erg/s per n_H]
-__device__ void therm(const MocBalanceArgs& a, const Rates& r, const CellState& s, double comp[4],
-                      double* heat, double* cool) {
-    const double T = s.Te, Ne = s.Ne;
+// n-level cooling of one ion, register-resident solve when the level count fits NL
+template <int NL>
+__device__ double ion_cooling(const MocBalanceArgs& a, int atom, double T, double Ne, double scale) {
+    if constexpr (NL <= 8) return nlevel_cool_fixed<NL>(a, atom, T, Ne, scale, 1.9865e-16);
+    else return nlevel_lines(a, atom, T, Ne, 0., false, scale, 1.9865e-16);
+}
+
+// thermBalance (:887-1215): heating and cooling [erg/s per n_H]
+template <int NL>
+__device__ void therm(const MocBalanceArgs& a, const Rates& r, const Lds& L, double T, double Ne,
+                      double H, double heatef, const double comp[4], double* heat, double* cool,
+                      int tid) {
+    // forbidden-line cooling, one ion with data per lane
+    for (int k = tid; k < a.nNlv; k += 64) {
+        const int e = a.nlvSlot[k], ion = a.nlvIon[k];
+        L.cool[k] = ion_cooling<NL>(a, a.nlvAtom[k], T, Ne, a.abund[e] * L.x(e, ion));
+    }
+    __syncthreads();
     double coolFF = 0., coolRec = 0., coolColl, coolCollH = 0., heatSte = 0., Te4 = T / 1.e4;
     const double sq = kBoltz * T / sqrt(T);
     for (int e = 0; e < a.NE; ++e) {  // H-like recombination, elements in ascending Z
         const int z = a.elemZ[e];
         if (!(a.NS > z)) continue;
-        const double Np = s.X[e][z + 1] * a.abund[e];
+        const double Np = L.x(e, z + 1) * a.abund[e];
         const double lt = log10(T / (double)(z * z));
         Te4 = T / (z * z * 1.e4);
         const double bFF = z * (1.0108464E-11 + 9.7930778E-13 * lt - 6.6433144E-13 * lt * lt +
@@ -354,7 +406,7 @@ __device__ void therm(const MocBalanceArgs& a, const Rates& r, const CellState& 
     for (int e = 0; e < a.NE; ++e) {  // He-like
         const int z = a.elemZ[e];
         if (z < 2 || !(a.NS > z)) continue;
-        const double Np = s.X[e][z] * a.abund[e];
+        const double Np = L.x(e, z) * a.abund[e];
         const double lt = log10(4. * T / (double)(z * z));
         Te4 = 4. * T / (z * z * 1.e4);
         const double bFF = (z / 2.) * (1.070073e-11 - 2.5730207e-13 * lt + 2.109134e-13 * lt * lt);
@@ -366,120 +418,134 @@ __device__ void therm(const MocBalanceArgs& a, const Rates& r, const CellState& 
     const int eH = a.elemSlot[1];
     // H Ly-alpha/beta excitation, with Te4 as the loops above left it
     coolColl = T > 5000. ? (2.47e-8 * exp(-118338. / T) * pow(Te4, -0.228) +
-                            1.32e-8 * exp(-140252. / T) * pow(Te4, -0.460)) * kHcRyd * s.X[eH][1] * Ne
+                            1.32e-8 * exp(-140252. / T) * pow(Te4, -0.460)) * kHcRyd * L.x(eH, 1) * Ne
                          : 0.;
     for (int e = 0; e < a.NE; ++e) {
         const int z = a.elemZ[e];
-        for (int ion = 1; ion <= imin(z, a.NS - 1); ++ion) coolCollH += a.abund[e] * s.X[e][ion] * s.cI2[e][ion] * Ne;
+        for (int ion = 1; ion <= imin(z, a.NS - 1); ++ion) coolCollH += a.abund[e] * L.x(e, ion) * L.c2(e, ion) * Ne;
     }
-    for (int e = 0; e < a.NE; ++e) {
-        const int z = a.elemZ[e];
-        if (z < 3) continue;
-        for (int ion = 1; ion <= imin(z + 1, a.NS); ++ion) {
-            const int at = a.atomIdx[z * 32 + ion];
-            if (at >= 0) coolColl += nlevel_lines(a, at, T, Ne, 0., false, a.abund[e] * s.X[e][ion], 1.9865e-16);
-        }
-    }
+    for (int k = 0; k < a.nNlv; ++k) coolColl += L.cool[k];
     for (int sp = 0; sp < a.nSpec; ++sp) {
         const int e = a.specElem[sp], ion = a.specIon[sp];
         const int z = a.elemZ[e];
         if (ion > imin(z, a.NS - 1)) continue;
         for (int sh = a.specShell[sp]; sh < a.specShell[sp + 1]; ++sh)
-            heatSte += r.heat(sh, z < 3 ? 1.0 : s.heatef) * s.X[e][ion] * a.abund[e];
+            heatSte += r.heat(sh, z < 3 ? 1.0 : heatef) * L.x(e, ion) * a.abund[e];
     }
     *heat = heatSte;
     *cool = coolFF + coolRec + coolColl + coolCollH;
     if (a.lgCompton) {
-        const double cHeat = (6.65e-25 * comp[0] * kRyd2erg / 8.184e-7) / s.H;
-        const double cCool = (6.65e-25 * 6.75e-10 * T * comp[1]) / s.H;
+        const double cHeat = (6.65e-25 * comp[0] * kRyd2erg / 8.184e-7) / H;
+        const double cCool = (6.65e-25 * 6.75e-10 * T * comp[1]) / H;
         *heat += cHeat * Ne + comp[2];
         *cool += cCool * Ne;
     }
+    __syncthreads();  // L.cool is rewritten by the next call
 }
 
+template <int NL>
 __global__ void __launch_bounds__(64) k_balance(MocBalanceArgs a) {
-    const int cell = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    if (cell > a.nCells) return;
+    extern __shared__ double lds[];
+    const int cell = blockIdx.x + 1;
+    const int tid = threadIdx.x;
     const int C1 = a.nCells + 1;
-    a.lgBlack[cell] = 0;
+    const int NE = a.NE, NS = a.NS, S2 = NS + 2;
     Rates r{a.R, C1, cell, a.nShells};
     if (!(r.at(a.colHit) > 0.)) {  // no packet reached the cell (lgHit, :127-171)
-        a.lgBlack[cell] = 1;
-        atomicAdd(&a.counters[0], 1ull);
-        a.nIterT[cell] = 0;
+        if (tid == 0) {
+            a.lgBlack[cell] = 1;
+            atomicAdd(&a.counters[0], 1ull);
+            a.nIterT[cell] = 0;
+        }
         return;
     }
-    CellState s;
-    s.Te = a.Te[cell];
-    s.Ne = a.Ne[cell];
-    s.H = a.Hden[cell];
-    for (int e = 0; e < a.NE; ++e) {
-        s.X[e][0] = 0.;
-        for (int ion = 1; ion <= a.NS; ++ion) s.X[e][ion] = a.ionDen[((size_t)(ion - 1) * a.NE + e) * C1 + cell];
+    Lds L;
+    L.S2 = S2;
+    L.X = lds;
+    L.alpha = L.X + NE * S2;
+    L.cI1 = L.alpha + NE * S2;
+    L.cI2 = L.cI1 + NE * S2;
+    L.out = L.cI2 + NE * S2;
+    L.in = L.out + NE * S2;
+    L.mat = L.in + NE * S2;
+    L.cool = L.mat + NE * S2;
+    L.diO = L.cool + (a.nNlv > 0 ? a.nNlv : 1);
+    L.hot = (int*)(L.diO + MOC_MAXS + 1);
+    for (int k = tid; k < NE * S2; k += 64) {
+        const int e = k / S2, ion = k % S2;
+        L.X[k] = (ion >= 1 && ion <= NS) ? a.ionDen[((size_t)(ion - 1) * NE + e) * C1 + cell] : 0.0;
     }
+    __syncthreads();
+    double Te = a.Te[cell], Ne = a.Ne[cell];
+    const double H = a.Hden[cell];
     const int eH = a.elemSlot[1], eHe = a.elemSlot[2];
-    const double XOld = s.X[eH][1];
+    const double XOld = L.x(eH, 1);
     double Thigh = 0., Tlow = 0., rHigh = 0., rLow = 0.;
     int nT = 0;
-    bool ib = true;
+    bool ib = true, noTe = false;
     for (;;) {
         ++nT;
-        recomb(a, s);
+        recomb(a, L, Te, tid);
         // secondary-ionisation heating efficiency (photoionisation :3605-3627)
-        double xe = 0.;
-        for (int e = 0; e < a.NE; ++e) xe += s.X[e][1] * a.abund[e] * s.H;
-        xe = s.Ne / xe;
-        if (xe > 0.95) s.heatef = 1.;
+        double xe = 0., heatef;
+        for (int e = 0; e < NE; ++e) xe += L.x(e, 1) * a.abund[e] * H;
+        xe = Ne / xe;
+        if (xe > 0.95) heatef = 1.;
         else {
             xe = xe < 1.e-4 ? 1.e-4 : xe;
-            s.heatef = 0.9971 * (1. - pow(1. - pow(xe, 0.2663), 1.3163));
+            heatef = 0.9971 * (1. - pow(1. - pow(xe, 0.2663), 1.3163));
         }
-        collion(a, s);
-        s.nIterX = 1;
+        collion(a, L, Te, tid);
         double comp[4] = {0., 0., 0., 0.};
         if (a.lgCompton) {
             comp[0] = r.at(a.colComp + 0);
             comp[1] = r.at(a.colComp + 1);
-            comp[2] = (r.at(a.colComp + 2) * s.X[eH][1] * a.abund[eH] +
-                       (eHe >= 0 ? r.at(a.colComp + 3) * s.X[eHe][1] * a.abund[eHe] * 2. : 0.)) * kRyd2erg;
+            comp[2] = (r.at(a.colComp + 2) * L.x(eH, 1) * a.abund[eH] +
+                       (eHe >= 0 ? r.at(a.colComp + 3) * L.x(eHe, 1) * a.abund[eHe] * 2. : 0.)) * kRyd2erg;
         }
         double revFrozen[2] = {0., 0.};
-        if (a.elemSlot[7] >= 0) revFrozen[0] = rev_chex(a, s, 7, 1);
-        if (a.elemSlot[8] >= 0) revFrozen[1] = rev_chex(a, s, 8, 1);
-        ib = ion_balance(a, r, s, revFrozen);
+        if (a.elemSlot[7] >= 0) revFrozen[0] = rev_chex(a, L, Te, H, 7, 1);
+        if (a.elemSlot[8] >= 0) revFrozen[1] = rev_chex(a, L, Te, H, 8, 1);
+        ib = ion_balance(a, r, L, Te, Ne, H, revFrozen, tid);
         double heat, cool;
-        therm(a, r, s, comp, &heat, &cool);
+        therm<NL>(a, r, L, Te, Ne, H, heatef, comp, &heat, &cool, tid);
         const double res = heat - cool;
         if (fabs(res) >= 0.02 * (fabs(cool) + fabs(heat)) / 2.) {
             if (nT < 20) {
                 if (res < 0.) {
                     rLow = res;
-                    Tlow = s.Te;
-                    s.Te = Thigh != 0. ? Tlow - (Thigh - Tlow) * rLow / (rHigh - rLow) : s.Te / 1.2;
+                    Tlow = Te;
+                    Te = Thigh != 0. ? Tlow - (Thigh - Tlow) * rLow / (rHigh - rLow) : Te / 1.2;
                 } else {
                     rHigh = res;
-                    Thigh = s.Te;
-                    s.Te = Tlow != 0. ? Tlow - (Thigh - Tlow) * rLow / (rHigh - rLow) : s.Te * 1.2;
+                    Thigh = Te;
+                    Te = Tlow != 0. ? Tlow - (Thigh - Tlow) * rLow / (rHigh - rLow) : Te * 1.2;
                 }
                 if (nT >= 6 && fabs(Thigh - Tlow) <= 0.002 * (Thigh + Tlow)) {
-                    if (res < 0.) { s.Te = Thigh - 100.; Thigh = 0.; }
-                    else { s.Te = Tlow + 100.; Tlow = 0.; }
+                    if (res < 0.) { Te = Thigh - 100.; Thigh = 0.; }
+                    else { Te = Tlow + 100.; Tlow = 0.; }
                 }
-                if (s.Te <= 0.) s.Te = 1.;
+                if (Te <= 0.) Te = 1.;
                 continue;
             }
-            s.Te = Thigh == 0. ? Tlow : (Tlow == 0. ? Thigh : (Thigh + Tlow) * 0.5);
-            atomicAdd(&a.counters[2], 1ull);
+            Te = Thigh == 0. ? Tlow : (Tlow == 0. ? Thigh : (Thigh + Tlow) * 0.5);
+            noTe = true;
         }
         break;
     }
-    if (!ib) atomicAdd(&a.counters[1], 1ull);
-    a.Te[cell] = s.Te;
-    a.Ne[cell] = s.Ne;
-    for (int e = 0; e < a.NE; ++e)
-        for (int ion = 1; ion <= a.NS; ++ion) a.ionDen[((size_t)(ion - 1) * a.NE + e) * C1 + cell] = s.X[e][ion];
-    a.lgConverged[cell] = fabs((s.X[eH][1] - XOld) / XOld) <= a.XHILimit ? 1 : 0;
-    a.nIterT[cell] = nT;
+    for (int k = tid; k < NE * S2; k += 64) {
+        const int e = k / S2, ion = k % S2;
+        if (ion >= 1 && ion <= NS) a.ionDen[((size_t)(ion - 1) * NE + e) * C1 + cell] = L.X[k];
+    }
+    if (tid == 0) {
+        if (noTe) atomicAdd(&a.counters[2], 1ull);
+        if (!ib) atomicAdd(&a.counters[1], 1ull);
+        a.lgBlack[cell] = 0;
+        a.Te[cell] = Te;
+        a.Ne[cell] = Ne;
+        a.lgConverged[cell] = fabs((L.x(eH, 1) - XOld) / XOld) <= a.XHILimit ? 1 : 0;
+        a.nIterT[cell] = nT;
+    }
 }
 
 }  // namespace
@@ -489,7 +555,15 @@ int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a, const double* J, c
     const int C1 = a.nCells + 1;
     dim3 g1((unsigned)((C1 + RC - 1) / RC), (unsigned)((ncol + RN - 1) / RN));
     hipLaunchKernelGGL(k_rates, g1, dim3(256), 0, ctx->stream, J, C1, ctx->nbins, W, ncol, R);
-    hipLaunchKernelGGL(k_balance, dim3((unsigned)((a.nCells + 63) / 64)), dim3(64), 0, ctx->stream, a);
+    const size_t lds = balance_lds_bytes(a.NE, a.NS, a.nNlv);
+    if (a.nCells > 0) {
+        if (a.maxLev <= 5)
+            hipLaunchKernelGGL(k_balance<5>, dim3((unsigned)a.nCells), dim3(64), lds, ctx->stream, a);
+        else if (a.maxLev <= 8)
+            hipLaunchKernelGGL(k_balance<8>, dim3((unsigned)a.nCells), dim3(64), lds, ctx->stream, a);
+        else
+            hipLaunchKernelGGL(k_balance<MOC_MAXLEV>, dim3((unsigned)a.nCells), dim3(64), lds, ctx->stream, a);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         ctx->err = std::string("balance launch: ") + hipGetErrorString(e);

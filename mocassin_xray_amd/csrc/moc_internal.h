@@ -39,6 +39,7 @@ struct MocTransportArgs {
     const T* vy;
     const T* vz;
     const int* active;  // [nx*ny*nz], x fastest
+    int dense;          // all cells active, id = 1 + iz + nz*(iy + ny*ix)
     T R_out;
     // mesh & tables
     int nbins, nAngles, lgCompton;
@@ -135,6 +136,10 @@ struct MocBalanceArgs {
     const int* atomRecKind;   // [nAtoms] recombination feed: 0 none, 1 fit per level, 2 log-T cubic
     const double* atomRec;    // [levels][11] per-level recombination coefficients
     const int* atomRecValid;  // [levels]
+    int nNlv;                 // ions with n-level data, in (element slot, stage) order
+    const int* nlvSlot;
+    const int* nlvIon;
+    const int* nlvAtom;
     int colHit, colComp;      // rate columns of the hit test and the Compton sums
     const double* R;          // [ncol][C1] rates
     double* Te;
@@ -221,6 +226,7 @@ struct MocEmission {
 struct MocGrid {
     bool attached = false;
     int nx = 0, ny = 0, nz = 0, nCells = 0, sym = 0;
+    bool dense = false;   // every cell active, numbered z fastest, then y, then x
     double R_out = 0.0;
     std::vector<double> ax, ay, az;   // host copies
     std::vector<int> active;

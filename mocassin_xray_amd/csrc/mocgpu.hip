@@ -5,6 +5,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -324,6 +325,17 @@ int moc_grid_attach(moc_ctx* ctx, int ig, const moc_grid_desc* gd) {
     g.az.assign(gd->zAxis, gd->zAxis + g.nz);
     const size_t ncell3 = (size_t)g.nx * g.ny * g.nz;
     g.active.assign(gd->active, gd->active + ncell3);
+    // every cell active and numbered with z fastest, then y, then x: the id is
+    // computed instead of looked up in the transport
+    g.dense = true;
+    for (int i = 0; i < g.nx && g.dense; ++i)
+        for (int j = 0; j < g.ny && g.dense; ++j)
+            for (int k = 0; k < g.nz; ++k)
+                if (g.active[(size_t)i + (size_t)g.nx * ((size_t)j + (size_t)g.ny * k)] !=
+                    1 + k + g.nz * (j + g.ny * i)) {
+                    g.dense = false;
+                    break;
+                }
     for (size_t i = 0; i < ncell3; ++i)
         if (g.active[i] < 0 || g.active[i] > g.nCells) {
             ctx->err = "moc_grid_attach: active ids must lie in [0, nCells] (no sub-grids)";
@@ -602,7 +614,7 @@ int moc_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ct
         MocTransportArgs<double> a;
         a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nCells = g.nCells; a.sym = g.sym;
         a.ax = g.d_ax; a.ay = g.d_ay; a.az = g.d_az; a.vx = g.d_vx; a.vy = g.d_vy; a.vz = g.d_vz;
-        a.active = g.d_active; a.R_out = g.R_out;
+        a.active = g.d_active; a.dense = g.dense ? 1 : 0; a.R_out = g.R_out;
         a.nbins = ctx->nbins; a.nAngles = ctx->nAngles; a.lgCompton = ctx->lgCompton;
         a.ionEdge1 = ctx->ionEdge1; a.nuArray = ctx->d_nu; a.KNsigmaT = ctx->d_KNsigmaT;
         a.KNcdf = ctx->d_KNcdf; a.Pcomp = ctx->d_Pcomp;
@@ -617,7 +629,7 @@ int moc_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ct
         MocTransportArgs<float> a;
         a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nCells = g.nCells; a.sym = g.sym;
         a.ax = g.f_ax; a.ay = g.f_ay; a.az = g.f_az; a.vx = g.f_vx; a.vy = g.f_vy; a.vz = g.f_vz;
-        a.active = g.d_active; a.R_out = (float)g.R_out;
+        a.active = g.d_active; a.dense = g.dense ? 1 : 0; a.R_out = (float)g.R_out;
         a.nbins = ctx->nbins; a.nAngles = ctx->nAngles; a.lgCompton = ctx->lgCompton;
         a.ionEdge1 = (float)ctx->ionEdge1; a.nuArray = ctx->f_nu; a.KNsigmaT = ctx->f_KNsigmaT;
         a.KNcdf = ctx->f_KNcdf; a.Pcomp = ctx->f_Pcomp;
@@ -715,6 +727,23 @@ int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* t) {
     PUT(a.chexIdx, chexIdx.data(), 31 * 32, "chexIdx");
     PUT(a.chexTab, t->chexFit, 4 * (size_t)t->nChex, "chexFit");
     PUT(a.atomIdx, t->atomIdx, 31 * 32, "atomIdx");
+    {
+        // the ions whose n-level cooling thermBalance sums, in its order (:1065-1093)
+        std::vector<int> ns, ni, na;
+        for (int e = 0; e < at.nElemUsed; ++e) {
+            const int z = at.elemZ[e];
+            if (z < 3) continue;
+            for (int ion = 1; ion <= std::min(z + 1, at.nstages); ++ion) {
+                const int k = t->atomIdx[z * 32 + ion];
+                if (k < 0) continue;
+                ns.push_back(e); ni.push_back(ion); na.push_back(k);
+            }
+        }
+        a.nNlv = (int)na.size();
+        PUT(a.nlvSlot, ns.data(), ns.size(), "nlvSlot");
+        PUT(a.nlvIon, ni.data(), ni.size(), "nlvIon");
+        PUT(a.nlvAtom, na.data(), na.size(), "nlvAtom");
+    }
     b.atomIdx.assign(t->atomIdx, t->atomIdx + 31 * 32);
     PUT(a.atomMeta, t->atomMeta, 8 * (size_t)t->nAtoms, "atomMeta");
     PUT(a.atomAtp, t->atomAtp, t->nAtoms, "atomAtp");

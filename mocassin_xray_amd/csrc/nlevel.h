@@ -126,3 +126,130 @@ __device__ double nlevel_lines(const MocBalanceArgs& a, int atom, double Te, dou
 }
 
 }  // namespace
+
+namespace {
+
+// The same cooling sum (rec = false) with the level matrix held in registers:
+// NL is a compile-time bound on the level count (n <= NL), every loop is
+// unrolled over NL and predicated on n, and the collision rates of a pair are
+// recomputed where a row needs them instead of being stored.  The arithmetic
+// on the valid entries is the generic routine's, operation for operation.
+template <int NL>
+__device__ double nlevel_cool_fixed(const MocBalanceArgs& a, int atom, double Te, double Ne,
+                                    double scale, double unit) {
+    const int64_t* m = a.atomMeta + 8 * atom;
+    const int nf = (int)m[0], nT = (int)m[1];
+    const double* lt = a.atomLogT + m[2];
+    const double* g = a.atomG + m[3];
+    const double* en = a.atomE + m[3];
+    const double* A = a.atomA + m[4];
+    const double* q = a.atomQom + m[5];
+    const double* q2 = a.atomQ2 + m[5];
+    const int iup2p = (int)m[6];
+    const double atp = a.atomAtp[atom];
+    const int n = nf < a.maxLev ? nf : a.maxLev;
+    const double sqrTe = sqrt(Te), lT = log10(Te);
+    int klo = 0, khi = nT - 1;
+    double sa = 0., sb = 0., h = 1.;
+    const bool below = lT <= lt[0], above = lT >= lt[nT - 1];
+    if (nT > 2 && !below && !above) {
+        while (khi - klo > 1) {
+            const int k = (khi + klo) >> 1;
+            if (lt[k] > lT) khi = k; else klo = k;
+        }
+        h = lt[khi] - lt[klo];
+        sa = (lt[khi] - lT) / h;
+        sb = (lT - lt[klo]) / h;
+    }
+    // effective rate coefficient qe[p][q] of the generic routine
+    auto qef = [&](int p, int r) -> double {
+        const int i = p < r ? p : r, j = p < r ? r : p;
+        const size_t pp = (size_t)i * nf + j;
+        double ups;
+        if (nT == 1) ups = q[pp];
+        else if (nT == 2) ups = q[pp] + (q[(size_t)nf * nf + pp] - q[pp]) / (lt[1] - lt[0]) * (lT - lt[0]);
+        else if (below) ups = q[pp];
+        else if (above) ups = q[(size_t)(nT - 1) * nf * nf + pp];
+        else {
+            const double ylo = q[(size_t)klo * nf * nf + pp], yhi = q[(size_t)khi * nf * nf + pp];
+            const double dlo = q2[(size_t)klo * nf * nf + pp], dhi = q2[(size_t)khi * nf * nf + pp];
+            ups = sa * ylo + sb * yhi + ((sa * sa * sa - sa) * dlo + (sb * sb * sb - sb) * dhi) * (h * h) / 6.;
+        }
+        const double c = 8.63e-6 * ups / sqrTe;
+        if (en[i] <= en[j]) return p == i ? c * exp((en[i] - en[j]) * 1.4388463 / Te) / g[i] : c / g[j];
+        return p == i ? c / g[j] : c * exp((en[j] - en[i]) * 1.4388463 / Te) / g[i];
+    };
+    double x[NL][NL], y[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        y[i] = 0.;
+#pragma unroll
+        for (int j = 0; j < NL; ++j) x[i][j] = 0.;
+    }
+#pragma unroll
+    for (int j = 0; j < NL; ++j) x[0][j] = 1.;
+    y[0] = 1.;
+#pragma unroll
+    for (int i = 1; i < NL; ++i) {
+        if (i >= n) break;
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+            if (j >= n) break;
+            if (j == i) continue;
+            x[i][j] += Ne * qef(j, i);
+            x[i][i] -= Ne * qef(i, j);
+            if (j > i) x[i][j] += A[(size_t)j * nf + i];
+            else x[i][i] -= A[(size_t)i * nf + j];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NL - 1; ++i) {
+        if (i >= n - 1) break;
+#pragma unroll
+        for (int k = i + 1; k < NL; ++k) {
+            if (k >= n) break;
+            const double f = x[k][i] / x[i][i];
+#pragma unroll
+            for (int j = i + 1; j < NL; ++j) {
+                if (j >= n) break;
+                x[k][j] -= x[i][j] * f;
+            }
+            y[k] -= y[i] * f;
+        }
+    }
+#pragma unroll
+    for (int k = NL - 1; k >= 0; --k) {
+        if (k >= n) continue;
+        if (k == n - 1) {
+            y[k] /= x[k][k];
+            continue;
+        }
+#pragma unroll
+        for (int j = k + 1; j < NL; ++j) {
+            if (j >= n) break;
+            y[k] -= y[j] * x[k][j];
+        }
+        y[k] /= x[k][k];
+    }
+    double sumN = 0.;
+#pragma unroll
+    for (int i = 0; i < NL; ++i)
+        if (i < n) sumN += y[i];
+    double tot = 0.;
+#pragma unroll
+    for (int i = 0; i < NL - 1; ++i) {
+        if (i >= n - 1) break;
+#pragma unroll
+        for (int j = i + 1; j < NL; ++j) {
+            if (j >= n) break;
+            double av = A[(size_t)j * nf + i];
+            if (av == 0.) continue;
+            if (j + 1 == iup2p && i == 0 && atp > 0.) av -= atp;
+            const double em = av * (en[j] - en[i]) * (y[j] / sumN) * scale * unit;
+            if (em > 1.e-35) tot += em;
+        }
+    }
+    return tot;
+}
+
+}  // namespace

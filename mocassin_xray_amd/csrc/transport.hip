@@ -141,8 +141,11 @@ __device__ __forceinline__ void nudge(T pos, T dir, int& idx, const T* __restric
     }
 }
 
+// active cell id of (ix, iy, iz); a fully active grid numbered by active_map
+// order needs no lookup, which takes one dependent memory access off a segment
 template <typename T>
 __device__ __forceinline__ int cell_of(const MocTransportArgs<T>& a, int ix, int iy, int iz) {
+    if (a.dense) return 1 + iz + a.nz * (iy + a.ny * ix);
     return a.active[(size_t)ix + (size_t)a.nx * ((size_t)iy + (size_t)a.ny * (size_t)iz)];
 }
 
@@ -244,6 +247,12 @@ __device__ void segment(const MocTransportArgs<T>& a, Pk<T>& p, MocRng& rng, Cnt
         if (p.y <= a.ay[0]) { if (p.v < T(0)) p.v = -p.v; p.y = a.ay[0]; }
         if (p.z <= a.az[0]) { if (p.w < T(0)) p.w = -p.w; p.z = a.az[0]; }
     }
+    // the opacity of the current cell is fetched before the wall distances are
+    // worked out; a wall snap below moves the packet to a neighbour (rarely) and
+    // the value is fetched again for that cell
+    const size_t stride = (size_t)(a.nCells + 1);
+    const int cell0 = cell_of(a, p.ix, p.iy, p.iz);
+    const T kap0 = a.opacity[(size_t)p.nu0 * stride + (size_t)cell0];
     T dSx = T(0), dSy = T(0), dSz = T(0);
     if (!wall_dist(p.x, p.u, p.ix, a.ax, nx, dSx) || !wall_dist(p.y, p.v, p.iy, a.ay, ny, dSy) ||
         !wall_dist(p.z, p.w, p.iz, a.az, nz, dSz)) {
@@ -269,8 +278,7 @@ __device__ void segment(const MocTransportArgs<T>& a, Pk<T>& p, MocRng& rng, Cnt
     }
 
     const int cell = cell_of(a, p.ix, p.iy, p.iz);
-    const size_t stride = (size_t)(a.nCells + 1);
-    const T kap = a.opacity[(size_t)p.nu0 * stride + (size_t)cell];
+    const T kap = cell == cell0 ? kap0 : a.opacity[(size_t)p.nu0 * stride + (size_t)cell];
     const T tauCell = dS * kap;
     const T dV = a.vx[p.ix] * a.vy[p.iy] * a.vz[p.iz];
 
@@ -447,9 +455,9 @@ int launch(moc_ctx* ctx, const MocTransportArgs<T>& a, float* ms) {
     int64_t lanes = a.n < lanes_max ? a.n : lanes_max;
     const int block = 256;
     const int blocks = (int)((lanes + block - 1) / block);
-    hipEventRecord(ctx->ev0, ctx->stream);
+    (void)hipEventRecord(ctx->ev0, ctx->stream);
     hipLaunchKernelGGL(k_transport<T>, dim3(blocks), dim3(block), 0, ctx->stream, a);
-    hipEventRecord(ctx->ev1, ctx->stream);
+    (void)hipEventRecord(ctx->ev1, ctx->stream);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         ctx->err = std::string("transport launch: ") + hipGetErrorString(e);
@@ -460,7 +468,7 @@ int launch(moc_ctx* ctx, const MocTransportArgs<T>& a, float* ms) {
         ctx->err = std::string("transport kernel: ") + hipGetErrorString(e);
         return MOC_ERR_HIP;
     }
-    if (ms) hipEventElapsedTime(ms, ctx->ev0, ctx->ev1);
+    if (ms) (void)hipEventElapsedTime(ms, ctx->ev0, ctx->ev1);
     return MOC_OK;
 }
 
