@@ -45,6 +45,20 @@ struct Pk {
     int state;
     int emits;       // emission events of this packet
     int segs;        // segments of the current pathSegment call
+    int kcell;       // cell whose opacity in bin nu0 is held in kap (-1: none)
+    T kap;
+};
+
+// axes and volume factors staged in LDS: segment walks read them every step,
+// and LDS reads do not queue behind the J atomics in the vector memory counter
+template <typename T>
+struct Geo {
+    const T* ax;
+    const T* ay;
+    const T* az;
+    const T* vx;
+    const T* vy;
+    const T* vz;
 };
 
 template <typename T>
@@ -100,44 +114,39 @@ __device__ __forceinline__ void unit_vector(MocRng& rng, T& u, T& v, T& w) {
 // distance to the next wall along one axis; walls sit at axis midpoints and
 // the outer faces at the first/last axis point (photon_mod.f90:1990-2094).
 // Returns false when the packet snaps onto the outer face and is dropped.
+// One division per axis: the target wall is selected first (the midpoint sum
+// is commutative, so both directions form it as ax[idx] + ax[j]).
 template <typename T>
-__device__ __forceinline__ bool wall_dist(T& pos, T dir, int& idx, const T* __restrict__ ax,
-                                          int n, T& dS) {
-    if (dir > T(0)) {
-        if (idx < n - 1) {
-            const T m = (ax[idx + 1] + ax[idx]) / T(2);
-            dS = (m - pos) / dir;
-            if (tabs(dS) < T(1.e-5)) { pos = m; ++idx; }
-        } else {
-            dS = (ax[n - 1] - pos) / dir;
-            if (tabs(dS) < T(1.e-5)) { pos = ax[n - 1]; return false; }
-        }
-    } else if (dir < T(0)) {
-        if (idx > 0) {
-            const T m = (ax[idx] + ax[idx - 1]) / T(2);
-            dS = (m - pos) / dir;
-            if (tabs(dS) < T(1.e-5)) { pos = m; --idx; }
-        } else {
-            dS = (ax[0] - pos) / dir;
-            if (tabs(dS) < T(1.e-5)) pos = ax[0];
-        }
-    } else {
+__device__ __forceinline__ bool wall_dist(T& pos, T dir, int& idx, const T* ax, int n, T& dS) {
+    const bool up = dir > T(0);
+    if (!up && !(dir < T(0))) {
         dS = ax[n - 1];
+        return true;
+    }
+    const bool inner = up ? idx < n - 1 : idx > 0;
+    const int j = up ? idx + 1 : idx - 1;
+    const T t = inner ? (ax[idx] + ax[j]) / T(2) : (up ? ax[n - 1] : ax[0]);
+    dS = (t - pos) / dir;
+    if (tabs(dS) < T(1.e-5)) {
+        pos = t;
+        if (inner) idx = j;
+        else if (up) return false;
     }
     return true;
 }
 
 // the 2 % index nudge at the start of each segment (photon_mod.f90:1882-1919);
 // the ratio uses the signed midpoint as the reference does.
+// Only the wall ahead can move the index (an upward step cannot be undone by
+// the downward test, which needs dir < 0), so one ratio per axis is formed.
 template <typename T>
-__device__ __forceinline__ void nudge(T pos, T dir, int& idx, const T* __restrict__ ax, int n) {
-    if (idx < n - 1) {
-        const T m = (ax[idx] + ax[idx + 1]) / T(2);
-        if (dir > T(0) && tabs(pos - m) / m < T(2.e-2)) ++idx;
-    }
-    if (idx > 0) {
-        const T m = (ax[idx] + ax[idx - 1]) / T(2);
-        if (dir < T(0) && tabs(pos - m) / m < T(2.e-2)) --idx;
+__device__ __forceinline__ void nudge(T pos, T dir, int& idx, const T* ax, int n) {
+    const bool up = dir > T(0) && idx < n - 1;
+    const bool dn = dir < T(0) && idx > 0;
+    if (up || dn) {
+        const int j = up ? idx + 1 : idx - 1;
+        const T m = (ax[idx] + ax[j]) / T(2);
+        if (tabs(pos - m) / m < T(2.e-2)) idx = j;
     }
 }
 
@@ -149,9 +158,10 @@ __device__ __forceinline__ int cell_of(const MocTransportArgs<T>& a, int ix, int
     return a.active[(size_t)ix + (size_t)a.nx * ((size_t)iy + (size_t)a.ny * (size_t)iz)];
 }
 
+// per-lane event counts; a lane handles at most a few million events per
+// launch, and the wave sums are taken in 64 bits
 struct Cnt {
-    unsigned long long seg = 0, emit = 0, absb = 0, sca = 0, esc = 0, line = 0, lost = 0,
-                       trap = 0, err = 0;
+    unsigned seg = 0, emit = 0, absb = 0, sca = 0, esc = 0, line = 0, lost = 0, trap = 0, err = 0;
     double qphot = 0.0;
 };
 
@@ -166,7 +176,7 @@ __device__ __forceinline__ void tally_escape(const MocTransportArgs<T>& a, const
 // newPhotonPacket + initPhotonPacket + the escape/line branches of
 // energyPacketRun, then the start of pathSegment (tau = -ln(1-R)).
 template <typename T>
-__device__ void emit(const MocTransportArgs<T>& a, Pk<T>& p, MocRng& rng, Cnt& c) {
+__device__ __forceinline__ void emit(const MocTransportArgs<T>& a, Pk<T>& p, MocRng& rng, Cnt& c) {
     ++c.emit;
     ++p.emits;
     int bad = 0;
@@ -203,6 +213,8 @@ __device__ void emit(const MocTransportArgs<T>& a, Pk<T>& p, MocRng& rng, Cnt& c
     p.passProb = -log(T(1) - rng.uniform<T>());
     p.segs = 0;
     p.state = ST_FLY;
+    p.kcell = p.origin;
+    p.kap = a.opacity[(size_t)p.nu0 * (size_t)(a.nCells + 1) + (size_t)p.origin];
 }
 
 // Compton scattering (photon_mod.f90:6805-6844): the KN angle is used as the
@@ -222,9 +234,13 @@ __device__ __forceinline__ void compton(const MocTransportArgs<T>& a, Pk<T>& p, 
     p.nu = nn;
 }
 
-// one trip of the pathSegment loop (photon_mod.f90:1880-4038)
+// one trip of the pathSegment loop (photon_mod.f90:1880-4038).  The J
+// contribution of the step is added last, after the opacity of the next cell
+// has been requested: the vector memory counter retires in order, so the next
+// step's wait for that opacity does not include the atomic's round trip.
 template <typename T>
-__device__ void segment(const MocTransportArgs<T>& a, Pk<T>& p, MocRng& rng, Cnt& c) {
+__device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const Geo<T>& g, Pk<T>& p,
+                                        MocRng& rng, Cnt& c) {
     if (p.segs >= MOC_SAFE_LIMIT) {
         ++c.lost;
         p.state = ST_DONE;
@@ -234,35 +250,36 @@ __device__ void segment(const MocTransportArgs<T>& a, Pk<T>& p, MocRng& rng, Cnt
     ++c.seg;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
 
-    nudge(p.x, p.u, p.ix, a.ax, nx);
-    nudge(p.y, p.v, p.iy, a.ay, ny);
-    nudge(p.z, p.w, p.iz, a.az, nz);
+    nudge(p.x, p.u, p.ix, g.ax, nx);
+    nudge(p.y, p.v, p.iy, g.ay, ny);
+    nudge(p.z, p.w, p.iz, g.az, nz);
     if (p.ix < 0 || p.ix >= nx || p.iy < 0 || p.iy >= ny || p.iz < 0 || p.iz >= nz) {
         ++c.err;
         p.state = ST_DONE;
         return;
     }
     if (a.sym) {  // mirror walls of the octant model
-        if (p.x <= a.ax[0]) { if (p.u < T(0)) p.u = -p.u; p.x = a.ax[0]; }
-        if (p.y <= a.ay[0]) { if (p.v < T(0)) p.v = -p.v; p.y = a.ay[0]; }
-        if (p.z <= a.az[0]) { if (p.w < T(0)) p.w = -p.w; p.z = a.az[0]; }
+        if (p.x <= g.ax[0]) { if (p.u < T(0)) p.u = -p.u; p.x = g.ax[0]; }
+        if (p.y <= g.ay[0]) { if (p.v < T(0)) p.v = -p.v; p.y = g.ay[0]; }
+        if (p.z <= g.az[0]) { if (p.w < T(0)) p.w = -p.w; p.z = g.az[0]; }
     }
-    // the opacity of the current cell is fetched before the wall distances are
-    // worked out; a wall snap below moves the packet to a neighbour (rarely) and
-    // the value is fetched again for that cell
-    const size_t stride = (size_t)(a.nCells + 1);
+    // opacity of the current cell: the prefetched value unless a nudge moved
+    // the packet; a wall snap below moves it to a neighbour (rarely) and the
+    // value is fetched again for that cell
+    const size_t row = (size_t)p.nu0 * (size_t)(a.nCells + 1);
     const int cell0 = cell_of(a, p.ix, p.iy, p.iz);
-    const T kap0 = a.opacity[(size_t)p.nu0 * stride + (size_t)cell0];
+    T kap0 = p.kap;
+    if (cell0 != p.kcell) kap0 = a.opacity[row + (size_t)cell0];
     T dSx = T(0), dSy = T(0), dSz = T(0);
-    if (!wall_dist(p.x, p.u, p.ix, a.ax, nx, dSx) || !wall_dist(p.y, p.v, p.iy, a.ay, ny, dSy) ||
-        !wall_dist(p.z, p.w, p.iz, a.az, nz, dSz)) {
+    if (!wall_dist(p.x, p.u, p.ix, g.ax, nx, dSx) || !wall_dist(p.y, p.v, p.iy, g.ay, ny, dSy) ||
+        !wall_dist(p.z, p.w, p.iz, g.az, nz, dSz)) {
         ++c.lost;
         p.state = ST_DONE;
         return;
     }
-    if (tabs(dSx) < T(1.e-5)) dSx = a.ax[nx - 1];
-    if (tabs(dSy) < T(1.e-5)) dSy = a.ay[ny - 1];
-    if (tabs(dSz) < T(1.e-5)) dSz = a.az[nz - 1];
+    if (tabs(dSx) < T(1.e-5)) dSx = g.ax[nx - 1];
+    if (tabs(dSy) < T(1.e-5)) dSy = g.ay[ny - 1];
+    if (tabs(dSz) < T(1.e-5)) dSz = g.az[nz - 1];
     dSx = tabs(dSx);
     dSy = tabs(dSy);
     dSz = tabs(dSz);
@@ -278,27 +295,30 @@ __device__ void segment(const MocTransportArgs<T>& a, Pk<T>& p, MocRng& rng, Cnt
     }
 
     const int cell = cell_of(a, p.ix, p.iy, p.iz);
-    const T kap = cell == cell0 ? kap0 : a.opacity[(size_t)p.nu0 * stride + (size_t)cell];
+    T kap = kap0;
+    if (cell != cell0) kap = a.opacity[row + (size_t)cell];
     const T tauCell = dS * kap;
-    const T dV = a.vx[p.ix] * a.vy[p.iy] * a.vz[p.iz];
+    const T dV = g.vx[p.ix] * g.vy[p.iy] * g.vz[p.iz];
+    double* const Jc = a.J + row + (size_t)cell;
 
     if (p.absTau + tauCell > p.passProb && cell > 0) {
         // the packet interacts inside this cell
         const T dl = (p.passProb - p.absTau) / kap;
+        const double dJ = (double)(dl * a.deltaE / dV);
         p.x = p.x + dl * p.u;
         p.y = p.y + dl * p.v;
         p.z = p.z + dl * p.w;
         if (a.sym) {
-            if (p.x <= a.ax[0]) { if (p.u < T(0)) p.u = -p.u; p.x = a.ax[0]; }
-            if (p.y <= a.ay[0]) { if (p.v < T(0)) p.v = -p.v; p.y = a.ay[0]; }
-            if (p.z <= a.az[0]) { if (p.w < T(0)) p.w = -p.w; p.z = a.az[0]; }
+            if (p.x <= g.ax[0]) { if (p.u < T(0)) p.u = -p.u; p.x = g.ax[0]; }
+            if (p.y <= g.ay[0]) { if (p.v < T(0)) p.v = -p.v; p.y = g.ay[0]; }
+            if (p.z <= g.az[0]) { if (p.w < T(0)) p.w = -p.w; p.z = g.az[0]; }
         }
-        atomicAdd(&a.J[(size_t)p.nu0 * stride + (size_t)cell], (double)(dl * a.deltaE / dV));
         if (a.R_out > T(0)) {
             // radius with the 1e10 scaling of photon_mod.f90:2294 (no FP32 overflow)
             const T sx = p.x / T(1.e10), sy = p.y / T(1.e10), sz = p.z / T(1.e10);
             const T rr = sqrt(sx * sx + sy * sy + sz * sz) * T(1.e10);
             if (rr >= a.R_out) {
+                atomicAdd(Jc, dJ);
                 tally_escape(a, p, c);
                 p.state = ST_DONE;
                 return;
@@ -313,8 +333,12 @@ __device__ void segment(const MocTransportArgs<T>& a, Pk<T>& p, MocRng& rng, Cnt
             p.origin = cell;
             p.absTau = T(0);
             p.passProb = -log(T(1) - rng.uniform<T>());
+            p.kcell = cell;
+            p.kap = a.opacity[(size_t)p.nu0 * (size_t)(a.nCells + 1) + (size_t)cell];
+            atomicAdd(Jc, dJ);
             return;
         }
+        atomicAdd(Jc, dJ);
         // absorbed: re-emitted as a diffuse packet from this position
         ++c.absb;
         if (p.emits >= MOC_RECURSION_LIMIT) {
@@ -328,8 +352,7 @@ __device__ void segment(const MocTransportArgs<T>& a, Pk<T>& p, MocRng& rng, Cnt
     }
 
     // the packet crosses the cell
-    if (cell > 0)
-        atomicAdd(&a.J[(size_t)p.nu0 * stride + (size_t)cell], (double)(dS * a.deltaE / dV));
+    const double dJ = (double)(dS * a.deltaE / dV);
     p.absTau = p.absTau + tauCell;
     p.x = p.x + dS * p.u;
     p.y = p.y + dS * p.v;
@@ -341,31 +364,33 @@ __device__ void segment(const MocTransportArgs<T>& a, Pk<T>& p, MocRng& rng, Cnt
     else if (dS == dSz && p.w > T(0)) ++p.iz;
     else if (dS == dSz && p.w < T(0)) --p.iz;
 
+    bool esc = false;
     if (!a.sym) {
-        bool ret = false;
-        if (p.y < a.ay[0] || p.iy < 0) { p.iy = 0; ret = true; }
-        if (p.y > a.ay[ny - 1] || p.iy > ny - 1) { p.iy = ny - 1; ret = true; }
-        if (p.x <= a.ax[0] || p.ix < 0) { p.ix = 0; ret = true; }
-        if (p.x >= a.ax[nx - 1] || p.ix > nx - 1) { p.ix = nx - 1; ret = true; }
-        if (p.z <= a.az[0] || p.iz < 0) { p.iz = 0; ret = true; }
-        if (p.z >= a.az[nz - 1] || p.iz > nz - 1) { p.iz = nz - 1; ret = true; }
-        if (ret) {
-            tally_escape(a, p, c);
-            p.state = ST_DONE;
-            return;
-        }
+        if (p.y < g.ay[0] || p.iy < 0) { p.iy = 0; esc = true; }
+        if (p.y > g.ay[ny - 1] || p.iy > ny - 1) { p.iy = ny - 1; esc = true; }
+        if (p.x <= g.ax[0] || p.ix < 0) { p.ix = 0; esc = true; }
+        if (p.x >= g.ax[nx - 1] || p.ix > nx - 1) { p.ix = nx - 1; esc = true; }
+        if (p.z <= g.az[0] || p.iz < 0) { p.iz = 0; esc = true; }
+        if (p.z >= g.az[nz - 1] || p.iz > nz - 1) { p.iz = nz - 1; esc = true; }
     }
-    if ((!a.sym && (p.x <= a.ax[0] || p.y <= a.ay[0] || p.z <= a.az[0])) ||
-        p.x >= a.ax[nx - 1] || p.y >= a.ay[ny - 1] || p.z >= a.az[nz - 1] || p.ix > nx - 1 ||
-        p.iy > ny - 1 || p.iz > nz - 1) {
+    if (!esc && ((!a.sym && (p.x <= g.ax[0] || p.y <= g.ay[0] || p.z <= g.az[0])) ||
+                 p.x >= g.ax[nx - 1] || p.y >= g.ay[ny - 1] || p.z >= g.az[nz - 1] ||
+                 p.ix > nx - 1 || p.iy > ny - 1 || p.iz > nz - 1))
+        esc = true;
+    if (!esc) {
+        if (a.sym) {
+            if (p.x <= g.ax[0] || p.ix < 0) { if (p.u < T(0)) p.u = -p.u; p.ix = 0; p.x = g.ax[0]; }
+            if (p.y <= g.ay[0] || p.iy < 0) { if (p.v < T(0)) p.v = -p.v; p.iy = 0; p.y = g.ay[0]; }
+            if (p.z <= g.az[0] || p.iz < 0) { if (p.w < T(0)) p.w = -p.w; p.iz = 0; p.z = g.az[0]; }
+        }
+        // every index is inside the grid here: request the next cell's opacity
+        p.kcell = cell_of(a, p.ix, p.iy, p.iz);
+        p.kap = a.opacity[row + (size_t)p.kcell];
+    }
+    if (cell > 0) atomicAdd(Jc, dJ);
+    if (esc) {
         tally_escape(a, p, c);
         p.state = ST_DONE;
-        return;
-    }
-    if (a.sym) {
-        if (p.x <= a.ax[0] || p.ix < 0) { if (p.u < T(0)) p.u = -p.u; p.ix = 0; p.x = a.ax[0]; }
-        if (p.y <= a.ay[0] || p.iy < 0) { if (p.v < T(0)) p.v = -p.v; p.iy = 0; p.y = a.ay[0]; }
-        if (p.z <= a.az[0] || p.iz < 0) { if (p.w < T(0)) p.w = -p.w; p.iz = 0; p.z = a.az[0]; }
     }
 }
 
@@ -380,17 +405,52 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Emissions wait for company: an emission walks a CDF row by bisection, a
+// chain of dependent loads, so it runs when at least kEmitBatch lanes of the
+// wave need one (or no lane is flying); until then those lanes sit out while
+// the others keep stepping their segments, kSegsPerTrip per trip through the
+// loop.  Packet ids are taken a wave at a time from the work counter.  The
+// values were chosen by timing the 32^3 nebula (1e7 packets, 1000 bins):
+// batch 4/8/16/24/32/40/48 gave 45/34/21/17/16/17/19 ms, and two segments per
+// trip took 16 to 15.5 ms.
+constexpr int kEmitBatch = 32;
+constexpr int kSegsPerTrip = 2;
+
+extern __shared__ __align__(16) unsigned char transport_lds[];
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_transport(MocTransportArgs<T> a) {
+    // stage the axes and volume factors: [ax ay az vx vy vz]
+    T* const sg = reinterpret_cast<T*>(transport_lds);
+    {
+        const int nx = a.nx, ny = a.ny, nz = a.nz, nt = nx + ny + nz;
+        for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+            const bool bx = i < nx, by = !bx && i < nx + ny;
+            const int k = bx ? i : (by ? i - nx : i - nx - ny);
+            sg[i] = bx ? a.ax[k] : (by ? a.ay[k] : a.az[k]);
+            sg[nt + i] = bx ? a.vx[k] : (by ? a.vy[k] : a.vz[k]);
+        }
+        __syncthreads();
+    }
+    const Geo<T> g{sg, sg + a.nx, sg + a.nx + a.ny, sg + a.nx + a.ny + a.nz,
+                   sg + 2 * a.nx + a.ny + a.nz, sg + 2 * a.nx + 2 * a.ny + a.nz};
     Cnt c;
     Pk<T> p;
     p.state = ST_DONE;
+    p.kcell = -1;
     MocRng rng;
     bool more = true;
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
     while (true) {
-        if (p.state == ST_DONE) {
-            if (more) {
-                const unsigned long long id = atomicAdd(&a.counters[MOC_CNT_NEXT], 1ull);
+        const unsigned long long want = __ballot(p.state == ST_DONE && more);
+        if (want) {
+            const int leader = __ffsll((long long)want) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(&a.counters[MOC_CNT_NEXT], (unsigned long long)__popcll(want));
+            base = __shfl(base, leader, 64);
+            if (p.state == ST_DONE && more) {
+                const unsigned long long id = base + (unsigned long long)__popcll(want & below);
                 if ((int64_t)id < a.n) {
                     rng.init(a.seed, id, a.ctr_base);
                     p.x = a.star[0];
@@ -406,27 +466,34 @@ __global__ void __launch_bounds__(256) k_transport(MocTransportArgs<T> a) {
                     more = false;
                 }
             }
-            if (!more) break;
         }
-        if (p.state == ST_EMIT) emit(a, p, rng, c);
-        if (p.state == ST_FLY) segment(a, p, rng, c);
+        const unsigned long long emitters = __ballot(p.state == ST_EMIT);
+        const unsigned long long flyers = __ballot(p.state == ST_FLY);
+        if (!emitters && !flyers && !__ballot(more)) break;
+        if (emitters && (__popcll(emitters) >= kEmitBatch || !flyers || !__ballot(more))) {
+            if (p.state == ST_EMIT) emit(a, p, rng, c);
+        }
+#pragma unroll 1
+        for (int k = 0; k < kSegsPerTrip; ++k)
+            if (p.state == ST_FLY) segment(a, g, p, rng, c);
     }
     // all lanes of the wave have left the loop: reduce the counters once
-    unsigned long long* g = a.counters;
-    const unsigned long long s0 = wave_sum(c.seg), s1 = wave_sum(c.emit), s2 = wave_sum(c.absb),
-                             s3 = wave_sum(c.sca), s4 = wave_sum(c.esc), s5 = wave_sum(c.line),
-                             s6 = wave_sum(c.lost), s7 = wave_sum(c.trap), s8 = wave_sum(c.err);
+    unsigned long long* cnt = a.counters;
+    using u64 = unsigned long long;
+    const u64 s0 = wave_sum((u64)c.seg), s1 = wave_sum((u64)c.emit), s2 = wave_sum((u64)c.absb),
+              s3 = wave_sum((u64)c.sca), s4 = wave_sum((u64)c.esc), s5 = wave_sum((u64)c.line),
+              s6 = wave_sum((u64)c.lost), s7 = wave_sum((u64)c.trap), s8 = wave_sum((u64)c.err);
     const double q = wave_sum(c.qphot);
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&g[MOC_CNT_SEGMENTS], s0);
-        atomicAdd(&g[MOC_CNT_EMISSIONS], s1);
-        atomicAdd(&g[MOC_CNT_ABSORBED], s2);
-        atomicAdd(&g[MOC_CNT_SCATTERED], s3);
-        atomicAdd(&g[MOC_CNT_ESCAPED], s4);
-        atomicAdd(&g[MOC_CNT_LINES], s5);
-        atomicAdd(&g[MOC_CNT_LOST], s6);
-        atomicAdd(&g[MOC_CNT_TRAPPED], s7);
-        atomicAdd(&g[MOC_CNT_ERROR], s8);
+        atomicAdd(&cnt[MOC_CNT_SEGMENTS], s0);
+        atomicAdd(&cnt[MOC_CNT_EMISSIONS], s1);
+        atomicAdd(&cnt[MOC_CNT_ABSORBED], s2);
+        atomicAdd(&cnt[MOC_CNT_SCATTERED], s3);
+        atomicAdd(&cnt[MOC_CNT_ESCAPED], s4);
+        atomicAdd(&cnt[MOC_CNT_LINES], s5);
+        atomicAdd(&cnt[MOC_CNT_LOST], s6);
+        atomicAdd(&cnt[MOC_CNT_TRAPPED], s7);
+        atomicAdd(&cnt[MOC_CNT_ERROR], s8);
         atomicAdd(a.Qphot, q);
     }
 }
@@ -443,6 +510,9 @@ __global__ void k_scale(double* __restrict__ p, int64_t n, double f) {
     for (int64_t k = i; k < n; k += st) p[k] *= f;
 }
 
+// LDS for the staged geometry of one workgroup (2 x (nx+ny+nz) values)
+constexpr size_t kMaxGeoLds = 32768;
+
 template <typename T>
 int launch(moc_ctx* ctx, const MocTransportArgs<T>& a, float* ms) {
     if (a.n <= 0) {
@@ -455,8 +525,14 @@ int launch(moc_ctx* ctx, const MocTransportArgs<T>& a, float* ms) {
     int64_t lanes = a.n < lanes_max ? a.n : lanes_max;
     const int block = 256;
     const int blocks = (int)((lanes + block - 1) / block);
+    const size_t lds = 2 * (size_t)(a.nx + a.ny + a.nz) * sizeof(T);
+    if (lds > kMaxGeoLds) {
+        ctx->err = "transport: the grid axes (" + std::to_string(a.nx + a.ny + a.nz) +
+                   " points) exceed the " + std::to_string(kMaxGeoLds) + " B LDS staging area";
+        return MOC_ERR_ARG;
+    }
     (void)hipEventRecord(ctx->ev0, ctx->stream);
-    hipLaunchKernelGGL(k_transport<T>, dim3(blocks), dim3(block), 0, ctx->stream, a);
+    hipLaunchKernelGGL(k_transport<T>, dim3(blocks), dim3(block), lds, ctx->stream, a);
     (void)hipEventRecord(ctx->ev1, ctx->stream);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
