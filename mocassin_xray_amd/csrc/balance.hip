@@ -9,15 +9,16 @@
 //               R[cell, col] = sum_nu J[nu, cell] W[nu, col] with weight columns
 //               built once on the host; tiles of 64 cells x 32 columns stage J
 //               and W through LDS, J planes are read cell-fastest (coalesced).
-//  k_balance -- one lane per cell: the T_e root-find (iterateT :277-482)
+//  k_balance -- one wave per cell: the T_e root-find (iterateT :277-482)
 //               around the ion-fraction chain (ionBalance :1575-1992) and the
 //               heating/cooling integrals (thermBalance :887-1215) with the
 //               n-level forbidden-line cooling (equilibrium, emission_mod.f90:
 //               2505-3106).  Everything in FP64.  Recombination coefficients
 //               (calcAlpha :2298-2413) and collisional ionisation (:3552-3582)
 //               are evaluated per iteration from tables resident in HBM/L2;
-//               the Upsilon(T) splines use second derivatives computed once on
-//               the host.
+//               the cell's rate columns and the shell index tables are staged
+//               in LDS; the Upsilon(T) splines use second derivatives computed
+//               once on the host.
 #include <hip/hip_runtime.h>
 
 #include "moc_internal.h"
@@ -103,6 +104,14 @@ __host__ __device__ inline size_t balance_lds_bytes(int NE, int NS, int nNlv) {
     const size_t rows = (size_t)7 * NE * (NS + 2);
     return (rows + (size_t)(nNlv > 0 ? nNlv : 1) + MOC_MAXS + 1) * sizeof(double) + 16;
 }
+
+// the staged rate columns and index tables that follow the state rows
+inline size_t balance_stage_bytes(const MocBalanceArgs& a) {
+    const size_t ints = (size_t)a.NE * 32 + (size_t)a.nSpec + 1 + 2 * (size_t)a.nShells;
+    return (size_t)a.ncol * sizeof(double) + ((ints * sizeof(int) + 7) & ~(size_t)7);
+}
+// staging is used while one workgroup's LDS stays within this
+constexpr size_t kMaxBalanceLds = 49152;
 
 // dielectronic recombination of the ions with more than 11 electrons
 // (Aldrovandi & Pequignot above 6e4 K, Nussbaumer & Storey below;
@@ -230,11 +239,24 @@ __device__ __forceinline__ double dEk(int z, int ion) {
 }
 
 // photoionisation rates of the cell from k_rates (per shell), with the
-// reverse charge exchange added to the outer shell (:3715-3830)
+// reverse charge exchange added to the outer shell (:3715-3830), and the
+// species/shell index tables the chains walk.  With ST the rate columns and
+// the index tables are LDS copies made when the cell starts (the chains read
+// them on one lane per element, where every global load is a serial
+// round trip); without ST they are read from the [ncol][C1] rate array and
+// the tables in HBM.
+template <bool ST>
 struct Rates {
     const double* R;
-    int C1, cell, nShells;
-    __device__ double at(int col) const { return R[(size_t)col * C1 + cell]; }
+    int C1, cell;
+    const int* specOf;     // [NE*32] (slot, stage) -> species or -1
+    const int* specShell;  // [S+1]
+    const int* shInfo;     // [nShells] Auger count | (1 + reverse charge-exchange slot) << 8
+    const int* shAug;      // [nShells] first auger[] entry of the shell
+    __device__ double at(int col) const {
+        if constexpr (ST) return R[col];
+        else return R[(size_t)col * C1 + cell];
+    }
     __device__ double gamma(int s) const { return at(3 * s); }
     __device__ double heat(int s, double heatef) const { return at(3 * s + 1) + heatef * at(3 * s + 2); }
 };
@@ -246,22 +268,24 @@ __device__ double rev_chex(const MocBalanceArgs& a, const Lds& L, double Te, dou
     return chex(a, z, ion, Te, true) * (1. - L.x(a.elemSlot[1], 1)) * H * 2. * exp(-dEk(z, ion) / Te) / g0g1;
 }
 
-__device__ double gamma_out(const MocBalanceArgs& a, const Rates& r, int spec) {
+template <class RT>
+__device__ double gamma_out(const RT& r, int spec) {
     double g = 0.0;
-    for (int sh = a.specShell[spec]; sh < a.specShell[spec + 1]; ++sh) g += r.gamma(sh);
+    for (int sh = r.specShell[spec]; sh < r.specShell[spec + 1]; ++sh) g += r.gamma(sh);
     return g;
 }
 
 // H or He ratio chain (:1715-1790) on the lane that owns element zz
-__device__ void hhe_chain(const MocBalanceArgs& a, const Rates& r, const Lds& L, int zz, double Te,
+template <class RT>
+__device__ void hhe_chain(const MocBalanceArgs& a, const RT& r, const Lds& L, int zz, double Te,
                           double Ne, double H, double H0, double collIonH) {
     const int NS = a.NS, e = a.elemSlot[zz];
     double ratio[3] = {0., 0., 0.};
     for (int ion = 1; ion <= imin(zz, NS - 1); ++ion) {
-        const int spec = a.specOf[e * 32 + ion];
+        const int spec = r.specOf[e * 32 + ion];
         const double cx = chex(a, zz, ion, Te, true);
         if (zz > 1 || ion > 1) collIonH = 0.;
-        const double g = (spec >= 0 ? gamma_out(a, r, spec) : 0.0) + 1.e-20;
+        const double g = (spec >= 0 ? gamma_out(r, spec) : 0.0) + 1.e-20;
         ratio[ion] = (g + collIonH * Ne) / (Ne * L.al(e, ion) + cx * H * H0);
     }
     double prod[3] = {1., 1., 1.}, den = 1.;
@@ -278,7 +302,8 @@ __device__ void hhe_chain(const MocBalanceArgs& a, const Rates& r, const Lds& L,
 }
 
 // bidiagonal chain of one heavy element with Auger spreading (:1826-1929)
-__device__ void heavy_chain(const MocBalanceArgs& a, const Rates& r, const Lds& L, int e, double Te,
+template <class RT>
+__device__ void heavy_chain(const MocBalanceArgs& a, const RT& r, const Lds& L, int e, double Te,
                             double Ne, double H, int nIterX, const double revFrozen[2]) {
     const int NS = a.NS, z = a.elemZ[e];
     double* out = L.out + e * L.S2;
@@ -288,19 +313,19 @@ __device__ void heavy_chain(const MocBalanceArgs& a, const Rates& r, const Lds& 
     const int top = imin(NS, z + 1);
     for (int ion = 1; ion <= imin(z, NS - 1); ++ion) {
         out[ion] += L.c1(e, ion) * Ne;
-        const int spec = a.specOf[e * 32 + ion];
+        const int spec = r.specOf[e * 32 + ion];
         if (spec < 0) continue;
-        for (int sh = a.specShell[spec]; sh < a.specShell[spec + 1]; ++sh) {
-            const int shn = a.shellNum[sh];
+        for (int sh = r.specShell[spec]; sh < r.specShell[spec + 1]; ++sh) {
+            // the outer shell of N I / O I takes the reverse charge exchange
+            const int info = r.shInfo[sh], cx = info >> 8, na = info & 0xff;
             double g = r.gamma(sh);
-            if (shn == a.outShell[z * 32 + (z - ion + 1)] && ion == 1 && (z == 7 || z == 8))
-                g += revFrozen[z - 7];
+            if (cx) g += revFrozen[cx - 1];
             out[ion] += g;
-            const int na = a.nauger[(z * 32 + ion) * 11 + shn];
+            const double* aug = a.auger + r.shAug[sh];
             for (int ne = 2; ne <= na; ++ne) {
                 const int mx = imin(ion + ne - 1, imin(NS - 1, z));
                 const double ratio = L.x(e, mx) > 1e-30 ? L.x(e, ion) / L.x(e, mx) : 1.;
-                out[mx] += g * a.auger[((z * 32 + ion) * 11 + shn) * 11 + ne] * ratio;
+                out[mx] += g * aug[ne] * ratio;
             }
         }
     }
@@ -330,7 +355,8 @@ __device__ void heavy_chain(const MocBalanceArgs& a, const Rates& r, const Lds& 
 }
 
 // ionBalance (:1575-1992); returns false when 15 passes did not settle H/He
-__device__ bool ion_balance(const MocBalanceArgs& a, const Rates& r, const Lds& L, double Te,
+template <class RT>
+__device__ bool ion_balance(const MocBalanceArgs& a, const RT& r, const Lds& L, double Te,
                             double& Ne, double H, const double revFrozen[2], int tid) {
     const int NS = a.NS, NE = a.NE;
     const int eH = a.elemSlot[1], eHe = a.elemSlot[2];
@@ -377,8 +403,8 @@ __device__ double ion_cooling(const MocBalanceArgs& a, int atom, double T, doubl
 }
 
 // thermBalance (:887-1215): heating and cooling [erg/s per n_H]
-template <int NL>
-__device__ void therm(const MocBalanceArgs& a, const Rates& r, const Lds& L, double T, double Ne,
+template <int NL, class RT>
+__device__ void therm(const MocBalanceArgs& a, const RT& r, const Lds& L, double T, double Ne,
                       double H, double heatef, const double comp[4], double* heat, double* cool,
                       int tid) {
     // forbidden-line cooling, one ion with data per lane
@@ -429,7 +455,7 @@ __device__ void therm(const MocBalanceArgs& a, const Rates& r, const Lds& L, dou
         const int e = a.specElem[sp], ion = a.specIon[sp];
         const int z = a.elemZ[e];
         if (ion > imin(z, a.NS - 1)) continue;
-        for (int sh = a.specShell[sp]; sh < a.specShell[sp + 1]; ++sh)
+        for (int sh = r.specShell[sp]; sh < r.specShell[sp + 1]; ++sh)
             heatSte += r.heat(sh, z < 3 ? 1.0 : heatef) * L.x(e, ion) * a.abund[e];
     }
     *heat = heatSte;
@@ -443,14 +469,38 @@ __device__ void therm(const MocBalanceArgs& a, const Rates& r, const Lds& L, dou
     __syncthreads();  // L.cool is rewritten by the next call
 }
 
-template <int NL>
-__global__ void __launch_bounds__(64) k_balance(MocBalanceArgs a) {
+// Four waves per SIMD: the per-cell work is chains of dependent FP64 steps,
+// so latency hiding comes from other cells' waves.  At the compiler's own
+// allocation (256 VGPRs, one wave per SIMD) the nebula32 balance took 20 ms;
+// capped at 128 VGPRs with a few hundred bytes of spills it takes 13.6 ms.
+template <int NL, bool ST>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+k_balance(MocBalanceArgs a) {
     extern __shared__ double lds[];
     const int cell = blockIdx.x + 1;
     const int tid = threadIdx.x;
     const int C1 = a.nCells + 1;
     const int NE = a.NE, NS = a.NS, S2 = NS + 2;
-    Rates r{a.R, C1, cell, a.nShells};
+    Rates<ST> r{a.R, C1, cell, a.specOf, a.specShell, a.shInfo, a.shAug};
+    if constexpr (ST) {
+        // stage the cell's rate columns and the index tables after the state rows
+        double* Rl = lds + balance_lds_bytes(NE, NS, a.nNlv) / sizeof(double);
+        int* iL = reinterpret_cast<int*>(Rl + a.ncol);
+        int* sOf = iL;
+        int* sSh = sOf + NE * 32;
+        int* sIn = sSh + a.nSpec + 1;
+        int* sAu = sIn + a.nShells;
+        for (int k = tid; k < a.ncol; k += 64) Rl[k] = a.R[(size_t)k * C1 + cell];
+        for (int k = tid; k < NE * 32; k += 64) sOf[k] = a.specOf[k];
+        for (int k = tid; k <= a.nSpec; k += 64) sSh[k] = a.specShell[k];
+        for (int k = tid; k < a.nShells; k += 64) { sIn[k] = a.shInfo[k]; sAu[k] = a.shAug[k]; }
+        __syncthreads();
+        r.R = Rl;
+        r.specOf = sOf;
+        r.specShell = sSh;
+        r.shInfo = sIn;
+        r.shAug = sAu;
+    }
     if (!(r.at(a.colHit) > 0.)) {  // no packet reached the cell (lgHit, :127-171)
         if (tid == 0) {
             a.lgBlack[cell] = 1;
@@ -555,14 +605,22 @@ int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a, const double* J, c
     const int C1 = a.nCells + 1;
     dim3 g1((unsigned)((C1 + RC - 1) / RC), (unsigned)((ncol + RN - 1) / RN));
     hipLaunchKernelGGL(k_rates, g1, dim3(256), 0, ctx->stream, J, C1, ctx->nbins, W, ncol, R);
-    const size_t lds = balance_lds_bytes(a.NE, a.NS, a.nNlv);
+    const size_t base = balance_lds_bytes(a.NE, a.NS, a.nNlv);
+    const size_t staged = base + balance_stage_bytes(a);
+    const bool st = staged <= kMaxBalanceLds;
+    const size_t lds = st ? staged : base;
+    const dim3 grid((unsigned)a.nCells), block(64);
     if (a.nCells > 0) {
-        if (a.maxLev <= 5)
-            hipLaunchKernelGGL(k_balance<5>, dim3((unsigned)a.nCells), dim3(64), lds, ctx->stream, a);
-        else if (a.maxLev <= 8)
-            hipLaunchKernelGGL(k_balance<8>, dim3((unsigned)a.nCells), dim3(64), lds, ctx->stream, a);
-        else
-            hipLaunchKernelGGL(k_balance<MOC_MAXLEV>, dim3((unsigned)a.nCells), dim3(64), lds, ctx->stream, a);
+        if (a.maxLev <= 5) {
+            if (st) hipLaunchKernelGGL((k_balance<5, true>), grid, block, lds, ctx->stream, a);
+            else hipLaunchKernelGGL((k_balance<5, false>), grid, block, lds, ctx->stream, a);
+        } else if (a.maxLev <= 8) {
+            if (st) hipLaunchKernelGGL((k_balance<8, true>), grid, block, lds, ctx->stream, a);
+            else hipLaunchKernelGGL((k_balance<8, false>), grid, block, lds, ctx->stream, a);
+        } else {
+            if (st) hipLaunchKernelGGL((k_balance<MOC_MAXLEV, true>), grid, block, lds, ctx->stream, a);
+            else hipLaunchKernelGGL((k_balance<MOC_MAXLEV, false>), grid, block, lds, ctx->stream, a);
+        }
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

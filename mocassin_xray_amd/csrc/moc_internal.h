@@ -107,6 +107,8 @@ struct MocBalanceArgs {
     const int* specOf;     // [NE*32] (slot, stage) -> species or -1
     const int* specShell;  // [S+1] shell range of each species
     const int* shellNum;   // [nShells] shell number (1 = K)
+    const int* shInfo;     // [nShells] Auger electron count | (1 + N I/O I outer-shell slot) << 8
+    const int* shAug;      // [nShells] index of the shell's auger[...][0] entry
     const int* outShell;   // [31*32] (Z, nElec) -> outer shell
     const int* statW;      // [31*32] g0 (Z, nElec)
     const int* statW1;     // [31*32] g1
@@ -141,6 +143,7 @@ struct MocBalanceArgs {
     const int* nlvIon;
     const int* nlvAtom;
     int colHit, colComp;      // rate columns of the hit test and the Compton sums
+    int ncol;                 // rate columns per cell
     const double* R;          // [ncol][C1] rates
     double* Te;
     double* Ne;

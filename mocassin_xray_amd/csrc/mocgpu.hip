@@ -724,6 +724,31 @@ int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* t) {
     PUT(a.cion_th, t->cionTherm, 31 * 32, "cionTherm");
     PUT(a.nauger, t->nAuger, 31 * 32 * 11, "nAuger");
     PUT(a.auger, t->auger, 31 * 32 * 11 * 11, "auger");
+    {
+        // per-shell view of the Auger and reverse charge-exchange tables
+        // (ionBalance :1826-1929): the chains then index by shell alone
+        std::vector<int> info((size_t)std::max(t->nShells, 1), 0), aug((size_t)std::max(t->nShells, 1), 0);
+        for (int sp = 0; sp < S; ++sp) {
+            const int z = at.elemZ[at.specElem[sp]], ion = at.specIon[sp];
+            for (int sh = t->speciesShell[sp]; sh < t->speciesShell[sp + 1]; ++sh) {
+                const int shn = t->shellNum[sh];
+                if (shn < 0 || shn > 10 || ion < 1 || ion > 31 || z - ion + 1 < 0) {
+                    ctx->err = "moc_set_balance: shell number or stage out of range";
+                    return MOC_ERR_ARG;
+                }
+                const int na = t->nAuger[(z * 32 + ion) * 11 + shn];
+                if (na < 0 || na > 10) {
+                    ctx->err = "moc_set_balance: Auger electron count above 10";
+                    return MOC_ERR_ARG;
+                }
+                const bool cx = shn == t->outShell[z * 32 + (z - ion + 1)] && ion == 1 && (z == 7 || z == 8);
+                info[sh] = na | ((cx ? z - 6 : 0) << 8);
+                aug[sh] = ((z * 32 + ion) * 11 + shn) * 11;
+            }
+        }
+        PUT(a.shInfo, info.data(), info.size(), "shInfo");
+        PUT(a.shAug, aug.data(), aug.size(), "shAug");
+    }
     PUT(a.chexIdx, chexIdx.data(), 31 * 32, "chexIdx");
     PUT(a.chexTab, t->chexFit, 4 * (size_t)t->nChex, "chexFit");
     PUT(a.atomIdx, t->atomIdx, 31 * 32, "atomIdx");
@@ -783,6 +808,7 @@ int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* t) {
     a.lgCompton = ctx->lgCompton;
     a.colComp = 3 * t->nShells;
     a.colHit = 3 * t->nShells + 4;
+    a.ncol = b.ncol;
     a.R = p_d;  // placeholder: W pointer kept in R until the launch (set per grid)
     b.set = true;
     return ensure_balance_grid(ctx);
