@@ -398,7 +398,7 @@ __device__ bool ion_balance(const MocBalanceArgs& a, const RT& r, const Lds& L, 
 // n-level cooling of one ion, register-resident solve when the level count fits NL
 template <int NL>
 __device__ double ion_cooling(const MocBalanceArgs& a, int atom, double T, double Ne, double scale) {
-    if constexpr (NL <= 8) return nlevel_cool_fixed<NL>(a, atom, T, Ne, scale, 1.9865e-16);
+    if constexpr (NL <= 8) return nlevel_fixed<NL>(a, atom, T, Ne, 0., false, scale, 1.9865e-16);
     else return nlevel_lines(a, atom, T, Ne, 0., false, scale, 1.9865e-16);
 }
 

@@ -175,6 +175,7 @@ struct MocEmissionArgs {
     const double* tk;        // [ntk]
     int nlim[3];             // H I, He I, He II
     const int* edge[3];      // [nlim] 1-based bin pointers
+    const int* gpair;        // [3][B] fit pair (1-based) that covers each bin, 0 for none
     const double* lg[3];     // [ntk][nlim] log10 gamma
     int HIP, HeIP, HeIIP;    // threshold bins (He: 0 when off)
     const double* xsHI;

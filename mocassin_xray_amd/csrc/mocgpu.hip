@@ -940,6 +940,16 @@ int moc_set_emission(moc_ctx* ctx, const moc_emission_tables* t) {
         EPUT(a.edge[k], t->edgeP[k], t->nlim[k], "edgeP");
         EPUT(a.lg[k], t->logGamma[k], (size_t)t->ntk * t->nlim[k], "logGamma");
     }
+    {
+        // the free-bound fit pair that covers each bin: the last pair p with
+        // edge[2p-2] <= i < edge[2p-1] (fb_ff :437-507, a later pair overwrites)
+        std::vector<int> gp((size_t)3 * B, 0);
+        for (int k = 0; k < 3; ++k)
+            for (int p = 1; p <= t->nlim[k] / 2; ++p)
+                for (int i = t->edgeP[k][2 * p - 2]; i <= t->edgeP[k][2 * p - 1] - 1; ++i)
+                    gp[(size_t)k * B + i - 1] = p;
+        EPUT(a.gpair, gp.data(), gp.size(), "gpair");
+    }
     EPUT(a.xsHI, t->xsHI, B - t->HIPnuP + 1, "xsHI");
     if (heOn) {
         EPUT(a.xsHeI, t->xsHeI, B - t->HeIPnuP + 1, "xsHeI");

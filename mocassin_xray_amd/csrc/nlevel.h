@@ -129,14 +129,15 @@ __device__ double nlevel_lines(const MocBalanceArgs& a, int atom, double Te, dou
 
 namespace {
 
-// The same cooling sum (rec = false) with the level matrix held in registers:
-// NL is a compile-time bound on the level count (n <= NL), every loop is
-// unrolled over NL and predicated on n, and the collision rates of a pair are
-// recomputed where a row needs them instead of being stored.  The arithmetic
-// on the valid entries is the generic routine's, operation for operation.
+// The same sum with the level matrix held in registers: NL is a compile-time
+// bound on the level count (n <= NL), every loop is unrolled over NL and
+// predicated on n, and the collision rates of a pair are recomputed where a
+// row needs them instead of being stored.  The arithmetic on the valid
+// entries is the generic routine's, operation for operation, including the
+// recombination feed when rec is set.
 template <int NL>
-__device__ double nlevel_cool_fixed(const MocBalanceArgs& a, int atom, double Te, double Ne,
-                                    double scale, double unit) {
+__device__ double nlevel_fixed(const MocBalanceArgs& a, int atom, double Te, double Ne,
+                               double ionDenUp, bool rec, double scale, double unit) {
     const int64_t* m = a.atomMeta + 8 * atom;
     const int nf = (int)m[0], nT = (int)m[1];
     const double* lt = a.atomLogT + m[2];
@@ -200,6 +201,29 @@ __device__ double nlevel_cool_fixed(const MocBalanceArgs& a, int atom, double Te
             x[i][i] -= Ne * qef(i, j);
             if (j > i) x[i][j] += A[(size_t)j * nf + i];
             else x[i][i] -= A[(size_t)i * nf + j];
+        }
+    }
+    const int kind = rec ? a.atomRecKind[atom] : 0;
+    if (kind != 0) {
+        const double* rd = a.atomRec + 11 * m[3];
+        const int* rv = a.atomRecValid + m[3];
+        const double t4 = Te * 1.e-4;
+#pragma unroll
+        for (int i = 1; i < NL; ++i) {
+            if (i >= n) break;
+            double al = 0.;
+            if (rv[i]) {
+                const double* r = rd + 11 * i;
+                if (kind == 1) {
+                    const double z = r[1], tz = t4 / (z * z);
+                    al = 1.e-13 * r[0] * z * r[2] * pow(tz, r[3]) / (1. + r[4] * pow(tz, r[5])) +
+                         1.0e-12 * r[0] * (r[6] / t4 + r[7] + r[8] * t4 + r[9] * t4 * t4) * pow(t4, -1.5) *
+                             exp(-r[10] * 1.e4 / Te);
+                } else {
+                    al = pow(10., r[0] + r[1] * lT + r[2] * lT * lT + r[3] * lT * lT * lT);
+                }
+            }
+            y[i] = -Ne * ionDenUp * al;
         }
     }
 #pragma unroll
