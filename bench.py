@@ -40,13 +40,23 @@ def build_model(name: str):
 
 
 def transport_bytes(segments: float, emissions: float, escaped: float, nbins: int,
-                    real_bytes: int) -> float:
-    """Algorithmic HBM bytes of a transport launch: per segment kappa + cell
-    id + J read-modify-write; per emission totalLines + the CDF bisection; per
-    escape the tally read-modify-write."""
-    seg = real_bytes + 4 + 16
+                    real_bytes: int, dense: bool) -> float:
+    """Algorithmic HBM bytes of a transport launch: per segment kappa + J
+    read-modify-write (+ the cell id from the active map unless every cell
+    is active and the id is computed); per emission totalLines + the CDF
+    bisection; per escape the tally read-modify-write."""
+    seg = real_bytes + (0 if dense else 4) + 16
     emit = 4 + real_bytes * math.ceil(math.log2(nbins))
     return segments * seg + emissions * emit + escaped * 16.0
+
+
+def dense_grid(g) -> bool:
+    """Every cell active and numbered 1 + iz + nz*(iy + ny*ix): the transport
+    then computes cell ids instead of reading the active map."""
+    import numpy as np
+    nx, ny, nz = g.active.shape
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    return bool(np.array_equal(g.active, 1 + iz + nz * (iy + ny * ix)))
 
 
 def profiled_traffic(workload: str):
@@ -207,7 +217,7 @@ def main():
     dt_64, st_64, s_64 = res["fp64"]
     kms = mean(st_mix, "transport_ms")
     bytes_mix = transport_bytes(mean(st_mix, "segments"), mean(st_mix, "emissions"),
-                                mean(st_mix, "escaped"), m.tables.nbins, 4)
+                                mean(st_mix, "escaped"), m.tables.nbins, 4, dense_grid(m.grid))
     achieved = bytes_mix / (kms * 1e-3) / 1e9
     out = {
         "metric": "packets/s of the Monte Carlo iteration (mixed precision) + balance cells/s",
