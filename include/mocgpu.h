@@ -29,6 +29,13 @@
  *                      iteration_mod.f90:944-1031
  *   moc_upload/moc_download <- direct access to grid_type components
  *                      common_mod.f90:308-371
+ *   moc_set_dust    <- dustDriver(grid) dust_mod.f90:14 (table part, with
+ *                      makeDustXsec's mixture ph_mod.f90:926-1585 and
+ *                      dustEmissionInt dust_mod.f90:146-184 as inputs)
+ *   moc_dust_opacity <- the dust block of iterateMC iteration_mod.f90:194-246
+ *   moc_dust_emission <- setDustPDF emission_mod.f90:1921-1986 (dust only)
+ *   moc_dust_temperature <- getDustT update_mod.f90:3346-3453 with the
+ *                      dust-only convergence of updateCell :240-267
  */
 #ifndef MOCGPU_H
 #define MOCGPU_H
@@ -73,6 +80,11 @@ extern "C" {
 #define MOC_F_CONVERGED 10 /* lgConverged (int32) */
 #define MOC_F_BLACK 11     /* lgBlack (int32) */
 #define MOC_F_EMISSION 12  /* diffuse continuum emissivity (cell, bin) */
+#define MOC_F_NDUST 13     /* grain number density [cm^-3] (cell) */
+#define MOC_F_TDUST 14     /* Tdust(0:nSpecies, 0:nSizes, 0:nCells) [K], species fastest */
+#define MOC_F_DUSTPDF 15   /* dust emission CDF (cell, bin), dust-only runs */
+#define MOC_F_ABSOPAC 16   /* dust absorption kappa [cm^-1] (cell, bin) */
+#define MOC_F_SCAOPAC 17   /* dust scattering kappa [cm^-1] (cell, bin) */
 
 typedef struct moc_ctx moc_ctx;
 
@@ -238,6 +250,25 @@ typedef struct moc_emission_tables {
     int32_t lymP[4];             /* bins of the He II Lyman lines, 0 = below the mesh */
 } moc_emission_tables;
 
+/* Grains (dustDriver, dust_mod.f90:14-186): the per-grain cross-sections of
+ * every (species, size), the Kirchhoff emission integrals and the mixture.
+ * Built once per mesh (the package's dust.build_dust). */
+typedef struct moc_dust_tables {
+    int32_t nSpecies, nSizes;
+    int32_t nTemps;              /* temperatures of emIntegral: 1..nTemps K (3000) */
+    int32_t lgGas;               /* 0: dust only (absorbed packets re-emitted from dustPDF);
+                                    1: gas + dust (re-emitted as diffuse packets) */
+    int32_t lgPhotoelectric;     /* grain photoelectric heating and gas-grain collisions
+                                    (update_mod.f90:484-884); must be 0 */
+    const double* grainAbun;     /* [nSpecies] */
+    const double* grainWeight;   /* [nSizes] */
+    const double* TdustSublime;  /* [nSpecies] K */
+    const double* Cabs;          /* [nSpecies][nSizes][nbins] cm^2 per grain */
+    const double* Csca;          /* [nSpecies][nSizes][nbins] */
+    const double* emIntegral;    /* [nSpecies][nSizes][nTemps] dustEmIntegral */
+    double XHILimit;             /* dust-only convergence limit on the mean grain T */
+} moc_dust_tables;
+
 /* Create a context: uploads the mesh and source tables. */
 int moc_create(moc_ctx** ctx, const moc_tables* tables);
 void moc_destroy(moc_ctx* ctx);
@@ -262,6 +293,27 @@ int moc_set_atomic(moc_ctx* ctx, const moc_atomic* atomic);
 /* kappa_nu of every cell from ionDen, Ne, Te, Hden (ionizationDriver,
  * ionization_mod.f90:26-477; free-free Gaunt factors at each cell's T_e). */
 int moc_opacity(moc_ctx* ctx, int ig);
+
+/* Upload the grain tables (after moc_grid_attach): allocates Ndust, Tdust
+ * (50 K, as dustDriver), absOpac, scaOpac.  Dust-only contexts need no
+ * atomic, balance or emission tables; their dust emission CDF is held in the
+ * recPDF storage (MOC_F_DUSTPDF and MOC_F_RECPDF are the same array). */
+int moc_set_dust(moc_ctx* ctx, const moc_dust_tables* tables);
+
+/* The dust block of iterateMC (iteration_mod.f90:194-246): absOpac, scaOpac
+ * of every cell from Ndust and the grains below their sublimation
+ * temperature, and opacity = gas opacity (gas + dust) + absOpac + scaOpac. */
+int moc_dust_opacity(moc_ctx* ctx, int ig);
+
+/* setDustPDF for every active cell (emission_mod.f90:1921-1986; dust only). */
+int moc_dust_emission(moc_ctx* ctx, int ig);
+
+/* getDustT for every active cell (update_mod.f90:3346-3453) from the
+ * normalised J: Tdust of every (species, size), the size-weighted species
+ * means and the abundance-weighted mean; cells no packet reached keep
+ * theirs (lgBlack).  In dust-only runs lgConverged comes from the change of
+ * the mean grain temperature (:240-267). */
+int moc_dust_temperature(moc_ctx* ctx, int ig, moc_update_stats* stats);
 
 /* Upload the balance tables (after moc_set_atomic). */
 int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* tables);

@@ -22,9 +22,10 @@ MOC_LAYOUT_FORTRAN, MOC_LAYOUT_CELLROW = 0, 1
 FIELDS = {
     "opacity": 1, "Jste": 2, "recPDF": 3, "totalLines": 4, "Ne": 5, "Te": 6,
     "Hden": 7, "escaped": 8, "ionDen": 9, "lgConverged": 10, "lgBlack": 11,
-    "emission": 12,
+    "emission": 12, "Ndust": 13, "Tdust": 14, "dustPDF": 15, "absOpac": 16, "scaOpac": 17,
 }
-CELLBIN_FIELDS = {"opacity", "Jste", "recPDF", "escaped", "emission"}
+CELLBIN_FIELDS = {"opacity", "Jste", "recPDF", "escaped", "emission", "dustPDF", "absOpac",
+                  "scaOpac"}
 
 # every symbol include/mocgpu.h declares
 EXPORTS = [
@@ -32,6 +33,7 @@ EXPORTS = [
     "moc_download", "moc_reset_tallies", "moc_transport", "moc_normalize_j",
     "moc_device_ptr", "moc_version", "moc_set_atomic", "moc_opacity", "moc_set_balance",
     "moc_update", "moc_set_emission", "moc_emission", "moc_set_packets", "moc_iterate",
+    "moc_set_dust", "moc_dust_opacity", "moc_dust_emission", "moc_dust_temperature",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -95,6 +97,15 @@ class MocEmissionTables(C.Structure):
         ("hlDens", _dp), ("hlLines", _dp), ("hbA", _dp), ("hbB", _dp), ("rbEdge", _dp),
         ("r2aEdge", _dp), ("heiRec", _dp), ("heiiLyman", _dp), ("j2TsP", C.c_int32),
         ("lymP", C.c_int32 * 4),
+    ]
+
+
+class MocDustTables(C.Structure):
+    _fields_ = [
+        ("nSpecies", C.c_int32), ("nSizes", C.c_int32), ("nTemps", C.c_int32),
+        ("lgGas", C.c_int32), ("lgPhotoelectric", C.c_int32), ("grainAbun", _dp),
+        ("grainWeight", _dp), ("TdustSublime", _dp), ("Cabs", _dp), ("Csca", _dp),
+        ("emIntegral", _dp), ("XHILimit", C.c_double),
     ]
 
 

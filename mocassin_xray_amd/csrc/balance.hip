@@ -600,11 +600,15 @@ k_balance(MocBalanceArgs a) {
 
 }  // namespace
 
+void moc_launch_rates(moc_ctx* ctx, const double* J, int C1, const double* W, int ncol, double* R) {
+    dim3 g1((unsigned)((C1 + RC - 1) / RC), (unsigned)((ncol + RN - 1) / RN));
+    hipLaunchKernelGGL(k_rates, g1, dim3(256), 0, ctx->stream, J, C1, ctx->nbins, W, ncol, R);
+}
+
 int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a, const double* J, const double* W,
                        int ncol, double* R) {
     const int C1 = a.nCells + 1;
-    dim3 g1((unsigned)((C1 + RC - 1) / RC), (unsigned)((ncol + RN - 1) / RN));
-    hipLaunchKernelGGL(k_rates, g1, dim3(256), 0, ctx->stream, J, C1, ctx->nbins, W, ncol, R);
+    moc_launch_rates(ctx, J, C1, W, ncol, R);
     const size_t base = balance_lds_bytes(a.NE, a.NS, a.nNlv);
     const size_t staged = base + balance_stage_bytes(a);
     const bool st = staged <= kMaxBalanceLds;

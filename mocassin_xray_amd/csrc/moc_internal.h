@@ -52,7 +52,10 @@ struct MocTransportArgs {
     // per-cell
     const T* opacity;     // [B][C+1]   (cell fastest)
     const T* Ne;          // [C+1]
-    const T* recPDF;      // [C+1][B]   (bin fastest)
+    const T* recPDF;      // [C+1][B]   (bin fastest); the dust CDF in dust-only runs
+    const T* scaOpac;     // [B][C+1]   dust scattering kappa (lgDust)
+    int lgDust;           // grains present: dust scattering/absorption, no early escape
+    int dustOnly;         // no gas: re-emission from the dust CDF, no line packets
     const T* totalLines;  // [C+1]
     // tallies
     double* J;            // [B][C+1]
@@ -227,6 +230,56 @@ struct MocEmission {
     std::vector<void*> owned;
 };
 
+// Grain tables of a context (moc_set_dust)
+struct MocDust {
+    bool set = false;
+    int nS = 0, nA = 0, nT = 0, lgGas = 0;
+    double XHILimit = 0.;
+    std::vector<void*> owned;   // device allocations of this table set
+    double *d_abun = nullptr, *d_weight = nullptr, *d_tsub = nullptr;
+    double *d_Cabs = nullptr, *d_Csca = nullptr, *d_emInt = nullptr;
+    double* d_widFlx = nullptr; // [B]
+    double* d_W = nullptr;      // [B][nS*nA+1] Cabs/pi and the hit column
+    double* d_R = nullptr;      // [nS*nA+1][C+1]
+    unsigned long long* d_counters = nullptr;  // [2]
+};
+
+// Dust kernel view (dust.hip)
+struct MocDustArgs {
+    int nCells, B, nS, nA, nT, dustOnly;
+    const double* abun;     // [nS]
+    const double* weight;   // [nA]
+    const double* tsub;     // [nS]
+    const double* Cabs;     // [nS][nA][B]
+    const double* Csca;     // [nS][nA][B]
+    const double* emInt;    // [nS][nA][nT]
+    const double* nu;       // [B]
+    const double* widFlx;   // [B]
+    const double* Ndust;    // [C+1]
+    double* Tdust;          // [C+1][nA+1][nS+1]
+    double* absOpac;        // [B][C+1]
+    double* scaOpac;        // [B][C+1]
+    const double* gasOpac;  // [B][C+1] or null (dust only)
+    double* opacity;        // [B][C+1]
+    double* pdf;            // [C+1][B] dust-only emission CDF
+    const double* R;        // [nS*nA+1][C+1] absorption integrals, last row the hit test
+    int* lgConverged;
+    int* lgBlack;
+    unsigned long long* counters;  // noHit, converged
+    double XHILimit;
+};
+
+// getFlux('blackbody') (continuum_mod.f90:358-400): the Planck function over
+// h in erg/(cm^2 s Hz sr) / h, with its Wien and Rayleigh-Jeans branches
+__host__ __device__ inline double moc_planck_h(double nu, double T) {
+    const double constant = 0.5250229 / 6.6262e-27;
+    const double x = 157893.94 * nu / T;
+    if (x > 86.) return constant * nu * nu * nu * exp(-x);
+    const double den = exp(x) - 1.;
+    if (den <= 0.) return 3.32154e-6 * nu * nu * T / 6.6262e-27;
+    return constant * nu * nu * nu / den;
+}
+
 struct MocGrid {
     bool attached = false;
     int nx = 0, ny = 0, nz = 0, nCells = 0, sym = 0;
@@ -256,6 +309,13 @@ struct MocGrid {
     float* f_recPDF = nullptr;
     float* f_totalLines = nullptr;
     bool f32_dirty = true;
+    // grains (moc_set_dust)
+    double* d_Ndust = nullptr;       // [C+1]
+    double* d_Tdust = nullptr;       // [C+1][nSizes+1][nSpecies+1] (species fastest)
+    double* d_absOpac = nullptr;     // [B][C+1]
+    double* d_scaOpac = nullptr;     // [B][C+1]
+    double* d_gasOpac = nullptr;     // [B][C+1] gas kappa of gas + dust runs
+    float* f_scaOpac = nullptr;
     double* d_ionDen = nullptr;      // [NS][NE][C+1]
     int* d_conv = nullptr;           // [C+1] lgConverged
     int* d_black = nullptr;          // [C+1] lgBlack
@@ -301,6 +361,7 @@ struct moc_ctx {
     MocAtomic atomic;
     MocBalance balance;
     MocEmission emission;
+    MocDust dust;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -312,5 +373,11 @@ void moc_convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_
 void moc_scale_f64(double* p, int64_t n, double f, hipStream_t s);
 int moc_launch_opacity(moc_ctx* ctx, const MocOpacityArgs& a);
 int moc_launch_emission(moc_ctx* ctx, const MocEmissionArgs& a);
+int moc_launch_dust_opacity(moc_ctx* ctx, const MocDustArgs& a);
+int moc_launch_dust_pdf(moc_ctx* ctx, const MocDustArgs& a);
+int moc_launch_dust_temp(moc_ctx* ctx, const MocDustArgs& a, const double* J, const double* W, int ncol,
+                         double* R);
+// R[col][cell] = sum_nu J[nu][cell] W[nu][col] (k_rates, balance.hip)
+void moc_launch_rates(moc_ctx* ctx, const double* J, int C1, const double* W, int ncol, double* R);
 int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a, const double* J, const double* W,
                        int ncol, double* R);

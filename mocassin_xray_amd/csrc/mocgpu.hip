@@ -62,7 +62,15 @@ void free_grid(MocGrid& g) {
     dfree(g.f_opacity); dfree(g.f_Ne); dfree(g.f_recPDF); dfree(g.f_totalLines);
     dfree(g.d_ionDen);
     dfree(g.d_conv); dfree(g.d_black); dfree(g.d_nIterT); dfree(g.d_rates);
+    dfree(g.d_Ndust); dfree(g.d_Tdust); dfree(g.d_absOpac); dfree(g.d_scaOpac);
+    dfree(g.d_gasOpac); dfree(g.f_scaOpac);
     g = MocGrid();
+}
+
+void free_dust(MocDust& d) {
+    for (void* p : d.owned) (void)hipFree(p);
+    d.owned.clear();
+    d = MocDust();
 }
 
 void free_balance(MocBalance& b) {
@@ -77,6 +85,21 @@ void free_emission(MocEmission& e) {
     for (void* p : e.owned) (void)hipFree(p);
     e.owned.clear();
     e = MocEmission();
+}
+
+// copy a host table into a new device buffer owned by the grain set
+template <typename T>
+int dust_put(moc_ctx* ctx, const T* h, size_t n, T** out, const char* what) {
+    T* d = nullptr;
+    int rc = dalloc_n(ctx, &d, n ? n : 1, what);
+    if (rc) return rc;
+    ctx->dust.owned.push_back((void*)d);
+    if (n && h) {
+        rc = h2d(ctx, d, h, n * sizeof(T), what);
+        if (rc) return rc;
+    }
+    *out = d;
+    return MOC_OK;
 }
 
 // copy a host table into a new device buffer owned by the emission set
@@ -172,7 +195,8 @@ std::vector<double> widths(const std::vector<double>& a, bool sym) {
 }
 
 bool is_cellbin(int f) {
-    return f == MOC_F_OPACITY || f == MOC_F_JSTE || f == MOC_F_RECPDF || f == MOC_F_ESCAPED;
+    return f == MOC_F_OPACITY || f == MOC_F_JSTE || f == MOC_F_RECPDF || f == MOC_F_ESCAPED ||
+           f == MOC_F_DUSTPDF || f == MOC_F_ABSOPAC || f == MOC_F_SCAOPAC;
 }
 
 double* field_ptr(moc_ctx* ctx, int f) {
@@ -187,18 +211,26 @@ double* field_ptr(moc_ctx* ctx, int f) {
         case MOC_F_HDEN: return g.d_Hden;
         case MOC_F_ESCAPED: return g.d_escaped;
         case MOC_F_IONDEN: return g.d_ionDen;
+        case MOC_F_NDUST: return g.d_Ndust;
+        case MOC_F_TDUST: return g.d_Tdust;
+        case MOC_F_DUSTPDF: return ctx->dust.set && !ctx->dust.lgGas ? g.d_recPDF : nullptr;
+        case MOC_F_ABSOPAC: return g.d_absOpac;
+        case MOC_F_SCAOPAC: return g.d_scaOpac;
         default: return nullptr;
     }
 }
 
 // device storage order of a (cell, bin) field: recPDF is bin-fastest (rows
 // searched per cell), the others cell-fastest (planes walked per bin)
-int device_layout(int f) { return f == MOC_F_RECPDF ? MOC_LAYOUT_CELLROW : MOC_LAYOUT_FORTRAN; }
+int device_layout(int f) {
+    return f == MOC_F_RECPDF || f == MOC_F_DUSTPDF ? MOC_LAYOUT_CELLROW : MOC_LAYOUT_FORTRAN;
+}
 
 int64_t field_count(moc_ctx* ctx, int f) {
     const int64_t C1 = ctx->grid.nCells + 1;
     if (is_cellbin(f)) return C1 * ctx->nbins;
     if (f == MOC_F_IONDEN) return ionden_count(ctx);
+    if (f == MOC_F_TDUST) return C1 * (ctx->dust.nA + 1) * (ctx->dust.nS + 1);
     return C1;
 }
 
@@ -210,6 +242,7 @@ int refresh_f32(moc_ctx* ctx) {
     moc_convert_f64_to_f32(g.d_recPDF, g.f_recPDF, C1 * B, ctx->stream);
     moc_convert_f64_to_f32(g.d_Ne, g.f_Ne, C1, ctx->stream);
     moc_convert_f64_to_f32(g.d_totalLines, g.f_totalLines, C1, ctx->stream);
+    if (g.d_scaOpac) moc_convert_f64_to_f32(g.d_scaOpac, g.f_scaOpac, C1 * B, ctx->stream);
     MOC_TRY(hipGetLastError(), "f32 refresh");
     g.f32_dirty = false;
     return MOC_OK;
@@ -296,6 +329,7 @@ void moc_destroy(moc_ctx* ctx) {
     free_atomic(ctx->atomic);
     free_balance(ctx->balance);
     free_emission(ctx->emission);
+    free_dust(ctx->dust);
     dfree(ctx->d_nu); dfree(ctx->d_KNsigmaT); dfree(ctx->d_KNcdf); dfree(ctx->d_Pcomp);
     dfree(ctx->d_sourceCDF);
     dfree(ctx->f_nu); dfree(ctx->f_KNsigmaT); dfree(ctx->f_KNcdf); dfree(ctx->f_Pcomp);
@@ -315,6 +349,7 @@ int moc_grid_attach(moc_ctx* ctx, int ig, const moc_grid_desc* gd) {
         return MOC_ERR_ARG;
     }
     free_grid(ctx->grid);
+    free_dust(ctx->dust);  // its per-cell arrays belonged to the old grid
     MocGrid& g = ctx->grid;
     g.nx = gd->nx; g.ny = gd->ny; g.nz = gd->nz;
     g.nCells = gd->nCells;
@@ -462,6 +497,32 @@ int moc_set_atomic(moc_ctx* ctx, const moc_atomic* at) {
 int run_opacity(moc_ctx* ctx, bool skipUnchanged, float* ms);
 int run_emission(moc_ctx* ctx, bool skipUnchanged, float* ms);
 
+MocDustArgs dust_args(moc_ctx* ctx) {
+    MocGrid& g = ctx->grid;
+    MocDust& d = ctx->dust;
+    MocDustArgs a{};
+    a.nCells = g.nCells; a.B = ctx->nbins; a.nS = d.nS; a.nA = d.nA; a.nT = d.nT;
+    a.dustOnly = d.lgGas ? 0 : 1;
+    a.abun = d.d_abun; a.weight = d.d_weight; a.tsub = d.d_tsub;
+    a.Cabs = d.d_Cabs; a.Csca = d.d_Csca; a.emInt = d.d_emInt;
+    a.nu = ctx->d_nu; a.widFlx = d.d_widFlx;
+    a.Ndust = g.d_Ndust; a.Tdust = g.d_Tdust; a.absOpac = g.d_absOpac; a.scaOpac = g.d_scaOpac;
+    a.gasOpac = d.lgGas ? g.d_gasOpac : nullptr;
+    a.opacity = g.d_opacity;
+    a.pdf = g.d_recPDF;
+    a.R = d.d_R;
+    a.lgConverged = g.d_conv; a.lgBlack = g.d_black;
+    a.counters = d.d_counters;
+    a.XHILimit = d.XHILimit;
+    return a;
+}
+
+int launch_dust_opacity(moc_ctx* ctx) {
+    const int rc = moc_launch_dust_opacity(ctx, dust_args(ctx));
+    ctx->grid.f32_dirty = true;
+    return rc;
+}
+
 int moc_opacity(moc_ctx* ctx, int ig) {
     if (!ctx) return MOC_ERR_ARG;
     MocGrid& g = ctx->grid;
@@ -481,12 +542,18 @@ int run_opacity(moc_ctx* ctx, bool skipUnchanged, float* ms) {
     a.specElem = at.d_specElem; a.specIon = at.d_specIon; a.elemZ = at.d_elemZ;
     a.abund = at.d_abund; a.sigma = at.d_sigma; a.brems = at.d_brems; a.nuArray = ctx->d_nu;
     a.log10nu = at.d_log10nu; a.KNsigmaT = ctx->d_KNsigmaT; a.ionDen = g.d_ionDen;
-    a.Ne = g.d_Ne; a.Te = g.d_Te; a.Hden = g.d_Hden; a.opacity = g.d_opacity;
+    a.Ne = g.d_Ne; a.Te = g.d_Te; a.Hden = g.d_Hden;
+    // with grains the gas kappa keeps a plane of its own and the total is
+    // formed by the dust block, so the dust of a cell whose gas part is kept
+    // is not added twice
+    const bool dust = ctx->dust.set && ctx->dust.lgGas;
+    a.opacity = dust ? g.d_gasOpac : g.d_opacity;
     a.TeOld = skipUnchanged ? g.d_TeOld : nullptr;
     a.NeOld = skipUnchanged ? g.d_NeOld : nullptr;
     MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
     int rc = moc_launch_opacity(ctx, a);
     if (rc) return rc;
+    if (dust && (rc = launch_dust_opacity(ctx))) return rc;
     MOC_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
     MOC_TRY(hipEventSynchronize(ctx->ev1), "opacity");
     if (ms) (void)hipEventElapsedTime(ms, ctx->ev0, ctx->ev1);
@@ -620,6 +687,8 @@ int moc_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ct
         a.KNcdf = ctx->d_KNcdf; a.Pcomp = ctx->d_Pcomp;
         a.sourceCDF = ctx->d_sourceCDF + (size_t)s * ctx->nbins;
         a.opacity = g.d_opacity; a.Ne = g.d_Ne; a.recPDF = g.d_recPDF; a.totalLines = g.d_totalLines;
+        a.scaOpac = g.d_scaOpac; a.lgDust = ctx->dust.set ? 1 : 0;
+        a.dustOnly = ctx->dust.set && !ctx->dust.lgGas ? 1 : 0;
         a.J = g.d_J; a.escaped = g.d_escaped; a.Qphot = ctx->d_Qphot; a.counters = ctx->d_counters;
         for (int k = 0; k < 3; ++k) { a.star[k] = ctx->starPos[3 * s + k]; a.starIdx[k] = ctx->starIdx[3 * s + k] - 1; }
         a.deltaE = deltaE; a.n = n; a.seed = seed; a.ctr_base = ctr_base;
@@ -635,6 +704,8 @@ int moc_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ct
         a.KNcdf = ctx->f_KNcdf; a.Pcomp = ctx->f_Pcomp;
         a.sourceCDF = ctx->f_sourceCDF + (size_t)s * ctx->nbins;
         a.opacity = g.f_opacity; a.Ne = g.f_Ne; a.recPDF = g.f_recPDF; a.totalLines = g.f_totalLines;
+        a.scaOpac = g.f_scaOpac; a.lgDust = ctx->dust.set ? 1 : 0;
+        a.dustOnly = ctx->dust.set && !ctx->dust.lgGas ? 1 : 0;
         a.J = g.d_J; a.escaped = g.d_escaped; a.Qphot = ctx->d_Qphot; a.counters = ctx->d_counters;
         for (int k = 0; k < 3; ++k) { a.star[k] = (float)ctx->starPos[3 * s + k]; a.starIdx[k] = ctx->starIdx[3 * s + k] - 1; }
         a.deltaE = (float)deltaE; a.n = n; a.seed = seed; a.ctr_base = ctr_base;
@@ -817,6 +888,8 @@ int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* t) {
 int moc_update(moc_ctx* ctx, int ig, moc_update_stats* stats) {
     if (!ctx) return MOC_ERR_ARG;
     MocGrid& g = ctx->grid;
+    if (g.attached && ig == 1 && ctx->dust.set && !ctx->dust.lgGas)  // updateCell without gas (:240-267)
+        return moc_dust_temperature(ctx, ig, stats);
     if (!g.attached || ig != 1 || !ctx->balance.set || !g.d_ionDen) {
         ctx->err = "moc_update: needs an attached grid, moc_set_atomic and moc_set_balance";
         return MOC_ERR_STATE;
@@ -843,6 +916,11 @@ int moc_update(moc_ctx* ctx, int ig, moc_update_stats* stats) {
     MOC_TRY(hipMemcpyAsync(g.d_NeOld, g.d_Ne, C1b, hipMemcpyDeviceToDevice, ctx->stream), "NeOld");
     MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
     if ((rc = moc_launch_balance(ctx, a, g.d_J, W, b.ncol, g.d_rates))) return rc;
+    if (ctx->dust.set) {  // getDustT after iterateT (update_mod.f90:224-227)
+        MocDust& d = ctx->dust;
+        if ((rc = moc_launch_dust_temp(ctx, dust_args(ctx), g.d_J, d.d_W, d.nS * d.nA + 1, d.d_R)))
+            return rc;
+    }
     MOC_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
     MOC_TRY(hipEventSynchronize(ctx->ev1), "balance kernel");
     float ms = 0.f;
@@ -1049,16 +1127,34 @@ int moc_set_packets(moc_ctx* ctx, int iStar, int64_t n) {
 int moc_iterate(moc_ctx* ctx, int ig, int nIterateMC, uint64_t seed, moc_iteration_stats* stats) {
     if (!ctx) return MOC_ERR_ARG;
     MocGrid& g = ctx->grid;
-    if (!g.attached || ig != 1 || !ctx->emission.set || !ctx->balance.set || !g.d_ionDen) {
-        ctx->err = "moc_iterate: needs an attached grid, moc_set_atomic, moc_set_balance and moc_set_emission";
+    const bool dustOnly = ctx->dust.set && !ctx->dust.lgGas;
+    if (!g.attached || ig != 1 ||
+        (!dustOnly && (!ctx->emission.set || !ctx->balance.set || !g.d_ionDen))) {
+        ctx->err = "moc_iterate: needs an attached grid and either moc_set_atomic, moc_set_balance and "
+                   "moc_set_emission, or moc_set_dust without gas";
         return MOC_ERR_STATE;
     }
     if (nIterateMC < 1 || nIterateMC > (1 << 23)) { ctx->err = "moc_iterate: nIterateMC"; return MOC_ERR_ARG; }
     const bool skip = nIterateMC > 1;
     float tOp = 0.f, tEm = 0.f;
     int rc;
-    if ((rc = run_opacity(ctx, skip, &tOp))) return rc;
-    if ((rc = run_emission(ctx, skip, &tEm))) return rc;
+    if (dustOnly) {
+        // dust kappa and setDustPDF of every cell (no skips without gas, :368-386)
+        MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
+        if ((rc = launch_dust_opacity(ctx))) return rc;
+        MOC_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
+        MOC_TRY(hipEventSynchronize(ctx->ev1), "dust opacity");
+        (void)hipEventElapsedTime(&tOp, ctx->ev0, ctx->ev1);
+        MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
+        if ((rc = moc_launch_dust_pdf(ctx, dust_args(ctx)))) return rc;
+        MOC_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
+        MOC_TRY(hipEventSynchronize(ctx->ev1), "dust emission");
+        (void)hipEventElapsedTime(&tEm, ctx->ev0, ctx->ev1);
+        g.f32_dirty = true;
+    } else {
+        if ((rc = run_opacity(ctx, skip, &tOp))) return rc;
+        if ((rc = run_emission(ctx, skip, &tEm))) return rc;
+    }
     if ((rc = moc_reset_tallies(ctx, ig))) return rc;
     double packets = 0.0, tTr = 0.0, seg = 0.0, emi = 0.0, esc = 0.0;
     for (int s = 1; s <= ctx->nStars; ++s) {
@@ -1089,6 +1185,128 @@ int moc_iterate(moc_ctx* ctx, int ig, int nIterateMC, uint64_t seed, moc_iterati
         stats->segments = seg;
         stats->emissions = emi;
         stats->escaped = esc;
+    }
+    return MOC_OK;
+}
+
+int moc_set_dust(moc_ctx* ctx, const moc_dust_tables* t) {
+    if (!ctx) return MOC_ERR_ARG;
+    MocGrid& g = ctx->grid;
+    if (!g.attached) { ctx->err = "moc_set_dust: attach the grid first"; return MOC_ERR_STATE; }
+    if (!t || t->nSpecies < 1 || t->nSizes < 1 || t->nTemps < 2 || !t->grainAbun || !t->grainWeight ||
+        !t->TdustSublime || !t->Cabs || !t->Csca || !t->emIntegral || (t->lgGas != 0 && t->lgGas != 1)) {
+        ctx->err = "moc_set_dust: invalid grain tables";
+        return MOC_ERR_ARG;
+    }
+    if (t->lgPhotoelectric) {
+        ctx->err = "moc_set_dust: grain photoelectric heating and gas-grain collisions are not built "
+                   "(run with noPhotoelectric)";
+        return MOC_ERR_ARG;
+    }
+    if ((int64_t)t->nSpecies * t->nSizes + 1 > 4096) {
+        ctx->err = "moc_set_dust: more than 4095 (species, size) pairs";
+        return MOC_ERR_ARG;
+    }
+    free_dust(ctx->dust);
+    MocDust& d = ctx->dust;
+    const int B = ctx->nbins, nS = t->nSpecies, nA = t->nSizes, nT = t->nTemps, nG = nS * nA;
+    const size_t C1 = (size_t)g.nCells + 1;
+    d.nS = nS; d.nA = nA; d.nT = nT; d.lgGas = t->lgGas; d.XHILimit = t->XHILimit;
+    int rc;
+    // the absorption integrals sum_nu Cabs J/pi as rate columns, plus the hit test
+    std::vector<double> W((size_t)B * (nG + 1));
+    for (int b = 0; b < B; ++b) {
+        for (int k = 0; k < nG; ++k) W[(size_t)b * (nG + 1) + k] = t->Cabs[(size_t)k * B + b] / 3.141592654;
+        W[(size_t)b * (nG + 1) + nG] = 1.0;
+    }
+    if ((rc = dust_put(ctx, t->grainAbun, (size_t)nS, &d.d_abun, "grainAbun")) ||
+        (rc = dust_put(ctx, t->grainWeight, (size_t)nA, &d.d_weight, "grainWeight")) ||
+        (rc = dust_put(ctx, t->TdustSublime, (size_t)nS, &d.d_tsub, "TdustSublime")) ||
+        (rc = dust_put(ctx, t->Cabs, (size_t)nG * B, &d.d_Cabs, "Cabs")) ||
+        (rc = dust_put(ctx, t->Csca, (size_t)nG * B, &d.d_Csca, "Csca")) ||
+        (rc = dust_put(ctx, t->emIntegral, (size_t)nG * nT, &d.d_emInt, "emIntegral")) ||
+        (rc = dust_put(ctx, ctx->widFlx.data(), (size_t)B, &d.d_widFlx, "widFlx")) ||
+        (rc = dust_put(ctx, W.data(), W.size(), &d.d_W, "dust W")) ||
+        (rc = dust_put(ctx, (const double*)nullptr, (size_t)(nG + 1) * C1, &d.d_R, "dust R")) ||
+        (rc = dust_put(ctx, (const unsigned long long*)nullptr, 2, &d.d_counters, "dust counters")))
+        return rc;
+    // per-cell arrays of the grid: Ndust 0, Tdust 50 K (dust_mod.f90:66), kappa 0
+    const size_t nTd = C1 * (nA + 1) * (nS + 1);
+    dfree(g.d_Ndust); dfree(g.d_Tdust); dfree(g.d_absOpac); dfree(g.d_scaOpac);
+    dfree(g.d_gasOpac); dfree(g.f_scaOpac);
+    if ((rc = dalloc_n(ctx, &g.d_Ndust, C1, "Ndust")) || (rc = dalloc_n(ctx, &g.d_Tdust, nTd, "Tdust")) ||
+        (rc = dalloc_n(ctx, &g.d_absOpac, C1 * B, "absOpac")) ||
+        (rc = dalloc_n(ctx, &g.d_scaOpac, C1 * B, "scaOpac")) ||
+        (rc = dalloc_n(ctx, &g.f_scaOpac, C1 * B, "scaOpac32")) ||
+        (t->lgGas && (rc = dalloc_n(ctx, &g.d_gasOpac, C1 * B, "gas opacity"))))
+        return rc;
+    if (!g.d_conv && ((rc = dalloc_n(ctx, &g.d_conv, C1, "lgConverged")) ||
+                      (rc = dalloc_n(ctx, &g.d_black, C1, "lgBlack"))))
+        return rc;
+    std::vector<double> t50(nTd, 50.0);
+    if ((rc = h2d(ctx, g.d_Tdust, t50.data(), nTd * 8, "Tdust"))) return rc;
+    MOC_TRY(hipMemsetAsync(g.d_Ndust, 0, C1 * 8, ctx->stream), "zero");
+    MOC_TRY(hipMemsetAsync(g.d_absOpac, 0, C1 * B * 8, ctx->stream), "zero");
+    MOC_TRY(hipMemsetAsync(g.d_scaOpac, 0, C1 * B * 8, ctx->stream), "zero");
+    if (g.d_gasOpac) MOC_TRY(hipMemsetAsync(g.d_gasOpac, 0, C1 * B * 8, ctx->stream), "zero");
+    MOC_TRY(hipMemsetAsync(g.d_conv, 0, C1 * 4, ctx->stream), "zero");
+    MOC_TRY(hipMemsetAsync(g.d_black, 0, C1 * 4, ctx->stream), "zero");
+    MOC_TRY(hipStreamSynchronize(ctx->stream), "moc_set_dust");
+    d.set = true;
+    g.f32_dirty = true;
+    return MOC_OK;
+}
+
+int moc_dust_opacity(moc_ctx* ctx, int ig) {
+    if (!ctx) return MOC_ERR_ARG;
+    if (!ctx->grid.attached || ig != 1 || !ctx->dust.set) {
+        ctx->err = "moc_dust_opacity: needs an attached grid and moc_set_dust";
+        return MOC_ERR_STATE;
+    }
+    if (ctx->dust.lgGas) return run_opacity(ctx, false, nullptr);  // gas kappa, then the dust block
+    int rc = launch_dust_opacity(ctx);
+    if (rc) return rc;
+    MOC_TRY(hipStreamSynchronize(ctx->stream), "dust opacity");
+    return MOC_OK;
+}
+
+int moc_dust_emission(moc_ctx* ctx, int ig) {
+    if (!ctx) return MOC_ERR_ARG;
+    if (!ctx->grid.attached || ig != 1 || !ctx->dust.set || ctx->dust.lgGas) {
+        ctx->err = "moc_dust_emission: needs an attached grid and moc_set_dust with lgGas = 0";
+        return MOC_ERR_STATE;
+    }
+    int rc = moc_launch_dust_pdf(ctx, dust_args(ctx));
+    if (rc) return rc;
+    MOC_TRY(hipStreamSynchronize(ctx->stream), "dust emission");
+    ctx->grid.f32_dirty = true;
+    return MOC_OK;
+}
+
+int moc_dust_temperature(moc_ctx* ctx, int ig, moc_update_stats* stats) {
+    if (!ctx) return MOC_ERR_ARG;
+    MocGrid& g = ctx->grid;
+    if (!g.attached || ig != 1 || !ctx->dust.set) {
+        ctx->err = "moc_dust_temperature: needs an attached grid and moc_set_dust";
+        return MOC_ERR_STATE;
+    }
+    MocDust& d = ctx->dust;
+    MOC_TRY(hipMemsetAsync(d.d_counters, 0, 2 * sizeof(unsigned long long), ctx->stream), "counters");
+    MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
+    int rc = moc_launch_dust_temp(ctx, dust_args(ctx), g.d_J, d.d_W, d.nS * d.nA + 1, d.d_R);
+    if (rc) return rc;
+    MOC_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
+    MOC_TRY(hipEventSynchronize(ctx->ev1), "dust temperature");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    unsigned long long cnt[2];
+    MOC_TRY(hipMemcpy(cnt, d.d_counters, sizeof(cnt), hipMemcpyDeviceToHost), "counters");
+    if (stats) {
+        stats->noHit = (double)cnt[0];
+        stats->noIonBal = 0.;
+        stats->noTeBal = 0.;
+        stats->converged = (double)cnt[1];
+        stats->kernel_ms = ms;
     }
     return MOC_OK;
 }

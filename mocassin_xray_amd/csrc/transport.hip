@@ -3,8 +3,9 @@
 // and diffuse re-emission, escape tallies.
 //
 // Semantics follow MOCASSIN's energyPacketDriver (photon_mod.f90:28-414) for a
-// single Cartesian grid without sub-grids, plane-parallel illumination, dust
-// or fluorescence:
+// single Cartesian grid without sub-grids, plane-parallel illumination or
+// fluorescence, with grains (dust scattering and absorption, dust-emitted
+// packets) when the grid has them:
 //   energyPacketRun          photon_mod.f90:418-741
 //   initPhotonPacket         photon_mod.f90:744-896 (randomUnitVector
 //                            vector_mod.f90:303-314, octant folding :875-882)
@@ -187,11 +188,13 @@ __device__ __forceinline__ void emit(const MocTransportArgs<T>& a, Pk<T>& p, Moc
     } else {
         const int cell = cell_of(a, p.ix, p.iy, p.iz);
         if (cell <= 0) { ++c.err; p.state = ST_DONE; return; }
-        const T r = T(1) - rng.uniform<T>();
-        if (r <= a.totalLines[cell]) {  // line photon: leaves the continuum transfer
-            ++c.line;
-            p.state = ST_DONE;
-            return;
+        if (!a.dustOnly) {  // a dustEmi packet (:1673-1741) is never a line
+            const T r = T(1) - rng.uniform<T>();
+            if (r <= a.totalLines[cell]) {  // line photon: leaves the continuum transfer
+                ++c.line;
+                p.state = ST_DONE;
+                return;
+            }
         }
         p.nu0 = get_nu2(a.recPDF + (size_t)cell * (size_t)a.nbins, a.nbins, rng, &bad);
     }
@@ -204,7 +207,7 @@ __device__ __forceinline__ void emit(const MocTransportArgs<T>& a, Pk<T>& p, Moc
     }
     p.origin = cell_of(a, p.ix, p.iy, p.iz);
     p.nu = a.nuArray[p.nu0];
-    if (p.nu < a.ionEdge1) {  // below every edge and no dust: escapes at once
+    if (!a.lgDust && p.nu < a.ionEdge1) {  // below every edge and no dust: escapes at once
         tally_escape(a, p, c);
         p.state = ST_DONE;
         return;
@@ -323,6 +326,34 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const Geo<
                 p.state = ST_DONE;
                 return;
             }
+        }
+        if (a.lgDust) {
+            // grains (:2385-2500): scattered with probability scaOpac/kappa,
+            // isotropically; otherwise absorbed and re-emitted (dustEmi
+            // without gas, diffuse with it); no Compton scattering
+            const T pSca = a.scaOpac[row + (size_t)cell] / kap;
+            const T rd = T(1) - rng.uniform<T>();
+            if (rd <= pSca) {
+                ++c.sca;
+                unit_vector(rng, p.u, p.v, p.w);
+                p.origin = cell;
+                p.absTau = T(0);
+                p.passProb = -log(T(1) - rng.uniform<T>());
+                p.kcell = cell;
+                p.kap = kap;
+                atomicAdd(Jc, dJ);
+                return;
+            }
+            atomicAdd(Jc, dJ);
+            ++c.absb;
+            if (p.emits >= MOC_RECURSION_LIMIT) {
+                ++c.trap;
+                p.state = ST_DONE;
+                return;
+            }
+            p.stellar = 0;
+            p.state = ST_EMIT;
+            return;
         }
         T probSca = T(0);
         if (a.lgCompton) probSca = a.KNsigmaT[p.nu0] * a.Ne[cell] / kap;

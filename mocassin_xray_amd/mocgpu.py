@@ -22,8 +22,9 @@ import numpy as np
 
 from . import _lib
 from ._lib import (CELLBIN_FIELDS, FIELDS, MOC_DT_F64, MOC_DT_I32, MOC_LAYOUT_FORTRAN,
-                   MOC_PREC_FP64, MOC_PREC_MIXED, MocAtomic, MocBalanceTables, MocEmissionTables,
-                   MocGridDesc, MocIterationStats, MocTables, MocTransportStats, MocUpdateStats)
+                   MOC_PREC_FP64, MOC_PREC_MIXED, MocAtomic, MocBalanceTables, MocDustTables,
+                   MocEmissionTables, MocGridDesc, MocIterationStats, MocTables, MocTransportStats,
+                   MocUpdateStats)
 
 
 class MocError(RuntimeError):
@@ -153,6 +154,15 @@ def emission_arrays(et) -> dict:
     }
 
 
+def dust_arrays(dt, lgGas: bool, XHILimit: float) -> dict:
+    return {
+        "nSpecies": dt.nSpecies, "nSizes": dt.nSizes, "nTemps": int(dt.emIntegral.shape[2]),
+        "lgGas": 1 if lgGas else 0, "lgPhotoelectric": 0, "grainAbun": _c(dt.grainAbun),
+        "grainWeight": _c(dt.grainWeight), "TdustSublime": _c(dt.Tsub), "Cabs": _c(dt.Cabs),
+        "Csca": _c(dt.Csca), "emIntegral": _c(dt.emIntegral), "XHILimit": float(XHILimit),
+    }
+
+
 def to_struct(cls, d: dict):
     """The ctypes struct `cls` filled from a field dict (arrays stay owned by d)."""
     kw = {}
@@ -225,6 +235,8 @@ class MocGPU:
         c1 = self.grid.nCells + 1
         if name == "ionDen":
             return c1 * self.n_elem * self.nstages
+        if name == "Tdust":
+            return c1 * (self.dust_shape[0] + 1) * (self.dust_shape[1] + 1)
         return c1 * self.nbins if name in CELLBIN_FIELDS else c1
 
     # ----------------------------------------------------------------- atomic
@@ -267,6 +279,31 @@ class MocGPU:
         self._check(self._L.moc_update(self._ctx, 1, C.byref(st)), "moc_update")
         return st.as_dict()
 
+    # ------------------------------------------------------------------- dust
+    def set_dust(self, dt, lgGas: bool, XHILimit: float = 0.05) -> None:
+        """Grain tables (a dust.DustTables built on the same mesh); after
+        grid_attach.  Tdust starts at 50 K, Ndust at 0."""
+        self._dust_keep = dust_arrays(dt, lgGas, XHILimit)
+        t = to_struct(MocDustTables, self._dust_keep)
+        self._check(self._L.moc_set_dust(self._ctx, C.byref(t)), "moc_set_dust")
+        self.dust_shape = (dt.nSpecies, dt.nSizes)
+
+    def dustOpacity(self):
+        """absOpac, scaOpac and the total opacity of every cell (the dust
+        block of iterateMC, iteration_mod.f90:194-246)."""
+        self._check(self._L.moc_dust_opacity(self._ctx, 1), "moc_dust_opacity")
+
+    def setDustPDF(self):
+        """Dust emission CDF of every cell (emission_mod.f90:1921-1986)."""
+        self._check(self._L.moc_dust_emission(self._ctx, 1), "moc_dust_emission")
+
+    def getDustT(self) -> dict:
+        """Grain temperatures of every cell from the normalised J
+        (update_mod.f90:3346-3453)."""
+        st = MocUpdateStats()
+        self._check(self._L.moc_dust_temperature(self._ctx, 1, C.byref(st)), "moc_dust_temperature")
+        return st.as_dict()
+
     def download_flags(self, name: str) -> np.ndarray:
         """lgConverged / lgBlack as int32 (nCells+1,)."""
         n = self.grid.nCells + 1
@@ -282,6 +319,10 @@ class MocGPU:
         if name == "ionDen":             # (C+1, nElemUsed, nstages) -> Fortran order
             assert a.shape == (c1, self.n_elem, self.nstages), a.shape
             buf = np.ascontiguousarray(a.transpose(2, 1, 0))
+        elif name == "Tdust":            # (C+1, nSizes+1, nSpecies+1): species fastest
+            nS, nA = self.dust_shape
+            assert a.shape == (c1, nA + 1, nS + 1), a.shape
+            buf = np.ascontiguousarray(a)
         elif name in CELLBIN_FIELDS:
             assert a.shape == (c1, self.nbins), (name, a.shape)
             buf = np.ascontiguousarray(a.T)      # Fortran order: cell fastest
@@ -301,6 +342,9 @@ class MocGPU:
             return np.ascontiguousarray(buf.reshape(self.nbins, c1).T)
         if name == "ionDen":
             return np.ascontiguousarray(buf.reshape(self.nstages, self.n_elem, c1).transpose(2, 1, 0))
+        if name == "Tdust":
+            nS, nA = self.dust_shape
+            return buf.reshape(c1, nA + 1, nS + 1)
         return buf
 
     # -------------------------------------------------------------- transport

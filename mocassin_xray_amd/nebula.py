@@ -214,3 +214,53 @@ def gas_nebula(n: int = 32, nbins: int = 1000, n_packets: int = 10_000_000,
     return gas_model("nebula%d" % n, n, 1.e18, 1000., NEBULA_ELEMS,
                      10.0 ** np.array(NEBULA_LOG_ABUND), nstages, nbins, 100.0, 1.e5,
                      10 ** 1.5, n_packets, maxLev=5)
+
+
+# ------------------------------------------------------------------ dust models
+@dataclass
+class DustModel:
+    """A dust-only model (nebComposition noGas) ready for the Monte Carlo
+    iteration: grid, mesh/source tables, grain tables and Ndust."""
+    name: str
+    tables: SourceTables
+    grid: CartesianGrid
+    dust: object           # dust.DustTables
+    Ndust: np.ndarray      # (C+1,)
+    n_packets: int
+    XHILimit: float = 0.05
+
+    def load(self, g) -> None:
+        """Upload everything into a MocGPU context."""
+        g.grid_attach(self.grid)
+        g.set_dust(self.dust, lgGas=False, XHILimit=self.XHILimit)
+        g.upload("Ndust", self.Ndust)
+
+
+def shell_axes(n: int, R: float, a1: float) -> np.ndarray:
+    """Octant axis 0, a1, ... R with geometric spacing after the first cell."""
+    return np.concatenate([[0.0], np.geomspace(a1, R, n - 1)])
+
+
+def dust_model(name: str, n: int, R: float, R_in: float, ndust: float, species: list,
+               abundances, radius, weights, nbins: int, nu_min: float, nu_max: float,
+               T_star: float, L36: float, n_packets: int) -> DustModel:
+    from . import dust as D
+    ax = shell_axes(n, R, R / 2.0 ** (n - 1) * 2.0)
+    act, ncells, nd = T.active_map(ax, ax, ax, np.full((n, n, n), ndust), R_in=R_in, R_out=R)
+    grid = CartesianGrid(ax, ax.copy(), ax.copy(), act, ncells, True, 0.0)
+    nu = T.log_mesh(nbins, nu_min, nu_max)
+    tabs = source_tables(nu, T_star, L36, n_packets, True, n, n, n, lgCompton=False,
+                         ion_edge1=H_EDGE)
+    dt = D.build_dust(nu, tabs.widFlx, species, abundances, radius, weights)
+    return DustModel(name, tabs, grid, dt, nd, n_packets)
+
+
+def dust_shell(n: int = 16, nbins: int = 300, n_packets: int = 1_000_000,
+               tau: float = 1.0) -> DustModel:
+    """Spherical silicate shell around a cool star, one octant (the layout of
+    the reference's dust benchmarks p0tau1/10/100): 16^3 with geometric
+    axes out to 2.18e17 cm, 0.16 um grains of sil-dlee with n_d = 1.117e-8
+    tau cm^-3 beyond 1.5e14 cm, 2500 K blackbody of 3.826e37 erg/s, 300
+    log points over 1e-5 - 10 Ryd."""
+    return dust_model("dustshell%d" % n, n, 2.18e17, 1.5e14, 1.117e-8 * tau, ["sil-dlee"], [1.0],
+                      [0.16], [1.0], nbins, 1.e-5, 10.0, 2500.0, 38.26, n_packets)

@@ -320,6 +320,24 @@ static int path_segment(const oracle_transport_in* in, oracle_transport_out* out
                                  (rz / 1.e10) * (rz / 1.e10)) * 1.e10;
                 if (rr >= in->R_out) { escape_tally(in, out, pk, cnt); return 0; }
             }
+            if (in->lgDust) {
+                /* grains (:2385-2500): scattered isotropically with probability
+                 * scaOpac/opacity, else absorbed; no Compton scattering */
+                probSca = in->scaOpac[PLANE(in, pk->nuP, cell)] / kap;
+                r = 1. - orng_u(g);
+                if (r <= probSca) {
+                    cnt->c[ORC_SCATTERED] += 1;
+                    random_unit_vector(g, &vx, &vy, &vz);
+                    pk->origin = cell;
+                    absTau = 0.;
+                    passProb = -log(1. - orng_u(g));
+                    continue;
+                }
+                cnt->c[ORC_ABSORBED] += 1;
+                pk->x = rx; pk->y = ry; pk->z = rz;
+                pk->xP = xP; pk->yP = yP; pk->zP = zP;
+                return 1; /* "dustEmi" without gas, "diffuse" with it */
+            }
             probSca = in->lgCompton ? in->KNsigmaT[pk->nuP - 1] * in->Ne[cell] / kap : 0.;
             r = 1. - orng_u(g);
             if (r <= probSca) {
@@ -392,10 +410,12 @@ static int energy_packet_run(const oracle_transport_in* in, oracle_transport_out
         int cell = ACT(in, pk->xP, pk->yP, pk->zP);
         double r;
         if (cell <= 0) { cnt->c[ORC_ERRORS] += 1; return 0; }
-        r = 1. - orng_u(g);
-        if (r <= in->totalLines[cell]) { /* line photon (:1605-1634) */
-            cnt->c[ORC_LINES] += 1;
-            return 0;
+        if (!in->dustOnly) { /* dustEmi packets (:1673-1741) are never lines */
+            r = 1. - orng_u(g);
+            if (r <= in->totalLines[cell]) { /* line photon (:1605-1634) */
+                cnt->c[ORC_LINES] += 1;
+                return 0;
+            }
         }
         pk->nuP = getnu2_draw(in->recPDF + (size_t)cell * in->nbins, in->nbins, g, &bad);
     }
@@ -408,7 +428,7 @@ static int energy_packet_run(const oracle_transport_in* in, oracle_transport_out
     }
     pk->origin = ACT(in, pk->xP, pk->yP, pk->zP);
     pk->nu = in->nuArray[pk->nuP - 1];
-    if (pk->nu < in->ionEdge1) { /* :634-719 */
+    if (!in->lgDust && pk->nu < in->ionEdge1) { /* :634-719 */
         escape_tally(in, out, pk, cnt);
         return 0;
     }

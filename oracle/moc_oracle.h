@@ -40,6 +40,9 @@ typedef struct oracle_transport_in {
     const double* Ne;         /* [C+1] */
     const double* recPDF;     /* [C+1][B] bin fastest */
     const double* totalLines; /* [C+1] */
+    const double* scaOpac;    /* [B][C+1] dust scattering kappa, NULL without grains */
+    int lgDust;               /* grains present */
+    int dustOnly;             /* no gas: re-emission from recPDF (= dustPDF), no lines */
     double star[3];
     int32_t starIdx[3];       /* 1-based */
     double deltaE;

@@ -1,0 +1,38 @@
+/* ORACLE -- test infrastructure only: grain routines (see moc_oracle_dust.c). */
+#ifndef MOC_ORACLE_DUST_H
+#define MOC_ORACLE_DUST_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_dust_in {
+    int nCells, B, nS, nA, nT, dustOnly;
+    const double* abun;     /* [nS] grainAbun */
+    const double* weight;   /* [nA] grainWeight */
+    const double* tsub;     /* [nS] TdustSublime */
+    const double* Cabs;     /* [nS][nA][B] */
+    const double* Csca;     /* [nS][nA][B] */
+    const double* emInt;    /* [nS][nA][nT] dustEmIntegral */
+    const double* nu;       /* [B] */
+    const double* widFlx;   /* [B] */
+    const double* Ndust;    /* [C+1] */
+    double XHILimit;
+} oracle_dust_in;
+
+double oracle_planck_h(double nu, double T);
+void oracle_dust_em_integral(const double* Cabs, const double* nu, const double* widFlx, int B,
+                             const double* temps, int nT, double* out);
+/* Tdust: [C+1][nA+1][nS+1]; planes [B][C+1]; gasOpac may be NULL */
+void oracle_dust_opacity(const oracle_dust_in* in, const double* Tdust, const double* gasOpac,
+                         double* opacity, double* absOpac, double* scaOpac);
+/* pdf: [C+1][B] */
+void oracle_dust_pdf(const oracle_dust_in* in, const double* Tdust, double* pdf);
+/* J: [B][C+1] normalised; Tdust updated in place; counters = noHit, converged */
+int oracle_dust_temperature(const oracle_dust_in* in, const double* J, double* Tdust,
+                            int32_t* lgConverged, int32_t* lgBlack, double counters[2]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -25,6 +25,7 @@ class TransportIn(C.Structure):
         ("nbins", C.c_int), ("nAngles", C.c_int), ("lgCompton", C.c_int), ("ionEdge1", C.c_double),
         ("nuArray", _dp), ("KNsigmaT", _dp), ("KNcdf", _dp), ("Pcomp", _dp), ("sourceCDF", _dp),
         ("opacity", _dp), ("Ne", _dp), ("recPDF", _dp), ("totalLines", _dp),
+        ("scaOpac", _dp), ("lgDust", C.c_int), ("dustOnly", C.c_int),
         ("star", C.c_double * 3), ("starIdx", C.c_int32 * 3), ("deltaE", C.c_double),
         ("n", C.c_int64), ("seed", C.c_uint64), ("ctr_base", C.c_uint32), ("nthreads", C.c_int),
     ]
@@ -82,7 +83,8 @@ def locate(xa, x) -> int:
 
 
 def transport(model_tables, grid, fields, n, seed, ctr_base=0, nthreads=1, istar=0):
-    """Run the oracle transport.  Returns (J[C+1,B], escaped[C+1,B], counters, Qphot)."""
+    """Run the oracle transport.  Returns (J[C+1,B], escaped[C+1,B], counters, Qphot).
+    With grains, fields holds scaOpac (C+1, B) and dustOnly (bool)."""
     t = model_tables
     B = t.nbins
     C1 = grid.nCells + 1
@@ -105,6 +107,8 @@ def transport(model_tables, grid, fields, n, seed, ctr_base=0, nthreads=1, istar
         KNcdf=d("kncdf", t.KNcdf), Pcomp=d("pc", t.Pcomp), sourceCDF=d("src", src),
         opacity=d("op", np.asarray(fields["opacity"]).T), Ne=d("ne", fields["Ne"]),
         recPDF=d("rec", fields["recPDF"]), totalLines=d("tl", fields["totalLines"]),
+        scaOpac=d("sca", np.asarray(fields["scaOpac"]).T) if "scaOpac" in fields else None,
+        lgDust=int("scaOpac" in fields), dustOnly=int(bool(fields.get("dustOnly", False))),
         deltaE=deltaE, n=int(n), seed=int(seed), ctr_base=int(ctr_base), nthreads=int(nthreads))
     pos = np.asarray(t.starPosition).reshape(-1, 3)[istar]
     idx = np.asarray(t.starIndex).reshape(-1, 3)[istar]
@@ -403,3 +407,88 @@ def nlevel_lines(bt, atom, Te, Ne, ionDenUp, rec, scale=1.0, unit=1.9865e9):
     t, k = balance_tables(bt, [1], np.ones(1), 2, True)
     return L.oracle_nlevel_lines(C.byref(t), int(atom), float(Te), float(Ne), float(ionDenUp),
                                  int(rec), float(scale), float(unit))
+
+
+# ----------------------------------------------------------------------- dust
+class DustIn(C.Structure):
+    _fields_ = [
+        ("nCells", C.c_int), ("B", C.c_int), ("nS", C.c_int), ("nA", C.c_int), ("nT", C.c_int),
+        ("dustOnly", C.c_int), ("abun", _dp), ("weight", _dp), ("tsub", _dp), ("Cabs", _dp),
+        ("Csca", _dp), ("emInt", _dp), ("nu", _dp), ("widFlx", _dp), ("Ndust", _dp),
+        ("XHILimit", C.c_double),
+    ]
+
+
+def _dust_in(dt, nu, widflx, Ndust, dustOnly, XHILimit=0.05):
+    keep = {}
+
+    def d(name, a):
+        keep[name] = np.ascontiguousarray(a, dtype=np.float64)
+        return keep[name].ctypes.data_as(_dp)
+
+    inp = DustIn(nCells=int(np.asarray(Ndust).size - 1), B=int(np.asarray(nu).size),
+                 nS=dt.nSpecies, nA=dt.nSizes, nT=int(dt.emIntegral.shape[2]),
+                 dustOnly=int(bool(dustOnly)), abun=d("a", dt.grainAbun), weight=d("w", dt.grainWeight),
+                 tsub=d("t", dt.Tsub), Cabs=d("ca", dt.Cabs), Csca=d("cs", dt.Csca),
+                 emInt=d("e", dt.emIntegral), nu=d("nu", nu), widFlx=d("wf", widflx),
+                 Ndust=d("nd", Ndust), XHILimit=float(XHILimit))
+    return inp, keep
+
+
+def planck_h(nu, T):
+    f = lib().oracle_planck_h
+    f.restype = C.c_double
+    f.argtypes = [C.c_double, C.c_double]
+    return np.array([f(float(v), float(T)) for v in np.atleast_1d(nu)])
+
+
+def dust_em_integral(Cabs_row, nu, widflx, temps):
+    c = np.ascontiguousarray(Cabs_row, dtype=np.float64)
+    n_ = np.ascontiguousarray(nu, dtype=np.float64)
+    w = np.ascontiguousarray(widflx, dtype=np.float64)
+    t = np.ascontiguousarray(temps, dtype=np.float64)
+    out = np.zeros(t.size)
+    lib().oracle_dust_em_integral(c.ctypes.data_as(_dp), n_.ctypes.data_as(_dp),
+                                  w.ctypes.data_as(_dp), C.c_int(n_.size), t.ctypes.data_as(_dp),
+                                  C.c_int(t.size), out.ctypes.data_as(_dp))
+    return out
+
+
+def dust_opacity(dt, nu, widflx, Ndust, Tdust, gasOpac=None):
+    """(opacity, absOpac, scaOpac) as (C+1, B); Tdust (C+1, nA+1, nS+1)."""
+    inp, keep = _dust_in(dt, nu, widflx, Ndust, gasOpac is None)
+    C1, B = np.asarray(Ndust).size, np.asarray(nu).size
+    td = np.ascontiguousarray(Tdust, dtype=np.float64)
+    gas = None if gasOpac is None else np.ascontiguousarray(np.asarray(gasOpac).T, dtype=np.float64)
+    op, ab, sc = np.zeros(B * C1), np.zeros(B * C1), np.zeros(B * C1)
+    if gas is not None:
+        op[:] = gas.reshape(-1)
+    lib().oracle_dust_opacity(C.byref(inp), td.ctypes.data_as(_dp),
+                              None if gas is None else gas.ctypes.data_as(_dp),
+                              op.ctypes.data_as(_dp), ab.ctypes.data_as(_dp), sc.ctypes.data_as(_dp))
+    f = lambda a: a.reshape(B, C1).T.copy()
+    return f(op), f(ab), f(sc)
+
+
+def dust_pdf(dt, nu, widflx, Ndust, Tdust):
+    """setDustPDF rows (C+1, B)."""
+    inp, keep = _dust_in(dt, nu, widflx, Ndust, True)
+    C1, B = np.asarray(Ndust).size, np.asarray(nu).size
+    td = np.ascontiguousarray(Tdust, dtype=np.float64)
+    pdf = np.zeros(C1 * B)
+    lib().oracle_dust_pdf(C.byref(inp), td.ctypes.data_as(_dp), pdf.ctypes.data_as(_dp))
+    return pdf.reshape(C1, B)
+
+
+def dust_temperature(dt, nu, widflx, Ndust, Tdust, J, dustOnly=True, XHILimit=0.05):
+    """getDustT: returns (Tdust, lgConverged, lgBlack, noHit, converged); J (C+1, B)."""
+    inp, keep = _dust_in(dt, nu, widflx, Ndust, dustOnly, XHILimit)
+    C1 = np.asarray(Ndust).size
+    td = np.ascontiguousarray(Tdust, dtype=np.float64).copy()
+    j = np.ascontiguousarray(np.asarray(J).T, dtype=np.float64)
+    conv = np.zeros(C1, dtype=np.int32)
+    black = np.zeros(C1, dtype=np.int32)
+    cnt = (C.c_double * 2)()
+    lib().oracle_dust_temperature(C.byref(inp), j.ctypes.data_as(_dp), td.ctypes.data_as(_dp),
+                                  conv.ctypes.data_as(_ip), black.ctypes.data_as(_ip), cnt)
+    return td, conv, black, cnt[0], cnt[1]

@@ -1,0 +1,263 @@
+"""Grains: the host tables, the oracle's dust routines pinned by closed forms
+and conservation laws, and on the GPU the dust opacity, emission CDF, grain
+temperatures, dust transport and dust-only iterations against the oracle.
+
+Pins (CPU):
+  Planck function      Wien and Rayleigh-Jeans limits of getFlux, and the
+                       Stefan-Boltzmann integral over a fine mesh
+  emission integrals   a grey grain radiates 4 Cabs sigma T^4 (dust_mod.f90:
+                       146-184 in closed form); host table = oracle sum
+  getDustT             a grain in a blackbody field of T0 (J = 4 pi B dnu,
+                       Kirchhoff) comes out at T0
+  transport            without gas every packet ends escaping: the escaped
+                       energy is n deltaE exactly, with and without absorption
+  sublimation          grains at or above TdustSublime drop out of kappa and
+                       of the emission CDF
+  Mie                  the Bohren & Huffman test sphere and the Rayleigh limit
+"""
+import numpy as np
+import pytest
+
+from mocassin_xray_amd import dust as D
+from mocassin_xray_amd import nebula
+from mocassin_xray_amd import tables as T
+
+SIGMA_SB = 5.67040e-5
+H = 6.6262e-27
+FR1RYD = 3.28984e15
+
+
+@pytest.fixture(scope="module")
+def shell():
+    return nebula.dust_shell(n=8, nbins=120, n_packets=4000)
+
+
+def tdust_of(m, T=None, rng=None):
+    C1 = m.grid.nCells + 1
+    nS, nA = m.dust.nSpecies, m.dust.nSizes
+    Td = np.full((C1, nA + 1, nS + 1), 50.0)
+    if rng is not None:
+        Td[1:, 1:, 1:] = rng.uniform(20.0, 900.0, size=(C1 - 1, nA, nS))
+    elif T is not None:
+        Td[:] = T
+    return Td
+
+
+# ---------------------------------------------------------------- CPU pins
+def test_mie_reference_sphere_and_rayleigh_limit():
+    x = 2 * np.pi * 0.525 / 0.6328
+    qe, qs = D.mie_q(np.array([x]), np.array([1.55 + 0j]))
+    assert abs(qe[0] - 3.10543) < 2e-5 and abs(qs[0] - 3.10543) < 2e-5
+    m = np.array([1.7 + 0.03j])
+    x = np.array([1e-3])
+    qe, qs = D.mie_q(x, m)
+    a = (m ** 2 - 1) / (m ** 2 + 2)
+    np.testing.assert_allclose(qe - qs, 4 * x * a.imag, rtol=1e-5)
+    np.testing.assert_allclose(qs, 8 / 3 * x ** 4 * abs(a) ** 2, rtol=1e-5)
+
+
+def test_planck_limits_and_stefan_boltzmann(oracle):
+    T0 = 300.0
+    nu = np.array([1e-7, 0.5])                 # x = 5e-5 and 263
+    b = oracle.planck_h(nu, T0)
+    rj = 3.32154e-6 * nu[0] ** 2 * T0 / H
+    assert abs(b[0] / rj - 1) < 2e-3           # the reference's RJ constant is 0.1 % off
+    wien = 0.5250229 / H * nu[1] ** 3 * np.exp(-157893.94 * nu[1] / T0)
+    assert b[1] == pytest.approx(wien, rel=1e-13)
+    mesh = np.geomspace(1e-7, 1.0, 20000)
+    w = T.widths_of(mesh)
+    integral = np.sum(oracle.planck_h(mesh, T0) * H * FR1RYD * w)
+    assert integral == pytest.approx(SIGMA_SB * T0 ** 4 / np.pi, rel=2e-3)
+
+
+def test_grey_grain_emission_integral(oracle):
+    mesh = np.geomspace(1e-7, 1.0, 6000)
+    w = T.widths_of(mesh)
+    sigma = 3.0e-10
+    C = np.full((1, 1, mesh.size), sigma)
+    emi = D.emission_integrals(C, mesh, w, n_temps=400)
+    temps = np.array([20.0, 100.0, 400.0])
+    want = 4 * sigma * SIGMA_SB * temps ** 4 / np.pi
+    np.testing.assert_allclose(emi[0, 0, temps.astype(int) - 1], want, rtol=3e-3)
+    got = oracle.dust_em_integral(C[0, 0], mesh, w, temps)
+    np.testing.assert_allclose(emi[0, 0, temps.astype(int) - 1], got, rtol=1e-12)
+
+
+def test_grain_temperature_of_a_blackbody_field(oracle, shell):
+    m = shell
+    nu, w = m.tables.nuArray, m.tables.widFlx
+    C1 = m.grid.nCells + 1
+    T0 = np.array([0.0] + [37.25 + 11.0 * (c % 70) for c in range(1, C1)])
+    J = np.zeros((C1, nu.size))
+    for c in range(1, C1):
+        J[c] = 4 * np.pi * oracle.planck_h(nu, T0[c]) * H * FR1RYD * w
+    Td, conv, black, nohit, nconv = oracle.dust_temperature(m.dust, nu, w, m.Ndust, tdust_of(m), J)
+    assert nohit == 0 and black[1:].sum() == 0
+    np.testing.assert_allclose(Td[1:, 1, 1], T0[1:], atol=0.02)
+    np.testing.assert_allclose(Td[1:, 0, 1], Td[1:, 1, 1] * m.dust.grainWeight[0], rtol=1e-15)
+    assert conv[1:].sum() == nconv
+    J[3] = 0.0                                   # a cell no packet reached keeps its grains
+    Td2, conv2, black2, nohit2, _ = oracle.dust_temperature(m.dust, nu, w, m.Ndust,
+                                                            tdust_of(m, 77.0), J)
+    assert nohit2 == 1 and black2[3] == 1 and np.all(Td2[3] == 77.0)
+
+
+def test_sublimated_grains_leave_kappa_and_emission(oracle):
+    rng = np.random.default_rng(20261203 + 3)
+    nu = np.geomspace(1e-4, 5.0, 40)
+    w = T.widths_of(nu)
+    dt = D.build_dust(nu, w, ["sil-dlee", "grph1-dl"], [0.6, 0.4], [0.01, 0.05, 0.2],
+                      [1.0, 0.5, 0.1])
+    C1 = 6
+    nd = np.r_[0.0, rng.uniform(1e-9, 1e-7, C1 - 1)]
+    Td = np.full((C1, 4, 3), 100.0)
+    op, ab, sc = oracle.dust_opacity(dt, nu, w, nd, Td)
+    want_s = np.zeros((C1, nu.size))
+    want_a = np.zeros((C1, nu.size))
+    for s in range(2):
+        for a in range(3):
+            f = dt.grainAbun[s] * dt.grainWeight[a]
+            want_s[1:] += f * nd[1:, None] * dt.Csca[s, a][None]
+            want_a[1:] += f * nd[1:, None] * dt.Cabs[s, a][None]
+    np.testing.assert_allclose(sc, want_s, rtol=1e-13)
+    np.testing.assert_allclose(ab, want_a, rtol=1e-13)
+    np.testing.assert_allclose(op, want_s + want_a, rtol=1e-13)
+    Td[2, :, 1] = dt.Tsub[0]                     # species 1 sublimated in cell 2
+    op2, ab2, sc2 = oracle.dust_opacity(dt, nu, w, nd, Td)
+    part = sum(dt.grainAbun[1] * dt.grainWeight[a] * nd[2] * dt.Csca[1, a] for a in range(3))
+    np.testing.assert_allclose(sc2[2], part, rtol=1e-13)
+    np.testing.assert_allclose(sc2[3], sc[3], rtol=0)
+    pdf = oracle.dust_pdf(dt, nu, w, nd, Td)
+    assert np.all(pdf[1:, -1] == 1.0) and np.all(np.diff(pdf[1:], axis=1) >= 0)
+    inc = np.zeros(nu.size)
+    for a in range(3):
+        inc += dt.Cabs[1, a] * oracle.planck_h(nu, 100.0) * w * dt.grainWeight[a] * dt.grainAbun[1]
+    np.testing.assert_allclose(pdf[2, :-1], (np.cumsum(inc) / inc.sum())[:-1], rtol=1e-12)
+
+
+def test_dust_only_transport_conserves_energy(oracle, shell):
+    m = shell
+    C1, B = m.grid.nCells + 1, m.tables.nbins
+    for absorb in (False, True):
+        Td = tdust_of(m, 40.0)
+        dt = m.dust
+        if not absorb:       # pure scatterers: every packet escapes with its first frequency
+            dt = D.DustTables(dt.species, dt.grainAbun, dt.radius, dt.grainWeight, dt.Tsub, dt.rho,
+                              np.zeros_like(dt.Cabs), dt.Csca * 5.0, dt.emIntegral)
+        op, ab, sc = oracle.dust_opacity(dt, m.tables.nuArray, m.tables.widFlx, m.Ndust * 3.0, Td)
+        pdf = oracle.dust_pdf(m.dust, m.tables.nuArray, m.tables.widFlx, m.Ndust, Td)
+        fields = {"opacity": op, "Ne": np.zeros(C1), "recPDF": pdf, "totalLines": np.zeros(C1),
+                  "scaOpac": sc, "dustOnly": True}
+        n = 3000
+        J, E, cnt, _ = oracle.transport(m.tables, m.grid, fields, n, 11, nthreads=4)
+        dE = m.tables.Lstar[0] / m.tables.nPhotons[0]
+        assert cnt["errors"] == 0 and cnt["lines"] == 0 and cnt["lost"] == 0
+        assert cnt["escaped"] == n
+        assert E.sum() == pytest.approx(n * dE, rel=1e-12)
+        assert cnt["scattered"] > n
+        if absorb:
+            assert cnt["absorbed"] > 0 and cnt["emissions"] == n + cnt["absorbed"]
+        else:
+            assert cnt["absorbed"] == 0
+            # the escaped spectrum is the stellar one: no bin below the source's support
+            assert E[:, m.tables.sourceCDF[0] == 0].sum() == 0.0
+
+
+# ---------------------------------------------------------------- GPU parity
+@pytest.mark.gpu
+def test_gpu_dust_opacity_pdf_temperature_match_oracle(oracle):
+    from mocassin_xray_amd import MocGPU
+    m = nebula.dust_shell(n=10, nbins=150, n_packets=20000)
+    rng = np.random.default_rng(20261203 + 3)
+    Td = tdust_of(m, rng=rng)
+    Td[5, 1, 1] = 2000.0                          # above the 1400 K sublimation point
+    nu, w = m.tables.nuArray, m.tables.widFlx
+    print(f"case dust_shell10: cells={m.grid.nCells} bins={nu.size}", flush=True)
+    with MocGPU(m.tables, "fp64") as g:
+        m.load(g)
+        g.upload("Tdust", Td)
+        g.dustOpacity()
+        op, ab, sc = (g.download(k) for k in ("opacity", "absOpac", "scaOpac"))
+        g.setDustPDF()
+        pdf = g.download("dustPDF")
+        C1 = m.grid.nCells + 1
+        J = np.zeros((C1, nu.size))
+        for c in range(1, C1):
+            J[c] = 4 * np.pi * oracle.planck_h(nu, 30.0 + 7.0 * (c % 40)) * H * FR1RYD * w
+        J[7] = 0.0
+        g.upload("Jste", J)
+        st = g.getDustT()
+        Tg = g.download("Tdust")
+        conv = g.download_flags("lgConverged")
+        black = g.download_flags("lgBlack")
+    op_o, ab_o, sc_o = oracle.dust_opacity(m.dust, nu, w, m.Ndust, Td)
+    np.testing.assert_allclose(op[1:], op_o[1:], rtol=1e-13, atol=0)
+    np.testing.assert_allclose(ab[1:], ab_o[1:], rtol=1e-13, atol=0)
+    np.testing.assert_allclose(sc[1:], sc_o[1:], rtol=1e-13, atol=0)
+    pdf_o = oracle.dust_pdf(m.dust, nu, w, m.Ndust, Td)
+    np.testing.assert_allclose(pdf[1:], pdf_o[1:], rtol=1e-11, atol=1e-14)
+    To, conv_o, black_o, nohit_o, nconv_o = oracle.dust_temperature(m.dust, nu, w, m.Ndust, Td, J)
+    np.testing.assert_allclose(Tg[1:], To[1:], rtol=1e-10)
+    assert st["noHit"] == nohit_o == 1 and np.all(black == black_o)
+    assert np.all(conv[1:] == conv_o[1:]) and st["converged"] == nconv_o
+
+
+@pytest.mark.gpu
+def test_gpu_dust_transport_matches_oracle(oracle):
+    from mocassin_xray_amd import MocGPU
+    m = nebula.dust_shell(n=10, nbins=150, n_packets=20000, tau=10.0)
+    C1 = m.grid.nCells + 1
+    nu, w = m.tables.nuArray, m.tables.widFlx
+    Td = tdust_of(m, rng=np.random.default_rng(5))
+    n, seed = 20000, 20261203
+    print(f"case dust_shell10 tau10: n={n} cells={m.grid.nCells}", flush=True)
+    with MocGPU(m.tables, "fp64") as g:
+        m.load(g)
+        g.upload("Tdust", Td)
+        g.dustOpacity()
+        g.setDustPDF()
+        st = g.energyPacketDriver(1, n, seed=seed)
+        J, E = g.download("Jste"), g.download("escaped")
+    op, ab, sc = oracle.dust_opacity(m.dust, nu, w, m.Ndust, Td)
+    pdf = oracle.dust_pdf(m.dust, nu, w, m.Ndust, Td)
+    fields = {"opacity": op, "Ne": np.zeros(C1), "recPDF": pdf, "totalLines": np.zeros(C1),
+              "scaOpac": sc, "dustOnly": True}
+    Jo, Eo, co, _ = oracle.transport(m.tables, m.grid, fields, n, seed, nthreads=8)
+    assert co["errors"] == 0 and co["absorbed"] > n and co["scattered"] > n
+    for k in ("segments", "emissions", "absorbed", "scattered", "escaped", "lines", "lost"):
+        assert abs(st[k] - co[k]) <= 2e-3 * max(co[k], 1.0) + 1, (k, st[k], co[k])
+    l1 = np.abs(J[1:] - Jo[1:]).sum() / Jo[1:].sum()
+    assert l1 < 1e-3, l1
+    dE = m.tables.Lstar[0] / m.tables.nPhotons[0]
+    assert E.sum() == pytest.approx(st["escaped"] * dE, rel=1e-12)
+
+
+@pytest.mark.gpu
+def test_gpu_dust_only_iterations_match_oracle_chain(oracle):
+    from mocassin_xray_amd import MocGPU
+    m = nebula.dust_shell(n=8, nbins=120, n_packets=30000)
+    nu, w = m.tables.nuArray, m.tables.widFlx
+    C1 = m.grid.nCells + 1
+    print(f"case dust_shell8: two iterations, cells={m.grid.nCells}", flush=True)
+    with MocGPU(m.tables, "fp64") as g:
+        m.load(g)
+        Td0 = g.download("Tdust")
+        st1 = g.iterateMC(1, 99)
+        J1, Td1 = g.download("Jste"), g.download("Tdust")
+        st2 = g.iterateMC(2, 99)
+        Td2 = g.download("Tdust")
+    # iteration 1 on the oracle from the same start
+    op, ab, sc = oracle.dust_opacity(m.dust, nu, w, m.Ndust, Td0)
+    pdf = oracle.dust_pdf(m.dust, nu, w, m.Ndust, Td0)
+    fields = {"opacity": op, "Ne": np.zeros(C1), "recPDF": pdf, "totalLines": np.zeros(C1),
+              "scaOpac": sc, "dustOnly": True}
+    Jo = oracle.transport(m.tables, m.grid, fields, 30000, 99, ctr_base=257, nthreads=8)[0]
+    Jo = Jo * 1.e-9 / 8.0
+    assert np.abs(J1[1:] - Jo[1:]).sum() / Jo[1:].sum() < 2e-3
+    To = oracle.dust_temperature(m.dust, nu, w, m.Ndust, Td0, J1)[0]
+    np.testing.assert_allclose(Td1[1:], To[1:], rtol=1e-10)
+    # the grains warm from the 50 K start and the second iteration moves them less
+    d1 = np.abs(Td1[1:, 0, 0] - 50.0).mean()
+    d2 = np.abs(Td2[1:, 0, 0] - Td1[1:, 0, 0]).mean()
+    assert d2 < d1
+    assert st2["convPercent"] >= st1["convPercent"]
