@@ -213,6 +213,7 @@ struct CellCtx {
     double aEffHeI;    // He I 2^1S effective recombination (HeI2photSub :781)
     GauntT g1, g2;     // Z = 1 and Z = 2 Gaunt fits at Te
     TwoPhotT tp;       // two-photon temperature factors
+    double dustScale;  // hPlanck 4 pi 1e25 cRyd Ndust / Hden (setDiffusePDF :1749-1752)
     double corr;       // He I 2^3S line energy (setDiffusePDF :1611-1622)
     double ly[4];      // He II Lyman line energies (:1646-1673)
 };
@@ -221,7 +222,7 @@ struct CellCtx {
 // line energies of setDiffusePDF added in their bins
 __device__ double bin_energy(const MocEmissionArgs& a, const CellCtx& c, const double (*sLoc)[MOC_EM_MAXLIM],
                              const double (*sFit)[MOC_EM_MAXLIM / 2], const double (*sX)[MOC_MAXS + 1],
-                             int i) {
+                             const double* sTd, int i) {
     const double nu = a.nu[i - 1];
     const double Te = c.Te, Ne = c.Ne;
     double gam[3] = {0., 0., 0.};
@@ -303,6 +304,20 @@ __device__ double bin_energy(const MocEmissionArgs& a, const CellCtx& c, const d
     if (eHI < 1.e-15) eHI = 0.;
     if (eHeI < 1.e-15) eHeI = 0.;
     if (eHeII < 1.e-15) eHeII = 0.;
+    // grains below sublimation radiate Cabs B_nu(Tdust) (sizes outer, species inner)
+    double sDust = 0.;
+    if (a.nDS > 0) {
+        double acc = 0.;
+        const double wf = a.widFlx[i - 1];
+        for (int ai = 0; ai < a.nDA; ++ai)
+            for (int s = 0; s < a.nDS; ++s) {
+                const double T = sTd[s * a.nDA + ai];
+                if (!(T < a.dTsub[s])) continue;
+                acc = acc + a.dCabs[((size_t)s * a.nDA + ai) * a.B + (i - 1)] * moc_planck_h(nu, T) * wf *
+                                a.dWeight[ai] * a.dAbun[s];
+            }
+        sDust = acc * c.dustScale;
+    }
     const double w = a.widFlx[i - 1];
     const double sHI = kCRyd * w * eHI / 1.e15;
     double sHeI = kCRyd * w * eHeI / 1.e15;
@@ -312,6 +327,7 @@ __device__ double bin_energy(const MocEmissionArgs& a, const CellCtx& c, const d
         for (int k = 0; k < 4; ++k)
             if (i == a.lymP[k]) sHeII += c.ly[k];
     }
+    if (a.nDS > 0) return sHI + sHeI + sHeII + sDust;
     return sHI + sHeI + sHeII;
 }
 
@@ -338,6 +354,7 @@ k_emission(MocEmissionArgs a) {
     __shared__ double sFit[3][MOC_EM_MAXLIM / 2];
     __shared__ double sWave[kThreads / 64];
     __shared__ double sLines[kThreads / 64];
+    __shared__ double sTd[MOC_EM_MAXGRAIN];   // grain temperatures of the cell [species][size]
     const int cell = blockIdx.x + 1;
     if (moc_unchanged(a.TeOld, a.NeOld, a.Te, a.Ne, cell)) return;  // uniform per workgroup
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -390,6 +407,15 @@ k_emission(MocEmissionArgs a) {
     c.g1 = gaunt_setup(0., log10(Te));
     c.g2 = gaunt_setup(0.30103, log10(Te));
     c.tp = two_phot_setup(Te);
+    c.dustScale = 0.;
+    if (a.nDS > 0) {
+        for (int k = tid; k < a.nDS * a.nDA; k += kThreads) {
+            const int s = k / a.nDA, ai = k % a.nDA;
+            sTd[k] = a.Tdust[((size_t)cell * (a.nDA + 1) + ai + 1) * (a.nDS + 1) + s + 1];
+        }
+        c.dustScale = 6.6262e-27 * 4. * kPi * 1.e25 * kCRyd;
+        c.dustScale = c.dustScale * a.Ndust[cell] / a.Hden[cell];
+    }
     c.corr = 0.;
     for (int k = 0; k < 4; ++k) c.ly[k] = 0.;
     if (c.heOn) {
@@ -424,7 +450,7 @@ k_emission(MocEmissionArgs a) {
     double carry = 0.;
     for (int b0 = 0; b0 < B; b0 += kThreads) {
         const int b = b0 + tid;
-        const double v = b < B ? bin_energy(a, c, sLoc, sFit, sX, b + 1) : 0.;
+        const double v = b < B ? bin_energy(a, c, sLoc, sFit, sX, sTd, b + 1) : 0.;
         double sc = v;
         for (int off = 1; off < 64; off <<= 1) {
             const double t = __shfl_up(sc, off, 64);

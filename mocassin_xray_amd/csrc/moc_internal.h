@@ -160,6 +160,7 @@ struct MocBalanceArgs {
 
 // Emission kernel view (emission.hip)
 #define MOC_EM_MAXLIM 64   // points of a gamma fit table
+#define MOC_EM_MAXGRAIN 256 // (species, size) pairs of gas + dust emission
 // the skip test of iterateMC for one cell
 __device__ __forceinline__ bool moc_unchanged(const double* TeOld, const double* NeOld,
                                               const double* Te, const double* Ne, int cell) {
@@ -179,6 +180,15 @@ struct MocEmissionArgs {
     int nlim[3];             // H I, He I, He II
     const int* edge[3];      // [nlim] 1-based bin pointers
     const int* gpair;        // [3][B] fit pair (1-based) that covers each bin, 0 for none
+    // grains of gas + dust runs (nDS = 0 without): setDiffusePDF :1685-1752
+    int nDS, nDA;
+    const double* dAbun;     // [nDS]
+    const double* dWeight;   // [nDA]
+    const double* dTsub;     // [nDS]
+    const double* dCabs;     // [nDS][nDA][B]
+    const double* Tdust;     // [C+1][nDA+1][nDS+1]
+    const double* Ndust;     // [C+1]
+    const double* Hden;      // [C+1]
     const double* lg[3];     // [ntk][nlim] log10 gamma
     int HIP, HeIP, HeIIP;    // threshold bins (He: 0 when off)
     const double* xsHI;

@@ -1105,6 +1105,13 @@ int run_emission(moc_ctx* ctx, bool skipUnchanged, float* kernel_ms) {
     a.totalLines = g.d_totalLines;
     a.TeOld = skipUnchanged ? g.d_TeOld : nullptr;
     a.NeOld = skipUnchanged ? g.d_NeOld : nullptr;
+    a.nDS = a.nDA = 0;
+    if (ctx->dust.set && ctx->dust.lgGas) {  // dust emission joins the diffuse CDF
+        MocDust& d = ctx->dust;
+        a.nDS = d.nS; a.nDA = d.nA;
+        a.dAbun = d.d_abun; a.dWeight = d.d_weight; a.dTsub = d.d_tsub; a.dCabs = d.d_Cabs;
+        a.Tdust = g.d_Tdust; a.Ndust = g.d_Ndust; a.Hden = g.d_Hden;
+    }
     MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
     int rc = moc_launch_emission(ctx, a);
     if (rc) return rc;
@@ -1203,8 +1210,9 @@ int moc_set_dust(moc_ctx* ctx, const moc_dust_tables* t) {
                    "(run with noPhotoelectric)";
         return MOC_ERR_ARG;
     }
-    if ((int64_t)t->nSpecies * t->nSizes + 1 > 4096) {
-        ctx->err = "moc_set_dust: more than 4095 (species, size) pairs";
+    if ((int64_t)t->nSpecies * t->nSizes + 1 > 4096 ||
+        (t->lgGas && (int64_t)t->nSpecies * t->nSizes > MOC_EM_MAXGRAIN)) {
+        ctx->err = "moc_set_dust: too many (species, size) pairs (4095; 256 with gas)";
         return MOC_ERR_ARG;
     }
     free_dust(ctx->dust);

@@ -264,3 +264,63 @@ def dust_shell(n: int = 16, nbins: int = 300, n_packets: int = 1_000_000,
     log points over 1e-5 - 10 Ryd."""
     return dust_model("dustshell%d" % n, n, 2.18e17, 1.5e14, 1.117e-8 * tau, ["sil-dlee"], [1.0],
                       [0.16], [1.0], nbins, 1.e-5, 10.0, 2500.0, 38.26, n_packets)
+
+
+@dataclass
+class GasDustModel(GasModel):
+    """A gas model with grains (MdMg dust-to-gas mass ratio), run without the
+    grain photoelectric terms (the reference's noPhotoelectric keyword)."""
+    dust: object = None     # dust.DustTables
+    Ndust: np.ndarray = None
+
+    def load(self, g) -> None:
+        super().load(g)
+        g.set_dust(self.dust, lgGas=True)
+        g.upload("Ndust", self.Ndust)
+
+
+def gas_dust_model(name: str, n: int, R: float, hden, elems: list, abund: np.ndarray,
+                   nstages: int, nbins: int, nu_min: float, nu_max: float, T_star: float,
+                   L36: float, n_packets: int, species: list, grain_abun, radius, weights,
+                   mdmg: float, maxLev: int = 5) -> GasDustModel:
+    """hden is a number or an (n, n, n) array of n_H per cell."""
+    from . import atomic, balance, emission
+    from . import dust as D
+    ax = T.symmetric_axes(n, R)
+    hd = np.broadcast_to(np.asarray(hden, dtype=np.float64), (n, n, n))
+    act, ncells, hcell = T.active_map(ax, ax, ax, np.ascontiguousarray(hd))
+    grid = CartesianGrid(ax, ax.copy(), ax.copy(), act, ncells, True, 0.0)
+    nu = T.reference_gas_mesh(nbins, nu_min, nu_max, atomic.ion_edges(elems, nstages, nu_max))
+    tabs = source_tables(nu, T_star, L36, n_packets, True, n, n, n, True)
+    xs = atomic.build_xsec(nu, elems, nstages)
+    bt = balance.build_balance(xs, maxLev=maxLev)
+    et = emission.build_emission(xs, tabs.widFlx)
+    st = initial_state(elems, nstages, hcell)
+    dt = D.build_dust(nu, tabs.widFlx, species, grain_abun, radius, weights)
+    nd = D.ndust_from_mdmg(dt, mdmg, hcell, elems, abund)
+    nd[0] = 0.0
+    return GasDustModel(name, tabs, grid, list(elems), np.asarray(abund, dtype=np.float64),
+                        nstages, xs, bt, et, st, n_packets, dust=dt, Ndust=nd)
+
+
+def lognormal_density(n: int, mean: float = 300.0, sigma_ln: float = 1.0,
+                      seed: int = 20261203 + 3) -> np.ndarray:
+    """n_H per cell drawn log-normal with the given mean (mu = ln mean - sigma^2/2)."""
+    rng = np.random.default_rng(seed)
+    return rng.lognormal(np.log(mean) - 0.5 * sigma_ln ** 2, sigma_ln, size=(n, n, n))
+
+
+def gas_dust_nebula(n: int = 64, nbins: int = 700, n_packets: int = 10_000_000) -> GasDustModel:
+    """Dusty nebula, one octant: n^3 with log-normal n_H (mean 300, sigma_ln 1),
+    edge 1e18 cm, the seven elements of the gas nebula, dust-to-gas mass ratio
+    0.01 of sil-dlee and grph1-dl (0.5 each) in 10 MRN sizes 0.005-0.25 um,
+    50 kK blackbody of 10^37.5 erg/s, a gas mesh over 1e-5 - 25 Ryd."""
+    a, w = _mrn()
+    return gas_dust_model("dustneb%d" % n, n, 1.e18, lognormal_density(n), NEBULA_ELEMS,
+                          10.0 ** np.array(NEBULA_LOG_ABUND), 7, nbins, 1.001e-5, 25.0, 5.e4,
+                          10 ** 1.5, n_packets, ["sil-dlee", "grph1-dl"], [0.5, 0.5], a, w, 0.01)
+
+
+def _mrn():
+    from . import dust as D
+    return D.mrn_sizes(10, 0.005, 0.25, 3.5)

@@ -39,6 +39,7 @@
 #include "moc_oracle.h"
 #include "moc_oracle_balance.h"
 #include "moc_oracle_emission.h"
+#include "moc_oracle_dust.h"
 
 #define PI_REF 3.141592654   /* constants_mod.f90 */
 #define HCRYD 2.1799153e-11  /* continuum_mod.f90:15 */
@@ -437,7 +438,8 @@ static void emission_one(const oracle_emission_tables* t, const prep_t* p, oracl
     const double log10Ne = log10(NeGrid);
     const double Ne = NeGrid < 1.e-5 ? 1.e-5 : NeGrid;
     const double *eHI = w, *eHeI = w + B, *eHeII = w + 2 * B;
-    double *sHI = w + 12 * B, *sHeI = w + 13 * B, *sHeII = w + 14 * B;
+    double *sHI = w + 12 * B, *sHeI = w + 13 * B, *sHeII = w + 14 * B, *sDust = w + 15 * B;
+    double normDust = 0.;
     double normHI = 0., normHeI = 0., normHeII = 0., normRec, normFor, normalize, lines;
     double* row = s->recPDF + (size_t)cell * B;
     cellcomp_t c;
@@ -471,10 +473,36 @@ static void emission_one(const oracle_emission_tables* t, const prep_t* p, oracl
             normHeII += ly[k];
         }
     }
+    if (s->dustS > 0) { /* dust emission (:1685-1752): sizes outer, species inner */
+        const double cst = 6.6262e-27 * 4. * 3.141592654 * 1.e25 * CRYD;
+        const int nS = s->dustS, nA = s->dustA;
+        int ai, sp;
+        for (i = 0; i < B; i++) sDust[i] = 0.;
+        for (ai = 1; ai <= nA; ai++)
+            for (sp = 1; sp <= nS; sp++) {
+                const double T = s->Tdust[((size_t)cell * (nA + 1) + ai) * (nS + 1) + sp];
+                const double* c = s->dustCabs + ((size_t)(sp - 1) * nA + (ai - 1)) * B;
+                if (!(T < s->dustTsub[sp - 1])) continue;
+                for (i = 0; i < B; i++) {
+                    const double v = c[i] * oracle_planck_h(t->bal->nuArray[i], T) * t->widFlx[i] *
+                                     s->dustWeight[ai - 1] * s->dustAbun[sp - 1];
+                    sDust[i] = sDust[i] + v;
+                    normDust = normDust + v;
+                }
+            }
+        for (i = 0; i < B; i++) sDust[i] = sDust[i] * cst * s->Ndust[cell] / s->Hden[cell];
+        normDust = normDust * cst * s->Ndust[cell] / s->Hden[cell];
+    }
     normalize = normHI + normHeI + normHeII;
+    if (s->dustS > 0) normalize = normalize + normDust;
     /* :1804-1835 cumulative sum, normalised, last entry 1 */
-    row[0] = sHI[0] + sHeI[0] + sHeII[0];
-    for (i = 1; i < B; i++) row[i] = row[i - 1] + (sHI[i] + sHeI[i] + sHeII[i]);
+    if (s->dustS > 0) {
+        row[0] = sHI[0] + sHeI[0] + sHeII[0] + sDust[0];
+        for (i = 1; i < B; i++) row[i] = row[i - 1] + sHI[i] + sHeI[i] + sHeII[i] + sDust[i];
+    } else {
+        row[0] = sHI[0] + sHeI[0] + sHeII[0];
+        for (i = 1; i < B; i++) row[i] = row[i - 1] + (sHI[i] + sHeI[i] + sHeII[i]);
+    }
     {
         const double last = row[B - 1];
         for (i = 0; i < B; i++) row[i] = row[i] / last;
@@ -495,7 +523,7 @@ int oracle_emission(const oracle_emission_tables* t, oracle_emission_state* s, i
 #pragma omp parallel num_threads(nthreads > 1 ? nthreads : 1)
 #endif
     {
-        double* w = (double*)malloc(sizeof(double) * 15 * (size_t)t->bal->nbins);
+        double* w = (double*)malloc(sizeof(double) * 16 * (size_t)t->bal->nbins);
         int cell;
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 4)

@@ -55,6 +55,15 @@ typedef struct oracle_emission_state {
     double* normCont;       /* [C+1] out, may be NULL: HI+HeI+HeII continuum */
     double* normRec;        /* [C+1] out, may be NULL: recombination lines */
     double* normFor;        /* [C+1] out, may be NULL: collisional lines */
+    /* grains of gas + dust runs (dustS = 0 without): setDiffusePDF :1685-1752 */
+    int dustS, dustA;
+    const double* dustAbun;   /* [dustS] */
+    const double* dustWeight; /* [dustA] */
+    const double* dustTsub;   /* [dustS] */
+    const double* dustCabs;   /* [dustS][dustA][B] */
+    const double* Tdust;      /* [C+1][dustA+1][dustS+1] */
+    const double* Ndust;      /* [C+1] */
+    const double* Hden;       /* [C+1] */
 } oracle_emission_state;
 
 /* returns 0, or -1 when two gamma edges share a bin (fb_ff stops there) */

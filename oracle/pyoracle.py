@@ -316,6 +316,8 @@ class EmissionStateC(C.Structure):
     _fields_ = [
         ("nCells", C.c_int), ("Te", _dp), ("Ne", _dp), ("ionDen", _dp), ("recPDF", _dp),
         ("totalLines", _dp), ("normCont", _dp), ("normRec", _dp), ("normFor", _dp),
+        ("dustS", C.c_int), ("dustA", C.c_int), ("dustAbun", _dp), ("dustWeight", _dp),
+        ("dustTsub", _dp), ("dustCabs", _dp), ("Tdust", _dp), ("Ndust", _dp), ("Hden", _dp),
     ]
 
 
@@ -356,9 +358,11 @@ def emission_tables(et, bt, elem_z, abund, nstages):
     return e, k
 
 
-def emission(et, bt, elem_z, abund, nstages, Te, Ne, ionDen, nthreads=1):
+def emission(et, bt, elem_z, abund, nstages, Te, Ne, ionDen, nthreads=1, dust=None):
     """Oracle emissionDriver over all cells.  ionDen is (C+1, nElemUsed,
-    nstages).  Returns recPDF (C+1, B), totalLines and the per-cell sums."""
+    nstages).  Returns recPDF (C+1, B), totalLines and the per-cell sums.
+    dust = (DustTables, Tdust (C+1, nA+1, nS+1), Ndust, Hden) adds the grains
+    of a gas + dust run."""
     L = lib()
     L.oracle_emission.argtypes = [C.POINTER(EmissionTablesC), C.POINTER(EmissionStateC), C.c_int]
     L.oracle_emission.restype = C.c_int
@@ -373,6 +377,14 @@ def emission(et, bt, elem_z, abund, nstages, Te, Ne, ionDen, nthreads=1):
     dp = lambda n: s[n].ctypes.data_as(_dp)
     st = EmissionStateC(nCells=C1 - 1, Te=dp("Te"), Ne=dp("Ne"), ionDen=dp("X"), recPDF=dp("pdf"),
                         totalLines=dp("tl"), normCont=dp("nc"), normRec=dp("nr"), normFor=dp("nf"))
+    if dust is not None:
+        dt, Td, Nd, Hd = dust
+        for name, a in (("da", dt.grainAbun), ("dw", dt.grainWeight), ("dt", dt.Tsub),
+                        ("dc", dt.Cabs), ("td", Td), ("nd", Nd), ("hd", Hd)):
+            s[name] = np.ascontiguousarray(a, dtype=np.float64)
+        st.dustS, st.dustA = dt.nSpecies, dt.nSizes
+        st.dustAbun, st.dustWeight, st.dustTsub = dp("da"), dp("dw"), dp("dt")
+        st.dustCabs, st.Tdust, st.Ndust, st.Hden = dp("dc"), dp("td"), dp("nd"), dp("hd")
     if L.oracle_emission(C.byref(t), C.byref(st), int(nthreads)) != 0:
         raise ValueError("frequency grid too coarse for the free-bound edges")
     return {"recPDF": s["pdf"], "totalLines": s["tl"], "normCont": s["nc"], "normRec": s["nr"],

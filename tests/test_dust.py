@@ -261,3 +261,54 @@ def test_gpu_dust_only_iterations_match_oracle_chain(oracle):
     d2 = np.abs(Td2[1:, 0, 0] - Td1[1:, 0, 0]).mean()
     assert d2 < d1
     assert st2["convPercent"] >= st1["convPercent"]
+
+
+def small_dusty(n_packets):
+    return nebula.gas_dust_model("dusty6", 6, 1.e18, 300., nebula.NEBULA_ELEMS,
+                                 10.0 ** np.array(nebula.NEBULA_LOG_ABUND), 7, 300, 1.001e-5, 25.0,
+                                 5.e4, 10 ** 1.5, n_packets, ["sil-dlee", "grph1-dl"], [0.5, 0.5],
+                                 [0.01, 0.05, 0.2], [1.0, 0.3, 0.05], 0.01)
+
+
+@pytest.mark.gpu
+def test_gpu_gas_dust_iteration_matches_oracle_chain(oracle):
+    """Gas + grains (noPhotoelectric): kappa = gas + dust, the dust emission in
+    the diffuse CDF, dust scattering/absorption of packets, then the gas balance
+    and the grain temperatures on the device's J."""
+    from mocassin_xray_amd import MocGPU
+    m = small_dusty(20000)
+    s0 = m.state
+    nu, w = m.tables.nuArray, m.tables.widFlx
+    seed = 31
+    print(f"case gas+dust dusty6: cells={m.grid.nCells} bins={nu.size}", flush=True)
+    with MocGPU(m.tables, "fp64") as g:
+        m.load(g)
+        Td0 = g.download("Tdust")
+        st = g.iterateMC(1, seed)
+        got = {k: g.download(k) for k in ("opacity", "absOpac", "scaOpac", "recPDF", "totalLines",
+                                          "Jste", "Te", "Ne", "ionDen", "Tdust")}
+    E, Z, NS = m.elems, m.abund, m.nstages
+    gas = oracle.opacity(m.xsec, m.tables.KNsigmaT, True, E, Z, NS, s0["ionDen"], s0["Ne"],
+                         s0["Te"], s0["Hden"])
+    op_o, ab_o, sc_o = oracle.dust_opacity(m.dust, nu, w, m.Ndust, Td0, gasOpac=gas)
+    np.testing.assert_allclose(got["opacity"][1:], op_o[1:], rtol=1e-11, atol=1e-300)
+    np.testing.assert_allclose(got["scaOpac"][1:], sc_o[1:], rtol=1e-13, atol=0)
+    assert np.all(got["absOpac"][1:] > 0)
+    em = oracle.emission(m.et, m.bt, E, Z, NS, s0["Te"], s0["Ne"], s0["ionDen"], nthreads=8,
+                         dust=(m.dust, Td0, m.Ndust, s0["Hden"]))
+    np.testing.assert_allclose(got["recPDF"][1:], em["recPDF"][1:], rtol=1e-11, atol=1e-14)
+    np.testing.assert_allclose(got["totalLines"][1:], em["totalLines"][1:], rtol=1e-11)
+    fields = {"opacity": got["opacity"], "Ne": s0["Ne"], "recPDF": got["recPDF"],
+              "totalLines": got["totalLines"], "scaOpac": got["scaOpac"], "dustOnly": False}
+    Jo, Eo, co, qo = oracle.transport(m.tables, m.grid, fields, 20000, seed, ctr_base=257,
+                                      nthreads=8)
+    assert co["errors"] == 0 and co["scattered"] > 0 and co["absorbed"] > 0
+    Jo = Jo * 1.e-9 / 8.0
+    assert np.abs(got["Jste"][1:] - Jo[1:]).sum() / Jo[1:].sum() < 2e-3
+    up = oracle.update(m.bt, E, Z, NS, True, got["Jste"], s0["Te"], s0["Ne"], s0["Hden"],
+                       s0["ionDen"], nthreads=8)
+    np.testing.assert_allclose(got["Te"][1:], up["Te"][1:], rtol=1e-9)
+    np.testing.assert_allclose(got["ionDen"][1:], up["ionDen"][1:], rtol=1e-8, atol=1e-25)
+    To = oracle.dust_temperature(m.dust, nu, w, m.Ndust, Td0, got["Jste"], dustOnly=False)[0]
+    np.testing.assert_allclose(got["Tdust"][1:], To[1:], rtol=1e-10)
+    assert st["convPercent"] == pytest.approx(100.0 * up["lgConverged"][1:].mean())
