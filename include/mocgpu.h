@@ -85,6 +85,10 @@ extern "C" {
 #define MOC_F_DUSTPDF 15   /* dust emission CDF (cell, bin), dust-only runs */
 #define MOC_F_ABSOPAC 16   /* dust absorption kappa [cm^-1] (cell, bin) */
 #define MOC_F_SCAOPAC 17   /* dust scattering kappa [cm^-1] (cell, bin) */
+/* grain terms of the last gas update of a photoelectric gas + dust run,
+ * (0:nCells, nSpecies*nSizes), cell fastest, grain (species-1)*nSizes + size-1 */
+#define MOC_F_GRAINCOLL 18 /* gas-grain collision heating of the grains (gasDustColl_d) */
+#define MOC_F_GRAINPE 19   /* photoelectric cooling of the grains (photoelHeat_d) */
 
 typedef struct moc_ctx moc_ctx;
 
@@ -258,8 +262,9 @@ typedef struct moc_dust_tables {
     int32_t nTemps;              /* temperatures of emIntegral: 1..nTemps K (3000) */
     int32_t lgGas;               /* 0: dust only (absorbed packets re-emitted from dustPDF);
                                     1: gas + dust (re-emitted as diffuse packets) */
-    int32_t lgPhotoelectric;     /* grain photoelectric heating and gas-grain collisions
-                                    (update_mod.f90:484-884); must be 0 */
+    int32_t lgPhotoelectric;     /* grain charge, photoelectric heating and gas-grain
+                                    collisions (update_mod.f90:484-884); gas + dust only,
+                                    after moc_set_balance */
     const double* grainAbun;     /* [nSpecies] */
     const double* grainWeight;   /* [nSizes] */
     const double* TdustSublime;  /* [nSpecies] K */
@@ -267,6 +272,13 @@ typedef struct moc_dust_tables {
     const double* Csca;          /* [nSpecies][nSizes][nbins] */
     const double* emIntegral;    /* [nSpecies][nSizes][nTemps] dustEmIntegral */
     double XHILimit;             /* dust-only convergence limit on the mean grain T */
+    /* lgPhotoelectric only (may be NULL otherwise): */
+    const double* grainRadius;   /* [nSizes] um */
+    const double* grainVn;       /* [nSpecies] potential of a neutral grain [Ryd] */
+    const double* MsurfAtom;     /* [nSpecies] mass of a surface atom [amu] */
+    const double* ipRyd;         /* [31*32] (Z, stage >= 2): the energy [Ryd] released when the
+                                    stage recombines, nu of the first bin of the outer shell of
+                                    stage-1 (setDustGasCollHeatCool :814-823) */
 } moc_dust_tables;
 
 /* Create a context: uploads the mesh and source tables. */

@@ -23,7 +23,9 @@ FIELDS = {
     "opacity": 1, "Jste": 2, "recPDF": 3, "totalLines": 4, "Ne": 5, "Te": 6,
     "Hden": 7, "escaped": 8, "ionDen": 9, "lgConverged": 10, "lgBlack": 11,
     "emission": 12, "Ndust": 13, "Tdust": 14, "dustPDF": 15, "absOpac": 16, "scaOpac": 17,
+    "grainColl": 18, "grainPE": 19,
 }
+GRAIN_FIELDS = {"grainColl", "grainPE"}
 CELLBIN_FIELDS = {"opacity", "Jste", "recPDF", "escaped", "emission", "dustPDF", "absOpac",
                   "scaOpac"}
 
@@ -105,7 +107,8 @@ class MocDustTables(C.Structure):
         ("nSpecies", C.c_int32), ("nSizes", C.c_int32), ("nTemps", C.c_int32),
         ("lgGas", C.c_int32), ("lgPhotoelectric", C.c_int32), ("grainAbun", _dp),
         ("grainWeight", _dp), ("TdustSublime", _dp), ("Cabs", _dp), ("Csca", _dp),
-        ("emIntegral", _dp), ("XHILimit", C.c_double),
+        ("emIntegral", _dp), ("XHILimit", C.c_double), ("grainRadius", _dp), ("grainVn", _dp),
+        ("MsurfAtom", _dp), ("ipRyd", _dp),
     ]
 
 

@@ -106,12 +106,14 @@ __host__ __device__ inline size_t balance_lds_bytes(int NE, int NS, int nNlv) {
 }
 
 // the staged rate columns and index tables that follow the state rows
-inline size_t balance_stage_bytes(const MocBalanceArgs& a) {
+__host__ __device__ inline size_t balance_stage_bytes(const MocBalanceArgs& a) {
     const size_t ints = (size_t)a.NE * 32 + (size_t)a.nSpec + 1 + 2 * (size_t)a.nShells;
     return (size_t)a.ncol * sizeof(double) + ((ints * sizeof(int) + 7) & ~(size_t)7);
 }
 // staging is used while one workgroup's LDS stays within this
 constexpr size_t kMaxBalanceLds = 49152;
+// LDS ceiling of the kernels with grain terms (gfx950: 160 KiB per CU)
+constexpr size_t kMaxGrainLds = 65536;
 
 // dielectronic recombination of the ions with more than 11 electrons
 // (Aldrovandi & Pequignot above 6e4 K, Nussbaumer & Storey below;
@@ -395,6 +397,194 @@ __device__ bool ion_balance(const MocBalanceArgs& a, const RT& r, const Lds& L, 
     }
 }
 
+
+// ------------------------------------------------------------ grains (PE)
+// Grain charge, photoelectric heating and gas-grain collisions of gas + dust
+// runs (update_mod.f90:484-884), one grain (species, size) per lane inside
+// each T-iteration.  Per grain the order of operations is the reference's;
+// the gas heating and cooling sums over grains are taken in grain order.
+constexpr double kAmu = 1.66053e-24;    // constants_mod.f90:11
+constexpr double kMe = 9.109382e-28;    // constants_mod.f90:17
+constexpr double kPiB = 3.141592654;     // constants_mod.f90:45
+constexpr double kY0 = 0.5, kY1 = 0.2;  // updateCell (Baldwin et al. 1991)
+__constant__ double c_aweight[30] = {1.0080, 4.0026, 6.941, 9.0122, 10.811, 12.0111, 14.0067,
+                                     15.9994, 18.9984, 20.179, 22.9898, 24.305, 26.9815, 28.086,
+                                     30.9738, 32.06, 35.453, 39.948, 39.102, 40.08, 44.956, 47.90,
+                                     50.9414, 51.996, 54.9380, 55.847, 58.9332, 58.71, 63.546,
+                                     65.37};
+
+// LDS rows of the grain terms: the cell's photon flux J/(hcRyd nu) per bin,
+// the ion fractions at the start of the update, and per grain the charge,
+// the photoelectric / collision terms of the grains and their gas parts
+struct GrainLds {
+    double *q, *X0, *Vg, *pe, *coll, *pg, *cg;
+};
+
+__host__ __device__ inline size_t grain_lds_bytes(int B, int NE, int NS, int nG) {
+    return ((size_t)B + (size_t)NE * (NS + 2) + 5 * (size_t)(nG > 0 ? nG : 1)) * sizeof(double);
+}
+
+// locate (interpolation_mod.f90:49-106) on the increasing mesh, 1-based
+__device__ int locate_mesh(const double* xa, int n, double x) {
+    if (x > xa[n - 1]) return n;
+    if (x < xa[0]) return 0;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int m = (lo + hi) >> 1;
+        if (xa[m] > x) hi = m; else lo = m + 1;
+    }
+    return lo;
+}
+
+// getGrainEmission (:557-604)
+__device__ double g_emission(const MocBalanceArgs& a, const GrainLds& G, int g, int s, int ai,
+                             double Vg) {
+    const int B = a.B;
+    const double Vn = a.gVn[s];
+    const double thres = Vn + Vg > Vn ? Vn + Vg : Vn;
+    const double* cabs = a.gCabs + (size_t)g * B;
+    const double r = a.gRadius[ai];
+    double sum = 0.;
+    for (int i = locate_mesh(a.gNu, B, thres) + 1; i <= B; ++i) {
+        const double nu = a.gNu[i - 1];
+        double Yn = kY0 * (1. - Vn / nu);
+        if (Yn > kY1) Yn = kY1;
+        double f = 1. - Vg / (nu - Vn);
+        f = f < 0. ? 0. : (f > 1. ? 1. : f);
+        const double Yhat = Yn * f;
+        const double photFlux = G.q[i - 1] * (4. * kPiB);
+        const double Qa = cabs[i - 1] / (1.e-8 * (r * r));
+        sum = sum + Yhat * photFlux * Qa;
+    }
+    return sum;
+}
+
+// getGrainRecombination (:609-682): electrons less the ions of the state at
+// the start of the update
+__device__ double g_recombination(const MocBalanceArgs& a, const Lds& L, const GrainLds& G,
+                                  double T, double Ne, double H, double Vg) {
+    const double kT = 6.336e-6 * T, e8 = (1.1045e-15) * T / kPiB;
+    double vmean = sqrt(e8 / kMe), eta = -Vg / kT;
+    eta = eta <= 0. ? 1. - eta : exp(-eta);
+    double rec = 0. + Ne * vmean * 1. * eta;
+    for (int e = 0; e < a.NE; ++e) {
+        const int z = a.elemZ[e];
+        for (int istage = 2; istage <= imin(z + 1, a.NS); ++istage) {
+            const double cpDen = G.X0[e * L.S2 + istage] * a.abund[e] * H;
+            const double mcp = c_aweight[z - 1] * kAmu;
+            vmean = sqrt(e8 / mcp);
+            double et = (double)(istage - 1) * Vg / kT;
+            et = et <= 0. ? 1. - et : exp(-et);
+            rec = rec - cpDen * vmean * 1. * et;
+        }
+    }
+    return rec;
+}
+
+// setGrainPotential (:484-553): secant on emission = recombination from 0
+// and 0.05 Ryd, steps capped at 5 % of the threshold, 100 passes at most;
+// the slope starts at 0 for each grain
+__device__ double g_potential(const MocBalanceArgs& a, const Lds& L, const GrainLds& G, int g,
+                              int s, int ai, double T, double Ne, double H) {
+    double Vg = 0., VgOld = 0., EmiOld = 0., RecOld = 0., slope = 0.;
+    const double Vn = a.gVn[s];
+    for (int n = 1; n <= 100; ++n) {
+        if (n == 1) {
+            Vg = 0.;
+            VgOld = Vg;
+            EmiOld = g_emission(a, G, g, s, ai, Vg);
+            RecOld = g_recombination(a, L, G, T, Ne, H, Vg);
+            Vg = VgOld + 0.05;
+        }
+        const double thr = Vn + Vg > Vn ? Vn + Vg : Vn;
+        const double Emi = g_emission(a, G, g, s, ai, Vg);
+        const double Rec = g_recombination(a, L, G, T, Ne, H, Vg);
+        double delta = Emi - Rec;
+        const double mx = Emi + Rec > 1.e-35 ? Emi + Rec : 1.e-35;
+        double d1 = fabs(delta / (0.5 * mx));
+        if (d1 < 0.005) return Vg;
+        if (Vg != VgOld) {
+            const double fac = (Emi - EmiOld) - (Rec - RecOld);
+            if (fac != 0.) slope = fac / (Vg - VgOld);
+        }
+        VgOld = Vg;
+        RecOld = Rec;
+        EmiOld = Emi;
+        d1 = -delta / slope;
+        delta = fabs(d1);
+        if (0.05 * thr < delta) delta = 0.05 * thr;
+        delta = copysign(delta, d1);
+        Vg = Vg + delta;
+    }
+    return Vg;
+}
+
+// setPhotoelHeatCool and setDustGasCollHeatCool (:686-884) of one grain:
+// its photoelectric cooling and collision heating (into G.pe, G.coll) and
+// its shares of the gas heating and cooling (G.pg, G.cg, before the common
+// factors)
+__device__ void g_terms(const MocBalanceArgs& a, const Lds& L, const GrainLds& G, int cell, int g,
+                        int s, int ai, double T, double Ne, double H) {
+    const int B = a.B, nS = a.nGS;
+    const double Vn = a.gVn[s], V = G.Vg[g];
+    const double* cabs = a.gCabs + (size_t)g * B;
+    const double th = Vn + V > Vn ? Vn + V : Vn;
+    int thP = locate_mesh(a.gNu, B, th);
+    if (thP <= 0) thP = 1;
+    double pe = 0., pg = 0.;
+    for (int i = thP; i <= B; ++i) {
+        const double nu = a.gNu[i - 1], dn = nu - Vn;
+        double Yn = kY0 * (1. - Vn / nu);
+        if (Yn > kY1) Yn = kY1;
+        double f = 1. - V / dn;
+        f = f < 0. ? 0. : (f > 1. ? 1. : f);
+        const double Yhat = Yn * f;
+        const double photFlux = G.q[i - 1];
+        double m2 = (dn * dn - V * V) / dn;
+        if (m2 < 0.) m2 = 0.;
+        const double EY = Yn * 0.5 * (dn < m2 ? dn : m2);
+        pe = pe + cabs[i - 1] * photFlux * EY * kHcRyd / kPiB;
+        pg = pg + cabs[i - 1] * photFlux * (EY - Yhat * V) * a.gAbun[s] * a.gWeight[ai];
+    }
+    const double kT = 6.336e-6 * T, e8 = (1.1045e-15) * T / kPiB;
+    const double r = a.gRadius[ai];
+    const double Tg = a.Tdust[((size_t)cell * (a.nGA + 1) + ai + 1) * (nS + 1) + s + 1];
+    double coll = 0., cg = 0., IPerg = 0.;
+    for (int e = 0; e < a.NE; ++e) {
+        const int z = a.elemZ[e];
+        for (int istage = 1; istage <= imin(a.NS, z + 1); ++istage) {
+            const double Z = (double)(istage - 1);
+            const double mcp = c_aweight[z - 1] * kAmu;
+            const double vmean = sqrt(e8 / mcp);
+            if (istage > 1) IPerg = a.gIp[z * 32 + istage] * kRyd2erg;
+            const double S = istage == 1 ? 2. * mcp * a.gMsurf[s] / ((mcp + a.gMsurf[s]) * (mcp + a.gMsurf[s])) : 1.;
+            const double psi = Z * V / kT;
+            double eta, xi;
+            if (psi <= 0.) { eta = 1. - psi; xi = 1. - psi / 2.; }
+            else { eta = exp(-psi); xi = (1. + psi / 2.) * eta; }
+            const double X = L.x(e, istage);
+            coll = coll + X * a.abund[e] * H * kPiB * r * r * 1.e-8 * S * vmean *
+                              (2. * kT * kRyd2erg * xi - eta * (Z * V * kRyd2erg - IPerg + 2. * kBoltz * Tg));
+            cg = cg + X * a.gWeight[ai] * a.gAbun[s] * a.Ndust[cell] * a.abund[e] * kPiB * r * r * 1.e-8 * S *
+                          vmean * (2. * kT * kRyd2erg * xi - eta * 2. * kBoltz * Tg);
+        }
+    }
+    {  // electrons
+        const double vmean = sqrt(e8 / kMe), S = 1., Z = -1.;
+        const double psi = Z * V / kT;
+        double eta, xi;
+        if (psi <= 0.) { eta = 1. - psi; xi = 1. - psi / 2.; }
+        else { eta = exp(-psi); xi = (1. + psi / 2.) * eta; }
+        coll = coll + Ne * kPiB * r * r * 1.e-8 * S * vmean * (2. * kT * kRyd2erg * xi - eta * (Z * V * kRyd2erg));
+        cg = cg + Ne * a.gWeight[ai] * a.gAbun[s] * a.Ndust[cell] * kPiB * r * r * 1.e-8 * S * vmean *
+                      (2. * kT * kRyd2erg * xi) / H;
+    }
+    G.pe[g] = pe;
+    G.pg[g] = pg;
+    G.coll[g] = coll / (4. * kPiB);
+    G.cg[g] = cg;
+}
+
 // n-level cooling of one ion, register-resident solve when the level count fits NL
 template <int NL>
 __device__ double ion_cooling(const MocBalanceArgs& a, int atom, double T, double Ne, double scale) {
@@ -406,7 +596,7 @@ __device__ double ion_cooling(const MocBalanceArgs& a, int atom, double T, doubl
 template <int NL, class RT>
 __device__ void therm(const MocBalanceArgs& a, const RT& r, const Lds& L, double T, double Ne,
                       double H, double heatef, const double comp[4], double* heat, double* cool,
-                      int tid) {
+                      int tid, double grainHeat = 0., double grainCool = 0., bool grains = false) {
     // forbidden-line cooling, one ion with data per lane
     for (int k = tid; k < a.nNlv; k += 64) {
         const int e = a.nlvSlot[k], ion = a.nlvIon[k];
@@ -466,6 +656,10 @@ __device__ void therm(const MocBalanceArgs& a, const RT& r, const Lds& L, double
         *heat += cHeat * Ne + comp[2];
         *cool += cCool * Ne;
     }
+    if (grains) {  // thermBalance :1205-1208
+        *cool = *cool + grainCool;
+        *heat = *heat + grainHeat;
+    }
     __syncthreads();  // L.cool is rewritten by the next call
 }
 
@@ -473,7 +667,9 @@ __device__ void therm(const MocBalanceArgs& a, const RT& r, const Lds& L, double
 // so latency hiding comes from other cells' waves.  At the compiler's own
 // allocation (256 VGPRs, one wave per SIMD) the nebula32 balance took 20 ms;
 // capped at 128 VGPRs with a few hundred bytes of spills it takes 13.6 ms.
-template <int NL, bool ST>
+// PE: gas + dust runs with grain charge, photoelectric heating and
+// gas-grain collisions (the rows of GrainLds follow the other LDS rows).
+template <int NL, bool ST, bool PE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 k_balance(MocBalanceArgs a) {
     extern __shared__ double lds[];
@@ -525,6 +721,21 @@ k_balance(MocBalanceArgs a) {
         const int e = k / S2, ion = k % S2;
         L.X[k] = (ion >= 1 && ion <= NS) ? a.ionDen[((size_t)(ion - 1) * NE + e) * C1 + cell] : 0.0;
     }
+    const int nG = PE ? a.nGS * a.nGA : 0;
+    GrainLds G{};
+    if constexpr (PE) {
+        size_t off = balance_lds_bytes(NE, NS, a.nNlv);
+        if constexpr (ST) off += balance_stage_bytes(a);
+        G.q = lds + off / sizeof(double);
+        G.X0 = G.q + a.B;
+        G.Vg = G.X0 + NE * S2;
+        G.pe = G.Vg + nG;
+        G.coll = G.pe + nG;
+        G.pg = G.coll + nG;
+        G.cg = G.pg + nG;
+        for (int b = tid; b < a.B; b += 64) G.q[b] = a.J[(size_t)b * C1 + cell] / (kHcRyd * a.gNu[b]);
+        for (int k = tid; k < NE * S2; k += 64) G.X0[k] = L.X[k];
+    }
     __syncthreads();
     double Te = a.Te[cell], Ne = a.Ne[cell];
     const double H = a.Hden[cell];
@@ -558,7 +769,23 @@ k_balance(MocBalanceArgs a) {
         if (a.elemSlot[8] >= 0) revFrozen[1] = rev_chex(a, L, Te, H, 8, 1);
         ib = ion_balance(a, r, L, Te, Ne, H, revFrozen, tid);
         double heat, cool;
-        therm<NL>(a, r, L, Te, Ne, H, heatef, comp, &heat, &cool, tid);
+        if constexpr (PE) {  // grain charges and their heating/cooling terms (:318-333)
+            for (int g = tid; g < nG; g += 64) {
+                const int s = g / a.nGA, ai = g % a.nGA;
+                G.Vg[g] = g_potential(a, L, G, g, s, ai, Te, Ne, H);
+                g_terms(a, L, G, cell, g, s, ai, Te, Ne, H);
+            }
+            __syncthreads();
+            double pg = 0., cg = 0.;
+            for (int g = 0; g < nG; ++g) {
+                pg = pg + G.pg[g];
+                cg = cg + G.cg[g];
+            }
+            const double peG = pg * a.Ndust[cell] * kHcRyd / H, collG = cg / (4. * kPiB);
+            therm<NL>(a, r, L, Te, Ne, H, heatef, comp, &heat, &cool, tid, peG, collG, true);
+        } else {
+            therm<NL>(a, r, L, Te, Ne, H, heatef, comp, &heat, &cool, tid);
+        }
         const double res = heat - cool;
         if (fabs(res) >= 0.02 * (fabs(cool) + fabs(heat)) / 2.) {
             if (nT < 20) {
@@ -587,6 +814,12 @@ k_balance(MocBalanceArgs a) {
         const int e = k / S2, ion = k % S2;
         if (ion >= 1 && ion <= NS) a.ionDen[((size_t)(ion - 1) * NE + e) * C1 + cell] = L.X[k];
     }
+    if constexpr (PE) {  // the grain terms of the last T iteration, for getDustT
+        for (int g = tid; g < nG; g += 64) {
+            a.collD[(size_t)g * C1 + cell] = G.coll[g];
+            a.peD[(size_t)g * C1 + cell] = G.pe[g];
+        }
+    }
     if (tid == 0) {
         if (noTe) atomicAdd(&a.counters[2], 1ull);
         if (!ib) atomicAdd(&a.counters[1], 1ull);
@@ -612,18 +845,27 @@ int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a, const double* J, c
     const size_t base = balance_lds_bytes(a.NE, a.NS, a.nNlv);
     const size_t staged = base + balance_stage_bytes(a);
     const bool st = staged <= kMaxBalanceLds;
-    const size_t lds = st ? staged : base;
+    const bool pe = a.nGS > 0 && a.nGA > 0;
+    const size_t lds = (st ? staged : base) + (pe ? grain_lds_bytes(a.B, a.NE, a.NS, a.nGS * a.nGA) : 0);
     const dim3 grid((unsigned)a.nCells), block(64);
+    if (pe && lds > kMaxGrainLds) {
+        ctx->err = "balance: grain tables need " + std::to_string(lds) + " bytes of LDS per cell (limit " +
+                   std::to_string(kMaxGrainLds) + ")";
+        return MOC_ERR_ARG;
+    }
     if (a.nCells > 0) {
-        if (a.maxLev <= 5) {
-            if (st) hipLaunchKernelGGL((k_balance<5, true>), grid, block, lds, ctx->stream, a);
-            else hipLaunchKernelGGL((k_balance<5, false>), grid, block, lds, ctx->stream, a);
+        if (pe) {
+            if (st) hipLaunchKernelGGL((k_balance<MOC_MAXLEV, true, true>), grid, block, lds, ctx->stream, a);
+            else hipLaunchKernelGGL((k_balance<MOC_MAXLEV, false, true>), grid, block, lds, ctx->stream, a);
+        } else if (a.maxLev <= 5) {
+            if (st) hipLaunchKernelGGL((k_balance<5, true, false>), grid, block, lds, ctx->stream, a);
+            else hipLaunchKernelGGL((k_balance<5, false, false>), grid, block, lds, ctx->stream, a);
         } else if (a.maxLev <= 8) {
-            if (st) hipLaunchKernelGGL((k_balance<8, true>), grid, block, lds, ctx->stream, a);
-            else hipLaunchKernelGGL((k_balance<8, false>), grid, block, lds, ctx->stream, a);
+            if (st) hipLaunchKernelGGL((k_balance<8, true, false>), grid, block, lds, ctx->stream, a);
+            else hipLaunchKernelGGL((k_balance<8, false, false>), grid, block, lds, ctx->stream, a);
         } else {
-            if (st) hipLaunchKernelGGL((k_balance<MOC_MAXLEV, true>), grid, block, lds, ctx->stream, a);
-            else hipLaunchKernelGGL((k_balance<MOC_MAXLEV, false>), grid, block, lds, ctx->stream, a);
+            if (st) hipLaunchKernelGGL((k_balance<MOC_MAXLEV, true, false>), grid, block, lds, ctx->stream, a);
+            else hipLaunchKernelGGL((k_balance<MOC_MAXLEV, false, false>), grid, block, lds, ctx->stream, a);
         }
     }
     hipError_t e = hipGetLastError();

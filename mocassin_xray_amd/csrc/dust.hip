@@ -145,7 +145,9 @@ __global__ void __launch_bounds__(kThreads) k_dust_temp(MocDustArgs a) {
     for (int s = 1; s <= a.nS; ++s) {
         for (int ai = 1; ai <= a.nA; ++ai) {
             const int g = (s - 1) * a.nA + (ai - 1);
-            const double x = a.R[(size_t)g * C1 + cell];
+            double x = a.R[(size_t)g * C1 + cell];
+            if (a.collD)  // gas-grain collisions and photoelectric cooling (:3390-3393)
+                x = x + a.collD[(size_t)g * C1 + cell] - a.peD[(size_t)g * C1 + cell];
             const double* E = a.emInt + (size_t)g * a.nT;
             const int iT = locate_inc(E, a.nT, x);
             double T;

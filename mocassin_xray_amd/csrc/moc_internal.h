@@ -147,6 +147,21 @@ struct MocBalanceArgs {
     const int* nlvAtom;
     int colHit, colComp;      // rate columns of the hit test and the Compton sums
     int ncol;                 // rate columns per cell
+    // grains with photoelectric heating and gas-grain collisions (nGS = 0: none)
+    int nGS, nGA, B;
+    const double* gAbun;      // [nGS]
+    const double* gWeight;    // [nGA]
+    const double* gRadius;    // [nGA] um
+    const double* gVn;        // [nGS] Ryd
+    const double* gMsurf;     // [nGS] amu
+    const double* gCabs;      // [nGS][nGA][B]
+    const double* gIp;        // [31*32] Ryd released on recombination of (Z, stage)
+    const double* gNu;        // [B]
+    const double* J;          // [B][C1] normalised J
+    const double* Tdust;      // [C1][nGA+1][nGS+1]
+    const double* Ndust;      // [C1]
+    double* collD;            // [nGS*nGA][C1] out: gas-grain collision heating of the grains
+    double* peD;              // [nGS*nGA][C1] out: photoelectric cooling of the grains
     const double* R;          // [ncol][C1] rates
     double* Te;
     double* Ne;
@@ -249,6 +264,8 @@ struct MocDust {
     double *d_abun = nullptr, *d_weight = nullptr, *d_tsub = nullptr;
     double *d_Cabs = nullptr, *d_Csca = nullptr, *d_emInt = nullptr;
     double* d_widFlx = nullptr; // [B]
+    int lgPE = 0;               // grain charge, photoelectric heating, gas-grain collisions
+    double *d_radius = nullptr, *d_vn = nullptr, *d_msurf = nullptr, *d_ip = nullptr;
     double* d_W = nullptr;      // [B][nS*nA+1] Cabs/pi and the hit column
     double* d_R = nullptr;      // [nS*nA+1][C+1]
     unsigned long long* d_counters = nullptr;  // [2]
@@ -273,6 +290,8 @@ struct MocDustArgs {
     double* opacity;        // [B][C+1]
     double* pdf;            // [C+1][B] dust-only emission CDF
     const double* R;        // [nS*nA+1][C+1] absorption integrals, last row the hit test
+    const double* collD;    // [nS*nA][C+1] gas-grain collision heating, or null
+    const double* peD;      // [nS*nA][C+1] photoelectric cooling, or null
     int* lgConverged;
     int* lgBlack;
     unsigned long long* counters;  // noHit, converged
@@ -325,6 +344,8 @@ struct MocGrid {
     double* d_absOpac = nullptr;     // [B][C+1]
     double* d_scaOpac = nullptr;     // [B][C+1]
     double* d_gasOpac = nullptr;     // [B][C+1] gas kappa of gas + dust runs
+    double* d_collD = nullptr;       // [nS*nA][C+1] photoelectric runs: collision heating
+    double* d_peD = nullptr;         // [nS*nA][C+1] and photoelectric cooling of the grains
     float* f_scaOpac = nullptr;
     double* d_ionDen = nullptr;      // [NS][NE][C+1]
     int* d_conv = nullptr;           // [C+1] lgConverged

@@ -63,7 +63,7 @@ void free_grid(MocGrid& g) {
     dfree(g.d_ionDen);
     dfree(g.d_conv); dfree(g.d_black); dfree(g.d_nIterT); dfree(g.d_rates);
     dfree(g.d_Ndust); dfree(g.d_Tdust); dfree(g.d_absOpac); dfree(g.d_scaOpac);
-    dfree(g.d_gasOpac); dfree(g.f_scaOpac);
+    dfree(g.d_gasOpac); dfree(g.f_scaOpac); dfree(g.d_collD); dfree(g.d_peD);
     g = MocGrid();
 }
 
@@ -216,6 +216,8 @@ double* field_ptr(moc_ctx* ctx, int f) {
         case MOC_F_DUSTPDF: return ctx->dust.set && !ctx->dust.lgGas ? g.d_recPDF : nullptr;
         case MOC_F_ABSOPAC: return g.d_absOpac;
         case MOC_F_SCAOPAC: return g.d_scaOpac;
+        case MOC_F_GRAINCOLL: return g.d_collD;
+        case MOC_F_GRAINPE: return g.d_peD;
         default: return nullptr;
     }
 }
@@ -231,6 +233,7 @@ int64_t field_count(moc_ctx* ctx, int f) {
     if (is_cellbin(f)) return C1 * ctx->nbins;
     if (f == MOC_F_IONDEN) return ionden_count(ctx);
     if (f == MOC_F_TDUST) return C1 * (ctx->dust.nA + 1) * (ctx->dust.nS + 1);
+    if (f == MOC_F_GRAINCOLL || f == MOC_F_GRAINPE) return C1 * ctx->dust.nA * ctx->dust.nS;
     return C1;
 }
 
@@ -511,6 +514,8 @@ MocDustArgs dust_args(moc_ctx* ctx) {
     a.opacity = g.d_opacity;
     a.pdf = g.d_recPDF;
     a.R = d.d_R;
+    a.collD = d.lgPE ? g.d_collD : nullptr;
+    a.peD = d.lgPE ? g.d_peD : nullptr;
     a.lgConverged = g.d_conv; a.lgBlack = g.d_black;
     a.counters = d.d_counters;
     a.XHILimit = d.XHILimit;
@@ -909,6 +914,14 @@ int moc_update(moc_ctx* ctx, int ig, moc_update_stats* stats) {
     a.lgBlack = g.d_black;
     a.nIterT = g.d_nIterT;
     a.counters = b.d_counters;
+    a.nGS = a.nGA = 0;
+    if (ctx->dust.set && ctx->dust.lgPE) {  // grain terms inside the T iteration
+        MocDust& d = ctx->dust;
+        a.nGS = d.nS; a.nGA = d.nA; a.B = ctx->nbins;
+        a.gAbun = d.d_abun; a.gWeight = d.d_weight; a.gRadius = d.d_radius; a.gVn = d.d_vn;
+        a.gMsurf = d.d_msurf; a.gCabs = d.d_Cabs; a.gIp = d.d_ip; a.gNu = ctx->d_nu; a.J = g.d_J;
+        a.Tdust = g.d_Tdust; a.Ndust = g.d_Ndust; a.collD = g.d_collD; a.peD = g.d_peD;
+    }
     MOC_TRY(hipMemsetAsync(b.d_counters, 0, 4 * sizeof(unsigned long long), ctx->stream), "counters");
     // TeOld, NeOld = Te, Ne before the balance (iteration_mod.f90:1139-1142)
     const size_t C1b = ((size_t)g.nCells + 1) * sizeof(double);
@@ -1205,9 +1218,10 @@ int moc_set_dust(moc_ctx* ctx, const moc_dust_tables* t) {
         ctx->err = "moc_set_dust: invalid grain tables";
         return MOC_ERR_ARG;
     }
-    if (t->lgPhotoelectric) {
-        ctx->err = "moc_set_dust: grain photoelectric heating and gas-grain collisions are not built "
-                   "(run with noPhotoelectric)";
+    if (t->lgPhotoelectric && (!t->lgGas || !ctx->balance.set || !t->grainRadius || !t->grainVn ||
+                               !t->MsurfAtom || !t->ipRyd)) {
+        ctx->err = "moc_set_dust: the photoelectric terms need gas, moc_set_balance first and the "
+                   "grain radii, work functions, surface masses and recombination energies";
         return MOC_ERR_ARG;
     }
     if ((int64_t)t->nSpecies * t->nSizes + 1 > 4096 ||
@@ -1238,10 +1252,24 @@ int moc_set_dust(moc_ctx* ctx, const moc_dust_tables* t) {
         (rc = dust_put(ctx, (const double*)nullptr, (size_t)(nG + 1) * C1, &d.d_R, "dust R")) ||
         (rc = dust_put(ctx, (const unsigned long long*)nullptr, 2, &d.d_counters, "dust counters")))
         return rc;
+    d.lgPE = t->lgPhotoelectric ? 1 : 0;
+    if (d.lgPE &&
+        ((rc = dust_put(ctx, t->grainRadius, (size_t)nA, &d.d_radius, "grainRadius")) ||
+         (rc = dust_put(ctx, t->grainVn, (size_t)nS, &d.d_vn, "grainVn")) ||
+         (rc = dust_put(ctx, t->MsurfAtom, (size_t)nS, &d.d_msurf, "MsurfAtom")) ||
+         (rc = dust_put(ctx, t->ipRyd, (size_t)31 * 32, &d.d_ip, "ipRyd"))))
+        return rc;
     // per-cell arrays of the grid: Ndust 0, Tdust 50 K (dust_mod.f90:66), kappa 0
     const size_t nTd = C1 * (nA + 1) * (nS + 1);
     dfree(g.d_Ndust); dfree(g.d_Tdust); dfree(g.d_absOpac); dfree(g.d_scaOpac);
-    dfree(g.d_gasOpac); dfree(g.f_scaOpac);
+    dfree(g.d_gasOpac); dfree(g.f_scaOpac); dfree(g.d_collD); dfree(g.d_peD);
+    if (d.lgPE && ((rc = dalloc_n(ctx, &g.d_collD, C1 * nG, "collD")) ||
+                   (rc = dalloc_n(ctx, &g.d_peD, C1 * nG, "peD"))))
+        return rc;
+    if (d.lgPE) {
+        MOC_TRY(hipMemsetAsync(g.d_collD, 0, C1 * nG * 8, ctx->stream), "zero");
+        MOC_TRY(hipMemsetAsync(g.d_peD, 0, C1 * nG * 8, ctx->stream), "zero");
+    }
     if ((rc = dalloc_n(ctx, &g.d_Ndust, C1, "Ndust")) || (rc = dalloc_n(ctx, &g.d_Tdust, nTd, "Tdust")) ||
         (rc = dalloc_n(ctx, &g.d_absOpac, C1 * B, "absOpac")) ||
         (rc = dalloc_n(ctx, &g.d_scaOpac, C1 * B, "scaOpac")) ||

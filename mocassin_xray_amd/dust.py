@@ -145,6 +145,8 @@ class DustTables:
     Cabs: np.ndarray         # [nS][nSizes][B] cm^2 per grain
     Csca: np.ndarray         # [nS][nSizes][B]
     emIntegral: np.ndarray   # [nS][nSizes][N_TEMPS] dustEmIntegral
+    Vn: np.ndarray = None    # [nS] potential of a neutral grain [Ryd] (grainVn)
+    Msurf: np.ndarray = None # [nS] surface atom mass [amu] (MsurfAtom)
 
     @property
     def nSpecies(self) -> int:
@@ -177,11 +179,13 @@ def build_dust(nu: np.ndarray, widflx: np.ndarray, species: list, abundances,
     Csca = np.zeros((nS, nA, B))
     Tsub = np.zeros(nS)
     rho = np.zeros(nS)
+    vn = np.zeros(nS)
+    ms = np.zeros(nS)
     lam_um = C_LIGHT / (nu * FR1RYD) * 1.e4
     for s, name in enumerate(species):
         n_re, n_im = refractive_index_on_mesh(name, nu)
         hdr = optical_constants(name)[3]
-        Tsub[s], rho[s] = hdr[0], hdr[1]
+        Tsub[s], rho[s], vn[s], ms[s] = hdr[0], hdr[1], hdr[2], hdr[3]
         m = n_re + 1j * n_im
         for a_i, a in enumerate(radius):
             x = 2.0 * PI_REF * a / lam_um
@@ -191,7 +195,7 @@ def build_dust(nu: np.ndarray, widflx: np.ndarray, species: list, abundances,
             Csca[s, a_i] = qs * geo
     emi = emission_integrals(Cabs, nu, widflx)
     return DustTables(list(species), np.asarray(abundances, dtype=np.float64), radius, gw,
-                      Tsub, rho, Cabs, Csca, emi)
+                      Tsub, rho, Cabs, Csca, emi, vn, ms)
 
 
 def ndust_from_mdmg(dt: DustTables, mdmg: float, hden: np.ndarray, elem_z, abund) -> np.ndarray:
@@ -205,3 +209,19 @@ def ndust_from_mdmg(dt: DustTables, mdmg: float, hden: np.ndarray, elem_z, abund
             den += (1.3333 * PI_REF * (dt.radius[a_i] * 1.e-4) ** 3 * dt.rho[s] *
                     dt.grainWeight[a_i] * dt.grainAbun[s])
     return gas_cell * mdmg * np.asarray(hden) / den
+
+
+def ip_ryd(xsec) -> np.ndarray:
+    """[31*32] (Z, stage): nu [Ryd] of the first bin of the outermost shell of
+    stage - 1, the energy released when `stage` recombines (the IPerg of
+    setDustGasCollHeatCool, update_mod.f90:814-823); 0 where undefined."""
+    out = np.zeros(31 * 32)
+    best = {}
+    for sh in xsec.shells:
+        k = (sh.elem, sh.ion)
+        if k not in best or sh.shell > best[k].shell:
+            best[k] = sh
+    for (z, ion), sh in best.items():
+        if ion + 1 < 32:
+            out[z * 32 + ion + 1] = xsec.nu[sh.lo - 1]
+    return out

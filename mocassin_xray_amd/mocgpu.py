@@ -21,8 +21,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import (CELLBIN_FIELDS, FIELDS, MOC_DT_F64, MOC_DT_I32, MOC_LAYOUT_FORTRAN,
-                   MOC_PREC_FP64, MOC_PREC_MIXED, MocAtomic, MocBalanceTables, MocDustTables,
+from ._lib import (CELLBIN_FIELDS, FIELDS, GRAIN_FIELDS, MOC_DT_F64, MOC_DT_I32,
+                   MOC_LAYOUT_FORTRAN, MOC_PREC_FP64, MOC_PREC_MIXED, MocAtomic, MocBalanceTables, MocDustTables,
                    MocEmissionTables, MocGridDesc, MocIterationStats, MocTables, MocTransportStats,
                    MocUpdateStats)
 
@@ -154,12 +154,19 @@ def emission_arrays(et) -> dict:
     }
 
 
-def dust_arrays(dt, lgGas: bool, XHILimit: float) -> dict:
+def dust_arrays(dt, lgGas: bool, XHILimit: float, xsec=None) -> dict:
+    """xsec (the gas cross-section stack) switches on the grain photoelectric
+    and gas-grain collision terms (lgPhotoelectric)."""
+    from .dust import ip_ryd
+    pe = xsec is not None
     return {
         "nSpecies": dt.nSpecies, "nSizes": dt.nSizes, "nTemps": int(dt.emIntegral.shape[2]),
-        "lgGas": 1 if lgGas else 0, "lgPhotoelectric": 0, "grainAbun": _c(dt.grainAbun),
-        "grainWeight": _c(dt.grainWeight), "TdustSublime": _c(dt.Tsub), "Cabs": _c(dt.Cabs),
-        "Csca": _c(dt.Csca), "emIntegral": _c(dt.emIntegral), "XHILimit": float(XHILimit),
+        "lgGas": 1 if lgGas else 0, "lgPhotoelectric": 1 if pe else 0,
+        "grainAbun": _c(dt.grainAbun), "grainWeight": _c(dt.grainWeight),
+        "TdustSublime": _c(dt.Tsub), "Cabs": _c(dt.Cabs), "Csca": _c(dt.Csca),
+        "emIntegral": _c(dt.emIntegral), "XHILimit": float(XHILimit),
+        "grainRadius": _c(dt.radius) if pe else None, "grainVn": _c(dt.Vn) if pe else None,
+        "MsurfAtom": _c(dt.Msurf) if pe else None, "ipRyd": _c(ip_ryd(xsec)) if pe else None,
     }
 
 
@@ -237,6 +244,8 @@ class MocGPU:
             return c1 * self.n_elem * self.nstages
         if name == "Tdust":
             return c1 * (self.dust_shape[0] + 1) * (self.dust_shape[1] + 1)
+        if name in GRAIN_FIELDS:
+            return c1 * self.dust_shape[0] * self.dust_shape[1]
         return c1 * self.nbins if name in CELLBIN_FIELDS else c1
 
     # ----------------------------------------------------------------- atomic
@@ -280,10 +289,13 @@ class MocGPU:
         return st.as_dict()
 
     # ------------------------------------------------------------------- dust
-    def set_dust(self, dt, lgGas: bool, XHILimit: float = 0.05) -> None:
+    def set_dust(self, dt, lgGas: bool, XHILimit: float = 0.05, photoelectric_xsec=None) -> None:
         """Grain tables (a dust.DustTables built on the same mesh); after
-        grid_attach.  Tdust starts at 50 K, Ndust at 0."""
-        self._dust_keep = dust_arrays(dt, lgGas, XHILimit)
+        grid_attach.  Tdust starts at 50 K, Ndust at 0.  With gas, passing the
+        cross-section stack as photoelectric_xsec switches on the grain
+        charge, photoelectric heating and gas-grain collisions (after
+        set_balance)."""
+        self._dust_keep = dust_arrays(dt, lgGas, XHILimit, photoelectric_xsec)
         t = to_struct(MocDustTables, self._dust_keep)
         self._check(self._L.moc_set_dust(self._ctx, C.byref(t)), "moc_set_dust")
         self.dust_shape = (dt.nSpecies, dt.nSizes)
@@ -345,6 +357,8 @@ class MocGPU:
         if name == "Tdust":
             nS, nA = self.dust_shape
             return buf.reshape(c1, nA + 1, nS + 1)
+        if name in GRAIN_FIELDS:         # (nSpecies*nSizes, C+1)
+            return buf.reshape(self.dust_shape[0] * self.dust_shape[1], c1)
         return buf
 
     # -------------------------------------------------------------- transport

@@ -268,21 +268,24 @@ def dust_shell(n: int = 16, nbins: int = 300, n_packets: int = 1_000_000,
 
 @dataclass
 class GasDustModel(GasModel):
-    """A gas model with grains (MdMg dust-to-gas mass ratio), run without the
-    grain photoelectric terms (the reference's noPhotoelectric keyword)."""
+    """A gas model with grains (MdMg dust-to-gas mass ratio).  photoelectric
+    switches on grain charging, photoelectric heating and gas-grain
+    collisions; without it the run is the reference's noPhotoelectric one."""
     dust: object = None     # dust.DustTables
     Ndust: np.ndarray = None
+    photoelectric: bool = False
 
     def load(self, g) -> None:
         super().load(g)
-        g.set_dust(self.dust, lgGas=True)
+        g.set_dust(self.dust, lgGas=True,
+                   photoelectric_xsec=self.xsec if self.photoelectric else None)
         g.upload("Ndust", self.Ndust)
 
 
 def gas_dust_model(name: str, n: int, R: float, hden, elems: list, abund: np.ndarray,
                    nstages: int, nbins: int, nu_min: float, nu_max: float, T_star: float,
                    L36: float, n_packets: int, species: list, grain_abun, radius, weights,
-                   mdmg: float, maxLev: int = 5) -> GasDustModel:
+                   mdmg: float, maxLev: int = 5, photoelectric: bool = False) -> GasDustModel:
     """hden is a number or an (n, n, n) array of n_H per cell."""
     from . import atomic, balance, emission
     from . import dust as D
@@ -300,7 +303,8 @@ def gas_dust_model(name: str, n: int, R: float, hden, elems: list, abund: np.nda
     nd = D.ndust_from_mdmg(dt, mdmg, hcell, elems, abund)
     nd[0] = 0.0
     return GasDustModel(name, tabs, grid, list(elems), np.asarray(abund, dtype=np.float64),
-                        nstages, xs, bt, et, st, n_packets, dust=dt, Ndust=nd)
+                        nstages, xs, bt, et, st, n_packets, dust=dt, Ndust=nd,
+                        photoelectric=photoelectric)
 
 
 def lognormal_density(n: int, mean: float = 300.0, sigma_ln: float = 1.0,

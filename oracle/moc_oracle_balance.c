@@ -27,6 +27,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "moc_oracle.h"
 #include "moc_oracle_balance.h"
 
 #define NEL 30
@@ -252,6 +253,7 @@ double oracle_nlevel_lines(const oracle_balance_tables* t, int atom, double Te, 
 typedef struct {
     double Te, Ne, H;
     double X[31][32];      /* ionDen by (Z, stage) */
+    double X0[31][32];     /* ionDen at the start of the cell update (grid%ionDen) */
     double ab[31];         /* abundance by Z */
     int on[31];
     double alpha[31][32];  /* alphaTot */
@@ -587,6 +589,202 @@ static int ion_balance(const oracle_balance_tables* t, ocell* c) {
 }
 
 /* thermBalance (:887-1215): heating and cooling integrals */
+/* ---- grains: charge, photoelectric heating, gas-grain collisions ---------- */
+#define ORC_AMU 1.66053e-24    /* constants_mod.f90:11 */
+#define ORC_ME 9.109382e-28    /* constants_mod.f90:17 */
+#define ORC_PIB 3.141592654    /* constants_mod.f90:45 */
+#define GY0 0.5                /* updateCell Y0, Y1 (Baldwin et al. 1991) */
+#define GY1 0.2
+static const double AWEIGHT[30] = {1.0080, 4.0026, 6.941, 9.0122, 10.811, 12.0111, 14.0067, 15.9994,
+                                   18.9984, 20.179, 22.9898, 24.305, 26.9815, 28.086, 30.9738, 32.06,
+                                   35.453, 39.948, 39.102, 40.08, 44.956, 47.90, 50.9414, 51.996,
+                                   54.9380, 55.847, 58.9332, 58.71, 63.546, 65.37};
+
+/* getGrainEmission (update_mod.f90:557-604) */
+static double grain_emission(const oracle_balance_tables* t, const oracle_balance_state* st, int cell,
+                             int g, int s, int a, double Vg) {
+    const int C1 = st->nCells + 1, B = t->nbins;
+    const double Vn = st->gVn[s];
+    const double thres = Vn + Vg > Vn ? Vn + Vg : Vn;
+    const double* cabs = st->gCabs + (size_t)g * B;
+    const double r = st->gRadius[a];
+    double sum = 0.;
+    int i, ip = oracle_locate(t->nuArray, B, thres) + 1;
+    for (i = ip; i <= B; i++) {
+        const double nu = t->nuArray[i - 1];
+        double Yn = GY0 * (1. - Vn / nu), Yhat, photFlux, Qa, f;
+        if (Yn > GY1) Yn = GY1;
+        f = 1. - Vg / (nu - Vn);
+        if (f < 0.) f = 0.;
+        if (f > 1.) f = 1.;
+        Yhat = Yn * f;
+        photFlux = st->J[(size_t)(i - 1) * C1 + cell] / (HCRYD * nu);
+        photFlux = photFlux * (4. * ORC_PIB);
+        Qa = cabs[i - 1] / (1.e-8 * (r * r));
+        sum = sum + Yhat * photFlux * Qa;
+    }
+    return sum;
+}
+
+/* getGrainRecombination (update_mod.f90:609-682): electrons minus the ions of
+ * the grid state at the start of the update */
+static double grain_recombination(const oracle_balance_tables* t, const ocell* c, double Vg) {
+    const double kT = 6.336e-6 * c->Te;
+    const double e8 = (1.1045e-15) * c->Te / ORC_PIB;
+    double vmean = sqrt(e8 / ORC_ME), eta = -Vg / kT, rec;
+    int z, istage;
+    eta = eta <= 0. ? 1. - eta : exp(-eta);
+    rec = 0. + c->Ne * vmean * 1. * eta;
+    for (z = 1; z <= NEL; z++) {
+        if (!c->on[z]) continue;
+        for (istage = 2; istage <= min_i(z + 1, t->nstages); istage++) {
+            const double cpDen = c->X0[z][istage] * c->ab[z] * c->H;
+            const double mcp = AWEIGHT[z - 1] * ORC_AMU;
+            const double Z = (double)(istage - 1);
+            vmean = sqrt(e8 / mcp);
+            eta = Z * Vg / kT;
+            eta = eta <= 0. ? 1. - eta : exp(-eta);
+            rec = rec - cpDen * vmean * 1. * eta;
+        }
+    }
+    return rec;
+}
+
+/* setGrainPotential (update_mod.f90:484-553): secant iteration on grain
+ * emission = recombination, started at 0 and 0.05 Ryd, steps capped at 5 % of
+ * the threshold, 100 passes at most.  The secant slope starts at 0 for each
+ * grain (the reference's SAVEd slope carries over between grains). */
+static double grain_potential(const oracle_balance_tables* t, const oracle_balance_state* st,
+                              const ocell* c, int cell, int g, int s, int a) {
+    double Vg = 0., VgOld = 0., EmiOld = 0., RecOld = 0., slope = 0.;
+    int n;
+    for (n = 1; n <= 100; n++) {
+        double thr, Emi, Rec, delta, d1, mx;
+        if (n == 1) {
+            Vg = 0.;
+            VgOld = Vg;
+            EmiOld = grain_emission(t, st, cell, g, s, a, Vg);
+            RecOld = grain_recombination(t, c, Vg);
+            Vg = VgOld + 0.05;
+        }
+        thr = st->gVn[s] + Vg > st->gVn[s] ? st->gVn[s] + Vg : st->gVn[s];
+        Emi = grain_emission(t, st, cell, g, s, a, Vg);
+        Rec = grain_recombination(t, c, Vg);
+        delta = Emi - Rec;
+        mx = Emi + Rec > 1.e-35 ? Emi + Rec : 1.e-35;
+        d1 = fabs(delta / (0.5 * mx));
+        if (d1 < 0.005) return Vg;
+        if (Vg != VgOld) {
+            const double fac = (Emi - EmiOld) - (Rec - RecOld);
+            if (fac != 0.) slope = fac / (Vg - VgOld);
+        }
+        VgOld = Vg;
+        RecOld = Rec;
+        EmiOld = Emi;
+        d1 = -delta / slope;
+        delta = fabs(d1);
+        if (0.05 * thr < delta) delta = 0.05 * thr;
+        delta = copysign(delta, d1);
+        Vg = Vg + delta;
+    }
+    return Vg;
+}
+
+/* ionisation potential [erg] released when stage istage of z recombines: the
+ * first bin of the outermost shell of stage istage-1 */
+static double ip_erg(const oracle_balance_tables* t, int z, int ion) {
+    int k, best = -1, bestNum = -1;
+    for (k = 0; k < t->nShells; k++)
+        if (t->shZ[k] == z && t->shIon[k] == ion && t->shNum[k] > bestNum) {
+            bestNum = t->shNum[k];
+            best = k;
+        }
+    return best < 0 ? 0. : t->nuArray[t->shLo[best] - 1] * RYD2ERG;
+}
+
+/* setPhotoelHeatCool and setDustGasCollHeatCool (update_mod.f90:686-884) */
+static void grain_terms(const oracle_balance_tables* t, const oracle_balance_state* st, const ocell* c,
+                        int cell, const double* Vg, double* peD, double* collD, double* peG,
+                        double* collG) {
+    const int C1 = st->nCells + 1, B = t->nbins, nS = st->gS, nA = st->gA;
+    const double T = c->Te, Ne = c->Ne;
+    const double kT = 6.336e-6 * T, e8 = (1.1045e-15) * T / ORC_PIB;
+    const double* Td = st->Tdust + (size_t)cell * (nA + 1) * (nS + 1);
+    double pg = 0., cg = 0., IPerg = 0., vmean;
+    int s, a, i, z, istage;
+    for (s = 0; s < nS; s++)
+        for (a = 0; a < nA; a++) {
+            const int g = s * nA + a;
+            const double Vn = st->gVn[s], V = Vg[g];
+            const double th = Vn + V > Vn ? Vn + V : Vn;
+            const double* cabs = st->gCabs + (size_t)g * B;
+            int thP = oracle_locate(t->nuArray, B, th);
+            if (thP <= 0) thP = 1;
+            peD[g] = 0.;
+            for (i = thP; i <= B; i++) {
+                const double nu = t->nuArray[i - 1], dn = nu - Vn;
+                double Yn = GY0 * (1. - Vn / nu), f, Yhat, photFlux, EY, m2;
+                if (Yn > GY1) Yn = GY1;
+                f = 1. - V / dn;
+                if (f < 0.) f = 0.;
+                if (f > 1.) f = 1.;
+                Yhat = Yn * f;
+                photFlux = st->J[(size_t)(i - 1) * C1 + cell] / (HCRYD * nu);
+                m2 = (dn * dn - V * V) / dn;
+                if (m2 < 0.) m2 = 0.;
+                EY = Yn * 0.5 * (dn < m2 ? dn : m2);
+                peD[g] = peD[g] + cabs[i - 1] * photFlux * EY * HCRYD / ORC_PIB;
+                pg = pg + cabs[i - 1] * photFlux * (EY - Yhat * V) * st->gAbun[s] * st->gWeight[a];
+            }
+        }
+    *peG = pg * st->Ndust[cell] * HCRYD / c->H;
+    for (s = 0; s < nS * nA; s++) collD[s] = 0.;
+    for (z = 1; z <= NEL; z++) {
+        if (!c->on[z]) continue;
+        for (istage = 1; istage <= min_i(t->nstages, z + 1); istage++) {
+            const double Z = (double)(istage - 1);
+            const double mcp = AWEIGHT[z - 1] * ORC_AMU;
+            vmean = sqrt(e8 / mcp);
+            if (istage > 1) IPerg = ip_erg(t, z, istage - 1);
+            for (s = 0; s < nS; s++) {
+                const double S = istage == 1 ? 2. * mcp * st->gMsurf[s] / pow(mcp + st->gMsurf[s], 2.) : 1.;
+                for (a = 0; a < nA; a++) {
+                    const int g = s * nA + a;
+                    const double r = st->gRadius[a], V = Vg[g];
+                    const double psi = Z * V / kT;
+                    const double Tg = Td[(size_t)(a + 1) * (nS + 1) + s + 1];
+                    double eta, xi;
+                    if (psi <= 0.) { eta = 1. - psi; xi = 1. - psi / 2.; }
+                    else { eta = exp(-psi); xi = (1. + psi / 2.) * eta; }
+                    collD[g] = collD[g] + c->X[z][istage] * c->ab[z] * c->H * ORC_PIB * r * r * 1.e-8 * S *
+                                              vmean *
+                                              (2. * kT * RYD2ERG * xi -
+                                               eta * (Z * V * RYD2ERG - IPerg + 2. * KBOLTZ * Tg));
+                    cg = cg + c->X[z][istage] * st->gWeight[a] * st->gAbun[s] * st->Ndust[cell] * c->ab[z] *
+                                  ORC_PIB * r * r * 1.e-8 * S * vmean *
+                                  (2. * kT * RYD2ERG * xi - eta * 2. * KBOLTZ * Tg);
+                }
+            }
+        }
+    }
+    vmean = sqrt(e8 / ORC_ME);
+    for (s = 0; s < nS; s++)
+        for (a = 0; a < nA; a++) {
+            const int g = s * nA + a;
+            const double r = st->gRadius[a], V = Vg[g], S = 1., Z = -1.;
+            const double psi = Z * V / kT;
+            double eta, xi;
+            if (psi <= 0.) { eta = 1. - psi; xi = 1. - psi / 2.; }
+            else { eta = exp(-psi); xi = (1. + psi / 2.) * eta; }
+            collD[g] = collD[g] + Ne * ORC_PIB * r * r * 1.e-8 * S * vmean *
+                                      (2. * kT * RYD2ERG * xi - eta * (Z * V * RYD2ERG));
+            cg = cg + Ne * st->gWeight[a] * st->gAbun[s] * st->Ndust[cell] * ORC_PIB * r * r * 1.e-8 * S * vmean *
+                          (2. * kT * RYD2ERG * xi) / c->H;
+        }
+    for (s = 0; s < nS * nA; s++) collD[s] = collD[s] / (4. * ORC_PIB);
+    *collG = cg / (4. * ORC_PIB);
+}
+
 static void therm_balance(const oracle_balance_tables* t, ocell* c, double* heat, double* cool) {
     const int NS = t->nstages;
     const double T = c->Te, Ne = c->Ne;
@@ -658,7 +856,9 @@ static void update_one(const oracle_balance_tables* t, oracle_balance_state* st,
     const int C1 = st->nCells + 1, NE = t->nElemUsed, NS = t->nstages;
     ocell c;
     double XOld, Thigh = 0., Tlow = 0., resHigh = 0., resLow = 0.;
-    int e, ion, b, hit = 0, nT = 0, ibConv = 1;
+    double Vg[4096], peD[4096], collD[4096];
+    int e, ion, b, hit = 0, nT = 0, ibConv = 1, g;
+    const int nG = st->gS * st->gA;
     for (b = 0; b < t->nbins; b++)
         if (st->J[(size_t)b * C1 + cell] > 0.) { hit = 1; break; }
     st->lgBlack[cell] = 0;
@@ -678,6 +878,7 @@ static void update_one(const oracle_balance_tables* t, oracle_balance_state* st,
         c.ab[z] = t->abund[e];
         for (ion = 1; ion <= NS; ion++) c.X[z][ion] = st->ionDen[((size_t)(ion - 1) * NE + e) * C1 + cell];
     }
+    memcpy(c.X0, c.X, sizeof(c.X));
     XOld = c.X[1][1];
     for (;;) {
         double heat, cool, res;
@@ -688,7 +889,16 @@ static void update_one(const oracle_balance_tables* t, oracle_balance_state* st,
         c.nIterX = 1;
         if (t->lgCompton) compton_rates(t, st->J, C1, cell, &c);
         ibConv = ion_balance(t, &c);
-        therm_balance(t, &c, &heat, &cool);
+        if (nG > 0) {  /* grain charges, photoelectric and collision terms (:318-333) */
+            double peG = 0., collG = 0.;
+            for (g = 0; g < nG; g++) Vg[g] = grain_potential(t, st, &c, cell, g, g / st->gA, g % st->gA);
+            grain_terms(t, st, &c, cell, Vg, peD, collD, &peG, &collG);
+            therm_balance(t, &c, &heat, &cool);
+            cool = cool + collG;  /* thermBalance :1205-1208 */
+            heat = heat + peG;
+        } else {
+            therm_balance(t, &c, &heat, &cool);
+        }
         res = heat - cool;
         if (fabs(res) >= 0.02 * (fabs(cool) + fabs(heat)) / 2.) {
             if (nT < 20) {
@@ -718,6 +928,11 @@ static void update_one(const oracle_balance_tables* t, oracle_balance_state* st,
         break;
     }
     if (!ibConv) *nIon += 1.;
+    for (g = 0; g < nG; g++) {
+        st->collD[(size_t)g * C1 + cell] = collD[g];
+        st->peD[(size_t)g * C1 + cell] = peD[g];
+        if (st->grainPot) st->grainPot[(size_t)g * C1 + cell] = Vg[g];
+    }
     st->Te[cell] = c.Te;
     st->Ne[cell] = c.Ne;
     for (e = 0; e < NE; e++) {

@@ -60,6 +60,20 @@ typedef struct oracle_balance_state {
     int32_t* nIterT;            /* [C+1] out (T iterations), may be NULL */
     double XHILimit;
     double noHit, noIonBal, noTeBal; /* out */
+    /* grains of gas + dust runs with photoelectric heating and gas-grain
+     * collisions (update_mod.f90:484-884); gS = 0 without */
+    int gS, gA;
+    const double* gAbun;    /* [gS] grainAbun */
+    const double* gWeight;  /* [gA] grainWeight */
+    const double* gRadius;  /* [gA] um */
+    const double* gVn;      /* [gS] grainVn [Ryd] */
+    const double* gMsurf;   /* [gS] MsurfAtom [amu] */
+    const double* gCabs;    /* [gS][gA][B] */
+    const double* Tdust;    /* [C+1][gA+1][gS+1] */
+    const double* Ndust;    /* [C+1] */
+    double* collD;          /* [gS*gA][C+1] out: gasDustColl_d of the last T-iteration */
+    double* peD;            /* [gS*gA][C+1] out: photoelHeat_d of the last T-iteration */
+    double* grainPot;       /* [gS*gA][C+1] out, may be NULL */
 } oracle_balance_state;
 
 int oracle_update(const oracle_balance_tables* t, oracle_balance_state* s, int nthreads);

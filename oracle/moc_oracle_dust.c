@@ -17,7 +17,7 @@
  * of known T (a dilute blackbody for which Kirchhoff's law fixes T); the
  * opacity and PDF sums against their definitions on hand-made inputs, and
  * against the sublimation switch.  The quantum-heating branches (lgQHeat)
- * and the photoelectric terms of gas + dust runs are not restated.
+ * are not restated.
  */
 #include <math.h>
 #include <stdint.h>
@@ -142,6 +142,8 @@ int oracle_dust_temperature(const oracle_dust_in* in, const double* J, double* T
                 double x = 0., T;
                 int iT;
                 for (i = 0; i < in->B; i++) x = x + c[i] * (J[(size_t)i * C1 + cell] / ORC_PI);
+                if (in->collD)  /* gas-grain collisions and photoelectric cooling (:3390-3393) */
+                    x = x + in->collD[g * C1 + cell] - in->peD[g * C1 + cell];
                 iT = locate_inc(E, in->nT, x);
                 if (iT <= 0) T = 1.;
                 else if (iT >= in->nT) T = (double)in->nT;

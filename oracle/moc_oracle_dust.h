@@ -18,6 +18,8 @@ typedef struct oracle_dust_in {
     const double* widFlx;   /* [B] */
     const double* Ndust;    /* [C+1] */
     double XHILimit;
+    const double* collD;    /* [nS*nA][C+1] gasDustColl_d, or NULL */
+    const double* peD;      /* [nS*nA][C+1] photoelHeat_d, or NULL */
 } oracle_dust_in;
 
 double oracle_planck_h(double nu, double T);

@@ -205,6 +205,9 @@ class BalanceStateC(C.Structure):
         ("nCells", C.c_int), ("J", _dp), ("Te", _dp), ("Ne", _dp), ("Hden", _dp), ("ionDen", _dp),
         ("lgConverged", _ip), ("lgBlack", _ip), ("nIterT", _ip), ("XHILimit", C.c_double),
         ("noHit", C.c_double), ("noIonBal", C.c_double), ("noTeBal", C.c_double),
+        ("gS", C.c_int), ("gA", C.c_int), ("gAbun", _dp), ("gWeight", _dp), ("gRadius", _dp),
+        ("gVn", _dp), ("gMsurf", _dp), ("gCabs", _dp), ("Tdust", _dp), ("Ndust", _dp),
+        ("collD", _dp), ("peD", _dp), ("grainPot", _dp),
     ]
 
 
@@ -271,10 +274,14 @@ def nlevel_cooling(bt, atom, Te, Ne, scale=1.0):
     return L.oracle_nlevel_cooling(C.byref(t), int(atom), float(Te), float(Ne), float(scale))
 
 
-def update(bt, elem_z, abund, nstages, lgCompton, J, Te, Ne, Hden, ionDen, nthreads=1):
+def update(bt, elem_z, abund, nstages, lgCompton, J, Te, Ne, Hden, ionDen, nthreads=1,
+           grains=None):
     """Oracle updateCell over all cells.  J is (C+1, B) normalised; ionDen is
     (C+1, nElemUsed, nstages).  Returns a dict with the new Te, Ne, ionDen,
-    flags, iteration counts and counters."""
+    flags, iteration counts and counters.  grains = (DustTables, Tdust
+    (C+1, nA+1, nS+1), Ndust (C+1,)) adds the grain charge, photoelectric and
+    gas-grain collision terms; the result then also holds collD, peD and
+    grainPot, each (nS*nA, C+1)."""
     L = lib()
     L.oracle_update.argtypes = [C.POINTER(BalanceTablesC), C.POINTER(BalanceStateC), C.c_int]
     L.oracle_update.restype = C.c_int
@@ -293,11 +300,26 @@ def update(bt, elem_z, abund, nstages, lgCompton, J, Te, Ne, Hden, ionDen, nthre
                        ionDen=s["X"].ctypes.data_as(_dp), lgConverged=s["conv"].ctypes.data_as(_ip),
                        lgBlack=s["black"].ctypes.data_as(_ip), nIterT=s["nit"].ctypes.data_as(_ip),
                        XHILimit=float(bt.XHILimit))
+    if grains is not None:
+        dt, Tdust, Ndust = grains
+        nG = dt.nSpecies * dt.nSizes
+        for name, v in (("gAbun", dt.grainAbun), ("gWeight", dt.grainWeight), ("gRadius", dt.radius),
+                        ("gVn", dt.Vn), ("gMsurf", dt.Msurf), ("gCabs", dt.Cabs), ("Tdust", Tdust),
+                        ("Ndust", Ndust)):
+            s[name] = np.ascontiguousarray(v, dtype=np.float64)
+            setattr(st, name, s[name].ctypes.data_as(_dp))
+        for name in ("collD", "peD", "grainPot"):
+            s[name] = np.zeros((nG, C1))
+            setattr(st, name, s[name].ctypes.data_as(_dp))
+        st.gS, st.gA = dt.nSpecies, dt.nSizes
     L.oracle_update(C.byref(t), C.byref(st), int(nthreads))
-    return {"Te": s["Te"], "Ne": s["Ne"],
-            "ionDen": s["X"].transpose(2, 1, 0).copy(), "lgConverged": s["conv"],
-            "lgBlack": s["black"], "nIterT": s["nit"], "noHit": st.noHit,
-            "noIonBal": st.noIonBal, "noTeBal": st.noTeBal}
+    out = {"Te": s["Te"], "Ne": s["Ne"],
+           "ionDen": s["X"].transpose(2, 1, 0).copy(), "lgConverged": s["conv"],
+           "lgBlack": s["black"], "nIterT": s["nit"], "noHit": st.noHit,
+           "noIonBal": st.noIonBal, "noTeBal": st.noTeBal}
+    if grains is not None:
+        out.update(collD=s["collD"], peD=s["peD"], grainPot=s["grainPot"])
+    return out
 
 
 class EmissionTablesC(C.Structure):
@@ -427,11 +449,11 @@ class DustIn(C.Structure):
         ("nCells", C.c_int), ("B", C.c_int), ("nS", C.c_int), ("nA", C.c_int), ("nT", C.c_int),
         ("dustOnly", C.c_int), ("abun", _dp), ("weight", _dp), ("tsub", _dp), ("Cabs", _dp),
         ("Csca", _dp), ("emInt", _dp), ("nu", _dp), ("widFlx", _dp), ("Ndust", _dp),
-        ("XHILimit", C.c_double),
+        ("XHILimit", C.c_double), ("collD", _dp), ("peD", _dp),
     ]
 
 
-def _dust_in(dt, nu, widflx, Ndust, dustOnly, XHILimit=0.05):
+def _dust_in(dt, nu, widflx, Ndust, dustOnly, XHILimit=0.05, collD=None, peD=None):
     keep = {}
 
     def d(name, a):
@@ -444,6 +466,9 @@ def _dust_in(dt, nu, widflx, Ndust, dustOnly, XHILimit=0.05):
                  tsub=d("t", dt.Tsub), Cabs=d("ca", dt.Cabs), Csca=d("cs", dt.Csca),
                  emInt=d("e", dt.emIntegral), nu=d("nu", nu), widFlx=d("wf", widflx),
                  Ndust=d("nd", Ndust), XHILimit=float(XHILimit))
+    if collD is not None:
+        inp.collD = d("cd", collD)
+        inp.peD = d("pd", peD)
     return inp, keep
 
 
@@ -492,9 +517,12 @@ def dust_pdf(dt, nu, widflx, Ndust, Tdust):
     return pdf.reshape(C1, B)
 
 
-def dust_temperature(dt, nu, widflx, Ndust, Tdust, J, dustOnly=True, XHILimit=0.05):
-    """getDustT: returns (Tdust, lgConverged, lgBlack, noHit, converged); J (C+1, B)."""
-    inp, keep = _dust_in(dt, nu, widflx, Ndust, dustOnly, XHILimit)
+def dust_temperature(dt, nu, widflx, Ndust, Tdust, J, dustOnly=True, XHILimit=0.05, collD=None,
+                     peD=None):
+    """getDustT: returns (Tdust, lgConverged, lgBlack, noHit, converged); J (C+1, B).
+    collD, peD (nS*nA, C+1): the grains' collision heating and photoelectric
+    cooling of a photoelectric gas + dust update."""
+    inp, keep = _dust_in(dt, nu, widflx, Ndust, dustOnly, XHILimit, collD, peD)
     C1 = np.asarray(Ndust).size
     td = np.ascontiguousarray(Tdust, dtype=np.float64).copy()
     j = np.ascontiguousarray(np.asarray(J).T, dtype=np.float64)

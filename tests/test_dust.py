@@ -163,6 +163,151 @@ def test_dust_only_transport_conserves_energy(oracle, shell):
             assert E[:, m.tables.sourceCDF[0] == 0].sum() == 0.0
 
 
+# ------------------------------------------- grain charge and photoelectric terms
+HCRYD = 2.1799153e-11
+PI_REF = 3.141592654
+ME = 9.109382e-28
+AMU = 1.66053e-24
+
+
+@pytest.fixture(scope="module")
+def pe_model():
+    return nebula.gas_dust_model("pe4", 4, 1.e18, 300., nebula.NEBULA_ELEMS,
+                                 10.0 ** np.array(nebula.NEBULA_LOG_ABUND), 7, 300, 1.001e-5, 25.0,
+                                 5.e4, 10 ** 1.5, 1000, ["sil-dlee", "grph1-dl"], [0.5, 0.5],
+                                 [0.01, 0.05, 0.2], [1.0, 0.3, 0.05], 0.01, photoelectric=True)
+
+
+def dilute_field(oracle, m, mask=None):
+    """4 pi W B_nu(50 kK) dnu per cell, W = 1e-15 x (1 .. 5); bins outside mask zero."""
+    nu, w = m.tables.nuArray, m.tables.widFlx
+    C1 = m.grid.nCells + 1
+    bb = 4 * np.pi * oracle.planck_h(nu, 5.e4) * H * FR1RYD * w
+    if mask is not None:
+        bb = np.where(mask, bb, 0.0)
+    J = np.zeros((C1, nu.size))
+    for c in range(1, C1):
+        J[c] = 1.e-15 * (1 + c % 5) * bb
+    return J
+
+
+def pe_update(oracle, m, J):
+    s0 = m.state
+    Td = np.full((m.grid.nCells + 1, m.dust.nSizes + 1, m.dust.nSpecies + 1), 50.0)
+    up = oracle.update(m.bt, m.elems, m.abund, m.nstages, True, J, s0["Te"], s0["Ne"], s0["Hden"],
+                       s0["ionDen"], nthreads=8, grains=(m.dust, Td, m.Ndust))
+    return up, Td
+
+
+def charge_rates(m, J, cell, g, V, Te, Ne):
+    """Photoemission and net recombination currents of grain g at potential V
+    (Baldwin et al. 1991 yields; electrons less the ions of the starting
+    state), evaluated from the definitions."""
+    dt, nu = m.dust, m.tables.nuArray
+    s, a = divmod(g, dt.nSizes)
+    Vn, r = dt.Vn[s], dt.radius[a]
+    on = nu > Vn + max(V, 0.0)
+    Yn = np.minimum(0.5 * (1 - Vn / nu), 0.2)
+    Yhat = Yn * np.clip(1 - V / (nu - Vn), 0.0, 1.0)
+    q = J[cell] / (HCRYD * nu) * 4 * PI_REF
+    emi = np.sum(np.where(on, Yhat * q * dt.Cabs[s, a] / (1e-8 * r * r), 0.0))
+    kT, e8 = 6.336e-6 * Te, 1.1045e-15 * Te / PI_REF
+    eta = lambda x: 1 - x if x <= 0 else np.exp(-x)
+    rec = Ne * np.sqrt(e8 / ME) * eta(-V / kT)
+    X0, H_ = m.state["ionDen"][cell], m.state["Hden"][cell]
+    for e, z in enumerate(m.elems):
+        for st in range(2, min(z + 1, m.nstages) + 1):
+            rec -= (X0[e, st - 1] * m.abund[e] * H_ * np.sqrt(e8 / (D.A_WEIGHT[z - 1] * AMU)) *
+                    eta((st - 1) * V / kT))
+    return emi, rec
+
+
+def test_grain_potential_balances_charging_currents(oracle, pe_model):
+    m = pe_model
+    J = dilute_field(oracle, m)
+    up, _ = pe_update(oracle, m, J)
+    nG = m.dust.nSpecies * m.dust.nSizes
+    checked = 0
+    for c in range(1, m.grid.nCells + 1):
+        if up["nIterT"][c] >= 20:        # Te moved after the last grain solve
+            continue
+        for g in range(nG):
+            V = up["grainPot"][g, c]
+            emi, rec = charge_rates(m, J, c, g, V, up["Te"][c], up["Ne"][c])
+            assert abs(emi - rec) < 0.005 * 0.5 * max(emi + rec, 1e-35), (c, g, V, emi, rec)
+            assert V > 0.0                 # a UV-lit grain charges positive
+            checked += 1
+    assert checked >= nG * m.grid.nCells // 2
+    # hot gas heats 50 K grains; photoelectrons cool them
+    assert np.all(up["collD"][:, 1:] > 0) and np.all(up["peD"][:, 1:] > 0)
+
+
+def test_grain_without_photoemission(oracle, pe_model):
+    """No photon above any work function: no photoelectric cooling of any
+    grain (the sum starts at the bin below the threshold, which holds no J
+    here).  Without ionising photons Te never balances, so the potential is
+    not checked against the currents (Te moves after the last grain solve)."""
+    m = pe_model
+    nu = m.tables.nuArray
+    J = dilute_field(oracle, m, nu < 0.5 * m.dust.Vn.min())
+    up, _ = pe_update(oracle, m, J)
+    assert np.all(up["peD"][:, 1:] == 0.0)
+    assert np.all(np.isfinite(up["grainPot"][:, 1:]))
+    assert np.all(up["collD"][:, 1:] != 0.0)
+
+
+def test_monochromatic_photoelectric_cooling(oracle, pe_model):
+    """One bin at nu0: peD = Cabs (J0 / h nu0) <E> / pi with the mean
+    photoelectron energy <E> = Y (nu0 - Vn)/2, cut where the grain potential
+    reflects the slower half (min(dn, (dn^2 - V^2)/dn))."""
+    m = pe_model
+    nu = m.tables.nuArray
+    i0 = int(np.searchsorted(nu, 1.5))
+    J = dilute_field(oracle, m, np.arange(nu.size) == i0)
+    up, _ = pe_update(oracle, m, J)
+    dt = m.dust
+    for c in range(1, m.grid.nCells + 1):
+        for g in range(dt.nSpecies * dt.nSizes):
+            s, a = divmod(g, dt.nSizes)
+            V, Vn, n0 = up["grainPot"][g, c], dt.Vn[s], nu[i0]
+            assert n0 > Vn + max(V, 0)
+            dn = n0 - Vn
+            EY = min(0.5 * (1 - Vn / n0), 0.2) * 0.5 * min(dn, max((dn * dn - V * V) / dn, 0.0))
+            want = dt.Cabs[s, a, i0] * J[c, i0] / (HCRYD * n0) * EY * HCRYD / PI_REF
+            assert up["peD"][g, c] == pytest.approx(want, rel=1e-12)
+            if up["nIterT"][c] < 20:
+                emi, rec = charge_rates(m, J, c, g, V, up["Te"][c], up["Ne"][c])
+                assert abs(emi - rec) < 0.005 * 0.5 * max(emi + rec, 1e-35)
+
+
+def test_recombination_energies_are_the_ionisation_edges(pe_model):
+    """The energy a recombining ion releases to a grain is its ionisation
+    potential: the mesh point at the H, He and He+ edges (1, 1.807, 4 Ryd)."""
+    ip = D.ip_ryd(pe_model.xsec)
+    assert ip[1 * 32 + 2] == pytest.approx(1.0, rel=2e-2)
+    assert ip[2 * 32 + 2] == pytest.approx(1.807, rel=2e-2)
+    assert ip[2 * 32 + 3] == pytest.approx(4.0, rel=2e-2)
+    assert ip[1 * 32 + 1] == 0.0 and ip[8 * 32 + 2] == pytest.approx(1.0, rel=5e-2)
+
+
+def test_dust_temperature_takes_grain_heating(oracle, pe_model):
+    """getDustT absorbs sum Cabs J/pi + collD - peD: equal terms cancel, net
+    collisional heating warms the grains."""
+    m = pe_model
+    nu, w = m.tables.nuArray, m.tables.widFlx
+    J = dilute_field(oracle, m)
+    Td0 = tdust_of(m)
+    nG, C1 = m.dust.nSpecies * m.dust.nSizes, m.grid.nCells + 1
+    base = oracle.dust_temperature(m.dust, nu, w, m.Ndust, Td0, J, dustOnly=False)[0]
+    x = np.full((nG, C1), 3.e-6)
+    same = oracle.dust_temperature(m.dust, nu, w, m.Ndust, Td0, J, dustOnly=False, collD=x,
+                                   peD=x)[0]
+    np.testing.assert_allclose(same[1:], base[1:], rtol=1e-10)
+    warm = oracle.dust_temperature(m.dust, nu, w, m.Ndust, Td0, J, dustOnly=False, collD=x,
+                                   peD=np.zeros_like(x))[0]
+    assert np.all(warm[1:, 1:, 1:] > base[1:, 1:, 1:])
+
+
 # ---------------------------------------------------------------- GPU parity
 @pytest.mark.gpu
 def test_gpu_dust_opacity_pdf_temperature_match_oracle(oracle):
@@ -310,5 +455,39 @@ def test_gpu_gas_dust_iteration_matches_oracle_chain(oracle):
     np.testing.assert_allclose(got["Te"][1:], up["Te"][1:], rtol=1e-9)
     np.testing.assert_allclose(got["ionDen"][1:], up["ionDen"][1:], rtol=1e-8, atol=1e-25)
     To = oracle.dust_temperature(m.dust, nu, w, m.Ndust, Td0, got["Jste"], dustOnly=False)[0]
+    np.testing.assert_allclose(got["Tdust"][1:], To[1:], rtol=1e-10)
+    assert st["convPercent"] == pytest.approx(100.0 * up["lgConverged"][1:].mean())
+
+
+@pytest.mark.gpu
+def test_gpu_photoelectric_gas_dust_iteration_matches_oracle_chain(oracle):
+    """Gas + grains with grain charging, photoelectric heating and gas-grain
+    collisions: the balance's grain terms (charge solve per grain inside every
+    T iteration) and the grain temperatures that take them, on the device's J."""
+    from mocassin_xray_amd import MocGPU
+    m = nebula.gas_dust_model("dustype6", 6, 1.e18, 300., nebula.NEBULA_ELEMS,
+                              10.0 ** np.array(nebula.NEBULA_LOG_ABUND), 7, 300, 1.001e-5, 25.0,
+                              5.e4, 10 ** 1.5, 20000, ["sil-dlee", "grph1-dl"], [0.5, 0.5],
+                              [0.01, 0.05, 0.2], [1.0, 0.3, 0.05], 0.01, photoelectric=True)
+    s0 = m.state
+    nu, w = m.tables.nuArray, m.tables.widFlx
+    print(f"case photoelectric gas+dust: cells={m.grid.nCells} bins={nu.size}", flush=True)
+    with MocGPU(m.tables, "fp64") as g:
+        m.load(g)
+        Td0 = g.download("Tdust")
+        st = g.iterateMC(1, 37)
+        got = {k: g.download(k) for k in ("Jste", "Te", "Ne", "ionDen", "Tdust", "grainColl",
+                                          "grainPE")}
+    E, Z, NS = m.elems, m.abund, m.nstages
+    up = oracle.update(m.bt, E, Z, NS, True, got["Jste"], s0["Te"], s0["Ne"], s0["Hden"],
+                       s0["ionDen"], nthreads=8, grains=(m.dust, Td0, m.Ndust))
+    np.testing.assert_allclose(got["Te"][1:], up["Te"][1:], rtol=1e-9)
+    np.testing.assert_allclose(got["Ne"][1:], up["Ne"][1:], rtol=1e-9)
+    np.testing.assert_allclose(got["ionDen"][1:], up["ionDen"][1:], rtol=1e-8, atol=1e-25)
+    np.testing.assert_allclose(got["grainColl"][:, 1:], up["collD"][:, 1:], rtol=1e-9)
+    np.testing.assert_allclose(got["grainPE"][:, 1:], up["peD"][:, 1:], rtol=1e-9)
+    assert np.all(got["grainPE"][:, 1:] > 0)
+    To = oracle.dust_temperature(m.dust, nu, w, m.Ndust, Td0, got["Jste"], dustOnly=False,
+                                 collD=up["collD"], peD=up["peD"])[0]
     np.testing.assert_allclose(got["Tdust"][1:], To[1:], rtol=1e-10)
     assert st["convPercent"] == pytest.approx(100.0 * up["lgConverged"][1:].mean())
