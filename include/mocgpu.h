@@ -278,7 +278,7 @@ typedef struct moc_dust_tables {
     const double* MsurfAtom;     /* [nSpecies] mass of a surface atom [amu] */
     const double* ipRyd;         /* [31*32] (Z, stage >= 2): the energy [Ryd] released when the
                                     stage recombines, nu of the first bin of the outer shell of
-                                    stage-1 (setDustGasCollHeatCool :814-823) */
+                                    stage-1 (setDustGasCollHeatCool :796-806) */
 } moc_dust_tables;
 
 /* Create a context: uploads the mesh and source tables. */

@@ -214,7 +214,7 @@ def ndust_from_mdmg(dt: DustTables, mdmg: float, hden: np.ndarray, elem_z, abund
 def ip_ryd(xsec) -> np.ndarray:
     """[31*32] (Z, stage): nu [Ryd] of the first bin of the outermost shell of
     stage - 1, the energy released when `stage` recombines (the IPerg of
-    setDustGasCollHeatCool, update_mod.f90:814-823); 0 where undefined."""
+    setDustGasCollHeatCool, update_mod.f90:796-806); 0 where undefined."""
     out = np.zeros(31 * 32)
     best = {}
     for sh in xsec.shells:

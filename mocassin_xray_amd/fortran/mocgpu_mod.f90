@@ -27,7 +27,9 @@ module mocgpu_mod
   integer(c_int), parameter, public :: MOC_LAYOUT_FORTRAN = 0, MOC_LAYOUT_CELLROW = 1
   integer(c_int), parameter, public :: MOC_F_OPACITY = 1, MOC_F_JSTE = 2, MOC_F_RECPDF = 3, &
        MOC_F_TOTALLINES = 4, MOC_F_NE = 5, MOC_F_TE = 6, MOC_F_HDEN = 7, MOC_F_ESCAPED = 8, &
-       MOC_F_IONDEN = 9, MOC_F_CONVERGED = 10, MOC_F_BLACK = 11
+       MOC_F_IONDEN = 9, MOC_F_CONVERGED = 10, MOC_F_BLACK = 11, MOC_F_EMISSION = 12, &
+       MOC_F_NDUST = 13, MOC_F_TDUST = 14, MOC_F_DUSTPDF = 15, MOC_F_ABSOPAC = 16, &
+       MOC_F_SCAOPAC = 17, MOC_F_GRAINCOLL = 18, MOC_F_GRAINPE = 19
 
   type, bind(C), public :: moc_tables
      integer(c_int32_t) :: nbins, precision
@@ -87,6 +89,14 @@ module mocgpu_mod
      type(c_ptr) :: hlMap, hlDens, hlLines, hbA, hbB, rbEdge, r2aEdge, heiRec, heiiLyman
      integer(c_int32_t) :: j2TsP, lymP(4)
   end type moc_emission_tables
+
+  ! grain tables (dustDriver); Tdust is (0:nSpecies, 0:nSizes, 0:nCells)
+  type, bind(C), public :: moc_dust_tables
+     integer(c_int32_t) :: nSpecies, nSizes, nTemps, lgGas, lgPhotoelectric
+     type(c_ptr) :: grainAbun, grainWeight, TdustSublime, Cabs, Csca, emIntegral
+     real(c_double) :: XHILimit
+     type(c_ptr) :: grainRadius, grainVn, MsurfAtom, ipRyd
+  end type moc_dust_tables
 
   type, bind(C), public :: moc_transport_stats
      real(c_double) :: Qphot, segments, emissions, absorbed, scattered, escaped, lines, lost, &
@@ -204,14 +214,36 @@ module mocgpu_mod
        type(c_ptr), value :: ctx
        integer(c_int), value :: ig
      end function moc_normalize_j
+     integer(c_int) function moc_set_dust(ctx, tables) bind(C, name="moc_set_dust")
+       import :: c_int, c_ptr, moc_dust_tables
+       type(c_ptr), value :: ctx
+       type(moc_dust_tables), intent(in) :: tables
+     end function moc_set_dust
+     integer(c_int) function moc_dust_opacity(ctx, ig) bind(C, name="moc_dust_opacity")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ig
+     end function moc_dust_opacity
+     integer(c_int) function moc_dust_emission(ctx, ig) bind(C, name="moc_dust_emission")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ig
+     end function moc_dust_emission
+     integer(c_int) function moc_dust_temperature(ctx, ig, stats) &
+          bind(C, name="moc_dust_temperature")
+       import :: c_int, c_ptr, moc_update_stats
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ig
+       type(moc_update_stats), intent(out) :: stats
+     end function moc_dust_temperature
   end interface
 
   public :: moc_create, moc_destroy, moc_last_error, moc_version, moc_grid_attach, moc_upload, &
        moc_download, moc_set_atomic, moc_opacity, moc_set_balance, moc_set_emission, &
        moc_emission, moc_set_packets, moc_iterate, moc_update, moc_reset_tallies, moc_transport, &
-       moc_normalize_j
+       moc_normalize_j, moc_set_dust, moc_dust_opacity, moc_dust_emission, moc_dust_temperature
   public :: moc_check, c_string, put_field, get_field, ionizationDriver, emissionDriver, &
-       energyPacketDriver, updateCell, MCIterationDriver
+       energyPacketDriver, updateCell, MCIterationDriver, dustDriver, getDustT
 
   ! real(8) arrays, or default-real ones as the reference's grid_type holds
   interface put_field
@@ -385,6 +417,23 @@ contains
     type(moc_update_stats), intent(out) :: stats
     call moc_check(ctx, moc_update(ctx, 1, stats), "updateCell")
   end subroutine updateCell
+
+  ! dustDriver (dust_mod.f90:14-186): the grain tables, built on the host
+  ! (cross-sections, weights, Kirchhoff integrals), go to the device; then
+  ! put_field(ctx, MOC_F_NDUST, grid(1)%Ndust).  The dust opacity, the dust
+  ! emission CDF and getDustT run inside moc_iterate and updateCell.
+  subroutine dustDriver(ctx, tables)
+    type(c_ptr), intent(in) :: ctx
+    type(moc_dust_tables), intent(in) :: tables
+    call moc_check(ctx, moc_set_dust(ctx, tables), "dustDriver")
+  end subroutine dustDriver
+
+  ! getDustT (update_mod.f90:3346-3453) over every cell on its own
+  subroutine getDustT(ctx, stats)
+    type(c_ptr), intent(in) :: ctx
+    type(moc_update_stats), intent(out) :: stats
+    call moc_check(ctx, moc_dust_temperature(ctx, 1, stats), "getDustT")
+  end subroutine getDustT
 
   ! The iterateMC loop (iteration_mod.f90:32-1497 without its output files):
   ! one device iteration per pass, the converged-cell percentage against
