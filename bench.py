@@ -36,6 +36,8 @@ def build_model(name: str):
         return nebula.gas_nebula(n=32, nbins=1000, n_packets=10_000_000)
     if name == "hsphere16":
         return nebula.gas_hsphere(n=16, nbins=400, n_packets=1_000_000)
+    if name == "dustneb64":
+        return nebula.gas_dust_nebula(n=64, nbins=700, n_packets=100_000_000)
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -86,8 +88,10 @@ def run_path(m, prec: str, steps: int, warmup: int):
             stats.append(g.iterateMC(s, SEED))
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        state = {k: g.download(k) for k in ("Te", "Ne", "ionDen", "Jste", "opacity", "recPDF",
-                                            "totalLines")}
+        keys = ["Te", "Ne", "ionDen", "Jste", "opacity", "recPDF", "totalLines"]
+        if getattr(m, "dust", None) is not None:
+            keys += ["Tdust", "scaOpac"]
+        state = {k: g.download(k) for k in keys}
     return dt, stats, state
 
 
@@ -141,25 +145,38 @@ def cpu_baseline(m, gs: dict, target_s: float = 20.0):
     scale = (C1 - 1) / (sub.size - 1)
     part = lambda a: np.ascontiguousarray(a[sub])
     H = m.state["Hden"]
+    dust = getattr(m, "dust", None)
+    nu, wf = m.tables.nuArray, m.tables.widFlx
     t0 = time.perf_counter()
-    O.opacity(m.xsec, m.tables.KNsigmaT, True, E, Z, NS, part(gs["ionDen"]), part(gs["Ne"]),
-              part(gs["Te"]), part(H))
+    gas = O.opacity(m.xsec, m.tables.KNsigmaT, True, E, Z, NS, part(gs["ionDen"]),
+                    part(gs["Ne"]), part(gs["Te"]), part(H))
+    if dust is not None:
+        O.dust_opacity(dust, nu, wf, part(m.Ndust), part(gs["Tdust"]), gasOpac=gas)
     t_op = (time.perf_counter() - t0) * scale
     t0 = time.perf_counter()
     O.emission(m.et, m.bt, E, Z, NS, part(gs["Te"]), part(gs["Ne"]), part(gs["ionDen"]),
-               nthreads=threads)
+               nthreads=threads,
+               dust=None if dust is None else (dust, part(gs["Tdust"]), part(m.Ndust), part(H)))
     t_em = (time.perf_counter() - t0) * scale
     t0 = time.perf_counter()
-    O.update(m.bt, E, Z, NS, True, part(gs["Jste"]), part(gs["Te"]), part(gs["Ne"]), part(H),
-             part(gs["ionDen"]), nthreads=threads)
+    grains = None
+    if dust is not None and getattr(m, "photoelectric", False):
+        grains = (dust, part(gs["Tdust"]), part(m.Ndust))
+    up = O.update(m.bt, E, Z, NS, True, part(gs["Jste"]), part(gs["Te"]), part(gs["Ne"]),
+                  part(H), part(gs["ionDen"]), nthreads=threads, grains=grains)
+    if dust is not None:
+        O.dust_temperature(dust, nu, wf, part(m.Ndust), part(gs["Tdust"]), part(gs["Jste"]),
+                           dustOnly=False, collD=up.get("collD"), peD=up.get("peD"))
     t_up = (time.perf_counter() - t0) * scale
     fields = {"opacity": gs["opacity"], "Ne": gs["Ne"], "recPDF": gs["recPDF"],
               "totalLines": gs["totalLines"]}
+    if dust is not None:
+        fields.update(scaOpac=gs["scaOpac"], dustOnly=False)
     n = 20000
     t0 = time.perf_counter()
     O.transport(m.tables, m.grid, fields, n, 5, nthreads=threads)
     dt = time.perf_counter() - t0
-    n2 = int(min(max(n * target_s / max(dt, 1e-3), n), 5_000_000))
+    n2 = int(min(max(n * target_s / max(dt, 1e-3), n), 5_000_000, m.n_packets))
     t0 = time.perf_counter()
     O.transport(m.tables, m.grid, fields, n2, 6, nthreads=threads)
     t_tr = (time.perf_counter() - t0) * m.n_packets / n2

@@ -8,7 +8,9 @@
 //               (compton :3834-3897).  They are one contraction
 //               R[cell, col] = sum_nu J[nu, cell] W[nu, col] with weight columns
 //               built once on the host; tiles of 64 cells x 32 columns stage J
-//               and W through LDS, J planes are read cell-fastest (coalesced).
+//               and W through LDS, J planes are read cell-fastest (coalesced),
+//               and R is written a row per cell (the cell's columns together,
+//               as the balance reads them).
 //  k_balance -- one wave per cell: the T_e root-find (iterateT :277-482)
 //               around the ion-fraction chain (ionBalance :1575-1992) and the
 //               heating/cooling integrals (thermBalance :887-1215) with the
@@ -37,9 +39,11 @@ constexpr int RC = 64;  // cells per tile
 constexpr int RK = 32;  // bins per LDS stage
 constexpr int RN = 32;  // columns per tile
 
-__global__ void __launch_bounds__(256) k_rates(const double* __restrict__ J, int C1, int B,
-                                               const double* __restrict__ W, int ncol,
-                                               double* __restrict__ R) {
+// cells cBase .. cBase + nc - 1 of the J planes (row stride ldJ) into
+// R[local cell][col]
+__global__ void __launch_bounds__(256) k_rates(const double* __restrict__ J, int ldJ, int cBase,
+                                               int nc, int B, const double* __restrict__ W,
+                                               int ncol, double* __restrict__ R) {
     __shared__ double sJ[RK][RC];
     __shared__ double sW[RK][RN];
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 4 waves x 8 columns
@@ -50,7 +54,7 @@ __global__ void __launch_bounds__(256) k_rates(const double* __restrict__ J, int
     for (int b0 = 0; b0 < B; b0 += RK) {
         for (int k = threadIdx.x; k < RK * RC; k += 256) {
             const int kb = k / RC, kc = k % RC, b = b0 + kb, c = c0 + kc;
-            sJ[kb][kc] = (b < B && c < C1) ? J[(size_t)b * C1 + c] : 0.0;
+            sJ[kb][kc] = (b < B && c < nc) ? J[(size_t)b * ldJ + cBase + c] : 0.0;
         }
         for (int k = threadIdx.x; k < RK * RN; k += 256) {
             const int kb = k / RN, kn = k % RN, b = b0 + kb, n = n0 + kn;
@@ -64,12 +68,15 @@ __global__ void __launch_bounds__(256) k_rates(const double* __restrict__ J, int
         }
         __syncthreads();
     }
-    const int c = c0 + tx;
-    if (c >= C1) return;
+    // the tile goes out through LDS so that each cell's 32 columns are one
+    // contiguous write
+    double* sOut = &sJ[0][0];  // [RC][RN]
 #pragma unroll
-    for (int q = 0; q < RN / 4; ++q) {
-        const int n = n0 + ty * (RN / 4) + q;
-        if (n < ncol) R[(size_t)n * C1 + c] = acc[q];
+    for (int q = 0; q < RN / 4; ++q) sOut[tx * RN + ty * (RN / 4) + q] = acc[q];
+    __syncthreads();
+    for (int k = threadIdx.x; k < RC * RN; k += 256) {
+        const int c = c0 + k / RN, n = n0 + k % RN;
+        if (c < nc && n < ncol) R[(size_t)c * ncol + n] = sOut[k];
     }
 }
 
@@ -89,19 +96,20 @@ __device__ __forceinline__ double t3(const double* a, int i, int z, int n) {
     return a[((size_t)i * 31 + z) * 31 + n];
 }
 
-// LDS rows of one cell, each [NE][S2] with S2 = nstages + 2 (stage index 0..NS+1)
+// LDS rows of one cell, each [NE][S2] with S2 = nstages + 2 (stage index 0..NS+1):
+// ion fractions, recombination and collisional-ionisation coefficients, and
+// the work row of the heavy-element chains
 struct Lds {
-    double *X, *alpha, *cI1, *cI2, *out, *in, *mat, *cool, *diO;
+    double *X, *alpha, *cI1, *out, *cool, *diO;
     int* hot;
     int S2;
     __device__ double& x(int e, int i) const { return X[e * S2 + i]; }
     __device__ double& al(int e, int i) const { return alpha[e * S2 + i]; }
     __device__ double& c1(int e, int i) const { return cI1[e * S2 + i]; }
-    __device__ double& c2(int e, int i) const { return cI2[e * S2 + i]; }
 };
 
 __host__ __device__ inline size_t balance_lds_bytes(int NE, int NS, int nNlv) {
-    const size_t rows = (size_t)7 * NE * (NS + 2);
+    const size_t rows = (size_t)4 * NE * (NS + 2);
     return (rows + (size_t)(nNlv > 0 ? nNlv : 1) + MOC_MAXS + 1) * sizeof(double) + 16;
 }
 
@@ -112,7 +120,7 @@ __host__ __device__ inline size_t balance_stage_bytes(const MocBalanceArgs& a) {
 }
 // staging is used while one workgroup's LDS stays within this
 constexpr size_t kMaxBalanceLds = 49152;
-// LDS ceiling of the kernels with grain terms (gfx950: 160 KiB per CU)
+// LDS ceiling of one balance workgroup (gfx950: 160 KiB per CU)
 constexpr size_t kMaxGrainLds = 65536;
 
 // dielectronic recombination of the ions with more than 11 electrons
@@ -197,7 +205,7 @@ __device__ void recomb(const MocBalanceArgs& a, const Lds& L, double T, int tid)
 __device__ void collion(const MocBalanceArgs& a, const Lds& L, double Te, int tid) {
     const int NE = a.NE, NS = a.NS, S2 = L.S2;
     const double te = Te * 8.617385e-05;
-    for (int k = tid; k < NE * S2; k += 64) { L.cI1[k] = 0.0; L.cI2[k] = 0.0; }
+    for (int k = tid; k < NE * S2; k += 64) L.cI1[k] = 0.0;
     if (tid == 0) *L.hot = NE * S2;
     __syncthreads();
     for (int k = tid; k < NE * S2; k += 64) {
@@ -213,12 +221,11 @@ __device__ void collion(const MocBalanceArgs& a, const Lds& L, double Te, int ti
         const double c = t3(a.cion_cf, 2, z, el) * (1.0 + t3(a.cion_cf, 1, z, el) * sqrt(u)) /
                          (t3(a.cion_cf, 3, z, el) + u) * pow(u, t3(a.cion_cf, 4, z, el)) * exp(-u);
         L.cI1[k] = c;
-        L.cI2[k] = a.cion_th[z * 32 + ion] * c * kBoltz;
     }
     __syncthreads();
     const int hot = *L.hot;
     for (int k = tid; k < NE * S2; k += 64)
-        if (k >= hot) { L.cI1[k] = 0.0; L.cI2[k] = 0.0; }
+        if (k >= hot) L.cI1[k] = 0.0;
     __syncthreads();
 }
 
@@ -249,15 +256,15 @@ __device__ __forceinline__ double dEk(int z, int ion) {
 // the tables in HBM.
 template <bool ST>
 struct Rates {
-    const double* R;
-    int C1, cell;
+    const double* R;       // the cell's row of rates (staged: in LDS)
+    int ld, cell;
     const int* specOf;     // [NE*32] (slot, stage) -> species or -1
     const int* specShell;  // [S+1]
     const int* shInfo;     // [nShells] Auger count | (1 + reverse charge-exchange slot) << 8
     const int* shAug;      // [nShells] first auger[] entry of the shell
     __device__ double at(int col) const {
         if constexpr (ST) return R[col];
-        else return R[(size_t)col * C1 + cell];
+        else return R[(size_t)cell * ld + col];
     }
     __device__ double gamma(int s) const { return at(3 * s); }
     __device__ double heat(int s, double heatef) const { return at(3 * s + 1) + heatef * at(3 * s + 2); }
@@ -308,10 +315,8 @@ template <class RT>
 __device__ void heavy_chain(const MocBalanceArgs& a, const RT& r, const Lds& L, int e, double Te,
                             double Ne, double H, int nIterX, const double revFrozen[2]) {
     const int NS = a.NS, z = a.elemZ[e];
-    double* out = L.out + e * L.S2;
-    double* in = L.in + e * L.S2;
-    double* mat = L.mat + e * L.S2;
-    for (int i = 0; i < L.S2; ++i) { out[i] = 0.; in[i] = 0.; }
+    double* out = L.out + e * L.S2;  // rates out of each stage, then the chain products
+    for (int i = 0; i < L.S2; ++i) out[i] = 0.;
     const int top = imin(NS, z + 1);
     for (int ion = 1; ion <= imin(z, NS - 1); ++ion) {
         out[ion] += L.c1(e, ion) * Ne;
@@ -331,15 +336,19 @@ __device__ void heavy_chain(const MocBalanceArgs& a, const RT& r, const Lds& L, 
             }
         }
     }
+    // mat(ion) = mat(ion-1) out(ion-1) / in(ion-1) with mat(1) = 1 and
+    // in = Ne alpha + chex H0 H; mat(ion-1) replaces out(ion-1) once used
     const int eH = a.elemSlot[1];
-    for (int ion = 1; ion <= imin(z, NS - 1); ++ion)
-        in[ion] = Ne * L.al(e, ion) + chex(a, z, ion, Te, false) * H * L.x(eH, 1);
-    mat[1] = 1.;
-    double sum = 1.;
+    double m = 1., sum = 1.;
     for (int ion = 2; ion <= top; ++ion) {
-        mat[ion] = mat[ion - 1] * out[ion - 1] / in[ion - 1];
-        sum += mat[ion];
+        const double in = Ne * L.al(e, ion - 1) + chex(a, z, ion - 1, Te, false) * H * L.x(eH, 1);
+        const double mn = m * out[ion - 1] / in;
+        out[ion - 1] = m;
+        m = mn;
+        sum += mn;
     }
+    out[top] = m;
+    const double* mat = out;
     sum = 1. / sum;
     L.x(e, top) = mat[top] * sum;
     for (int ion = 1; ion <= imin(z, NS - 1); ++ion) {
@@ -638,7 +647,10 @@ __device__ void therm(const MocBalanceArgs& a, const RT& r, const Lds& L, double
                          : 0.;
     for (int e = 0; e < a.NE; ++e) {
         const int z = a.elemZ[e];
-        for (int ion = 1; ion <= imin(z, a.NS - 1); ++ion) coolCollH += a.abund[e] * L.x(e, ion) * L.c2(e, ion) * Ne;
+        for (int ion = 1; ion <= imin(z, a.NS - 1); ++ion) {
+            const double c2 = a.cion_th[z * 32 + ion] * L.c1(e, ion) * kBoltz;  // ionisation cooling
+            coolCollH += a.abund[e] * L.x(e, ion) * c2 * Ne;
+        }
     }
     for (int k = 0; k < a.nNlv; ++k) coolColl += L.cool[k];
     for (int sp = 0; sp < a.nSpec; ++sp) {
@@ -673,11 +685,11 @@ template <int NL, bool ST, bool PE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 k_balance(MocBalanceArgs a) {
     extern __shared__ double lds[];
-    const int cell = blockIdx.x + 1;
+    const int cell = a.cell0 + blockIdx.x;
     const int tid = threadIdx.x;
     const int C1 = a.nCells + 1;
     const int NE = a.NE, NS = a.NS, S2 = NS + 2;
-    Rates<ST> r{a.R, C1, cell, a.specOf, a.specShell, a.shInfo, a.shAug};
+    Rates<ST> r{a.R, a.ncol, (int)blockIdx.x, a.specOf, a.specShell, a.shInfo, a.shAug};
     if constexpr (ST) {
         // stage the cell's rate columns and the index tables after the state rows
         double* Rl = lds + balance_lds_bytes(NE, NS, a.nNlv) / sizeof(double);
@@ -686,7 +698,7 @@ k_balance(MocBalanceArgs a) {
         int* sSh = sOf + NE * 32;
         int* sIn = sSh + a.nSpec + 1;
         int* sAu = sIn + a.nShells;
-        for (int k = tid; k < a.ncol; k += 64) Rl[k] = a.R[(size_t)k * C1 + cell];
+        for (int k = tid; k < a.ncol; k += 64) Rl[k] = a.R[(size_t)blockIdx.x * a.ncol + k];
         for (int k = tid; k < NE * 32; k += 64) sOf[k] = a.specOf[k];
         for (int k = tid; k <= a.nSpec; k += 64) sSh[k] = a.specShell[k];
         for (int k = tid; k < a.nShells; k += 64) { sIn[k] = a.shInfo[k]; sAu[k] = a.shAug[k]; }
@@ -710,11 +722,8 @@ k_balance(MocBalanceArgs a) {
     L.X = lds;
     L.alpha = L.X + NE * S2;
     L.cI1 = L.alpha + NE * S2;
-    L.cI2 = L.cI1 + NE * S2;
-    L.out = L.cI2 + NE * S2;
-    L.in = L.out + NE * S2;
-    L.mat = L.in + NE * S2;
-    L.cool = L.mat + NE * S2;
+    L.out = L.cI1 + NE * S2;
+    L.cool = L.out + NE * S2;
     L.diO = L.cool + (a.nNlv > 0 ? a.nNlv : 1);
     L.hot = (int*)(L.diO + MOC_MAXS + 1);
     for (int k = tid; k < NE * S2; k += 64) {
@@ -831,42 +840,57 @@ k_balance(MocBalanceArgs a) {
     }
 }
 
-}  // namespace
-
-void moc_launch_rates(moc_ctx* ctx, const double* J, int C1, const double* W, int ncol, double* R) {
-    dim3 g1((unsigned)((C1 + RC - 1) / RC), (unsigned)((ncol + RN - 1) / RN));
-    hipLaunchKernelGGL(k_rates, g1, dim3(256), 0, ctx->stream, J, C1, ctx->nbins, W, ncol, R);
+// the n-level register solve is sized by the largest level count in use
+template <bool PE>
+void launch_balance(moc_ctx* ctx, const MocBalanceArgs& a, bool st, dim3 grid, dim3 block, size_t lds) {
+    if (a.maxLev <= 5) {
+        if (st) hipLaunchKernelGGL((k_balance<5, true, PE>), grid, block, lds, ctx->stream, a);
+        else hipLaunchKernelGGL((k_balance<5, false, PE>), grid, block, lds, ctx->stream, a);
+    } else if (a.maxLev <= 8) {
+        if (st) hipLaunchKernelGGL((k_balance<8, true, PE>), grid, block, lds, ctx->stream, a);
+        else hipLaunchKernelGGL((k_balance<8, false, PE>), grid, block, lds, ctx->stream, a);
+    } else {
+        if (st) hipLaunchKernelGGL((k_balance<MOC_MAXLEV, true, PE>), grid, block, lds, ctx->stream, a);
+        else hipLaunchKernelGGL((k_balance<MOC_MAXLEV, false, PE>), grid, block, lds, ctx->stream, a);
+    }
 }
 
-int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a, const double* J, const double* W,
-                       int ncol, double* R) {
+}  // namespace
+
+void launch_rates(moc_ctx* ctx, const double* J, int ldJ, int c0, int nc, const double* W, int ncol,
+                  double* R) {
+    dim3 g1((unsigned)((nc + RC - 1) / RC), (unsigned)((ncol + RN - 1) / RN));
+    hipLaunchKernelGGL(k_rates, g1, dim3(256), 0, ctx->stream, J, ldJ, c0, nc, ctx->nbins, W, ncol, R);
+}
+
+void moc_launch_rates(moc_ctx* ctx, const double* J, int C1, const double* W, int ncol, double* R) {
+    launch_rates(ctx, J, C1, 0, C1, W, ncol, R);
+}
+
+int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a0, const double* J, const double* W,
+                       int ncol, double* R, int chunk) {
+    MocBalanceArgs a = a0;
     const int C1 = a.nCells + 1;
-    moc_launch_rates(ctx, J, C1, W, ncol, R);
     const size_t base = balance_lds_bytes(a.NE, a.NS, a.nNlv);
     const size_t staged = base + balance_stage_bytes(a);
     const bool st = staged <= kMaxBalanceLds;
     const bool pe = a.nGS > 0 && a.nGA > 0;
     const size_t lds = (st ? staged : base) + (pe ? grain_lds_bytes(a.B, a.NE, a.NS, a.nGS * a.nGA) : 0);
-    const dim3 grid((unsigned)a.nCells), block(64);
-    if (pe && lds > kMaxGrainLds) {
-        ctx->err = "balance: grain tables need " + std::to_string(lds) + " bytes of LDS per cell (limit " +
+    if (lds > kMaxGrainLds) {
+        ctx->err = "balance: a cell needs " + std::to_string(lds) + " bytes of LDS (limit " +
                    std::to_string(kMaxGrainLds) + ")";
         return MOC_ERR_ARG;
     }
-    if (a.nCells > 0) {
-        if (pe) {
-            if (st) hipLaunchKernelGGL((k_balance<MOC_MAXLEV, true, true>), grid, block, lds, ctx->stream, a);
-            else hipLaunchKernelGGL((k_balance<MOC_MAXLEV, false, true>), grid, block, lds, ctx->stream, a);
-        } else if (a.maxLev <= 5) {
-            if (st) hipLaunchKernelGGL((k_balance<5, true, false>), grid, block, lds, ctx->stream, a);
-            else hipLaunchKernelGGL((k_balance<5, false, false>), grid, block, lds, ctx->stream, a);
-        } else if (a.maxLev <= 8) {
-            if (st) hipLaunchKernelGGL((k_balance<8, true, false>), grid, block, lds, ctx->stream, a);
-            else hipLaunchKernelGGL((k_balance<8, false, false>), grid, block, lds, ctx->stream, a);
-        } else {
-            if (st) hipLaunchKernelGGL((k_balance<MOC_MAXLEV, true, false>), grid, block, lds, ctx->stream, a);
-            else hipLaunchKernelGGL((k_balance<MOC_MAXLEV, false, false>), grid, block, lds, ctx->stream, a);
-        }
+    if (chunk < 1) { ctx->err = "balance: no rate buffer"; return MOC_ERR_STATE; }
+    a.R = R;
+    for (int c0 = 1; c0 <= a.nCells; c0 += chunk) {
+        const int n = a.nCells - c0 + 1 < chunk ? a.nCells - c0 + 1 : chunk;
+        launch_rates(ctx, J, C1, c0, n, W, ncol, R);
+        a.cell0 = c0;
+        a.nLaunch = n;
+        const dim3 grid((unsigned)n), block(64);
+        if (pe) launch_balance<true>(ctx, a, st, grid, block, lds);
+        else launch_balance<false>(ctx, a, st, grid, block, lds);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

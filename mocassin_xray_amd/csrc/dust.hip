@@ -6,8 +6,11 @@
 //                    the (species, size) pairs below their sublimation
 //                    temperature, in the reference's species-then-size order,
 //                    and opacity = gas kappa + (scaOpac + absOpac).  One lane
-//                    per (bin, cell), cells fastest, so the planes are written
-//                    in coalesced rows.
+//                    per cell walks the bins: the cell's grain temperatures
+//                    are read once into a sublimation mask, the cross-sections
+//                    of a bin are the same for the whole wave (scalar loads),
+//                    and each bin's planes are written in coalesced rows.
+//                    More than 64 grains: one lane per (bin, cell).
 //  k_dust_pdf     -- setDustPDF (emission_mod.f90:1921-1986), dust-only runs:
 //                    one workgroup per cell; each lane forms whole bins
 //                    sum Cabs B_nu(Tdust)/h widFlx grainWeight grainAbun over
@@ -17,7 +20,8 @@
 //  k_dust_temp    -- getDustT (update_mod.f90:3346-3453): the absorption
 //                    integrals sum_nu Cabs J/pi of every grain come from the
 //                    k_rates contraction with columns Cabs/pi (and a column
-//                    of ones for the hit test); one lane per cell then
+//                    of ones for the hit test), a row per cell; one lane per
+//                    cell then
 //                    inverts the Kirchhoff table (locate + linear
 //                    interpolation in T) and forms the size- and
 //                    abundance-weighted means.
@@ -33,6 +37,40 @@ constexpr double kPiRef = 3.141592654;  // constants_mod.f90:45
 
 __device__ __forceinline__ size_t tdust_at(const MocDustArgs& a, int cell, int s, int ai) {
     return ((size_t)cell * (a.nA + 1) + ai) * (a.nS + 1) + s;
+}
+
+__global__ void __launch_bounds__(kThreads) k_dust_opacity_cells(MocDustArgs a) {
+    const int C1 = a.nCells + 1;
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= C1) return;
+    const int B = a.B, nA = a.nA, nS = a.nS;
+    if (cell == 0) {
+        if (a.dustOnly)
+            for (int b = 0; b < B; ++b) a.opacity[(size_t)b * C1] = 0.;
+        return;
+    }
+    uint64_t live = 0;  // grains below their sublimation temperature
+    for (int s = 1; s <= nS; ++s)
+        for (int ai = 1; ai <= nA; ++ai)
+            if (a.Tdust[tdust_at(a, cell, s, ai)] < a.tsub[s - 1])
+                live |= 1ull << ((s - 1) * nA + ai - 1);
+    const double nd = a.Ndust[cell];
+    for (int b = 0; b < B; ++b) {
+        double sca = 0., ab = 0.;
+        for (int g = 0; g < nS * nA; ++g) {
+            if (!((live >> g) & 1ull)) continue;
+            const int s = g / nA, ai = g - s * nA;
+            const size_t k = (size_t)g * B + b;
+            const double f = a.abun[s] * a.weight[ai] * nd;
+            sca = sca + f * a.Csca[k];
+            ab = ab + f * a.Cabs[k];
+        }
+        const size_t t = (size_t)b * C1 + cell;
+        a.scaOpac[t] = sca;
+        a.absOpac[t] = ab;
+        const double base = a.gasOpac ? a.gasOpac[t] : 0.;
+        a.opacity[t] = base + (sca + ab);
+    }
 }
 
 __global__ void __launch_bounds__(kThreads) k_dust_opacity(MocDustArgs a) {
@@ -131,7 +169,8 @@ __global__ void __launch_bounds__(kThreads) k_dust_temp(MocDustArgs a) {
     const int cell = blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (cell > a.nCells) return;
     const int C1 = a.nCells + 1, nG = a.nS * a.nA;
-    if (!(a.R[(size_t)nG * C1 + cell] > 0.)) {  // no packet reached the cell: Tdust kept
+    const double* Rc = a.R + (size_t)cell * (nG + 1);  // the cell's absorption integrals, hit last
+    if (!(Rc[nG] > 0.)) {  // no packet reached the cell: Tdust kept
         if (a.dustOnly) {
             a.lgBlack[cell] = 1;
             a.lgConverged[cell] = 0;
@@ -145,7 +184,7 @@ __global__ void __launch_bounds__(kThreads) k_dust_temp(MocDustArgs a) {
     for (int s = 1; s <= a.nS; ++s) {
         for (int ai = 1; ai <= a.nA; ++ai) {
             const int g = (s - 1) * a.nA + (ai - 1);
-            double x = a.R[(size_t)g * C1 + cell];
+            double x = Rc[g];
             if (a.collD)  // gas-grain collisions and photoelectric cooling (:3390-3393)
                 x = x + a.collD[(size_t)g * C1 + cell] - a.peD[(size_t)g * C1 + cell];
             const double* E = a.emInt + (size_t)g * a.nT;
@@ -179,6 +218,11 @@ int check(moc_ctx* ctx, const char* what) {
 }  // namespace
 
 int moc_launch_dust_opacity(moc_ctx* ctx, const MocDustArgs& a) {
+    if (a.nS * a.nA <= 64) {
+        const unsigned blocks = (unsigned)((a.nCells + 1 + kThreads - 1) / kThreads);
+        hipLaunchKernelGGL(k_dust_opacity_cells, dim3(blocks), dim3(kThreads), 0, ctx->stream, a);
+        return check(ctx, "dust opacity launch");
+    }
     const int64_t n = (int64_t)(a.nCells + 1) * a.B;
     int64_t blocks = (n + kThreads - 1) / kThreads;
     if (blocks > 65536) blocks = 65536;

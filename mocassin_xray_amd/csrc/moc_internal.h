@@ -162,7 +162,8 @@ struct MocBalanceArgs {
     const double* Ndust;      // [C1]
     double* collD;            // [nGS*nGA][C1] out: gas-grain collision heating of the grains
     double* peD;              // [nGS*nGA][C1] out: photoelectric cooling of the grains
-    const double* R;          // [ncol][C1] rates
+    const double* R;          // [nLaunch][ncol] rates of cells cell0 .. cell0 + nLaunch - 1
+    int cell0, nLaunch;       // the cells of one launch (a chunk of the grid)
     double* Te;
     double* Ne;
     const double* Hden;
@@ -267,7 +268,7 @@ struct MocDust {
     int lgPE = 0;               // grain charge, photoelectric heating, gas-grain collisions
     double *d_radius = nullptr, *d_vn = nullptr, *d_msurf = nullptr, *d_ip = nullptr;
     double* d_W = nullptr;      // [B][nS*nA+1] Cabs/pi and the hit column
-    double* d_R = nullptr;      // [nS*nA+1][C+1]
+    double* d_R = nullptr;      // [C+1][nS*nA+1] absorption integrals, a row per cell
     unsigned long long* d_counters = nullptr;  // [2]
 };
 
@@ -289,7 +290,7 @@ struct MocDustArgs {
     const double* gasOpac;  // [B][C+1] or null (dust only)
     double* opacity;        // [B][C+1]
     double* pdf;            // [C+1][B] dust-only emission CDF
-    const double* R;        // [nS*nA+1][C+1] absorption integrals, last row the hit test
+    const double* R;        // [C+1][nS*nA+1] absorption integrals, last column the hit test
     const double* collD;    // [nS*nA][C+1] gas-grain collision heating, or null
     const double* peD;      // [nS*nA][C+1] photoelectric cooling, or null
     int* lgConverged;
@@ -351,8 +352,9 @@ struct MocGrid {
     int* d_conv = nullptr;           // [C+1] lgConverged
     int* d_black = nullptr;          // [C+1] lgBlack
     int* d_nIterT = nullptr;         // [C+1]
-    double* d_rates = nullptr;       // [ncol][C+1]
+    double* d_rates = nullptr;       // [rates_chunk][ncol]: the rates of one chunk of cells
     int rates_cols = 0;
+    int rates_chunk = 0;             // cells per balance launch (bounds the rate buffer)
 };
 
 struct MocBalance {
@@ -410,5 +412,6 @@ int moc_launch_dust_temp(moc_ctx* ctx, const MocDustArgs& a, const double* J, co
                          double* R);
 // R[col][cell] = sum_nu J[nu][cell] W[nu][col] (k_rates, balance.hip)
 void moc_launch_rates(moc_ctx* ctx, const double* J, int C1, const double* W, int ncol, double* R);
+// the balance over every cell, in chunks of `chunk` cells whose rates fit R ([ncol][chunk])
 int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a, const double* J, const double* W,
-                       int ncol, double* R);
+                       int ncol, double* R, int chunk);

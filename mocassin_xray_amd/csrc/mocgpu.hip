@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -134,6 +135,20 @@ int bal_put(moc_ctx* ctx, const T* h, size_t n, const T** out, const char* what)
     return MOC_OK;
 }
 
+// the rate buffer of the balance holds at most this many bytes (environment
+// MOCGPU_RATES_BYTES overrides it): larger grids (many cells times many
+// shells) run the rates and the balance in chunks
+constexpr size_t kRatesBytes = (size_t)8 << 30;
+
+size_t rates_budget() {
+    const char* s = getenv("MOCGPU_RATES_BYTES");
+    if (s && *s) {
+        const long long v = atoll(s);
+        if (v > 0) return (size_t)v;
+    }
+    return kRatesBytes;
+}
+
 // per-grid balance buffers (flags, iteration counts, rate matrix)
 int ensure_balance_grid(moc_ctx* ctx) {
     MocGrid& g = ctx->grid;
@@ -145,9 +160,15 @@ int ensure_balance_grid(moc_ctx* ctx) {
                       (rc = dalloc_n(ctx, &g.d_nIterT, C1, "nIterT"))))
         return rc;
     if (g.rates_cols != ctx->balance.ncol) {
+        // the rates of a chunk of cells at a time, within kRatesBytes
+        const size_t ncol = (size_t)ctx->balance.ncol;
+        size_t chunk = rates_budget() / (8 * ncol);
+        if (chunk < 64) chunk = 64;
+        if (chunk > (size_t)g.nCells) chunk = g.nCells > 0 ? (size_t)g.nCells : 1;
         dfree(g.d_rates);
-        if ((rc = dalloc_n(ctx, &g.d_rates, C1 * ctx->balance.ncol, "rates"))) return rc;
+        if ((rc = dalloc_n(ctx, &g.d_rates, chunk * ncol, "rates"))) return rc;
         g.rates_cols = ctx->balance.ncol;
+        g.rates_chunk = (int)chunk;
     }
     MOC_TRY(hipMemsetAsync(g.d_conv, 0, C1 * 4, ctx->stream), "zero flags");
     MOC_TRY(hipMemsetAsync(g.d_black, 0, C1 * 4, ctx->stream), "zero flags");
@@ -928,7 +949,7 @@ int moc_update(moc_ctx* ctx, int ig, moc_update_stats* stats) {
     MOC_TRY(hipMemcpyAsync(g.d_TeOld, g.d_Te, C1b, hipMemcpyDeviceToDevice, ctx->stream), "TeOld");
     MOC_TRY(hipMemcpyAsync(g.d_NeOld, g.d_Ne, C1b, hipMemcpyDeviceToDevice, ctx->stream), "NeOld");
     MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
-    if ((rc = moc_launch_balance(ctx, a, g.d_J, W, b.ncol, g.d_rates))) return rc;
+    if ((rc = moc_launch_balance(ctx, a, g.d_J, W, b.ncol, g.d_rates, g.rates_chunk))) return rc;
     if (ctx->dust.set) {  // getDustT after iterateT (update_mod.f90:224-227)
         MocDust& d = ctx->dust;
         if ((rc = moc_launch_dust_temp(ctx, dust_args(ctx), g.d_J, d.d_W, d.nS * d.nA + 1, d.d_R)))

@@ -314,15 +314,77 @@ def lognormal_density(n: int, mean: float = 300.0, sigma_ln: float = 1.0,
     return rng.lognormal(np.log(mean) - 0.5 * sigma_ln ** 2, sigma_ln, size=(n, n, n))
 
 
-def gas_dust_nebula(n: int = 64, nbins: int = 700, n_packets: int = 10_000_000) -> GasDustModel:
+def gas_dust_nebula(n: int = 64, nbins: int = 700, n_packets: int = 10_000_000,
+                    photoelectric: bool = True) -> GasDustModel:
     """Dusty nebula, one octant: n^3 with log-normal n_H (mean 300, sigma_ln 1),
     edge 1e18 cm, the seven elements of the gas nebula, dust-to-gas mass ratio
     0.01 of sil-dlee and grph1-dl (0.5 each) in 10 MRN sizes 0.005-0.25 um,
-    50 kK blackbody of 10^37.5 erg/s, a gas mesh over 1e-5 - 25 Ryd."""
+    50 kK blackbody of 10^37.5 erg/s, a gas mesh over 1e-5 - 25 Ryd.  Grain
+    charging and photoelectric heating are on, as in the reference's default
+    (lgPhotoelectric, common_mod.f90:424)."""
     a, w = _mrn()
     return gas_dust_model("dustneb%d" % n, n, 1.e18, lognormal_density(n), NEBULA_ELEMS,
                           10.0 ** np.array(NEBULA_LOG_ABUND), 7, nbins, 1.001e-5, 25.0, 5.e4,
-                          10 ** 1.5, n_packets, ["sil-dlee", "grph1-dl"], [0.5, 0.5], a, w, 0.01)
+                          10 ** 1.5, n_packets, ["sil-dlee", "grph1-dl"], [0.5, 0.5], a, w, 0.01,
+                          photoelectric=photoelectric)
+
+
+# ------------------------------------------------------------- X-ray nebula
+# Solar abundances H-Zn, log eps (H = 12), Anders & Grevesse (1989, Geochim.
+# Cosmochim. Acta 53, 197), the photospheric values with meteoritic ones for
+# Li, Be, B where the photosphere is depleted.
+SOLAR_LOG_EPS = [12.00, 10.99, 3.31, 1.42, 2.88, 8.56, 8.05, 8.93, 4.48, 8.09, 6.31, 7.58, 6.48,
+                 7.55, 5.57, 7.27, 5.27, 6.56, 5.13, 6.34, 3.09, 4.93, 4.02, 5.68, 5.53, 7.51,
+                 4.91, 6.25, 4.29, 4.67]
+
+
+def clumpy_density(n: int, fill: float = 0.1, n_clump: float = 1000.0, n_inter: float = 10.0,
+                   seed: int = 20261203 + 5) -> np.ndarray:
+    """n^3 cells: a fraction `fill` of them (drawn without replacement) at
+    n_clump, the rest at n_inter."""
+    rng = np.random.default_rng(seed)
+    hd = np.full(n ** 3, n_inter)
+    hd[rng.choice(n ** 3, size=int(round(fill * n ** 3)), replace=False)] = n_clump
+    return hd.reshape(n, n, n)
+
+
+def xray_source(nu: np.ndarray, T_bb: float = 1.e5, pl_lo: float = 0.01, pl_hi: float = 10.0,
+                pl_fraction: float = 0.5) -> np.ndarray:
+    """F_nu of a power law F_nu ~ nu^-1 between pl_lo and pl_hi keV plus a
+    blackbody of T_bb, the power law carrying pl_fraction of the luminosity
+    (a tabulated source spectrum, contShape file)."""
+    ryd_kev = 13.6057e-3
+    wid = T.widths_of(nu)
+    pl = np.where((nu >= pl_lo / ryd_kev) & (nu <= pl_hi / ryd_kev), 1.0 / nu, 0.0)
+    bb = T.blackbody_flux(nu, T_bb)
+    return (pl_fraction * pl / np.sum(pl * wid) + (1.0 - pl_fraction) * bb / np.sum(bb * wid))
+
+
+def xray_nebula(n: int = 128, nbins: int = 2000, n_packets: int = 1_000_000_000,
+                R: float = 3.e18, L36: float = 10 ** 1.5, maxLev: int = 5) -> GasModel:
+    """Clumpy X-ray-illuminated nebula, one octant: n^3 cells, 10 % of them at
+    n_H = 1000 and the rest at 10, edge R; all 30 elements H-Zn at solar
+    abundances with every ionisation stage (31), inner-shell photoionisation
+    and Auger yields; a source of F_nu ~ nu^-1 over 0.01-10 keV plus a 1e5 K
+    blackbody in equal luminosity; the gas mesh of nbins points over
+    1.001e-5 - 735 Ryd with every ion edge resolved (edges that coincide
+    leave a few points fewer)."""
+    from . import atomic, balance, emission
+    elems = list(range(1, 31))
+    abund = 10.0 ** (np.array(SOLAR_LOG_EPS) - 12.0)
+    nstages = 31
+    nu_max = 735.0
+    ax = T.symmetric_axes(n, R)
+    act, ncells, hcell = T.active_map(ax, ax, ax, np.ascontiguousarray(clumpy_density(n)))
+    grid = CartesianGrid(ax, ax.copy(), ax.copy(), act, ncells, True, 0.0)
+    nu = T.reference_gas_mesh(nbins, 1.001e-5, nu_max, atomic.ion_edges(elems, nstages, nu_max),
+                              strict=False)
+    tabs = source_tables(nu, 1.e5, L36, n_packets, True, n, n, n, True, flux=xray_source(nu))
+    xs = atomic.build_xsec(nu, elems, nstages)
+    bt = balance.build_balance(xs, maxLev=maxLev)
+    et = emission.build_emission(xs, tabs.widFlx)
+    st = initial_state(elems, nstages, hcell)
+    return GasModel("xray%d" % n, tabs, grid, elems, abund, nstages, xs, bt, et, st, n_packets)
 
 
 def _mrn():

@@ -39,14 +39,15 @@ def log_mesh(nbins: int, nu_min: float, nu_max: float) -> np.ndarray:
 
 
 def reference_gas_mesh(nbins: int, nu_min: float, nu_max: float,
-                       ion_edges: np.ndarray) -> np.ndarray:
+                       ion_edges: np.ndarray, strict: bool = True) -> np.ndarray:
     """The gas-only log mesh of initCartesianGrid (grid_mod.f90:251-375).
 
     Each H-series edge and each ionisation edge below nuMax adds the triplet
     (e - 0.0003, e, e + 0.0003); edges below nuMin pull the log part of the
     mesh down to (edge - 0.0006); the rest of the nbins points are log-spaced
     by repeated multiplication; the result is sorted ascending (sortUp keeps
-    distinct values).
+    distinct values).  Where edges coincide the mesh has fewer than nbins
+    distinct points: an error when strict, else the shorter mesh.
     """
     pts = []
     nu_lo, nu_hi = nu_min, nu_max
@@ -70,7 +71,7 @@ def reference_gas_mesh(nbins: int, nu_min: float, nu_max: float,
     for _ in range(i_count - 1):
         logs.append(10.0 ** (np.log10(logs[-1]) + step))
     mesh = np.unique(np.array(pts + logs, dtype=np.float64))
-    if mesh.size != nbins:
+    if strict and mesh.size != nbins:
         raise ValueError(f"mesh has {mesh.size} distinct points, expected {nbins}")
     return mesh
 

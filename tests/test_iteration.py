@@ -132,3 +132,33 @@ def test_hydrogen_sphere_converges_to_stromgren_radius():
     assert 8000.0 < Tm < 20000.0          # no metal coolants: a hot pure-H nebula
     assert (Vion / Vs) ** (1 / 3) == pytest.approx(1.0, abs=0.10)
     assert hist[-1]["convPercent"] > hist[0]["convPercent"]
+
+
+@pytest.mark.gpu
+def test_balance_in_cell_chunks_matches_one_launch(monkeypatch):
+    """A rate buffer smaller than the grid runs the rates and the balance in
+    chunks of cells (here 100 of 512, the last one short): on the same J the
+    state is bit for bit that of one launch."""
+    from mocassin_xray_amd import MocGPU
+    m = nebula.gas_model("nebula8", 8, 1.e18, 1000., nebula.NEBULA_ELEMS,
+                         10.0 ** np.array(nebula.NEBULA_LOG_ABUND), 7, 300, 100.0, 1.e5,
+                         10 ** 1.5, 5000)
+    monkeypatch.delenv("MOCGPU_RATES_BYTES", raising=False)
+    print("case balance chunks: J of one iteration", flush=True)
+    with MocGPU(m.tables, "fp64") as g:
+        m.load(g)
+        g.iterateMC(1, 99)
+        J = g.download("Jste")
+    out = []
+    for budget in (None, 100 * 8 * m.bt.W.shape[1]):
+        if budget is not None:
+            monkeypatch.setenv("MOCGPU_RATES_BYTES", str(budget))
+        print(f"case balance chunks: budget={budget}", flush=True)
+        with MocGPU(m.tables, "fp64") as g:
+            m.load(g)
+            g.upload("Jste", J)
+            st = g.updateCell()
+            out.append(({k: g.download(k) for k in ("Te", "Ne", "ionDen")}, st))
+    for k in ("Te", "Ne", "ionDen"):
+        np.testing.assert_array_equal(out[1][0][k], out[0][0][k])
+    assert out[1][1]["converged"] == out[0][1]["converged"]
