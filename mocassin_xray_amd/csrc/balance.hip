@@ -96,21 +96,26 @@ __device__ __forceinline__ double t3(const double* a, int i, int z, int n) {
     return a[((size_t)i * 31 + z) * 31 + n];
 }
 
-// LDS rows of one cell, each [NE][S2] with S2 = nstages + 2 (stage index 0..NS+1):
-// ion fractions, recombination and collisional-ionisation coefficients, and
-// the work row of the heavy-element chains
+// LDS rows of one cell: ion fractions, recombination and collisional-
+// ionisation coefficients, and the work row of the heavy-element chains.
+// Each row packs the elements one after the other, element e holding stages
+// 0 .. min(Z+1, NS) from off[e].  Loops that spread (element, stage) over the
+// lanes run over k = e * S2 + stage (S2 = NS + 2) and skip stages past the end.
 struct Lds {
     double *X, *alpha, *cI1, *out, *cool, *diO;
     int* hot;
+    int* off;  // [NE + 1]
     int S2;
-    __device__ double& x(int e, int i) const { return X[e * S2 + i]; }
-    __device__ double& al(int e, int i) const { return alpha[e * S2 + i]; }
-    __device__ double& c1(int e, int i) const { return cI1[e * S2 + i]; }
+    __device__ int len(int e) const { return off[e + 1] - off[e]; }
+    __device__ double& x(int e, int i) const { return X[off[e] + i]; }
+    __device__ double& al(int e, int i) const { return alpha[off[e] + i]; }
+    __device__ double& c1(int e, int i) const { return cI1[off[e] + i]; }
 };
 
-__host__ __device__ inline size_t balance_lds_bytes(int NE, int NS, int nNlv) {
-    const size_t rows = (size_t)4 * NE * (NS + 2);
-    return (rows + (size_t)(nNlv > 0 ? nNlv : 1) + MOC_MAXS + 1) * sizeof(double) + 16;
+__host__ __device__ inline size_t balance_lds_bytes(int nX, int nNlv) {
+    const size_t rows = (size_t)4 * nX;
+    return (rows + (size_t)(nNlv > 0 ? nNlv : 1) + MOC_MAXS + 1) * sizeof(double) +
+           (MOC_MAXE + 2) * sizeof(int);
 }
 
 // the staged rate columns and index tables that follow the state rows
@@ -152,7 +157,7 @@ __device__ double direc(const MocBalanceArgs& a, int z, int ion, double T) {
 __device__ void recomb(const MocBalanceArgs& a, const Lds& L, double T, int tid) {
     const int NE = a.NE, NS = a.NS, S2 = L.S2;
     const int eO = a.elemSlot[8];
-    for (int k = tid; k < NE * S2; k += 64) L.alpha[k] = 0.0;
+    for (int k = tid; k < a.nX; k += 64) L.alpha[k] = 0.0;
     for (int i = tid; i <= MOC_MAXS; i += 64)
         L.diO[i] = (eO >= 0 && i >= 1 && i < 7 && i <= NS) ? direc(a, 8, i, T) : 0.0;
     __syncthreads();
@@ -205,7 +210,7 @@ __device__ void recomb(const MocBalanceArgs& a, const Lds& L, double T, int tid)
 __device__ void collion(const MocBalanceArgs& a, const Lds& L, double Te, int tid) {
     const int NE = a.NE, NS = a.NS, S2 = L.S2;
     const double te = Te * 8.617385e-05;
-    for (int k = tid; k < NE * S2; k += 64) L.cI1[k] = 0.0;
+    for (int k = tid; k < a.nX; k += 64) L.cI1[k] = 0.0;
     if (tid == 0) *L.hot = NE * S2;
     __syncthreads();
     for (int k = tid; k < NE * S2; k += 64) {
@@ -220,12 +225,14 @@ __device__ void collion(const MocBalanceArgs& a, const Lds& L, double Te, int ti
         }
         const double c = t3(a.cion_cf, 2, z, el) * (1.0 + t3(a.cion_cf, 1, z, el) * sqrt(u)) /
                          (t3(a.cion_cf, 3, z, el) + u) * pow(u, t3(a.cion_cf, 4, z, el)) * exp(-u);
-        L.cI1[k] = c;
+        L.c1(e, ion) = c;
     }
     __syncthreads();
     const int hot = *L.hot;
-    for (int k = tid; k < NE * S2; k += 64)
-        if (k >= hot) L.cI1[k] = 0.0;
+    for (int k = tid; k < NE * S2; k += 64) {
+        const int e = k / S2, ion = k % S2;
+        if (k >= hot && ion < L.len(e)) L.c1(e, ion) = 0.0;
+    }
     __syncthreads();
 }
 
@@ -315,8 +322,8 @@ template <class RT>
 __device__ void heavy_chain(const MocBalanceArgs& a, const RT& r, const Lds& L, int e, double Te,
                             double Ne, double H, int nIterX, const double revFrozen[2]) {
     const int NS = a.NS, z = a.elemZ[e];
-    double* out = L.out + e * L.S2;  // rates out of each stage, then the chain products
-    for (int i = 0; i < L.S2; ++i) out[i] = 0.;
+    double* out = L.out + L.off[e];  // rates out of each stage, then the chain products
+    for (int i = 0; i < L.len(e); ++i) out[i] = 0.;
     const int top = imin(NS, z + 1);
     for (int ion = 1; ion <= imin(z, NS - 1); ++ion) {
         out[ion] += L.c1(e, ion) * Ne;
@@ -429,8 +436,8 @@ struct GrainLds {
     double *q, *X0, *Vg, *pe, *coll, *pg, *cg;
 };
 
-__host__ __device__ inline size_t grain_lds_bytes(int B, int NE, int NS, int nG) {
-    return ((size_t)B + (size_t)NE * (NS + 2) + 5 * (size_t)(nG > 0 ? nG : 1)) * sizeof(double);
+__host__ __device__ inline size_t grain_lds_bytes(int B, int nX, int nG) {
+    return ((size_t)B + (size_t)nX + 5 * (size_t)(nG > 0 ? nG : 1)) * sizeof(double);
 }
 
 // locate (interpolation_mod.f90:49-106) on the increasing mesh, 1-based
@@ -479,7 +486,7 @@ __device__ double g_recombination(const MocBalanceArgs& a, const Lds& L, const G
     for (int e = 0; e < a.NE; ++e) {
         const int z = a.elemZ[e];
         for (int istage = 2; istage <= imin(z + 1, a.NS); ++istage) {
-            const double cpDen = G.X0[e * L.S2 + istage] * a.abund[e] * H;
+            const double cpDen = G.X0[L.off[e] + istage] * a.abund[e] * H;
             const double mcp = c_aweight[z - 1] * kAmu;
             vmean = sqrt(e8 / mcp);
             double et = (double)(istage - 1) * Vg / kT;
@@ -692,7 +699,7 @@ k_balance(MocBalanceArgs a) {
     Rates<ST> r{a.R, a.ncol, (int)blockIdx.x, a.specOf, a.specShell, a.shInfo, a.shAug};
     if constexpr (ST) {
         // stage the cell's rate columns and the index tables after the state rows
-        double* Rl = lds + balance_lds_bytes(NE, NS, a.nNlv) / sizeof(double);
+        double* Rl = lds + balance_lds_bytes(a.nX, a.nNlv) / sizeof(double);
         int* iL = reinterpret_cast<int*>(Rl + a.ncol);
         int* sOf = iL;
         int* sSh = sOf + NE * 32;
@@ -720,30 +727,35 @@ k_balance(MocBalanceArgs a) {
     Lds L;
     L.S2 = S2;
     L.X = lds;
-    L.alpha = L.X + NE * S2;
-    L.cI1 = L.alpha + NE * S2;
-    L.out = L.cI1 + NE * S2;
-    L.cool = L.out + NE * S2;
+    L.alpha = L.X + a.nX;
+    L.cI1 = L.alpha + a.nX;
+    L.out = L.cI1 + a.nX;
+    L.cool = L.out + a.nX;
     L.diO = L.cool + (a.nNlv > 0 ? a.nNlv : 1);
     L.hot = (int*)(L.diO + MOC_MAXS + 1);
+    L.off = L.hot + 1;
+    if (tid <= NE) L.off[tid] = a.xoff[tid];
+    __syncthreads();
     for (int k = tid; k < NE * S2; k += 64) {
         const int e = k / S2, ion = k % S2;
-        L.X[k] = (ion >= 1 && ion <= NS) ? a.ionDen[((size_t)(ion - 1) * NE + e) * C1 + cell] : 0.0;
+        if (ion < L.len(e))
+            L.x(e, ion) = (ion >= 1 && ion <= NS) ? a.ionDen[((size_t)(ion - 1) * NE + e) * C1 + cell] : 0.0;
     }
     const int nG = PE ? a.nGS * a.nGA : 0;
     GrainLds G{};
     if constexpr (PE) {
-        size_t off = balance_lds_bytes(NE, NS, a.nNlv);
+        size_t off = balance_lds_bytes(a.nX, a.nNlv);
         if constexpr (ST) off += balance_stage_bytes(a);
         G.q = lds + off / sizeof(double);
         G.X0 = G.q + a.B;
-        G.Vg = G.X0 + NE * S2;
+        G.Vg = G.X0 + a.nX;
         G.pe = G.Vg + nG;
         G.coll = G.pe + nG;
         G.pg = G.coll + nG;
         G.cg = G.pg + nG;
         for (int b = tid; b < a.B; b += 64) G.q[b] = a.J[(size_t)b * C1 + cell] / (kHcRyd * a.gNu[b]);
-        for (int k = tid; k < NE * S2; k += 64) G.X0[k] = L.X[k];
+        __syncthreads();  // X was loaded by other lanes
+        for (int k = tid; k < a.nX; k += 64) G.X0[k] = L.X[k];
     }
     __syncthreads();
     double Te = a.Te[cell], Ne = a.Ne[cell];
@@ -821,7 +833,8 @@ k_balance(MocBalanceArgs a) {
     }
     for (int k = tid; k < NE * S2; k += 64) {
         const int e = k / S2, ion = k % S2;
-        if (ion >= 1 && ion <= NS) a.ionDen[((size_t)(ion - 1) * NE + e) * C1 + cell] = L.X[k];
+        if (ion >= 1 && ion <= NS && ion < L.len(e))
+            a.ionDen[((size_t)(ion - 1) * NE + e) * C1 + cell] = L.x(e, ion);
     }
     if constexpr (PE) {  // the grain terms of the last T iteration, for getDustT
         for (int g = tid; g < nG; g += 64) {
@@ -871,11 +884,11 @@ int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a0, const double* J, 
                        int ncol, double* R, int chunk) {
     MocBalanceArgs a = a0;
     const int C1 = a.nCells + 1;
-    const size_t base = balance_lds_bytes(a.NE, a.NS, a.nNlv);
+    const size_t base = balance_lds_bytes(a.nX, a.nNlv);
     const size_t staged = base + balance_stage_bytes(a);
     const bool st = staged <= kMaxBalanceLds;
     const bool pe = a.nGS > 0 && a.nGA > 0;
-    const size_t lds = (st ? staged : base) + (pe ? grain_lds_bytes(a.B, a.NE, a.NS, a.nGS * a.nGA) : 0);
+    const size_t lds = (st ? staged : base) + (pe ? grain_lds_bytes(a.B, a.nX, a.nGS * a.nGA) : 0);
     if (lds > kMaxGrainLds) {
         ctx->err = "balance: a cell needs " + std::to_string(lds) + " bytes of LDS (limit " +
                    std::to_string(kMaxGrainLds) + ")";

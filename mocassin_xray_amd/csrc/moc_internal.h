@@ -147,6 +147,9 @@ struct MocBalanceArgs {
     const int* nlvAtom;
     int colHit, colComp;      // rate columns of the hit test and the Compton sums
     int ncol;                 // rate columns per cell
+    int nX;                   // entries of a packed per-cell stage row (sum over elements
+                              // of min(Z+1, NS) + 1)
+    int xoff[MOC_MAXE + 1];   // start of each element in a packed row
     // grains with photoelectric heating and gas-grain collisions (nGS = 0: none)
     int nGS, nGA, B;
     const double* gAbun;      // [nGS]

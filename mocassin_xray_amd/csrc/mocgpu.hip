@@ -896,6 +896,12 @@ int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* t) {
     a.specIon = at.d_specIon;
     a.NE = at.nElemUsed;
     a.NS = at.nstages;
+    a.nX = 0;  // packed stage rows: stages 0 .. min(Z+1, NS) of each element
+    for (int e = 0; e < at.nElemUsed; ++e) {
+        a.xoff[e] = a.nX;
+        a.nX += std::min(at.elemZ[e] + 1, at.nstages) + 1;
+    }
+    a.xoff[at.nElemUsed] = a.nX;
     a.nSpec = S;
     a.nShells = t->nShells;
     a.nDiel = t->nDiel;
