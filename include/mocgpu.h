@@ -290,6 +290,21 @@ const char* moc_last_error(const moc_ctx* ctx);
  * and zeroes them. */
 int moc_grid_attach(moc_ctx* ctx, int ig, const moc_grid_desc* grid);
 
+/* A batch of independent grids in one context (no counterpart in the
+ * reference, which runs one nebula per process; `multiGrids` there is nested
+ * refinement, not batching): nGrids Cartesian grids that share the frequency
+ * mesh, the element set and every table.  Grid k (0-based) is lit by star k+1
+ * of moc_tables, whose position and cell indices are in grid k's own frame.
+ * The grids' cells are numbered one grid after the other -- cell c of grid k
+ * is cell cellBase[k] + c, cellBase[k] the sum of nCells of grids 0..k-1 --
+ * and every per-cell field (MOC_F_*) is over these numbers.  The grids must
+ * share lgSymmetricXYZ.  moc_iterate then transports every grid's packets in
+ * one launch (packet i of grid k draws from the Philox stream of packet id
+ * (k << 32) + i, counter base nIterateMC * 256 + 1) and runs opacity,
+ * emission and balance once over all cells; moc_transport(iStar) transports
+ * star iStar's packets in grid iStar-1.  Grains are not supported in a batch. */
+int moc_grid_attach_batch(moc_ctx* ctx, int nGrids, const moc_grid_desc* grids);
+
 /* Copy a per-cell field between host and device.  `count` is the number of
  * elements of the host buffer; (cell, bin) fields take `layout`. */
 int moc_upload(moc_ctx* ctx, int field, int ig, const void* host, int64_t count,

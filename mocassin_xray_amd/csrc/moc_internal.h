@@ -27,6 +27,20 @@ enum {
     MOC_CNT_N
 };
 
+// One grid of a batch as the transport sees it: its axes and volume factors,
+// its active map (global cell ids; null when every cell is active and the id
+// is base + 1 + iz + nz*(iy + ny*ix)) and its star.
+template <typename T>
+struct MocGridView {
+    const T *ax, *ay, *az, *vx, *vy, *vz;
+    const int* active;
+    const T* sourceCDF;   // [B] of the grid's star
+    int nx, ny, nz, base, sym;
+    int starIdx[3];       // 0-based
+    T R_out, deltaE;
+    T star[3];
+};
+
 // Transport view of a grid in arithmetic type T (see transport.hip).
 template <typename T>
 struct MocTransportArgs {
@@ -70,6 +84,11 @@ struct MocTransportArgs {
     int64_t n;
     uint64_t seed;
     uint32_t ctr_base;
+    // batched grids: packet ids gridFirst[k] .. gridFirst[k+1]-1 belong to grid k
+    // (RNG packet id (k << 32) + local id); nGrids = 0 for a single grid
+    int nGrids;
+    const MocGridView<T>* grids;  // [nGrids]
+    const int64_t* gridFirst;     // [nGrids + 1]
 };
 
 // Opacity kernel view (opacity.hip)
@@ -316,6 +335,16 @@ __host__ __device__ inline double moc_planck_h(double nu, double T) {
 struct MocGrid {
     bool attached = false;
     int nx = 0, ny = 0, nz = 0, nCells = 0, sym = 0;
+    // a batch of independent grids (nGrids > 1): per-grid sizes, first cell,
+    // offsets into the concatenated axes and active maps; grid k is lit by
+    // star k
+    int nGrids = 1;
+    std::vector<int> gnx, gny, gnz, gbase, gdense;
+    std::vector<size_t> goffx, goffy, goffz, goffa;
+    std::vector<double> gR_out;
+    MocGridView<double>* d_view64 = nullptr;
+    MocGridView<float>* d_view32 = nullptr;
+    int64_t* d_gridFirst = nullptr;
     bool dense = false;   // every cell active, numbered z fastest, then y, then x
     double R_out = 0.0;
     std::vector<double> ax, ay, az;   // host copies

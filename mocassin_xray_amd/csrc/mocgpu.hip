@@ -52,6 +52,7 @@ int h2d(moc_ctx* ctx, void* d, const void* h, size_t bytes, const char* what) {
 }
 
 void free_grid(MocGrid& g) {
+    dfree(g.d_view64); dfree(g.d_view32); dfree(g.d_gridFirst);
     dfree(g.d_ax); dfree(g.d_ay); dfree(g.d_az);
     dfree(g.d_vx); dfree(g.d_vy); dfree(g.d_vz);
     dfree(g.d_active);
@@ -272,6 +273,10 @@ int refresh_f32(moc_ctx* ctx) {
     return MOC_OK;
 }
 
+int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who);
+int run_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ctr_base,
+                  moc_transport_stats* stats);
+
 }  // namespace
 
 extern "C" {
@@ -367,54 +372,118 @@ void moc_destroy(moc_ctx* ctx) {
 
 int moc_grid_attach(moc_ctx* ctx, int ig, const moc_grid_desc* gd) {
     if (!ctx) return MOC_ERR_ARG;
-    if (ig != 1 || !gd || gd->nx < 2 || gd->ny < 2 || gd->nz < 2 || !gd->xAxis || !gd->yAxis ||
-        !gd->zAxis || !gd->active || gd->nCells < 1) {
+    if (ig != 1) {
         ctx->err = "moc_grid_attach: invalid grid (one grid, ig=1, n>=2 per axis)";
         return MOC_ERR_ARG;
     }
-    free_grid(ctx->grid);
-    free_dust(ctx->dust);  // its per-cell arrays belonged to the old grid
-    MocGrid& g = ctx->grid;
-    g.nx = gd->nx; g.ny = gd->ny; g.nz = gd->nz;
-    g.nCells = gd->nCells;
-    g.sym = gd->lgSymmetricXYZ ? 1 : 0;
-    g.R_out = gd->R_out;
-    g.ax.assign(gd->xAxis, gd->xAxis + g.nx);
-    g.ay.assign(gd->yAxis, gd->yAxis + g.ny);
-    g.az.assign(gd->zAxis, gd->zAxis + g.nz);
-    const size_t ncell3 = (size_t)g.nx * g.ny * g.nz;
-    g.active.assign(gd->active, gd->active + ncell3);
-    // every cell active and numbered with z fastest, then y, then x: the id is
-    // computed instead of looked up in the transport
-    g.dense = true;
-    for (int i = 0; i < g.nx && g.dense; ++i)
-        for (int j = 0; j < g.ny && g.dense; ++j)
-            for (int k = 0; k < g.nz; ++k)
-                if (g.active[(size_t)i + (size_t)g.nx * ((size_t)j + (size_t)g.ny * k)] !=
-                    1 + k + g.nz * (j + g.ny * i)) {
-                    g.dense = false;
-                    break;
-                }
-    for (size_t i = 0; i < ncell3; ++i)
-        if (g.active[i] < 0 || g.active[i] > g.nCells) {
-            ctx->err = "moc_grid_attach: active ids must lie in [0, nCells] (no sub-grids)";
+    return attach_grids(ctx, 1, gd, "moc_grid_attach");
+}
+
+int moc_grid_attach_batch(moc_ctx* ctx, int nGrids, const moc_grid_desc* grids) {
+    if (!ctx) return MOC_ERR_ARG;
+    if (nGrids < 1 || !grids) { ctx->err = "moc_grid_attach_batch: nGrids >= 1 grids"; return MOC_ERR_ARG; }
+    return attach_grids(ctx, nGrids, grids, "moc_grid_attach_batch");
+}
+
+}  // extern "C"
+
+namespace {
+
+// Attach nG grids; more than one is a batch of independent grids whose cells
+// are numbered one grid after the other and whose axes and active maps are
+// concatenated (active ids become global).
+int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who) {
+    const std::string w(who);
+    for (int k = 0; k < nG; ++k) {
+        const moc_grid_desc* gd = gds + k;
+        if (gd->nx < 2 || gd->ny < 2 || gd->nz < 2 || !gd->xAxis || !gd->yAxis || !gd->zAxis ||
+            !gd->active || gd->nCells < 1) {
+            ctx->err = w + ": invalid grid " + std::to_string(k) + " (n>=2 per axis, nCells>=1)";
             return MOC_ERR_ARG;
         }
-    for (int s = 0; s < ctx->nStars; ++s) {
-        const int* si = &ctx->starIdx[3 * s];
-        if (si[0] < 1 || si[0] > g.nx || si[1] < 1 || si[1] > g.ny || si[2] < 1 || si[2] > g.nz) {
-            ctx->err = "moc_grid_attach: star cell index outside the grid";
+        if ((gd->lgSymmetricXYZ ? 1 : 0) != (gds->lgSymmetricXYZ ? 1 : 0)) {
+            ctx->err = w + ": the grids of a batch share lgSymmetricXYZ";
             return MOC_ERR_ARG;
         }
     }
-    const std::vector<double> vx = widths(g.ax, g.sym), vy = widths(g.ay, g.sym),
-                              vz = widths(g.az, g.sym);
+    if (nG > 1 && ctx->nStars != nG) {
+        ctx->err = w + ": a batch of " + std::to_string(nG) + " grids needs one star per grid (" +
+                   std::to_string(ctx->nStars) + " stars)";
+        return MOC_ERR_ARG;
+    }
+    int64_t total = 0;
+    for (int k = 0; k < nG; ++k) total += gds[k].nCells;
+    if (total > 2000000000LL) { ctx->err = w + ": more than 2e9 cells"; return MOC_ERR_ARG; }
+    free_grid(ctx->grid);
+    free_dust(ctx->dust);  // its per-cell arrays belonged to the old grid
+    MocGrid& g = ctx->grid;
+    g.nGrids = nG;
+    g.nx = gds->nx; g.ny = gds->ny; g.nz = gds->nz;
+    g.nCells = (int)total;
+    g.sym = gds->lgSymmetricXYZ ? 1 : 0;
+    g.R_out = gds->R_out;
+    std::vector<double> vx, vy, vz;
+    int base = 0;
+    for (int k = 0; k < nG; ++k) {
+        const moc_grid_desc* gd = gds + k;
+        const int nx = gd->nx, ny = gd->ny, nz = gd->nz;
+        g.gnx.push_back(nx); g.gny.push_back(ny); g.gnz.push_back(nz);
+        g.gbase.push_back(base);
+        g.gR_out.push_back(gd->R_out);
+        g.goffx.push_back(g.ax.size()); g.goffy.push_back(g.ay.size());
+        g.goffz.push_back(g.az.size()); g.goffa.push_back(g.active.size());
+        const std::vector<double> ax(gd->xAxis, gd->xAxis + nx), ay(gd->yAxis, gd->yAxis + ny),
+            az(gd->zAxis, gd->zAxis + nz);
+        g.ax.insert(g.ax.end(), ax.begin(), ax.end());
+        g.ay.insert(g.ay.end(), ay.begin(), ay.end());
+        g.az.insert(g.az.end(), az.begin(), az.end());
+        const std::vector<double> wx = widths(ax, g.sym), wy = widths(ay, g.sym), wz = widths(az, g.sym);
+        vx.insert(vx.end(), wx.begin(), wx.end());
+        vy.insert(vy.end(), wy.begin(), wy.end());
+        vz.insert(vz.end(), wz.begin(), wz.end());
+        // every cell active and numbered with z fastest, then y, then x: the id
+        // is computed instead of looked up in the transport
+        const size_t n3 = (size_t)nx * ny * nz;
+        bool dense = true;
+        for (size_t i = 0; i < n3; ++i) {
+            const int id = gd->active[i];
+            if (id < 0 || id > gd->nCells) {
+                ctx->err = w + ": active ids must lie in [0, nCells] (no sub-grids)";
+                free_grid(g);
+                return MOC_ERR_ARG;
+            }
+            const size_t ix = i % nx, iy = (i / nx) % ny, iz = i / ((size_t)nx * ny);
+            if (id != 1 + (int)(iz + nz * (iy + ny * ix))) dense = false;
+            g.active.push_back(id > 0 ? base + id : 0);
+        }
+        g.gdense.push_back(dense ? 1 : 0);
+        if (nG > 1) {  // star k lights grid k
+            const int* si = &ctx->starIdx[3 * k];
+            if (si[0] < 1 || si[0] > nx || si[1] < 1 || si[1] > ny || si[2] < 1 || si[2] > nz) {
+                ctx->err = w + ": star cell index outside its grid " + std::to_string(k);
+                free_grid(g);
+                return MOC_ERR_ARG;
+            }
+        }
+        base += gd->nCells;
+    }
+    g.dense = nG == 1 && g.gdense[0];
+    if (nG == 1)
+        for (int s = 0; s < ctx->nStars; ++s) {
+            const int* si = &ctx->starIdx[3 * s];
+            if (si[0] < 1 || si[0] > g.nx || si[1] < 1 || si[1] > g.ny || si[2] < 1 || si[2] > g.nz) {
+                ctx->err = w + ": star cell index outside the grid";
+                free_grid(g);
+                return MOC_ERR_ARG;
+            }
+        }
     const size_t C1 = (size_t)g.nCells + 1, B = (size_t)ctx->nbins;
+    const size_t NX = g.ax.size(), NY = g.ay.size(), NZ = g.az.size(), NA = g.active.size();
     int rc;
-    if ((rc = dalloc_n(ctx, &g.d_ax, g.nx, "ax")) || (rc = dalloc_n(ctx, &g.d_ay, g.ny, "ay")) ||
-        (rc = dalloc_n(ctx, &g.d_az, g.nz, "az")) || (rc = dalloc_n(ctx, &g.d_vx, g.nx, "vx")) ||
-        (rc = dalloc_n(ctx, &g.d_vy, g.ny, "vy")) || (rc = dalloc_n(ctx, &g.d_vz, g.nz, "vz")) ||
-        (rc = dalloc_n(ctx, &g.d_active, ncell3, "active")) ||
+    if ((rc = dalloc_n(ctx, &g.d_ax, NX, "ax")) || (rc = dalloc_n(ctx, &g.d_ay, NY, "ay")) ||
+        (rc = dalloc_n(ctx, &g.d_az, NZ, "az")) || (rc = dalloc_n(ctx, &g.d_vx, NX, "vx")) ||
+        (rc = dalloc_n(ctx, &g.d_vy, NY, "vy")) || (rc = dalloc_n(ctx, &g.d_vz, NZ, "vz")) ||
+        (rc = dalloc_n(ctx, &g.d_active, NA, "active")) ||
         (rc = dalloc_n(ctx, &g.d_opacity, C1 * B, "opacity")) ||
         (rc = dalloc_n(ctx, &g.d_Ne, C1, "Ne")) || (rc = dalloc_n(ctx, &g.d_Te, C1, "Te")) ||
         (rc = dalloc_n(ctx, &g.d_Hden, C1, "Hden")) ||
@@ -423,23 +492,26 @@ int moc_grid_attach(moc_ctx* ctx, int ig, const moc_grid_desc* gd) {
         (rc = dalloc_n(ctx, &g.d_totalLines, C1, "totalLines")) ||
         (rc = dalloc_n(ctx, &g.d_J, C1 * B, "J")) ||
         (rc = dalloc_n(ctx, &g.d_escaped, C1 * B, "escaped")) ||
-        (rc = dalloc_n(ctx, &g.f_ax, g.nx, "ax32")) || (rc = dalloc_n(ctx, &g.f_ay, g.ny, "ay32")) ||
-        (rc = dalloc_n(ctx, &g.f_az, g.nz, "az32")) || (rc = dalloc_n(ctx, &g.f_vx, g.nx, "vx32")) ||
-        (rc = dalloc_n(ctx, &g.f_vy, g.ny, "vy32")) || (rc = dalloc_n(ctx, &g.f_vz, g.nz, "vz32")) ||
+        (rc = dalloc_n(ctx, &g.f_ax, NX, "ax32")) || (rc = dalloc_n(ctx, &g.f_ay, NY, "ay32")) ||
+        (rc = dalloc_n(ctx, &g.f_az, NZ, "az32")) || (rc = dalloc_n(ctx, &g.f_vx, NX, "vx32")) ||
+        (rc = dalloc_n(ctx, &g.f_vy, NY, "vy32")) || (rc = dalloc_n(ctx, &g.f_vz, NZ, "vz32")) ||
         (rc = dalloc_n(ctx, &g.f_opacity, C1 * B, "opacity32")) ||
         (rc = dalloc_n(ctx, &g.f_Ne, C1, "Ne32")) ||
         (rc = dalloc_n(ctx, &g.f_recPDF, C1 * B, "recPDF32")) ||
-        (rc = dalloc_n(ctx, &g.f_totalLines, C1, "totalLines32"))) {
+        (rc = dalloc_n(ctx, &g.f_totalLines, C1, "totalLines32")) ||
+        (nG > 1 && ((rc = dalloc_n(ctx, &g.d_view64, (size_t)nG, "grid views")) ||
+                    (rc = dalloc_n(ctx, &g.d_view32, (size_t)nG, "grid views")) ||
+                    (rc = dalloc_n(ctx, &g.d_gridFirst, (size_t)nG + 1, "grid packets"))))) {
         free_grid(g);
         return rc;
     }
-    if ((rc = h2d(ctx, g.d_ax, g.ax.data(), g.nx * 8, "ax")) ||
-        (rc = h2d(ctx, g.d_ay, g.ay.data(), g.ny * 8, "ay")) ||
-        (rc = h2d(ctx, g.d_az, g.az.data(), g.nz * 8, "az")) ||
-        (rc = h2d(ctx, g.d_vx, vx.data(), g.nx * 8, "vx")) ||
-        (rc = h2d(ctx, g.d_vy, vy.data(), g.ny * 8, "vy")) ||
-        (rc = h2d(ctx, g.d_vz, vz.data(), g.nz * 8, "vz")) ||
-        (rc = h2d(ctx, g.d_active, g.active.data(), ncell3 * 4, "active"))) {
+    if ((rc = h2d(ctx, g.d_ax, g.ax.data(), NX * 8, "ax")) ||
+        (rc = h2d(ctx, g.d_ay, g.ay.data(), NY * 8, "ay")) ||
+        (rc = h2d(ctx, g.d_az, g.az.data(), NZ * 8, "az")) ||
+        (rc = h2d(ctx, g.d_vx, vx.data(), NX * 8, "vx")) ||
+        (rc = h2d(ctx, g.d_vy, vy.data(), NY * 8, "vy")) ||
+        (rc = h2d(ctx, g.d_vz, vz.data(), NZ * 8, "vz")) ||
+        (rc = h2d(ctx, g.d_active, g.active.data(), NA * 4, "active"))) {
         free_grid(g);
         return rc;
     }
@@ -451,12 +523,12 @@ int moc_grid_attach(moc_ctx* ctx, int ig, const moc_grid_desc* gd) {
     MOC_TRY(hipMemsetAsync(g.d_Te, 0, C1 * 8, ctx->stream), "zero");
     MOC_TRY(hipMemsetAsync(g.d_Hden, 0, C1 * 8, ctx->stream), "zero");
     MOC_TRY(hipMemsetAsync(g.d_totalLines, 0, C1 * 8, ctx->stream), "zero");
-    moc_convert_f64_to_f32(g.d_ax, g.f_ax, g.nx, ctx->stream);
-    moc_convert_f64_to_f32(g.d_ay, g.f_ay, g.ny, ctx->stream);
-    moc_convert_f64_to_f32(g.d_az, g.f_az, g.nz, ctx->stream);
-    moc_convert_f64_to_f32(g.d_vx, g.f_vx, g.nx, ctx->stream);
-    moc_convert_f64_to_f32(g.d_vy, g.f_vy, g.ny, ctx->stream);
-    moc_convert_f64_to_f32(g.d_vz, g.f_vz, g.nz, ctx->stream);
+    moc_convert_f64_to_f32(g.d_ax, g.f_ax, (int64_t)NX, ctx->stream);
+    moc_convert_f64_to_f32(g.d_ay, g.f_ay, (int64_t)NY, ctx->stream);
+    moc_convert_f64_to_f32(g.d_az, g.f_az, (int64_t)NZ, ctx->stream);
+    moc_convert_f64_to_f32(g.d_vx, g.f_vx, (int64_t)NX, ctx->stream);
+    moc_convert_f64_to_f32(g.d_vy, g.f_vy, (int64_t)NY, ctx->stream);
+    moc_convert_f64_to_f32(g.d_vz, g.f_vz, (int64_t)NZ, ctx->stream);
     MOC_TRY(hipStreamSynchronize(ctx->stream), "grid attach");
     g.f32_dirty = true;
     g.attached = true;
@@ -464,6 +536,10 @@ int moc_grid_attach(moc_ctx* ctx, int ig, const moc_grid_desc* gd) {
     if (rc2) return rc2;
     return ensure_balance_grid(ctx);
 }
+
+}  // namespace
+
+extern "C" {
 
 int moc_set_atomic(moc_ctx* ctx, const moc_atomic* at) {
     if (!ctx) return MOC_ERR_ARG;
@@ -696,9 +772,67 @@ int moc_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ct
     MocGrid& g = ctx->grid;
     if (!g.attached) { ctx->err = "moc_transport: no grid attached"; return MOC_ERR_STATE; }
     if (iStar < 1 || iStar > ctx->nStars || n < 0) { ctx->err = "moc_transport: iStar/n"; return MOC_ERR_ARG; }
-    const int s = iStar - 1;
+    return run_transport(ctx, iStar, n, seed, ctr_base, stats);
+}
+
+}  // extern "C"
+
+namespace {
+
+// One transport launch: star iStar's n packets, or (iStar = 0, a batch of
+// grids) every star's nPhotons packets, each in its own grid.
+int run_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ctr_base,
+                  moc_transport_stats* stats) {
+    MocGrid& g = ctx->grid;
+    const bool batch = g.nGrids > 1;
+    const int s = iStar > 0 ? iStar - 1 : 0;
     const double deltaE = ctx->Lstar[s] / ctx->nPhotons[s];  // photon_mod.f90:88
     if (!(deltaE >= 0.0)) { ctx->err = "moc_transport: deltaEUsed < 0"; return MOC_ERR_ARG; }
+    // a batch: per-grid packet ranges and views (star k lights grid k)
+    std::vector<int64_t> first;
+    std::vector<MocGridView<double>> v64;
+    std::vector<MocGridView<float>> v32;
+    if (batch) {
+        const int nG = g.nGrids;
+        first.assign((size_t)nG + 1, 0);
+        for (int k = 0; k < nG; ++k) {
+            const int64_t cnt = iStar == 0 ? (int64_t)ctx->nPhotons[k] : (k == s ? n : 0);
+            if (cnt < 0) { ctx->err = "moc_transport: negative packet count"; return MOC_ERR_ARG; }
+            first[k + 1] = first[k] + cnt;
+            if (cnt > 0xffffffffLL) { ctx->err = "moc_transport: more than 2^32 packets in one grid"; return MOC_ERR_ARG; }
+        }
+        n = first[nG];
+        v64.resize(nG);
+        v32.resize(nG);
+        for (int k = 0; k < nG; ++k) {
+            const double dE = ctx->nPhotons[k] > 0 ? ctx->Lstar[k] / ctx->nPhotons[k] : 0.0;
+            if (!(dE >= 0.0)) { ctx->err = "moc_transport: deltaEUsed < 0"; return MOC_ERR_ARG; }
+            MocGridView<double>& a = v64[k];
+            MocGridView<float>& b = v32[k];
+            a.ax = g.d_ax + g.goffx[k]; a.ay = g.d_ay + g.goffy[k]; a.az = g.d_az + g.goffz[k];
+            a.vx = g.d_vx + g.goffx[k]; a.vy = g.d_vy + g.goffy[k]; a.vz = g.d_vz + g.goffz[k];
+            b.ax = g.f_ax + g.goffx[k]; b.ay = g.f_ay + g.goffy[k]; b.az = g.f_az + g.goffz[k];
+            b.vx = g.f_vx + g.goffx[k]; b.vy = g.f_vy + g.goffy[k]; b.vz = g.f_vz + g.goffz[k];
+            a.active = b.active = g.gdense[k] ? nullptr : g.d_active + g.goffa[k];
+            a.sourceCDF = ctx->d_sourceCDF + (size_t)k * ctx->nbins;
+            b.sourceCDF = ctx->f_sourceCDF + (size_t)k * ctx->nbins;
+            a.nx = b.nx = g.gnx[k]; a.ny = b.ny = g.gny[k]; a.nz = b.nz = g.gnz[k];
+            a.base = b.base = g.gbase[k];
+            a.sym = b.sym = g.sym;
+            for (int d = 0; d < 3; ++d) {
+                a.starIdx[d] = b.starIdx[d] = ctx->starIdx[3 * k + d] - 1;
+                a.star[d] = ctx->starPos[3 * k + d];
+                b.star[d] = (float)ctx->starPos[3 * k + d];
+            }
+            a.R_out = g.gR_out[k]; b.R_out = (float)g.gR_out[k];
+            a.deltaE = dE; b.deltaE = (float)dE;
+        }
+        int rc;
+        if ((rc = h2d(ctx, g.d_gridFirst, first.data(), first.size() * 8, "grid packets")) ||
+            (rc = h2d(ctx, g.d_view64, v64.data(), v64.size() * sizeof(MocGridView<double>), "grid views")) ||
+            (rc = h2d(ctx, g.d_view32, v32.data(), v32.size() * sizeof(MocGridView<float>), "grid views")))
+            return rc;
+    }
     MOC_TRY(hipMemsetAsync(ctx->d_counters, 0, MOC_CNT_N * sizeof(unsigned long long), ctx->stream), "counters");
     MOC_TRY(hipMemsetAsync(ctx->d_Qphot, 0, sizeof(double), ctx->stream), "Qphot");
     float ms = 0.f;
@@ -718,6 +852,7 @@ int moc_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ct
         a.J = g.d_J; a.escaped = g.d_escaped; a.Qphot = ctx->d_Qphot; a.counters = ctx->d_counters;
         for (int k = 0; k < 3; ++k) { a.star[k] = ctx->starPos[3 * s + k]; a.starIdx[k] = ctx->starIdx[3 * s + k] - 1; }
         a.deltaE = deltaE; a.n = n; a.seed = seed; a.ctr_base = ctr_base;
+        a.nGrids = batch ? g.nGrids : 0; a.grids = g.d_view64; a.gridFirst = g.d_gridFirst;
         rc = moc_launch_transport_f64(ctx, a, &ms);
     } else {
         if ((rc = refresh_f32(ctx))) return rc;
@@ -735,6 +870,7 @@ int moc_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ct
         a.J = g.d_J; a.escaped = g.d_escaped; a.Qphot = ctx->d_Qphot; a.counters = ctx->d_counters;
         for (int k = 0; k < 3; ++k) { a.star[k] = (float)ctx->starPos[3 * s + k]; a.starIdx[k] = ctx->starIdx[3 * s + k] - 1; }
         a.deltaE = (float)deltaE; a.n = n; a.seed = seed; a.ctr_base = ctr_base;
+        a.nGrids = batch ? g.nGrids : 0; a.grids = g.d_view32; a.gridFirst = g.d_gridFirst;
         rc = moc_launch_transport_f32(ctx, a, &ms);
     }
     if (rc) return rc;
@@ -763,6 +899,10 @@ int moc_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ct
     }
     return MOC_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* t) {
     if (!ctx) return MOC_ERR_ARG;
@@ -1204,7 +1344,17 @@ int moc_iterate(moc_ctx* ctx, int ig, int nIterateMC, uint64_t seed, moc_iterati
     }
     if ((rc = moc_reset_tallies(ctx, ig))) return rc;
     double packets = 0.0, tTr = 0.0, seg = 0.0, emi = 0.0, esc = 0.0;
-    for (int s = 1; s <= ctx->nStars; ++s) {
+    if (g.nGrids > 1) {  // every grid's packets in one launch (RNG packet id (k << 32) + local)
+        moc_transport_stats ts{};
+        const uint32_t ctr = (uint32_t)nIterateMC * 256u + 1u;
+        if ((rc = run_transport(ctx, 0, 0, seed, ctr, &ts))) return rc;
+        for (int s = 0; s < ctx->nStars; ++s) packets += (double)(int64_t)ctx->nPhotons[s];
+        tTr = ts.kernel_ms;
+        seg = ts.segments;
+        emi = ts.emissions;
+        esc = ts.escaped;
+    }
+    for (int s = 1; g.nGrids == 1 && s <= ctx->nStars; ++s) {
         const int64_t n = (int64_t)ctx->nPhotons[s - 1];
         moc_transport_stats ts{};
         const uint32_t ctr = (uint32_t)nIterateMC * 256u + (uint32_t)s;
@@ -1238,6 +1388,10 @@ int moc_iterate(moc_ctx* ctx, int ig, int nIterateMC, uint64_t seed, moc_iterati
 
 int moc_set_dust(moc_ctx* ctx, const moc_dust_tables* t) {
     if (!ctx) return MOC_ERR_ARG;
+    if (ctx->grid.attached && ctx->grid.nGrids > 1) {
+        ctx->err = "moc_set_dust: grains are not supported in a batch of grids";
+        return MOC_ERR_STATE;
+    }
     MocGrid& g = ctx->grid;
     if (!g.attached) { ctx->err = "moc_set_dust: attach the grid first"; return MOC_ERR_STATE; }
     if (!t || t->nSpecies < 1 || t->nSizes < 1 || t->nTemps < 2 || !t->grainAbun || !t->grainWeight ||

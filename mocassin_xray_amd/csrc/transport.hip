@@ -50,17 +50,13 @@ struct Pk {
     T kap;
 };
 
-// axes and volume factors staged in LDS: segment walks read them every step,
-// and LDS reads do not queue behind the J atomics in the vector memory counter
+// The grid a packet moves in (MocGridView): for a single grid it is built
+// once per lane from the launch's (uniform) arguments, with the axes and
+// volume factors staged in LDS -- segment walks read them every step, and LDS
+// reads do not queue behind the J atomics in the vector memory counter; in a
+// batch of grids each packet loads its grid's view from global memory.
 template <typename T>
-struct Geo {
-    const T* ax;
-    const T* ay;
-    const T* az;
-    const T* vx;
-    const T* vy;
-    const T* vz;
-};
+using View = MocGridView<T>;
 
 template <typename T>
 __device__ __forceinline__ T tabs(T a) { return a < T(0) ? -a : a; }
@@ -154,9 +150,9 @@ __device__ __forceinline__ void nudge(T pos, T dir, int& idx, const T* ax, int n
 // active cell id of (ix, iy, iz); a fully active grid numbered by active_map
 // order needs no lookup, which takes one dependent memory access off a segment
 template <typename T>
-__device__ __forceinline__ int cell_of(const MocTransportArgs<T>& a, int ix, int iy, int iz) {
-    if (a.dense) return 1 + iz + a.nz * (iy + a.ny * ix);
-    return a.active[(size_t)ix + (size_t)a.nx * ((size_t)iy + (size_t)a.ny * (size_t)iz)];
+__device__ __forceinline__ int cell_of(const View<T>& v, int ix, int iy, int iz) {
+    if (!v.active) return v.base + 1 + iz + v.nz * (iy + v.ny * ix);
+    return v.active[(size_t)ix + (size_t)v.nx * ((size_t)iy + (size_t)v.ny * (size_t)iz)];
 }
 
 // per-lane event counts; a lane handles at most a few million events per
@@ -167,26 +163,27 @@ struct Cnt {
 };
 
 template <typename T>
-__device__ __forceinline__ void tally_escape(const MocTransportArgs<T>& a, const Pk<T>& p,
-                                             Cnt& c) {
+__device__ __forceinline__ void tally_escape(const MocTransportArgs<T>& a, const View<T>& v,
+                                             const Pk<T>& p, Cnt& c) {
     atomicAdd(&a.escaped[(size_t)p.nu0 * (size_t)(a.nCells + 1) + (size_t)p.origin],
-              (double)a.deltaE);
+              (double)v.deltaE);
     ++c.esc;
 }
 
 // newPhotonPacket + initPhotonPacket + the escape/line branches of
 // energyPacketRun, then the start of pathSegment (tau = -ln(1-R)).
 template <typename T>
-__device__ __forceinline__ void emit(const MocTransportArgs<T>& a, Pk<T>& p, MocRng& rng, Cnt& c) {
+__device__ __forceinline__ void emit(const MocTransportArgs<T>& a, const View<T>& v, Pk<T>& p,
+                                     MocRng& rng, Cnt& c) {
     ++c.emit;
     ++p.emits;
     int bad = 0;
     if (p.stellar) {
-        p.nu0 = get_nu2(a.sourceCDF, a.nbins, rng, &bad);
+        p.nu0 = get_nu2(v.sourceCDF, a.nbins, rng, &bad);
         const T nu = a.nuArray[p.nu0];
-        if (nu >= T(1)) c.qphot += (double)a.deltaE / (kRyd2ergQ * (double)nu);
+        if (nu >= T(1)) c.qphot += (double)v.deltaE / (kRyd2ergQ * (double)nu);
     } else {
-        const int cell = cell_of(a, p.ix, p.iy, p.iz);
+        const int cell = cell_of(v, p.ix, p.iy, p.iz);
         if (cell <= 0) { ++c.err; p.state = ST_DONE; return; }
         if (!a.dustOnly) {  // a dustEmi packet (:1673-1741) is never a line
             const T r = T(1) - rng.uniform<T>();
@@ -200,15 +197,15 @@ __device__ __forceinline__ void emit(const MocTransportArgs<T>& a, Pk<T>& p, Moc
     }
     if (bad) ++c.err;
     unit_vector(rng, p.u, p.v, p.w);
-    if (a.sym && p.stellar) {
+    if (v.sym && p.stellar) {
         p.u = tabs(p.u);
         p.v = tabs(p.v);
         p.w = tabs(p.w);
     }
-    p.origin = cell_of(a, p.ix, p.iy, p.iz);
+    p.origin = cell_of(v, p.ix, p.iy, p.iz);
     p.nu = a.nuArray[p.nu0];
     if (!a.lgDust && p.nu < a.ionEdge1) {  // below every edge and no dust: escapes at once
-        tally_escape(a, p, c);
+        tally_escape(a, v, p, c);
         p.state = ST_DONE;
         return;
     }
@@ -242,7 +239,7 @@ __device__ __forceinline__ void compton(const MocTransportArgs<T>& a, Pk<T>& p, 
 // has been requested: the vector memory counter retires in order, so the next
 // step's wait for that opacity does not include the atomic's round trip.
 template <typename T>
-__device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const Geo<T>& g, Pk<T>& p,
+__device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const View<T>& g, Pk<T>& p,
                                         MocRng& rng, Cnt& c) {
     if (p.segs >= MOC_SAFE_LIMIT) {
         ++c.lost;
@@ -251,7 +248,7 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const Geo<
     }
     ++p.segs;
     ++c.seg;
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
 
     nudge(p.x, p.u, p.ix, g.ax, nx);
     nudge(p.y, p.v, p.iy, g.ay, ny);
@@ -261,7 +258,7 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const Geo<
         p.state = ST_DONE;
         return;
     }
-    if (a.sym) {  // mirror walls of the octant model
+    if (g.sym) {  // mirror walls of the octant model
         if (p.x <= g.ax[0]) { if (p.u < T(0)) p.u = -p.u; p.x = g.ax[0]; }
         if (p.y <= g.ay[0]) { if (p.v < T(0)) p.v = -p.v; p.y = g.ay[0]; }
         if (p.z <= g.az[0]) { if (p.w < T(0)) p.w = -p.w; p.z = g.az[0]; }
@@ -270,7 +267,7 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const Geo<
     // the packet; a wall snap below moves it to a neighbour (rarely) and the
     // value is fetched again for that cell
     const size_t row = (size_t)p.nu0 * (size_t)(a.nCells + 1);
-    const int cell0 = cell_of(a, p.ix, p.iy, p.iz);
+    const int cell0 = cell_of(g, p.ix, p.iy, p.iz);
     T kap0 = p.kap;
     if (cell0 != p.kcell) kap0 = a.opacity[row + (size_t)cell0];
     T dSx = T(0), dSy = T(0), dSz = T(0);
@@ -297,7 +294,7 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const Geo<
         return;
     }
 
-    const int cell = cell_of(a, p.ix, p.iy, p.iz);
+    const int cell = cell_of(g, p.ix, p.iy, p.iz);
     T kap = kap0;
     if (cell != cell0) kap = a.opacity[row + (size_t)cell];
     const T tauCell = dS * kap;
@@ -307,22 +304,22 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const Geo<
     if (p.absTau + tauCell > p.passProb && cell > 0) {
         // the packet interacts inside this cell
         const T dl = (p.passProb - p.absTau) / kap;
-        const double dJ = (double)(dl * a.deltaE / dV);
+        const double dJ = (double)(dl * g.deltaE / dV);
         p.x = p.x + dl * p.u;
         p.y = p.y + dl * p.v;
         p.z = p.z + dl * p.w;
-        if (a.sym) {
+        if (g.sym) {
             if (p.x <= g.ax[0]) { if (p.u < T(0)) p.u = -p.u; p.x = g.ax[0]; }
             if (p.y <= g.ay[0]) { if (p.v < T(0)) p.v = -p.v; p.y = g.ay[0]; }
             if (p.z <= g.az[0]) { if (p.w < T(0)) p.w = -p.w; p.z = g.az[0]; }
         }
-        if (a.R_out > T(0)) {
+        if (g.R_out > T(0)) {
             // radius with the 1e10 scaling of photon_mod.f90:2294 (no FP32 overflow)
             const T sx = p.x / T(1.e10), sy = p.y / T(1.e10), sz = p.z / T(1.e10);
             const T rr = sqrt(sx * sx + sy * sy + sz * sz) * T(1.e10);
-            if (rr >= a.R_out) {
+            if (rr >= g.R_out) {
                 atomicAdd(Jc, dJ);
-                tally_escape(a, p, c);
+                tally_escape(a, g, p, c);
                 p.state = ST_DONE;
                 return;
             }
@@ -383,7 +380,7 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const Geo<
     }
 
     // the packet crosses the cell
-    const double dJ = (double)(dS * a.deltaE / dV);
+    const double dJ = (double)(dS * g.deltaE / dV);
     p.absTau = p.absTau + tauCell;
     p.x = p.x + dS * p.u;
     p.y = p.y + dS * p.v;
@@ -396,7 +393,7 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const Geo<
     else if (dS == dSz && p.w < T(0)) --p.iz;
 
     bool esc = false;
-    if (!a.sym) {
+    if (!g.sym) {
         if (p.y < g.ay[0] || p.iy < 0) { p.iy = 0; esc = true; }
         if (p.y > g.ay[ny - 1] || p.iy > ny - 1) { p.iy = ny - 1; esc = true; }
         if (p.x <= g.ax[0] || p.ix < 0) { p.ix = 0; esc = true; }
@@ -404,23 +401,23 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const Geo<
         if (p.z <= g.az[0] || p.iz < 0) { p.iz = 0; esc = true; }
         if (p.z >= g.az[nz - 1] || p.iz > nz - 1) { p.iz = nz - 1; esc = true; }
     }
-    if (!esc && ((!a.sym && (p.x <= g.ax[0] || p.y <= g.ay[0] || p.z <= g.az[0])) ||
+    if (!esc && ((!g.sym && (p.x <= g.ax[0] || p.y <= g.ay[0] || p.z <= g.az[0])) ||
                  p.x >= g.ax[nx - 1] || p.y >= g.ay[ny - 1] || p.z >= g.az[nz - 1] ||
                  p.ix > nx - 1 || p.iy > ny - 1 || p.iz > nz - 1))
         esc = true;
     if (!esc) {
-        if (a.sym) {
+        if (g.sym) {
             if (p.x <= g.ax[0] || p.ix < 0) { if (p.u < T(0)) p.u = -p.u; p.ix = 0; p.x = g.ax[0]; }
             if (p.y <= g.ay[0] || p.iy < 0) { if (p.v < T(0)) p.v = -p.v; p.iy = 0; p.y = g.ay[0]; }
             if (p.z <= g.az[0] || p.iz < 0) { if (p.w < T(0)) p.w = -p.w; p.iz = 0; p.z = g.az[0]; }
         }
         // every index is inside the grid here: request the next cell's opacity
-        p.kcell = cell_of(a, p.ix, p.iy, p.iz);
+        p.kcell = cell_of(g, p.ix, p.iy, p.iz);
         p.kap = a.opacity[row + (size_t)p.kcell];
     }
     if (cell > 0) atomicAdd(Jc, dJ);
     if (esc) {
-        tally_escape(a, p, c);
+        tally_escape(a, g, p, c);
         p.state = ST_DONE;
     }
 }
@@ -449,11 +446,13 @@ constexpr int kSegsPerTrip = 2;
 
 extern __shared__ __align__(16) unsigned char transport_lds[];
 
-template <typename T>
+// BATCH: the packets of a batch of grids (a.nGrids > 0), each in its own grid
+template <typename T, bool BATCH>
 __global__ void __launch_bounds__(256) k_transport(MocTransportArgs<T> a) {
-    // stage the axes and volume factors: [ax ay az vx vy vz]
-    T* const sg = reinterpret_cast<T*>(transport_lds);
-    {
+    View<T> g;
+    if constexpr (!BATCH) {
+        // stage the axes and volume factors: [ax ay az vx vy vz]
+        T* const sg = reinterpret_cast<T*>(transport_lds);
         const int nx = a.nx, ny = a.ny, nz = a.nz, nt = nx + ny + nz;
         for (int i = threadIdx.x; i < nt; i += blockDim.x) {
             const bool bx = i < nx, by = !bx && i < nx + ny;
@@ -462,9 +461,14 @@ __global__ void __launch_bounds__(256) k_transport(MocTransportArgs<T> a) {
             sg[nt + i] = bx ? a.vx[k] : (by ? a.vy[k] : a.vz[k]);
         }
         __syncthreads();
+        g.ax = sg; g.ay = sg + nx; g.az = sg + nx + ny;
+        g.vx = sg + nt; g.vy = sg + nt + nx; g.vz = sg + nt + nx + ny;
+        g.active = a.dense ? nullptr : a.active;
+        g.sourceCDF = a.sourceCDF;
+        g.nx = nx; g.ny = ny; g.nz = nz; g.base = 0; g.sym = a.sym;
+        g.R_out = a.R_out;
+        g.deltaE = a.deltaE;
     }
-    const Geo<T> g{sg, sg + a.nx, sg + a.nx + a.ny, sg + a.nx + a.ny + a.nz,
-                   sg + 2 * a.nx + a.ny + a.nz, sg + 2 * a.nx + 2 * a.ny + a.nz};
     Cnt c;
     Pk<T> p;
     p.state = ST_DONE;
@@ -483,13 +487,31 @@ __global__ void __launch_bounds__(256) k_transport(MocTransportArgs<T> a) {
             if (p.state == ST_DONE && more) {
                 const unsigned long long id = base + (unsigned long long)__popcll(want & below);
                 if ((int64_t)id < a.n) {
-                    rng.init(a.seed, id, a.ctr_base);
-                    p.x = a.star[0];
-                    p.y = a.star[1];
-                    p.z = a.star[2];
-                    p.ix = a.starIdx[0];
-                    p.iy = a.starIdx[1];
-                    p.iz = a.starIdx[2];
+                    if constexpr (BATCH) {
+                        // the grid of this packet: last k with gridFirst[k] <= id
+                        int lo = 0, hi = a.nGrids;
+                        while (hi - lo > 1) {
+                            const int mid = (lo + hi) >> 1;
+                            if (a.gridFirst[mid] <= (int64_t)id) lo = mid; else hi = mid;
+                        }
+                        const unsigned long long local = id - (unsigned long long)a.gridFirst[lo];
+                        rng.init(a.seed, ((unsigned long long)lo << 32) + local, a.ctr_base);
+                        g = a.grids[lo];
+                        p.x = g.star[0];
+                        p.y = g.star[1];
+                        p.z = g.star[2];
+                        p.ix = g.starIdx[0];
+                        p.iy = g.starIdx[1];
+                        p.iz = g.starIdx[2];
+                    } else {
+                        rng.init(a.seed, id, a.ctr_base);
+                        p.x = a.star[0];
+                        p.y = a.star[1];
+                        p.z = a.star[2];
+                        p.ix = a.starIdx[0];
+                        p.iy = a.starIdx[1];
+                        p.iz = a.starIdx[2];
+                    }
                     p.stellar = 1;
                     p.emits = 0;
                     p.state = ST_EMIT;
@@ -502,7 +524,7 @@ __global__ void __launch_bounds__(256) k_transport(MocTransportArgs<T> a) {
         const unsigned long long flyers = __ballot(p.state == ST_FLY);
         if (!emitters && !flyers && !__ballot(more)) break;
         if (emitters && (__popcll(emitters) >= kEmitBatch || !flyers || !__ballot(more))) {
-            if (p.state == ST_EMIT) emit(a, p, rng, c);
+            if (p.state == ST_EMIT) emit(a, g, p, rng, c);
         }
 #pragma unroll 1
         for (int k = 0; k < kSegsPerTrip; ++k)
@@ -556,14 +578,17 @@ int launch(moc_ctx* ctx, const MocTransportArgs<T>& a, float* ms) {
     int64_t lanes = a.n < lanes_max ? a.n : lanes_max;
     const int block = 256;
     const int blocks = (int)((lanes + block - 1) / block);
-    const size_t lds = 2 * (size_t)(a.nx + a.ny + a.nz) * sizeof(T);
+    const size_t lds = a.nGrids > 0 ? 0 : 2 * (size_t)(a.nx + a.ny + a.nz) * sizeof(T);
     if (lds > kMaxGeoLds) {
         ctx->err = "transport: the grid axes (" + std::to_string(a.nx + a.ny + a.nz) +
                    " points) exceed the " + std::to_string(kMaxGeoLds) + " B LDS staging area";
         return MOC_ERR_ARG;
     }
     (void)hipEventRecord(ctx->ev0, ctx->stream);
-    hipLaunchKernelGGL(k_transport<T>, dim3(blocks), dim3(block), lds, ctx->stream, a);
+    if (a.nGrids > 0)
+        hipLaunchKernelGGL((k_transport<T, true>), dim3(blocks), dim3(block), lds, ctx->stream, a);
+    else
+        hipLaunchKernelGGL((k_transport<T, false>), dim3(blocks), dim3(block), lds, ctx->stream, a);
     (void)hipEventRecord(ctx->ev1, ctx->stream);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

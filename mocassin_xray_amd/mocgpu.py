@@ -65,6 +65,25 @@ class CartesianGrid:
     extra: dict = field(default_factory=dict)
 
 
+@dataclass
+class BatchGrid:
+    """A batch of grids attached together: grid k's cell c is global cell
+    cellBase[k] + c."""
+    grids: list
+
+    @property
+    def nCells(self) -> int:
+        return int(sum(g.nCells for g in self.grids))
+
+    @property
+    def cellBase(self) -> list:
+        out, b = [], 0
+        for g in self.grids:
+            out.append(b)
+            b += g.nCells
+        return out
+
+
 def _c(a, dtype=np.float64):
     return np.ascontiguousarray(a, dtype=dtype)
 
@@ -237,6 +256,15 @@ class MocGPU:
         d = to_struct(MocGridDesc, self._grid_keep)
         self._check(self._L.moc_grid_attach(self._ctx, 1, C.byref(d)), "moc_grid_attach")
         self.grid = g
+
+    def grid_attach_batch(self, grids: list) -> None:
+        """A batch of independent grids (one star per grid, in order); their
+        cells are numbered one grid after the other."""
+        self._grid_keep = [grid_arrays(g) for g in grids]
+        arr = (MocGridDesc * len(grids))(*[to_struct(MocGridDesc, d) for d in self._grid_keep])
+        self._check(self._L.moc_grid_attach_batch(self._ctx, len(grids), arr),
+                    "moc_grid_attach_batch")
+        self.grid = BatchGrid(list(grids))
 
     def _count(self, name: str) -> int:
         c1 = self.grid.nCells + 1

@@ -329,6 +329,94 @@ def gas_dust_nebula(n: int = 64, nbins: int = 700, n_packets: int = 10_000_000,
                           photoelectric=photoelectric)
 
 
+# ------------------------------------------------------------- batched grids
+@dataclass
+class BatchModel:
+    """Independent gas nebulae in one context: one grid and one star each,
+    a shared mesh, element set and tables; cells numbered grid after grid."""
+    name: str
+    tables: SourceTables       # nStars = number of grids, star k lights grid k
+    grids: list                # CartesianGrid per member
+    members: list              # SourceTables of each member alone (its star)
+    elems: list
+    abund: np.ndarray
+    nstages: int
+    xsec: object
+    bt: object
+    et: object
+    state: dict                # concatenated: row 0, then each grid's cells
+    n_packets: int             # total per iteration
+
+    @property
+    def cellBase(self) -> list:
+        out, b = [], 0
+        for g in self.grids:
+            out.append(b)
+            b += g.nCells
+        return out
+
+    def load(self, g) -> None:
+        g.grid_attach_batch(self.grids)
+        g.set_atomic(self.elems, self.abund, self.nstages, self.xsec)
+        g.set_balance(self.bt)
+        g.set_emission(self.et)
+        for k in ("ionDen", "Te", "Ne", "Hden"):
+            g.upload(k, self.state[k])
+
+    def member(self, k: int) -> GasModel:
+        """Grid k on its own, with its star and the slice of the state."""
+        b, n = self.cellBase[k], self.grids[k].nCells
+        rows = np.r_[0, np.arange(b + 1, b + n + 1)]
+        st = {key: np.ascontiguousarray(v[rows]) for key, v in self.state.items()}
+        return GasModel(f"{self.name}[{k}]", self.members[k], self.grids[k], self.elems,
+                        self.abund, self.nstages, self.xsec, self.bt, self.et, st,
+                        int(self.members[k].nPhotons[0]))
+
+
+def batch_nebulae(n_grids: int = 256, nbins: int = 400, seed: int = 20261203 + 4,
+                  edge=(8, 48), hden=(10.0, 1.e4), T_star=(3.e4, 1.5e5), packets=(1.e5, 1.e6),
+                  L36: float = 10.0, nstages: int = 7, maxLev: int = 5) -> BatchModel:
+    """n_grids seeded nebulae of uneven size, one octant each: n ~ U{edge}
+    cells per axis, uniform n_H log-uniform over `hden`, a blackbody of T
+    uniform over T_star and L36 x 1e36 erg/s, packets log-uniform over
+    `packets`, the seven elements of the gas nebula; each grid's edge is 1.5
+    Stromgren radii of 1.8e47 ionising photons/s at its n_H (alpha_B
+    2.6e-13).  A shared mesh of nbins points over 1-100 Ryd."""
+    from . import atomic, balance, emission
+    rng = np.random.default_rng(seed)
+    elems = NEBULA_ELEMS
+    abund = 10.0 ** np.array(NEBULA_LOG_ABUND)
+    nu = T.reference_gas_mesh(nbins, 1.0, 100.0, atomic.ion_edges(elems, nstages, 100.0))
+    xs = atomic.build_xsec(nu, elems, nstages)
+    bt = balance.build_balance(xs, maxLev=maxLev)
+    grids, members, hcells = [], [], []
+    for k in range(n_grids):
+        n = int(rng.integers(edge[0], edge[1] + 1))
+        nh = float(np.exp(rng.uniform(np.log(hden[0]), np.log(hden[1]))))
+        ts = float(rng.uniform(T_star[0], T_star[1]))
+        npk = int(np.exp(rng.uniform(np.log(packets[0]), np.log(packets[1]))))
+        R = 1.5 * (3.0 * 1.8e47 / (4.0 * np.pi * nh * nh * 2.6e-13)) ** (1.0 / 3.0)
+        ax = T.symmetric_axes(n, R)
+        act, nc, hc = T.active_map(ax, ax, ax, np.full((n, n, n), nh))
+        grids.append(CartesianGrid(ax, ax.copy(), ax.copy(), act, nc, True, 0.0))
+        members.append(source_tables(nu, ts, L36, npk, True, n, n, n, True))
+        hcells.append(hc)
+    t0 = members[0]
+    tabs = SourceTables(nuArray=nu, widFlx=t0.widFlx, ionEdge1=t0.ionEdge1, KNsigmaT=t0.KNsigmaT,
+                        KNcdf=t0.KNcdf, Pcomp=t0.Pcomp,
+                        sourceCDF=np.vstack([m.sourceCDF for m in members]),
+                        Lstar=np.concatenate([m.Lstar for m in members]),
+                        nPhotons=np.concatenate([m.nPhotons for m in members]),
+                        starPosition=np.vstack([np.reshape(m.starPosition, (1, 3)) for m in members]),
+                        starIndex=np.vstack([np.reshape(m.starIndex, (1, 3)) for m in members]),
+                        lgCompton=True)
+    et = emission.build_emission(xs, tabs.widFlx)
+    hall = np.concatenate([[0.0]] + [h[1:] for h in hcells])
+    st = initial_state(elems, nstages, hall)
+    return BatchModel("batch%d" % n_grids, tabs, grids, members, list(elems), abund, nstages,
+                      xs, bt, et, st, int(tabs.nPhotons.sum()))
+
+
 # ------------------------------------------------------------- X-ray nebula
 # Solar abundances H-Zn, log eps (H = 12), Anders & Grevesse (1989, Geochim.
 # Cosmochim. Acta 53, 197), the photospheric values with meteoritic ones for

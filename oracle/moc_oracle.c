@@ -455,7 +455,7 @@ int oracle_transport(const oracle_transport_in* in, oracle_transport_out* out) {
             opacket_t pk;
             orng_t g;
             int e, reRun = 1;
-            orng_init(&g, in->seed, (uint64_t)ip, in->ctr_base);
+            orng_init(&g, in->seed, in->id_base + (uint64_t)ip, in->ctr_base);
             memset(&pk, 0, sizeof(pk));
             pk.x = in->star[0];
             pk.y = in->star[1];

@@ -28,6 +28,7 @@ class TransportIn(C.Structure):
         ("scaOpac", _dp), ("lgDust", C.c_int), ("dustOnly", C.c_int),
         ("star", C.c_double * 3), ("starIdx", C.c_int32 * 3), ("deltaE", C.c_double),
         ("n", C.c_int64), ("seed", C.c_uint64), ("ctr_base", C.c_uint32), ("nthreads", C.c_int),
+        ("id_base", C.c_uint64),
     ]
 
 
@@ -82,9 +83,10 @@ def locate(xa, x) -> int:
     return lib().oracle_locate(a.ctypes.data_as(_dp), a.size, float(x))
 
 
-def transport(model_tables, grid, fields, n, seed, ctr_base=0, nthreads=1, istar=0):
+def transport(model_tables, grid, fields, n, seed, ctr_base=0, nthreads=1, istar=0, id_base=0):
     """Run the oracle transport.  Returns (J[C+1,B], escaped[C+1,B], counters, Qphot).
-    With grains, fields holds scaOpac (C+1, B) and dustOnly (bool)."""
+    With grains, fields holds scaOpac (C+1, B) and dustOnly (bool).  Packet i
+    draws from the RNG stream of packet id id_base + i."""
     t = model_tables
     B = t.nbins
     C1 = grid.nCells + 1
@@ -109,7 +111,8 @@ def transport(model_tables, grid, fields, n, seed, ctr_base=0, nthreads=1, istar
         recPDF=d("rec", fields["recPDF"]), totalLines=d("tl", fields["totalLines"]),
         scaOpac=d("sca", np.asarray(fields["scaOpac"]).T) if "scaOpac" in fields else None,
         lgDust=int("scaOpac" in fields), dustOnly=int(bool(fields.get("dustOnly", False))),
-        deltaE=deltaE, n=int(n), seed=int(seed), ctr_base=int(ctr_base), nthreads=int(nthreads))
+        deltaE=deltaE, n=int(n), seed=int(seed), ctr_base=int(ctr_base), nthreads=int(nthreads),
+        id_base=int(id_base))
     pos = np.asarray(t.starPosition).reshape(-1, 3)[istar]
     idx = np.asarray(t.starIndex).reshape(-1, 3)[istar]
     for k in range(3):
