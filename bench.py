@@ -42,6 +42,8 @@ def build_model(name: str):
         return nebula.xray_nebula(n=128, nbins=2000, n_packets=1_000_000_000)
     if name == "xray32":
         return nebula.xray_nebula(n=32, nbins=2000, n_packets=10_000_000)
+    if name == "batch256":
+        return nebula.batch_nebulae(n_grids=256, nbins=400)
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -101,9 +103,13 @@ def run_path(m, prec: str, steps: int, warmup: int):
     return dt, stats, state
 
 
+def n_cells(m) -> int:
+    return sum(g.nCells for g in m.grids) if hasattr(m, "grids") else m.grid.nCells
+
+
 def big_model(m) -> bool:
     """More than 5e8 (cell, bin) entries: its planes are not copied to the host."""
-    return (m.grid.nCells + 1) * m.tables.nbins > 500_000_000
+    return (n_cells(m) + 1) * m.tables.nbins > 500_000_000
 
 
 def state_error(m, a: dict, b: dict) -> dict:
@@ -111,7 +117,9 @@ def state_error(m, a: dict, b: dict) -> dict:
     (weights n_H dV), and a chi2-like J distance over cells."""
     import numpy as np
     from mocassin_xray_amd import tables as T
-    dV = T.cell_volumes(m.grid.xAxis, m.grid.yAxis, m.grid.zAxis, m.grid.active, True)[1:]
+    grids = m.grids if hasattr(m, "grids") else [m.grid]
+    dV = np.concatenate([T.cell_volumes(g.xAxis, g.yAxis, g.zAxis, g.active, True)[1:]
+                         for g in grids])
     w = dV * m.state["Hden"][1:]
     Ta, Tb = a["Te"][1:], b["Te"][1:]
     te = abs(np.sum(w * Ta) - np.sum(w * Tb)) / np.sum(w * Tb)
@@ -139,6 +147,38 @@ def copy_calibration(nbytes: int = 1 << 30, reps: int = 3):
         b.copy_(a)
     torch.cuda.synchronize()
     print(f"copy_calibration: {reps} copies of {nbytes} bytes", file=sys.stderr, flush=True)
+
+
+def cpu_baseline_batch(m, n_members: int = 8):
+    """The oracle on a bounded sample of a batch: the first n_members grids,
+    each through one whole iteration on its own from its starting state
+    (opacity, emission, its packets, J normalisation, balance), scaled by
+    packets to the whole batch."""
+    import numpy as np
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as O
+    threads = min(os.cpu_count() or 1, 16)
+    E, Z, NS = m.elems, m.abund, m.nstages
+    t_tot, pk = 0.0, 0
+    for k in range(n_members):
+        mk = m.member(k)
+        s0 = mk.state
+        t0 = time.perf_counter()
+        op = O.opacity(m.xsec, m.tables.KNsigmaT, True, E, Z, NS, s0["ionDen"], s0["Ne"], s0["Te"],
+                       s0["Hden"])
+        em = O.emission(m.et, m.bt, E, Z, NS, s0["Te"], s0["Ne"], s0["ionDen"], nthreads=threads)
+        fields = {"opacity": op, "Ne": s0["Ne"], "recPDF": em["recPDF"],
+                  "totalLines": em["totalLines"]}
+        J = O.transport(mk.tables, mk.grid, fields, mk.n_packets, 5, nthreads=threads)[0]
+        J *= 1.e-9 / 8.0 if mk.grid.lgSymmetricXYZ else 1.e-9
+        O.update(m.bt, E, Z, NS, True, J, s0["Te"], s0["Ne"], s0["Hden"], s0["ionDen"],
+                 nthreads=threads)
+        t_tot += time.perf_counter() - t0
+        pk += mk.n_packets
+    return {"value": pk / t_tot, "unit": "packets/s", "cores": threads, "kind": "port",
+            "sample": (f"the first iteration of the first {n_members} of {len(m.grids)} grids of "
+                       f"{m.name}, each run whole by the oracle (C, FP64, OpenMP): {pk} packets, "
+                       f"{sum(g.nCells for g in m.grids[:n_members])} cells")}
 
 
 def cpu_baseline(m, gs: dict, target_s: float = 20.0):
@@ -230,7 +270,7 @@ def main():
         return
     m = build_model(args.workload)
     n = m.n_packets
-    C = m.grid.nCells
+    C = n_cells(m)
     print(f"bench: workload {m.name} cells={C} bins={m.tables.nbins} packets={n}",
           file=sys.stderr, flush=True)
     res = {}
@@ -246,8 +286,9 @@ def main():
     dt_mix, st_mix, s_mix = res["mixed"]
     dt_64, st_64, s_64 = res["fp64"]
     kms = mean(st_mix, "transport_ms")
+    dense = all(dense_grid(g) for g in m.grids) if hasattr(m, "grids") else dense_grid(m.grid)
     bytes_mix = transport_bytes(mean(st_mix, "segments"), mean(st_mix, "emissions"),
-                                mean(st_mix, "escaped"), m.tables.nbins, 4, dense_grid(m.grid))
+                                mean(st_mix, "escaped"), m.tables.nbins, 4, dense)
     achieved = bytes_mix / (kms * 1e-3) / 1e9
     out = {
         "metric": "packets/s of the Monte Carlo iteration (mixed precision) + balance cells/s",
@@ -285,7 +326,9 @@ def main():
                      "traffic": profiled_traffic(args.workload),
                      "kernel": "k_transport<float>", "kernel_ms": kms},
     }
-    if not args.no_cpu and not big_model(m):
+    if not args.no_cpu and hasattr(m, "grids"):
+        out["cpu_baseline"] = cpu_baseline_batch(m)
+    elif not args.no_cpu and not big_model(m):
         out["cpu_baseline"] = cpu_baseline(m, s_mix)
     print(json.dumps(out))
 
