@@ -1,7 +1,8 @@
 // Per-cell diffuse emission on gfx950 (emissionDriver, emission_mod.f90:36-252).
 //
-// One workgroup of four waves per cell.  The continuum part is data-parallel
-// over frequency: each lane evaluates whole bins -- the H I / He I / He II
+// Two kernels per cell.  k_emission, one workgroup of four waves per cell,
+// takes the continuum, data-parallel over frequency: each lane evaluates
+// whole bins -- the H I / He I / He II
 // free-bound fits between their edge pointers (fb_ff :389-507), the Milne
 // relation above the edges (:515-550), the heavy-ion free-bound sum
 // (:552-616), free-free with the Hummer Gaunt factor at the cell's T_e
@@ -9,11 +10,11 @@
 // workgroup turns the bin energies c*dnu*(e_HI+e_HeI+e_HeII) into the
 // normalised CDF recPDF(cell, :) with a wave-shuffle scan (setDiffusePDF
 // :1593-1835), 256 bins per step, the running total carried across steps.
-// The line part -- hydrogenic recombination lines per element
-// (RecLinesEmission :856-1108), He I recombination lines (:1110-1152) and the
-// n-level forbidden lines with the recombination feed (forLines :1481-1526)
-// -- is a list of tasks spread over the lanes and reduced across the
-// workgroup; its sum against the continuum gives totalLines (:1904-1910).
+// k_em_lines, one wave per cell and run first, takes the lines --
+// hydrogenic recombination lines per element (RecLinesEmission :856-1108),
+// He I recombination lines (:1110-1152) and the n-level forbidden lines with
+// the recombination feed (forLines :1481-1526) -- one task per lane; their
+// sum against the continuum gives totalLines (:1904-1910).
 // All arithmetic is FP64.  The row of recPDF is written once unnormalised and
 // rescaled in place by the lanes that wrote it.
 #include <hip/hip_runtime.h>
@@ -220,9 +221,15 @@ struct CellCtx {
 
 // c * dnu * (e_HI + e_HeI + e_HeII) of bin i (1-based), with the He I / He II
 // line energies of setDiffusePDF added in their bins
+// Milne factors of the heavy ions: exp(-(nu - nu_lo) h/kT) for the bins of a
+// 256-bin pass that starts at bin b0 (0-based) is exp(-(nu - nu_b0) h/kT)
+// (ePass, one per lane) times exp(-(nu_b0 - nu_lo) h/kT), for the ions whose
+// edge is at or below nu_b0; both factors lie in (0, 1].  sF holds the second
+// factor times the ion's Gaunt-free weight g X abundance, once per pass.  An
+// edge inside the pass takes the direct exponential.
 __device__ double bin_energy(const MocEmissionArgs& a, const CellCtx& c, const double (*sLoc)[MOC_EM_MAXLIM],
                              const double (*sFit)[MOC_EM_MAXLIM / 2], const double (*sX)[MOC_MAXS + 1],
-                             const double* sTd, int i) {
+                             const double* sTd, int i, int b0, double ePass, const double* sF) {
     const double nu = a.nu[i - 1];
     const double Te = c.Te, Ne = c.Ne;
     double gam[3] = {0., 0., 0.};
@@ -232,7 +239,7 @@ __device__ double bin_energy(const MocEmissionArgs& a, const CellCtx& c, const d
         const int p = a.gpair[k * a.B + i - 1];
         if (p > 0) {
             const int iup = a.edge[k][2 * p - 1];
-            gam[k] = pow(10., sLoc[k][2 * p - 1] + sFit[k][p - 1] * (nu - a.nu[iup - 1]));
+            gam[k] = exp10(sLoc[k][2 * p - 1] + sFit[k][p - 1] * (nu - a.nu[iup - 1]));
         }
     }
     const double xlf = a.log10nu[i - 1];
@@ -255,34 +262,24 @@ __device__ double bin_energy(const MocEmissionArgs& a, const CellCtx& c, const d
             }
         }
     }
-    // heavy-ion free-bound (Z = 3..19 entries in Z, ion order)
-    // (the cross-sections of a batch of ions are requested together, then the
-    // terms are added in ion order)
-    double gHv = 0.;
-    constexpr int kFbBatch = 8;
-    for (int n0 = 0; n0 < a.nFb; n0 += kFbBatch) {
-        double xs[kFbBatch];
-#pragma unroll
-        for (int q = 0; q < kFbBatch; ++q) {
-            const int n = n0 + q;
-            xs[q] = 0.;
-            if (n < a.nFb) {
-                const int lo = a.fbLo[n];
-                if (i >= lo && i < a.fbHi[n]) xs[q] = a.fbXs[a.fbOff[n] + (i - lo)];
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < kFbBatch; ++q) {
-            const int n = n0 + q;
-            if (n >= a.nFb) break;
-            const int lo = a.fbLo[n];
-            if (i >= lo && i < a.fbHi[n]) {
-                const int s = a.fbSlot[n];
-                const double ex = exp((-nu + a.nu[lo - 1]) * kHcRydK / Te);
-                gHv += ex * xs[q] * a.fbG[n] * sX[s][a.fbIon[n] + 1] * a.abund[s];
-            }
+    // heavy-ion free-bound (Z = 3..19 entries in Z, ion order): an ion whose
+    // edge is at or below the pass start contributes xs * sF (its weight
+    // exp(-(nu_b0 - nu_lo) h/kT) g X abundance), all times ePass; an edge
+    // inside the pass takes the direct exponential
+    double gFac = 0., gDir = 0.;
+    for (int n = 0; n < a.nFb; ++n) {
+        const int lo = a.fbLo[n];
+        if (i < lo || i >= a.fbHi[n]) continue;
+        const double xs = a.fbXs[a.fbOff[n] + (i - lo)];
+        if (lo - 1 <= b0) {
+            gFac += xs * sF[n];
+        } else {
+            const int s = a.fbSlot[n];
+            const double ex = exp((-nu + a.nu[lo - 1]) * kHcRydK / Te);
+            gDir += ex * xs * a.fbG[n] * sX[s][a.fbIon[n] + 1] * a.abund[s];
         }
     }
+    double gHv = ePass * gFac + gDir;
     gHv = gHv * fourPi * kMilne * kHcRyd * factor * 1.e20 * 1.e20;
     // two-photon continua
     const double t2HI = hydro2phot(nu, 1, c.tp, Ne);
@@ -344,17 +341,18 @@ __device__ double em_nlevel(const MocBalanceArgs& b, int atom, double Te, double
     else return nlevel_lines(b, atom, Te, Ne, up, true, scale, 1.9865e9);
 }
 
-// Four waves per SIMD (registers capped at 128): 6.5 ms for the 32^3 nebula
-// against 8.1 ms at the compiler's own 250 VGPRs and two waves.
-template <int NL>
+// The diffuse continuum and its CDF, one workgroup of four waves per cell;
+// the cell's line emissivity comes from k_em_lines.  Registers capped at 128
+// (four waves per SIMD, a few hundred bytes of spills): 3 % faster on the
+// 32^3 nebula than the compiler's 226.
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
 k_emission(MocEmissionArgs a) {
     __shared__ double sX[MOC_MAXE][MOC_MAXS + 1];
     __shared__ double sLoc[3][MOC_EM_MAXLIM];
     __shared__ double sFit[3][MOC_EM_MAXLIM / 2];
     __shared__ double sWave[kThreads / 64];
-    __shared__ double sLines[kThreads / 64];
     __shared__ double sTd[MOC_EM_MAXGRAIN];   // grain temperatures of the cell [species][size]
+    __shared__ double sF[MOC_EM_MAXFB];       // heavy-ion Milne factors of a pass
     const int cell = blockIdx.x + 1;
     if (moc_unchanged(a.TeOld, a.NeOld, a.Te, a.Ne, cell)) return;  // uniform per workgroup
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -424,33 +422,21 @@ k_emission(MocEmissionArgs a) {
         c.ly[3] = pow(10., -0.792 * log10(Te) + 6.01) * Ne * c.XHe3 * c.abHe;
         for (int k = 0; k < 3; ++k) c.ly[k] = a.heiiLyman[2 * k] * c.ly[3];
     }
-    // ---- lines: hydrogenic elements, He I, n-level ions
-    double lines = 0.;
-    // tasks dealt round-robin over the waves so that all four work
-    const int nTasks = a.nHyd + kHeILines + a.nFor;
-    for (int t = lane * (kThreads / 64) + wv; t < nTasks; t += kThreads) {
-        if (t < a.nHyd) {
-            const int s = a.hydSlot[t], z = a.hydZ[t];
-            lines += hydro_element(a, z, sX[s][z + 1], a.abund[s], Te, Ne, log10Ne);
-        } else if (t < a.nHyd + kHeILines) {
-            if (c.heOn) lines += hei_line(a, t - a.nHyd, Te, Ne, NeGrid, c.XHe2, c.abHe);
-        } else {
-            const int k = t - a.nHyd - kHeILines;
-            const int s = a.forSlot[k], ion = a.forIon[k];
-            const double X = sX[s][ion];
-            const double up = (ion < a.forTop[k] && X > 0.) ? sX[s][ion + 1] / X : 0.;
-            lines += em_nlevel<NL>(a.bal, a.forAtom[k], Te, Ne, up, a.abund[s] * X);
-        }
-    }
-    lines = wave_sum(lines);
-    if (lane == 0) sLines[wv] = lines;
-    __syncthreads();
     // ---- continuum CDF: scan of the bin energies, 256 bins per step
     double* row = a.recPDF + (size_t)cell * B;
     double carry = 0.;
     for (int b0 = 0; b0 < B; b0 += kThreads) {
         const int b = b0 + tid;
-        const double v = b < B ? bin_energy(a, c, sLoc, sFit, sX, sTd, b + 1) : 0.;
+        const double nuA = a.nu[b0];
+        for (int n = tid; n < a.nFb; n += kThreads) {
+            const int lo = a.fbLo[n], sl = a.fbSlot[n];
+            sF[n] = lo - 1 <= b0 ? exp((-nuA + a.nu[lo - 1]) * kHcRydK / Te) * a.fbG[n] *
+                                       sX[sl][a.fbIon[n] + 1] * a.abund[sl]
+                                 : 0.;
+        }
+        __syncthreads();
+        const double ePass = b < B ? exp((-a.nu[b] + nuA) * kHcRydK / Te) : 0.;
+        const double v = b < B ? bin_energy(a, c, sLoc, sFit, sX, sTd, b + 1, b0, ePass, sF) : 0.;
         double sc = v;
         for (int off = 1; off < 64; off <<= 1) {
             const double t = __shfl_up(sc, off, 64);
@@ -469,10 +455,51 @@ k_emission(MocEmissionArgs a) {
     const double total = carry;
     for (int b = tid; b < B; b += kThreads) row[b] = b == B - 1 ? 1. : row[b] / total;
     if (tid == 0) {
-        double L = 0.;
-        for (int k = 0; k < kThreads / 64; ++k) L += sLines[k];
+        const double L = a.lineSum[cell];
         a.totalLines[cell] = L / (total + L);
     }
+}
+
+// The lines of a cell (RecLinesEmission :856-1153, forLines :1481-1526): the
+// hydrogenic elements, the 34 He I lines and the n-level ions are separate
+// tasks, one per lane of a single wave per cell; their sum goes to lineSum
+// for the CDF kernel.
+template <int NL>
+__global__ void __launch_bounds__(64) k_em_lines(MocEmissionArgs a) {
+    __shared__ double sX[MOC_MAXE][MOC_MAXS + 1];
+    const int cell = blockIdx.x + 1;
+    if (moc_unchanged(a.TeOld, a.NeOld, a.Te, a.Ne, cell)) return;
+    const int lane = threadIdx.x;
+    const int C1 = a.C1;
+    const double NeGrid = a.Ne[cell], Te = a.Te[cell];
+    const double log10Ne = log10(NeGrid);
+    const double Ne = NeGrid < 1.e-5 ? 1.e-5 : NeGrid;
+    for (int k = lane; k < a.NE * a.NS; k += 64) {
+        const int e = k % a.NE, ion = k / a.NE + 1;
+        sX[e][ion] = a.ionDen[(size_t)k * C1 + cell];
+    }
+    __syncthreads();
+    const bool heOn = a.heSlot >= 0 && a.xsHeI && a.xsHeII;
+    const double abHe = heOn ? a.abund[a.heSlot] : 0.;
+    const double XHe2 = heOn ? sX[a.heSlot][2] : 0.;
+    double lines = 0.;
+    const int nTasks = a.nHyd + kHeILines + a.nFor;
+    for (int t = lane; t < nTasks; t += 64) {
+        if (t < a.nHyd) {
+            const int s = a.hydSlot[t], z = a.hydZ[t];
+            lines += hydro_element(a, z, sX[s][z + 1], a.abund[s], Te, Ne, log10Ne);
+        } else if (t < a.nHyd + kHeILines) {
+            if (heOn) lines += hei_line(a, t - a.nHyd, Te, Ne, NeGrid, XHe2, abHe);
+        } else {
+            const int k = t - a.nHyd - kHeILines;
+            const int s = a.forSlot[k], ion = a.forIon[k];
+            const double X = sX[s][ion];
+            const double up = (ion < a.forTop[k] && X > 0.) ? sX[s][ion + 1] / X : 0.;
+            lines += em_nlevel<NL>(a.bal, a.forAtom[k], Te, Ne, up, a.abund[s] * X);
+        }
+    }
+    lines = wave_sum(lines);
+    if (lane == 0) a.lineSum[cell] = lines;
 }
 
 }  // namespace
@@ -480,9 +507,11 @@ k_emission(MocEmissionArgs a) {
 int moc_launch_emission(moc_ctx* ctx, const MocEmissionArgs& a) {
     if (a.nCells < 1) return MOC_OK;
     const int maxLev = a.bal.maxLev;
-    if (maxLev <= 5) hipLaunchKernelGGL(k_emission<5>, dim3(a.nCells), dim3(kThreads), 0, ctx->stream, a);
-    else if (maxLev <= 8) hipLaunchKernelGGL(k_emission<8>, dim3(a.nCells), dim3(kThreads), 0, ctx->stream, a);
-    else hipLaunchKernelGGL(k_emission<MOC_MAXLEV>, dim3(a.nCells), dim3(kThreads), 0, ctx->stream, a);
+    const dim3 grid((unsigned)a.nCells);
+    if (maxLev <= 5) hipLaunchKernelGGL(k_em_lines<5>, grid, dim3(64), 0, ctx->stream, a);
+    else if (maxLev <= 8) hipLaunchKernelGGL(k_em_lines<8>, grid, dim3(64), 0, ctx->stream, a);
+    else hipLaunchKernelGGL(k_em_lines<MOC_MAXLEV>, grid, dim3(64), 0, ctx->stream, a);
+    hipLaunchKernelGGL(k_emission, grid, dim3(kThreads), 0, ctx->stream, a);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         ctx->err = std::string("emission launch: ") + hipGetErrorString(e);

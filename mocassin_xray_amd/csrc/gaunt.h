@@ -33,6 +33,7 @@ __device__ __forceinline__ double hd(int j, int k) { return c_hummer[(k - 1) * 8
 struct GauntT {
     double c[9];
     double con, gamma2;
+    double lg2;  // log10(gamma2)
 };
 
 __device__ GauntT gaunt_setup(double z, double log10Te) {
@@ -43,6 +44,7 @@ __device__ GauntT gaunt_setup(double z, double log10Te) {
     else xlrkt = 5.1983649 - log10Te;
     const double txg = 0.66666667 * (2.0 * z + xlrkt);
     g.gamma2 = pow(10., txg * 1.5);
+    g.lg2 = log10(g.gamma2);
     g.con = 0.72727273 * xlrkt + 0.90909091;
     for (int j = 1; j <= 8; ++j) {
         double b[12];
@@ -60,8 +62,7 @@ __device__ GauntT gaunt_setup(double z, double log10Te) {
 
 __device__ double gaunt_eval(const GauntT& g, double xlf) {
     const double txu = 0.72727273 * xlf + g.con;
-    const double u = pow(10., (txu - .90909091) / .72727273);
-    if (fabs(txu) <= 2.0) {
+    if (fabs(txu) <= 2.0) {  // the Chebyshev range needs no u
         double b[9];
         b[8] = g.c[8];
         b[7] = txu * b[8] + g.c[7];
@@ -72,18 +73,21 @@ __device__ double gaunt_eval(const GauntT& g, double xlf) {
         }
         return b[1] - b[3];
     }
-    if (log10(u) < -4.0) {
-        if (g.gamma2 < 0.3) return 0.551329 * log(2.24593 / u);
-        return 0.551329 * log(0.944931 / u / sqrt(g.gamma2));
-    }
+    const double u = exp10((txu - .90909091) / .72727273);
+    // txu > 2 puts log10(u) above 1.5, so the small-u branch below never
+    // takes these bins: testing txu first skips a log10 and keeps the result
     if (txu > 2.0) {
-        if (log10(g.gamma2) < -3.) return sqrt(0.9549297 / u);
-        if (log10(g.gamma2) < 0.0) {
+        if (g.lg2 < -3.) return sqrt(0.9549297 / u);
+        if (g.lg2 < 0.0) {
             const double slope = (sqrt(12 * g.gamma2 / u) - sqrt(0.9549297 / u)) / 3.0;
-            return sqrt(12 * g.gamma2 / u) + slope * log10(g.gamma2);
+            return sqrt(12 * g.gamma2 / u) + slope * g.lg2;
         }
         const double v = sqrt(12. * g.gamma2 / u);
         return v > 1.05 ? 1.05 : v;
+    }
+    if (log10(u) < -4.0) {
+        if (g.gamma2 < 0.3) return 0.551329 * log(2.24593 / u);
+        return 0.551329 * log(0.944931 / u / sqrt(g.gamma2));
     }
     return 0.0;
 }

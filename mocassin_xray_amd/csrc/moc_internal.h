@@ -199,6 +199,7 @@ struct MocBalanceArgs {
 // Emission kernel view (emission.hip)
 #define MOC_EM_MAXLIM 64   // points of a gamma fit table
 #define MOC_EM_MAXGRAIN 256 // (species, size) pairs of gas + dust emission
+#define MOC_EM_MAXFB 1024   // heavy-ion free-bound entries
 // the skip test of iterateMC for one cell
 __device__ __forceinline__ bool moc_unchanged(const double* TeOld, const double* NeOld,
                                               const double* Te, const double* Ne, int cell) {
@@ -267,6 +268,7 @@ struct MocEmissionArgs {
     const double* ionDen;    // [NS][NE][C1]
     double* recPDF;          // [C1][B]
     double* totalLines;      // [C1]
+    double* lineSum;         // [C1] line emissivity of each cell (lines kernel -> CDF kernel)
     const double* TeOld;     // skip test as for the opacity (iteration_mod.f90:318-349); NULL: none
     const double* NeOld;
     MocBalanceArgs bal;      // n-level atoms
@@ -361,6 +363,7 @@ struct MocGrid {
     double* d_NeOld = nullptr;
     double* d_recPDF = nullptr;      // [C+1][B]
     double* d_totalLines = nullptr;  // [C+1]
+    double* d_lineSum = nullptr;     // [C+1] line emissivity, between the two emission kernels
     double* d_J = nullptr;           // [B][C+1]
     double* d_escaped = nullptr;     // [B][C+1]
     // FP32 shadow copies for the mixed path (refreshed when dirty)

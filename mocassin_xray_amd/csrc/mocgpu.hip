@@ -58,7 +58,7 @@ void free_grid(MocGrid& g) {
     dfree(g.d_active);
     dfree(g.d_opacity); dfree(g.d_Ne); dfree(g.d_Te); dfree(g.d_Hden);
     dfree(g.d_TeOld); dfree(g.d_NeOld);
-    dfree(g.d_recPDF); dfree(g.d_totalLines); dfree(g.d_J); dfree(g.d_escaped);
+    dfree(g.d_recPDF); dfree(g.d_totalLines); dfree(g.d_J); dfree(g.d_escaped); dfree(g.d_lineSum);
     dfree(g.f_ax); dfree(g.f_ay); dfree(g.f_az);
     dfree(g.f_vx); dfree(g.f_vy); dfree(g.f_vz);
     dfree(g.f_opacity); dfree(g.f_Ne); dfree(g.f_recPDF); dfree(g.f_totalLines);
@@ -490,6 +490,7 @@ int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who
         (rc = dalloc_n(ctx, &g.d_TeOld, C1, "TeOld")) || (rc = dalloc_n(ctx, &g.d_NeOld, C1, "NeOld")) ||
         (rc = dalloc_n(ctx, &g.d_recPDF, C1 * B, "recPDF")) ||
         (rc = dalloc_n(ctx, &g.d_totalLines, C1, "totalLines")) ||
+        (rc = dalloc_n(ctx, &g.d_lineSum, C1, "line sums")) ||
         (rc = dalloc_n(ctx, &g.d_J, C1 * B, "J")) ||
         (rc = dalloc_n(ctx, &g.d_escaped, C1 * B, "escaped")) ||
         (rc = dalloc_n(ctx, &g.f_ax, NX, "ax32")) || (rc = dalloc_n(ctx, &g.f_ay, NY, "ay32")) ||
@@ -1149,7 +1150,7 @@ int moc_set_emission(moc_ctx* ctx, const moc_emission_tables* t) {
         ctx->err = "moc_set_emission: H / He ground-state cross-sections";
         return MOC_ERR_ARG;
     }
-    if (t->j2TsP < 1 || t->j2TsP > B || t->nFb < 0 || t->nHlFiles < 1) {
+    if (t->j2TsP < 1 || t->j2TsP > B || t->nFb < 0 || t->nFb > MOC_EM_MAXFB || t->nHlFiles < 1) {
         ctx->err = "moc_set_emission: invalid line pointers or table sizes";
         return MOC_ERR_ARG;
     }
@@ -1283,6 +1284,7 @@ int run_emission(moc_ctx* ctx, bool skipUnchanged, float* kernel_ms) {
     a.ionDen = g.d_ionDen;
     a.recPDF = g.d_recPDF;
     a.totalLines = g.d_totalLines;
+    a.lineSum = g.d_lineSum;
     a.TeOld = skipUnchanged ? g.d_TeOld : nullptr;
     a.NeOld = skipUnchanged ? g.d_NeOld : nullptr;
     a.nDS = a.nDA = 0;

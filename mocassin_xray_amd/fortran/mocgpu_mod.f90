@@ -135,6 +135,15 @@ module mocgpu_mod
        integer(c_int), value :: ig
        type(moc_grid_desc), intent(in) :: grid
      end function moc_grid_attach
+     ! a batch of independent grids (star k lights grid k; cells numbered grid
+     ! after grid)
+     integer(c_int) function moc_grid_attach_batch(ctx, nGrids, grids) &
+          bind(C, name="moc_grid_attach_batch")
+       import :: c_int, c_ptr, moc_grid_desc
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nGrids
+       type(moc_grid_desc), intent(in) :: grids(*)
+     end function moc_grid_attach_batch
      integer(c_int) function moc_upload(ctx, field, ig, host, count, dtype, layout) &
           bind(C, name="moc_upload")
        import :: c_int, c_ptr, c_int64_t
@@ -241,7 +250,8 @@ module mocgpu_mod
   public :: moc_create, moc_destroy, moc_last_error, moc_version, moc_grid_attach, moc_upload, &
        moc_download, moc_set_atomic, moc_opacity, moc_set_balance, moc_set_emission, &
        moc_emission, moc_set_packets, moc_iterate, moc_update, moc_reset_tallies, moc_transport, &
-       moc_normalize_j, moc_set_dust, moc_dust_opacity, moc_dust_emission, moc_dust_temperature
+       moc_normalize_j, moc_set_dust, moc_dust_opacity, moc_dust_emission, moc_dust_temperature, &
+       moc_grid_attach_batch
   public :: moc_check, c_string, put_field, get_field, ionizationDriver, emissionDriver, &
        energyPacketDriver, updateCell, MCIterationDriver, dustDriver, getDustT
 
