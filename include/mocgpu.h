@@ -290,6 +290,21 @@ const char* moc_last_error(const moc_ctx* ctx);
  * and zeroes them. */
 int moc_grid_attach(moc_ctx* ctx, int ig, const moc_grid_desc* grid);
 
+/* Volume averages of outputGas (output_mod.f90:642-730) over the cells that
+ * packets reached (lgBlack < 1): for each element slot e and stage i,
+ *   TeVol(e, i)     = sum X Te Ne H dV / sum X Ne H dV
+ *   ionDenVol(e, i) = sum X Ne H dV / sum over the element's stages of the same
+ * (Harrington et al. 1982), dV the cell volume (getVolume).  Both outputs
+ * are [nstages][nElemUsed], element fastest (the reference's TeVol(0, elem,
+ * ion) order), 0 above stage min(Z+1, nstages). */
+int moc_volume_averages(moc_ctx* ctx, int ig, double* TeVol, double* ionDenVol);
+
+/* writeSED (output_mod.f90:2947-3064), the all-angle column: sed[nbins] =
+ * the escaped energy of each bin summed over the cells, times 8 for an
+ * octant (lgSymmetricXYZ) model; totalE (may be NULL) = its sum.  nu*F(nu) of
+ * SED.out is sed * nuArray / widFlx. */
+int moc_sed(moc_ctx* ctx, int ig, double* sed, double* totalE);
+
 /* A batch of independent grids in one context (no counterpart in the
  * reference, which runs one nebula per process; `multiGrids` there is nested
  * refinement, not batching): nGrids Cartesian grids that share the frequency

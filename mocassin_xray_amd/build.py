@@ -18,7 +18,8 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libmocgpu.so"
 FORTRAN = PKG / "fortran"
 
-HIP_SOURCES = ["mocgpu.hip", "transport.hip", "opacity.hip", "balance.hip", "emission.hip", "dust.hip"]
+HIP_SOURCES = ["mocgpu.hip", "transport.hip", "opacity.hip", "balance.hip", "emission.hip", "dust.hip",
+               "output.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

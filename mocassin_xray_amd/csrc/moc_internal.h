@@ -364,6 +364,7 @@ struct MocGrid {
     double* d_recPDF = nullptr;      // [C+1][B]
     double* d_totalLines = nullptr;  // [C+1]
     double* d_lineSum = nullptr;     // [C+1] line emissivity, between the two emission kernels
+    double* d_vol = nullptr;         // [C+1] cell volumes [1e45 cm^3] (getVolume)
     double* d_J = nullptr;           // [B][C+1]
     double* d_escaped = nullptr;     // [B][C+1]
     // FP32 shadow copies for the mixed path (refreshed when dirty)
@@ -447,6 +448,10 @@ int moc_launch_dust_temp(moc_ctx* ctx, const MocDustArgs& a, const double* J, co
                          double* R);
 // R[col][cell] = sum_nu J[nu][cell] W[nu][col] (k_rates, balance.hip)
 void moc_launch_rates(moc_ctx* ctx, const double* J, int C1, const double* W, int ncol, double* R);
+int moc_launch_volavg(moc_ctx* ctx, const double* ionDen, const double* Te, const double* Ne,
+                      const double* H, const double* vol, const int* black, int C1, int nPairs,
+                      int nChunks, double* part);
+int moc_launch_sed(moc_ctx* ctx, const double* escaped, int C1, int B, double* sed);
 // the balance over every cell, in chunks of `chunk` cells whose rates fit R ([ncol][chunk])
 int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a, const double* J, const double* W,
                        int ncol, double* R, int chunk);

@@ -59,6 +59,7 @@ void free_grid(MocGrid& g) {
     dfree(g.d_opacity); dfree(g.d_Ne); dfree(g.d_Te); dfree(g.d_Hden);
     dfree(g.d_TeOld); dfree(g.d_NeOld);
     dfree(g.d_recPDF); dfree(g.d_totalLines); dfree(g.d_J); dfree(g.d_escaped); dfree(g.d_lineSum);
+    dfree(g.d_vol);
     dfree(g.f_ax); dfree(g.f_ay); dfree(g.f_az);
     dfree(g.f_vx); dfree(g.f_vy); dfree(g.f_vz);
     dfree(g.f_opacity); dfree(g.f_Ne); dfree(g.f_recPDF); dfree(g.f_totalLines);
@@ -423,6 +424,7 @@ int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who
     g.sym = gds->lgSymmetricXYZ ? 1 : 0;
     g.R_out = gds->R_out;
     std::vector<double> vx, vy, vz;
+    std::vector<double> vol((size_t)total + 1, 0.0);  // getVolume of every cell
     int base = 0;
     for (int k = 0; k < nG; ++k) {
         const moc_grid_desc* gd = gds + k;
@@ -455,6 +457,7 @@ int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who
             const size_t ix = i % nx, iy = (i / nx) % ny, iz = i / ((size_t)nx * ny);
             if (id != 1 + (int)(iz + nz * (iy + ny * ix))) dense = false;
             g.active.push_back(id > 0 ? base + id : 0);
+            if (id > 0) vol[(size_t)base + id] = wx[ix] * wy[iy] * wz[iz];
         }
         g.gdense.push_back(dense ? 1 : 0);
         if (nG > 1) {  // star k lights grid k
@@ -491,6 +494,7 @@ int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who
         (rc = dalloc_n(ctx, &g.d_recPDF, C1 * B, "recPDF")) ||
         (rc = dalloc_n(ctx, &g.d_totalLines, C1, "totalLines")) ||
         (rc = dalloc_n(ctx, &g.d_lineSum, C1, "line sums")) ||
+        (rc = dalloc_n(ctx, &g.d_vol, C1, "cell volumes")) ||
         (rc = dalloc_n(ctx, &g.d_J, C1 * B, "J")) ||
         (rc = dalloc_n(ctx, &g.d_escaped, C1 * B, "escaped")) ||
         (rc = dalloc_n(ctx, &g.f_ax, NX, "ax32")) || (rc = dalloc_n(ctx, &g.f_ay, NY, "ay32")) ||
@@ -512,7 +516,8 @@ int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who
         (rc = h2d(ctx, g.d_vx, vx.data(), NX * 8, "vx")) ||
         (rc = h2d(ctx, g.d_vy, vy.data(), NY * 8, "vy")) ||
         (rc = h2d(ctx, g.d_vz, vz.data(), NZ * 8, "vz")) ||
-        (rc = h2d(ctx, g.d_active, g.active.data(), NA * 4, "active"))) {
+        (rc = h2d(ctx, g.d_active, g.active.data(), NA * 4, "active")) ||
+        (rc = h2d(ctx, g.d_vol, vol.data(), C1 * 8, "cell volumes"))) {
         free_grid(g);
         return rc;
     }
@@ -1527,6 +1532,86 @@ int moc_dust_temperature(moc_ctx* ctx, int ig, moc_update_stats* stats) {
         stats->converged = (double)cnt[1];
         stats->kernel_ms = ms;
     }
+    return MOC_OK;
+}
+
+int moc_volume_averages(moc_ctx* ctx, int ig, double* TeVol, double* ionDenVol) {
+    if (!ctx) return MOC_ERR_ARG;
+    MocGrid& g = ctx->grid;
+    if (!g.attached || ig != 1 || !ctx->atomic.set || !g.d_ionDen) {
+        ctx->err = "moc_volume_averages: needs an attached grid and moc_set_atomic";
+        return MOC_ERR_STATE;
+    }
+    if (!TeVol || !ionDenVol) { ctx->err = "moc_volume_averages: null output"; return MOC_ERR_ARG; }
+    const int NE = ctx->atomic.nElemUsed, NS = ctx->atomic.nstages, C1 = g.nCells + 1;
+    const int nPairs = NE * NS;
+    int nChunks = (g.nCells + 4095) / 4096;
+    nChunks = nChunks < 1 ? 1 : (nChunks > 256 ? 256 : nChunks);
+    double* d_part = nullptr;
+    int rc;
+    if ((rc = dalloc_n(ctx, &d_part, (size_t)nPairs * nChunks * 2, "volume-average partials"))) return rc;
+    rc = moc_launch_volavg(ctx, g.d_ionDen, g.d_Te, g.d_Ne, g.d_Hden, g.d_vol, g.d_black, C1, nPairs,
+                           nChunks, d_part);
+    std::vector<double> part((size_t)nPairs * nChunks * 2);
+    if (!rc) {
+        const hipError_t e1 = hipMemcpyAsync(part.data(), d_part, part.size() * 8, hipMemcpyDeviceToHost,
+                                             ctx->stream);
+        const hipError_t e2 = hipStreamSynchronize(ctx->stream);
+        if (e1 != hipSuccess || e2 != hipSuccess) {
+            ctx->err = std::string("moc_volume_averages: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2);
+            rc = MOC_ERR_HIP;
+        }
+    }
+    dfree(d_part);
+    if (rc) return rc;
+    // sums over the chunks in order, then Harrington et al. (1982): <X> over
+    // the element's stages, <Te> weighted by the ion (:682-730)
+    std::vector<double> T((size_t)nPairs, 0.0), W((size_t)nPairs, 0.0);
+    for (int p = 0; p < nPairs; ++p)
+        for (int c = 0; c < nChunks; ++c) {
+            T[p] += part[((size_t)p * nChunks + c) * 2];
+            W[p] += part[((size_t)p * nChunks + c) * 2 + 1];
+        }
+    for (int e = 0; e < NE; ++e) {
+        const int top = std::min(ctx->atomic.elemZ[e] + 1, NS);
+        double den = 0.0;
+        for (int ion = 1; ion <= top; ++ion) den += W[(size_t)(ion - 1) * NE + e];
+        for (int ion = 1; ion <= NS; ++ion) {
+            const size_t k = (size_t)(ion - 1) * NE + e;
+            if (ion > top) { TeVol[k] = 0.0; ionDenVol[k] = 0.0; continue; }
+            TeVol[k] = W[k] <= 0.0 ? 0.0 : T[k] / W[k];
+            ionDenVol[k] = den <= 0.0 ? 0.0 : W[k] / den;
+        }
+    }
+    return MOC_OK;
+}
+
+int moc_sed(moc_ctx* ctx, int ig, double* sed, double* totalE) {
+    if (!ctx) return MOC_ERR_ARG;
+    MocGrid& g = ctx->grid;
+    if (!g.attached || ig != 1) { ctx->err = "moc_sed: no grid attached"; return MOC_ERR_STATE; }
+    if (!sed) { ctx->err = "moc_sed: null output"; return MOC_ERR_ARG; }
+    const int B = ctx->nbins;
+    double* d_sed = nullptr;
+    int rc;
+    if ((rc = dalloc_n(ctx, &d_sed, (size_t)B, "SED"))) return rc;
+    rc = moc_launch_sed(ctx, g.d_escaped, g.nCells + 1, B, d_sed);
+    if (!rc) {
+        const hipError_t e1 = hipMemcpyAsync(sed, d_sed, (size_t)B * 8, hipMemcpyDeviceToHost, ctx->stream);
+        const hipError_t e2 = hipStreamSynchronize(ctx->stream);
+        if (e1 != hipSuccess || e2 != hipSuccess) {
+            ctx->err = std::string("moc_sed: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2);
+            rc = MOC_ERR_HIP;
+        }
+    }
+    dfree(d_sed);
+    if (rc) return rc;
+    double tot = 0.0;
+    for (int b = 0; b < B; ++b) {
+        if (g.sym) sed[b] = sed[b] * 8.;  // the octant's tallies for the whole nebula (:3008)
+        tot += sed[b];
+    }
+    if (totalE) *totalE = tot;
     return MOC_OK;
 }
 

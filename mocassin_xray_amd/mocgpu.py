@@ -21,6 +21,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
+from ._lib import _dp
 from ._lib import (CELLBIN_FIELDS, FIELDS, GRAIN_FIELDS, MOC_DT_F64, MOC_DT_I32,
                    MOC_LAYOUT_FORTRAN, MOC_PREC_FP64, MOC_PREC_MIXED, MocAtomic, MocBalanceTables, MocDustTables,
                    MocEmissionTables, MocGridDesc, MocIterationStats, MocTables, MocTransportStats,
@@ -315,6 +316,25 @@ class MocGPU:
         st = MocUpdateStats()
         self._check(self._L.moc_update(self._ctx, 1, C.byref(st)), "moc_update")
         return st.as_dict()
+
+    # ----------------------------------------------------------------- output
+    def volume_averages(self):
+        """outputGas volume averages (output_mod.f90:642-730): (TeVol,
+        ionDenVol), each (nElemUsed, nstages), over the cells packets reached."""
+        ne, ns = self.n_elem, self.nstages
+        T = np.zeros(ne * ns)
+        X = np.zeros(ne * ns)
+        self._check(self._L.moc_volume_averages(self._ctx, 1, T.ctypes.data_as(_dp),
+                                                X.ctypes.data_as(_dp)), "moc_volume_averages")
+        return T.reshape(ns, ne).T.copy(), X.reshape(ns, ne).T.copy()
+
+    def sed(self):
+        """writeSED (output_mod.f90:2947-3064): escaped energy per bin summed
+        over the cells (x 8 for an octant model) and its total."""
+        out = np.zeros(self.nbins)
+        tot = C.c_double(0.0)
+        self._check(self._L.moc_sed(self._ctx, 1, out.ctypes.data_as(_dp), C.byref(tot)), "moc_sed")
+        return out, tot.value
 
     # ------------------------------------------------------------------- dust
     def set_dust(self, dt, lgGas: bool, XHILimit: float = 0.05, photoelectric_xsec=None) -> None:

@@ -1,0 +1,30 @@
+/* ORACLE -- test infrastructure only (see moc_oracle_output.c). */
+#ifndef MOC_ORACLE_OUTPUT_H
+#define MOC_ORACLE_OUTPUT_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_output_in {
+    int nx, ny, nz, nCells, sym;
+    const double *xAxis, *yAxis, *zAxis; /* [n] cm */
+    const int32_t* active;               /* [nx*ny*nz], x fastest */
+    int NE, NS;
+    const int32_t* elemZ;                /* [NE] */
+    const double* Te;                    /* [C+1] */
+    const double* Ne;                    /* [C+1] */
+    const double* Hden;                  /* [C+1] */
+    const double* ionDen;                /* [NS][NE][C+1] */
+    const int32_t* lgBlack;              /* [C+1] or NULL */
+} oracle_output_in;
+
+/* TeVol, ionDenVol: [NS][NE], element fastest */
+int oracle_volume_averages(const oracle_output_in* in, double* TeVol, double* ionDenVol);
+/* sed: [nbins]; escaped: [nbins][C+1]; returns the total */
+double oracle_sed(const double* escaped, int nCells, int nbins, int sym, double* sed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

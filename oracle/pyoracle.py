@@ -535,3 +535,52 @@ def dust_temperature(dt, nu, widflx, Ndust, Tdust, J, dustOnly=True, XHILimit=0.
     lib().oracle_dust_temperature(C.byref(inp), j.ctypes.data_as(_dp), td.ctypes.data_as(_dp),
                                   conv.ctypes.data_as(_ip), black.ctypes.data_as(_ip), cnt)
     return td, conv, black, cnt[0], cnt[1]
+
+
+# ---- output reductions (outputGas volume averages, writeSED) ----------------
+class OutputIn(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nCells", C.c_int), ("sym", C.c_int),
+        ("xAxis", _dp), ("yAxis", _dp), ("zAxis", _dp), ("active", _ip), ("NE", C.c_int),
+        ("NS", C.c_int), ("elemZ", _ip), ("Te", _dp), ("Ne", _dp), ("Hden", _dp), ("ionDen", _dp),
+        ("lgBlack", _ip),
+    ]
+
+
+def volume_averages(grid, elem_z, Te, Ne, Hden, ionDen, lgBlack=None):
+    """outputGas volume averages: (TeVol, ionDenVol), each (nElemUsed, nstages);
+    ionDen (C+1, nElemUsed, nstages)."""
+    keep = {}
+
+    def d(name, a, dt=np.float64, ptr=_dp):
+        keep[name] = np.ascontiguousarray(a, dtype=dt)
+        return keep[name].ctypes.data_as(ptr)
+
+    X = np.asarray(ionDen, dtype=np.float64)
+    C1, NE, NS = X.shape
+    inp = OutputIn(nx=len(grid.xAxis), ny=len(grid.yAxis), nz=len(grid.zAxis), nCells=C1 - 1,
+                   sym=int(bool(grid.lgSymmetricXYZ)), xAxis=d("x", grid.xAxis),
+                   yAxis=d("y", grid.yAxis), zAxis=d("z", grid.zAxis),
+                   active=d("a", np.asarray(grid.active).reshape(-1, order="F"), np.int32, _ip),
+                   NE=NE, NS=NS, elemZ=d("ez", elem_z, np.int32, _ip), Te=d("te", Te),
+                   Ne=d("ne", Ne), Hden=d("h", Hden), ionDen=d("X", X.transpose(2, 1, 0)),
+                   lgBlack=None if lgBlack is None else d("bl", lgBlack, np.int32, _ip))
+    T = np.zeros(NS * NE)
+    W = np.zeros(NS * NE)
+    f = lib().oracle_volume_averages
+    f.restype = C.c_int
+    rc = f(C.byref(inp), T.ctypes.data_as(_dp), W.ctypes.data_as(_dp))
+    assert rc == 0
+    return T.reshape(NS, NE).T.copy(), W.reshape(NS, NE).T.copy()
+
+
+def sed(escaped, sym):
+    """writeSED all-angle column: (sed[B], total); escaped (C+1, B)."""
+    E = np.ascontiguousarray(np.asarray(escaped, dtype=np.float64).T)
+    B, C1 = E.shape
+    out = np.zeros(B)
+    f = lib().oracle_sed
+    f.restype = C.c_double
+    f.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _dp]
+    tot = f(E.ctypes.data_as(_dp), C1 - 1, B, int(bool(sym)), out.ctypes.data_as(_dp))
+    return out, tot
