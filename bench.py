@@ -100,6 +100,7 @@ def run_path(m, prec: str, steps: int, warmup: int):
         if big_model(m):               # (cell, bin) planes of tens of GB stay on the device
             keys = ["Te", "Ne", "ionDen"]
         state = {k: g.download(k) for k in keys}
+        state["volavg"] = g.volume_averages()
     return dt, stats, state
 
 
@@ -129,6 +130,12 @@ def state_error(m, a: dict, b: dict) -> dict:
     big = xb > 1e-3
     xrel = float(np.max(np.abs(xa[big] - xb[big]) / xb[big]))
     out = {"Te_volavg_rel": float(te), "ion_frac_volavg_max_rel": xrel, "J_cell_rel_rms": None}
+    if "volavg" in a and "volavg" in b:
+        # outputGas definitions: <Te> per ion and <X> (Harrington et al. 1982)
+        (Ta_i, Xa_i), (Tb_i, Xb_i) = a["volavg"], b["volavg"]
+        on = Xb_i > 1e-3
+        out["ionic_Te_max_rel"] = float(np.max(np.abs(Ta_i[on] - Tb_i[on]) / Tb_i[on]))
+        out["ionic_frac_max_rel"] = float(np.max(np.abs(Xa_i[on] - Xb_i[on]) / Xb_i[on]))
     if "Jste" in a and "Jste" in b:
         Ja, Jb = a["Jste"][1:].sum(axis=1), b["Jste"][1:].sum(axis=1)
         hit = (Ja + Jb) > 0
