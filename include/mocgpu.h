@@ -299,6 +299,19 @@ int moc_grid_attach(moc_ctx* ctx, int ig, const moc_grid_desc* grid);
  * ion) order), 0 above stage min(Z+1, nstages). */
 int moc_volume_averages(moc_ctx* ctx, int ig, double* TeVol, double* ionDenVol);
 
+/* Line luminosities of outputGas (output_mod.f90:397-432): over the cells
+ * packets reached, the sum of H dV times the emissivity of every line of every
+ * n-level ion (forLines, emission_mod.f90:1481-1526: equilibrium with the
+ * recombination feed, times abundance, ionDen and 1.9865e9), and Hbeta = the
+ * sum of hydroLines(1,4,2) H dV over the cells where that exceeds 1e-35.  The
+ * ions are those with atomic data, Z then stage ascending; nIons and nLev (the
+ * level count, maxLev) come back first; with lines non-NULL, lines[(k * nLev
+ * + i) * nLev + j] is ion k's line from level j+1 down to level i+1 (i < j;
+ * other entries 0), count >= nIons * nLev * nLev, and ionZ / ionStage (may be
+ * NULL) name the ions.  Lines relative to Hbeta are lines / Hbeta. */
+int moc_line_luminosities(moc_ctx* ctx, int ig, int* nIons, int* nLev, int* ionZ, int* ionStage,
+                          double* lines, int64_t count, double* Hbeta);
+
 /* writeSED (output_mod.f90:2947-3064), the all-angle column: sed[nbins] =
  * the escaped energy of each bin summed over the cells, times 8 for an
  * octant (lgSymmetricXYZ) model; totalE (may be NULL) = its sum.  nu*F(nu) of

@@ -36,7 +36,7 @@ EXPORTS = [
     "moc_device_ptr", "moc_version", "moc_set_atomic", "moc_opacity", "moc_set_balance",
     "moc_update", "moc_set_emission", "moc_emission", "moc_set_packets", "moc_iterate",
     "moc_set_dust", "moc_dust_opacity", "moc_dust_emission", "moc_dust_temperature",
-    "moc_grid_attach_batch", "moc_volume_averages", "moc_sed",
+    "moc_grid_attach_batch", "moc_volume_averages", "moc_sed", "moc_line_luminosities",
 ]
 
 _dp = C.POINTER(C.c_double)

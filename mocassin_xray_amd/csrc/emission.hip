@@ -32,6 +32,7 @@ constexpr double kCRyd = 3.2898423e15;     // continuum_mod.f90:16
 constexpr double kHPlanck = 6.6262e-27;    // constants_mod.f90:14
 constexpr double kMilne = 4.9874105e-6;    // fb_ff :515
 constexpr int kThreads = 256;
+constexpr int kLineLumLds = 49152;  // line accumulators of one ion group
 
 // hydroLinesTemps (grid_mod.f90:76-77)
 __constant__ double c_hlT[12] = {0.05, 0.10, 0.30, 0.50, 0.75, 1.0, 1.25, 1.50, 2.0, 3.0, 5.0, 10.0};
@@ -111,9 +112,31 @@ __device__ double hl_lines(const MocEmissionArgs& a, int f, double Ne, double Hb
     return acc;
 }
 
-// hydrogenic recombination lines of element izp (RecLinesEmission :890-1099)
-__device__ double hydro_element(const MocEmissionArgs& a, int izp, double Xup, double ab, double Te,
-                                double Ne, double log10Ne) {
+// one line (iup -> ilow) of table file f interpolated in density, times Hb
+__device__ double hl_line(const MocEmissionArgs& a, int f, double Ne, double Hb, int iup, int ilow) {
+    const double* dens = a.hlDens + (size_t)f * 14;
+    const double* L = a.hlLines + (size_t)f * 14 * 16 * 9;
+    const int idenp = hl_idenp(a, f, Ne);
+    const double* L0 = L + (size_t)(idenp - 1) * 16 * 9;
+    if ((dens[12] > 0. && idenp < 13) || (dens[12] == 0. && idenp < 9)) {
+        const double* L1 = L0 + 16 * 9;
+        const double w = (Ne - dens[idenp - 1]) / (dens[idenp] - dens[idenp - 1]);
+        const double l0 = L0[iup * 9 + ilow];
+        return (l0 + (L1[iup * 9 + ilow] - l0) * w) * Hb;
+    }
+    return L0[iup * 9 + ilow] * Hb;
+}
+
+struct HydroLines {
+    int f;       // table file
+    double Ne;   // density the table is read at
+    double Hb;   // Hbeta emissivity the table is scaled by
+};
+
+// hydrogenic recombination lines of element izp (RecLinesEmission :890-1099):
+// the table and the scale; hl_lines sums them, hl_line picks one
+__device__ HydroLines hydro_setup(const MocEmissionArgs& a, int izp, double Xup, double ab, double Te,
+                                  double Ne, double log10Ne) {
     const double T4 = Te / 10000.;
     const int elUp = T4 > c_hlT[11] ? 2 : 8;
     double Hb;
@@ -163,7 +186,7 @@ __device__ double hydro_element(const MocEmissionArgs& a, int izp, double Xup, d
             Hb = pow(10., yy1 - (yy1 - yy2) * (xx - x1) / (x2 - x1));
         }
         Hb = Hb * Ne * Xup * ab;
-        return hl_lines(a, f, Ne, Hb);
+        return HydroLines{f, Ne, Hb};
     }
     // heavier hydrogenic ions: He II tables at the Z-scaled T and n_e
     const double z2 = (double)(izp * izp);
@@ -176,7 +199,13 @@ __device__ double hydro_element(const MocEmissionArgs& a, int izp, double Xup, d
     const double Bf = a.hbB[4] * exp(-zf * zf / 2.) + a.hbB[7];
     double Hb2 = ((double)(izp * izp * izp) / 8.) * pow(10., Af + Bf * lTeZ);
     Hb2 = Hb2 * Ne * Xup * ab;
-    return hl_lines(a, f, Nez, Hb2);
+    return HydroLines{f, Nez, Hb2};
+}
+
+__device__ double hydro_element(const MocEmissionArgs& a, int izp, double Xup, double ab, double Te,
+                                double Ne, double log10Ne) {
+    const HydroLines h = hydro_setup(a, izp, Xup, ab, Te, Ne, log10Ne);
+    return hl_lines(a, h.f, h.Ne, h.Hb);
 }
 
 // He I recombination line i of the 34 (:1110-1152); the T4 clamps are those
@@ -502,7 +531,91 @@ __global__ void __launch_bounds__(64) k_em_lines(MocEmissionArgs a) {
     if (lane == 0) a.lineSum[cell] = lines;
 }
 
+// Line luminosities of outputGas (output_mod.f90:397-432): over the cells
+// packets reached, sum of each n-level line's emissivity (forLines, per
+// transition) times H dV, and of the Hbeta emissivity times H dV where that
+// exceeds 1e-35.  One wave per block walks a chunk of cells; lane k owns
+// n-level ion k0 + k (and k + 64, ...) of the launch's ion group and
+// accumulates its lines in LDS; the block writes its sums, which the host adds
+// block by block.  The first group also carries Hbeta.
+template <int NL>
+__global__ void __launch_bounds__(64) k_line_lum(MocEmissionArgs a, const double* __restrict__ vol,
+                                                 const int* __restrict__ black,
+                                                 const double* __restrict__ H, int nChunks, int L,
+                                                 int k0, int nk, double* __restrict__ part) {
+    __shared__ double sX[MOC_MAXE][MOC_MAXS + 1];
+    extern __shared__ double acc[];  // [nk][L][L]: ions k0 .. k0+nk-1
+    const int lane = threadIdx.x;
+    const int C1 = a.C1, nL = nk * L * L;
+    for (int k = lane; k < nL; k += 64) acc[k] = 0.;
+    double hb = 0.;
+    const int per = (C1 - 1 + nChunks - 1) / nChunks;
+    const int c0 = 1 + blockIdx.x * per;
+    const int c1 = c0 + per < C1 ? c0 + per : C1;
+    const int hs = a.hSlot;
+    for (int cell = c0; cell < c1; ++cell) {
+        if (black && black[cell] > 0) continue;
+        __syncthreads();  // sX of the previous cell is no longer read
+        for (int k = lane; k < a.NE * a.NS; k += 64) {
+            const int e = k % a.NE, ion = k / a.NE + 1;
+            sX[e][ion] = a.ionDen[(size_t)k * C1 + cell];
+        }
+        __syncthreads();
+        const double NeGrid = a.Ne[cell], Te = a.Te[cell];
+        const double log10Ne = log10(NeGrid);
+        const double Ne = NeGrid < 1.e-5 ? 1.e-5 : NeGrid;
+        const double w = H[cell] * vol[cell];
+        for (int kk = lane; kk < nk; kk += 64) {
+            const int k = k0 + kk;
+            const int s = a.forSlot[k], ion = a.forIon[k];
+            const double X = sX[s][ion];
+            const double up = (ion < a.forTop[k] && X > 0.) ? sX[s][ion + 1] / X : 0.;
+            if constexpr (NL <= 8)
+                nlevel_fixed<NL>(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9,
+                                 acc + (size_t)kk * L * L, w, L);
+            else
+                nlevel_lines(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9,
+                             acc + (size_t)kk * L * L, w, L);
+        }
+        if (lane == 0 && k0 == 0 && hs >= 0 && a.NS > 1) {  // hydroLines(1,4,2) (RecLinesEmission)
+            const HydroLines h = hydro_setup(a, 1, sX[hs][2], a.abund[hs], Te, Ne, log10Ne);
+            const double v = hl_line(a, h.f, h.Ne, h.Hb, 4, 2) * H[cell] * vol[cell];
+            if (v > 1.e-35) hb += v;
+        }
+    }
+    __syncthreads();
+    double* out = part + (size_t)blockIdx.x * (nL + 1);
+    for (int k = lane; k < nL; k += 64) out[k] = acc[k];
+    if (lane == 0) out[nL] = hb;
+}
+
 }  // namespace
+
+int moc_line_lum_group(int L) {
+    const int g = kLineLumLds / (L * L * (int)sizeof(double));
+    return g < 1 ? 1 : g;
+}
+
+int moc_launch_line_lum(moc_ctx* ctx, const MocEmissionArgs& a, const double* vol, const int* black,
+                        const double* H, int nChunks, int L, int k0, int nk, double* part) {
+    const size_t lds = (size_t)nk * L * L * sizeof(double);
+    if (nk < (a.nFor > 0 ? 1 : 0) || k0 < 0 || k0 + nk > a.nFor || lds > (size_t)kLineLumLds) {
+        ctx->err = "line luminosities: ion group " + std::to_string(k0) + "+" + std::to_string(nk) +
+                   " of " + std::to_string(a.nFor) + " does not fit";
+        return MOC_ERR_ARG;
+    }
+    const int maxLev = a.bal.maxLev;
+    const dim3 grid((unsigned)nChunks), block(64);
+    if (maxLev <= 5) hipLaunchKernelGGL(k_line_lum<5>, grid, block, lds, ctx->stream, a, vol, black, H, nChunks, L, k0, nk, part);
+    else if (maxLev <= 8) hipLaunchKernelGGL(k_line_lum<8>, grid, block, lds, ctx->stream, a, vol, black, H, nChunks, L, k0, nk, part);
+    else hipLaunchKernelGGL(k_line_lum<MOC_MAXLEV>, grid, block, lds, ctx->stream, a, vol, black, H, nChunks, L, k0, nk, part);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        ctx->err = std::string("line luminosities launch: ") + hipGetErrorString(e);
+        return MOC_ERR_HIP;
+    }
+    return MOC_OK;
+}
 
 int moc_launch_emission(moc_ctx* ctx, const MocEmissionArgs& a) {
     if (a.nCells < 1) return MOC_OK;

@@ -452,6 +452,11 @@ int moc_launch_volavg(moc_ctx* ctx, const double* ionDen, const double* Te, cons
                       const double* H, const double* vol, const int* black, int C1, int nPairs,
                       int nChunks, double* part);
 int moc_launch_sed(moc_ctx* ctx, const double* escaped, int C1, int B, double* sed);
+// Forbidden-line luminosities of ions k0 .. k0+nk-1 (nk <= moc_line_lum_group(L))
+// and, with k0 == 0, Hbeta: part[block][nk*L*L + 1].
+int moc_line_lum_group(int L);
+int moc_launch_line_lum(moc_ctx* ctx, const MocEmissionArgs& a, const double* vol, const int* black,
+                        const double* H, int nChunks, int L, int k0, int nk, double* part);
 // the balance over every cell, in chunks of `chunk` cells whose rates fit R ([ncol][chunk])
 int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a, const double* J, const double* W,
                        int ncol, double* R, int chunk);

@@ -1586,6 +1586,76 @@ int moc_volume_averages(moc_ctx* ctx, int ig, double* TeVol, double* ionDenVol) 
     return MOC_OK;
 }
 
+int moc_line_luminosities(moc_ctx* ctx, int ig, int* nIons, int* nLev, int* ionZ, int* ionStage,
+                          double* lines, int64_t count, double* Hbeta) {
+    if (!ctx) return MOC_ERR_ARG;
+    MocGrid& g = ctx->grid;
+    if (!g.attached || ig != 1 || !ctx->emission.set || !ctx->balance.set || !g.d_ionDen) {
+        ctx->err = "moc_line_luminosities: needs an attached grid, moc_set_balance and moc_set_emission";
+        return MOC_ERR_STATE;
+    }
+    MocEmissionArgs a = ctx->emission.a;
+    a.bal = ctx->balance.a;  // the n-level atoms
+    const int L = a.bal.maxLev;
+    if (L < 1) { ctx->err = "moc_line_luminosities: no n-level atoms (maxLev < 1)"; return MOC_ERR_STATE; }
+    if (nIons) *nIons = a.nFor;
+    if (nLev) *nLev = L;
+    if (!lines) return MOC_OK;
+    const int64_t nL = (int64_t)a.nFor * L * L;
+    if (count < nL) { ctx->err = "moc_line_luminosities: output shorter than nIons*nLev*nLev"; return MOC_ERR_ARG; }
+    if (ionZ || ionStage) {
+        std::vector<int> slot(a.nFor), ion(a.nFor);
+        if (a.nFor > 0) {
+            MOC_TRY(hipMemcpy(slot.data(), a.forSlot, a.nFor * 4, hipMemcpyDeviceToHost), "ion list");
+            MOC_TRY(hipMemcpy(ion.data(), a.forIon, a.nFor * 4, hipMemcpyDeviceToHost), "ion list");
+        }
+        for (int k = 0; k < a.nFor; ++k) {
+            if (ionZ) ionZ[k] = ctx->atomic.elemZ[slot[k]];
+            if (ionStage) ionStage[k] = ion[k];
+        }
+    }
+    a.nCells = g.nCells;
+    a.C1 = g.nCells + 1;
+    a.Te = g.d_Te; a.Ne = g.d_Ne; a.ionDen = g.d_ionDen;
+    int nChunks = (g.nCells + 127) / 128;
+    nChunks = nChunks < 1 ? 1 : (nChunks > 1024 ? 1024 : nChunks);
+    // ions in groups whose accumulators fit a workgroup's LDS; Hbeta with the first
+    const int grp = moc_line_lum_group(L);
+    const int nGroups = a.nFor > 0 ? (a.nFor + grp - 1) / grp : 1;
+    const int64_t stride = (int64_t)(a.nFor < grp ? a.nFor : grp) * L * L + 1;
+    double* d_part = nullptr;
+    int rc;
+    if ((rc = dalloc_n(ctx, &d_part, (size_t)nChunks * stride, "line partials"))) return rc;
+    std::vector<double> part((size_t)nChunks * stride);
+    for (int64_t k = 0; k < nL; ++k) lines[k] = 0.0;
+    double hb = 0.0;
+    for (int gi = 0; gi < nGroups && !rc; ++gi) {
+        const int k0 = gi * grp;
+        const int nk = a.nFor - k0 < grp ? a.nFor - k0 : grp;
+        const int64_t n = (int64_t)nk * L * L;
+        rc = moc_launch_line_lum(ctx, a, g.d_vol, g.d_black, g.d_Hden, nChunks, L, k0, nk, d_part);
+        if (rc) break;
+        const hipError_t e1 = hipMemcpyAsync(part.data(), d_part, (size_t)nChunks * (n + 1) * 8,
+                                             hipMemcpyDeviceToHost, ctx->stream);
+        const hipError_t e2 = hipStreamSynchronize(ctx->stream);
+        if (e1 != hipSuccess || e2 != hipSuccess) {
+            ctx->err = std::string("moc_line_luminosities: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2);
+            rc = MOC_ERR_HIP;
+            break;
+        }
+        double* out = lines + (int64_t)k0 * L * L;
+        for (int c = 0; c < nChunks; ++c) {  // blocks in order: the sum does not depend on timing
+            const double* pc = part.data() + (size_t)c * (n + 1);
+            for (int64_t k = 0; k < n; ++k) out[k] += pc[k];
+            if (gi == 0) hb += pc[n];
+        }
+    }
+    dfree(d_part);
+    if (rc) return rc;
+    if (Hbeta) *Hbeta = hb;
+    return MOC_OK;
+}
+
 int moc_sed(moc_ctx* ctx, int ig, double* sed, double* totalE) {
     if (!ctx) return MOC_ERR_ARG;
     MocGrid& g = ctx->grid;

@@ -9,13 +9,16 @@
 namespace {
 
 // Sum of the line emissivities A_ji dE_ij n_j * scale * unit of ion `atom`
-// that exceed 1e-35.  With rec, levels 2.. are fed by recombination from the
+// that exceed 1e-35.  With lineAcc, every line's emissivity (lower i, upper
+// j: fLineEm(i, j), :2987-3005) times w is also added to lineAcc[i * ld + j].
+// With rec, levels 2.. are fed by recombination from the
 // next stage (ionDenUp = X(i+1)/X(i)) at the rates of the ion's data file
 // (:2828-2877; kind 1: power-law radiative plus dielectronic fit per level,
 // kind 2: cubic in log T).  Upsilon(T) is the natural cubic spline in log T
 // with second derivatives computed on the host, clamped at the table ends.
 __device__ double nlevel_lines(const MocBalanceArgs& a, int atom, double Te, double Ne,
-                               double ionDenUp, bool rec, double scale, double unit) {
+                               double ionDenUp, bool rec, double scale, double unit,
+                               double* lineAcc = nullptr, double w = 0., int ld = 0) {
     const int64_t* m = a.atomMeta + 8 * atom;
     const int nf = (int)m[0], nT = (int)m[1];
     const double* lt = a.atomLogT + m[2];
@@ -121,6 +124,7 @@ __device__ double nlevel_lines(const MocBalanceArgs& a, int atom, double Te, dou
             if (j + 1 == iup2p && i == 0 && atp > 0.) av -= atp;
             const double em = av * (en[j] - en[i]) * (y[j] / sumN) * scale * unit;
             if (em > 1.e-35) tot += em;
+            if (lineAcc) lineAcc[i * ld + j] += em * w;
         }
     return tot;
 }
@@ -137,7 +141,8 @@ namespace {
 // recombination feed when rec is set.
 template <int NL>
 __device__ double nlevel_fixed(const MocBalanceArgs& a, int atom, double Te, double Ne,
-                               double ionDenUp, bool rec, double scale, double unit) {
+                               double ionDenUp, bool rec, double scale, double unit,
+                               double* lineAcc = nullptr, double w = 0., int ld = 0) {
     const int64_t* m = a.atomMeta + 8 * atom;
     const int nf = (int)m[0], nT = (int)m[1];
     const double* lt = a.atomLogT + m[2];
@@ -271,6 +276,7 @@ __device__ double nlevel_fixed(const MocBalanceArgs& a, int atom, double Te, dou
             if (j + 1 == iup2p && i == 0 && atp > 0.) av -= atp;
             const double em = av * (en[j] - en[i]) * (y[j] / sumN) * scale * unit;
             if (em > 1.e-35) tot += em;
+            if (lineAcc) lineAcc[i * ld + j] += em * w;
         }
     }
     return tot;

@@ -245,13 +245,39 @@ module mocgpu_mod
        integer(c_int), value :: ig
        type(moc_update_stats), intent(out) :: stats
      end function moc_dust_temperature
+     ! outputGas volume averages: TeVol, ionDenVol (nElemUsed, nstages)
+     integer(c_int) function moc_volume_averages(ctx, ig, TeVol, ionDenVol) &
+          bind(C, name="moc_volume_averages")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ig
+       real(c_double), intent(out) :: TeVol(*), ionDenVol(*)
+     end function moc_volume_averages
+     ! outputGas line luminosities (lines(j, i, k): ion k, upper level j,
+     ! lower level i) and Hbeta; lines as c_null_ptr returns the sizes only
+     integer(c_int) function moc_line_luminosities(ctx, ig, nIons, nLev, ionZ, ionStage, lines, &
+          count, Hbeta) bind(C, name="moc_line_luminosities")
+       import :: c_int, c_ptr, c_int64_t, c_double
+       type(c_ptr), value :: ctx, ionZ, ionStage, lines
+       integer(c_int), value :: ig
+       integer(c_int), intent(out) :: nIons, nLev
+       integer(c_int64_t), value :: count
+       real(c_double), intent(out) :: Hbeta
+     end function moc_line_luminosities
+     ! writeSED, all-angle column
+     integer(c_int) function moc_sed(ctx, ig, sed, totalE) bind(C, name="moc_sed")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ig
+       real(c_double), intent(out) :: sed(*), totalE
+     end function moc_sed
   end interface
 
   public :: moc_create, moc_destroy, moc_last_error, moc_version, moc_grid_attach, moc_upload, &
        moc_download, moc_set_atomic, moc_opacity, moc_set_balance, moc_set_emission, &
        moc_emission, moc_set_packets, moc_iterate, moc_update, moc_reset_tallies, moc_transport, &
        moc_normalize_j, moc_set_dust, moc_dust_opacity, moc_dust_emission, moc_dust_temperature, &
-       moc_grid_attach_batch
+       moc_grid_attach_batch, moc_volume_averages, moc_line_luminosities, moc_sed
   public :: moc_check, c_string, put_field, get_field, ionizationDriver, emissionDriver, &
        energyPacketDriver, updateCell, MCIterationDriver, dustDriver, getDustT
 

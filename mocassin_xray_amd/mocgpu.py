@@ -21,7 +21,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import _dp
+from ._lib import _dp, _ip
 from ._lib import (CELLBIN_FIELDS, FIELDS, GRAIN_FIELDS, MOC_DT_F64, MOC_DT_I32,
                    MOC_LAYOUT_FORTRAN, MOC_PREC_FP64, MOC_PREC_MIXED, MocAtomic, MocBalanceTables, MocDustTables,
                    MocEmissionTables, MocGridDesc, MocIterationStats, MocTables, MocTransportStats,
@@ -327,6 +327,24 @@ class MocGPU:
         self._check(self._L.moc_volume_averages(self._ctx, 1, T.ctypes.data_as(_dp),
                                                 X.ctypes.data_as(_dp)), "moc_volume_averages")
         return T.reshape(ns, ne).T.copy(), X.reshape(ns, ne).T.copy()
+
+    def line_luminosities(self):
+        """outputGas line luminosities (output_mod.f90:397-432): (lines, ions,
+        Hbeta) with lines[k, i, j] ion k's line from level j+1 to level i+1
+        summed over the reached cells times H dV, ions[k] = (Z, stage)."""
+        n, L = C.c_int(0), C.c_int(0)
+        self._check(self._L.moc_line_luminosities(self._ctx, 1, C.byref(n), C.byref(L), None, None,
+                                                  None, C.c_int64(0), None), "moc_line_luminosities")
+        nI, nL = n.value, L.value
+        lines = np.zeros(max(nI * nL * nL, 1))
+        z = np.zeros(max(nI, 1), dtype=np.int32)
+        st = np.zeros(max(nI, 1), dtype=np.int32)
+        hb = C.c_double(0.0)
+        self._check(self._L.moc_line_luminosities(
+            self._ctx, 1, C.byref(n), C.byref(L), z.ctypes.data_as(_ip), st.ctypes.data_as(_ip),
+            lines.ctypes.data_as(_dp), C.c_int64(lines.size), C.byref(hb)), "moc_line_luminosities")
+        return (lines[:nI * nL * nL].reshape(nI, nL, nL), list(zip(z[:nI].tolist(), st[:nI].tolist())),
+                hb.value)
 
     def sed(self):
         """writeSED (output_mod.f90:2947-3064): escaped energy per bin summed

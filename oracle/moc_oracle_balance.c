@@ -133,7 +133,8 @@ static double splint_(const double* xa, const double* ya, const double* y2, int 
  * as forLines/thermBalance do: every line emissivity A_ji dE_ij n_j, scaled
  * by abundance * ionDen * 1.9865e-16, that exceeds 1e-35 [erg/s per n_H] */
 static double nlevel_lines(const oracle_balance_tables* t, int atom, double Te, double Ne,
-                           double ionDenUp, int rec, double scale, double unit) {
+                           double ionDenUp, int rec, double scale, double unit, double* lineEm,
+                           int ld) {
     const int64_t* m = t->atom_meta + 7 * atom;
     const int nfile = (int)m[0], ntemp = (int)m[1];
     const double* lt = t->atom_logt + m[2];
@@ -233,6 +234,7 @@ static double nlevel_lines(const oracle_balance_tables* t, int atom, double Te, 
                 if (j == iup2p && i == 1 && atp > 0.) a -= atp;
                 em = a * (e[j - 1] - e[i - 1]) * nn[j] * scale * unit;
                 if (em > 1.e-35) cool += em;
+                if (lineEm) lineEm[(i - 1) * ld + (j - 1)] = em; /* fLineEm(i, j) (:2999) */
             }
         }
 #undef AA
@@ -241,12 +243,18 @@ static double nlevel_lines(const oracle_balance_tables* t, int atom, double Te, 
 
 static double nlevel_cooling(const oracle_balance_tables* t, int atom, double Te, double Ne,
                              double scale) {
-    return nlevel_lines(t, atom, Te, Ne, 0., 0, scale, 1.9865e-16);
+    return nlevel_lines(t, atom, Te, Ne, 0., 0, scale, 1.9865e-16, NULL, 0);
 }
 
 double oracle_nlevel_lines(const oracle_balance_tables* t, int atom, double Te, double Ne,
                            double ionDenUp, int rec, double scale, double unit) {
-    return nlevel_lines(t, atom, Te, Ne, ionDenUp, rec, scale, unit);
+    return nlevel_lines(t, atom, Te, Ne, ionDenUp, rec, scale, unit, NULL, 0);
+}
+
+double oracle_nlevel_line_em(const oracle_balance_tables* t, int atom, double Te, double Ne,
+                             double ionDenUp, int rec, double scale, double unit, double* lineEm,
+                             int ld) {
+    return nlevel_lines(t, atom, Te, Ne, ionDenUp, rec, scale, unit, lineEm, ld);
 }
 
 /* ---- per-cell state ------------------------------------------------------ */

@@ -96,6 +96,11 @@ double oracle_nlevel_cooling(const oracle_balance_tables* t, int atom, double Te
  * are fed by recombination from the next stage (ionDenUp = X(i+1)/X(i)). */
 double oracle_nlevel_lines(const oracle_balance_tables* t, int atom, double Te, double Ne,
                            double ionDenUp, int rec, double scale, double unit);
+/* the same, with every line's emissivity fLineEm(i, j) (lower i, upper j,
+ * 1-based levels) also stored in lineEm[(i-1) * ld + (j-1)] */
+double oracle_nlevel_line_em(const oracle_balance_tables* t, int atom, double Te, double Ne,
+                             double ionDenUp, int rec, double scale, double unit, double* lineEm,
+                             int ld);
 #ifdef __cplusplus
 }
 #endif

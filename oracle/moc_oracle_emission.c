@@ -40,6 +40,7 @@
 #include "moc_oracle_balance.h"
 #include "moc_oracle_emission.h"
 #include "moc_oracle_dust.h"
+#include "moc_oracle_output.h"
 
 #define PI_REF 3.141592654   /* constants_mod.f90 */
 #define HCRYD 2.1799153e-11  /* continuum_mod.f90:15 */
@@ -294,8 +295,24 @@ static void continuum(const oracle_emission_tables* t, const prep_t* p, const ce
 }
 
 /* RecLinesEmission :856-1153, summed as setDiffusePDF :1772-1782 */
+/* one line of table file f interpolated in density, times Hb */
+static double hl_line(const oracle_emission_tables* t, int f, double Ne, double Hb, int iup, int ilow) {
+    const double* dens = t->hlDens + (size_t)f * 14;
+    const double* L = t->hlLines + (size_t)f * 14 * 16 * 9;
+    const int idenp = hl_idenp(t, f, Ne);
+#define HL(id, u, l) L[(((size_t)((id)-1)) * 16 + (u)) * 9 + (l)]
+    if ((dens[12] > 0. && idenp < 13) || (dens[12] == 0. && idenp < 9)) {
+        const double w = (Ne - dens[idenp - 1]) / (dens[idenp] - dens[idenp - 1]);
+        return (HL(idenp, iup, ilow) + (HL(idenp + 1, iup, ilow) - HL(idenp, iup, ilow)) * w) * Hb;
+    }
+    return HL(idenp, iup, ilow) * Hb;
+#undef HL
+}
+
+/* RecLinesEmission (:856-1153) summed; hbeta (may be NULL) receives
+ * hydroLines(1,4,2) */
 static double rec_lines(const oracle_emission_tables* t, const cellcomp_t* c, double Te, double Ne,
-                        double NeGrid, double log10Ne) {
+                        double NeGrid, double log10Ne, double* hbeta) {
     const int NS = t->bal->nstages;
     const double T4 = Te / 10000.;
     const int elUp = T4 > HL_TEMPS[11] ? 2 : 8;
@@ -349,6 +366,7 @@ static double rec_lines(const oracle_emission_tables* t, const cellcomp_t* c, do
         }
         Hb = Hb * Ne * c->X[izp][izp + 1] * c->ab[izp];
         normRec += hl_lines(t, f, Ne, Hb);
+        if (izp == 1 && hbeta) *hbeta = hl_line(t, f, Ne, Hb, 4, 2);
     }
     for (izp = elUp + 1; izp <= 30; izp++) {
         double T4z, lTeZ, Nez, lNez, Hb, zf, Af, Bf;
@@ -446,7 +464,7 @@ static void emission_one(const oracle_emission_tables* t, const prep_t* p, oracl
     int i;
     cell_comp(t->bal, s->ionDen, (size_t)C1, (size_t)cell, &c);
     continuum(t, p, &c, Te, Ne, w);
-    normRec = rec_lines(t, &c, Te, Ne, NeGrid, log10Ne);
+    normRec = rec_lines(t, &c, Te, Ne, NeGrid, log10Ne, NULL);
     normFor = for_lines(t, &c, Te, Ne);
     /* setDiffusePDF :1593-1693 */
     for (i = 0; i < B; i++) {
@@ -543,5 +561,53 @@ int oracle_emission_coeffs(const oracle_emission_tables* t, double Te, double Ne
     cell_comp(t->bal, X, 1, 0, &c);
     continuum(t, &p, &c, Te, Ne < 1.e-5 ? 1.e-5 : Ne, out);
     prep_free(&p);
+    return 0;
+}
+
+/* ---- line luminosities of outputGas (output_mod.f90:397-432) ------------- */
+int oracle_line_luminosities(const oracle_emission_tables* t, const oracle_emission_state* s,
+                             const oracle_output_in* g, int L, double* lines, double* hbeta) {
+    const oracle_balance_tables* b = t->bal;
+    const int NS = b->nstages;
+    const size_t C1 = (size_t)s->nCells + 1;
+    double* em = (double*)calloc((size_t)L * L, sizeof(double));
+    int i, j, k, z, ion, q;
+    if (!em) return -1;
+    for (q = 0; q < 31 * 32 * L * L; q++) lines[q] = 0.;
+    *hbeta = 0.;
+    for (i = 0; i < g->nx; i++)
+        for (j = 0; j < g->ny; j++)
+            for (k = 0; k < g->nz; k++) {
+                const int cell = g->active[(size_t)i + (size_t)g->nx * ((size_t)j + (size_t)g->ny * k)];
+                cellcomp_t c;
+                double Te, NeGrid, Ne, dV, H, hb = 0.;
+                if (cell <= 0) continue;
+                if (g->lgBlack && g->lgBlack[cell] >= 1) continue;
+                cell_comp(b, s->ionDen, C1, (size_t)cell, &c);
+                Te = s->Te[cell];
+                NeGrid = s->Ne[cell];
+                Ne = NeGrid < 1.e-5 ? 1.e-5 : NeGrid;
+                H = g->Hden[cell];
+                dV = oracle_cell_width(g->xAxis, g->nx, i, g->sym) *
+                     oracle_cell_width(g->yAxis, g->ny, j, g->sym) *
+                     oracle_cell_width(g->zAxis, g->nz, k, g->sym);
+                (void)rec_lines(t, &c, Te, Ne, NeGrid, log10(NeGrid), &hb);
+                if (hb * H * dV > 1.e-35) *hbeta += hb * H * dV;
+                for (z = 3; z <= 30; z++) {
+                    const int top = z + 1 < NS ? z + 1 : NS;
+                    if (!c.on[z]) continue;
+                    for (ion = 1; ion <= top; ion++) {
+                        const int a = b->atom_idx[z * 32 + ion];
+                        double up = 0.;
+                        double* out = lines + (size_t)(z * 32 + ion) * L * L;
+                        if (a < 0) continue;
+                        if (ion < top && c.X[z][ion] > 0.) up = c.X[z][ion + 1] / c.X[z][ion];
+                        for (q = 0; q < L * L; q++) em[q] = 0.;
+                        oracle_nlevel_line_em(b, a, Te, Ne, up, 1, c.ab[z] * c.X[z][ion], 1.9865e9, em, L);
+                        for (q = 0; q < L * L; q++) out[q] += em[q] * H * dV;
+                    }
+                }
+            }
+    free(em);
     return 0;
 }

@@ -10,6 +10,7 @@
 #include <stdint.h>
 
 #include "moc_oracle_balance.h"
+#include "moc_oracle_output.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -68,6 +69,13 @@ typedef struct oracle_emission_state {
 
 /* returns 0, or -1 when two gamma edges share a bin (fb_ff stops there) */
 int oracle_emission(const oracle_emission_tables* t, oracle_emission_state* s, int nthreads);
+/* outputGas line luminosities (output_mod.f90:397-432) over the cells of g
+ * (reference loop order, lgBlack < 1): lines[(Z*32 + stage)][L][L], entry
+ * (i, j) = sum of fLineEm(i+1, j+1) H dV; hbeta = sum of hydroLines(1,4,2) H dV
+ * over the cells where that exceeds 1e-35.  s supplies Te, Ne, ionDen; g the
+ * grid, Hden and lgBlack. */
+int oracle_line_luminosities(const oracle_emission_tables* t, const oracle_emission_state* s,
+                             const oracle_output_in* g, int L, double* lines, double* hbeta);
 
 /* the per-bin emission coefficients of one cell [1e-40 erg cm^3/s/Hz per n_H]:
  * out[0..B) HI (fb+ff+2q + heavy fb), [B..2B) HeI, [2B..3B) HeII, and the

@@ -20,7 +20,7 @@
 
 #include "moc_oracle_output.h"
 
-static double width(const double* a, int n, int i, int sym) {
+double oracle_cell_width(const double* a, int n, int i, int sym) {
     double d;
     if (i > 0 && i < n - 1) d = fabs(a[i + 1] - a[i - 1]) / 2.;
     else if (i == 0) d = sym ? fabs(a[1] - a[0]) / 2. : fabs(a[1] - a[0]);
@@ -42,8 +42,9 @@ int oracle_volume_averages(const oracle_output_in* in, double* TeVol, double* io
                 double dV, f1, f2;
                 if (c <= 0) continue;
                 if (in->lgBlack && in->lgBlack[c] >= 1) continue;
-                dV = width(in->xAxis, in->nx, i, in->sym) * width(in->yAxis, in->ny, j, in->sym) *
-                     width(in->zAxis, in->nz, k, in->sym);
+                dV = oracle_cell_width(in->xAxis, in->nx, i, in->sym) *
+                     oracle_cell_width(in->yAxis, in->ny, j, in->sym) *
+                     oracle_cell_width(in->zAxis, in->nz, k, in->sym);
                 f1 = in->Te[c] * in->Ne[c] * in->Hden[c] * dV;
                 f2 = in->Ne[c] * in->Hden[c] * dV;
                 for (e = 0; e < NE; e++)

@@ -21,6 +21,8 @@ typedef struct oracle_output_in {
 
 /* TeVol, ionDenVol: [NS][NE], element fastest */
 int oracle_volume_averages(const oracle_output_in* in, double* TeVol, double* ionDenVol);
+/* width of cell i along one axis / 1e15 (getVolume, grid_mod.f90:2926-2985) */
+double oracle_cell_width(const double* a, int n, int i, int sym);
 /* sed: [nbins]; escaped: [nbins][C+1]; returns the total */
 double oracle_sed(const double* escaped, int nCells, int nbins, int sym, double* sed);
 

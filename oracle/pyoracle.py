@@ -547,9 +547,8 @@ class OutputIn(C.Structure):
     ]
 
 
-def volume_averages(grid, elem_z, Te, Ne, Hden, ionDen, lgBlack=None):
-    """outputGas volume averages: (TeVol, ionDenVol), each (nElemUsed, nstages);
-    ionDen (C+1, nElemUsed, nstages)."""
+def output_in(grid, elem_z, Te, Ne, Hden, ionDen, lgBlack=None):
+    """(OutputIn, arrays it points into); ionDen (C+1, nElemUsed, nstages)."""
     keep = {}
 
     def d(name, a, dt=np.float64, ptr=_dp):
@@ -565,6 +564,14 @@ def volume_averages(grid, elem_z, Te, Ne, Hden, ionDen, lgBlack=None):
                    NE=NE, NS=NS, elemZ=d("ez", elem_z, np.int32, _ip), Te=d("te", Te),
                    Ne=d("ne", Ne), Hden=d("h", Hden), ionDen=d("X", X.transpose(2, 1, 0)),
                    lgBlack=None if lgBlack is None else d("bl", lgBlack, np.int32, _ip))
+    return inp, keep
+
+
+def volume_averages(grid, elem_z, Te, Ne, Hden, ionDen, lgBlack=None):
+    """outputGas volume averages: (TeVol, ionDenVol), each (nElemUsed, nstages);
+    ionDen (C+1, nElemUsed, nstages)."""
+    inp, keep = output_in(grid, elem_z, Te, Ne, Hden, ionDen, lgBlack)
+    NE, NS = inp.NE, inp.NS
     T = np.zeros(NS * NE)
     W = np.zeros(NS * NE)
     f = lib().oracle_volume_averages
@@ -584,3 +591,21 @@ def sed(escaped, sym):
     f.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _dp]
     tot = f(E.ctypes.data_as(_dp), C1 - 1, B, int(bool(sym)), out.ctypes.data_as(_dp))
     return out, tot
+
+
+def line_luminosities(et, bt, elem_z, abund, nstages, grid, Te, Ne, Hden, ionDen, L,
+                      lgBlack=None):
+    """outputGas line luminosities: (lines, Hbeta) with lines[Z, stage] the
+    (L, L) array whose [i, j] is the line from level j+1 to level i+1 summed
+    over the reached cells times Hden dV (zero for ions without atomic data)."""
+    t, k = emission_tables(et, bt, elem_z, abund, nstages)
+    inp, keep = output_in(grid, elem_z, Te, Ne, Hden, ionDen, lgBlack)
+    st = EmissionStateC(nCells=inp.nCells, Te=inp.Te, Ne=inp.Ne, ionDen=inp.ionDen)
+    out = np.zeros((31 * 32, L, L))
+    hb = C.c_double(0.0)
+    f = lib().oracle_line_luminosities
+    f.argtypes = [C.POINTER(EmissionTablesC), C.POINTER(EmissionStateC), C.POINTER(OutputIn),
+                  C.c_int, _dp, C.POINTER(C.c_double)]
+    f.restype = C.c_int
+    assert f(C.byref(t), C.byref(st), C.byref(inp), int(L), out.ctypes.data_as(_dp), C.byref(hb)) == 0
+    return out.reshape(31, 32, L, L), hb.value

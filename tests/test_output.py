@@ -105,3 +105,82 @@ def test_gpu_sed_of_a_transparent_nebula_carries_the_luminosity():
         s, tot = g.sed()
     assert ts["escaped"] == 4000 and ts["absorbed"] == 0
     assert tot == pytest.approx(float(m.tables.Lstar[0]), rel=1e-12)
+
+
+def atom_arrays(bt, z, ion):
+    """Level energies [cm^-1] and A values of an ion's data file."""
+    d = bt.data
+    atom = int(d["atom_idx"][z, ion])
+    nl, nt, ot, ol, oa, oq, _ = (int(v) for v in d["atom_meta"][atom])
+    return atom, d["atom_e"][ol:ol + nl], d["atom_A"][oa:oa + nl * nl].reshape(nl, nl)
+
+
+def test_line_luminosities_pinned(oracle):
+    """One reached cell: the O III lines sum to its n-level emissivity (itself
+    pinned against a numpy level solve) times H dV; two lines from one upper
+    level stand in the ratio A dE (the populations cancel); only lower < upper
+    entries are filled; Hbeta against case B (Osterbrock & Ferland 2006
+    Table 4.2) for a cell at 1e4 K."""
+    m, Te, Ne, H, X = small_state()
+    C1 = m.grid.nCells + 1
+    L = m.bt.maxLev
+    black = np.ones(C1, dtype=np.int32)
+    black[5] = 0
+    Te[5], Ne[5] = 1.e4, 100.
+    h = m.elems.index(1)
+    X[5, h, :] = 0.0
+    X[5, h, 1] = 1.0                                   # hydrogen fully ionised
+    lines, hb = oracle.line_luminosities(m.et, m.bt, m.elems, m.abund, 7, m.grid, Te, Ne, H, X, L,
+                                         lgBlack=black)
+    dV = T.cell_volumes(m.grid.xAxis, m.grid.yAxis, m.grid.zAxis, m.grid.active, True)[5]
+    o = m.elems.index(8)
+    atom, e, A = atom_arrays(m.bt, 8, 3)
+    x3 = X[5, o, 2]
+    tot = oracle.nlevel_lines(m.bt, atom, Te[5], Ne[5], X[5, o, 3] / x3, 1, m.abund[o] * x3, 1.9865e9)
+    O3 = lines[8, 3]
+    assert O3.sum() == pytest.approx(tot * H[5] * dV, rel=1e-12)
+    assert np.all(np.tril(O3) == 0.0) and O3[0, 1] > 0.0
+    # [O III] 4959 (1D2 -> 3P1) / 5007 (1D2 -> 3P2)
+    assert O3[1, 3] / O3[2, 3] == pytest.approx(A[3, 1] * (e[3] - e[1]) / (A[3, 2] * (e[3] - e[2])),
+                                                rel=1e-13)
+    # every ion with data has lines; none of the others
+    have = {(z, i) for z in range(3, 31) for i in range(1, 8) if lines[z, i].any()}
+    want = {(z, i) for z in m.elems if z >= 3 for i in range(1, min(z + 1, 7) + 1)
+            if m.bt.data["atom_idx"][z, i] >= 0 and X[5, m.elems.index(z), i - 1] > 0}
+    assert have == want
+    assert hb / (Ne[5] * H[5] * dV) * 1e-25 == pytest.approx(1.24e-25, rel=0.03)
+    # a second reached cell adds its own contribution
+    black[9] = 0
+    l2, hb2 = oracle.line_luminosities(m.et, m.bt, m.elems, m.abund, 7, m.grid, Te, Ne, H, X, L,
+                                       lgBlack=black)
+    black[5] = 1
+    l9, hb9 = oracle.line_luminosities(m.et, m.bt, m.elems, m.abund, 7, m.grid, Te, Ne, H, X, L,
+                                       lgBlack=black)
+    np.testing.assert_allclose(l2, lines + l9, rtol=1e-13, atol=0)
+    assert hb2 == pytest.approx(hb + hb9, rel=1e-13)
+
+
+@pytest.mark.gpu
+def test_gpu_line_luminosities_match_oracle(oracle):
+    from mocassin_xray_amd import MocGPU
+    m, Te, Ne, H, X = small_state(seed=5)
+    print("case line luminosities neb6", flush=True)
+    with MocGPU(m.tables, "fp64") as g:
+        m.load(g)
+        for k, v in (("Te", Te), ("Ne", Ne), ("Hden", H), ("ionDen", X)):
+            g.upload(k, v)
+        g.iterateMC(1, 5)
+        black = g.download_flags("lgBlack")
+        state = {k: g.download(k) for k in ("Te", "Ne", "Hden", "ionDen")}
+        lines, ions, hb = g.line_luminosities()
+    L = lines.shape[1]
+    lo, hbo = oracle.line_luminosities(m.et, m.bt, m.elems, m.abund, 7, m.grid, state["Te"],
+                                       state["Ne"], state["Hden"], state["ionDen"], L, lgBlack=black)
+    assert len(ions) > 0 and len(set(ions)) == len(ions)
+    for k, (z, st) in enumerate(ions):
+        np.testing.assert_allclose(lines[k], lo[z, st], rtol=1e-10, atol=1e-300)
+    seen = np.zeros(lo.shape[:2], dtype=bool)
+    for z, st in ions:
+        seen[z, st] = True
+    assert not lo[~seen].any()
+    assert hb == pytest.approx(hbo, rel=1e-10) and hb > 0
