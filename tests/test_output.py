@@ -8,10 +8,10 @@ from mocassin_xray_amd import nebula
 from mocassin_xray_amd import tables as T
 
 
-def small_state(seed=3):
+def small_state(seed=3, maxLev=5):
     m = nebula.gas_model("neb6", 6, 1.e18, 1000., nebula.NEBULA_ELEMS,
                          10.0 ** np.array(nebula.NEBULA_LOG_ABUND), 7, 200, 100.0, 1.e5,
-                         10 ** 1.5, 1000)
+                         10 ** 1.5, 1000, maxLev=maxLev)
     rng = np.random.default_rng(seed)
     C1 = m.grid.nCells + 1
     X = rng.uniform(0.0, 1.0, size=(C1, 7, 7))
@@ -161,10 +161,13 @@ def test_line_luminosities_pinned(oracle):
 
 
 @pytest.mark.gpu
-def test_gpu_line_luminosities_match_oracle(oracle):
+@pytest.mark.parametrize("maxLev", [5, 27])
+def test_gpu_line_luminosities_match_oracle(oracle, maxLev):
+    """maxLev 27: the ions' accumulators no longer fit one workgroup's LDS
+    together, so they are reduced in several ion groups."""
     from mocassin_xray_amd import MocGPU
-    m, Te, Ne, H, X = small_state(seed=5)
-    print("case line luminosities neb6", flush=True)
+    m, Te, Ne, H, X = small_state(seed=5, maxLev=maxLev)
+    print(f"case line luminosities neb6 maxLev {maxLev}", flush=True)
     with MocGPU(m.tables, "fp64") as g:
         m.load(g)
         for k, v in (("Te", Te), ("Ne", Ne), ("Hden", H), ("ionDen", X)):
