@@ -5,7 +5,9 @@
  *   HeI2photSub :620-817, free_free / ffCoeff :818-855, RecLinesEmission
  *   :856-1153, forLines :1481-1526 (equilibrium with rec=.true.),
  *   setDiffusePDF :1528-1919 (gas only), with the edge pointers of
- *   initGammaCont (ph_mod.f90:31-202).
+ *   initGammaCont (ph_mod.f90:31-202); and the line luminosities of
+ *   outputGas (output_mod.f90:397-432): per-transition n-level lines and
+ *   Hbeta, times Hden dV, summed over the reached cells.
  * Output per cell: the diffuse-continuum CDF recPDF(cell, 1:nbins) and the
  * line fraction totalLines(cell).  Differences from the reference, each
  * deliberate:
@@ -28,7 +30,10 @@
  * to 1s), its hydrogenic scaling to He II and C VI, the two-photon energy per
  * decay, the free-free constant, the Hbeta emissivity, the gamma-fit
  * interpolation at a tabulated T, the recombination feed of the n-level
- * solve and the CDF assembly.  Parity unpinned: the He I recombination-line
+ * solve and the CDF assembly; by tests/test_output.py: the line
+ * luminosities (per-ion sums against the pinned n-level emissivity, ratios
+ * of lines from one upper level against A dE, Hbeta against case B, and
+ * additivity over cells).  Parity unpinned: the He I recombination-line
  * fits, the hydrogenic line tables above 3e4 K (rbEdge / r2aEdge branches)
  * and the free-bound of non-hydrogenic heavy ions.
  */
