@@ -296,7 +296,9 @@ int moc_grid_attach(moc_ctx* ctx, int ig, const moc_grid_desc* grid);
  *   ionDenVol(e, i) = sum X Ne H dV / sum over the element's stages of the same
  * (Harrington et al. 1982), dV the cell volume (getVolume).  Both outputs
  * are [nstages][nElemUsed], element fastest (the reference's TeVol(0, elem,
- * ion) order), 0 above stage min(Z+1, nstages). */
+ * ion) order), 0 above stage min(Z+1, nstages).  One nebula per context:
+ * a batch of grids (moc_grid_attach_batch) returns MOC_ERR_STATE, as do the
+ * line luminosities and the SED. */
 int moc_volume_averages(moc_ctx* ctx, int ig, double* TeVol, double* ionDenVol);
 
 /* Line luminosities of outputGas (output_mod.f90:397-432): over the cells

@@ -1542,6 +1542,10 @@ int moc_volume_averages(moc_ctx* ctx, int ig, double* TeVol, double* ionDenVol) 
         ctx->err = "moc_volume_averages: needs an attached grid and moc_set_atomic";
         return MOC_ERR_STATE;
     }
+    if (g.nGrids > 1) {  // one nebula's output: sums over a batch would mix its grids
+        ctx->err = "moc_volume_averages: the context holds a batch of grids; reduce one nebula per context";
+        return MOC_ERR_STATE;
+    }
     if (!TeVol || !ionDenVol) { ctx->err = "moc_volume_averages: null output"; return MOC_ERR_ARG; }
     const int NE = ctx->atomic.nElemUsed, NS = ctx->atomic.nstages, C1 = g.nCells + 1;
     const int nPairs = NE * NS;
@@ -1592,6 +1596,10 @@ int moc_line_luminosities(moc_ctx* ctx, int ig, int* nIons, int* nLev, int* ionZ
     MocGrid& g = ctx->grid;
     if (!g.attached || ig != 1 || !ctx->emission.set || !ctx->balance.set || !g.d_ionDen) {
         ctx->err = "moc_line_luminosities: needs an attached grid, moc_set_balance and moc_set_emission";
+        return MOC_ERR_STATE;
+    }
+    if (g.nGrids > 1) {  // one nebula's output: sums over a batch would mix its grids
+        ctx->err = "moc_line_luminosities: the context holds a batch of grids; reduce one nebula per context";
         return MOC_ERR_STATE;
     }
     MocEmissionArgs a = ctx->emission.a;
@@ -1660,6 +1668,10 @@ int moc_sed(moc_ctx* ctx, int ig, double* sed, double* totalE) {
     if (!ctx) return MOC_ERR_ARG;
     MocGrid& g = ctx->grid;
     if (!g.attached || ig != 1) { ctx->err = "moc_sed: no grid attached"; return MOC_ERR_STATE; }
+    if (g.nGrids > 1) {  // one nebula's output: sums over a batch would mix its grids
+        ctx->err = "moc_sed: the context holds a batch of grids; reduce one nebula per context";
+        return MOC_ERR_STATE;
+    }
     if (!sed) { ctx->err = "moc_sed: null output"; return MOC_ERR_ARG; }
     const int B = ctx->nbins;
     double* d_sed = nullptr;

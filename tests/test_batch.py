@@ -89,6 +89,9 @@ def test_gpu_batch_single_star_transport(oracle):
         ts = g.energyPacketDriver(2, 3000, seed=77, ctr_base=9)
         J = g.download("Jste")
         op, rec, tl = g.download("opacity"), g.download("recPDF"), g.download("totalLines")
+        for reduce in (g.volume_averages, g.sed, g.line_luminosities):  # one nebula per output
+            with pytest.raises(RuntimeError, match="batch of grids"):
+                reduce()
     b, n = m.cellBase[1], m.grids[1].nCells
     outside = np.r_[np.arange(1, b + 1), np.arange(b + n + 1, J.shape[0])]
     assert np.all(J[outside] == 0.0) and J[b + 1: b + n + 1].sum() > 0
