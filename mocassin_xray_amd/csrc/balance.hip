@@ -103,6 +103,7 @@ __device__ __forceinline__ double t3(const double* a, int i, int z, int n) {
 // lanes run over k = e * S2 + stage (S2 = NS + 2) and skip stages past the end.
 struct Lds {
     double *X, *alpha, *cI1, *out, *cool, *diO;
+    double* nlw;  // workspace of the wave-solved n-level ions
     int* hot;
     int* off;  // [NE + 1]
     int S2;
@@ -112,9 +113,10 @@ struct Lds {
     __device__ double& c1(int e, int i) const { return cI1[off[e] + i]; }
 };
 
-__host__ __device__ inline size_t balance_lds_bytes(int nX, int nNlv) {
+__host__ __device__ inline size_t balance_lds_bytes(int nX, int nNlv, int nBig) {
     const size_t rows = (size_t)4 * nX;
-    return (rows + (size_t)(nNlv > 0 ? nNlv : 1) + MOC_MAXS + 1) * sizeof(double) +
+    return (rows + (size_t)(nNlv > 0 ? nNlv : 1) + MOC_MAXS + 1 + (size_t)nlevel_wave_doubles(nBig)) *
+               sizeof(double) +
            (MOC_MAXE + 2) * sizeof(int);
 }
 
@@ -601,22 +603,25 @@ __device__ void g_terms(const MocBalanceArgs& a, const Lds& L, const GrainLds& G
     G.cg[g] = cg;
 }
 
-// n-level cooling of one ion, register-resident solve when the level count fits NL
-template <int NL>
-__device__ double ion_cooling(const MocBalanceArgs& a, int atom, double T, double Ne, double scale) {
-    if constexpr (NL <= 8) return nlevel_fixed<NL>(a, atom, T, Ne, 0., false, scale, 1.9865e-16);
-    else return nlevel_lines(a, atom, T, Ne, 0., false, scale, 1.9865e-16);
-}
-
 // thermBalance (:887-1215): heating and cooling [erg/s per n_H]
 template <int NL, class RT>
 __device__ void therm(const MocBalanceArgs& a, const RT& r, const Lds& L, double T, double Ne,
                       double H, double heatef, const double comp[4], double* heat, double* cool,
                       int tid, double grainHeat = 0., double grainCool = 0., bool grains = false) {
-    // forbidden-line cooling, one ion with data per lane
+    // forbidden-line cooling: ions of up to NL levels one per lane with the
+    // matrix in registers, larger ones by the whole wave in LDS
     for (int k = tid; k < a.nNlv; k += 64) {
+        const int atom = a.nlvAtom[k];
+        if (nlevel_count(a, atom) > NL) continue;
         const int e = a.nlvSlot[k], ion = a.nlvIon[k];
-        L.cool[k] = ion_cooling<NL>(a, a.nlvAtom[k], T, Ne, a.abund[e] * L.x(e, ion));
+        L.cool[k] = nlevel_fixed<NL>(a, atom, T, Ne, 0., false, a.abund[e] * L.x(e, ion), 1.9865e-16);
+    }
+    for (int b = 0; b < a.nNlvBig; ++b) {
+        const int k = a.nlvBig[b];
+        const int e = a.nlvSlot[k], ion = a.nlvIon[k];
+        const double v = nlevel_wave(a, a.nlvAtom[k], T, Ne, 0., false, a.abund[e] * L.x(e, ion), 1.9865e-16,
+                                     L.nlw, tid);
+        if (tid == 0) L.cool[k] = v;
     }
     __syncthreads();
     double coolFF = 0., coolRec = 0., coolColl, coolCollH = 0., heatSte = 0., Te4 = T / 1.e4;
@@ -699,7 +704,7 @@ k_balance(MocBalanceArgs a) {
     Rates<ST> r{a.R, a.ncol, (int)blockIdx.x, a.specOf, a.specShell, a.shInfo, a.shAug};
     if constexpr (ST) {
         // stage the cell's rate columns and the index tables after the state rows
-        double* Rl = lds + balance_lds_bytes(a.nX, a.nNlv) / sizeof(double);
+        double* Rl = lds + balance_lds_bytes(a.nX, a.nNlv, a.nBig) / sizeof(double);
         int* iL = reinterpret_cast<int*>(Rl + a.ncol);
         int* sOf = iL;
         int* sSh = sOf + NE * 32;
@@ -732,7 +737,8 @@ k_balance(MocBalanceArgs a) {
     L.out = L.cI1 + a.nX;
     L.cool = L.out + a.nX;
     L.diO = L.cool + (a.nNlv > 0 ? a.nNlv : 1);
-    L.hot = (int*)(L.diO + MOC_MAXS + 1);
+    L.nlw = L.diO + MOC_MAXS + 1;
+    L.hot = (int*)(L.nlw + nlevel_wave_doubles(a.nBig));
     L.off = L.hot + 1;
     if (tid <= NE) L.off[tid] = a.xoff[tid];
     __syncthreads();
@@ -744,7 +750,7 @@ k_balance(MocBalanceArgs a) {
     const int nG = PE ? a.nGS * a.nGA : 0;
     GrainLds G{};
     if constexpr (PE) {
-        size_t off = balance_lds_bytes(a.nX, a.nNlv);
+        size_t off = balance_lds_bytes(a.nX, a.nNlv, a.nBig);
         if constexpr (ST) off += balance_stage_bytes(a);
         G.q = lds + off / sizeof(double);
         G.X0 = G.q + a.B;
@@ -853,18 +859,16 @@ k_balance(MocBalanceArgs a) {
     }
 }
 
-// the n-level register solve is sized by the largest level count in use
+// the per-lane register solve is sized by the level counts in use (at most 8;
+// larger ions go to the wave solve)
 template <bool PE>
 void launch_balance(moc_ctx* ctx, const MocBalanceArgs& a, bool st, dim3 grid, dim3 block, size_t lds) {
     if (a.maxLev <= 5) {
         if (st) hipLaunchKernelGGL((k_balance<5, true, PE>), grid, block, lds, ctx->stream, a);
         else hipLaunchKernelGGL((k_balance<5, false, PE>), grid, block, lds, ctx->stream, a);
-    } else if (a.maxLev <= 8) {
+    } else {
         if (st) hipLaunchKernelGGL((k_balance<8, true, PE>), grid, block, lds, ctx->stream, a);
         else hipLaunchKernelGGL((k_balance<8, false, PE>), grid, block, lds, ctx->stream, a);
-    } else {
-        if (st) hipLaunchKernelGGL((k_balance<MOC_MAXLEV, true, PE>), grid, block, lds, ctx->stream, a);
-        else hipLaunchKernelGGL((k_balance<MOC_MAXLEV, false, PE>), grid, block, lds, ctx->stream, a);
     }
 }
 
@@ -884,7 +888,7 @@ int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a0, const double* J, 
                        int ncol, double* R, int chunk) {
     MocBalanceArgs a = a0;
     const int C1 = a.nCells + 1;
-    const size_t base = balance_lds_bytes(a.nX, a.nNlv);
+    const size_t base = balance_lds_bytes(a.nX, a.nNlv, a.nBig);
     const size_t staged = base + balance_stage_bytes(a);
     const bool st = staged <= kMaxBalanceLds;
     const bool pe = a.nGS > 0 && a.nGA > 0;

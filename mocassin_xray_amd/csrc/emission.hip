@@ -82,11 +82,14 @@ __device__ int hl_itemp(double T4) {
     return itemp;
 }
 
-__device__ int hl_idenp(const MocEmissionArgs& a, int f, double Ne) {
+// density bin of table file f: the last tabulated density below Ne
+// (RecLinesEmission :915-920); outputGas takes the last one at or below Ne
+// (ge, output_mod.f90:2054-2057)
+__device__ int hl_idenp(const MocEmissionArgs& a, int f, double Ne, bool ge = false) {
     const double* dens = a.hlDens + (size_t)f * 14;
     int idenp = 1;
     for (int iden = 1; iden <= 13; ++iden)
-        if (Ne > dens[iden - 1] && dens[iden - 1] > 0.) idenp = iden;
+        if ((ge ? Ne >= dens[iden - 1] : Ne > dens[iden - 1]) && dens[iden - 1] > 0.) idenp = iden;
     return idenp;
 }
 
@@ -113,10 +116,11 @@ __device__ double hl_lines(const MocEmissionArgs& a, int f, double Ne, double Hb
 }
 
 // one line (iup -> ilow) of table file f interpolated in density, times Hb
-__device__ double hl_line(const MocEmissionArgs& a, int f, double Ne, double Hb, int iup, int ilow) {
+__device__ double hl_line(const MocEmissionArgs& a, int f, double Ne, double Hb, int iup, int ilow,
+                          bool ge = false) {
     const double* dens = a.hlDens + (size_t)f * 14;
     const double* L = a.hlLines + (size_t)f * 14 * 16 * 9;
-    const int idenp = hl_idenp(a, f, Ne);
+    const int idenp = hl_idenp(a, f, Ne, ge);
     const double* L0 = L + (size_t)(idenp - 1) * 16 * 9;
     if ((dens[12] > 0. && idenp < 13) || (dens[12] == 0. && idenp < 9)) {
         const double* L1 = L0 + 16 * 9;
@@ -136,7 +140,7 @@ struct HydroLines {
 // hydrogenic recombination lines of element izp (RecLinesEmission :890-1099):
 // the table and the scale; hl_lines sums them, hl_line picks one
 __device__ HydroLines hydro_setup(const MocEmissionArgs& a, int izp, double Xup, double ab, double Te,
-                                  double Ne, double log10Ne) {
+                                  double Ne, double log10Ne, bool ge = false) {
     const double T4 = Te / 10000.;
     const int elUp = T4 > c_hlT[11] ? 2 : 8;
     double Hb;
@@ -152,7 +156,7 @@ __device__ HydroLines hydro_setup(const MocEmissionArgs& a, int izp, double Xup,
             Hb = pow(10., Af + Bf * log10(Te));
         } else {
             // H above 3e4 K, He above 1e5 K: the edge tables in density and T
-            const int idenp = hl_idenp(a, f, Ne);
+            const int idenp = hl_idenp(a, f, Ne, ge);
             const double* dens = a.hlDens + (size_t)f * 14;
             const double xx = log10(T4 * 1.e4);
             double yy1, yy2, x1, x2;
@@ -362,14 +366,6 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// n-level line emission of one ion, register-resident when the level count fits NL
-template <int NL>
-__device__ double em_nlevel(const MocBalanceArgs& b, int atom, double Te, double Ne, double up,
-                            double scale) {
-    if constexpr (NL <= 8) return nlevel_fixed<NL>(b, atom, Te, Ne, up, true, scale, 1.9865e9);
-    else return nlevel_lines(b, atom, Te, Ne, up, true, scale, 1.9865e9);
-}
-
 // The diffuse continuum and its CDF, one workgroup of four waves per cell;
 // the cell's line emissivity comes from k_em_lines.  Registers capped at 128
 // (four waves per SIMD, a few hundred bytes of spills): 3 % faster on the
@@ -490,12 +486,14 @@ k_emission(MocEmissionArgs a) {
 }
 
 // The lines of a cell (RecLinesEmission :856-1153, forLines :1481-1526): the
-// hydrogenic elements, the 34 He I lines and the n-level ions are separate
-// tasks, one per lane of a single wave per cell; their sum goes to lineSum
-// for the CDF kernel.
+// hydrogenic elements, the 34 He I lines and the n-level ions of up to NL
+// levels are separate tasks, one per lane of a single wave per cell; the
+// larger n-level ions are then solved one after the other by the whole wave
+// (matrix in LDS).  Their sum goes to lineSum for the CDF kernel.
 template <int NL>
 __global__ void __launch_bounds__(64) k_em_lines(MocEmissionArgs a) {
     __shared__ double sX[MOC_MAXE][MOC_MAXS + 1];
+    extern __shared__ double nlw[];  // workspace of the wave-solved ions
     const int cell = blockIdx.x + 1;
     if (moc_unchanged(a.TeOld, a.NeOld, a.Te, a.Ne, cell)) return;
     const int lane = threadIdx.x;
@@ -521,11 +519,21 @@ __global__ void __launch_bounds__(64) k_em_lines(MocEmissionArgs a) {
             if (heOn) lines += hei_line(a, t - a.nHyd, Te, Ne, NeGrid, XHe2, abHe);
         } else {
             const int k = t - a.nHyd - kHeILines;
+            const int atom = a.forAtom[k];
+            if (nlevel_count(a.bal, atom) > NL) continue;
             const int s = a.forSlot[k], ion = a.forIon[k];
             const double X = sX[s][ion];
             const double up = (ion < a.forTop[k] && X > 0.) ? sX[s][ion + 1] / X : 0.;
-            lines += em_nlevel<NL>(a.bal, a.forAtom[k], Te, Ne, up, a.abund[s] * X);
+            lines += nlevel_fixed<NL>(a.bal, atom, Te, Ne, up, true, a.abund[s] * X, 1.9865e9);
         }
+    }
+    for (int b = 0; b < a.nForBig; ++b) {
+        const int k = a.forBig[b];
+        const int s = a.forSlot[k], ion = a.forIon[k];
+        const double X = sX[s][ion];
+        const double up = (ion < a.forTop[k] && X > 0.) ? sX[s][ion + 1] / X : 0.;
+        const double v = nlevel_wave(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9, nlw, lane);
+        if (lane == 0) lines += v;
     }
     lines = wave_sum(lines);
     if (lane == 0) a.lineSum[cell] = lines;
@@ -544,9 +552,10 @@ __global__ void __launch_bounds__(64) k_line_lum(MocEmissionArgs a, const double
                                                  const double* __restrict__ H, int nChunks, int L,
                                                  int k0, int nk, double* __restrict__ part) {
     __shared__ double sX[MOC_MAXE][MOC_MAXS + 1];
-    extern __shared__ double acc[];  // [nk][L][L]: ions k0 .. k0+nk-1
+    extern __shared__ double acc[];  // [nk][L][L]: ions k0 .. k0+nk-1, then the wave solve's workspace
     const int lane = threadIdx.x;
     const int C1 = a.C1, nL = nk * L * L;
+    double* nlw = acc + nL;
     for (int k = lane; k < nL; k += 64) acc[k] = 0.;
     double hb = 0.;
     const int per = (C1 - 1 + nChunks - 1) / nChunks;
@@ -561,25 +570,32 @@ __global__ void __launch_bounds__(64) k_line_lum(MocEmissionArgs a, const double
             sX[e][ion] = a.ionDen[(size_t)k * C1 + cell];
         }
         __syncthreads();
-        const double NeGrid = a.Ne[cell], Te = a.Te[cell];
-        const double log10Ne = log10(NeGrid);
-        const double Ne = NeGrid < 1.e-5 ? 1.e-5 : NeGrid;
+        // outputGas takes the grid's n_e as it is (output_mod.f90:321), where
+        // the emission floors it at 1e-5
+        const double Ne = a.Ne[cell], Te = a.Te[cell];
+        const double log10Ne = log10(Ne);
         const double w = H[cell] * vol[cell];
         for (int kk = lane; kk < nk; kk += 64) {
             const int k = k0 + kk;
+            if (nlevel_count(a.bal, a.forAtom[k]) > NL) continue;
             const int s = a.forSlot[k], ion = a.forIon[k];
             const double X = sX[s][ion];
             const double up = (ion < a.forTop[k] && X > 0.) ? sX[s][ion + 1] / X : 0.;
-            if constexpr (NL <= 8)
-                nlevel_fixed<NL>(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9,
-                                 acc + (size_t)kk * L * L, w, L);
-            else
-                nlevel_lines(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9,
+            nlevel_fixed<NL>(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9,
                              acc + (size_t)kk * L * L, w, L);
         }
+        for (int b = 0; b < a.nForBig; ++b) {  // ions of more than NL levels, by the whole wave
+            const int k = a.forBig[b];
+            if (k < k0 || k >= k0 + nk) continue;
+            const int s = a.forSlot[k], ion = a.forIon[k];
+            const double X = sX[s][ion];
+            const double up = (ion < a.forTop[k] && X > 0.) ? sX[s][ion + 1] / X : 0.;
+            nlevel_wave(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9, nlw, lane,
+                        acc + (size_t)(k - k0) * L * L, w, L);
+        }
         if (lane == 0 && k0 == 0 && hs >= 0 && a.NS > 1) {  // hydroLines(1,4,2) (RecLinesEmission)
-            const HydroLines h = hydro_setup(a, 1, sX[hs][2], a.abund[hs], Te, Ne, log10Ne);
-            const double v = hl_line(a, h.f, h.Ne, h.Hb, 4, 2) * H[cell] * vol[cell];
+            const HydroLines h = hydro_setup(a, 1, sX[hs][2], a.abund[hs], Te, Ne, log10Ne, true);
+            const double v = hl_line(a, h.f, h.Ne, h.Hb, 4, 2, true) * H[cell] * vol[cell];
             if (v > 1.e-35) hb += v;
         }
     }
@@ -591,14 +607,19 @@ __global__ void __launch_bounds__(64) k_line_lum(MocEmissionArgs a, const double
 
 }  // namespace
 
-int moc_line_lum_group(int L) {
-    const int g = kLineLumLds / (L * L * (int)sizeof(double));
+// LDS of the wave solve's workspace in the line kernels
+static size_t em_wave_bytes(const MocEmissionArgs& a) {
+    return (size_t)nlevel_wave_doubles(a.forBigLev) * sizeof(double);
+}
+
+int moc_line_lum_group(const MocEmissionArgs& a, int L) {
+    const int g = (int)((kLineLumLds - em_wave_bytes(a)) / ((size_t)L * L * sizeof(double)));
     return g < 1 ? 1 : g;
 }
 
 int moc_launch_line_lum(moc_ctx* ctx, const MocEmissionArgs& a, const double* vol, const int* black,
                         const double* H, int nChunks, int L, int k0, int nk, double* part) {
-    const size_t lds = (size_t)nk * L * L * sizeof(double);
+    const size_t lds = (size_t)nk * L * L * sizeof(double) + em_wave_bytes(a);
     if (nk < (a.nFor > 0 ? 1 : 0) || k0 < 0 || k0 + nk > a.nFor || lds > (size_t)kLineLumLds) {
         ctx->err = "line luminosities: ion group " + std::to_string(k0) + "+" + std::to_string(nk) +
                    " of " + std::to_string(a.nFor) + " does not fit";
@@ -607,8 +628,7 @@ int moc_launch_line_lum(moc_ctx* ctx, const MocEmissionArgs& a, const double* vo
     const int maxLev = a.bal.maxLev;
     const dim3 grid((unsigned)nChunks), block(64);
     if (maxLev <= 5) hipLaunchKernelGGL(k_line_lum<5>, grid, block, lds, ctx->stream, a, vol, black, H, nChunks, L, k0, nk, part);
-    else if (maxLev <= 8) hipLaunchKernelGGL(k_line_lum<8>, grid, block, lds, ctx->stream, a, vol, black, H, nChunks, L, k0, nk, part);
-    else hipLaunchKernelGGL(k_line_lum<MOC_MAXLEV>, grid, block, lds, ctx->stream, a, vol, black, H, nChunks, L, k0, nk, part);
+    else hipLaunchKernelGGL(k_line_lum<8>, grid, block, lds, ctx->stream, a, vol, black, H, nChunks, L, k0, nk, part);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         ctx->err = std::string("line luminosities launch: ") + hipGetErrorString(e);
@@ -621,9 +641,9 @@ int moc_launch_emission(moc_ctx* ctx, const MocEmissionArgs& a) {
     if (a.nCells < 1) return MOC_OK;
     const int maxLev = a.bal.maxLev;
     const dim3 grid((unsigned)a.nCells);
-    if (maxLev <= 5) hipLaunchKernelGGL(k_em_lines<5>, grid, dim3(64), 0, ctx->stream, a);
-    else if (maxLev <= 8) hipLaunchKernelGGL(k_em_lines<8>, grid, dim3(64), 0, ctx->stream, a);
-    else hipLaunchKernelGGL(k_em_lines<MOC_MAXLEV>, grid, dim3(64), 0, ctx->stream, a);
+    const size_t lds = em_wave_bytes(a);
+    if (maxLev <= 5) hipLaunchKernelGGL(k_em_lines<5>, grid, dim3(64), lds, ctx->stream, a);
+    else hipLaunchKernelGGL(k_em_lines<8>, grid, dim3(64), lds, ctx->stream, a);
     hipLaunchKernelGGL(k_emission, grid, dim3(kThreads), 0, ctx->stream, a);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

@@ -164,6 +164,12 @@ struct MocBalanceArgs {
     const int* nlvSlot;
     const int* nlvIon;
     const int* nlvAtom;
+    // ions whose level count (min(file levels, maxLev)) exceeds 8 are solved
+    // by a whole wave with the matrix in LDS (nlevel_wave), the others one ion
+    // per lane with the matrix in registers
+    int nBig;                 // largest level count of those ions (0: none)
+    int nNlvBig;              // entries k of the nlv list solved by the wave
+    const int* nlvBig;
     int colHit, colComp;      // rate columns of the hit test and the Compton sums
     int ncol;                 // rate columns per cell
     int nX;                   // entries of a packed per-cell stage row (sum over elements
@@ -252,6 +258,9 @@ struct MocEmissionArgs {
     const int* forIon;
     const int* forAtom;
     const int* forTop;       // min(Z+1, nstages)
+    int nForBig;             // entries k of the for list solved by a whole wave (more than 8 levels)
+    const int* forBig;
+    int forBigLev;           // largest level count among them (0: none)
     const int* hlMap;        // [9][12]
     const double* hlDens;    // [files][14]
     const double* hlLines;   // [files][14][16][9]
@@ -401,6 +410,7 @@ struct MocBalance {
     std::vector<void*> owned;     // device allocations of this table set
     unsigned long long* d_counters = nullptr;
     std::vector<int> atomIdx;     // host copy of (Z, stage) -> n-level ion
+    std::vector<int> atomLevels;  // level count of each n-level ion as solved (maxLev cut)
 };
 
 struct MocAtomic {
@@ -454,7 +464,7 @@ int moc_launch_volavg(moc_ctx* ctx, const double* ionDen, const double* Te, cons
 int moc_launch_sed(moc_ctx* ctx, const double* escaped, int C1, int B, double* sed);
 // Forbidden-line luminosities of ions k0 .. k0+nk-1 (nk <= moc_line_lum_group(L))
 // and, with k0 == 0, Hbeta: part[block][nk*L*L + 1].
-int moc_line_lum_group(int L);
+int moc_line_lum_group(const MocEmissionArgs& a, int L);
 int moc_launch_line_lum(moc_ctx* ctx, const MocEmissionArgs& a, const double* vol, const int* black,
                         const double* H, int nChunks, int L, int k0, int nk, double* part);
 // the balance over every cell, in chunks of `chunk` cells whose rates fit R ([ncol][chunk])

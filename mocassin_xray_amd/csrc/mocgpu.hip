@@ -1011,6 +1011,19 @@ int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* t) {
         PUT(a.nlvSlot, ns.data(), ns.size(), "nlvSlot");
         PUT(a.nlvIon, ni.data(), ni.size(), "nlvIon");
         PUT(a.nlvAtom, na.data(), na.size(), "nlvAtom");
+        // ions of more than 8 levels (as cut at maxLev) are solved by a whole wave
+        std::vector<int> big;
+        a.nBig = 0;
+        for (int k = 0; k < (int)na.size(); ++k) {
+            const int n = (int)std::min<int64_t>(t->atomMeta[8 * na[k]], t->maxLev);
+            if (n > 8) { big.push_back(k); a.nBig = std::max(a.nBig, n); }
+        }
+        a.nNlvBig = (int)big.size();
+        PUT(a.nlvBig, big.data(), big.size(), "nlvBig");
+        // level counts as solved, for the emission's line lists
+        b.atomLevels.resize((size_t)t->nAtoms);
+        for (int k = 0; k < t->nAtoms; ++k)
+            b.atomLevels[k] = (int)std::min<int64_t>(t->atomMeta[8 * k], t->maxLev);
     }
     b.atomIdx.assign(t->atomIdx, t->atomIdx + 31 * 32);
     PUT(a.atomMeta, t->atomMeta, 8 * (size_t)t->nAtoms, "atomMeta");
@@ -1237,6 +1250,16 @@ int moc_set_emission(moc_ctx* ctx, const moc_emission_tables* t) {
     EPUT(a.forIon, forIon.data(), forIon.size(), "forIon");
     EPUT(a.forAtom, forAtom.data(), forAtom.size(), "forAtom");
     EPUT(a.forTop, forTop.data(), forTop.size(), "forTop");
+    {
+        // ions of more than 8 levels (as cut at maxLev) are solved by a whole wave
+        std::vector<int> big;
+        a.forBigLev = 0;
+        const std::vector<int>& lev = ctx->balance.atomLevels;
+        for (int k = 0; k < (int)forAtom.size(); ++k)
+            if (lev[forAtom[k]] > 8) { big.push_back(k); a.forBigLev = std::max(a.forBigLev, lev[forAtom[k]]); }
+        a.nForBig = (int)big.size();
+        EPUT(a.forBig, big.data(), big.size(), "forBig");
+    }
     EPUT(a.hlMap, t->hlMap, 9 * 12, "hlMap");
     EPUT(a.hlDens, t->hlDens, 14 * (size_t)t->nHlFiles, "hlDens");
     EPUT(a.hlLines, t->hlLines, 14 * 16 * 9 * (size_t)t->nHlFiles, "hlLines");
@@ -1628,7 +1651,7 @@ int moc_line_luminosities(moc_ctx* ctx, int ig, int* nIons, int* nLev, int* ionZ
     int nChunks = (g.nCells + 127) / 128;
     nChunks = nChunks < 1 ? 1 : (nChunks > 1024 ? 1024 : nChunks);
     // ions in groups whose accumulators fit a workgroup's LDS; Hbeta with the first
-    const int grp = moc_line_lum_group(L);
+    const int grp = moc_line_lum_group(a, L);
     const int nGroups = a.nFor > 0 ? (a.nFor + grp - 1) / grp : 1;
     const int64_t stride = (int64_t)(a.nFor < grp ? a.nFor : grp) * L * L + 1;
     double* d_part = nullptr;

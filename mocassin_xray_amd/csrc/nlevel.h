@@ -8,17 +8,38 @@
 
 namespace {
 
+// Level count of n-level ion `atom` as solved: its file's levels, cut at maxLev.
+__device__ __forceinline__ int nlevel_count(const MocBalanceArgs& a, int atom) {
+    const int nf = (int)a.atomMeta[8 * atom];
+    return nf < a.maxLev ? nf : a.maxLev;
+}
+
+// Doubles of LDS workspace nlevel_wave needs for ions of up to n levels.
+__host__ __device__ inline int nlevel_wave_doubles(int n) { return n > 0 ? 2 * n * n + 2 * n : 0; }
+
 // Sum of the line emissivities A_ji dE_ij n_j * scale * unit of ion `atom`
-// that exceed 1e-35.  With lineAcc, every line's emissivity (lower i, upper
-// j: fLineEm(i, j), :2987-3005) times w is also added to lineAcc[i * ld + j].
-// With rec, levels 2.. are fed by recombination from the
-// next stage (ionDenUp = X(i+1)/X(i)) at the rates of the ion's data file
-// (:2828-2877; kind 1: power-law radiative plus dielectronic fit per level,
-// kind 2: cubic in log T).  Upsilon(T) is the natural cubic spline in log T
-// with second derivatives computed on the host, clamped at the table ends.
-__device__ double nlevel_lines(const MocBalanceArgs& a, int atom, double Te, double Ne,
-                               double ionDenUp, bool rec, double scale, double unit,
-                               double* lineAcc = nullptr, double w = 0., int ld = 0) {
+// that exceed 1e-35 (equilibrium, emission_mod.f90:2505-3028; luSlv
+// :3031-3106), solved by one whole wave with the level matrix in LDS: for
+// the ions whose level count is too large for a per-lane register matrix.
+// Every lane of the (single-wave) workgroup calls it with the same
+// arguments; the return value is the same on every lane.  ws holds
+// nlevel_wave_doubles(n) doubles of LDS.
+//   rec      levels 2.. are fed by recombination from the next stage
+//            (ionDenUp = X(i+1)/X(i)) at the rates of the ion's data file
+//            (:2828-2877; kind 1: power-law radiative plus dielectronic fit per
+//            level, kind 2: cubic in log T);
+//   lineAcc  with it, every line's emissivity (lower i, upper j: fLineEm(i, j),
+//            :2987-3005) times w is added to lineAcc[i * ld + j] (one lane
+//            per line).
+// Upsilon(T) is the natural cubic spline in log T with second derivatives
+// computed on the host, clamped at the table ends.  Each matrix entry sees
+// the operations of a sequential evaluation in the same order (row sums and
+// the back substitution run on one lane; each elimination step updates the
+// rows below the pivot independently), so the result is the sequential one
+// bit for bit.
+__device__ double nlevel_wave(const MocBalanceArgs& a, int atom, double Te, double Ne,
+                              double ionDenUp, bool rec, double scale, double unit, double* ws,
+                              int lane, double* lineAcc = nullptr, double w = 0., int ld = 0) {
     const int64_t* m = a.atomMeta + 8 * atom;
     const int nf = (int)m[0], nT = (int)m[1];
     const double* lt = a.atomLogT + m[2];
@@ -30,11 +51,11 @@ __device__ double nlevel_lines(const MocBalanceArgs& a, int atom, double Te, dou
     const int iup2p = (int)m[6];
     const double atp = a.atomAtp[atom];
     const int n = nf < a.maxLev ? nf : a.maxLev;
-    double x[MOC_MAXLEV][MOC_MAXLEV], y[MOC_MAXLEV], qe[MOC_MAXLEV][MOC_MAXLEV];
+    double* Q = ws;              // [n][n] collision rates qe, later the line emissivities
+    double* X = Q + n * n;       // [n][n] level matrix
+    double* y = X + n * n;       // [n]
+    double* f = y + n;           // [n] elimination factors of one step
     const double sqrTe = sqrt(Te), lT = log10(Te);
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) { x[i][j] = 0.; qe[i][j] = 0.; }
-    // temperature bracket shared by every transition of this ion
     int klo = 0, khi = nT - 1;
     double sa = 0., sb = 0., h = 1.;
     const bool below = lT <= lt[0], above = lT >= lt[nT - 1];
@@ -47,49 +68,61 @@ __device__ double nlevel_lines(const MocBalanceArgs& a, int atom, double Te, dou
         sa = (lt[khi] - lT) / h;
         sb = (lT - lt[klo]) / h;
     }
-    for (int i = 0; i < n - 1; ++i)
-        for (int j = i + 1; j < n; ++j) {
-            const size_t p = (size_t)i * nf + j;
-            double ups;
-            if (nT == 1) ups = q[p];
-            else if (nT == 2) ups = q[p] + (q[(size_t)nf * nf + p] - q[p]) / (lt[1] - lt[0]) * (lT - lt[0]);
-            else if (below) ups = q[p];
-            else if (above) ups = q[(size_t)(nT - 1) * nf * nf + p];
-            else {
-                const double ylo = q[(size_t)klo * nf * nf + p], yhi = q[(size_t)khi * nf * nf + p];
-                const double dlo = q2[(size_t)klo * nf * nf + p], dhi = q2[(size_t)khi * nf * nf + p];
-                ups = sa * ylo + sb * yhi + ((sa * sa * sa - sa) * dlo + (sb * sb * sb - sb) * dhi) * (h * h) / 6.;
-            }
-            const double c = 8.63e-6 * ups / sqrTe;
-            if (en[i] <= en[j]) {
-                qe[i][j] = c * exp((en[i] - en[j]) * 1.4388463 / Te) / g[i];
-                qe[j][i] = c / g[j];
-            } else {
-                qe[j][i] = c * exp((en[j] - en[i]) * 1.4388463 / Te) / g[i];
-                qe[i][j] = c / g[j];
-            }
+    __syncthreads();  // the workspace may still be read by the previous call
+    // collision rates of every pair i < j, one pair per lane
+    const int np = n * (n - 1) / 2;
+    for (int p = lane; p < np; p += 64) {
+        int i = 0, r = p;
+        while (r >= n - 1 - i) { r -= n - 1 - i; ++i; }
+        const int j = i + 1 + r;
+        const size_t pp = (size_t)i * nf + j;
+        double ups;
+        if (nT == 1) ups = q[pp];
+        else if (nT == 2) ups = q[pp] + (q[(size_t)nf * nf + pp] - q[pp]) / (lt[1] - lt[0]) * (lT - lt[0]);
+        else if (below) ups = q[pp];
+        else if (above) ups = q[(size_t)(nT - 1) * nf * nf + pp];
+        else {
+            const double ylo = q[(size_t)klo * nf * nf + pp], yhi = q[(size_t)khi * nf * nf + pp];
+            const double dlo = q2[(size_t)klo * nf * nf + pp], dhi = q2[(size_t)khi * nf * nf + pp];
+            ups = sa * ylo + sb * yhi + ((sa * sa * sa - sa) * dlo + (sb * sb * sb - sb) * dhi) * (h * h) / 6.;
         }
-    for (int j = 0; j < n; ++j) x[0][j] = 1.;
-    y[0] = 1.;
-    for (int i = 1; i < n; ++i) {
+        const double c = 8.63e-6 * ups / sqrTe;
+        if (en[i] <= en[j]) {
+            Q[i * n + j] = c * exp((en[i] - en[j]) * 1.4388463 / Te) / g[i];
+            Q[j * n + i] = c / g[j];
+        } else {
+            Q[j * n + i] = c * exp((en[j] - en[i]) * 1.4388463 / Te) / g[i];
+            Q[i * n + j] = c / g[j];
+        }
+    }
+    __syncthreads();
+    // the rows of the matrix, one row per lane: row 0 the normalisation,
+    // the diagonal summed in the sequential order
+    const int kind = rec ? a.atomRecKind[atom] : 0;
+    for (int i = lane; i < n; i += 64) {
+        if (i == 0) {
+            for (int j = 0; j < n; ++j) X[j] = 1.;
+            y[0] = 1.;
+            continue;
+        }
+        double d = 0.;
         for (int j = 0; j < n; ++j) {
             if (j == i) continue;
-            x[i][j] += Ne * qe[j][i];
-            x[i][i] -= Ne * qe[i][j];
-            if (j > i) x[i][j] += A[(size_t)j * nf + i];
-            else x[i][i] -= A[(size_t)i * nf + j];
+            double v = 0.;
+            v += Ne * Q[j * n + i];
+            d -= Ne * Q[i * n + j];
+            if (j > i) v += A[(size_t)j * nf + i];
+            else d -= A[(size_t)i * nf + j];
+            X[i * n + j] = v;
         }
-        y[i] = 0.;
-    }
-    const int kind = rec ? a.atomRecKind[atom] : 0;
-    if (kind != 0) {
-        const double* rd = a.atomRec + 11 * m[3];
-        const int* rv = a.atomRecValid + m[3];
-        const double t4 = Te * 1.e-4;
-        for (int i = 1; i < n; ++i) {
+        X[i * n + i] = d;
+        double yi = 0.;
+        if (kind != 0) {
+            const int* rv = a.atomRecValid + m[3];
             double al = 0.;
             if (rv[i]) {
-                const double* r = rd + 11 * i;
+                const double* r = a.atomRec + 11 * m[3] + 11 * i;
+                const double t4 = Te * 1.e-4;
                 if (kind == 1) {
                     const double z = r[1], tz = t4 / (z * z);
                     al = 1.e-13 * r[0] * z * r[2] * pow(tz, r[3]) / (1. + r[4] * pow(tz, r[5])) +
@@ -99,32 +132,54 @@ __device__ double nlevel_lines(const MocBalanceArgs& a, int atom, double Te, dou
                     al = pow(10., r[0] + r[1] * lT + r[2] * lT * lT + r[3] * lT * lT * lT);
                 }
             }
-            y[i] = -Ne * ionDenUp * al;
+            yi = -Ne * ionDenUp * al;
+        }
+        y[i] = yi;
+    }
+    __syncthreads();
+    // elimination without pivoting, as luSlv: step i updates rows k > i
+    for (int i = 0; i < n - 1; ++i) {
+        for (int k = i + 1 + lane; k < n; k += 64) f[k] = X[k * n + i] / X[i * n + i];
+        __syncthreads();
+        const int mm = n - 1 - i;
+        for (int t = lane; t < mm * mm; t += 64) {
+            const int k = i + 1 + t / mm, j = i + 1 + t % mm;
+            X[k * n + j] -= X[i * n + j] * f[k];
+        }
+        for (int k = i + 1 + lane; k < n; k += 64) y[k] -= y[i] * f[k];
+        __syncthreads();
+    }
+    if (lane == 0) {
+        y[n - 1] /= X[(n - 1) * n + n - 1];
+        for (int k = n - 2; k >= 0; --k) {
+            double v = y[k];
+            for (int j = k + 1; j < n; ++j) v -= y[j] * X[k * n + j];
+            y[k] = v / X[k * n + k];
         }
     }
-    // elimination without pivoting, as luSlv
-    for (int i = 0; i < n - 1; ++i)
-        for (int k = i + 1; k < n; ++k) {
-            const double f = x[k][i] / x[i][i];
-            for (int j = i + 1; j < n; ++j) x[k][j] -= x[i][j] * f;
-            y[k] -= y[i] * f;
-        }
-    y[n - 1] /= x[n - 1][n - 1];
-    for (int k = n - 2; k >= 0; --k) {
-        for (int j = k + 1; j < n; ++j) y[k] -= y[j] * x[k][j];
-        y[k] /= x[k][k];
-    }
+    __syncthreads();
     double sumN = 0.;
     for (int i = 0; i < n; ++i) sumN += y[i];
+    // line emissivities, one line per lane, into Q (i < j)
+    for (int p = lane; p < np; p += 64) {
+        int i = 0, r = p;
+        while (r >= n - 1 - i) { r -= n - 1 - i; ++i; }
+        const int j = i + 1 + r;
+        double av = A[(size_t)j * nf + i];
+        double em = 0.;
+        if (av != 0.) {
+            if (j + 1 == iup2p && i == 0 && atp > 0.) av -= atp;
+            em = av * (en[j] - en[i]) * (y[j] / sumN) * scale * unit;
+            if (lineAcc) lineAcc[i * ld + j] += em * w;
+        }
+        Q[i * n + j] = em;
+    }
+    __syncthreads();
     double tot = 0.;
     for (int i = 0; i < n - 1; ++i)
         for (int j = i + 1; j < n; ++j) {
-            double av = A[(size_t)j * nf + i];
-            if (av == 0.) continue;
-            if (j + 1 == iup2p && i == 0 && atp > 0.) av -= atp;
-            const double em = av * (en[j] - en[i]) * (y[j] / sumN) * scale * unit;
+            const double em = Q[i * n + j];
             if (em > 1.e-35) tot += em;
-            if (lineAcc) lineAcc[i * ld + j] += em * w;
         }
     return tot;
 }

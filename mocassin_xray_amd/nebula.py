@@ -178,7 +178,10 @@ def initial_state(elems: list, nstages: int, hden_cells: np.ndarray, Te: float =
 
 def gas_model(name: str, n: int, R: float, hden: float, elems: list, abund: np.ndarray,
               nstages: int, nbins: int, nu_max: float, T_star: float, L36: float,
-              n_packets: int, maxLev: int = 5, lgCompton: bool = True) -> GasModel:
+              n_packets: int, maxLev: int = 27, lgCompton: bool = True) -> GasModel:
+    """maxLev cuts the n-level ions at that many levels; the default, 27, is
+    the largest level count of the packed atomic data, so every ion keeps the
+    levels of its file as equilibrium reads them (emission_mod.f90:2505-2754)."""
     from . import atomic, balance, emission
     ax = T.symmetric_axes(n, R)
     act, ncells, hcell = T.active_map(ax, ax, ax, np.full((n, n, n), hden))
@@ -206,14 +209,15 @@ NEBULA_LOG_ABUND = [0.0, -1.0, -3.6, -4.1, -3.3, -4.1, -5.0]
 
 
 def gas_nebula(n: int = 32, nbins: int = 1000, n_packets: int = 10_000_000,
-               nstages: int = 7) -> GasModel:
+               nstages: int = 7, maxLev: int = 5) -> GasModel:
     """Multi-element nebula, one octant: 32^3, edge 1e18 cm, n_H = 1000,
     H He C N O Ne S at log abundances 0 -1 -3.6 -4.1 -3.3 -4.1 -5, 100 kK
-    blackbody of 10^37.5 erg/s, 1000 points over 1-100 Ryd, 5-level
-    forbidden-line cooling."""
+    blackbody of 10^37.5 erg/s, 1000 points over 1-100 Ryd, forbidden-line
+    cooling with the ions cut at 5 levels (the workload's own definition;
+    maxLev=27 keeps every level of the data files, as the reference does)."""
     return gas_model("nebula%d" % n, n, 1.e18, 1000., NEBULA_ELEMS,
                      10.0 ** np.array(NEBULA_LOG_ABUND), nstages, nbins, 100.0, 1.e5,
-                     10 ** 1.5, n_packets, maxLev=5)
+                     10 ** 1.5, n_packets, maxLev=maxLev)
 
 
 # ------------------------------------------------------------------ dust models
@@ -285,7 +289,7 @@ class GasDustModel(GasModel):
 def gas_dust_model(name: str, n: int, R: float, hden, elems: list, abund: np.ndarray,
                    nstages: int, nbins: int, nu_min: float, nu_max: float, T_star: float,
                    L36: float, n_packets: int, species: list, grain_abun, radius, weights,
-                   mdmg: float, maxLev: int = 5, photoelectric: bool = False) -> GasDustModel:
+                   mdmg: float, maxLev: int = 27, photoelectric: bool = False) -> GasDustModel:
     """hden is a number or an (n, n, n) array of n_H per cell."""
     from . import atomic, balance, emission
     from . import dust as D
@@ -375,7 +379,7 @@ class BatchModel:
 
 def batch_nebulae(n_grids: int = 256, nbins: int = 400, seed: int = 20261203 + 4,
                   edge=(8, 48), hden=(10.0, 1.e4), T_star=(3.e4, 1.5e5), packets=(1.e5, 1.e6),
-                  L36: float = 10.0, nstages: int = 7, maxLev: int = 5) -> BatchModel:
+                  L36: float = 10.0, nstages: int = 7, maxLev: int = 27) -> BatchModel:
     """n_grids seeded nebulae of uneven size, one octant each: n ~ U{edge}
     cells per axis, uniform n_H log-uniform over `hden`, a blackbody of T
     uniform over T_star and L36 x 1e36 erg/s, packets log-uniform over
@@ -449,7 +453,7 @@ def xray_source(nu: np.ndarray, T_bb: float = 1.e5, pl_lo: float = 0.01, pl_hi: 
 
 
 def xray_nebula(n: int = 128, nbins: int = 2000, n_packets: int = 1_000_000_000,
-                R: float = 3.e18, L36: float = 10 ** 1.5, maxLev: int = 5) -> GasModel:
+                R: float = 3.e18, L36: float = 10 ** 1.5, maxLev: int = 27) -> GasModel:
     """Clumpy X-ray-illuminated nebula, one octant: n^3 cells, 10 % of them at
     n_H = 1000 and the rest at 10, edge R; all 30 elements H-Zn at solar
     abundances with every ionisation stage (31), inner-shell photoionisation

@@ -170,11 +170,13 @@ static int hl_itemp(double T4) {
     return itemp;
 }
 
-static int hl_idenp(const oracle_emission_tables* t, int f, double Ne) {
+/* :915-920 (Ne > dens); outputGas's copy takes Ne >= dens (ge,
+ * output_mod.f90:2054-2057) */
+static int hl_idenp(const oracle_emission_tables* t, int f, double Ne, int ge) {
     const double* dens = t->hlDens + (size_t)f * 14;
     int idenp = 1, iden;
     for (iden = 1; iden <= 13; iden++)
-        if (Ne > dens[iden - 1] && dens[iden - 1] > 0.) idenp = iden;
+        if ((ge ? Ne >= dens[iden - 1] : Ne > dens[iden - 1]) && dens[iden - 1] > 0.) idenp = iden;
     return idenp;
 }
 
@@ -183,7 +185,7 @@ static int hl_idenp(const oracle_emission_tables* t, int f, double Ne) {
 static double hl_lines(const oracle_emission_tables* t, int f, double Ne, double Hb) {
     const double* dens = t->hlDens + (size_t)f * 14;
     const double* L = t->hlLines + (size_t)f * 14 * 16 * 9;
-    const int idenp = hl_idenp(t, f, Ne);
+    const int idenp = hl_idenp(t, f, Ne, 0);
     double acc = 0.;
     int iup, ilow;
 #define HL(id, u, l) L[(((size_t)((id)-1)) * 16 + (u)) * 9 + (l)]
@@ -301,10 +303,11 @@ static void continuum(const oracle_emission_tables* t, const prep_t* p, const ce
 
 /* RecLinesEmission :856-1153, summed as setDiffusePDF :1772-1782 */
 /* one line of table file f interpolated in density, times Hb */
-static double hl_line(const oracle_emission_tables* t, int f, double Ne, double Hb, int iup, int ilow) {
+static double hl_line(const oracle_emission_tables* t, int f, double Ne, double Hb, int iup, int ilow,
+                      int ge) {
     const double* dens = t->hlDens + (size_t)f * 14;
     const double* L = t->hlLines + (size_t)f * 14 * 16 * 9;
-    const int idenp = hl_idenp(t, f, Ne);
+    const int idenp = hl_idenp(t, f, Ne, ge);
 #define HL(id, u, l) L[(((size_t)((id)-1)) * 16 + (u)) * 9 + (l)]
     if ((dens[12] > 0. && idenp < 13) || (dens[12] == 0. && idenp < 9)) {
         const double w = (Ne - dens[idenp - 1]) / (dens[idenp] - dens[idenp - 1]);
@@ -317,7 +320,7 @@ static double hl_line(const oracle_emission_tables* t, int f, double Ne, double 
 /* RecLinesEmission (:856-1153) summed; hbeta (may be NULL) receives
  * hydroLines(1,4,2) */
 static double rec_lines(const oracle_emission_tables* t, const cellcomp_t* c, double Te, double Ne,
-                        double NeGrid, double log10Ne, double* hbeta) {
+                        double NeGrid, double log10Ne, double* hbeta, int ge) {
     const int NS = t->bal->nstages;
     const double T4 = Te / 10000.;
     const int elUp = T4 > HL_TEMPS[11] ? 2 : 8;
@@ -328,7 +331,7 @@ static double rec_lines(const oracle_emission_tables* t, const cellcomp_t* c, do
         double Hb;
         if (!(c->on[izp] && NS > izp)) continue;
         f = t->hlMap[(izp - 1) * 12 + (hl_itemp(T4) - 1)];
-        idenp = hl_idenp(t, f, Ne);
+        idenp = hl_idenp(t, f, Ne, ge);
         if ((izp == 1 && T4 <= 3.) || (izp > 1 && T4 <= HL_TEMPS[11])) {
             const double* A = t->hbA + 4 * (izp - 1);
             const double* Bc = t->hbB + 4 * (izp - 1);
@@ -371,7 +374,7 @@ static double rec_lines(const oracle_emission_tables* t, const cellcomp_t* c, do
         }
         Hb = Hb * Ne * c->X[izp][izp + 1] * c->ab[izp];
         normRec += hl_lines(t, f, Ne, Hb);
-        if (izp == 1 && hbeta) *hbeta = hl_line(t, f, Ne, Hb, 4, 2);
+        if (izp == 1 && hbeta) *hbeta = hl_line(t, f, Ne, Hb, 4, 2, ge);
     }
     for (izp = elUp + 1; izp <= 30; izp++) {
         double T4z, lTeZ, Nez, lNez, Hb, zf, Af, Bf;
@@ -469,7 +472,7 @@ static void emission_one(const oracle_emission_tables* t, const prep_t* p, oracl
     int i;
     cell_comp(t->bal, s->ionDen, (size_t)C1, (size_t)cell, &c);
     continuum(t, p, &c, Te, Ne, w);
-    normRec = rec_lines(t, &c, Te, Ne, NeGrid, log10Ne, NULL);
+    normRec = rec_lines(t, &c, Te, Ne, NeGrid, log10Ne, NULL, 0);
     normFor = for_lines(t, &c, Te, Ne);
     /* setDiffusePDF :1593-1693 */
     for (i = 0; i < B; i++) {
@@ -591,12 +594,12 @@ int oracle_line_luminosities(const oracle_emission_tables* t, const oracle_emiss
                 cell_comp(b, s->ionDen, C1, (size_t)cell, &c);
                 Te = s->Te[cell];
                 NeGrid = s->Ne[cell];
-                Ne = NeGrid < 1.e-5 ? 1.e-5 : NeGrid;
+                Ne = NeGrid; /* outputGas uses the grid's n_e unfloored (output_mod.f90:321) */
                 H = g->Hden[cell];
                 dV = oracle_cell_width(g->xAxis, g->nx, i, g->sym) *
                      oracle_cell_width(g->yAxis, g->ny, j, g->sym) *
                      oracle_cell_width(g->zAxis, g->nz, k, g->sym);
-                (void)rec_lines(t, &c, Te, Ne, NeGrid, log10(NeGrid), &hb);
+                (void)rec_lines(t, &c, Te, Ne, NeGrid, log10(NeGrid), &hb, 1);
                 if (hb * H * dV > 1.e-35) *hbeta += hb * H * dV;
                 for (z = 3; z <= 30; z++) {
                     const int top = z + 1 < NS ? z + 1 : NS;

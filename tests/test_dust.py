@@ -491,3 +491,54 @@ def test_gpu_photoelectric_gas_dust_iteration_matches_oracle_chain(oracle):
                                  collD=up["collD"], peD=up["peD"])[0]
     np.testing.assert_allclose(got["Tdust"][1:], To[1:], rtol=1e-10)
     assert st["convPercent"] == pytest.approx(100.0 * up["lgConverged"][1:].mean())
+
+
+@pytest.mark.gpu
+def test_gpu_dusty_lognormal_nebula_matches_oracle_chain(oracle):
+    """The dusty-nebula workload at 6^3: seeded log-normal n_H (mean 300,
+    sigma_ln 1), silicate and graphite at 10 MRN sizes each (20 grains, one
+    photoelectric grain per lane), grain charging and photoelectric heating
+    on, every n-level ion at its file's level count; one moc_iterate call
+    against the oracle at every stage."""
+    from mocassin_xray_amd import MocGPU
+    m = nebula.gas_dust_nebula(n=6, nbins=300, n_packets=20000)
+    assert m.dust.nSizes == 10 and m.dust.nSpecies == 2 and m.bt.maxLev == 27
+    s0 = m.state
+    hd = s0["Hden"][1:]
+    assert hd.min() < 100.0 and hd.max() > 1000.0        # the densities vary cell to cell
+    nu, w = m.tables.nuArray, m.tables.widFlx
+    seed = 43
+    print(f"case dustneb6 lognormal 20 grains: cells={m.grid.nCells} bins={nu.size}", flush=True)
+    with MocGPU(m.tables, "fp64") as g:
+        m.load(g)
+        Td0 = g.download("Tdust")
+        st = g.iterateMC(1, seed)
+        got = {k: g.download(k) for k in ("opacity", "scaOpac", "recPDF", "totalLines", "Jste",
+                                          "Te", "Ne", "ionDen", "Tdust", "grainColl", "grainPE")}
+    E, Z, NS = m.elems, m.abund, m.nstages
+    gas = oracle.opacity(m.xsec, m.tables.KNsigmaT, True, E, Z, NS, s0["ionDen"], s0["Ne"],
+                         s0["Te"], s0["Hden"])
+    op_o, ab_o, sc_o = oracle.dust_opacity(m.dust, nu, w, m.Ndust, Td0, gasOpac=gas)
+    np.testing.assert_allclose(got["opacity"][1:], op_o[1:], rtol=1e-11, atol=1e-300)
+    em = oracle.emission(m.et, m.bt, E, Z, NS, s0["Te"], s0["Ne"], s0["ionDen"], nthreads=8,
+                         dust=(m.dust, Td0, m.Ndust, s0["Hden"]))
+    np.testing.assert_allclose(got["recPDF"][1:], em["recPDF"][1:], rtol=1e-11, atol=1e-14)
+    np.testing.assert_allclose(got["totalLines"][1:], em["totalLines"][1:], rtol=1e-11)
+    fields = {"opacity": got["opacity"], "Ne": s0["Ne"], "recPDF": got["recPDF"],
+              "totalLines": got["totalLines"], "scaOpac": got["scaOpac"], "dustOnly": False}
+    Jo, Eo, co, qo = oracle.transport(m.tables, m.grid, fields, 20000, seed, ctr_base=257,
+                                      nthreads=8)
+    assert co["errors"] == 0 and co["scattered"] > 0
+    Jo = Jo * 1.e-9 / 8.0
+    assert np.abs(got["Jste"][1:] - Jo[1:]).sum() / Jo[1:].sum() < 2e-3
+    up = oracle.update(m.bt, E, Z, NS, True, got["Jste"], s0["Te"], s0["Ne"], s0["Hden"],
+                       s0["ionDen"], nthreads=8, grains=(m.dust, Td0, m.Ndust))
+    np.testing.assert_allclose(got["Te"][1:], up["Te"][1:], rtol=1e-9)
+    np.testing.assert_allclose(got["Ne"][1:], up["Ne"][1:], rtol=1e-9)
+    np.testing.assert_allclose(got["ionDen"][1:], up["ionDen"][1:], rtol=1e-8, atol=1e-25)
+    np.testing.assert_allclose(got["grainColl"][:, 1:], up["collD"][:, 1:], rtol=1e-9)
+    np.testing.assert_allclose(got["grainPE"][:, 1:], up["peD"][:, 1:], rtol=1e-9)
+    To = oracle.dust_temperature(m.dust, nu, w, m.Ndust, Td0, got["Jste"], dustOnly=False,
+                                 collD=up["collD"], peD=up["peD"])[0]
+    np.testing.assert_allclose(got["Tdust"][1:], To[1:], rtol=1e-10)
+    assert st["convPercent"] == pytest.approx(100.0 * up["lgConverged"][1:].mean())

@@ -21,18 +21,24 @@ def test_initial_state_matches_setgrid_guess():
     assert np.all(st["Ne"] == hd) and np.all(st["Te"][1:] == 1e4)
 
 
-def small_nebula(n_packets):
+def small_nebula(n_packets, maxLev=27):
     return nebula.gas_model("nebula6", 6, 1.e18, 1000., nebula.NEBULA_ELEMS,
                             10.0 ** np.array(nebula.NEBULA_LOG_ABUND), 7, 300, 100.0, 1.e5,
-                            10 ** 1.5, n_packets)
+                            10 ** 1.5, n_packets, maxLev=maxLev)
 
 
 @pytest.mark.gpu
-def test_one_iteration_matches_oracle_chain(oracle):
-    m = small_nebula(20000)
+@pytest.mark.parametrize("maxLev", [5, 27])
+def test_one_iteration_matches_oracle_chain(oracle, maxLev):
+    """maxLev 5: every n-level ion is solved one per lane in registers;
+    maxLev 27 (the data files' level counts, up to 17 here): the ions of more
+    than 8 levels are solved by a whole wave with the matrix in LDS, in the
+    balance (cooling) and in the emission (lines)."""
+    m = small_nebula(20000, maxLev)
     s0 = m.state
     seed = 4242
-    print(f"case iteration-nebula6: cells={m.grid.nCells} bins={m.tables.nbins}", flush=True)
+    print(f"case iteration-nebula6 maxLev {maxLev}: cells={m.grid.nCells} bins={m.tables.nbins}",
+          flush=True)
     from mocassin_xray_amd import MocGPU
     with MocGPU(m.tables, "fp64") as g:
         m.load(g)

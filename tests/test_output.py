@@ -160,8 +160,28 @@ def test_line_luminosities_pinned(oracle):
     assert hb2 == pytest.approx(hb + hb9, rel=1e-13)
 
 
+def test_line_luminosities_take_the_grid_density_unfloored(oracle):
+    """outputGas solves the levels at the grid's own n_e (output_mod.f90:321);
+    the emission floors n_e at 1e-5 (emission_mod.f90:85-86).  At n_e of
+    1e-8 and 2e-8, far below every critical density, each collisionally
+    excited line scales as n_e: doubling n_e doubles the [O III] 5007
+    luminosity (a floor at 1e-5 would leave it unchanged)."""
+    m, Te, Ne, H, X = small_state()
+    C1 = m.grid.nCells + 1
+    black = np.ones(C1, dtype=np.int32)
+    black[5] = 0
+    out = []
+    for ne in (1.e-8, 2.e-8):
+        Ne[5], Te[5] = ne, 1.e4
+        lines, _ = oracle.line_luminosities(m.et, m.bt, m.elems, m.abund, 7, m.grid, Te, Ne, H, X,
+                                            m.bt.maxLev, lgBlack=black)
+        out.append(lines[8, 3, 2, 3])
+    assert out[0] > 0.0
+    assert out[1] / out[0] == pytest.approx(2.0, rel=1e-5)
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("maxLev", [5, 27])
+@pytest.mark.parametrize("maxLev", [5, 6, 27])
 def test_gpu_line_luminosities_match_oracle(oracle, maxLev):
     """maxLev 27: the ions' accumulators no longer fit one workgroup's LDS
     together, so they are reduced in several ion groups."""
