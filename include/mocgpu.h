@@ -301,6 +301,17 @@ int moc_grid_attach(moc_ctx* ctx, int ig, const moc_grid_desc* grid);
  * line luminosities and the SED. */
 int moc_volume_averages(moc_ctx* ctx, int ig, double* TeVol, double* ionDenVol);
 
+/* moc_volume_averages without a device allocation: the same outputs, the
+ * partial sums held in a fixed device buffer of 4 MiB (2^19 doubles) that
+ * each GPU keeps from the first moc_create on it until the process ends.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Returns MOC_ERR_ARG when nElemUsed * nstages * (chunks
+ * of 4096 cells, at most 256) * 2 + nElemUsed * nstages * 2 exceeds the
+ * buffer.  Every call on one GPU shares the buffer: calls that overlap on one
+ * GPU (from several threads or contexts) race. */
+int moc_volume_averages_noalloc(moc_ctx* ctx, int ig, double* TeVol, double* ionDenVol);
+
 /* Line luminosities of outputGas (output_mod.f90:397-432): over the cells
  * packets reached, the sum of H dV times the emissivity of every line of every
  * n-level ion (forLines, emission_mod.f90:1481-1526: equilibrium with the

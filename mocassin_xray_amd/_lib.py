@@ -37,6 +37,7 @@ EXPORTS = [
     "moc_update", "moc_set_emission", "moc_emission", "moc_set_packets", "moc_iterate",
     "moc_set_dust", "moc_dust_opacity", "moc_dust_emission", "moc_dust_temperature",
     "moc_grid_attach_batch", "moc_volume_averages", "moc_sed", "moc_line_luminosities",
+    "moc_volume_averages_noalloc",
 ]
 
 _dp = C.POINTER(C.c_double)

@@ -443,6 +443,7 @@ struct moc_ctx {
     MocDust dust;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double* d_scratch = nullptr;  // the device's fixed buffer (allocation-free entry point), not owned
 };
 
 // launchers (transport.hip)
@@ -462,6 +463,9 @@ int moc_launch_volavg(moc_ctx* ctx, const double* ionDen, const double* Te, cons
                       const double* H, const double* vol, const int* black, int C1, int nPairs,
                       int nChunks, double* part);
 int moc_launch_sed(moc_ctx* ctx, const double* escaped, int C1, int B, double* sed);
+// sums[2p], sums[2p+1] = the volume-average partials of pair p added over the chunks in order
+int moc_launch_volavg_sum(moc_ctx* ctx, const double* part, int nPairs, int nChunks, int C1,
+                          double* sums);
 // Forbidden-line luminosities of ions k0 .. k0+nk-1 (nk <= moc_line_lum_group(L))
 // and, with k0 == 0, Hbeta: part[block][nk*L*L + 1].
 int moc_line_lum_group(const MocEmissionArgs& a, int L);

@@ -7,12 +7,48 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "moc_internal.h"
 
+// Device buffer of the allocation-free entry point moc_volume_averages_noalloc:
+// the volume-average partials (at most 31 x 31 ion pairs x 256 chunks x 2) and
+// their chunk sums.  Each GPU the library runs on holds its own copy until the
+// process ends; every call on a GPU shares it (calls that overlap on one GPU race).
+constexpr size_t kScratchDoubles = (size_t)1 << 19;  // 4 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace {
+
+// device address of `scratch` per device, looked up once per device (with that
+// device current) by moc_create; host code uses these, never the symbol
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+double* g_scratchPtr[kMaxDevices] = {};
+
+int scratch_setup(moc_ctx* ctx) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        ctx->err = std::string("scratch buffer: device ") + std::to_string(dev) + ": " +
+                   hipGetErrorString(e);
+        return MOC_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (!g_scratchPtr[dev]) {
+        double* p = nullptr;
+        e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            ctx->err = std::string("scratch buffer: hipGetSymbolAddress: ") + hipGetErrorString(e);
+            return MOC_ERR_HIP;
+        }
+        g_scratchPtr[dev] = p;
+    }
+    ctx->d_scratch = g_scratchPtr[dev];
+    return MOC_OK;
+}
 
 #define MOC_TRY(call, what)                                                              \
     do {                                                                                 \
@@ -325,6 +361,8 @@ int moc_create(moc_ctx** out, const moc_tables* t) {
             return MOC_ERR_ARG;
         }
     MOC_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "stream");
+    int rc0 = scratch_setup(ctx);
+    if (rc0) return rc0;
     MOC_TRY(hipEventCreate(&ctx->ev0), "event");
     MOC_TRY(hipEventCreate(&ctx->ev1), "event");
     const size_t nA = (size_t)B * (size_t)t->nAngles, nS = (size_t)B * (size_t)t->nStars;
@@ -1558,6 +1596,39 @@ int moc_dust_temperature(moc_ctx* ctx, int ig, moc_update_stats* stats) {
     return MOC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// cell chunks of the volume-average partials: 4096 cells each, 1..256 of them
+int volavg_chunks(int nCells) {
+    const int n = (nCells + 4095) / 4096;
+    return n < 1 ? 1 : (n > 256 ? 256 : n);
+}
+
+// Harrington et al. (1982) from the chunk-summed TW[2 * pair] = sum X Te Ne H dV,
+// TW[2 * pair + 1] = sum X Ne H dV: <X> over the element's stages, <Te>
+// weighted by the ion (output_mod.f90:682-730)
+void volavg_finish(moc_ctx* ctx, const double* TW, double* TeVol, double* ionDenVol) {
+    const int NE = ctx->atomic.nElemUsed, NS = ctx->atomic.nstages;
+    for (int e = 0; e < NE; ++e) {
+        const int top = std::min(ctx->atomic.elemZ[e] + 1, NS);
+        double den = 0.0;
+        for (int ion = 1; ion <= top; ++ion) den += TW[2 * ((size_t)(ion - 1) * NE + e) + 1];
+        for (int ion = 1; ion <= NS; ++ion) {
+            const size_t k = (size_t)(ion - 1) * NE + e;
+            if (ion > top) { TeVol[k] = 0.0; ionDenVol[k] = 0.0; continue; }
+            const double T = TW[2 * k], W = TW[2 * k + 1];
+            TeVol[k] = W <= 0.0 ? 0.0 : T / W;
+            ionDenVol[k] = den <= 0.0 ? 0.0 : W / den;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
 int moc_volume_averages(moc_ctx* ctx, int ig, double* TeVol, double* ionDenVol) {
     if (!ctx) return MOC_ERR_ARG;
     MocGrid& g = ctx->grid;
@@ -1572,8 +1643,7 @@ int moc_volume_averages(moc_ctx* ctx, int ig, double* TeVol, double* ionDenVol) 
     if (!TeVol || !ionDenVol) { ctx->err = "moc_volume_averages: null output"; return MOC_ERR_ARG; }
     const int NE = ctx->atomic.nElemUsed, NS = ctx->atomic.nstages, C1 = g.nCells + 1;
     const int nPairs = NE * NS;
-    int nChunks = (g.nCells + 4095) / 4096;
-    nChunks = nChunks < 1 ? 1 : (nChunks > 256 ? 256 : nChunks);
+    const int nChunks = volavg_chunks(g.nCells);
     double* d_part = nullptr;
     int rc;
     if ((rc = dalloc_n(ctx, &d_part, (size_t)nPairs * nChunks * 2, "volume-average partials"))) return rc;
@@ -1591,25 +1661,54 @@ int moc_volume_averages(moc_ctx* ctx, int ig, double* TeVol, double* ionDenVol) 
     }
     dfree(d_part);
     if (rc) return rc;
-    // sums over the chunks in order, then Harrington et al. (1982): <X> over
-    // the element's stages, <Te> weighted by the ion (:682-730)
-    std::vector<double> T((size_t)nPairs, 0.0), W((size_t)nPairs, 0.0);
+    // sums over the chunks in order
+    std::vector<double> TW((size_t)nPairs * 2, 0.0);
     for (int p = 0; p < nPairs; ++p)
         for (int c = 0; c < nChunks; ++c) {
-            T[p] += part[((size_t)p * nChunks + c) * 2];
-            W[p] += part[((size_t)p * nChunks + c) * 2 + 1];
+            TW[2 * p] += part[((size_t)p * nChunks + c) * 2];
+            TW[2 * p + 1] += part[((size_t)p * nChunks + c) * 2 + 1];
         }
-    for (int e = 0; e < NE; ++e) {
-        const int top = std::min(ctx->atomic.elemZ[e] + 1, NS);
-        double den = 0.0;
-        for (int ion = 1; ion <= top; ++ion) den += W[(size_t)(ion - 1) * NE + e];
-        for (int ion = 1; ion <= NS; ++ion) {
-            const size_t k = (size_t)(ion - 1) * NE + e;
-            if (ion > top) { TeVol[k] = 0.0; ionDenVol[k] = 0.0; continue; }
-            TeVol[k] = W[k] <= 0.0 ? 0.0 : T[k] / W[k];
-            ionDenVol[k] = den <= 0.0 ? 0.0 : W[k] / den;
-        }
+    volavg_finish(ctx, TW.data(), TeVol, ionDenVol);
+    return MOC_OK;
+}
+
+int moc_volume_averages_noalloc(moc_ctx* ctx, int ig, double* TeVol, double* ionDenVol) {
+    if (!ctx) return MOC_ERR_ARG;
+    MocGrid& g = ctx->grid;
+    if (!g.attached || ig != 1 || !ctx->atomic.set || !g.d_ionDen) {
+        ctx->err = "moc_volume_averages_noalloc: needs an attached grid and moc_set_atomic";
+        return MOC_ERR_STATE;
     }
+    if (g.nGrids > 1) {
+        ctx->err = "moc_volume_averages_noalloc: the context holds a batch of grids; reduce one nebula per context";
+        return MOC_ERR_STATE;
+    }
+    if (!TeVol || !ionDenVol) { ctx->err = "moc_volume_averages_noalloc: null output"; return MOC_ERR_ARG; }
+    if (!ctx->d_scratch) { ctx->err = "moc_volume_averages_noalloc: no scratch buffer"; return MOC_ERR_STATE; }
+    const int NE = ctx->atomic.nElemUsed, NS = ctx->atomic.nstages, C1 = g.nCells + 1;
+    const int nPairs = NE * NS;
+    const int nChunks = volavg_chunks(g.nCells);
+    const size_t need = (size_t)nPairs * nChunks * 2 + (size_t)nPairs * 2;
+    if (nPairs > MOC_MAXE * MOC_MAXS || need > kScratchDoubles) {
+        ctx->err = "moc_volume_averages_noalloc: " + std::to_string(need) +
+                   " doubles of partial sums exceed the fixed buffer of " + std::to_string(kScratchDoubles);
+        return MOC_ERR_ARG;
+    }
+    double* d_part = ctx->d_scratch;
+    double* d_sums = ctx->d_scratch + (size_t)nPairs * nChunks * 2;
+    int rc = moc_launch_volavg(ctx, g.d_ionDen, g.d_Te, g.d_Ne, g.d_Hden, g.d_vol, g.d_black, C1, nPairs,
+                               nChunks, d_part);
+    if (!rc) rc = moc_launch_volavg_sum(ctx, d_part, nPairs, nChunks, C1, d_sums);
+    if (rc) return rc;
+    double TW[2 * MOC_MAXE * MOC_MAXS];
+    const hipError_t e1 = hipMemcpyAsync(TW, d_sums, (size_t)nPairs * 2 * sizeof(double),
+                                         hipMemcpyDeviceToHost, ctx->stream);
+    const hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        ctx->err = std::string("moc_volume_averages_noalloc: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2);
+        return MOC_ERR_HIP;
+    }
+    volavg_finish(ctx, TW, TeVol, ionDenVol);
     return MOC_OK;
 }
 

@@ -6,6 +6,8 @@
 //              the ion's ionDen plane read cell-fastest; each block writes its
 //              two partial sums, which the host adds chunk by chunk in order,
 //              so the result does not depend on scheduling.
+//  k_volavg_sum -- the same in-order chunk sums on the device, for the entry
+//              point that keeps its partials in a fixed device buffer.
 //  k_sed    -- writeSED (:2947-3064): the escaped energy of every bin summed
 //              over the cells (rows 0..nCells), one block per bin.
 // All arithmetic is FP64.
@@ -68,6 +70,22 @@ __global__ void __launch_bounds__(kThreads) k_sed(const double* __restrict__ esc
     if (threadIdx.x == 0) sed[b] = s;
 }
 
+// the partial sums of k_volavg added chunk by chunk in order (the order the
+// host uses), one (ion) pair per lane; with no cells the sums are zero
+__global__ void __launch_bounds__(kThreads) k_volavg_sum(const double* __restrict__ part, int nPairs,
+                                                         int nChunks, int any, double* __restrict__ sums) {
+    const int p = blockIdx.x * kThreads + threadIdx.x;
+    if (p >= nPairs) return;
+    double t = 0., w = 0.;
+    if (any)
+        for (int c = 0; c < nChunks; ++c) {
+            t += part[((size_t)p * nChunks + c) * 2];
+            w += part[((size_t)p * nChunks + c) * 2 + 1];
+        }
+    sums[2 * p] = t;
+    sums[2 * p + 1] = w;
+}
+
 int check(moc_ctx* ctx, const char* what) {
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -92,4 +110,12 @@ int moc_launch_sed(moc_ctx* ctx, const double* escaped, int C1, int B, double* s
     if (B < 1) return MOC_OK;
     hipLaunchKernelGGL(k_sed, dim3((unsigned)B), dim3(kThreads), 0, ctx->stream, escaped, C1, sed);
     return check(ctx, "SED launch");
+}
+
+int moc_launch_volavg_sum(moc_ctx* ctx, const double* part, int nPairs, int nChunks, int C1,
+                          double* sums) {
+    if (nPairs < 1) return MOC_OK;
+    hipLaunchKernelGGL(k_volavg_sum, dim3((unsigned)((nPairs + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                       ctx->stream, part, nPairs, nChunks, C1 >= 2 ? 1 : 0, sums);
+    return check(ctx, "volume average sums launch");
 }

@@ -328,6 +328,17 @@ class MocGPU:
                                                 X.ctypes.data_as(_dp)), "moc_volume_averages")
         return T.reshape(ns, ne).T.copy(), X.reshape(ns, ne).T.copy()
 
+    def volume_averages_noalloc(self):
+        """volume_averages through moc_volume_averages_noalloc: the partial
+        sums in the library's fixed device buffer, no device allocation."""
+        ne, ns = self.n_elem, self.nstages
+        T = np.zeros(ne * ns)
+        X = np.zeros(ne * ns)
+        self._check(self._L.moc_volume_averages_noalloc(self._ctx, 1, T.ctypes.data_as(_dp),
+                                                        X.ctypes.data_as(_dp)),
+                    "moc_volume_averages_noalloc")
+        return T.reshape(ns, ne).T.copy(), X.reshape(ns, ne).T.copy()
+
     def line_luminosities(self):
         """outputGas line luminosities (output_mod.f90:397-432): (lines, ions,
         Hbeta) with lines[k, i, j] ion k's line from level j+1 to level i+1
