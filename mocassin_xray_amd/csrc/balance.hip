@@ -103,7 +103,6 @@ __device__ __forceinline__ double t3(const double* a, int i, int z, int n) {
 // lanes run over k = e * S2 + stage (S2 = NS + 2) and skip stages past the end.
 struct Lds {
     double *X, *alpha, *cI1, *out, *cool, *diO;
-    double* nlw;  // workspace of the wave-solved n-level ions
     int* hot;
     int* off;  // [NE + 1]
     int S2;
@@ -113,10 +112,9 @@ struct Lds {
     __device__ double& c1(int e, int i) const { return cI1[off[e] + i]; }
 };
 
-__host__ __device__ inline size_t balance_lds_bytes(int nX, int nNlv, int nBig) {
+__host__ __device__ inline size_t balance_lds_bytes(int nX, int nNlv) {
     const size_t rows = (size_t)4 * nX;
-    return (rows + (size_t)(nNlv > 0 ? nNlv : 1) + MOC_MAXS + 1 + (size_t)nlevel_wave_doubles(nBig)) *
-               sizeof(double) +
+    return (rows + (size_t)(nNlv > 0 ? nNlv : 1) + MOC_MAXS + 1) * sizeof(double) +
            (MOC_MAXE + 2) * sizeof(int);
 }
 
@@ -616,12 +614,23 @@ __device__ void therm(const MocBalanceArgs& a, const RT& r, const Lds& L, double
         const int e = a.nlvSlot[k], ion = a.nlvIon[k];
         L.cool[k] = nlevel_fixed<NL>(a, atom, T, Ne, 0., false, a.abund[e] * L.x(e, ion), 1.9865e-16);
     }
-    for (int b = 0; b < a.nNlvBig; ++b) {
-        const int k = a.nlvBig[b];
+    // 9-15 levels: four ions at a time, 16 lanes each; 16-27: two, 32 lanes each
+    // (none when every ion fits the per-lane solve of NL = 5)
+    if constexpr (NL > 5) {
+    for (int b0 = 0; b0 < a.nNlvT[0]; b0 += 4) {
+        const int b = b0 + (tid >> 4) < a.nNlvT[0] ? b0 + (tid >> 4) : b0;  // idle teams repeat one
+        const int k = a.nlvT[0][b];
         const int e = a.nlvSlot[k], ion = a.nlvIon[k];
-        const double v = nlevel_wave(a, a.nlvAtom[k], T, Ne, 0., false, a.abund[e] * L.x(e, ion), 1.9865e-16,
-                                     L.nlw, tid);
-        if (tid == 0) L.cool[k] = v;
+        const double v = nlevel_team<15>(a, a.nlvAtom[k], T, Ne, 0., false, a.abund[e] * L.x(e, ion), 1.9865e-16);
+        if ((tid & 15) == 0 && b0 + (tid >> 4) < a.nNlvT[0]) L.cool[k] = v;
+    }
+    for (int b0 = 0; b0 < a.nNlvT[1]; b0 += 2) {
+        const int b = b0 + (tid >> 5) < a.nNlvT[1] ? b0 + (tid >> 5) : b0;
+        const int k = a.nlvT[1][b];
+        const int e = a.nlvSlot[k], ion = a.nlvIon[k];
+        const double v = nlevel_team<27>(a, a.nlvAtom[k], T, Ne, 0., false, a.abund[e] * L.x(e, ion), 1.9865e-16);
+        if ((tid & 31) == 0 && b0 + (tid >> 5) < a.nNlvT[1]) L.cool[k] = v;
+    }
     }
     __syncthreads();
     double coolFF = 0., coolRec = 0., coolColl, coolCollH = 0., heatSte = 0., Te4 = T / 1.e4;
@@ -704,7 +713,7 @@ k_balance(MocBalanceArgs a) {
     Rates<ST> r{a.R, a.ncol, (int)blockIdx.x, a.specOf, a.specShell, a.shInfo, a.shAug};
     if constexpr (ST) {
         // stage the cell's rate columns and the index tables after the state rows
-        double* Rl = lds + balance_lds_bytes(a.nX, a.nNlv, a.nBig) / sizeof(double);
+        double* Rl = lds + balance_lds_bytes(a.nX, a.nNlv) / sizeof(double);
         int* iL = reinterpret_cast<int*>(Rl + a.ncol);
         int* sOf = iL;
         int* sSh = sOf + NE * 32;
@@ -737,8 +746,7 @@ k_balance(MocBalanceArgs a) {
     L.out = L.cI1 + a.nX;
     L.cool = L.out + a.nX;
     L.diO = L.cool + (a.nNlv > 0 ? a.nNlv : 1);
-    L.nlw = L.diO + MOC_MAXS + 1;
-    L.hot = (int*)(L.nlw + nlevel_wave_doubles(a.nBig));
+    L.hot = (int*)(L.diO + MOC_MAXS + 1);
     L.off = L.hot + 1;
     if (tid <= NE) L.off[tid] = a.xoff[tid];
     __syncthreads();
@@ -750,7 +758,7 @@ k_balance(MocBalanceArgs a) {
     const int nG = PE ? a.nGS * a.nGA : 0;
     GrainLds G{};
     if constexpr (PE) {
-        size_t off = balance_lds_bytes(a.nX, a.nNlv, a.nBig);
+        size_t off = balance_lds_bytes(a.nX, a.nNlv);
         if constexpr (ST) off += balance_stage_bytes(a);
         G.q = lds + off / sizeof(double);
         G.X0 = G.q + a.B;
@@ -888,7 +896,7 @@ int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a0, const double* J, 
                        int ncol, double* R, int chunk) {
     MocBalanceArgs a = a0;
     const int C1 = a.nCells + 1;
-    const size_t base = balance_lds_bytes(a.nX, a.nNlv, a.nBig);
+    const size_t base = balance_lds_bytes(a.nX, a.nNlv);
     const size_t staged = base + balance_stage_bytes(a);
     const bool st = staged <= kMaxBalanceLds;
     const bool pe = a.nGS > 0 && a.nGA > 0;

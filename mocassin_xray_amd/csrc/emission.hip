@@ -493,7 +493,6 @@ k_emission(MocEmissionArgs a) {
 template <int NL>
 __global__ void __launch_bounds__(64) k_em_lines(MocEmissionArgs a) {
     __shared__ double sX[MOC_MAXE][MOC_MAXS + 1];
-    extern __shared__ double nlw[];  // workspace of the wave-solved ions
     const int cell = blockIdx.x + 1;
     if (moc_unchanged(a.TeOld, a.NeOld, a.Te, a.Ne, cell)) return;
     const int lane = threadIdx.x;
@@ -527,13 +526,21 @@ __global__ void __launch_bounds__(64) k_em_lines(MocEmissionArgs a) {
             lines += nlevel_fixed<NL>(a.bal, atom, Te, Ne, up, true, a.abund[s] * X, 1.9865e9);
         }
     }
-    for (int b = 0; b < a.nForBig; ++b) {
-        const int k = a.forBig[b];
-        const int s = a.forSlot[k], ion = a.forIon[k];
-        const double X = sX[s][ion];
-        const double up = (ion < a.forTop[k] && X > 0.) ? sX[s][ion + 1] / X : 0.;
-        const double v = nlevel_wave(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9, nlw, lane);
-        if (lane == 0) lines += v;
+    // 9-15 levels: four ions at a time, 16 lanes each; 16-27: two, 32 lanes each
+    // (none when every ion fits the per-lane solve of NL = 5)
+    if constexpr (NL > 5)
+    for (int t = 0; t < 2; ++t) {
+        const int sh = t == 0 ? 4 : 5, per = 64 >> sh;
+        for (int b0 = 0; b0 < a.nForT[t]; b0 += per) {
+            const int bt = b0 + (lane >> sh);
+            const int k = a.forT[t][bt < a.nForT[t] ? bt : b0];  // idle teams repeat one
+            const int s = a.forSlot[k], ion = a.forIon[k];
+            const double X = sX[s][ion];
+            const double up = (ion < a.forTop[k] && X > 0.) ? sX[s][ion + 1] / X : 0.;
+            const double v = t == 0 ? nlevel_team<15>(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9)
+                                    : nlevel_team<27>(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9);
+            if ((lane & ((1 << sh) - 1)) == 0 && bt < a.nForT[t]) lines += v;
+        }
     }
     lines = wave_sum(lines);
     if (lane == 0) a.lineSum[cell] = lines;
@@ -552,10 +559,9 @@ __global__ void __launch_bounds__(64) k_line_lum(MocEmissionArgs a, const double
                                                  const double* __restrict__ H, int nChunks, int L,
                                                  int k0, int nk, double* __restrict__ part) {
     __shared__ double sX[MOC_MAXE][MOC_MAXS + 1];
-    extern __shared__ double acc[];  // [nk][L][L]: ions k0 .. k0+nk-1, then the wave solve's workspace
+    extern __shared__ double acc[];  // [nk][L][L]: ions k0 .. k0+nk-1
     const int lane = threadIdx.x;
     const int C1 = a.C1, nL = nk * L * L;
-    double* nlw = acc + nL;
     for (int k = lane; k < nL; k += 64) acc[k] = 0.;
     double hb = 0.;
     const int per = (C1 - 1 + nChunks - 1) / nChunks;
@@ -584,14 +590,22 @@ __global__ void __launch_bounds__(64) k_line_lum(MocEmissionArgs a, const double
             nlevel_fixed<NL>(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9,
                              acc + (size_t)kk * L * L, w, L);
         }
-        for (int b = 0; b < a.nForBig; ++b) {  // ions of more than NL levels, by the whole wave
-            const int k = a.forBig[b];
-            if (k < k0 || k >= k0 + nk) continue;
-            const int s = a.forSlot[k], ion = a.forIon[k];
-            const double X = sX[s][ion];
-            const double up = (ion < a.forTop[k] && X > 0.) ? sX[s][ion + 1] / X : 0.;
-            nlevel_wave(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9, nlw, lane,
-                        acc + (size_t)(k - k0) * L * L, w, L);
+        // ions of more than NL levels, by teams of 16 (9-15 levels) or 32 lanes;
+        // a team without an ion of this group solves one but adds nothing
+        if constexpr (NL > 5)
+        for (int t = 0; t < 2; ++t) {
+            const int sh = t == 0 ? 4 : 5, per = 64 >> sh;
+            for (int b0 = 0; b0 < a.nForT[t]; b0 += per) {
+                const int bt = b0 + (lane >> sh);
+                const int k = a.forT[t][bt < a.nForT[t] ? bt : b0];
+                const bool mine = bt < a.nForT[t] && k >= k0 && k < k0 + nk;
+                const int s = a.forSlot[k], ion = a.forIon[k];
+                const double X = sX[s][ion];
+                const double up = (ion < a.forTop[k] && X > 0.) ? sX[s][ion + 1] / X : 0.;
+                double* la = mine ? acc + (size_t)(k - k0) * L * L : nullptr;
+                if (t == 0) nlevel_team<15>(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9, la, w, L);
+                else nlevel_team<27>(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9, la, w, L);
+            }
         }
         if (lane == 0 && k0 == 0 && hs >= 0 && a.NS > 1) {  // hydroLines(1,4,2) (RecLinesEmission)
             const HydroLines h = hydro_setup(a, 1, sX[hs][2], a.abund[hs], Te, Ne, log10Ne, true);
@@ -607,19 +621,15 @@ __global__ void __launch_bounds__(64) k_line_lum(MocEmissionArgs a, const double
 
 }  // namespace
 
-// LDS of the wave solve's workspace in the line kernels
-static size_t em_wave_bytes(const MocEmissionArgs& a) {
-    return (size_t)nlevel_wave_doubles(a.forBigLev) * sizeof(double);
-}
-
 int moc_line_lum_group(const MocEmissionArgs& a, int L) {
-    const int g = (int)((kLineLumLds - em_wave_bytes(a)) / ((size_t)L * L * sizeof(double)));
+    (void)a;
+    const int g = (int)(kLineLumLds / ((size_t)L * L * sizeof(double)));
     return g < 1 ? 1 : g;
 }
 
 int moc_launch_line_lum(moc_ctx* ctx, const MocEmissionArgs& a, const double* vol, const int* black,
                         const double* H, int nChunks, int L, int k0, int nk, double* part) {
-    const size_t lds = (size_t)nk * L * L * sizeof(double) + em_wave_bytes(a);
+    const size_t lds = (size_t)nk * L * L * sizeof(double);
     if (nk < (a.nFor > 0 ? 1 : 0) || k0 < 0 || k0 + nk > a.nFor || lds > (size_t)kLineLumLds) {
         ctx->err = "line luminosities: ion group " + std::to_string(k0) + "+" + std::to_string(nk) +
                    " of " + std::to_string(a.nFor) + " does not fit";
@@ -641,9 +651,8 @@ int moc_launch_emission(moc_ctx* ctx, const MocEmissionArgs& a) {
     if (a.nCells < 1) return MOC_OK;
     const int maxLev = a.bal.maxLev;
     const dim3 grid((unsigned)a.nCells);
-    const size_t lds = em_wave_bytes(a);
-    if (maxLev <= 5) hipLaunchKernelGGL(k_em_lines<5>, grid, dim3(64), lds, ctx->stream, a);
-    else hipLaunchKernelGGL(k_em_lines<8>, grid, dim3(64), lds, ctx->stream, a);
+    if (maxLev <= 5) hipLaunchKernelGGL(k_em_lines<5>, grid, dim3(64), 0, ctx->stream, a);
+    else hipLaunchKernelGGL(k_em_lines<8>, grid, dim3(64), 0, ctx->stream, a);
     hipLaunchKernelGGL(k_emission, grid, dim3(kThreads), 0, ctx->stream, a);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

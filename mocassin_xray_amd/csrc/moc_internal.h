@@ -165,11 +165,11 @@ struct MocBalanceArgs {
     const int* nlvIon;
     const int* nlvAtom;
     // ions whose level count (min(file levels, maxLev)) exceeds 8 are solved
-    // by a whole wave with the matrix in LDS (nlevel_wave), the others one ion
-    // per lane with the matrix in registers
-    int nBig;                 // largest level count of those ions (0: none)
-    int nNlvBig;              // entries k of the nlv list solved by the wave
-    const int* nlvBig;
+    // by teams of lanes with the matrix in registers (nlevel_team): [0] the
+    // entries k of the nlv list with 9-15 levels (teams of 16 lanes), [1]
+    // those with 16-27 levels (teams of 32); the others one ion per lane
+    int nNlvT[2];
+    const int* nlvT[2];
     int colHit, colComp;      // rate columns of the hit test and the Compton sums
     int ncol;                 // rate columns per cell
     int nX;                   // entries of a packed per-cell stage row (sum over elements
@@ -258,9 +258,8 @@ struct MocEmissionArgs {
     const int* forIon;
     const int* forAtom;
     const int* forTop;       // min(Z+1, nstages)
-    int nForBig;             // entries k of the for list solved by a whole wave (more than 8 levels)
-    const int* forBig;
-    int forBigLev;           // largest level count among them (0: none)
+    int nForT[2];            // entries k of the for list solved by teams of 16 (9-15 levels)
+    const int* forT[2];      // and of 32 lanes (16-27 levels)
     const int* hlMap;        // [9][12]
     const double* hlDens;    // [files][14]
     const double* hlLines;   // [files][14][16][9]

@@ -1049,15 +1049,18 @@ int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* t) {
         PUT(a.nlvSlot, ns.data(), ns.size(), "nlvSlot");
         PUT(a.nlvIon, ni.data(), ni.size(), "nlvIon");
         PUT(a.nlvAtom, na.data(), na.size(), "nlvAtom");
-        // ions of more than 8 levels (as cut at maxLev) are solved by a whole wave
-        std::vector<int> big;
-        a.nBig = 0;
+        // ions of more than 8 levels (as cut at maxLev) are solved by teams of
+        // 16 (9-15 levels) or 32 lanes (16-27)
+        std::vector<int> t16, t32;
         for (int k = 0; k < (int)na.size(); ++k) {
             const int n = (int)std::min<int64_t>(t->atomMeta[8 * na[k]], t->maxLev);
-            if (n > 8) { big.push_back(k); a.nBig = std::max(a.nBig, n); }
+            if (n > 15) t32.push_back(k);
+            else if (n > 8) t16.push_back(k);
         }
-        a.nNlvBig = (int)big.size();
-        PUT(a.nlvBig, big.data(), big.size(), "nlvBig");
+        a.nNlvT[0] = (int)t16.size();
+        a.nNlvT[1] = (int)t32.size();
+        PUT(a.nlvT[0], t16.data(), t16.size(), "nlvT16");
+        PUT(a.nlvT[1], t32.data(), t32.size(), "nlvT32");
         // level counts as solved, for the emission's line lists
         b.atomLevels.resize((size_t)t->nAtoms);
         for (int k = 0; k < t->nAtoms; ++k)
@@ -1289,14 +1292,18 @@ int moc_set_emission(moc_ctx* ctx, const moc_emission_tables* t) {
     EPUT(a.forAtom, forAtom.data(), forAtom.size(), "forAtom");
     EPUT(a.forTop, forTop.data(), forTop.size(), "forTop");
     {
-        // ions of more than 8 levels (as cut at maxLev) are solved by a whole wave
-        std::vector<int> big;
-        a.forBigLev = 0;
+        // ions of more than 8 levels (as cut at maxLev) are solved by teams of
+        // 16 (9-15 levels) or 32 lanes (16-27)
+        std::vector<int> t16, t32;
         const std::vector<int>& lev = ctx->balance.atomLevels;
-        for (int k = 0; k < (int)forAtom.size(); ++k)
-            if (lev[forAtom[k]] > 8) { big.push_back(k); a.forBigLev = std::max(a.forBigLev, lev[forAtom[k]]); }
-        a.nForBig = (int)big.size();
-        EPUT(a.forBig, big.data(), big.size(), "forBig");
+        for (int k = 0; k < (int)forAtom.size(); ++k) {
+            if (lev[forAtom[k]] > 15) t32.push_back(k);
+            else if (lev[forAtom[k]] > 8) t16.push_back(k);
+        }
+        a.nForT[0] = (int)t16.size();
+        a.nForT[1] = (int)t32.size();
+        EPUT(a.forT[0], t16.data(), t16.size(), "forT16");
+        EPUT(a.forT[1], t32.data(), t32.size(), "forT32");
     }
     EPUT(a.hlMap, t->hlMap, 9 * 12, "hlMap");
     EPUT(a.hlDens, t->hlDens, 14 * (size_t)t->nHlFiles, "hlDens");
