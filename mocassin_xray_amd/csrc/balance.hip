@@ -601,36 +601,40 @@ __device__ void g_terms(const MocBalanceArgs& a, const Lds& L, const GrainLds& G
     G.cg[g] = cg;
 }
 
-// thermBalance (:887-1215): heating and cooling [erg/s per n_H]
-template <int NL, class RT>
+// forbidden-line cooling of the ions of list class c (nlevel_team_class),
+// P = 64 / W ions at a time, one per team of W lanes; idle teams solve the
+// group's first ion again and store nothing
+template <int NL>
+__device__ void team_cooling(const MocBalanceArgs& a, const Lds& L, int cnt, const int* list, double T,
+                             double Ne, int tid) {
+    constexpr int W = NL <= 8 ? 8 : (NL <= 16 ? 16 : 32), P = 64 / W;
+    for (int b0 = 0; b0 < cnt; b0 += P) {
+        const int bt = b0 + tid / W;
+        const int k = list[bt < cnt ? bt : b0];
+        const int e = a.nlvSlot[k], ion = a.nlvIon[k];
+        const double v = nlevel_team<NL>(a, a.nlvAtom[k], T, Ne, 0., false, a.abund[e] * L.x(e, ion), 1.9865e-16);
+        if ((tid & (W - 1)) == 0 && bt < cnt) L.cool[k] = v;
+    }
+}
+
+// thermBalance (:887-1215): heating and cooling [erg/s per n_H].  TEAMS: some
+// ions have more than kNlevLane levels (their solves run in teams of lanes).
+template <bool TEAMS, class RT>
 __device__ void therm(const MocBalanceArgs& a, const RT& r, const Lds& L, double T, double Ne,
                       double H, double heatef, const double comp[4], double* heat, double* cool,
                       int tid, double grainHeat = 0., double grainCool = 0., bool grains = false) {
-    // forbidden-line cooling: ions of up to NL levels one per lane with the
-    // matrix in registers, larger ones by the whole wave in LDS
+    // forbidden-line cooling: ions of up to kNlevLane levels one per lane with
+    // the matrix in registers, larger ones by teams of 8, 16 or 32 lanes
     for (int k = tid; k < a.nNlv; k += 64) {
         const int atom = a.nlvAtom[k];
-        if (nlevel_count(a, atom) > NL) continue;
+        if (nlevel_count(a, atom) > kNlevLane) continue;
         const int e = a.nlvSlot[k], ion = a.nlvIon[k];
-        L.cool[k] = nlevel_fixed<NL>(a, atom, T, Ne, 0., false, a.abund[e] * L.x(e, ion), 1.9865e-16);
+        L.cool[k] = nlevel_fixed<kNlevLane>(a, atom, T, Ne, 0., false, a.abund[e] * L.x(e, ion), 1.9865e-16);
     }
-    // 9-15 levels: four ions at a time, 16 lanes each; 16-27: two, 32 lanes each
-    // (none when every ion fits the per-lane solve of NL = 5)
-    if constexpr (NL > 5) {
-    for (int b0 = 0; b0 < a.nNlvT[0]; b0 += 4) {
-        const int b = b0 + (tid >> 4) < a.nNlvT[0] ? b0 + (tid >> 4) : b0;  // idle teams repeat one
-        const int k = a.nlvT[0][b];
-        const int e = a.nlvSlot[k], ion = a.nlvIon[k];
-        const double v = nlevel_team<15>(a, a.nlvAtom[k], T, Ne, 0., false, a.abund[e] * L.x(e, ion), 1.9865e-16);
-        if ((tid & 15) == 0 && b0 + (tid >> 4) < a.nNlvT[0]) L.cool[k] = v;
-    }
-    for (int b0 = 0; b0 < a.nNlvT[1]; b0 += 2) {
-        const int b = b0 + (tid >> 5) < a.nNlvT[1] ? b0 + (tid >> 5) : b0;
-        const int k = a.nlvT[1][b];
-        const int e = a.nlvSlot[k], ion = a.nlvIon[k];
-        const double v = nlevel_team<27>(a, a.nlvAtom[k], T, Ne, 0., false, a.abund[e] * L.x(e, ion), 1.9865e-16);
-        if ((tid & 31) == 0 && b0 + (tid >> 5) < a.nNlvT[1]) L.cool[k] = v;
-    }
+    if constexpr (TEAMS) {
+        team_cooling<8>(a, L, a.nNlvT[0], a.nlvT[0], T, Ne, tid);
+        team_cooling<16>(a, L, a.nNlvT[1], a.nlvT[1], T, Ne, tid);
+        team_cooling<27>(a, L, a.nNlvT[2], a.nlvT[2], T, Ne, tid);
     }
     __syncthreads();
     double coolFF = 0., coolRec = 0., coolColl, coolCollH = 0., heatSte = 0., Te4 = T / 1.e4;
@@ -702,7 +706,7 @@ __device__ void therm(const MocBalanceArgs& a, const RT& r, const Lds& L, double
 // capped at 128 VGPRs with a few hundred bytes of spills it takes 13.6 ms.
 // PE: gas + dust runs with grain charge, photoelectric heating and
 // gas-grain collisions (the rows of GrainLds follow the other LDS rows).
-template <int NL, bool ST, bool PE>
+template <bool TEAMS, bool ST, bool PE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 k_balance(MocBalanceArgs a) {
     extern __shared__ double lds[];
@@ -817,9 +821,9 @@ k_balance(MocBalanceArgs a) {
                 cg = cg + G.cg[g];
             }
             const double peG = pg * a.Ndust[cell] * kHcRyd / H, collG = cg / (4. * kPiB);
-            therm<NL>(a, r, L, Te, Ne, H, heatef, comp, &heat, &cool, tid, peG, collG, true);
+            therm<TEAMS>(a, r, L, Te, Ne, H, heatef, comp, &heat, &cool, tid, peG, collG, true);
         } else {
-            therm<NL>(a, r, L, Te, Ne, H, heatef, comp, &heat, &cool, tid);
+            therm<TEAMS>(a, r, L, Te, Ne, H, heatef, comp, &heat, &cool, tid);
         }
         const double res = heat - cool;
         if (fabs(res) >= 0.02 * (fabs(cool) + fabs(heat)) / 2.) {
@@ -867,16 +871,16 @@ k_balance(MocBalanceArgs a) {
     }
 }
 
-// the per-lane register solve is sized by the level counts in use (at most 8;
-// larger ions go to the wave solve)
+// the team solves are compiled in only where some ion has more than
+// kNlevLane levels
 template <bool PE>
 void launch_balance(moc_ctx* ctx, const MocBalanceArgs& a, bool st, dim3 grid, dim3 block, size_t lds) {
-    if (a.maxLev <= 5) {
-        if (st) hipLaunchKernelGGL((k_balance<5, true, PE>), grid, block, lds, ctx->stream, a);
-        else hipLaunchKernelGGL((k_balance<5, false, PE>), grid, block, lds, ctx->stream, a);
+    if (a.maxLev <= kNlevLane) {
+        if (st) hipLaunchKernelGGL((k_balance<false, true, PE>), grid, block, lds, ctx->stream, a);
+        else hipLaunchKernelGGL((k_balance<false, false, PE>), grid, block, lds, ctx->stream, a);
     } else {
-        if (st) hipLaunchKernelGGL((k_balance<8, true, PE>), grid, block, lds, ctx->stream, a);
-        else hipLaunchKernelGGL((k_balance<8, false, PE>), grid, block, lds, ctx->stream, a);
+        if (st) hipLaunchKernelGGL((k_balance<true, true, PE>), grid, block, lds, ctx->stream, a);
+        else hipLaunchKernelGGL((k_balance<true, false, PE>), grid, block, lds, ctx->stream, a);
     }
 }
 

@@ -485,12 +485,37 @@ k_emission(MocEmissionArgs a) {
     }
 }
 
-// The lines of a cell (RecLinesEmission :856-1153, forLines :1481-1526): the
-// hydrogenic elements, the 34 He I lines and the n-level ions of up to NL
-// levels are separate tasks, one per lane of a single wave per cell; the
-// larger n-level ions are then solved one after the other by the whole wave
-// (matrix in LDS).  Their sum goes to lineSum for the CDF kernel.
+// n-level line emission of the ions of team class c (nlevel_team_class), P =
+// 64 / W ions at a time, one per team of W lanes; with lineAcc (the line
+// luminosities) only the ions k0 .. k0 + nk - 1 of the launch's group add
+// their lines to acc.  Returns this lane's share of the line sum (the team's
+// sum on the team's first lane).
 template <int NL>
+__device__ double team_lines(const MocEmissionArgs& a, const double (*sX)[MOC_MAXS + 1], int cnt,
+                             const int* list, double Te, double Ne, int lane, double* acc = nullptr,
+                             double w = 0., int L = 0, int k0 = 0, int nk = 0) {
+    constexpr int W = NL <= 8 ? 8 : (NL <= 16 ? 16 : 32), P = 64 / W;
+    double lines = 0.;
+    for (int b0 = 0; b0 < cnt; b0 += P) {
+        const int bt = b0 + lane / W;
+        const int k = list[bt < cnt ? bt : b0];  // idle teams repeat one
+        const int s = a.forSlot[k], ion = a.forIon[k];
+        const double X = sX[s][ion];
+        const double up = (ion < a.forTop[k] && X > 0.) ? sX[s][ion + 1] / X : 0.;
+        double* la = nullptr;
+        if (acc && bt < cnt && k >= k0 && k < k0 + nk) la = acc + (size_t)(k - k0) * L * L;
+        const double v = nlevel_team<NL>(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9, la, w, L);
+        if ((lane & (W - 1)) == 0 && bt < cnt) lines += v;
+    }
+    return lines;
+}
+
+// The lines of a cell (RecLinesEmission :856-1153, forLines :1481-1526): the
+// hydrogenic elements, the 34 He I lines and the n-level ions of up to
+// kNlevLane levels are separate tasks, one per lane of a single wave per
+// cell; the larger n-level ions are then solved by teams of 8, 16 or 32
+// lanes.  Their sum goes to lineSum for the CDF kernel.
+template <bool TEAMS>
 __global__ void __launch_bounds__(64) k_em_lines(MocEmissionArgs a) {
     __shared__ double sX[MOC_MAXE][MOC_MAXS + 1];
     const int cell = blockIdx.x + 1;
@@ -519,28 +544,17 @@ __global__ void __launch_bounds__(64) k_em_lines(MocEmissionArgs a) {
         } else {
             const int k = t - a.nHyd - kHeILines;
             const int atom = a.forAtom[k];
-            if (nlevel_count(a.bal, atom) > NL) continue;
+            if (nlevel_count(a.bal, atom) > kNlevLane) continue;
             const int s = a.forSlot[k], ion = a.forIon[k];
             const double X = sX[s][ion];
             const double up = (ion < a.forTop[k] && X > 0.) ? sX[s][ion + 1] / X : 0.;
-            lines += nlevel_fixed<NL>(a.bal, atom, Te, Ne, up, true, a.abund[s] * X, 1.9865e9);
+            lines += nlevel_fixed<kNlevLane>(a.bal, atom, Te, Ne, up, true, a.abund[s] * X, 1.9865e9);
         }
     }
-    // 9-15 levels: four ions at a time, 16 lanes each; 16-27: two, 32 lanes each
-    // (none when every ion fits the per-lane solve of NL = 5)
-    if constexpr (NL > 5)
-    for (int t = 0; t < 2; ++t) {
-        const int sh = t == 0 ? 4 : 5, per = 64 >> sh;
-        for (int b0 = 0; b0 < a.nForT[t]; b0 += per) {
-            const int bt = b0 + (lane >> sh);
-            const int k = a.forT[t][bt < a.nForT[t] ? bt : b0];  // idle teams repeat one
-            const int s = a.forSlot[k], ion = a.forIon[k];
-            const double X = sX[s][ion];
-            const double up = (ion < a.forTop[k] && X > 0.) ? sX[s][ion + 1] / X : 0.;
-            const double v = t == 0 ? nlevel_team<15>(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9)
-                                    : nlevel_team<27>(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9);
-            if ((lane & ((1 << sh) - 1)) == 0 && bt < a.nForT[t]) lines += v;
-        }
+    if constexpr (TEAMS) {
+        lines += team_lines<8>(a, sX, a.nForT[0], a.forT[0], Te, Ne, lane);
+        lines += team_lines<16>(a, sX, a.nForT[1], a.forT[1], Te, Ne, lane);
+        lines += team_lines<27>(a, sX, a.nForT[2], a.forT[2], Te, Ne, lane);
     }
     lines = wave_sum(lines);
     if (lane == 0) a.lineSum[cell] = lines;
@@ -553,7 +567,7 @@ __global__ void __launch_bounds__(64) k_em_lines(MocEmissionArgs a) {
 // n-level ion k0 + k (and k + 64, ...) of the launch's ion group and
 // accumulates its lines in LDS; the block writes its sums, which the host adds
 // block by block.  The first group also carries Hbeta.
-template <int NL>
+template <bool TEAMS>
 __global__ void __launch_bounds__(64) k_line_lum(MocEmissionArgs a, const double* __restrict__ vol,
                                                  const int* __restrict__ black,
                                                  const double* __restrict__ H, int nChunks, int L,
@@ -583,29 +597,17 @@ __global__ void __launch_bounds__(64) k_line_lum(MocEmissionArgs a, const double
         const double w = H[cell] * vol[cell];
         for (int kk = lane; kk < nk; kk += 64) {
             const int k = k0 + kk;
-            if (nlevel_count(a.bal, a.forAtom[k]) > NL) continue;
+            if (nlevel_count(a.bal, a.forAtom[k]) > kNlevLane) continue;
             const int s = a.forSlot[k], ion = a.forIon[k];
             const double X = sX[s][ion];
             const double up = (ion < a.forTop[k] && X > 0.) ? sX[s][ion + 1] / X : 0.;
-            nlevel_fixed<NL>(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9,
+            nlevel_fixed<kNlevLane>(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9,
                              acc + (size_t)kk * L * L, w, L);
         }
-        // ions of more than NL levels, by teams of 16 (9-15 levels) or 32 lanes;
-        // a team without an ion of this group solves one but adds nothing
-        if constexpr (NL > 5)
-        for (int t = 0; t < 2; ++t) {
-            const int sh = t == 0 ? 4 : 5, per = 64 >> sh;
-            for (int b0 = 0; b0 < a.nForT[t]; b0 += per) {
-                const int bt = b0 + (lane >> sh);
-                const int k = a.forT[t][bt < a.nForT[t] ? bt : b0];
-                const bool mine = bt < a.nForT[t] && k >= k0 && k < k0 + nk;
-                const int s = a.forSlot[k], ion = a.forIon[k];
-                const double X = sX[s][ion];
-                const double up = (ion < a.forTop[k] && X > 0.) ? sX[s][ion + 1] / X : 0.;
-                double* la = mine ? acc + (size_t)(k - k0) * L * L : nullptr;
-                if (t == 0) nlevel_team<15>(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9, la, w, L);
-                else nlevel_team<27>(a.bal, a.forAtom[k], Te, Ne, up, true, a.abund[s] * X, 1.9865e9, la, w, L);
-            }
+        if constexpr (TEAMS) {  // ions of more than kNlevLane levels, by teams of lanes
+            team_lines<8>(a, sX, a.nForT[0], a.forT[0], Te, Ne, lane, acc, w, L, k0, nk);
+            team_lines<16>(a, sX, a.nForT[1], a.forT[1], Te, Ne, lane, acc, w, L, k0, nk);
+            team_lines<27>(a, sX, a.nForT[2], a.forT[2], Te, Ne, lane, acc, w, L, k0, nk);
         }
         if (lane == 0 && k0 == 0 && hs >= 0 && a.NS > 1) {  // hydroLines(1,4,2) (RecLinesEmission)
             const HydroLines h = hydro_setup(a, 1, sX[hs][2], a.abund[hs], Te, Ne, log10Ne, true);
@@ -637,8 +639,8 @@ int moc_launch_line_lum(moc_ctx* ctx, const MocEmissionArgs& a, const double* vo
     }
     const int maxLev = a.bal.maxLev;
     const dim3 grid((unsigned)nChunks), block(64);
-    if (maxLev <= 5) hipLaunchKernelGGL(k_line_lum<5>, grid, block, lds, ctx->stream, a, vol, black, H, nChunks, L, k0, nk, part);
-    else hipLaunchKernelGGL(k_line_lum<8>, grid, block, lds, ctx->stream, a, vol, black, H, nChunks, L, k0, nk, part);
+    if (maxLev <= kNlevLane) hipLaunchKernelGGL(k_line_lum<false>, grid, block, lds, ctx->stream, a, vol, black, H, nChunks, L, k0, nk, part);
+    else hipLaunchKernelGGL(k_line_lum<true>, grid, block, lds, ctx->stream, a, vol, black, H, nChunks, L, k0, nk, part);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         ctx->err = std::string("line luminosities launch: ") + hipGetErrorString(e);
@@ -651,8 +653,8 @@ int moc_launch_emission(moc_ctx* ctx, const MocEmissionArgs& a) {
     if (a.nCells < 1) return MOC_OK;
     const int maxLev = a.bal.maxLev;
     const dim3 grid((unsigned)a.nCells);
-    if (maxLev <= 5) hipLaunchKernelGGL(k_em_lines<5>, grid, dim3(64), 0, ctx->stream, a);
-    else hipLaunchKernelGGL(k_em_lines<8>, grid, dim3(64), 0, ctx->stream, a);
+    if (maxLev <= kNlevLane) hipLaunchKernelGGL(k_em_lines<false>, grid, dim3(64), 0, ctx->stream, a);
+    else hipLaunchKernelGGL(k_em_lines<true>, grid, dim3(64), 0, ctx->stream, a);
     hipLaunchKernelGGL(k_emission, grid, dim3(kThreads), 0, ctx->stream, a);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

@@ -164,12 +164,12 @@ struct MocBalanceArgs {
     const int* nlvSlot;
     const int* nlvIon;
     const int* nlvAtom;
-    // ions whose level count (min(file levels, maxLev)) exceeds 8 are solved
-    // by teams of lanes with the matrix in registers (nlevel_team): [0] the
-    // entries k of the nlv list with 9-15 levels (teams of 16 lanes), [1]
-    // those with 16-27 levels (teams of 32); the others one ion per lane
-    int nNlvT[2];
-    const int* nlvT[2];
+    // ions whose level count (min(file levels, maxLev)) exceeds 5 are solved
+    // by teams of lanes with the matrix in registers (nlevel_team): the
+    // entries k of the nlv list with [0] 6-8 levels (teams of 8 lanes), [1]
+    // 9-16 (teams of 16), [2] 17-27 (teams of 32); the others one ion per lane
+    int nNlvT[3];
+    const int* nlvT[3];
     int colHit, colComp;      // rate columns of the hit test and the Compton sums
     int ncol;                 // rate columns per cell
     int nX;                   // entries of a packed per-cell stage row (sum over elements
@@ -258,8 +258,8 @@ struct MocEmissionArgs {
     const int* forIon;
     const int* forAtom;
     const int* forTop;       // min(Z+1, nstages)
-    int nForT[2];            // entries k of the for list solved by teams of 16 (9-15 levels)
-    const int* forT[2];      // and of 32 lanes (16-27 levels)
+    int nForT[3];            // entries k of the for list solved by teams of 8, 16 and 32 lanes
+    const int* forT[3];      // (6-8, 9-16, 17-27 levels)
     const int* hlMap;        // [9][12]
     const double* hlDens;    // [files][14]
     const double* hlLines;   // [files][14][16][9]

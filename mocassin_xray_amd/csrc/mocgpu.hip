@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "moc_internal.h"
+#include "nlevel.h"
 
 // Device buffer of the allocation-free entry point moc_volume_averages_noalloc:
 // the volume-average partials (at most 31 x 31 ion pairs x 256 chunks x 2) and
@@ -1049,18 +1050,17 @@ int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* t) {
         PUT(a.nlvSlot, ns.data(), ns.size(), "nlvSlot");
         PUT(a.nlvIon, ni.data(), ni.size(), "nlvIon");
         PUT(a.nlvAtom, na.data(), na.size(), "nlvAtom");
-        // ions of more than 8 levels (as cut at maxLev) are solved by teams of
-        // 16 (9-15 levels) or 32 lanes (16-27)
-        std::vector<int> t16, t32;
+        // ions of more than 5 levels (as cut at maxLev) are solved by teams of
+        // 8, 16 or 32 lanes
+        std::vector<int> tl[3];
         for (int k = 0; k < (int)na.size(); ++k) {
-            const int n = (int)std::min<int64_t>(t->atomMeta[8 * na[k]], t->maxLev);
-            if (n > 15) t32.push_back(k);
-            else if (n > 8) t16.push_back(k);
+            const int c = nlevel_team_class((int)std::min<int64_t>(t->atomMeta[8 * na[k]], t->maxLev));
+            if (c >= 0) tl[c].push_back(k);
         }
-        a.nNlvT[0] = (int)t16.size();
-        a.nNlvT[1] = (int)t32.size();
-        PUT(a.nlvT[0], t16.data(), t16.size(), "nlvT16");
-        PUT(a.nlvT[1], t32.data(), t32.size(), "nlvT32");
+        for (int c = 0; c < 3; ++c) {
+            a.nNlvT[c] = (int)tl[c].size();
+            PUT(a.nlvT[c], tl[c].data(), tl[c].size(), "nlvT");
+        }
         // level counts as solved, for the emission's line lists
         b.atomLevels.resize((size_t)t->nAtoms);
         for (int k = 0; k < t->nAtoms; ++k)
@@ -1292,18 +1292,18 @@ int moc_set_emission(moc_ctx* ctx, const moc_emission_tables* t) {
     EPUT(a.forAtom, forAtom.data(), forAtom.size(), "forAtom");
     EPUT(a.forTop, forTop.data(), forTop.size(), "forTop");
     {
-        // ions of more than 8 levels (as cut at maxLev) are solved by teams of
-        // 16 (9-15 levels) or 32 lanes (16-27)
-        std::vector<int> t16, t32;
+        // ions of more than 5 levels (as cut at maxLev) are solved by teams of
+        // 8, 16 or 32 lanes
+        std::vector<int> tl[3];
         const std::vector<int>& lev = ctx->balance.atomLevels;
         for (int k = 0; k < (int)forAtom.size(); ++k) {
-            if (lev[forAtom[k]] > 15) t32.push_back(k);
-            else if (lev[forAtom[k]] > 8) t16.push_back(k);
+            const int c = nlevel_team_class(lev[forAtom[k]]);
+            if (c >= 0) tl[c].push_back(k);
         }
-        a.nForT[0] = (int)t16.size();
-        a.nForT[1] = (int)t32.size();
-        EPUT(a.forT[0], t16.data(), t16.size(), "forT16");
-        EPUT(a.forT[1], t32.data(), t32.size(), "forT32");
+        for (int c = 0; c < 3; ++c) {
+            a.nForT[c] = (int)tl[c].size();
+            EPUT(a.forT[c], tl[c].data(), tl[c].size(), "forT");
+        }
     }
     EPUT(a.hlMap, t->hlMap, 9 * 12, "hlMap");
     EPUT(a.hlDens, t->hlDens, 14 * (size_t)t->nHlFiles, "hlDens");

@@ -1,6 +1,11 @@
 // n-level statistical equilibrium of one ion (equilibrium, emission_mod.f90:
 // 2505-3028, with luSlv :3031-3106), shared by the thermal balance (cooling,
 // rec = false) and the diffuse emission (line luminosity, rec = true).
+//
+// The collision-rate fits (the Upsilon(T) spline, the Boltzmann factor, the
+// recombination feed) are evaluated in the fit precision R (double, or float
+// in the mixed-precision form); the level matrix, its elimination and the
+// populations are always FP64.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -14,241 +19,115 @@ __device__ __forceinline__ int nlevel_count(const MocBalanceArgs& a, int atom) {
     return nf < a.maxLev ? nf : a.maxLev;
 }
 
-// Sum of the line emissivities A_ji dE_ij n_j * scale * unit of ion `atom`
-// that exceed 1e-35 (equilibrium, emission_mod.f90:2505-3028; luSlv
-// :3031-3106), solved by a team of W = 16 (NL = 15) or W = 32 (NL = 27)
-// lanes of one wave with the level matrix in registers: team lane c < n owns
-// column c of the matrix, the team's last lane owns the right-hand side (as
-// one more column, always to the right of the pivot), and the pivot column's
-// factors, the back-substitution entries and the emissivities travel between
-// lanes by shuffles.  The ion's level count n must not exceed NL.  Every lane
-// of the team calls it with the same ion; the return value is the same on
-// every lane of the team.  The loops are unrolled over NL and the entries
-// past n are carried but never read by the entries below n, whose operations
-// are those of a sequential evaluation in the same order (each column built
-// from row c of the collision rates, each elimination step x[k][j] -= x[i][j]
-// (x[k][i] / x[i][i]), the back substitution row by row on the right-hand
-// side's lane, the emissivities summed in (lower, upper) order), so the result
-// is the sequential one bit for bit.
-//   rec      levels 2.. are fed by recombination from the next stage
-//            (ionDenUp = X(i+1)/X(i)) at the rates of the ion's data file
-//            (:2828-2877; kind 1: power-law radiative plus dielectronic fit per
-//            level, kind 2: cubic in log T);
-//   lineAcc  with it, every line's emissivity (lower i, upper j: fLineEm(i, j),
-//            :2987-3005) times w is added to lineAcc[i * ld + j] by the lane of
-//            column j.
-template <int NL>
-__device__ __attribute__((noinline)) double nlevel_team(const MocBalanceArgs& a, int atom, double Te, double Ne,
-                              double ionDenUp, bool rec, double scale, double unit,
-                              double* lineAcc = nullptr, double w = 0., int ld = 0) {
-    constexpr int W = NL < 16 ? 16 : 32;
-    static_assert(NL < W, "the right-hand side needs a lane of its own");
-    const int c = (int)(threadIdx.x & (W - 1));  // column of this lane
-    constexpr int RHS = W - 1;
-    const int64_t* m = a.atomMeta + 8 * atom;
-    const int nf = (int)m[0], nT = (int)m[1];
-    const double* lt = a.atomLogT + m[2];
-    const double* g = a.atomG + m[3];
-    const double* en = a.atomE + m[3];
-    const double* A = a.atomA + m[4];
-    const double* q = a.atomQom + m[5];
-    const double* q2 = a.atomQ2 + m[5];
-    const int iup2p = (int)m[6];
-    const double atp = a.atomAtp[atom];
-    const int n = nf < a.maxLev ? nf : a.maxLev;
-    const double sqrTe = sqrt(Te), lT = log10(Te);
-    int klo = 0, khi = nT - 1;
-    double sa = 0., sb = 0., h = 1.;
-    const bool below = lT <= lt[0], above = lT >= lt[nT - 1];
-    if (nT > 2 && !below && !above) {
-        while (khi - klo > 1) {
-            const int k = (khi + klo) >> 1;
-            if (lt[k] > lT) khi = k; else klo = k;
+// One ion's data and the interpolation of its Upsilon(T) tables at T_e: the
+// natural cubic spline in log T (second derivatives from the host), clamped at
+// the table ends; linear in log T with two temperatures; the one value with
+// one (:2884-2905).  Every case is written as
+//   Upsilon = wa q[lo] + wb q[hi] + ca q2[lo] + cb q2[hi],
+// so that a pair costs four loads and no branch.
+template <class R>
+struct NlevIon {
+    int nf, n, iup2p;
+    const double *g, *en, *A, *q, *q2;
+    double atp;
+    size_t lo, hi;      // offsets of the bracketing tables
+    R wa, wb, ca, cb;   // weights of the bracketing values and second derivatives
+    R lT, Te, cs, kT;   // log10 T, T, 8.63e-6 / sqrt(T), 1.4388463 / T
+
+    __device__ NlevIon(const MocBalanceArgs& a, int atom, double Te_) {
+        const int64_t* m = a.atomMeta + 8 * atom;
+        nf = (int)m[0];
+        const int nT = (int)m[1];
+        const double* lt = a.atomLogT + m[2];
+        g = a.atomG + m[3];
+        en = a.atomE + m[3];
+        A = a.atomA + m[4];
+        q = a.atomQom + m[5];
+        q2 = a.atomQ2 + m[5];
+        iup2p = (int)m[6];
+        atp = a.atomAtp[atom];
+        n = nf < a.maxLev ? nf : a.maxLev;
+        Te = (R)Te_;
+        lT = log10(Te);
+        cs = (R)8.63e-6 / sqrt(Te);
+        kT = (R)1.4388463 / Te;
+        int klo = 0, khi = 0;
+        wa = 1;
+        wb = 0;
+        ca = 0;
+        cb = 0;
+        if (nT == 2) {
+            const R d = (R)lt[1] - (R)lt[0];
+            khi = 1;
+            wa = ((R)lt[1] - lT) / d;
+            wb = (lT - (R)lt[0]) / d;
+        } else if (nT > 2) {
+            if (lT >= (R)lt[nT - 1]) {
+                klo = khi = nT - 1;
+            } else if (lT > (R)lt[0]) {
+                khi = nT - 1;
+                while (khi - klo > 1) {
+                    const int k = (khi + klo) >> 1;
+                    if ((R)lt[k] > lT) khi = k; else klo = k;
+                }
+                const R h = (R)lt[khi] - (R)lt[klo];
+                wa = ((R)lt[khi] - lT) / h;
+                wb = (lT - (R)lt[klo]) / h;
+                ca = (wa * wa * wa - wa) * (h * h) / (R)6.;
+                cb = (wb * wb * wb - wb) * (h * h) / (R)6.;
+            }
         }
-        h = lt[khi] - lt[klo];
-        sa = (lt[khi] - lT) / h;
-        sb = (lT - lt[klo]) / h;
+        lo = (size_t)klo * nf * nf;
+        hi = (size_t)khi * nf * nf;
     }
-    // effective collision rate qe[p][r] (p != r) of the sequential routine
-    auto qef = [&](int p, int r) -> double {
-        const int i = p < r ? p : r, j = p < r ? r : p;
+
+    // collision rates of the pair i < j as equilibrium forms them
+    // (:2906-2936): qij from level i to level j and qji back; the Boltzmann
+    // factor goes with the upward rate and the lower level's weight
+    __device__ void pair(int i, int j, R& qij, R& qji) const {
         const size_t pp = (size_t)i * nf + j;
-        double ups;
-        if (nT == 1) ups = q[pp];
-        else if (nT == 2) ups = q[pp] + (q[(size_t)nf * nf + pp] - q[pp]) / (lt[1] - lt[0]) * (lT - lt[0]);
-        else if (below) ups = q[pp];
-        else if (above) ups = q[(size_t)(nT - 1) * nf * nf + pp];
-        else {
-            const double ylo = q[(size_t)klo * nf * nf + pp], yhi = q[(size_t)khi * nf * nf + pp];
-            const double dlo = q2[(size_t)klo * nf * nf + pp], dhi = q2[(size_t)khi * nf * nf + pp];
-            ups = sa * ylo + sb * yhi + ((sa * sa * sa - sa) * dlo + (sb * sb * sb - sb) * dhi) * (h * h) / 6.;
-        }
-        const double cc = 8.63e-6 * ups / sqrTe;
-        if (en[i] <= en[j]) return p == i ? cc * exp((en[i] - en[j]) * 1.4388463 / Te) / g[i] : cc / g[j];
-        return p == i ? cc / g[j] : cc * exp((en[j] - en[i]) * 1.4388463 / Te) / g[i];
-    };
-    // column c: row 0 the normalisation, x[i][c] = Ne qe[c][i] (+ A[c][i] above
-    // the diagonal), the diagonal minus the sum of row c's rates out of level c
-    double x[NL];
-    double d = 0.;
-#pragma unroll
-    for (int k = 0; k < NL; ++k) {
-        x[k] = 0.;
-        if (c < n && k < n && k != c) {
-            const double qk = qef(c, k);
-            if (k >= 1) {
-                x[k] += Ne * qk;
-                if (c > k) x[k] += A[(size_t)c * nf + k];
-            }
-            d -= Ne * qk;
-            if (k < c) d -= A[(size_t)c * nf + k];
-        }
+        const R ups = wa * (R)q[lo + pp] + wb * (R)q[hi + pp] + ca * (R)q2[lo + pp] + cb * (R)q2[hi + pp];
+        const R c = ups * cs;
+        const R ei = (R)en[i], ej = (R)en[j];
+        const bool up = ei <= ej;
+        const R bz = c * exp((up ? ei - ej : ej - ei) * kT) / (R)g[i];
+        const R dn = c / (R)g[j];
+        qij = up ? bz : dn;
+        qji = up ? dn : bz;
     }
-    x[0] = 1.;
-#pragma unroll
-    for (int k = 1; k < NL; ++k)
-        if (k == c) x[k] = d;
-    // the right-hand side: 1 in row 0, the recombination feed of each level
-    // (computed on the level's own lane) below
-    double feed = 0.;
-    const int kind = rec ? a.atomRecKind[atom] : 0;
-    if (kind != 0 && c >= 1 && c < n) {
-        const int* rv = a.atomRecValid + m[3];
-        double al = 0.;
-        if (rv[c]) {
-            const double* r = a.atomRec + 11 * m[3] + 11 * c;
-            const double t4 = Te * 1.e-4;
-            if (kind == 1) {
-                const double z = r[1], tz = t4 / (z * z);
-                al = 1.e-13 * r[0] * z * r[2] * pow(tz, r[3]) / (1. + r[4] * pow(tz, r[5])) +
-                     1.0e-12 * r[0] * (r[6] / t4 + r[7] + r[8] * t4 + r[9] * t4 * t4) * pow(t4, -1.5) *
-                         exp(-r[10] * 1.e4 / Te);
-            } else {
-                al = pow(10., r[0] + r[1] * lT + r[2] * lT * lT + r[3] * lT * lT * lT);
-            }
-        }
-        feed = -Ne * ionDenUp * al;
-    }
-#pragma unroll
-    for (int k = 1; k < NL; ++k) {
-        const double v = __shfl(feed, k, W);
-        if (c == RHS) x[k] = v;
-    }
-    // elimination without pivoting, as luSlv: the factors of step i come from
-    // column i; columns right of i (and the right-hand side) are updated
-#pragma unroll
-    for (int i = 0; i < NL - 1; ++i) {
-#pragma unroll
-        for (int k = i + 1; k < NL; ++k) {
-            const double f = __shfl(x[k] / x[i], i, W);
-            if (c > i) x[k] -= x[i] * f;
-        }
-    }
-    // back substitution on the right-hand side's lane (its x[] is y)
-#pragma unroll
-    for (int k = NL - 1; k >= 0; --k) {
-        double v = x[k];
-#pragma unroll
-        for (int j = k + 1; j < NL; ++j) {
-            const double xkj = __shfl(x[k], j, W);
-            if (j < n) v -= x[j] * xkj;
-        }
-        const double xkk = __shfl(x[k], k, W);
-        if (c == RHS && k < n) x[k] = v / xkk;
-    }
-    // populations: the sum on the right-hand side's lane, each level's own on its lane
-    double sumN = 0.;
-#pragma unroll
-    for (int i = 0; i < NL; ++i)
-        if (i < n) sumN += x[i];
-    sumN = __shfl(sumN, RHS, W);
-    double yc = 0.;
-#pragma unroll
-    for (int i = 0; i < NL; ++i) {
-        const double v = __shfl(x[i], RHS, W);
-        if (i == c) yc = v;
-    }
-    // emissivities of the lines out of level c (upper), into x[lower]
-#pragma unroll
-    for (int i = 0; i < NL; ++i) {
-        double em = 0.;
-        if (c < n && i < c) {
-            double av = A[(size_t)c * nf + i];
-            if (av != 0.) {
-                if (c + 1 == iup2p && i == 0 && atp > 0.) av -= atp;
-                em = av * (en[c] - en[i]) * (yc / sumN) * scale * unit;
-                if (lineAcc) lineAcc[i * ld + c] += em * w;
-            }
-        }
-        x[i] = em;
-    }
-    double tot = 0.;
-#pragma unroll
-    for (int i = 0; i < NL - 1; ++i)
-#pragma unroll
-        for (int j = i + 1; j < NL; ++j) {
-            const double em = __shfl(x[i], j, W);
-            if (j < n && em > 1.e-35) tot += em;
-        }
-    return tot;
-}
 
-}  // namespace
+    // recombination feed coefficient of level lev >= 1 (kind 1: radiative
+    // power law plus dielectronic fit, kind 2: cubic in log T; :2828-2877)
+    __device__ R rec_alpha(const MocBalanceArgs& a, int atomLevOff, int kind, int lev) const {
+        if (!a.atomRecValid[atomLevOff + lev]) return 0;
+        const double* r = a.atomRec + 11 * ((size_t)atomLevOff + lev);
+        const R t4 = Te * (R)1.e-4;
+        if (kind == 1) {
+            const R z = (R)r[1], tz = t4 / (z * z);
+            return (R)1.e-13 * (R)r[0] * z * (R)r[2] * pow(tz, (R)r[3]) / ((R)1. + (R)r[4] * pow(tz, (R)r[5])) +
+                   (R)1.0e-12 * (R)r[0] * ((R)r[6] / t4 + (R)r[7] + (R)r[8] * t4 + (R)r[9] * t4 * t4) *
+                       pow(t4, (R)-1.5) * exp(-(R)r[10] * (R)1.e4 / Te);
+        }
+        return pow((R)10., (R)r[0] + (R)r[1] * lT + (R)r[2] * lT * lT + (R)r[3] * lT * lT * lT);
+    }
+};
 
-namespace {
-
-// The same sum with the level matrix held in registers: NL is a compile-time
-// bound on the level count (n <= NL), every loop is unrolled over NL and
-// predicated on n, and the collision rates of a pair are recomputed where a
-// row needs them instead of being stored.  The arithmetic on the valid
-// entries is the generic routine's, operation for operation, including the
-// recombination feed when rec is set.
-template <int NL>
+// Sum of the line emissivities A_ji dE_ij n_j * scale * unit of ion `atom`
+// that exceed 1e-35, one ion per lane with the level matrix in registers: NL
+// is a compile-time bound on the level count (n <= NL), the loops are
+// unrolled over NL and predicated on n.  Each collision pair is evaluated
+// once and placed where the matrix takes it; every diagonal entry receives
+// its terms in the order of the sequential construction (:2944-2965).
+//   rec      levels 2.. are fed by recombination from the next stage
+//            (ionDenUp = X(i+1)/X(i)) at the rates of the ion's data file;
+//   lineAcc  with it, every line's emissivity (lower i, upper j: fLineEm(i, j),
+//            :2987-3005) times w is added to lineAcc[i * ld + j].
+template <int NL, class R = double>
 __device__ double nlevel_fixed(const MocBalanceArgs& a, int atom, double Te, double Ne,
                                double ionDenUp, bool rec, double scale, double unit,
                                double* lineAcc = nullptr, double w = 0., int ld = 0) {
-    const int64_t* m = a.atomMeta + 8 * atom;
-    const int nf = (int)m[0], nT = (int)m[1];
-    const double* lt = a.atomLogT + m[2];
-    const double* g = a.atomG + m[3];
-    const double* en = a.atomE + m[3];
-    const double* A = a.atomA + m[4];
-    const double* q = a.atomQom + m[5];
-    const double* q2 = a.atomQ2 + m[5];
-    const int iup2p = (int)m[6];
-    const double atp = a.atomAtp[atom];
-    const int n = nf < a.maxLev ? nf : a.maxLev;
-    const double sqrTe = sqrt(Te), lT = log10(Te);
-    int klo = 0, khi = nT - 1;
-    double sa = 0., sb = 0., h = 1.;
-    const bool below = lT <= lt[0], above = lT >= lt[nT - 1];
-    if (nT > 2 && !below && !above) {
-        while (khi - klo > 1) {
-            const int k = (khi + klo) >> 1;
-            if (lt[k] > lT) khi = k; else klo = k;
-        }
-        h = lt[khi] - lt[klo];
-        sa = (lt[khi] - lT) / h;
-        sb = (lT - lt[klo]) / h;
-    }
-    // effective rate coefficient qe[p][q] of the generic routine
-    auto qef = [&](int p, int r) -> double {
-        const int i = p < r ? p : r, j = p < r ? r : p;
-        const size_t pp = (size_t)i * nf + j;
-        double ups;
-        if (nT == 1) ups = q[pp];
-        else if (nT == 2) ups = q[pp] + (q[(size_t)nf * nf + pp] - q[pp]) / (lt[1] - lt[0]) * (lT - lt[0]);
-        else if (below) ups = q[pp];
-        else if (above) ups = q[(size_t)(nT - 1) * nf * nf + pp];
-        else {
-            const double ylo = q[(size_t)klo * nf * nf + pp], yhi = q[(size_t)khi * nf * nf + pp];
-            const double dlo = q2[(size_t)klo * nf * nf + pp], dhi = q2[(size_t)khi * nf * nf + pp];
-            ups = sa * ylo + sb * yhi + ((sa * sa * sa - sa) * dlo + (sb * sb * sb - sb) * dhi) * (h * h) / 6.;
-        }
-        const double c = 8.63e-6 * ups / sqrTe;
-        if (en[i] <= en[j]) return p == i ? c * exp((en[i] - en[j]) * 1.4388463 / Te) / g[i] : c / g[j];
-        return p == i ? c / g[j] : c * exp((en[j] - en[i]) * 1.4388463 / Te) / g[i];
-    };
+    const NlevIon<R> I(a, atom, Te);
+    const int n = I.n, nf = I.nf;
     double x[NL][NL], y[NL];
 #pragma unroll
     for (int i = 0; i < NL; ++i) {
@@ -259,40 +138,34 @@ __device__ double nlevel_fixed(const MocBalanceArgs& a, int atom, double Te, dou
 #pragma unroll
     for (int j = 0; j < NL; ++j) x[0][j] = 1.;
     y[0] = 1.;
+    // pairs in (i, j) order: row j takes x[j][i] and its diagonal's (j, i)
+    // terms while rows i take theirs when their own i comes round
 #pragma unroll
-    for (int i = 1; i < NL; ++i) {
-        if (i >= n) break;
+    for (int i = 0; i < NL - 1; ++i) {
 #pragma unroll
-        for (int j = 0; j < NL; ++j) {
-            if (j >= n) break;
-            if (j == i) continue;
-            x[i][j] += Ne * qef(j, i);
-            x[i][i] -= Ne * qef(i, j);
-            if (j > i) x[i][j] += A[(size_t)j * nf + i];
-            else x[i][i] -= A[(size_t)i * nf + j];
+        for (int j = i + 1; j < NL; ++j) {
+            if (j >= n) continue;
+            R qij, qji;
+            I.pair(i, j, qij, qji);
+            // row i (if not the normalisation row): x[i][j] += Ne qe[j][i] + A[j][i]
+            if (i >= 1) {
+                x[i][j] += Ne * (double)qji;
+                x[i][j] += I.A[(size_t)j * nf + i];
+                x[i][i] -= Ne * (double)qij;
+            }
+            // row j: x[j][i] += Ne qe[i][j]; diagonal -= Ne qe[j][i] + A[j][i]
+            x[j][i] += Ne * (double)qij;
+            x[j][j] -= Ne * (double)qji;
+            x[j][j] -= I.A[(size_t)j * nf + i];
         }
     }
     const int kind = rec ? a.atomRecKind[atom] : 0;
     if (kind != 0) {
-        const double* rd = a.atomRec + 11 * m[3];
-        const int* rv = a.atomRecValid + m[3];
-        const double t4 = Te * 1.e-4;
+        const int off = (int)a.atomMeta[8 * atom + 3];
 #pragma unroll
         for (int i = 1; i < NL; ++i) {
             if (i >= n) break;
-            double al = 0.;
-            if (rv[i]) {
-                const double* r = rd + 11 * i;
-                if (kind == 1) {
-                    const double z = r[1], tz = t4 / (z * z);
-                    al = 1.e-13 * r[0] * z * r[2] * pow(tz, r[3]) / (1. + r[4] * pow(tz, r[5])) +
-                         1.0e-12 * r[0] * (r[6] / t4 + r[7] + r[8] * t4 + r[9] * t4 * t4) * pow(t4, -1.5) *
-                             exp(-r[10] * 1.e4 / Te);
-                } else {
-                    al = pow(10., r[0] + r[1] * lT + r[2] * lT * lT + r[3] * lT * lT * lT);
-                }
-            }
-            y[i] = -Ne * ionDenUp * al;
+            y[i] = -Ne * ionDenUp * (double)I.rec_alpha(a, off, kind, i);
         }
     }
 #pragma unroll
@@ -313,10 +186,6 @@ __device__ double nlevel_fixed(const MocBalanceArgs& a, int atom, double Te, dou
 #pragma unroll
     for (int k = NL - 1; k >= 0; --k) {
         if (k >= n) continue;
-        if (k == n - 1) {
-            y[k] /= x[k][k];
-            continue;
-        }
 #pragma unroll
         for (int j = k + 1; j < NL; ++j) {
             if (j >= n) break;
@@ -335,15 +204,121 @@ __device__ double nlevel_fixed(const MocBalanceArgs& a, int atom, double Te, dou
 #pragma unroll
         for (int j = i + 1; j < NL; ++j) {
             if (j >= n) break;
-            double av = A[(size_t)j * nf + i];
+            double av = I.A[(size_t)j * nf + i];
             if (av == 0.) continue;
-            if (j + 1 == iup2p && i == 0 && atp > 0.) av -= atp;
-            const double em = av * (en[j] - en[i]) * (y[j] / sumN) * scale * unit;
+            if (j + 1 == I.iup2p && i == 0 && I.atp > 0.) av -= I.atp;
+            const double em = av * (I.en[j] - I.en[i]) * (y[j] / sumN) * scale * unit;
             if (em > 1.e-35) tot += em;
             if (lineAcc) lineAcc[i * ld + j] += em * w;
         }
     }
     return tot;
 }
+
+// The same sum for ions of up to NL = 8, 16 or 27 levels, solved by a team of
+// W = 8, 16 or 32 lanes of one wave with the level matrix in registers: team
+// lane k < n owns row k (its entries and its right-hand side).  The pivot
+// row's entries travel by shuffles (one division per lane and step); the back
+// substitution is column-oriented (row k's solution broadcast once, every row
+// above it updated in parallel); the populations and the line sums are
+// gathered on every lane of the team.  Every lane of the team calls it with
+// the same ion, and the return value is the same on every lane of the team.
+// The ion's level count must not exceed NL.  The operations on each entry are
+// those of luSlv; the row sums of the back substitution and the sum of the
+// lines are taken in a different (fixed) order than a sequential evaluation.
+template <int NL, class R = double>
+__device__ __attribute__((noinline)) double nlevel_team(const MocBalanceArgs& a, int atom, double Te,
+                                                        double Ne, double ionDenUp, bool rec,
+                                                        double scale, double unit,
+                                                        double* lineAcc = nullptr, double w = 0.,
+                                                        int ld = 0) {
+    constexpr int W = NL <= 8 ? 8 : (NL <= 16 ? 16 : 32);
+    static_assert(NL <= W, "a team row per level");
+    const int k = (int)(threadIdx.x & (W - 1));  // the row of this lane
+    const NlevIon<R> I(a, atom, Te);
+    const int n = I.n, nf = I.nf;
+    const bool mine = k < n;
+    // row k: row 0 the normalisation; row k >= 1 x[k][j] = Ne qe[j][k] + A[j][k]
+    // (j > k), the diagonal minus the rates out of level k in j order
+    double x[NL];
+    double d = 0.;
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+        x[j] = 0.;
+        if (mine && j < n && j != k) {
+            R qa, qb;  // the pair (min, max): qa from the lower index to the upper
+            I.pair(k < j ? k : j, k < j ? j : k, qa, qb);
+            const R qkj = k < j ? qa : qb, qjk = k < j ? qb : qa;
+            x[j] += Ne * (double)qjk;
+            if (j > k) x[j] += I.A[(size_t)j * nf + k];
+            d -= Ne * (double)qkj;
+            if (j < k) d -= I.A[(size_t)k * nf + j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+        if (k == 0) x[j] = 1.;
+        else if (j == k) x[j] = d;
+    }
+    double y = k == 0 ? 1. : 0.;
+    const int kind = rec ? a.atomRecKind[atom] : 0;
+    if (kind != 0 && k >= 1 && mine)
+        y = -Ne * ionDenUp * (double)I.rec_alpha(a, (int)a.atomMeta[8 * atom + 3], kind, k);
+    // elimination without pivoting: step i subtracts pivot row i from the rows below
+#pragma unroll
+    for (int i = 0; i < NL - 1; ++i) {
+        const double piv = __shfl(x[i], i, W);
+        const double f = x[i] / piv;
+        const bool below = k > i;
+#pragma unroll
+        for (int j = i + 1; j < NL; ++j) {
+            const double xij = __shfl(x[j], i, W);
+            if (below) x[j] -= xij * f;
+        }
+        const double yi = __shfl(y, i, W);
+        if (below) y -= yi * f;
+    }
+    // back substitution, column-oriented: the last row first
+#pragma unroll
+    for (int j = NL - 1; j >= 0; --j) {
+        if (k == j) y = y / x[j];
+        const double yj = __shfl(y, j, W);
+        if (k < j && j < n) y -= yj * x[j];
+    }
+    // populations: the sum over the levels on every lane, in level order
+    double sumN = 0.;
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+        const double yj = __shfl(y, j, W);
+        if (j < n) sumN += yj;
+    }
+    // lines out of level k (the upper level), summed on its lane, then over the team
+    const double pop = y / sumN;
+    double tot = 0.;
+#pragma unroll
+    for (int i = 0; i < NL - 1; ++i) {
+        if (mine && i < k) {
+            double av = I.A[(size_t)k * nf + i];
+            if (av != 0.) {
+                if (k + 1 == I.iup2p && i == 0 && I.atp > 0.) av -= I.atp;
+                const double em = av * (I.en[k] - I.en[i]) * pop * scale * unit;
+                if (em > 1.e-35) tot += em;
+                if (lineAcc) lineAcc[i * ld + k] += em * w;
+            }
+        }
+    }
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) tot += __shfl_xor(tot, o, W);
+    return tot;
+}
+
+}  // namespace
+
+namespace {
+
+// Team sizes of the n-level solves: ions of up to 5 levels one per lane, then
+// teams of 8, 16 and 32 lanes for 6-8, 9-16 and 17-27 levels.
+constexpr int kNlevLane = 5;
+__host__ __device__ inline int nlevel_team_class(int n) { return n <= kNlevLane ? -1 : (n <= 8 ? 0 : (n <= 16 ? 1 : 2)); }
 
 }  // namespace
