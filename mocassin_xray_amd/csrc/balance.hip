@@ -92,6 +92,12 @@ __global__ void __launch_bounds__(256) k_rates(const double* __restrict__ J, int
 
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
 
+// sum over the wave, the same value on every lane (butterfly)
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 __device__ __forceinline__ double t3(const double* a, int i, int z, int n) {
     return a[((size_t)i * 31 + z) * 31 + n];
 }
@@ -131,14 +137,15 @@ constexpr size_t kMaxGrainLds = 65536;
 // dielectronic recombination of the ions with more than 11 electrons
 // (Aldrovandi & Pequignot above 6e4 K, Nussbaumer & Storey below;
 // dielectronic :3204-3305): the last matching row wins
-__device__ double direc(const MocBalanceArgs& a, int z, int ion, double T) {
-    const double tt = T > 1000. ? T / 10000. : 0.1;
-    double v = 0.0;
-    if (T > 60000.) {
+template <class R>
+__device__ R direc(const MocBalanceArgs& a, int z, int ion, R T) {
+    const R tt = T > (R)1000. ? T / (R)10000. : (R)0.1;
+    R v = 0;
+    if (T > (R)60000.) {
         for (int i = 0; i < a.nAldro; ++i) {
             const double* r = a.aldro + 6 * i;
             if ((int)r[0] == z && (int)r[0] - (int)r[1] + 1 == ion)
-                v = r[2] * pow(T, -1.5) * exp(-r[4] / T) * (1. + r[3] * exp(-r[5] / T));
+                v = (R)r[2] * pow(T, (R)-1.5) * exp(-(R)r[4] / T) * ((R)1. + (R)r[3] * exp(-(R)r[5] / T));
         }
         return v;
     }
@@ -146,20 +153,23 @@ __device__ double direc(const MocBalanceArgs& a, int z, int ion, double T) {
         const double* r = a.diel + 8 * i;
         const int g = (int)r[7];
         if ((int)r[0] != z || (int)r[0] + 1 - (int)r[1] != ion) continue;
-        if (g == 0 || (g == 1 && T < 20000.) || (g == 2 && T >= 20000.))
-            v = 1.e-12 * (r[2] / tt + r[3] + r[4] * tt + r[5] * tt * tt) * pow(tt, -1.5) * exp(-r[6] / tt);
+        if (g == 0 || (g == 1 && T < (R)20000.) || (g == 2 && T >= (R)20000.))
+            v = (R)1.e-12 * ((R)r[2] / tt + (R)r[3] + (R)r[4] * tt + (R)r[5] * tt * tt) * pow(tt, (R)-1.5) *
+                exp(-(R)r[6] / tt);
     }
     return v;
 }
 
 // recombination coefficients of every ion (calcAlpha :2298-2413), one
-// (element, stage) per lane
-__device__ void recomb(const MocBalanceArgs& a, const Lds& L, double T, int tid) {
+// (element, stage) per lane; the fits in precision R, stored in FP64
+template <class R>
+__device__ void recomb(const MocBalanceArgs& a, const Lds& L, double Td, int tid) {
     const int NE = a.NE, NS = a.NS, S2 = L.S2;
     const int eO = a.elemSlot[8];
+    const R T = (R)Td;
     for (int k = tid; k < a.nX; k += 64) L.alpha[k] = 0.0;
     for (int i = tid; i <= MOC_MAXS; i += 64)
-        L.diO[i] = (eO >= 0 && i >= 1 && i < 7 && i <= NS) ? direc(a, 8, i, T) : 0.0;
+        L.diO[i] = (eO >= 0 && i >= 1 && i < 7 && i <= NS) ? (double)direc<R>(a, 8, i, T) : 0.0;
     __syncthreads();
     for (int k = tid; k < NE * S2; k += 64) {
         const int e = k / S2, ion = k % S2;
@@ -167,39 +177,41 @@ __device__ void recomb(const MocBalanceArgs& a, const Lds& L, double T, int tid)
         if (ion < 1 || ion > imin(NS - 1, z)) continue;
         const int nel = z - ion;
         if (nel <= 11) {  // Badnell radiative + dielectronic fits (:2314-2330)
-            const double bb = t3(a.rrb, 1, z, nel) + t3(a.rrb, 4, z, nel) * exp(-t3(a.rrb, 5, z, nel) / T);
-            const double s0 = sqrt(T / t3(a.rrb, 2, z, nel)), s1 = sqrt(T / t3(a.rrb, 3, z, nel));
-            double dr = 0.0;
+            const R bb = (R)t3(a.rrb, 1, z, nel) + (R)t3(a.rrb, 4, z, nel) * exp(-(R)t3(a.rrb, 5, z, nel) / T);
+            const R s0 = sqrt(T / (R)t3(a.rrb, 2, z, nel)), s1 = sqrt(T / (R)t3(a.rrb, 3, z, nel));
+            R dr = 0;
             for (int q = 0; q < 9; ++q) {
-                const double E = t3(a.drb_e, q, z, nel);
-                if (E / T < 87.) dr += t3(a.drb_c, q, z, nel) * exp(-E / T);
+                const R E = (R)t3(a.drb_e, q, z, nel);
+                if (E / T < (R)87.) dr += (R)t3(a.drb_c, q, z, nel) * exp(-E / T);
             }
-            L.al(e, ion) = t3(a.rrb, 0, z, nel) / (s0 * pow(1. + s0, 1. - bb) * pow(1. + s1, 1. + bb)) +
-                           dr * pow(T, -1.5);
+            L.al(e, ion) = (double)((R)t3(a.rrb, 0, z, nel) /
+                                        (s0 * pow((R)1. + s0, (R)1. - bb) * pow((R)1. + s1, (R)1. + bb)) +
+                                    dr * pow(T, (R)-1.5));
             continue;
         }
         // radRecFit + dielectronic, oxygen's dielectronic borrowed when zero (:2334-2362)
         const int n = z - ion + 1;
-        double rr;
+        R rr;
         if (n <= 3 || n == 11 || (z > 5 && z < 9) || z == 10 || (z == 26 && n > 11)) {
-            const double q = sqrt(T / t3(a.rnew, 2, z, n)), p = t3(a.rnew, 1, z, n);
-            rr = t3(a.rnew, 0, z, n) / (q * pow(q + 1., 1. - p) * pow(1. + sqrt(T / t3(a.rnew, 3, z, n)), 1. + p));
+            const R q = sqrt(T / (R)t3(a.rnew, 2, z, n)), p = (R)t3(a.rnew, 1, z, n);
+            rr = (R)t3(a.rnew, 0, z, n) /
+                 (q * pow(q + (R)1., (R)1. - p) * pow((R)1. + sqrt(T / (R)t3(a.rnew, 3, z, n)), (R)1. + p));
         } else {
-            const double q = T * 1.e-4;
+            const R q = T * (R)1.e-4;
             if (z == 26 && n <= 13) {
                 const double fe0[10] = {4.33e-10, 3.91e-10, 3.49e-10, 3.16e-10, 2.96e-10,
                                         2.59e-10, 2.24e-10, 1.91e-10, 1.68e-10, 1.46e-10};
                 const double fe1[10] = {0.531, 0.523, 0.521, 0.534, 0.557, 0.567, 0.579, 0.601, 0.602, 0.597};
                 const double fe2[10] = {5.77e-02, 6.15e-02, 6.22e-02, 6.02e-02, 5.79e-02,
                                         5.65e-02, 5.49e-02, 5.10e-02, 5.07e-02, 5.22e-02};
-                rr = fe0[n - 4] / pow(q, fe1[n - 4] + fe2[n - 4] * log10(q));
+                rr = (R)fe0[n - 4] / pow(q, (R)fe1[n - 4] + (R)fe2[n - 4] * log10(q));
             } else {
-                rr = t3(a.rrec, 0, z, n) / pow(q, t3(a.rrec, 1, z, n));
+                rr = (R)t3(a.rrec, 0, z, n) / pow(q, (R)t3(a.rrec, 1, z, n));
             }
         }
-        double di = direc(a, z, ion, T);
+        double di = (double)direc<R>(a, z, ion, T);
         if (di == 0. && ion < 7 && eO >= 0) di = L.diO[ion];
-        L.al(e, ion) = rr + (di > 0. ? di : 0.);
+        L.al(e, ion) = (double)rr + (di > 0. ? di : 0.);
     }
     __syncthreads();
 }
@@ -207,9 +219,10 @@ __device__ void recomb(const MocBalanceArgs& a, const Lds& L, double T, int tid)
 // Voronov collisional ionisation (:3552-3582), one (element, stage) per lane;
 // the reference leaves the loops at the first u > 80, so that entry and every
 // later one (in its loop order) stay zero
+template <class R>
 __device__ void collion(const MocBalanceArgs& a, const Lds& L, double Te, int tid) {
     const int NE = a.NE, NS = a.NS, S2 = L.S2;
-    const double te = Te * 8.617385e-05;
+    const R te = (R)Te * (R)8.617385e-05;
     for (int k = tid; k < a.nX; k += 64) L.cI1[k] = 0.0;
     if (tid == 0) *L.hot = NE * S2;
     __syncthreads();
@@ -218,14 +231,14 @@ __device__ void collion(const MocBalanceArgs& a, const Lds& L, double Te, int ti
         const int z = a.elemZ[e];
         if (ion < 1 || ion > imin(NS - 1, z - 1)) continue;
         const int el = z - ion + 1;
-        const double u = t3(a.cion_cf, 0, z, el) / te;
-        if (u > 80.0) {
+        const R u = (R)t3(a.cion_cf, 0, z, el) / te;
+        if (u > (R)80.0) {
             atomicMin(L.hot, k);
             continue;
         }
-        const double c = t3(a.cion_cf, 2, z, el) * (1.0 + t3(a.cion_cf, 1, z, el) * sqrt(u)) /
-                         (t3(a.cion_cf, 3, z, el) + u) * pow(u, t3(a.cion_cf, 4, z, el)) * exp(-u);
-        L.c1(e, ion) = c;
+        const R c = (R)t3(a.cion_cf, 2, z, el) * ((R)1.0 + (R)t3(a.cion_cf, 1, z, el) * sqrt(u)) /
+                    ((R)t3(a.cion_cf, 3, z, el) + u) * pow(u, (R)t3(a.cion_cf, 4, z, el)) * exp(-u);
+        L.c1(e, ion) = (double)c;
     }
     __syncthreads();
     const int hot = *L.hot;
@@ -237,17 +250,19 @@ __device__ void collion(const MocBalanceArgs& a, const Lds& L, double Te, int ti
 }
 
 // charge-exchange coefficient [cm^3/s]; T-scaled unless raw (ionBalance keeps the
-// heavy elements' coefficients unscaled)
-__device__ double chex(const MocBalanceArgs& a, int z, int ion, double T, bool scaled) {
+// heavy elements' coefficients unscaled); the fit in precision R
+template <class R>
+__device__ double chex(const MocBalanceArgs& a, int z, int ion, double Td, bool scaled) {
     const int k = a.chexIdx[z * 32 + ion];
     if (k < 0) return 0.0;
     const double* c = a.chexTab + 4 * k;
     const double v0 = c[0] * 1.e-9;
     if (!scaled) return v0;
-    if (T < 6000. || T > 5.e4) return 0.0;
-    const double t4 = T / 10000.;
-    const double v = v0 * pow(t4, c[1]) * (1. + c[2] * exp(c[3] * t4));
-    return v < 0. ? 0. : v;
+    const R T = (R)Td;
+    if (T < (R)6000. || T > (R)5.e4) return 0.0;
+    const R t4 = T / (R)10000.;
+    const R v = (R)v0 * pow(t4, (R)c[1]) * ((R)1. + (R)c[2] * exp((R)c[3] * t4));
+    return v < (R)0. ? 0. : (double)v;
 }
 
 __device__ __forceinline__ double dEk(int z, int ion) {
@@ -277,11 +292,13 @@ struct Rates {
     __device__ double heat(int s, double heatef) const { return at(3 * s + 1) + heatef * at(3 * s + 2); }
 };
 
+template <class R>
 __device__ double rev_chex(const MocBalanceArgs& a, const Lds& L, double Te, double H, int z, int ion) {
     if (!(dEk(z, ion) > 1.)) return 0.0;
     const int nel = z - ion + 1;
     const double g0g1 = (double)a.statW[z * 32 + nel] / (double)a.statW1[z * 32 + nel];
-    return chex(a, z, ion, Te, true) * (1. - L.x(a.elemSlot[1], 1)) * H * 2. * exp(-dEk(z, ion) / Te) / g0g1;
+    return chex<R>(a, z, ion, Te, true) * (1. - L.x(a.elemSlot[1], 1)) * H * 2. *
+           (double)exp(-(R)dEk(z, ion) / (R)Te) / g0g1;
 }
 
 template <class RT>
@@ -292,14 +309,14 @@ __device__ double gamma_out(const RT& r, int spec) {
 }
 
 // H or He ratio chain (:1715-1790) on the lane that owns element zz
-template <class RT>
+template <class R, class RT>
 __device__ void hhe_chain(const MocBalanceArgs& a, const RT& r, const Lds& L, int zz, double Te,
                           double Ne, double H, double H0, double collIonH) {
     const int NS = a.NS, e = a.elemSlot[zz];
     double ratio[3] = {0., 0., 0.};
     for (int ion = 1; ion <= imin(zz, NS - 1); ++ion) {
         const int spec = r.specOf[e * 32 + ion];
-        const double cx = chex(a, zz, ion, Te, true);
+        const double cx = chex<R>(a, zz, ion, Te, true);
         if (zz > 1 || ion > 1) collIonH = 0.;
         const double g = (spec >= 0 ? gamma_out(r, spec) : 0.0) + 1.e-20;
         ratio[ion] = (g + collIonH * Ne) / (Ne * L.al(e, ion) + cx * H * H0);
@@ -318,7 +335,7 @@ __device__ void hhe_chain(const MocBalanceArgs& a, const RT& r, const Lds& L, in
 }
 
 // bidiagonal chain of one heavy element with Auger spreading (:1826-1929)
-template <class RT>
+template <class R, class RT>
 __device__ void heavy_chain(const MocBalanceArgs& a, const RT& r, const Lds& L, int e, double Te,
                             double Ne, double H, int nIterX, const double revFrozen[2]) {
     const int NS = a.NS, z = a.elemZ[e];
@@ -348,7 +365,7 @@ __device__ void heavy_chain(const MocBalanceArgs& a, const RT& r, const Lds& L, 
     const int eH = a.elemSlot[1];
     double m = 1., sum = 1.;
     for (int ion = 2; ion <= top; ++ion) {
-        const double in = Ne * L.al(e, ion - 1) + chex(a, z, ion - 1, Te, false) * H * L.x(eH, 1);
+        const double in = Ne * L.al(e, ion - 1) + chex<R>(a, z, ion - 1, Te, false) * H * L.x(eH, 1);
         const double mn = m * out[ion - 1] / in;
         out[ion - 1] = m;
         m = mn;
@@ -373,7 +390,7 @@ __device__ void heavy_chain(const MocBalanceArgs& a, const RT& r, const Lds& L, 
 }
 
 // ionBalance (:1575-1992); returns false when 15 passes did not settle H/He
-template <class RT>
+template <class R, class RT>
 __device__ bool ion_balance(const MocBalanceArgs& a, const RT& r, const Lds& L, double Te,
                             double& Ne, double H, const double revFrozen[2], int tid) {
     const int NS = a.NS, NE = a.NE;
@@ -381,19 +398,20 @@ __device__ bool ion_balance(const MocBalanceArgs& a, const RT& r, const Lds& L, 
     int nIterX = 1;
     bool conv = true;
     for (;;) {
-        double expFact = 157893.94 / Te;
-        if (expFact > 75.) expFact = 75.;
-        const double collIonH = 2.75E-16 * Te * sqrt(Te) * (157893.94 / Te + 2.) * exp(-expFact);
+        const R Tr = (R)Te;
+        R expFact = (R)157893.94 / Tr;
+        if (expFact > (R)75.) expFact = (R)75.;
+        const double collIonH = (double)((R)2.75E-16 * Tr * sqrt(Tr) * ((R)157893.94 / Tr + (R)2.) * exp(-expFact));
         const double HIOld = L.x(eH, 1);
         const double HeIOld = eHe >= 0 ? L.x(eHe, 1) : 0.0;
         const double HeIIOld = eHe >= 0 ? L.x(eHe, 2) : 0.0;
         const double H0 = L.x(eH, 1);
         __syncthreads();
-        if (tid == 0) hhe_chain(a, r, L, 1, Te, Ne, H, H0, collIonH);
-        if (tid == 1 && eHe >= 0) hhe_chain(a, r, L, 2, Te, Ne, H, H0, collIonH);
+        if (tid == 0) hhe_chain<R>(a, r, L, 1, Te, Ne, H, H0, collIonH);
+        if (tid == 1 && eHe >= 0) hhe_chain<R>(a, r, L, 2, Te, Ne, H, H0, collIonH);
         __syncthreads();
         for (int e = tid; e < NE; e += 64)
-            if (a.elemZ[e] >= 3) heavy_chain(a, r, L, e, Te, Ne, H, nIterX, revFrozen);
+            if (a.elemZ[e] >= 3) heavy_chain<R>(a, r, L, e, Te, Ne, H, nIterX, revFrozen);
         __syncthreads();
         double ne = 0.;
         for (int e = 0; e < NE; ++e) {
@@ -604,7 +622,7 @@ __device__ void g_terms(const MocBalanceArgs& a, const Lds& L, const GrainLds& G
 // forbidden-line cooling of the ions of list class c (nlevel_team_class),
 // P = 64 / W ions at a time, one per team of W lanes; idle teams solve the
 // group's first ion again and store nothing
-template <int NL>
+template <int NL, class R>
 __device__ void team_cooling(const MocBalanceArgs& a, const Lds& L, int cnt, const int* list, double T,
                              double Ne, int tid) {
     constexpr int W = NL <= 8 ? 8 : (NL <= 16 ? 16 : 32), P = 64 / W;
@@ -612,14 +630,16 @@ __device__ void team_cooling(const MocBalanceArgs& a, const Lds& L, int cnt, con
         const int bt = b0 + tid / W;
         const int k = list[bt < cnt ? bt : b0];
         const int e = a.nlvSlot[k], ion = a.nlvIon[k];
-        const double v = nlevel_team<NL>(a, a.nlvAtom[k], T, Ne, 0., false, a.abund[e] * L.x(e, ion), 1.9865e-16);
+        const double v = nlevel_team<NL, R>(a, a.nlvAtom[k], T, Ne, 0., false, a.abund[e] * L.x(e, ion), 1.9865e-16);
         if ((tid & (W - 1)) == 0 && bt < cnt) L.cool[k] = v;
     }
 }
 
 // thermBalance (:887-1215): heating and cooling [erg/s per n_H].  TEAMS: some
 // ions have more than kNlevLane levels (their solves run in teams of lanes).
-template <bool TEAMS, class RT>
+// The cooling fits and the collision rates of the level solves in precision
+// R, every sum in FP64.
+template <bool TEAMS, class R, class RT>
 __device__ void therm(const MocBalanceArgs& a, const RT& r, const Lds& L, double T, double Ne,
                       double H, double heatef, const double comp[4], double* heat, double* cool,
                       int tid, double grainHeat = 0., double grainCool = 0., bool grains = false) {
@@ -629,62 +649,77 @@ __device__ void therm(const MocBalanceArgs& a, const RT& r, const Lds& L, double
         const int atom = a.nlvAtom[k];
         if (nlevel_count(a, atom) > kNlevLane) continue;
         const int e = a.nlvSlot[k], ion = a.nlvIon[k];
-        L.cool[k] = nlevel_fixed<kNlevLane>(a, atom, T, Ne, 0., false, a.abund[e] * L.x(e, ion), 1.9865e-16);
+        L.cool[k] = nlevel_fixed<kNlevLane, R>(a, atom, T, Ne, 0., false, a.abund[e] * L.x(e, ion), 1.9865e-16);
     }
     if constexpr (TEAMS) {
-        team_cooling<8>(a, L, a.nNlvT[0], a.nlvT[0], T, Ne, tid);
-        team_cooling<16>(a, L, a.nNlvT[1], a.nlvT[1], T, Ne, tid);
-        team_cooling<27>(a, L, a.nNlvT[2], a.nlvT[2], T, Ne, tid);
+        team_cooling<8, R>(a, L, a.nNlvT[0], a.nlvT[0], T, Ne, tid);
+        team_cooling<16, R>(a, L, a.nNlvT[1], a.nlvT[1], T, Ne, tid);
+        team_cooling<27, R>(a, L, a.nNlvT[2], a.nlvT[2], T, Ne, tid);
     }
     __syncthreads();
-    double coolFF = 0., coolRec = 0., coolColl, coolCollH = 0., heatSte = 0., Te4 = T / 1.e4;
-    const double sq = kBoltz * T / sqrt(T);
-    for (int e = 0; e < a.NE; ++e) {  // H-like recombination, elements in ascending Z
+    // The sums run over the lanes (elements, (element, stage) entries, n-level
+    // ions, species) and are then reduced over the wave: every lane ends with
+    // the same totals.
+    const R Tr = (R)T;
+    const R sq = (R)kBoltz * Tr / sqrt(Tr);
+    double coolFF = 0., coolRec = 0., coolCollH = 0., heatSte = 0., coolNl = 0.;
+    for (int e = tid; e < a.NE; e += 64) {
         const int z = a.elemZ[e];
-        if (!(a.NS > z)) continue;
-        const double Np = L.x(e, z + 1) * a.abund[e];
-        const double lt = log10(T / (double)(z * z));
-        Te4 = T / (z * z * 1.e4);
-        const double bFF = z * (1.0108464E-11 + 9.7930778E-13 * lt - 6.6433144E-13 * lt * lt +
-                                2.4793747E-13 * lt * lt * lt - 2.3938215E-14 * lt * lt * lt * lt);
-        const double bRec = z * (9.4255985E-11 - 4.04794384E-12 * lt - 1.0055237E-11 * lt * lt +
-                                 1.99266862E-12 * lt * lt * lt - 1.06681387E-13 * lt * lt * lt * lt);
-        coolFF += Np * Ne * bFF * sq;
-        coolRec += Np * Ne * bRec * sq;
-    }
-    Te4 = T / 1.e4;
-    for (int e = 0; e < a.NE; ++e) {  // He-like
-        const int z = a.elemZ[e];
-        if (z < 2 || !(a.NS > z)) continue;
-        const double Np = L.x(e, z) * a.abund[e];
-        const double lt = log10(4. * T / (double)(z * z));
-        Te4 = 4. * T / (z * z * 1.e4);
-        const double bFF = (z / 2.) * (1.070073e-11 - 2.5730207e-13 * lt + 2.109134e-13 * lt * lt);
-        const double bRec = (z / 2.) * (1.06926e-10 - 2.41756e-11 * lt + 1.14542e-12 * lt * lt -
-                                        5.06535e-13 * lt * lt * lt + 9.16745e-14 * lt * lt * lt * lt);
-        coolFF += Np * Ne * bFF * sq;
-        coolRec += Np * Ne * bRec * sq;
-    }
-    const int eH = a.elemSlot[1];
-    // H Ly-alpha/beta excitation, with Te4 as the loops above left it
-    coolColl = T > 5000. ? (2.47e-8 * exp(-118338. / T) * pow(Te4, -0.228) +
-                            1.32e-8 * exp(-140252. / T) * pow(Te4, -0.460)) * kHcRyd * L.x(eH, 1) * Ne
-                         : 0.;
-    for (int e = 0; e < a.NE; ++e) {
-        const int z = a.elemZ[e];
-        for (int ion = 1; ion <= imin(z, a.NS - 1); ++ion) {
-            const double c2 = a.cion_th[z * 32 + ion] * L.c1(e, ion) * kBoltz;  // ionisation cooling
-            coolCollH += a.abund[e] * L.x(e, ion) * c2 * Ne;
+        if (a.NS > z) {  // H-like recombination and free-free (Hummer 1994 fits)
+            const double Np = L.x(e, z + 1) * a.abund[e];
+            const R lt = log10(Tr / (R)(z * z));
+            const R bFF = (R)z * ((R)1.0108464E-11 + (R)9.7930778E-13 * lt - (R)6.6433144E-13 * lt * lt +
+                                  (R)2.4793747E-13 * lt * lt * lt - (R)2.3938215E-14 * lt * lt * lt * lt);
+            const R bRec = (R)z * ((R)9.4255985E-11 - (R)4.04794384E-12 * lt - (R)1.0055237E-11 * lt * lt +
+                                   (R)1.99266862E-12 * lt * lt * lt - (R)1.06681387E-13 * lt * lt * lt * lt);
+            coolFF += Np * Ne * (double)(bFF * sq);
+            coolRec += Np * Ne * (double)(bRec * sq);
+        }
+        if (z >= 2 && a.NS > z) {  // He-like (Hummer & Storey 1998 fits)
+            const double Np = L.x(e, z) * a.abund[e];
+            const R lt = log10((R)4. * Tr / (R)(z * z));
+            const R bFF = ((R)z / (R)2.) * ((R)1.070073e-11 - (R)2.5730207e-13 * lt + (R)2.109134e-13 * lt * lt);
+            const R bRec = ((R)z / (R)2.) * ((R)1.06926e-10 - (R)2.41756e-11 * lt + (R)1.14542e-12 * lt * lt -
+                                             (R)5.06535e-13 * lt * lt * lt + (R)9.16745e-14 * lt * lt * lt * lt);
+            coolFF += Np * Ne * (double)(bFF * sq);
+            coolRec += Np * Ne * (double)(bRec * sq);
         }
     }
-    for (int k = 0; k < a.nNlv; ++k) coolColl += L.cool[k];
-    for (int sp = 0; sp < a.nSpec; ++sp) {
+    for (int k = tid; k < a.NE * L.S2; k += 64) {  // collisional-ionisation cooling
+        const int e = k / L.S2, ion = k % L.S2;
+        const int z = a.elemZ[e];
+        if (ion < 1 || ion > imin(z, a.NS - 1)) continue;
+        const double c2 = a.cion_th[z * 32 + ion] * L.c1(e, ion) * kBoltz;
+        coolCollH += a.abund[e] * L.x(e, ion) * c2 * Ne;
+    }
+    for (int k = tid; k < a.nNlv; k += 64) coolNl += L.cool[k];
+    for (int sp = tid; sp < a.nSpec; sp += 64) {  // photoelectric heating of the gas
         const int e = a.specElem[sp], ion = a.specIon[sp];
         const int z = a.elemZ[e];
         if (ion > imin(z, a.NS - 1)) continue;
-        for (int sh = r.specShell[sp]; sh < r.specShell[sp + 1]; ++sh)
-            heatSte += r.heat(sh, z < 3 ? 1.0 : heatef) * L.x(e, ion) * a.abund[e];
+        double h = 0.;
+        for (int sh = r.specShell[sp]; sh < r.specShell[sp + 1]; ++sh) h += r.heat(sh, z < 3 ? 1.0 : heatef);
+        heatSte += h * L.x(e, ion) * a.abund[e];
     }
+    coolFF = wave_sum(coolFF);
+    coolRec = wave_sum(coolRec);
+    coolCollH = wave_sum(coolCollH);
+    coolNl = wave_sum(coolNl);
+    heatSte = wave_sum(heatSte);
+    // H Ly-alpha/beta excitation at the Te4 the reference's He-like loop leaves
+    // behind: that of the last element with a He-like stage
+    R Te4 = Tr / (R)1.e4;
+    for (int e = 0; e < a.NE; ++e) {
+        const int z = a.elemZ[e];
+        if (z >= 2 && a.NS > z) Te4 = (R)4. * Tr / ((R)(z * z) * (R)1.e4);
+    }
+    const int eH = a.elemSlot[1];
+    const double coolColl =
+        (T > 5000. ? (double)((R)2.47e-8 * exp((R)-118338. / Tr) * pow(Te4, (R)-0.228) +
+                              (R)1.32e-8 * exp((R)-140252. / Tr) * pow(Te4, (R)-0.460)) *
+                         kHcRyd * L.x(eH, 1) * Ne
+                   : 0.) +
+        coolNl;
     *heat = heatSte;
     *cool = coolFF + coolRec + coolColl + coolCollH;
     if (a.lgCompton) {
@@ -706,7 +741,7 @@ __device__ void therm(const MocBalanceArgs& a, const RT& r, const Lds& L, double
 // capped at 128 VGPRs with a few hundred bytes of spills it takes 13.6 ms.
 // PE: gas + dust runs with grain charge, photoelectric heating and
 // gas-grain collisions (the rows of GrainLds follow the other LDS rows).
-template <bool TEAMS, bool ST, bool PE>
+template <bool TEAMS, bool ST, bool PE, class R>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 k_balance(MocBalanceArgs a) {
     extern __shared__ double lds[];
@@ -785,7 +820,7 @@ k_balance(MocBalanceArgs a) {
     bool ib = true, noTe = false;
     for (;;) {
         ++nT;
-        recomb(a, L, Te, tid);
+        recomb<R>(a, L, Te, tid);
         // secondary-ionisation heating efficiency (photoionisation :3605-3627)
         double xe = 0., heatef;
         for (int e = 0; e < NE; ++e) xe += L.x(e, 1) * a.abund[e] * H;
@@ -793,9 +828,9 @@ k_balance(MocBalanceArgs a) {
         if (xe > 0.95) heatef = 1.;
         else {
             xe = xe < 1.e-4 ? 1.e-4 : xe;
-            heatef = 0.9971 * (1. - pow(1. - pow(xe, 0.2663), 1.3163));
+            heatef = (double)((R)0.9971 * ((R)1. - pow((R)1. - pow((R)xe, (R)0.2663), (R)1.3163)));
         }
-        collion(a, L, Te, tid);
+        collion<R>(a, L, Te, tid);
         double comp[4] = {0., 0., 0., 0.};
         if (a.lgCompton) {
             comp[0] = r.at(a.colComp + 0);
@@ -804,9 +839,9 @@ k_balance(MocBalanceArgs a) {
                        (eHe >= 0 ? r.at(a.colComp + 3) * L.x(eHe, 1) * a.abund[eHe] * 2. : 0.)) * kRyd2erg;
         }
         double revFrozen[2] = {0., 0.};
-        if (a.elemSlot[7] >= 0) revFrozen[0] = rev_chex(a, L, Te, H, 7, 1);
-        if (a.elemSlot[8] >= 0) revFrozen[1] = rev_chex(a, L, Te, H, 8, 1);
-        ib = ion_balance(a, r, L, Te, Ne, H, revFrozen, tid);
+        if (a.elemSlot[7] >= 0) revFrozen[0] = rev_chex<R>(a, L, Te, H, 7, 1);
+        if (a.elemSlot[8] >= 0) revFrozen[1] = rev_chex<R>(a, L, Te, H, 8, 1);
+        ib = ion_balance<R>(a, r, L, Te, Ne, H, revFrozen, tid);
         double heat, cool;
         if constexpr (PE) {  // grain charges and their heating/cooling terms (:318-333)
             for (int g = tid; g < nG; g += 64) {
@@ -821,9 +856,9 @@ k_balance(MocBalanceArgs a) {
                 cg = cg + G.cg[g];
             }
             const double peG = pg * a.Ndust[cell] * kHcRyd / H, collG = cg / (4. * kPiB);
-            therm<TEAMS>(a, r, L, Te, Ne, H, heatef, comp, &heat, &cool, tid, peG, collG, true);
+            therm<TEAMS, R>(a, r, L, Te, Ne, H, heatef, comp, &heat, &cool, tid, peG, collG, true);
         } else {
-            therm<TEAMS>(a, r, L, Te, Ne, H, heatef, comp, &heat, &cool, tid);
+            therm<TEAMS, R>(a, r, L, Te, Ne, H, heatef, comp, &heat, &cool, tid);
         }
         const double res = heat - cool;
         if (fabs(res) >= 0.02 * (fabs(cool) + fabs(heat)) / 2.) {
@@ -873,15 +908,23 @@ k_balance(MocBalanceArgs a) {
 
 // the team solves are compiled in only where some ion has more than
 // kNlevLane levels
+template <bool PE, class R>
+void launch_balance_r(moc_ctx* ctx, const MocBalanceArgs& a, bool st, dim3 grid, dim3 block, size_t lds) {
+    if (a.maxLev <= kNlevLane) {
+        if (st) hipLaunchKernelGGL((k_balance<false, true, PE, R>), grid, block, lds, ctx->stream, a);
+        else hipLaunchKernelGGL((k_balance<false, false, PE, R>), grid, block, lds, ctx->stream, a);
+    } else {
+        if (st) hipLaunchKernelGGL((k_balance<true, true, PE, R>), grid, block, lds, ctx->stream, a);
+        else hipLaunchKernelGGL((k_balance<true, false, PE, R>), grid, block, lds, ctx->stream, a);
+    }
+}
+
+// the team solves are compiled in only where some ion has more than
+// kNlevLane levels; the fits in FP32 for the mixed-precision form
 template <bool PE>
 void launch_balance(moc_ctx* ctx, const MocBalanceArgs& a, bool st, dim3 grid, dim3 block, size_t lds) {
-    if (a.maxLev <= kNlevLane) {
-        if (st) hipLaunchKernelGGL((k_balance<false, true, PE>), grid, block, lds, ctx->stream, a);
-        else hipLaunchKernelGGL((k_balance<false, false, PE>), grid, block, lds, ctx->stream, a);
-    } else {
-        if (st) hipLaunchKernelGGL((k_balance<true, true, PE>), grid, block, lds, ctx->stream, a);
-        else hipLaunchKernelGGL((k_balance<true, false, PE>), grid, block, lds, ctx->stream, a);
-    }
+    if (a.fitF32) launch_balance_r<PE, float>(ctx, a, st, grid, block, lds);
+    else launch_balance_r<PE, double>(ctx, a, st, grid, block, lds);
 }
 
 }  // namespace

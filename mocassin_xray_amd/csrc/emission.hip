@@ -499,6 +499,9 @@ __device__ double team_lines(const MocEmissionArgs& a, const double (*sX)[MOC_MA
     for (int b0 = 0; b0 < cnt; b0 += P) {
         const int bt = b0 + lane / W;
         const int k = list[bt < cnt ? bt : b0];  // idle teams repeat one
+        // the line luminosities take one ion group per launch: skip rounds
+        // without an ion of the group (the lists ascend in k)
+        if (acc && !__any(bt < cnt && k >= k0 && k < k0 + nk)) continue;
         const int s = a.forSlot[k], ion = a.forIon[k];
         const double X = sX[s][ion];
         const double up = (ion < a.forTop[k] && X > 0.) ? sX[s][ion + 1] / X : 0.;

@@ -120,6 +120,7 @@ struct MocOpacityArgs {
 #define MOC_MAXLEV 27   // levels of an n-level ion
 struct MocBalanceArgs {
     int nCells, NE, NS, maxLev, lgCompton, nShells, nSpec;
+    int fitF32;            // mixed-precision form: rate and cooling fits in FP32
     double XHILimit;
     const int* elemZ;      // [NE]
     const double* abund;   // [NE]

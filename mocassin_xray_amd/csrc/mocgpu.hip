@@ -1141,6 +1141,7 @@ int moc_update(moc_ctx* ctx, int ig, moc_update_stats* stats) {
     a.lgBlack = g.d_black;
     a.nIterT = g.d_nIterT;
     a.counters = b.d_counters;
+    a.fitF32 = ctx->precision == MOC_PREC_MIXED ? 1 : 0;  // the precision split of the mixed form
     a.nGS = a.nGA = 0;
     if (ctx->dust.set && ctx->dust.lgPE) {  // grain terms inside the T iteration
         MocDust& d = ctx->dust;

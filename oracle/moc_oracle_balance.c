@@ -487,6 +487,56 @@ static void compton_rates(const oracle_balance_tables* t, const double* J, int C
     c->cRecoil = (rH * c->X[1][1] * c->ab[1] + rHe * (c->on[2] ? c->X[2][1] * c->ab[2] : 0.) * 2.) * RYD2ERG;
 }
 
+/* the chain of one heavy element Z >= 3 with multi-electron Auger ejection
+ * (ionBalance :1826-1929), at the cell's Ne, with damping on the first pass */
+static void heavy_chain(const oracle_balance_tables* t, ocell* c, int z) {
+    const int NS = t->nstages;
+    double out_[32], in_[32], mat[33], sumMat;
+    int top = min_i(NS, z + 1), ion;
+    if (!c->on[z]) return;
+    memset(out_, 0, sizeof(out_));
+    memset(in_, 0, sizeof(in_));
+    for (ion = 1; ion <= min_i(z, NS - 1); ion++) {
+        int osh, g0, g1, sh;
+        outer_shell(z, z - ion + 1, &osh, &g0, &g1);
+        out_[ion] += c->cI1[z][ion] * c->Ne;
+        for (sh = 1; sh <= osh; sh++) {
+            int na = NAUG(z, ion, sh), ne;
+            out_[ion] += c->pI1[sh][z][ion];
+            for (ne = 2; ne <= na; ne++) {
+                int maxim = min_i(ion + ne - 1, min_i(NS - 1, z));
+                double ratio = c->X[z][maxim] > 1e-30 ? c->X[z][ion] / c->X[z][maxim] : 1.;
+                out_[maxim] += c->pI1[sh][z][ion] * AUG(z, ion, sh, ne) * ratio;
+            }
+        }
+    }
+    for (ion = 1; ion <= min_i(z, NS - 1); ion++) {
+        int found;
+        double chx = chex_scaled(z, ion, c->Te, 0, &found); /* not T-scaled here */
+        in_[ion] = c->Ne * c->alpha[z][ion] + chx * c->H * c->X[1][1];
+    }
+    mat[1] = 1.;
+    sumMat = 1.;
+    for (ion = 2; ion <= top; ion++) {
+        mat[ion] = mat[ion - 1] * out_[ion - 1] / in_[ion - 1];
+        sumMat += mat[ion];
+    }
+    sumMat = 1. / sumMat;
+    c->X[z][top] = mat[top] * sumMat;
+    for (ion = 1; ion <= min_i(z, NS - 1); ion++) {
+        double v;
+        if (c->nIterX < 2 && mat[ion] * sumMat > 0.) {
+            double old = c->X[z][ion] <= 0. ? 1.e-5 : c->X[z][ion];
+            v = pow(10., 0.5 * (log10(old) + log10(mat[ion] * sumMat)));
+        } else {
+            v = mat[ion] * sumMat;
+        }
+        if (v > XMAX) v = XMAX;
+        if (v < 1.e-20) v = 1.e-20;
+        c->X[z][ion] = v;
+    }
+}
+
 /* ionBalance (:1575-1992); returns lgConv */
 static int ion_balance(const oracle_balance_tables* t, ocell* c) {
     const int NS = t->nstages;
@@ -528,52 +578,7 @@ static int ion_balance(const oracle_balance_tables* t, ocell* c) {
             }
         }
         /* heavy elements: chain with multi-electron Auger ejection (:1826-1929) */
-        for (z = 3; z <= NEL; z++) {
-            double out_[32], in_[32], mat[33], sumMat;
-            int top = min_i(NS, z + 1);
-            if (!c->on[z]) continue;
-            memset(out_, 0, sizeof(out_));
-            memset(in_, 0, sizeof(in_));
-            for (ion = 1; ion <= min_i(z, NS - 1); ion++) {
-                int osh, g0, g1, sh;
-                outer_shell(z, z - ion + 1, &osh, &g0, &g1);
-                out_[ion] += c->cI1[z][ion] * c->Ne;
-                for (sh = 1; sh <= osh; sh++) {
-                    int na = NAUG(z, ion, sh), ne;
-                    out_[ion] += c->pI1[sh][z][ion];
-                    for (ne = 2; ne <= na; ne++) {
-                        int maxim = min_i(ion + ne - 1, min_i(NS - 1, z));
-                        double ratio = c->X[z][maxim] > 1e-30 ? c->X[z][ion] / c->X[z][maxim] : 1.;
-                        out_[maxim] += c->pI1[sh][z][ion] * AUG(z, ion, sh, ne) * ratio;
-                    }
-                }
-            }
-            for (ion = 1; ion <= min_i(z, NS - 1); ion++) {
-                int found;
-                double chx = chex_scaled(z, ion, c->Te, 0, &found); /* not T-scaled here */
-                in_[ion] = c->Ne * c->alpha[z][ion] + chx * c->H * c->X[1][1];
-            }
-            mat[1] = 1.;
-            sumMat = 1.;
-            for (ion = 2; ion <= top; ion++) {
-                mat[ion] = mat[ion - 1] * out_[ion - 1] / in_[ion - 1];
-                sumMat += mat[ion];
-            }
-            sumMat = 1. / sumMat;
-            c->X[z][top] = mat[top] * sumMat;
-            for (ion = 1; ion <= min_i(z, NS - 1); ion++) {
-                double v;
-                if (c->nIterX < 2 && mat[ion] * sumMat > 0.) {
-                    double old = c->X[z][ion] <= 0. ? 1.e-5 : c->X[z][ion];
-                    v = pow(10., 0.5 * (log10(old) + log10(mat[ion] * sumMat)));
-                } else {
-                    v = mat[ion] * sumMat;
-                }
-                if (v > XMAX) v = XMAX;
-                if (v < 1.e-20) v = 1.e-20;
-                c->X[z][ion] = v;
-            }
-        }
+        for (z = 3; z <= NEL; z++) heavy_chain(t, c, z);
         /* electron density (:1932-1963) */
         Ne = 0.;
         for (z = 1; z <= NEL; z++) {
@@ -793,7 +798,10 @@ static void grain_terms(const oracle_balance_tables* t, const oracle_balance_sta
     *collG = cg / (4. * ORC_PIB);
 }
 
-static void therm_balance(const oracle_balance_tables* t, ocell* c, double* heat, double* cool) {
+/* parts (may be NULL): free-free, recombination, H Ly-alpha/beta plus
+ * forbidden-line, collisional-ionisation cooling */
+static void therm_balance_parts(const oracle_balance_tables* t, ocell* c, double* heat, double* cool,
+                                double* parts) {
     const int NS = t->nstages;
     const double T = c->Te, Ne = c->Ne;
     double coolFF = 0., coolRec = 0., coolColl, coolCollH = 0., heatSte = 0., Te4 = T / 1.e4, lt;
@@ -852,10 +860,20 @@ static void therm_balance(const oracle_balance_tables* t, ocell* c, double* heat
     }
     *heat = heatSte;
     *cool = coolFF + coolRec + coolColl + coolCollH;
+    if (parts) {
+        parts[0] = coolFF;
+        parts[1] = coolRec;
+        parts[2] = coolColl;
+        parts[3] = coolCollH;
+    }
     if (t->lgCompton) {
         *heat += c->cHeat * Ne + c->cRecoil;
         *cool += c->cCool * Ne;
     }
+}
+
+static void therm_balance(const oracle_balance_tables* t, ocell* c, double* heat, double* cool) {
+    therm_balance_parts(t, c, heat, cool, NULL);
 }
 
 /* updateCell + iterateT for one cell */
@@ -980,4 +998,87 @@ void oracle_alpha(const oracle_balance_tables* t, double Te, double* out /* [31*
 double oracle_nlevel_cooling(const oracle_balance_tables* t, int atom, double Te, double Ne,
                              double scale) {
     return nlevel_cooling(t, atom, Te, Ne, scale);
+}
+
+/* The heavy-element chain of element Z alone (ionBalance :1826-1929) at the
+ * given Te, Ne, H and X(H0), from the per-stage alpha, collisional rate cI1
+ * and per-shell photoionisation rates pI1[shell][stage] (shells 1..10):
+ * X[stage] (1..31) in, out; damp != 0 takes the first-pass geometric
+ * damping. */
+void oracle_heavy_chain(const oracle_balance_tables* t, int z, double Te, double Ne, double H, double XH0,
+                        const double* alpha /* [32] */, const double* cI1 /* [32] */,
+                        const double* pI1 /* [11][32] */, int damp, double* X /* [32] */) {
+    ocell* c = (ocell*)calloc(1, sizeof(ocell));
+    int ion, sh;
+    if (!c) return;
+    c->Te = Te;
+    c->Ne = Ne;
+    c->H = H;
+    c->on[1] = 1;
+    c->on[z] = 1;
+    c->X[1][1] = XH0;
+    c->nIterX = damp ? 1 : 2;
+    for (ion = 0; ion < 32; ion++) {
+        c->X[z][ion] = X[ion];
+        c->alpha[z][ion] = alpha[ion];
+        c->cI1[z][ion] = cI1[ion];
+        for (sh = 0; sh < 11; sh++) c->pI1[sh][z][ion] = pI1[sh * 32 + ion];
+    }
+    heavy_chain(t, c, z);
+    for (ion = 0; ion < 32; ion++) X[ion] = c->X[z][ion];
+    free(c);
+}
+
+/* One T iteration of updateCell (update_mod.f90:15-482) for a cell at a given
+ * Te, from the state in s (J, Ne, Hden, ionDen): calcAlpha, the photo-
+ * ionisation and collisional rates, ionBalance and thermBalance; returns the
+ * heating and cooling.  No grains. */
+void oracle_cell_heat_cool(const oracle_balance_tables* t, const oracle_balance_state* st, int cell,
+                           double Te, double* heat, double* cool) {
+    const int C1 = st->nCells + 1, NE = t->nElemUsed, NS = t->nstages;
+    ocell* c = (ocell*)calloc(1, sizeof(ocell));
+    int e, ion;
+    if (!c) return;
+    c->Te = Te;
+    c->Ne = st->Ne[cell];
+    c->H = st->Hden[cell];
+    for (e = 0; e < NE; e++) {
+        int z = t->elemZ[e];
+        c->on[z] = 1;
+        c->ab[z] = t->abund[e];
+        for (ion = 1; ion <= NS; ion++) c->X[z][ion] = st->ionDen[((size_t)(ion - 1) * NE + e) * C1 + cell];
+    }
+    memcpy(c->X0, c->X, sizeof(c->X));
+    calc_alpha(t, c);
+    photo_ion(t, st->J, C1, cell, c);
+    coll_ion(t, c);
+    c->nIterX = 1;
+    if (t->lgCompton) compton_rates(t, st->J, C1, cell, c);
+    (void)ion_balance(t, c);
+    therm_balance(t, c, heat, cool);
+    free(c);
+}
+
+/* The cooling terms of thermBalance (:887-1215) of a cell at Te and Ne with
+ * ion fractions X[31*32] (Z, stage) of the elements on, no radiation field:
+ * parts = free-free, recombination, H Ly-alpha/beta plus forbidden lines,
+ * collisional ionisation [erg/s per n_H]. */
+void oracle_cooling_parts(const oracle_balance_tables* t, double Te, double Ne, const double* X,
+                          double* parts) {
+    ocell* c = (ocell*)calloc(1, sizeof(ocell));
+    double heat, cool;
+    int e, z, ion;
+    if (!c) return;
+    c->Te = Te;
+    c->Ne = Ne;
+    c->H = 1.;
+    for (e = 0; e < t->nElemUsed; e++) {
+        z = t->elemZ[e];
+        c->on[z] = 1;
+        c->ab[z] = t->abund[e];
+        for (ion = 0; ion < 32; ion++) c->X[z][ion] = X[z * 32 + ion];
+    }
+    coll_ion(t, c);
+    therm_balance_parts(t, c, &heat, &cool, parts);
+    free(c);
 }

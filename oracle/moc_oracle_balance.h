@@ -89,6 +89,12 @@ int oracle_update(const oracle_balance_tables* t, oracle_balance_state* s, int n
 extern "C" {
 #endif
 void oracle_alpha(const oracle_balance_tables* t, double Te, double* out);
+void oracle_heavy_chain(const oracle_balance_tables* t, int z, double Te, double Ne, double H, double XH0,
+                        const double* alpha, const double* cI1, const double* pI1, int damp, double* X);
+void oracle_cooling_parts(const oracle_balance_tables* t, double Te, double Ne, const double* X,
+                          double* parts);
+void oracle_cell_heat_cool(const oracle_balance_tables* t, const oracle_balance_state* st, int cell,
+                           double Te, double* heat, double* cool);
 double oracle_nlevel_cooling(const oracle_balance_tables* t, int atom, double Te, double Ne,
                              double scale);
 /* equilibrium (emission_mod.f90:2505-3028): sum of the line emissivities of

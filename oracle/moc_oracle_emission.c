@@ -317,6 +317,34 @@ static double hl_line(const oracle_emission_tables* t, int f, double Ne, double 
 #undef HL
 }
 
+/* He I recombination line i (0-based, the order of HeIrecLines.dat) per
+ * n_e n(He+) [erg cm^3/s] (:1110-1152; Benjamin, Skillman & Smits 1999 fits),
+ * the T4 clamps as written there, the density bin from the grid's n_e */
+static double hei_line_coeff(const oracle_emission_tables* t, int i, double Te, double Ne, double NeGrid) {
+    const double* c0 = t->heiRec + (size_t)i * 12;
+    double T4h = Te / 10000.;
+    int denint;
+    if (T4h < 0.5) T4h = 0.5;
+    if (T4h < 2.0) T4h = 2.0;
+    if (NeGrid <= 100.) denint = 0;
+    else if (NeGrid <= 1.e4) denint = 1;
+    else if (NeGrid <= 1.e6) denint = 2;
+    else denint = 3;
+#define HC(d, k) c0[((d)-1) * 4 + (k)-1]
+    if (denint > 0 && denint < 3) {
+        const double x1 = HC(denint, 1) * pow(T4h, HC(denint, 2)) * exp(HC(denint, 3) / T4h);
+        const double x2 = HC(denint + 1, 1) * pow(T4h, HC(denint + 1, 2)) * exp(HC(denint + 1, 3) / T4h);
+        return x1 + ((x2 - x1) * (Ne - pow(100., denint)) / (pow(100., denint + 1) - pow(100., denint)));
+    }
+    if (denint == 0) return HC(1, 1) * pow(T4h, HC(1, 2)) * exp(HC(1, 3) / T4h);
+    return HC(3, 1) * pow(T4h, HC(3, 2)) * exp(HC(3, 3) / T4h);
+#undef HC
+}
+
+double oracle_hei_line(const oracle_emission_tables* t, int i, double Te, double Ne) {
+    return hei_line_coeff(t, i, Te, Ne, Ne);
+}
+
 /* RecLinesEmission (:856-1153) summed; hbeta (may be NULL) receives
  * hydroLines(1,4,2) */
 static double rec_lines(const oracle_emission_tables* t, const cellcomp_t* c, double Te, double Ne,
@@ -393,33 +421,9 @@ static double rec_lines(const oracle_emission_tables* t, const cellcomp_t* c, do
         Hb = Hb * Ne * c->X[izp][izp + 1] * c->ab[izp];
         normRec += hl_lines(t, f, Nez, Hb);
     }
-    /* He I recombination lines (:1110-1152); T4 clamps as written there */
-    if (c->on[2]) {
-        double T4h = T4;
-        int denint;
-        if (T4h < 0.5) T4h = 0.5;
-        if (T4h < 2.0) T4h = 2.0;
-        if (NeGrid <= 100.) denint = 0;
-        else if (NeGrid <= 1.e4) denint = 1;
-        else if (NeGrid <= 1.e6) denint = 2;
-        else denint = 3;
-        for (i = 0; i < 34; i++) {
-            const double* c0 = t->heiRec + (size_t)i * 12;
-#define HC(d, k) c0[((d)-1) * 4 + (k)-1]
-            double v;
-            if (denint > 0 && denint < 3) {
-                const double x1 = HC(denint, 1) * pow(T4h, HC(denint, 2)) * exp(HC(denint, 3) / T4h);
-                const double x2 = HC(denint + 1, 1) * pow(T4h, HC(denint + 1, 2)) * exp(HC(denint + 1, 3) / T4h);
-                v = x1 + ((x2 - x1) * (Ne - pow(100., denint)) / (pow(100., denint + 1) - pow(100., denint)));
-            } else if (denint == 0) {
-                v = HC(1, 1) * pow(T4h, HC(1, 2)) * exp(HC(1, 3) / T4h);
-            } else {
-                v = HC(3, 1) * pow(T4h, HC(3, 2)) * exp(HC(3, 3) / T4h);
-            }
-#undef HC
-            normRec += v * Ne * c->ab[2] * c->X[2][2];
-        }
-    }
+    /* He I recombination lines (:1110-1152) */
+    if (c->on[2])
+        for (i = 0; i < 34; i++) normRec += hei_line_coeff(t, i, Te, Ne, NeGrid) * Ne * c->ab[2] * c->X[2][2];
     return normRec;
 }
 

@@ -68,6 +68,8 @@ typedef struct oracle_emission_state {
 } oracle_emission_state;
 
 /* returns 0, or -1 when two gamma edges share a bin (fb_ff stops there) */
+/* He I recombination line i of HeIrecLines.dat per n_e n(He+) [erg cm^3/s] */
+double oracle_hei_line(const oracle_emission_tables* t, int i, double Te, double Ne);
 int oracle_emission(const oracle_emission_tables* t, oracle_emission_state* s, int nthreads);
 /* outputGas line luminosities (output_mod.f90:397-432) over the cells of g
  * (reference loop order, lgBlack < 1): lines[(Z*32 + stage)][L][L], entry

@@ -325,6 +325,55 @@ def update(bt, elem_z, abund, nstages, lgCompton, J, Te, Ne, Hden, ionDen, nthre
     return out
 
 
+def heavy_chain(bt, z, Te, Ne, H, XH0, alpha, cI1, pI1, X, damp=False, nstages=31):
+    """The oracle's heavy-element chain of ionBalance for element z alone:
+    alpha, cI1 [32] and pI1 [11][32] (shell, stage) rates, X [32] the
+    stage fractions in; returns the new X [32]."""
+    L = lib()
+    L.oracle_heavy_chain.argtypes = [C.POINTER(BalanceTablesC), C.c_int, C.c_double, C.c_double,
+                                     C.c_double, C.c_double, _dp, _dp, _dp, C.c_int, _dp]
+    t, k = balance_tables(bt, [1, z], np.ones(2), nstages, True)
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    c = np.ascontiguousarray(cI1, dtype=np.float64)
+    p = np.ascontiguousarray(pI1, dtype=np.float64)
+    x = np.array(X, dtype=np.float64)
+    L.oracle_heavy_chain(C.byref(t), int(z), float(Te), float(Ne), float(H), float(XH0),
+                         a.ctypes.data_as(_dp), c.ctypes.data_as(_dp), p.ctypes.data_as(_dp),
+                         int(bool(damp)), x.ctypes.data_as(_dp))
+    return x
+
+
+def cell_heat_cool(bt, elem_z, abund, nstages, lgCompton, J, Ne, Hden, ionDen, cell, Te):
+    """One T iteration of the oracle's updateCell for `cell` at Te from the
+    given state (J (C+1, B), ionDen (C+1, nElemUsed, nstages)): (heat, cool)."""
+    L = lib()
+    L.oracle_cell_heat_cool.argtypes = [C.POINTER(BalanceTablesC), C.POINTER(BalanceStateC), C.c_int,
+                                        C.c_double, _dp, _dp]
+    t, k = balance_tables(bt, elem_z, abund, nstages, lgCompton)
+    C1 = J.shape[0]
+    s = {"J": np.ascontiguousarray(np.asarray(J, dtype=np.float64).T),
+         "Ne": np.array(Ne, dtype=np.float64), "H": np.ascontiguousarray(Hden, dtype=np.float64),
+         "X": np.ascontiguousarray(np.asarray(ionDen, dtype=np.float64).transpose(2, 1, 0))}
+    st = BalanceStateC(nCells=C1 - 1, J=s["J"].ctypes.data_as(_dp), Ne=s["Ne"].ctypes.data_as(_dp),
+                       Hden=s["H"].ctypes.data_as(_dp), ionDen=s["X"].ctypes.data_as(_dp))
+    h, c = C.c_double(0.), C.c_double(0.)
+    L.oracle_cell_heat_cool(C.byref(t), C.byref(st), int(cell), float(Te), C.byref(h), C.byref(c))
+    return h.value, c.value
+
+
+def cooling_parts(bt, elem_z, abund, nstages, Te, Ne, X):
+    """The oracle's thermBalance cooling terms at Te, Ne for ion fractions X
+    [31, 32] (Z, stage), no radiation field: (free-free, recombination,
+    Ly-alpha/beta + forbidden lines, collisional ionisation) [erg/s per n_H]."""
+    L = lib()
+    L.oracle_cooling_parts.argtypes = [C.POINTER(BalanceTablesC), C.c_double, C.c_double, _dp, _dp]
+    t, k = balance_tables(bt, elem_z, abund, nstages, False)
+    x = np.ascontiguousarray(X, dtype=np.float64).reshape(31 * 32)
+    out = np.zeros(4)
+    L.oracle_cooling_parts(C.byref(t), float(Te), float(Ne), x.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+    return out
+
+
 class EmissionTablesC(C.Structure):
     _fields_ = [
         ("bal", C.POINTER(BalanceTablesC)), ("widFlx", _dp), ("ntk", C.c_int), ("tk", _dp),
@@ -434,6 +483,16 @@ def emission_coeffs(et, bt, elem_z, abund, nstages, Te, Ne, X):
                                 out.ctypes.data_as(_dp)) != 0:
         raise ValueError("frequency grid too coarse for the free-bound edges")
     return {n: out[i * B:(i + 1) * B] for i, n in enumerate(EMISSION_PARTS)}
+
+
+def hei_line(et, bt, elem_z, abund, nstages, i, Te, Ne):
+    """He I recombination line i (order of HeIrecLines.dat) per n_e n(He+)
+    [erg cm^3/s] of the oracle's RecLinesEmission."""
+    L = lib()
+    L.oracle_hei_line.argtypes = [C.POINTER(EmissionTablesC), C.c_int, C.c_double, C.c_double]
+    L.oracle_hei_line.restype = C.c_double
+    t, k = emission_tables(et, bt, elem_z, abund, nstages)
+    return L.oracle_hei_line(C.byref(t), int(i), float(Te), float(Ne))
 
 
 def nlevel_lines(bt, atom, Te, Ne, ionDenUp, rec, scale=1.0, unit=1.9865e9):
