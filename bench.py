@@ -79,19 +79,19 @@ def profiled_traffic(workload: str):
     return None
 
 
-def run_path(m, prec: str, steps: int, warmup: int):
+def run_path(m, prec: str, steps: int, warmup: int, seed: int = SEED, outputs: bool = True):
     import torch
     from mocassin_xray_amd import MocGPU
     with MocGPU(m.tables, prec) as g:
         m.load(g)
         for w in range(1, warmup + 1):
-            g.iterateMC(w, SEED)
+            g.iterateMC(w, seed)
             print(f"bench: {prec} warmup iteration {w} done", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
         stats = []
         t0 = time.perf_counter()
         for s in range(warmup + 1, warmup + steps + 1):
-            stats.append(g.iterateMC(s, SEED))
+            stats.append(g.iterateMC(s, seed))
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         keys = ["Te", "Ne", "ionDen", "Jste", "opacity", "recPDF", "totalLines"]
@@ -100,16 +100,21 @@ def run_path(m, prec: str, steps: int, warmup: int):
         if big_model(m):               # (cell, bin) planes of tens of GB stay on the device
             keys = ["Te", "Ne", "ionDen"]
         state = {k: g.download(k) for k in keys}
-        if not hasattr(m, "grids"):  # the output reductions take one nebula per context
+        if outputs and not hasattr(m, "grids"):  # the output reductions take one nebula per context
             out_ms = {}
-            for name, fn in (("volume_averages", g.volume_averages),
-                             ("line_luminosities", g.line_luminosities), ("sed", g.sed)):
+            outs = [("volume_averages", g.volume_averages), ("line_luminosities", g.line_luminosities),
+                    ("sed", g.sed)]
+            if big_model(m):  # the line luminosities sweep every cell once per ion group
+                outs = [outs[0], outs[2]]
+            for name, fn in outs:
                 t1 = time.perf_counter()
                 res = fn()
                 out_ms[name] = (time.perf_counter() - t1) * 1e3
                 if name == "volume_averages":
                     state["volavg"] = res
             state["output_ms"] = out_ms
+        elif not hasattr(m, "grids"):
+            state["volavg"] = g.volume_averages()
     return dt, stats, state
 
 
@@ -258,6 +263,69 @@ def cpu_baseline(m, gs: dict, target_s: float = 20.0):
             "balance_cells_per_s": (C1 - 1) / t_up}
 
 
+def cpu_baseline_big(m, gs: dict, st_mix: list, target_s: float = 20.0):
+    """The oracle's iteration for a model too large to copy its (cell, bin)
+    planes to the host: opacity, emission and balance are timed on 1024
+    sampled cells of the device state and scaled to every cell; the transport
+    is timed per segment on the same model at 32^3 (its fields from one device
+    iteration) and scaled by the segments this model's packets walk (device
+    counters).  Extrapolated, as labelled in `sample`."""
+    import numpy as np
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as O
+    from mocassin_xray_amd import MocGPU, nebula
+    threads = min(os.cpu_count() or 1, 16)
+    C1 = m.grid.nCells + 1
+    E, Z, NS = m.elems, m.abund, m.nstages
+    k = max(C1 // 1024, 1)
+    sub = np.r_[0, np.arange(1, C1, k)]
+    scale = (C1 - 1) / (sub.size - 1)
+    part = lambda a: np.ascontiguousarray(a[sub])
+    H = m.state["Hden"]
+    t0 = time.perf_counter()
+    O.opacity(m.xsec, m.tables.KNsigmaT, True, E, Z, NS, part(gs["ionDen"]), part(gs["Ne"]),
+              part(gs["Te"]), part(H))
+    t_op = (time.perf_counter() - t0) * scale
+    t0 = time.perf_counter()
+    O.emission(m.et, m.bt, E, Z, NS, part(gs["Te"]), part(gs["Ne"]), part(gs["ionDen"]), nthreads=threads)
+    t_em = (time.perf_counter() - t0) * scale
+    # the balance needs J of the sampled cells: from the small model below
+    small = nebula.xray_nebula(n=32, nbins=m.tables.nbins, n_packets=1_000_000)
+    with MocGPU(small.tables, "fp64") as g:
+        small.load(g)
+        g.iterateMC(1, SEED)
+        fs = {k2: g.download(k2) for k2 in ("opacity", "recPDF", "totalLines", "Jste", "Ne", "Te",
+                                              "ionDen")}
+    Cs = small.grid.nCells + 1
+    ks = max(Cs // 1024, 1)
+    subs = np.r_[0, np.arange(1, Cs, ks)]
+    t0 = time.perf_counter()
+    O.update(small.bt, E, Z, NS, True, np.ascontiguousarray(fs["Jste"][subs]), fs["Te"][subs],
+             fs["Ne"][subs], small.state["Hden"][subs], np.ascontiguousarray(fs["ionDen"][subs]),
+             nthreads=threads)
+    t_up = (time.perf_counter() - t0) * (C1 - 1) / (subs.size - 1)
+    fields = {k2: fs[k2] for k2 in ("opacity", "Ne", "recPDF", "totalLines")}
+    n = 20000
+    t0 = time.perf_counter()
+    O.transport(small.tables, small.grid, fields, n, 5, nthreads=threads)
+    dt = time.perf_counter() - t0
+    n2 = int(min(max(n * target_s / max(dt, 1e-3), n), 2_000_000))
+    t0 = time.perf_counter()
+    _, _, cnt, _ = O.transport(small.tables, small.grid, fields, n2, 6, nthreads=threads)
+    per_seg = (time.perf_counter() - t0) / cnt["segments"]
+    segs = mean(st_mix, "segments")
+    t_tr = per_seg * segs
+    t_iter = t_op + t_em + t_tr + t_up
+    return {"value": m.n_packets / t_iter, "unit": "packets/s", "cores": threads, "kind": "port",
+            "sample": (f"one iteration of {m.name} extrapolated from the oracle (C, FP64, OpenMP): "
+                       f"opacity and emission on {sub.size - 1} of {C1 - 1} cells of the device state, "
+                       f"balance on {subs.size - 1} cells of the same model at 32^3, transport "
+                       f"per segment ({n2} packets over the 32^3 model) times the "
+                       f"{segs:.3e} segments of this model's iteration"),
+            "phase_s": {"opacity": t_op, "emission": t_em, "transport": t_tr, "balance": t_up},
+            "balance_cells_per_s": (C1 - 1) / t_up}
+
+
 def mean(stats, key):
     return sum(s[key] for s in stats) / len(stats)
 
@@ -269,6 +337,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="nebula32")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-band", action="store_true",
+                    help="skip the second FP64 run that measures the Monte Carlo noise band")
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
                     help="time one path only (for profiler runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -301,6 +371,13 @@ def main():
         return
     dt_mix, st_mix, s_mix = res["mixed"]
     dt_64, st_64, s_64 = res["fp64"]
+    band = None
+    if not args.no_band:
+        # Monte Carlo noise band: the FP64 path again with another packet
+        # stream (seed), the same iterations from the same start
+        _, _, s_64b = run_path(m, "fp64", args.steps, args.warmup, seed=SEED + 1, outputs=False)
+        print("bench: fp64 second-seed run done (noise band)", file=sys.stderr, flush=True)
+        band = state_error(m, s_64b, s_64)
     kms = mean(st_mix, "transport_ms")
     dense = all(dense_grid(g) for g in m.grids) if hasattr(m, "grids") else dense_grid(m.grid)
     bytes_mix = transport_bytes(mean(st_mix, "segments"), mean(st_mix, "emissions"),
@@ -335,6 +412,7 @@ def main():
         "output_ms": s_mix.get("output_ms"),
         "mixed_over_fp64": dt_64 / dt_mix,
         "mixed_vs_fp64_error": state_error(m, s_mix, s_64),
+        "mc_noise_band": band,
         "converged_percent": st_mix[-1]["convPercent"],
         "segments_per_packet": mean(st_mix, "segments") / n,
         "emissions_per_packet": mean(st_mix, "emissions") / n,
@@ -343,10 +421,16 @@ def main():
                      "traffic": profiled_traffic(args.workload),
                      "kernel": "k_transport<float>", "kernel_ms": kms},
     }
+    if band is not None:
+        err = out["mixed_vs_fp64_error"]
+        out["mixed_within_band"] = {
+            k: bool(err[k] <= max(0.01, 2.0 * band[k])) for k in ("Te_volavg_rel", "ion_frac_volavg_max_rel")}
     if not args.no_cpu and hasattr(m, "grids"):
         out["cpu_baseline"] = cpu_baseline_batch(m)
     elif not args.no_cpu and not big_model(m):
         out["cpu_baseline"] = cpu_baseline(m, s_mix)
+    elif not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_big(m, s_mix, st_mix)
     print(json.dumps(out))
 
 
