@@ -67,14 +67,15 @@ def dense_grid(g) -> bool:
     return bool(np.array_equal(g.active, 1 + iz + nz * (iy + ny * ix)))
 
 
-def profiled_traffic(workload: str):
-    """HBM bytes per launch of k_transport<float> from the FETCH_SIZE and
-    WRITE_SIZE counter runs recorded in profiles/ (None if not profiled)."""
+def profiled(workload: str, key: str):
+    """A figure of the k_transport<float> counter runs recorded in profiles/
+    (traffic_bytes_per_launch: HBM bytes per launch from FETCH_SIZE and
+    WRITE_SIZE; l2_hit_rate: TCC hits / requests), None if not profiled."""
     f = ROOT / "profiles" / f"transport_{workload}.txt"
     if not f.exists():
         return None
     for line in f.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
 
@@ -418,7 +419,8 @@ def main():
         "emissions_per_packet": mean(st_mix, "emissions") / n,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(args.workload),
+                     "traffic": profiled(args.workload, "traffic_bytes_per_launch"),
+                     "l2_hit_rate": profiled(args.workload, "l2_hit_rate"),
                      "kernel": "k_transport<float>", "kernel_ms": kms},
     }
     if band is not None:
