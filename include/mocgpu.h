@@ -401,6 +401,9 @@ typedef struct moc_iteration_stats {
     double packets;                  /* energy packets transported, all stars */
     double opacity_ms, emission_ms, transport_ms, update_ms; /* kernel times */
     double segments, emissions, escaped; /* transport counters, all stars */
+    double iteration_ms;             /* device time of the iteration: the sum of the four
+                                        phases, or the whole graph when it is replayed
+                                        (moc_set_graph; the phase times are then 0) */
 } moc_iteration_stats;
 
 /* Packets of star iStar per iteration (nPhotons(iStar)); the packet energy
@@ -415,6 +418,34 @@ int moc_set_packets(moc_ctx* ctx, int iStar, int64_t nPhotons);
  * Needs moc_set_atomic, moc_set_balance and moc_set_emission.  Packet
  * streams are keyed by (seed, nIterateMC, iStar). */
 int moc_iterate(moc_ctx* ctx, int ig, int nIterateMC, uint64_t seed, moc_iteration_stats* stats);
+
+/* Replay of moc_iterate as one graph launch.  No counterpart in the
+ * reference, whose iterateMC (iteration_mod.f90:32) is host-driven; it is
+ * for callers that run many iterations of a large model, where the launch
+ * and wait between the phases cost more than the small kernels.  With
+ * enable != 0, iterations from the second on capture their device work once
+ * per seed (opacity, emission, tally reset, every star's transport, the J
+ * normalisation, the balance and the grain temperatures) and replay it; the
+ * iteration number reaches the packet streams through device memory, so a
+ * replay draws exactly the packets of the eager call.  Any moc_set_* call,
+ * a new grid or new packet counts drop the captured graph.  Tuning values
+ * and arguments are those of the capture. */
+int moc_set_graph(moc_ctx* ctx, int enable);
+
+/* Launch parameters of the transport kernel.  No counterpart in the
+ * reference (its photon loop has no launch shape).  None of them changes
+ * which packets are drawn: each packet keeps its own RNG stream, so two
+ * settings differ only in the order of the atomic additions to J. */
+typedef struct moc_tuning {
+    int32_t emitBatch;      /* diffuse packets a wave waits for before it emits them
+                               together, 1..64 (32) */
+    int32_t segsPerTrip;    /* path segments a flying lane steps per trip through the
+                               wave's loop, 1..64 (2) */
+    int32_t transportWaves; /* resident waves of one transport launch, 64..2^20 (8192) */
+    int32_t graph;          /* moc_get_tuning: 1 when moc_set_graph is on */
+} moc_tuning;
+int moc_set_tuning(moc_ctx* ctx, const moc_tuning* t);
+int moc_get_tuning(moc_ctx* ctx, moc_tuning* t);
 
 /* updateCell for every active cell (update_mod.f90:15): from the normalised
  * J (moc_normalize_j) and Te, Ne, Hden, ionDen, writes Te, Ne, ionDen,

@@ -37,7 +37,7 @@ EXPORTS = [
     "moc_update", "moc_set_emission", "moc_emission", "moc_set_packets", "moc_iterate",
     "moc_set_dust", "moc_dust_opacity", "moc_dust_emission", "moc_dust_temperature",
     "moc_grid_attach_batch", "moc_volume_averages", "moc_sed", "moc_line_luminosities",
-    "moc_volume_averages_noalloc",
+    "moc_volume_averages_noalloc", "moc_set_graph", "moc_set_tuning", "moc_get_tuning",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -126,7 +126,14 @@ class MocIterationStats(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "convPercent", "noHitPercent", "noIonBalPercent", "noTeBalPercent", "packets",
         "opacity_ms", "emission_ms", "transport_ms", "update_ms", "segments", "emissions",
-        "escaped")]
+        "escaped", "iteration_ms")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class MocTuning(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("emitBatch", "segsPerTrip", "transportWaves", "graph")]
 
     def as_dict(self) -> dict:
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -191,6 +198,12 @@ def lib() -> C.CDLL:
     L.moc_set_packets.restype = C.c_int
     L.moc_iterate.argtypes = [vp, C.c_int, C.c_int, C.c_uint64, C.POINTER(MocIterationStats)]
     L.moc_iterate.restype = C.c_int
+    L.moc_set_graph.argtypes = [vp, C.c_int]
+    L.moc_set_graph.restype = C.c_int
+    L.moc_set_tuning.argtypes = [vp, C.POINTER(MocTuning)]
+    L.moc_set_tuning.restype = C.c_int
+    L.moc_get_tuning.argtypes = [vp, C.POINTER(MocTuning)]
+    L.moc_get_tuning.restype = C.c_int
     L.moc_version.argtypes = []
     L.moc_version.restype = C.c_char_p
     _LIB = L

@@ -84,6 +84,9 @@ struct MocTransportArgs {
     int64_t n;
     uint64_t seed;
     uint32_t ctr_base;
+    const uint32_t* ctrDev;  // non-null: the RNG counter base is *ctrDev + ctr_base (graph replays)
+    int emitBatch;           // lanes of a wave that wait to emit together
+    int segsPerTrip;         // segments per flying lane and trip through the loop
     // batched grids: packet ids gridFirst[k] .. gridFirst[k+1]-1 belong to grid k
     // (RNG packet id (k << 32) + local id); nGrids = 0 for a single grid
     int nGrids;
@@ -422,8 +425,25 @@ struct MocAtomic {
     double *d_abund = nullptr, *d_sigma = nullptr, *d_brems = nullptr, *d_log10nu = nullptr;
 };
 
+// Runtime tuning parameters (moc_set_tuning)
+struct MocTuning {
+    int emitBatch = 32;         // transport: lanes that wait to emit together
+    int segsPerTrip = 2;        // transport: segments per flying lane and loop trip
+    int transportWaves = 8192;  // transport: resident waves (the launch's lanes / 64)
+};
+
 struct moc_ctx {
     std::string err;
+    MocTuning tune;
+    // graph replay of moc_iterate (moc_set_graph): the iteration captured once
+    // per seed and replayed; the iteration number reaches the transport's RNG
+    // counter through d_ctr
+    bool graphOn = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    uint64_t graphSeed = 0;
+    uint32_t* d_ctr = nullptr;
+    uint32_t ctrHost = 0;  // source of the d_ctr upload, alive until the replay is waited for
     int precision = MOC_PREC_FP64;
     int nbins = 0, nAngles = 180, lgCompton = 1, nStars = 0;
     double ionEdge1 = 0.0;
@@ -451,6 +471,8 @@ int moc_launch_transport_f64(moc_ctx* ctx, const MocTransportArgs<double>& a, fl
 int moc_launch_transport_f32(moc_ctx* ctx, const MocTransportArgs<float>& a, float* ms);
 void moc_convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);
 void moc_scale_f64(double* p, int64_t n, double f, hipStream_t s);
+void moc_fill_zero_u64(unsigned long long* p, int64_t n, hipStream_t s);
+void moc_copy_f64(const double* src, double* dst, int64_t n, hipStream_t s);
 int moc_launch_opacity(moc_ctx* ctx, const MocOpacityArgs& a);
 int moc_launch_emission(moc_ctx* ctx, const MocEmissionArgs& a);
 int moc_launch_dust_opacity(moc_ctx* ctx, const MocDustArgs& a);

@@ -312,6 +312,8 @@ int refresh_f32(moc_ctx* ctx) {
 }
 
 int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who);
+MocBalanceArgs balance_args(moc_ctx* ctx, const double** W);
+void drop_graph(moc_ctx* ctx);  // a captured iteration holds the arguments of its time
 int run_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ctr_base,
                   moc_transport_stats* stats);
 
@@ -394,6 +396,8 @@ int moc_create(moc_ctx** out, const moc_tables* t) {
 
 void moc_destroy(moc_ctx* ctx) {
     if (!ctx) return;
+    drop_graph(ctx);
+    dfree(ctx->d_ctr);
     free_grid(ctx->grid);
     free_atomic(ctx->atomic);
     free_balance(ctx->balance);
@@ -433,6 +437,7 @@ namespace {
 // are numbered one grid after the other and whose axes and active maps are
 // concatenated (active ids become global).
 int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who) {
+    drop_graph(ctx);
     const std::string w(who);
     for (int k = 0; k < nG; ++k) {
         const moc_grid_desc* gd = gds + k;
@@ -588,6 +593,7 @@ extern "C" {
 
 int moc_set_atomic(moc_ctx* ctx, const moc_atomic* at) {
     if (!ctx) return MOC_ERR_ARG;
+    drop_graph(ctx);
     if (!at || at->nElemUsed < 1 || at->nElemUsed > 30 || at->nstages < 1 || at->nstages > 31 ||
         at->nSpecies < 0 || !at->elemZ || !at->abund || (at->nSpecies > 0 && (!at->speciesElem ||
         !at->speciesIon || !at->sigma)) || !at->brems) {
@@ -680,7 +686,7 @@ int moc_opacity(moc_ctx* ctx, int ig) {
     return run_opacity(ctx, false, nullptr);
 }
 
-int run_opacity(moc_ctx* ctx, bool skipUnchanged, float* ms) {
+MocOpacityArgs opacity_args(moc_ctx* ctx, bool skipUnchanged) {
     MocGrid& g = ctx->grid;
     MocAtomic& at = ctx->atomic;
     MocOpacityArgs a;
@@ -697,6 +703,13 @@ int run_opacity(moc_ctx* ctx, bool skipUnchanged, float* ms) {
     a.opacity = dust ? g.d_gasOpac : g.d_opacity;
     a.TeOld = skipUnchanged ? g.d_TeOld : nullptr;
     a.NeOld = skipUnchanged ? g.d_NeOld : nullptr;
+    return a;
+}
+
+int run_opacity(moc_ctx* ctx, bool skipUnchanged, float* ms) {
+    MocGrid& g = ctx->grid;
+    const bool dust = ctx->dust.set && ctx->dust.lgGas;
+    const MocOpacityArgs a = opacity_args(ctx, skipUnchanged);
     MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
     int rc = moc_launch_opacity(ctx, a);
     if (rc) return rc;
@@ -826,27 +839,70 @@ namespace {
 
 // One transport launch: star iStar's n packets, or (iStar = 0, a batch of
 // grids) every star's nPhotons packets, each in its own grid.
-int run_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ctr_base,
-                  moc_transport_stats* stats) {
+// transport arguments of source s (0-based) with n packets; the tuning of the
+// context; the batch views when the context holds a batch of grids
+MocTransportArgs<double> transport_args_f64(moc_ctx* ctx, int s, int64_t n, uint64_t seed,
+                                            uint32_t ctr_base, double deltaE) {
     MocGrid& g = ctx->grid;
-    const bool batch = g.nGrids > 1;
+    MocTransportArgs<double> a;
+    a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nCells = g.nCells; a.sym = g.sym;
+    a.ax = g.d_ax; a.ay = g.d_ay; a.az = g.d_az; a.vx = g.d_vx; a.vy = g.d_vy; a.vz = g.d_vz;
+    a.active = g.d_active; a.dense = g.dense ? 1 : 0; a.R_out = g.R_out;
+    a.nbins = ctx->nbins; a.nAngles = ctx->nAngles; a.lgCompton = ctx->lgCompton;
+    a.ionEdge1 = ctx->ionEdge1; a.nuArray = ctx->d_nu; a.KNsigmaT = ctx->d_KNsigmaT;
+    a.KNcdf = ctx->d_KNcdf; a.Pcomp = ctx->d_Pcomp;
+    a.sourceCDF = ctx->d_sourceCDF + (size_t)s * ctx->nbins;
+    a.opacity = g.d_opacity; a.Ne = g.d_Ne; a.recPDF = g.d_recPDF; a.totalLines = g.d_totalLines;
+    a.scaOpac = g.d_scaOpac; a.lgDust = ctx->dust.set ? 1 : 0;
+    a.dustOnly = ctx->dust.set && !ctx->dust.lgGas ? 1 : 0;
+    a.J = g.d_J; a.escaped = g.d_escaped; a.Qphot = ctx->d_Qphot; a.counters = ctx->d_counters;
+    for (int k = 0; k < 3; ++k) { a.star[k] = ctx->starPos[3 * s + k]; a.starIdx[k] = ctx->starIdx[3 * s + k] - 1; }
+    a.deltaE = deltaE; a.n = n; a.seed = seed; a.ctr_base = ctr_base; a.ctrDev = nullptr;
+    a.emitBatch = ctx->tune.emitBatch; a.segsPerTrip = ctx->tune.segsPerTrip;
+    a.nGrids = g.nGrids > 1 ? g.nGrids : 0; a.grids = g.d_view64; a.gridFirst = g.d_gridFirst;
+    return a;
+}
+
+MocTransportArgs<float> transport_args_f32(moc_ctx* ctx, int s, int64_t n, uint64_t seed,
+                                           uint32_t ctr_base, double deltaE) {
+    MocGrid& g = ctx->grid;
+    MocTransportArgs<float> a;
+    a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nCells = g.nCells; a.sym = g.sym;
+    a.ax = g.f_ax; a.ay = g.f_ay; a.az = g.f_az; a.vx = g.f_vx; a.vy = g.f_vy; a.vz = g.f_vz;
+    a.active = g.d_active; a.dense = g.dense ? 1 : 0; a.R_out = (float)g.R_out;
+    a.nbins = ctx->nbins; a.nAngles = ctx->nAngles; a.lgCompton = ctx->lgCompton;
+    a.ionEdge1 = (float)ctx->ionEdge1; a.nuArray = ctx->f_nu; a.KNsigmaT = ctx->f_KNsigmaT;
+    a.KNcdf = ctx->f_KNcdf; a.Pcomp = ctx->f_Pcomp;
+    a.sourceCDF = ctx->f_sourceCDF + (size_t)s * ctx->nbins;
+    a.opacity = g.f_opacity; a.Ne = g.f_Ne; a.recPDF = g.f_recPDF; a.totalLines = g.f_totalLines;
+    a.scaOpac = g.f_scaOpac; a.lgDust = ctx->dust.set ? 1 : 0;
+    a.dustOnly = ctx->dust.set && !ctx->dust.lgGas ? 1 : 0;
+    a.J = g.d_J; a.escaped = g.d_escaped; a.Qphot = ctx->d_Qphot; a.counters = ctx->d_counters;
+    for (int k = 0; k < 3; ++k) { a.star[k] = (float)ctx->starPos[3 * s + k]; a.starIdx[k] = ctx->starIdx[3 * s + k] - 1; }
+    a.deltaE = (float)deltaE; a.n = n; a.seed = seed; a.ctr_base = ctr_base; a.ctrDev = nullptr;
+    a.emitBatch = ctx->tune.emitBatch; a.segsPerTrip = ctx->tune.segsPerTrip;
+    a.nGrids = g.nGrids > 1 ? g.nGrids : 0; a.grids = g.d_view32; a.gridFirst = g.d_gridFirst;
+    return a;
+}
+
+// a batch of grids: per-grid packet ranges and views (star k lights grid k),
+// uploaded to the device; *n becomes the packet count of the whole batch
+int upload_batch_views(moc_ctx* ctx, int iStar, int64_t* n) {
+    MocGrid& g = ctx->grid;
     const int s = iStar > 0 ? iStar - 1 : 0;
-    const double deltaE = ctx->Lstar[s] / ctx->nPhotons[s];  // photon_mod.f90:88
-    if (!(deltaE >= 0.0)) { ctx->err = "moc_transport: deltaEUsed < 0"; return MOC_ERR_ARG; }
-    // a batch: per-grid packet ranges and views (star k lights grid k)
     std::vector<int64_t> first;
     std::vector<MocGridView<double>> v64;
     std::vector<MocGridView<float>> v32;
-    if (batch) {
+    {
         const int nG = g.nGrids;
         first.assign((size_t)nG + 1, 0);
         for (int k = 0; k < nG; ++k) {
-            const int64_t cnt = iStar == 0 ? (int64_t)ctx->nPhotons[k] : (k == s ? n : 0);
+            const int64_t cnt = iStar == 0 ? (int64_t)ctx->nPhotons[k] : (k == s ? *n : 0);
             if (cnt < 0) { ctx->err = "moc_transport: negative packet count"; return MOC_ERR_ARG; }
             first[k + 1] = first[k] + cnt;
             if (cnt > 0xffffffffLL) { ctx->err = "moc_transport: more than 2^32 packets in one grid"; return MOC_ERR_ARG; }
         }
-        n = first[nG];
+        *n = first[nG];
         v64.resize(nG);
         v32.resize(nG);
         for (int k = 0; k < nG; ++k) {
@@ -878,44 +934,26 @@ int run_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ct
             (rc = h2d(ctx, g.d_view32, v32.data(), v32.size() * sizeof(MocGridView<float>), "grid views")))
             return rc;
     }
+    return MOC_OK;
+}
+
+int run_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ctr_base,
+                  moc_transport_stats* stats) {
+    MocGrid& g = ctx->grid;
+    const int s = iStar > 0 ? iStar - 1 : 0;
+    const double deltaE = ctx->Lstar[s] / ctx->nPhotons[s];  // photon_mod.f90:88
+    if (!(deltaE >= 0.0)) { ctx->err = "moc_transport: deltaEUsed < 0"; return MOC_ERR_ARG; }
+    int rc;
+    if (g.nGrids > 1 && (rc = upload_batch_views(ctx, iStar, &n))) return rc;
     MOC_TRY(hipMemsetAsync(ctx->d_counters, 0, MOC_CNT_N * sizeof(unsigned long long), ctx->stream), "counters");
     MOC_TRY(hipMemsetAsync(ctx->d_Qphot, 0, sizeof(double), ctx->stream), "Qphot");
     float ms = 0.f;
-    int rc;
     if (ctx->precision == MOC_PREC_FP64) {
-        MocTransportArgs<double> a;
-        a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nCells = g.nCells; a.sym = g.sym;
-        a.ax = g.d_ax; a.ay = g.d_ay; a.az = g.d_az; a.vx = g.d_vx; a.vy = g.d_vy; a.vz = g.d_vz;
-        a.active = g.d_active; a.dense = g.dense ? 1 : 0; a.R_out = g.R_out;
-        a.nbins = ctx->nbins; a.nAngles = ctx->nAngles; a.lgCompton = ctx->lgCompton;
-        a.ionEdge1 = ctx->ionEdge1; a.nuArray = ctx->d_nu; a.KNsigmaT = ctx->d_KNsigmaT;
-        a.KNcdf = ctx->d_KNcdf; a.Pcomp = ctx->d_Pcomp;
-        a.sourceCDF = ctx->d_sourceCDF + (size_t)s * ctx->nbins;
-        a.opacity = g.d_opacity; a.Ne = g.d_Ne; a.recPDF = g.d_recPDF; a.totalLines = g.d_totalLines;
-        a.scaOpac = g.d_scaOpac; a.lgDust = ctx->dust.set ? 1 : 0;
-        a.dustOnly = ctx->dust.set && !ctx->dust.lgGas ? 1 : 0;
-        a.J = g.d_J; a.escaped = g.d_escaped; a.Qphot = ctx->d_Qphot; a.counters = ctx->d_counters;
-        for (int k = 0; k < 3; ++k) { a.star[k] = ctx->starPos[3 * s + k]; a.starIdx[k] = ctx->starIdx[3 * s + k] - 1; }
-        a.deltaE = deltaE; a.n = n; a.seed = seed; a.ctr_base = ctr_base;
-        a.nGrids = batch ? g.nGrids : 0; a.grids = g.d_view64; a.gridFirst = g.d_gridFirst;
+        MocTransportArgs<double> a = transport_args_f64(ctx, s, n, seed, ctr_base, deltaE);
         rc = moc_launch_transport_f64(ctx, a, &ms);
     } else {
         if ((rc = refresh_f32(ctx))) return rc;
-        MocTransportArgs<float> a;
-        a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nCells = g.nCells; a.sym = g.sym;
-        a.ax = g.f_ax; a.ay = g.f_ay; a.az = g.f_az; a.vx = g.f_vx; a.vy = g.f_vy; a.vz = g.f_vz;
-        a.active = g.d_active; a.dense = g.dense ? 1 : 0; a.R_out = (float)g.R_out;
-        a.nbins = ctx->nbins; a.nAngles = ctx->nAngles; a.lgCompton = ctx->lgCompton;
-        a.ionEdge1 = (float)ctx->ionEdge1; a.nuArray = ctx->f_nu; a.KNsigmaT = ctx->f_KNsigmaT;
-        a.KNcdf = ctx->f_KNcdf; a.Pcomp = ctx->f_Pcomp;
-        a.sourceCDF = ctx->f_sourceCDF + (size_t)s * ctx->nbins;
-        a.opacity = g.f_opacity; a.Ne = g.f_Ne; a.recPDF = g.f_recPDF; a.totalLines = g.f_totalLines;
-        a.scaOpac = g.f_scaOpac; a.lgDust = ctx->dust.set ? 1 : 0;
-        a.dustOnly = ctx->dust.set && !ctx->dust.lgGas ? 1 : 0;
-        a.J = g.d_J; a.escaped = g.d_escaped; a.Qphot = ctx->d_Qphot; a.counters = ctx->d_counters;
-        for (int k = 0; k < 3; ++k) { a.star[k] = (float)ctx->starPos[3 * s + k]; a.starIdx[k] = ctx->starIdx[3 * s + k] - 1; }
-        a.deltaE = (float)deltaE; a.n = n; a.seed = seed; a.ctr_base = ctr_base;
-        a.nGrids = batch ? g.nGrids : 0; a.grids = g.d_view32; a.gridFirst = g.d_gridFirst;
+        MocTransportArgs<float> a = transport_args_f32(ctx, s, n, seed, ctr_base, deltaE);
         rc = moc_launch_transport_f32(ctx, a, &ms);
     }
     if (rc) return rc;
@@ -951,6 +989,7 @@ extern "C" {
 
 int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* t) {
     if (!ctx) return MOC_ERR_ARG;
+    drop_graph(ctx);
     if (!ctx->atomic.set) { ctx->err = "moc_set_balance: call moc_set_atomic first"; return MOC_ERR_STATE; }
     MocAtomic& at = ctx->atomic;
     if (!t || t->maxLev < 1 || t->maxLev > MOC_MAXLEV || t->nShells < 0 || !t->speciesShell ||
@@ -1129,27 +1168,8 @@ int moc_update(moc_ctx* ctx, int ig, moc_update_stats* stats) {
     int rc = ensure_balance_grid(ctx);
     if (rc) return rc;
     MocBalance& b = ctx->balance;
-    MocBalanceArgs a = b.a;
-    const double* W = a.R;
-    a.nCells = g.nCells;
-    a.R = g.d_rates;
-    a.Te = g.d_Te;
-    a.Ne = g.d_Ne;
-    a.Hden = g.d_Hden;
-    a.ionDen = g.d_ionDen;
-    a.lgConverged = g.d_conv;
-    a.lgBlack = g.d_black;
-    a.nIterT = g.d_nIterT;
-    a.counters = b.d_counters;
-    a.fitF32 = ctx->precision == MOC_PREC_MIXED ? 1 : 0;  // the precision split of the mixed form
-    a.nGS = a.nGA = 0;
-    if (ctx->dust.set && ctx->dust.lgPE) {  // grain terms inside the T iteration
-        MocDust& d = ctx->dust;
-        a.nGS = d.nS; a.nGA = d.nA; a.B = ctx->nbins;
-        a.gAbun = d.d_abun; a.gWeight = d.d_weight; a.gRadius = d.d_radius; a.gVn = d.d_vn;
-        a.gMsurf = d.d_msurf; a.gCabs = d.d_Cabs; a.gIp = d.d_ip; a.gNu = ctx->d_nu; a.J = g.d_J;
-        a.Tdust = g.d_Tdust; a.Ndust = g.d_Ndust; a.collD = g.d_collD; a.peD = g.d_peD;
-    }
+    const double* W = nullptr;
+    const MocBalanceArgs a = balance_args(ctx, &W);
     MOC_TRY(hipMemsetAsync(b.d_counters, 0, 4 * sizeof(unsigned long long), ctx->stream), "counters");
     // TeOld, NeOld = Te, Ne before the balance (iteration_mod.f90:1139-1142)
     const size_t C1b = ((size_t)g.nCells + 1) * sizeof(double);
@@ -1183,8 +1203,45 @@ int moc_update(moc_ctx* ctx, int ig, moc_update_stats* stats) {
     return MOC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// updateCell arguments of the attached grid; *W receives the rate weights
+MocBalanceArgs balance_args(moc_ctx* ctx, const double** W) {
+    MocGrid& g = ctx->grid;
+    MocBalance& b = ctx->balance;
+    MocBalanceArgs a = b.a;
+    *W = a.R;
+    a.nCells = g.nCells;
+    a.R = g.d_rates;
+    a.Te = g.d_Te;
+    a.Ne = g.d_Ne;
+    a.Hden = g.d_Hden;
+    a.ionDen = g.d_ionDen;
+    a.lgConverged = g.d_conv;
+    a.lgBlack = g.d_black;
+    a.nIterT = g.d_nIterT;
+    a.counters = b.d_counters;
+    a.fitF32 = ctx->precision == MOC_PREC_MIXED ? 1 : 0;  // the precision split of the mixed form
+    a.nGS = a.nGA = 0;
+    if (ctx->dust.set && ctx->dust.lgPE) {  // grain terms inside the T iteration
+        MocDust& d = ctx->dust;
+        a.nGS = d.nS; a.nGA = d.nA; a.B = ctx->nbins;
+        a.gAbun = d.d_abun; a.gWeight = d.d_weight; a.gRadius = d.d_radius; a.gVn = d.d_vn;
+        a.gMsurf = d.d_msurf; a.gCabs = d.d_Cabs; a.gIp = d.d_ip; a.gNu = ctx->d_nu; a.J = g.d_J;
+        a.Tdust = g.d_Tdust; a.Ndust = g.d_Ndust; a.collD = g.d_collD; a.peD = g.d_peD;
+    }
+    return a;
+}
+
+}  // namespace
+
+extern "C" {
+
 int moc_set_emission(moc_ctx* ctx, const moc_emission_tables* t) {
     if (!ctx) return MOC_ERR_ARG;
+    drop_graph(ctx);
     if (!ctx->balance.set) { ctx->err = "moc_set_emission: call moc_set_balance first"; return MOC_ERR_STATE; }
     MocAtomic& at = ctx->atomic;
     const int B = ctx->nbins;
@@ -1347,7 +1404,7 @@ int moc_emission(moc_ctx* ctx, int ig, float* kernel_ms) {
     return run_emission(ctx, false, kernel_ms);
 }
 
-int run_emission(moc_ctx* ctx, bool skipUnchanged, float* kernel_ms) {
+MocEmissionArgs emission_args(moc_ctx* ctx, bool skipUnchanged) {
     MocGrid& g = ctx->grid;
     MocEmissionArgs a = ctx->emission.a;
     a.bal = ctx->balance.a;
@@ -1368,6 +1425,12 @@ int run_emission(moc_ctx* ctx, bool skipUnchanged, float* kernel_ms) {
         a.dAbun = d.d_abun; a.dWeight = d.d_weight; a.dTsub = d.d_tsub; a.dCabs = d.d_Cabs;
         a.Tdust = g.d_Tdust; a.Ndust = g.d_Ndust; a.Hden = g.d_Hden;
     }
+    return a;
+}
+
+int run_emission(moc_ctx* ctx, bool skipUnchanged, float* kernel_ms) {
+    MocGrid& g = ctx->grid;
+    const MocEmissionArgs a = emission_args(ctx, skipUnchanged);
     MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
     int rc = moc_launch_emission(ctx, a);
     if (rc) return rc;
@@ -1380,9 +1443,217 @@ int run_emission(moc_ctx* ctx, bool skipUnchanged, float* kernel_ms) {
     return MOC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+void drop_graph(moc_ctx* ctx) {
+    if (ctx->graphExec) (void)hipGraphExecDestroy(ctx->graphExec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graphExec = nullptr;
+    ctx->graph = nullptr;
+}
+
+// The work of one iteration from the second on (cells whose Te and Ne moved
+// by less than 0.5% keep their opacity and emission), enqueued on the
+// context's stream with no waits, events, host copies or memsets, so that it
+// can be captured as a graph: zero fills and device copies are kernels, the
+// FP32 copies of the transport's inputs are refreshed unconditionally, and
+// the transport reads its RNG counter base (nIterateMC * 256) from d_ctr.
+// nBatch: the packet count of a batch of grids (views uploaded beforehand).
+int enqueue_iteration(moc_ctx* ctx, uint64_t seed, int64_t nBatch) {
+    MocGrid& g = ctx->grid;
+    const bool dustOnly = ctx->dust.set && !ctx->dust.lgGas;
+    const int64_t C1 = (int64_t)g.nCells + 1, n = C1 * ctx->nbins;
+    int rc;
+    if (dustOnly) {  // kappa and setDustPDF of every cell
+        if ((rc = moc_launch_dust_opacity(ctx, dust_args(ctx))) ||
+            (rc = moc_launch_dust_pdf(ctx, dust_args(ctx))))
+            return rc;
+    } else {
+        if ((rc = moc_launch_opacity(ctx, opacity_args(ctx, true)))) return rc;
+        if (ctx->dust.set && (rc = moc_launch_dust_opacity(ctx, dust_args(ctx)))) return rc;
+        if ((rc = moc_launch_emission(ctx, emission_args(ctx, true)))) return rc;
+    }
+    // the tallies (moc_reset_tallies) and the transport counters
+    moc_fill_zero_u64(reinterpret_cast<unsigned long long*>(g.d_J), n, ctx->stream);
+    moc_fill_zero_u64(reinterpret_cast<unsigned long long*>(g.d_escaped), n, ctx->stream);
+    moc_fill_zero_u64(ctx->d_counters, MOC_CNT_N, ctx->stream);
+    moc_fill_zero_u64(reinterpret_cast<unsigned long long*>(ctx->d_Qphot), 1, ctx->stream);
+    if (ctx->precision != MOC_PREC_FP64) {
+        g.f32_dirty = true;
+        if ((rc = refresh_f32(ctx))) return rc;
+    }
+    for (int s = 0; s < (g.nGrids > 1 ? 1 : ctx->nStars); ++s) {
+        const int64_t np = g.nGrids > 1 ? nBatch : (int64_t)ctx->nPhotons[s];
+        const double deltaE = ctx->Lstar[s] / ctx->nPhotons[s];  // photon_mod.f90:88
+        if (!(deltaE >= 0.0)) { ctx->err = "moc_iterate: deltaEUsed < 0"; return MOC_ERR_ARG; }
+        const uint32_t cb = (uint32_t)s + 1u;  // (nIterateMC * 256 from d_ctr) + iStar
+        if (ctx->precision == MOC_PREC_FP64) {
+            MocTransportArgs<double> a = transport_args_f64(ctx, s, np, seed, cb, deltaE);
+            a.ctrDev = ctx->d_ctr;
+            rc = moc_launch_transport_f64(ctx, a, nullptr);
+        } else {
+            MocTransportArgs<float> a = transport_args_f32(ctx, s, np, seed, cb, deltaE);
+            a.ctrDev = ctx->d_ctr;
+            rc = moc_launch_transport_f32(ctx, a, nullptr);
+        }
+        if (rc) return rc;
+    }
+    // moc_normalize_j
+    moc_scale_f64(g.d_J, n, g.sym ? 1.e-9 / 8.0 : 1.e-9, ctx->stream);
+    if (g.sym) moc_scale_f64(g.d_escaped, n, 1.0 / 8.0, ctx->stream);
+    if (dustOnly) {  // moc_dust_temperature
+        MocDust& d = ctx->dust;
+        moc_fill_zero_u64(d.d_counters, 2, ctx->stream);
+        return moc_launch_dust_temp(ctx, dust_args(ctx), g.d_J, d.d_W, d.nS * d.nA + 1, d.d_R);
+    }
+    // moc_update
+    MocBalance& b = ctx->balance;
+    const double* W = nullptr;
+    const MocBalanceArgs a = balance_args(ctx, &W);
+    moc_fill_zero_u64(b.d_counters, 4, ctx->stream);
+    moc_copy_f64(g.d_Te, g.d_TeOld, C1, ctx->stream);
+    moc_copy_f64(g.d_Ne, g.d_NeOld, C1, ctx->stream);
+    if ((rc = moc_launch_balance(ctx, a, g.d_J, W, b.ncol, g.d_rates, g.rates_chunk))) return rc;
+    if (ctx->dust.set) {
+        MocDust& d = ctx->dust;
+        if ((rc = moc_launch_dust_temp(ctx, dust_args(ctx), g.d_J, d.d_W, d.nS * d.nA + 1, d.d_R)))
+            return rc;
+    }
+    MOC_TRY(hipGetLastError(), "iteration launch");
+    return MOC_OK;
+}
+
+int capture_iteration(moc_ctx* ctx, uint64_t seed) {
+    MocGrid& g = ctx->grid;
+    const bool dustOnly = ctx->dust.set && !ctx->dust.lgGas;
+    int rc;
+    // what allocates or uploads from the host happens before the capture
+    if (!ctx->d_ctr && (rc = dalloc_n(ctx, &ctx->d_ctr, 1, "iteration counter"))) return rc;
+    if (!dustOnly && (rc = ensure_balance_grid(ctx))) return rc;
+    int64_t nBatch = 0;
+    if (g.nGrids > 1 && (rc = upload_batch_views(ctx, 0, &nBatch))) return rc;
+    MOC_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+    rc = enqueue_iteration(ctx, seed, nBatch);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    g.f32_dirty = true;  // the conversions were recorded, not run
+    if (rc || e != hipSuccess || !graph) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!rc) {
+            ctx->err = std::string("graph capture: ") + hipGetErrorString(e);
+            rc = MOC_ERR_HIP;
+        }
+        return rc;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e2 != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        ctx->err = std::string("graph instantiate: ") + hipGetErrorString(e2);
+        return MOC_ERR_HIP;
+    }
+    ctx->graph = graph;
+    ctx->graphExec = exec;
+    ctx->graphSeed = seed;
+    return MOC_OK;
+}
+
+// moc_iterate from the second iteration on, as one graph launch
+int iterate_graph(moc_ctx* ctx, int nIterateMC, uint64_t seed, moc_iteration_stats* stats) {
+    MocGrid& g = ctx->grid;
+    const bool dustOnly = ctx->dust.set && !ctx->dust.lgGas;
+    int rc;
+    if (!ctx->graphExec || ctx->graphSeed != seed) {
+        drop_graph(ctx);
+        if ((rc = capture_iteration(ctx, seed))) return rc;
+    }
+    ctx->ctrHost = (uint32_t)nIterateMC * 256u;
+    MOC_TRY(hipMemcpyAsync(ctx->d_ctr, &ctx->ctrHost, sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream),
+            "iteration counter");
+    MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
+    MOC_TRY(hipGraphLaunch(ctx->graphExec, ctx->stream), "graph launch");
+    MOC_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
+    MOC_TRY(hipEventSynchronize(ctx->ev1), "iteration graph");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    g.f32_dirty = true;
+    unsigned long long cnt[MOC_CNT_N], ub[4] = {0, 0, 0, 0};
+    MOC_TRY(hipMemcpy(cnt, ctx->d_counters, sizeof(cnt), hipMemcpyDeviceToHost), "counters");
+    if (cnt[MOC_CNT_ERROR]) {
+        ctx->err = "moc_transport: " + std::to_string(cnt[MOC_CNT_ERROR]) +
+                   " packets reached an insane state (index outside the grid or emission "
+                   "from an inactive cell)";
+        return MOC_ERR_KERNEL;
+    }
+    if (!stats) return MOC_OK;
+    double converged = 0.0;
+    if (dustOnly) {
+        MOC_TRY(hipMemcpy(ub, ctx->dust.d_counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost),
+                "counters");
+        converged = (double)ub[1];
+        ub[1] = 0;
+    } else {
+        MOC_TRY(hipMemcpy(ub, ctx->balance.d_counters, sizeof(ub), hipMemcpyDeviceToHost), "counters");
+        std::vector<int> conv((size_t)g.nCells + 1);
+        MOC_TRY(hipMemcpy(conv.data(), g.d_conv, conv.size() * 4, hipMemcpyDeviceToHost), "conv");
+        for (int i = 1; i <= g.nCells; ++i) converged += conv[i];
+    }
+    double packets = 0.0;
+    for (int s = 0; s < ctx->nStars; ++s) packets += (double)(int64_t)ctx->nPhotons[s];
+    const double nc = g.nCells > 0 ? (double)g.nCells : 1.0;
+    stats->convPercent = 100.0 * converged / nc;
+    stats->noHitPercent = 100.0 * (double)ub[0] / nc;
+    stats->noIonBalPercent = 100.0 * (double)ub[1] / nc;
+    stats->noTeBalPercent = 100.0 * (double)ub[2] / nc;
+    stats->packets = packets;
+    stats->opacity_ms = stats->emission_ms = stats->transport_ms = stats->update_ms = 0.0;
+    stats->iteration_ms = ms;
+    stats->segments = (double)cnt[MOC_CNT_SEGMENTS];
+    stats->emissions = (double)cnt[MOC_CNT_EMISSIONS];
+    stats->escaped = (double)cnt[MOC_CNT_ESCAPED];
+    return MOC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int moc_set_graph(moc_ctx* ctx, int enable) {
+    if (!ctx) return MOC_ERR_ARG;
+    drop_graph(ctx);
+    ctx->graphOn = enable != 0;
+    return MOC_OK;
+}
+
+int moc_set_tuning(moc_ctx* ctx, const moc_tuning* t) {
+    if (!ctx) return MOC_ERR_ARG;
+    if (!t || t->emitBatch < 1 || t->emitBatch > 64 || t->segsPerTrip < 1 || t->segsPerTrip > 64 ||
+        t->transportWaves < 64 || t->transportWaves > (1 << 20)) {
+        ctx->err = "moc_set_tuning: emitBatch and segsPerTrip in 1..64, transportWaves in 64..2^20";
+        return MOC_ERR_ARG;
+    }
+    drop_graph(ctx);
+    ctx->tune.emitBatch = t->emitBatch;
+    ctx->tune.segsPerTrip = t->segsPerTrip;
+    ctx->tune.transportWaves = t->transportWaves;
+    return MOC_OK;
+}
+
+int moc_get_tuning(moc_ctx* ctx, moc_tuning* t) {
+    if (!ctx || !t) return MOC_ERR_ARG;
+    t->emitBatch = ctx->tune.emitBatch;
+    t->segsPerTrip = ctx->tune.segsPerTrip;
+    t->transportWaves = ctx->tune.transportWaves;
+    t->graph = ctx->graphOn ? 1 : 0;
+    return MOC_OK;
+}
+
 int moc_set_packets(moc_ctx* ctx, int iStar, int64_t n) {
     if (!ctx) return MOC_ERR_ARG;
     if (iStar < 1 || iStar > ctx->nStars || n < 1) { ctx->err = "moc_set_packets: iStar/n"; return MOC_ERR_ARG; }
+    if (ctx->nPhotons[iStar - 1] != (double)n) drop_graph(ctx);
     ctx->nPhotons[iStar - 1] = (double)n;
     return MOC_OK;
 }
@@ -1398,6 +1669,7 @@ int moc_iterate(moc_ctx* ctx, int ig, int nIterateMC, uint64_t seed, moc_iterati
         return MOC_ERR_STATE;
     }
     if (nIterateMC < 1 || nIterateMC > (1 << 23)) { ctx->err = "moc_iterate: nIterateMC"; return MOC_ERR_ARG; }
+    if (ctx->graphOn && nIterateMC > 1) return iterate_graph(ctx, nIterateMC, seed, stats);
     const bool skip = nIterateMC > 1;
     float tOp = 0.f, tEm = 0.f;
     int rc;
@@ -1455,6 +1727,7 @@ int moc_iterate(moc_ctx* ctx, int ig, int nIterateMC, uint64_t seed, moc_iterati
         stats->emission_ms = tEm;
         stats->transport_ms = tTr;
         stats->update_ms = us.kernel_ms;
+        stats->iteration_ms = tOp + tEm + tTr + us.kernel_ms;
         stats->segments = seg;
         stats->emissions = emi;
         stats->escaped = esc;
@@ -1464,6 +1737,7 @@ int moc_iterate(moc_ctx* ctx, int ig, int nIterateMC, uint64_t seed, moc_iterati
 
 int moc_set_dust(moc_ctx* ctx, const moc_dust_tables* t) {
     if (!ctx) return MOC_ERR_ARG;
+    drop_graph(ctx);
     if (ctx->grid.attached && ctx->grid.nGrids > 1) {
         ctx->err = "moc_set_dust: grains are not supported in a batch of grids";
         return MOC_ERR_STATE;

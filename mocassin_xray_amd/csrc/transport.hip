@@ -434,15 +434,14 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Emissions wait for company: an emission walks a CDF row by bisection, a
-// chain of dependent loads, so it runs when at least kEmitBatch lanes of the
+// chain of dependent loads, so it runs when at least a.emitBatch lanes of the
 // wave need one (or no lane is flying); until then those lanes sit out while
-// the others keep stepping their segments, kSegsPerTrip per trip through the
-// loop.  Packet ids are taken a wave at a time from the work counter.  The
-// values were chosen by timing the 32^3 nebula (1e7 packets, 1000 bins):
-// batch 4/8/16/24/32/40/48 gave 45/34/21/17/16/17/19 ms, and two segments per
-// trip took 16 to 15.5 ms.
-constexpr int kEmitBatch = 32;
-constexpr int kSegsPerTrip = 2;
+// the others keep stepping their segments, a.segsPerTrip per trip through the
+// loop.  Packet ids are taken a wave at a time from the work counter.  Both
+// are tuning parameters (moc_set_tuning; the package's autotuner times them
+// on the model): on the 32^3 nebula (1e7 packets, 1000 bins) batch
+// 4/8/16/24/32/40/48 gave 45/34/21/17/16/17/19 ms, and two segments per trip
+// 16 -> 15.5 ms.
 
 extern __shared__ __align__(16) unsigned char transport_lds[];
 
@@ -476,6 +475,8 @@ __global__ void __launch_bounds__(256) k_transport(MocTransportArgs<T> a) {
     MocRng rng;
     bool more = true;
     const int lane = threadIdx.x & 63;
+    const uint32_t ctrBase = a.ctrDev ? *a.ctrDev + a.ctr_base : a.ctr_base;
+    const int emitBatch = a.emitBatch, segsPerTrip = a.segsPerTrip;
     const unsigned long long below = (1ull << lane) - 1ull;
     while (true) {
         const unsigned long long want = __ballot(p.state == ST_DONE && more);
@@ -495,7 +496,7 @@ __global__ void __launch_bounds__(256) k_transport(MocTransportArgs<T> a) {
                             if (a.gridFirst[mid] <= (int64_t)id) lo = mid; else hi = mid;
                         }
                         const unsigned long long local = id - (unsigned long long)a.gridFirst[lo];
-                        rng.init(a.seed, ((unsigned long long)lo << 32) + local, a.ctr_base);
+                        rng.init(a.seed, ((unsigned long long)lo << 32) + local, ctrBase);
                         g = a.grids[lo];
                         p.x = g.star[0];
                         p.y = g.star[1];
@@ -504,7 +505,7 @@ __global__ void __launch_bounds__(256) k_transport(MocTransportArgs<T> a) {
                         p.iy = g.starIdx[1];
                         p.iz = g.starIdx[2];
                     } else {
-                        rng.init(a.seed, id, a.ctr_base);
+                        rng.init(a.seed, id, ctrBase);
                         p.x = a.star[0];
                         p.y = a.star[1];
                         p.z = a.star[2];
@@ -523,11 +524,11 @@ __global__ void __launch_bounds__(256) k_transport(MocTransportArgs<T> a) {
         const unsigned long long emitters = __ballot(p.state == ST_EMIT);
         const unsigned long long flyers = __ballot(p.state == ST_FLY);
         if (!emitters && !flyers && !__ballot(more)) break;
-        if (emitters && (__popcll(emitters) >= kEmitBatch || !flyers || !__ballot(more))) {
+        if (emitters && (__popcll(emitters) >= emitBatch || !flyers || !__ballot(more))) {
             if (p.state == ST_EMIT) emit(a, g, p, rng, c);
         }
 #pragma unroll 1
-        for (int k = 0; k < kSegsPerTrip; ++k)
+        for (int k = 0; k < segsPerTrip; ++k)
             if (p.state == ST_FLY) segment(a, g, p, rng, c);
     }
     // all lanes of the wave have left the loop: reduce the counters once
@@ -563,18 +564,38 @@ __global__ void k_scale(double* __restrict__ p, int64_t n, double f) {
     for (int64_t k = i; k < n; k += st) p[k] *= f;
 }
 
+// zero fill, for code that a caller may capture in a graph (no memset nodes)
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = i; k < n; k += st) p[k] = 0ull;
+}
+
+// device-to-device copy, for code that a caller may capture in a graph
+__global__ void k_copy_f64(const double* __restrict__ src, double* __restrict__ dst, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = i; k < n; k += st) dst[k] = src[k];
+}
+
 // LDS for the staged geometry of one workgroup (2 x (nx+ny+nz) values)
 constexpr size_t kMaxGeoLds = 32768;
 
+// ms == nullptr: enqueue only (no events, no wait), as inside a graph capture
 template <typename T>
 int launch(moc_ctx* ctx, const MocTransportArgs<T>& a, float* ms) {
     if (a.n <= 0) {
         if (ms) *ms = 0.f;
         return MOC_OK;
     }
-    // a resident wave count that fills 256 CUs several times over; lanes beyond
-    // the packet count exit at their first fetch
-    const int64_t lanes_max = 256LL * 32LL * 64LL;
+    if (a.emitBatch < 1 || a.emitBatch > 64 || a.segsPerTrip < 1 || a.segsPerTrip > 64) {
+        ctx->err = "transport: tuning parameters out of range (emit batch 1..64, segments per trip 1..64)";
+        return MOC_ERR_ARG;
+    }
+    // the resident wave count (a tuning parameter; the default fills the 256
+    // CUs several times over); lanes beyond the packet count exit at their
+    // first fetch
+    const int64_t lanes_max = 64LL * (int64_t)ctx->tune.transportWaves;
     int64_t lanes = a.n < lanes_max ? a.n : lanes_max;
     const int block = 256;
     const int blocks = (int)((lanes + block - 1) / block);
@@ -584,23 +605,24 @@ int launch(moc_ctx* ctx, const MocTransportArgs<T>& a, float* ms) {
                    " points) exceed the " + std::to_string(kMaxGeoLds) + " B LDS staging area";
         return MOC_ERR_ARG;
     }
-    (void)hipEventRecord(ctx->ev0, ctx->stream);
+    if (ms) (void)hipEventRecord(ctx->ev0, ctx->stream);
     if (a.nGrids > 0)
         hipLaunchKernelGGL((k_transport<T, true>), dim3(blocks), dim3(block), lds, ctx->stream, a);
     else
         hipLaunchKernelGGL((k_transport<T, false>), dim3(blocks), dim3(block), lds, ctx->stream, a);
-    (void)hipEventRecord(ctx->ev1, ctx->stream);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         ctx->err = std::string("transport launch: ") + hipGetErrorString(e);
         return MOC_ERR_HIP;
     }
+    if (!ms) return MOC_OK;
+    (void)hipEventRecord(ctx->ev1, ctx->stream);
     e = hipEventSynchronize(ctx->ev1);
     if (e != hipSuccess) {
         ctx->err = std::string("transport kernel: ") + hipGetErrorString(e);
         return MOC_ERR_HIP;
     }
-    if (ms) (void)hipEventElapsedTime(ms, ctx->ev0, ctx->ev1);
+    (void)hipEventElapsedTime(ms, ctx->ev0, ctx->ev1);
     return MOC_OK;
 }
 
@@ -623,4 +645,18 @@ void moc_scale_f64(double* p, int64_t n, double f, hipStream_t s) {
     int64_t blocks = (n + 255) / 256;
     if (blocks > 65535) blocks = 65535;
     hipLaunchKernelGGL(k_scale, dim3((unsigned)blocks), dim3(256), 0, s, p, n, f);
+}
+
+void moc_fill_zero_u64(unsigned long long* p, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+}
+
+void moc_copy_f64(const double* src, double* dst, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_copy_f64, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n);
 }

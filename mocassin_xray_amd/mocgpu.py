@@ -25,7 +25,7 @@ from ._lib import _dp, _ip
 from ._lib import (CELLBIN_FIELDS, FIELDS, GRAIN_FIELDS, MOC_DT_F64, MOC_DT_I32,
                    MOC_LAYOUT_FORTRAN, MOC_PREC_FP64, MOC_PREC_MIXED, MocAtomic, MocBalanceTables, MocDustTables,
                    MocEmissionTables, MocGridDesc, MocIterationStats, MocTables, MocTransportStats,
-                   MocUpdateStats)
+                   MocTuning, MocUpdateStats)
 
 
 class MocError(RuntimeError):
@@ -468,6 +468,28 @@ class MocGPU:
         self._check(self._L.moc_iterate(self._ctx, 1, int(nIterateMC), int(seed) & (2**64 - 1),
                                         C.byref(st)), "moc_iterate")
         return st.as_dict()
+
+    def set_graph(self, enable: bool) -> None:
+        """Replay iterations from the second on as one captured graph launch
+        (moc_set_graph); the packets drawn are those of the eager call."""
+        self._check(self._L.moc_set_graph(self._ctx, 1 if enable else 0), "moc_set_graph")
+
+    def get_tuning(self) -> dict:
+        """The transport's launch parameters and the graph switch."""
+        t = MocTuning()
+        self._check(self._L.moc_get_tuning(self._ctx, C.byref(t)), "moc_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, emitBatch: int | None = None, segsPerTrip: int | None = None,
+                   transportWaves: int | None = None) -> None:
+        """Set some of the transport's launch parameters (moc_set_tuning); the
+        others keep their values.  They change no packet's random stream."""
+        cur = self.get_tuning()
+        t = MocTuning(int(emitBatch if emitBatch is not None else cur["emitBatch"]),
+                      int(segsPerTrip if segsPerTrip is not None else cur["segsPerTrip"]),
+                      int(transportWaves if transportWaves is not None else cur["transportWaves"]),
+                      0)
+        self._check(self._L.moc_set_tuning(self._ctx, C.byref(t)), "moc_set_tuning")
 
     def MCIterationDriver(self, maxIterateMC: int, minConvergence: float = 100.0,
                           seed: int = 20261203, lgAutoPackets: bool = False,
