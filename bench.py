@@ -5,7 +5,9 @@ diffuse emission of every cell, the workload's energy packets through the
 grid, normalisation of J and the ionisation/thermal balance of every cell,
 with all state resident in HBM.  W untimed iterations run first from the
 reference's starting guess, then exactly K iterations are timed between
-device synchronisations.  `value` is packets/s of the whole iteration on the
+device synchronisations.  With --graph, iterations from the second on replay
+one captured HIP graph of the iteration (moc_set_graph), and one more eager
+iteration after the timed ones gives the per-phase kernel times.  `value` is packets/s of the whole iteration on the
 mixed-precision path (FP32 transport, FP64 balance); the all-FP64 path runs
 the same iterations beside it, and the difference of the two states after the
 same iterations is reported.  The oracle (plain C, FP64, OpenMP) is timed on
@@ -80,11 +82,17 @@ def profiled(workload: str, key: str):
     return None
 
 
-def run_path(m, prec: str, steps: int, warmup: int, seed: int = SEED, outputs: bool = True):
+def run_path(m, prec: str, steps: int, warmup: int, seed: int = SEED, outputs: bool = True,
+             graph: bool = False):
+    """W warmup and K timed iterations; with graph, iterations from the second
+    on replay one captured graph (captured in the warmup when W >= 2).  After
+    the state is read, one more iteration runs eagerly for the phase times."""
     import torch
     from mocassin_xray_amd import MocGPU
     with MocGPU(m.tables, prec) as g:
         m.load(g)
+        g.set_graph(graph)
+        tuning = g.get_tuning()
         for w in range(1, warmup + 1):
             g.iterateMC(w, seed)
             print(f"bench: {prec} warmup iteration {w} done", file=sys.stderr, flush=True)
@@ -101,6 +109,10 @@ def run_path(m, prec: str, steps: int, warmup: int, seed: int = SEED, outputs: b
         if big_model(m):               # (cell, bin) planes of tens of GB stay on the device
             keys = ["Te", "Ne", "ionDen"]
         state = {k: g.download(k) for k in keys}
+        state["tuning"] = tuning
+        if graph:  # the phases of one more iteration, eager (a graph replay times only the whole)
+            g.set_graph(False)
+            state["eager_stats"] = g.iterateMC(warmup + steps + 1, seed)
         if outputs and not hasattr(m, "grids"):  # the output reductions take one nebula per context
             out_ms = {}
             outs = [("volume_averages", g.volume_averages), ("line_luminosities", g.line_luminosities),
@@ -116,6 +128,11 @@ def run_path(m, prec: str, steps: int, warmup: int, seed: int = SEED, outputs: b
             state["output_ms"] = out_ms
         elif not hasattr(m, "grids"):
             state["volavg"] = g.volume_averages()
+    if graph:
+        # phase times and the transport counters of the same eager launch
+        stats = [dict(st, **{k: state["eager_stats"][k] for k in (
+            "opacity_ms", "emission_ms", "transport_ms", "update_ms", "segments", "emissions",
+            "escaped")}) for st in stats]
     return dt, stats, state
 
 
@@ -342,6 +359,8 @@ def main():
                     help="skip the second FP64 run that measures the Monte Carlo noise band")
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
                     help="time one path only (for profiler runs)")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay iterations 2.. as one captured HIP graph (moc_set_graph)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -364,7 +383,7 @@ def main():
     for prec in ("mixed", "fp64"):
         if args.only not in (None, prec):
             continue
-        dt, st, state = run_path(m, prec, args.steps, args.warmup)
+        dt, st, state = run_path(m, prec, args.steps, args.warmup, graph=args.graph)
         res[prec] = (dt, st, state)
         print(f"bench: {prec} {args.steps * n / dt:.3e} packets/s, "
               f"{dt / args.steps * 1e3:.1f} ms/iteration", file=sys.stderr, flush=True)
@@ -376,7 +395,8 @@ def main():
     if not args.no_band:
         # Monte Carlo noise band: the FP64 path again with another packet
         # stream (seed), the same iterations from the same start
-        _, _, s_64b = run_path(m, "fp64", args.steps, args.warmup, seed=SEED + 1, outputs=False)
+        _, _, s_64b = run_path(m, "fp64", args.steps, args.warmup, seed=SEED + 1, outputs=False,
+                               graph=args.graph)
         print("bench: fp64 second-seed run done (noise band)", file=sys.stderr, flush=True)
         band = state_error(m, s_64b, s_64)
     kms = mean(st_mix, "transport_ms")
@@ -411,6 +431,9 @@ def main():
                  "phase_ms": {k: mean(st_64, k + "_ms") for k in ("opacity", "emission",
                                                                   "transport", "update")}},
         "output_ms": s_mix.get("output_ms"),
+        "graph_replay": args.graph,
+        "transport_launch": {k: s_mix["tuning"][k] for k in ("emitBatch", "segsPerTrip",
+                                                             "transportWaves")},
         "mixed_over_fp64": dt_64 / dt_mix,
         "mixed_vs_fp64_error": state_error(m, s_mix, s_64),
         "mc_noise_band": band,

@@ -440,8 +440,9 @@ typedef struct moc_tuning {
     int32_t emitBatch;      /* diffuse packets a wave waits for before it emits them
                                together, 1..64 (32) */
     int32_t segsPerTrip;    /* path segments a flying lane steps per trip through the
-                               wave's loop, 1..64 (2) */
-    int32_t transportWaves; /* resident waves of one transport launch, 64..2^20 (8192) */
+                               wave's loop, 1..64 (32) */
+    int32_t transportWaves; /* resident waves of one transport launch, 64..2^20 (4096:
+                               four per SIMD) */
     int32_t graph;          /* moc_get_tuning: 1 when moc_set_graph is on */
 } moc_tuning;
 int moc_set_tuning(moc_ctx* ctx, const moc_tuning* t);

@@ -21,9 +21,9 @@ import json
 import time
 from pathlib import Path
 
-WAVES = (2048, 4096, 8192, 16384, 32768)
+WAVES = (1024, 2048, 4096, 8192, 16384)
 EMIT_BATCH = (8, 16, 24, 32, 48, 64)
-SEGS_PER_TRIP = (1, 2, 3, 4, 6)
+SEGS_PER_TRIP = (4, 8, 16, 24, 32, 48, 64)
 
 
 def _time(g, n: int, seed: int, repeats: int) -> float:

@@ -428,8 +428,8 @@ struct MocAtomic {
 // Runtime tuning parameters (moc_set_tuning)
 struct MocTuning {
     int emitBatch = 32;         // transport: lanes that wait to emit together
-    int segsPerTrip = 2;        // transport: segments per flying lane and loop trip
-    int transportWaves = 8192;  // transport: resident waves (the launch's lanes / 64)
+    int segsPerTrip = 32;       // transport: segments per flying lane and loop trip
+    int transportWaves = 4096;  // transport: resident waves (the launch's lanes / 64)
 };
 
 struct moc_ctx {
