@@ -109,8 +109,14 @@ module mocgpu_mod
 
   type, bind(C), public :: moc_iteration_stats
      real(c_double) :: convPercent, noHitPercent, noIonBalPercent, noTeBalPercent, packets, &
-          opacity_ms, emission_ms, transport_ms, update_ms, segments, emissions, escaped
+          opacity_ms, emission_ms, transport_ms, update_ms, segments, emissions, escaped, &
+          iteration_ms
   end type moc_iteration_stats
+
+  ! launch parameters of the transport (moc_set_tuning); graph is set by moc_get_tuning
+  type, bind(C), public :: moc_tuning
+     integer(c_int32_t) :: emitBatch, segsPerTrip, transportWaves, graph
+  end type moc_tuning
 
   interface
      integer(c_int) function moc_create(ctx, tables) bind(C, name="moc_create")
@@ -271,8 +277,26 @@ module mocgpu_mod
        integer(c_int), value :: ig
        real(c_double), intent(out) :: sed(*), totalE
      end function moc_sed
+     ! replay of moc_iterate as one captured graph (no counterpart in the reference)
+     integer(c_int) function moc_set_graph(ctx, enable) bind(C, name="moc_set_graph")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: enable
+     end function moc_set_graph
+     ! launch parameters of the transport (no counterpart in the reference)
+     integer(c_int) function moc_set_tuning(ctx, t) bind(C, name="moc_set_tuning")
+       import :: c_int, c_ptr, moc_tuning
+       type(c_ptr), value :: ctx
+       type(moc_tuning), intent(in) :: t
+     end function moc_set_tuning
+     integer(c_int) function moc_get_tuning(ctx, t) bind(C, name="moc_get_tuning")
+       import :: c_int, c_ptr, moc_tuning
+       type(c_ptr), value :: ctx
+       type(moc_tuning), intent(out) :: t
+     end function moc_get_tuning
   end interface
 
+  public :: moc_set_graph, moc_set_tuning, moc_get_tuning
   public :: moc_create, moc_destroy, moc_last_error, moc_version, moc_grid_attach, moc_upload, &
        moc_download, moc_set_atomic, moc_opacity, moc_set_balance, moc_set_emission, &
        moc_emission, moc_set_packets, moc_iterate, moc_update, moc_reset_tallies, moc_transport, &
