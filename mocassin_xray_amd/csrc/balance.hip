@@ -653,8 +653,11 @@ __device__ void therm(const MocBalanceArgs& a, const RT& r, const Lds& L, double
     }
     if constexpr (TEAMS) {
         team_cooling<8, R>(a, L, a.nNlvT[0], a.nlvT[0], T, Ne, tid);
-        team_cooling<16, R>(a, L, a.nNlvT[1], a.nlvT[1], T, Ne, tid);
-        team_cooling<27, R>(a, L, a.nNlvT[2], a.nlvT[2], T, Ne, tid);
+        team_cooling<10, R>(a, L, a.nNlvT[1], a.nlvT[1], T, Ne, tid);
+        team_cooling<13, R>(a, L, a.nNlvT[2], a.nlvT[2], T, Ne, tid);
+        team_cooling<16, R>(a, L, a.nNlvT[3], a.nlvT[3], T, Ne, tid);
+        team_cooling<17, R>(a, L, a.nNlvT[4], a.nlvT[4], T, Ne, tid);
+        team_cooling<27, R>(a, L, a.nNlvT[5], a.nlvT[5], T, Ne, tid);
     }
     __syncthreads();
     // The sums run over the lanes (elements, (element, stage) entries, n-level

@@ -556,8 +556,11 @@ __global__ void __launch_bounds__(64) k_em_lines(MocEmissionArgs a) {
     }
     if constexpr (TEAMS) {
         lines += team_lines<8>(a, sX, a.nForT[0], a.forT[0], Te, Ne, lane);
-        lines += team_lines<16>(a, sX, a.nForT[1], a.forT[1], Te, Ne, lane);
-        lines += team_lines<27>(a, sX, a.nForT[2], a.forT[2], Te, Ne, lane);
+        lines += team_lines<10>(a, sX, a.nForT[1], a.forT[1], Te, Ne, lane);
+        lines += team_lines<13>(a, sX, a.nForT[2], a.forT[2], Te, Ne, lane);
+        lines += team_lines<16>(a, sX, a.nForT[3], a.forT[3], Te, Ne, lane);
+        lines += team_lines<17>(a, sX, a.nForT[4], a.forT[4], Te, Ne, lane);
+        lines += team_lines<27>(a, sX, a.nForT[5], a.forT[5], Te, Ne, lane);
     }
     lines = wave_sum(lines);
     if (lane == 0) a.lineSum[cell] = lines;
@@ -609,8 +612,11 @@ __global__ void __launch_bounds__(64) k_line_lum(MocEmissionArgs a, const double
         }
         if constexpr (TEAMS) {  // ions of more than kNlevLane levels, by teams of lanes
             team_lines<8>(a, sX, a.nForT[0], a.forT[0], Te, Ne, lane, acc, w, L, k0, nk);
-            team_lines<16>(a, sX, a.nForT[1], a.forT[1], Te, Ne, lane, acc, w, L, k0, nk);
-            team_lines<27>(a, sX, a.nForT[2], a.forT[2], Te, Ne, lane, acc, w, L, k0, nk);
+            team_lines<10>(a, sX, a.nForT[1], a.forT[1], Te, Ne, lane, acc, w, L, k0, nk);
+            team_lines<13>(a, sX, a.nForT[2], a.forT[2], Te, Ne, lane, acc, w, L, k0, nk);
+            team_lines<16>(a, sX, a.nForT[3], a.forT[3], Te, Ne, lane, acc, w, L, k0, nk);
+            team_lines<17>(a, sX, a.nForT[4], a.forT[4], Te, Ne, lane, acc, w, L, k0, nk);
+            team_lines<27>(a, sX, a.nForT[5], a.forT[5], Te, Ne, lane, acc, w, L, k0, nk);
         }
         if (lane == 0 && k0 == 0 && hs >= 0 && a.NS > 1) {  // hydroLines(1,4,2) (RecLinesEmission)
             const HydroLines h = hydro_setup(a, 1, sX[hs][2], a.abund[hs], Te, Ne, log10Ne, true);

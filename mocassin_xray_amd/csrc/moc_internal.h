@@ -121,6 +121,10 @@ struct MocOpacityArgs {
 #define MOC_MAXE 30     // elements
 #define MOC_MAXS 31     // ionisation stages
 #define MOC_MAXLEV 27   // levels of an n-level ion
+// n-level ions of more than 5 levels are solved by teams of lanes, in classes
+// of at most 8, 10, 13, 16, 17 and 27 levels (nlevel.h)
+constexpr int kNlevTeams = 6;
+
 struct MocBalanceArgs {
     int nCells, NE, NS, maxLev, lgCompton, nShells, nSpec;
     int fitF32;            // mixed-precision form: rate and cooling fits in FP32
@@ -172,8 +176,8 @@ struct MocBalanceArgs {
     // by teams of lanes with the matrix in registers (nlevel_team): the
     // entries k of the nlv list with [0] 6-8 levels (teams of 8 lanes), [1]
     // 9-16 (teams of 16), [2] 17-27 (teams of 32); the others one ion per lane
-    int nNlvT[3];
-    const int* nlvT[3];
+    int nNlvT[kNlevTeams];
+    const int* nlvT[kNlevTeams];
     int colHit, colComp;      // rate columns of the hit test and the Compton sums
     int ncol;                 // rate columns per cell
     int nX;                   // entries of a packed per-cell stage row (sum over elements
@@ -262,8 +266,8 @@ struct MocEmissionArgs {
     const int* forIon;
     const int* forAtom;
     const int* forTop;       // min(Z+1, nstages)
-    int nForT[3];            // entries k of the for list solved by teams of 8, 16 and 32 lanes
-    const int* forT[3];      // (6-8, 9-16, 17-27 levels)
+    int nForT[kNlevTeams];   // entries k of the for list solved by teams of lanes, per
+    const int* forT[kNlevTeams];  // level class (nlevel_team_class)
     const int* hlMap;        // [9][12]
     const double* hlDens;    // [files][14]
     const double* hlLines;   // [files][14][16][9]

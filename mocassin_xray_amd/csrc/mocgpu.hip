@@ -1090,13 +1090,13 @@ int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* t) {
         PUT(a.nlvIon, ni.data(), ni.size(), "nlvIon");
         PUT(a.nlvAtom, na.data(), na.size(), "nlvAtom");
         // ions of more than 5 levels (as cut at maxLev) are solved by teams of
-        // 8, 16 or 32 lanes
-        std::vector<int> tl[3];
+        // 8, 16 or 32 lanes, in the level classes of nlevel_team_class
+        std::vector<int> tl[kNlevTeams];
         for (int k = 0; k < (int)na.size(); ++k) {
             const int c = nlevel_team_class((int)std::min<int64_t>(t->atomMeta[8 * na[k]], t->maxLev));
             if (c >= 0) tl[c].push_back(k);
         }
-        for (int c = 0; c < 3; ++c) {
+        for (int c = 0; c < kNlevTeams; ++c) {
             a.nNlvT[c] = (int)tl[c].size();
             PUT(a.nlvT[c], tl[c].data(), tl[c].size(), "nlvT");
         }
@@ -1351,14 +1351,14 @@ int moc_set_emission(moc_ctx* ctx, const moc_emission_tables* t) {
     EPUT(a.forTop, forTop.data(), forTop.size(), "forTop");
     {
         // ions of more than 5 levels (as cut at maxLev) are solved by teams of
-        // 8, 16 or 32 lanes
-        std::vector<int> tl[3];
+        // 8, 16 or 32 lanes, in the level classes of nlevel_team_class
+        std::vector<int> tl[kNlevTeams];
         const std::vector<int>& lev = ctx->balance.atomLevels;
         for (int k = 0; k < (int)forAtom.size(); ++k) {
             const int c = nlevel_team_class(lev[forAtom[k]]);
             if (c >= 0) tl[c].push_back(k);
         }
-        for (int c = 0; c < 3; ++c) {
+        for (int c = 0; c < kNlevTeams; ++c) {
             a.nForT[c] = (int)tl[c].size();
             EPUT(a.forT[c], tl[c].data(), tl[c].size(), "forT");
         }

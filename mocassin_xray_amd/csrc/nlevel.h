@@ -316,9 +316,18 @@ __device__ __attribute__((noinline)) double nlevel_team(const MocBalanceArgs& a,
 
 namespace {
 
-// Team sizes of the n-level solves: ions of up to 5 levels one per lane, then
-// teams of 8, 16 and 32 lanes for 6-8, 9-16 and 17-27 levels.
+// Team classes of the n-level solves: ions of up to 5 levels one per lane;
+// larger ones by teams of 8 lanes (6-8 levels), 16 lanes (9-10, 11-13, 14-16)
+// and 32 lanes (17, 18-27).  A class's solve is unrolled to its level bound,
+// so its steps follow the ion's size rather than the team's width.
 constexpr int kNlevLane = 5;
-__host__ __device__ inline int nlevel_team_class(int n) { return n <= kNlevLane ? -1 : (n <= 8 ? 0 : (n <= 16 ? 1 : 2)); }
+__host__ __device__ inline int nlevel_team_class(int n) {
+    if (n <= kNlevLane) return -1;
+    if (n <= 8) return 0;
+    if (n <= 10) return 1;
+    if (n <= 13) return 2;
+    if (n <= 16) return 3;
+    return n <= 17 ? 4 : 5;
+}
 
 }  // namespace
