@@ -413,12 +413,16 @@ __device__ bool ion_balance(const MocBalanceArgs& a, const RT& r, const Lds& L, 
         for (int e = tid; e < NE; e += 64)
             if (a.elemZ[e] >= 3) heavy_chain<R>(a, r, L, e, Te, Ne, H, nIterX, revFrozen);
         __syncthreads();
+        // free electrons: the stages of each element over the lanes, the
+        // per-lane partial sums then reduced over the wave
         double ne = 0.;
         for (int e = 0; e < NE; ++e) {
-            const int z = a.elemZ[e];
-            for (int ion = 2; ion <= imin(z + 1, NS); ++ion)
-                if (L.x(e, ion) >= 1.e-10) ne += (ion - 1) * a.abund[e] * L.x(e, ion);
+            const int top = imin(a.elemZ[e] + 1, NS);
+            const double ab = a.abund[e];
+            for (int ion = 2 + tid; ion <= top; ion += 64)
+                if (L.x(e, ion) >= 1.e-10) ne += (ion - 1) * ab * L.x(e, ion);
         }
+        ne = wave_sum(ne);
         ne *= H;
         Ne = ne == 0. ? 1. : ne;
         const double dHI = (L.x(eH, 1) - HIOld) / HIOld;
