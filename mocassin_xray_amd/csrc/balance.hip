@@ -107,20 +107,27 @@ __device__ __forceinline__ double t3(const double* a, int i, int z, int n) {
 // Each row packs the elements one after the other, element e holding stages
 // 0 .. min(Z+1, NS) from off[e].  Loops that spread (element, stage) over the
 // lanes run over k = e * S2 + stage (S2 = NS + 2) and skip stages past the end.
-struct Lds {
-    double *X, *alpha, *cI1, *out, *cool, *diO;
+// The recombination and collisional-ionisation rows are kept in the fit
+// precision S (FP32 in the mixed form, where the fits are FP32 anyway), which
+// makes a cell's LDS smaller and lets more cells run per CU.
+template <class S>
+struct LdsT {
+    double* X;
+    S *alpha, *cI1;
+    double *out, *cool, *diO;
     int* hot;
     int* off;  // [NE + 1]
     int S2;
     __device__ int len(int e) const { return off[e + 1] - off[e]; }
     __device__ double& x(int e, int i) const { return X[off[e] + i]; }
-    __device__ double& al(int e, int i) const { return alpha[off[e] + i]; }
-    __device__ double& c1(int e, int i) const { return cI1[off[e] + i]; }
+    __device__ S& al(int e, int i) const { return alpha[off[e] + i]; }
+    __device__ S& c1(int e, int i) const { return cI1[off[e] + i]; }
 };
 
-__host__ __device__ inline size_t balance_lds_bytes(int nX, int nNlv) {
-    const size_t rows = (size_t)4 * nX;
-    return (rows + (size_t)(nNlv > 0 ? nNlv : 1) + MOC_MAXS + 1) * sizeof(double) +
+// LDS of a cell's state rows; fitBytes: the size of one fit value (4 or 8)
+__host__ __device__ inline size_t balance_lds_bytes(int nX, int nNlv, size_t fitBytes) {
+    const size_t rows = (size_t)2 * nX * sizeof(double) + (size_t)2 * nX * fitBytes;
+    return rows + ((size_t)(nNlv > 0 ? nNlv : 1) + MOC_MAXS + 1) * sizeof(double) +
            (MOC_MAXE + 2) * sizeof(int);
 }
 
@@ -161,9 +168,9 @@ __device__ R direc(const MocBalanceArgs& a, int z, int ion, R T) {
 }
 
 // recombination coefficients of every ion (calcAlpha :2298-2413), one
-// (element, stage) per lane; the fits in precision R, stored in FP64
+// (element, stage) per lane; the fits in precision R, stored in R
 template <class R>
-__device__ void recomb(const MocBalanceArgs& a, const Lds& L, double Td, int tid) {
+__device__ void recomb(const MocBalanceArgs& a, const LdsT<R>& L, double Td, int tid) {
     const int NE = a.NE, NS = a.NS, S2 = L.S2;
     const int eO = a.elemSlot[8];
     const R T = (R)Td;
@@ -184,9 +191,9 @@ __device__ void recomb(const MocBalanceArgs& a, const Lds& L, double Td, int tid
                 const R E = (R)t3(a.drb_e, q, z, nel);
                 if (E / T < (R)87.) dr += (R)t3(a.drb_c, q, z, nel) * exp(-E / T);
             }
-            L.al(e, ion) = (double)((R)t3(a.rrb, 0, z, nel) /
-                                        (s0 * pow((R)1. + s0, (R)1. - bb) * pow((R)1. + s1, (R)1. + bb)) +
-                                    dr * pow(T, (R)-1.5));
+            L.al(e, ion) = (R)t3(a.rrb, 0, z, nel) /
+                               (s0 * pow((R)1. + s0, (R)1. - bb) * pow((R)1. + s1, (R)1. + bb)) +
+                           dr * pow(T, (R)-1.5);
             continue;
         }
         // radRecFit + dielectronic, oxygen's dielectronic borrowed when zero (:2334-2362)
@@ -211,7 +218,7 @@ __device__ void recomb(const MocBalanceArgs& a, const Lds& L, double Td, int tid
         }
         double di = (double)direc<R>(a, z, ion, T);
         if (di == 0. && ion < 7 && eO >= 0) di = L.diO[ion];
-        L.al(e, ion) = (double)rr + (di > 0. ? di : 0.);
+        L.al(e, ion) = (R)((double)rr + (di > 0. ? di : 0.));
     }
     __syncthreads();
 }
@@ -220,7 +227,7 @@ __device__ void recomb(const MocBalanceArgs& a, const Lds& L, double Td, int tid
 // the reference leaves the loops at the first u > 80, so that entry and every
 // later one (in its loop order) stay zero
 template <class R>
-__device__ void collion(const MocBalanceArgs& a, const Lds& L, double Te, int tid) {
+__device__ void collion(const MocBalanceArgs& a, const LdsT<R>& L, double Te, int tid) {
     const int NE = a.NE, NS = a.NS, S2 = L.S2;
     const R te = (R)Te * (R)8.617385e-05;
     for (int k = tid; k < a.nX; k += 64) L.cI1[k] = 0.0;
@@ -238,7 +245,7 @@ __device__ void collion(const MocBalanceArgs& a, const Lds& L, double Te, int ti
         }
         const R c = (R)t3(a.cion_cf, 2, z, el) * ((R)1.0 + (R)t3(a.cion_cf, 1, z, el) * sqrt(u)) /
                     ((R)t3(a.cion_cf, 3, z, el) + u) * pow(u, (R)t3(a.cion_cf, 4, z, el)) * exp(-u);
-        L.c1(e, ion) = (double)c;
+        L.c1(e, ion) = c;
     }
     __syncthreads();
     const int hot = *L.hot;
@@ -293,7 +300,7 @@ struct Rates {
 };
 
 template <class R>
-__device__ double rev_chex(const MocBalanceArgs& a, const Lds& L, double Te, double H, int z, int ion) {
+__device__ double rev_chex(const MocBalanceArgs& a, const LdsT<R>& L, double Te, double H, int z, int ion) {
     if (!(dEk(z, ion) > 1.)) return 0.0;
     const int nel = z - ion + 1;
     const double g0g1 = (double)a.statW[z * 32 + nel] / (double)a.statW1[z * 32 + nel];
@@ -310,7 +317,7 @@ __device__ double gamma_out(const RT& r, int spec) {
 
 // H or He ratio chain (:1715-1790) on the lane that owns element zz
 template <class R, class RT>
-__device__ void hhe_chain(const MocBalanceArgs& a, const RT& r, const Lds& L, int zz, double Te,
+__device__ void hhe_chain(const MocBalanceArgs& a, const RT& r, const LdsT<R>& L, int zz, double Te,
                           double Ne, double H, double H0, double collIonH) {
     const int NS = a.NS, e = a.elemSlot[zz];
     double ratio[3] = {0., 0., 0.};
@@ -336,7 +343,7 @@ __device__ void hhe_chain(const MocBalanceArgs& a, const RT& r, const Lds& L, in
 
 // bidiagonal chain of one heavy element with Auger spreading (:1826-1929)
 template <class R, class RT>
-__device__ void heavy_chain(const MocBalanceArgs& a, const RT& r, const Lds& L, int e, double Te,
+__device__ void heavy_chain(const MocBalanceArgs& a, const RT& r, const LdsT<R>& L, int e, double Te,
                             double Ne, double H, int nIterX, const double revFrozen[2]) {
     const int NS = a.NS, z = a.elemZ[e];
     double* out = L.out + L.off[e];  // rates out of each stage, then the chain products
@@ -391,7 +398,7 @@ __device__ void heavy_chain(const MocBalanceArgs& a, const RT& r, const Lds& L, 
 
 // ionBalance (:1575-1992); returns false when 15 passes did not settle H/He
 template <class R, class RT>
-__device__ bool ion_balance(const MocBalanceArgs& a, const RT& r, const Lds& L, double Te,
+__device__ bool ion_balance(const MocBalanceArgs& a, const RT& r, const LdsT<R>& L, double Te,
                             double& Ne, double H, const double revFrozen[2], int tid) {
     const int NS = a.NS, NE = a.NE;
     const int eH = a.elemSlot[1], eHe = a.elemSlot[2];
@@ -499,7 +506,8 @@ __device__ double g_emission(const MocBalanceArgs& a, const GrainLds& G, int g, 
 
 // getGrainRecombination (:609-682): electrons less the ions of the state at
 // the start of the update
-__device__ double g_recombination(const MocBalanceArgs& a, const Lds& L, const GrainLds& G,
+template <class LT>
+__device__ double g_recombination(const MocBalanceArgs& a, const LT& L, const GrainLds& G,
                                   double T, double Ne, double H, double Vg) {
     const double kT = 6.336e-6 * T, e8 = (1.1045e-15) * T / kPiB;
     double vmean = sqrt(e8 / kMe), eta = -Vg / kT;
@@ -522,7 +530,8 @@ __device__ double g_recombination(const MocBalanceArgs& a, const Lds& L, const G
 // setGrainPotential (:484-553): secant on emission = recombination from 0
 // and 0.05 Ryd, steps capped at 5 % of the threshold, 100 passes at most;
 // the slope starts at 0 for each grain
-__device__ double g_potential(const MocBalanceArgs& a, const Lds& L, const GrainLds& G, int g,
+template <class LT>
+__device__ double g_potential(const MocBalanceArgs& a, const LT& L, const GrainLds& G, int g,
                               int s, int ai, double T, double Ne, double H) {
     double Vg = 0., VgOld = 0., EmiOld = 0., RecOld = 0., slope = 0.;
     const double Vn = a.gVn[s];
@@ -561,7 +570,8 @@ __device__ double g_potential(const MocBalanceArgs& a, const Lds& L, const Grain
 // its photoelectric cooling and collision heating (into G.pe, G.coll) and
 // its shares of the gas heating and cooling (G.pg, G.cg, before the common
 // factors)
-__device__ void g_terms(const MocBalanceArgs& a, const Lds& L, const GrainLds& G, int cell, int g,
+template <class LT>
+__device__ void g_terms(const MocBalanceArgs& a, const LT& L, const GrainLds& G, int cell, int g,
                         int s, int ai, double T, double Ne, double H) {
     const int B = a.B, nS = a.nGS;
     const double Vn = a.gVn[s], V = G.Vg[g];
@@ -627,7 +637,7 @@ __device__ void g_terms(const MocBalanceArgs& a, const Lds& L, const GrainLds& G
 // P = 64 / W ions at a time, one per team of W lanes; idle teams solve the
 // group's first ion again and store nothing
 template <int NL, class R>
-__device__ void team_cooling(const MocBalanceArgs& a, const Lds& L, int cnt, const int* list, double T,
+__device__ void team_cooling(const MocBalanceArgs& a, const LdsT<R>& L, int cnt, const int* list, double T,
                              double Ne, int tid) {
     constexpr int W = NL <= 8 ? 8 : (NL <= 16 ? 16 : 32), P = 64 / W;
     for (int b0 = 0; b0 < cnt; b0 += P) {
@@ -644,7 +654,7 @@ __device__ void team_cooling(const MocBalanceArgs& a, const Lds& L, int cnt, con
 // The cooling fits and the collision rates of the level solves in precision
 // R, every sum in FP64.
 template <bool TEAMS, class R, class RT>
-__device__ void therm(const MocBalanceArgs& a, const RT& r, const Lds& L, double T, double Ne,
+__device__ void therm(const MocBalanceArgs& a, const RT& r, const LdsT<R>& L, double T, double Ne,
                       double H, double heatef, const double comp[4], double* heat, double* cool,
                       int tid, double grainHeat = 0., double grainCool = 0., bool grains = false) {
     // forbidden-line cooling: ions of up to kNlevLane levels one per lane with
@@ -759,7 +769,7 @@ k_balance(MocBalanceArgs a) {
     Rates<ST> r{a.R, a.ncol, (int)blockIdx.x, a.specOf, a.specShell, a.shInfo, a.shAug};
     if constexpr (ST) {
         // stage the cell's rate columns and the index tables after the state rows
-        double* Rl = lds + balance_lds_bytes(a.nX, a.nNlv) / sizeof(double);
+        double* Rl = lds + balance_lds_bytes(a.nX, a.nNlv, sizeof(R)) / sizeof(double);
         int* iL = reinterpret_cast<int*>(Rl + a.ncol);
         int* sOf = iL;
         int* sSh = sOf + NE * 32;
@@ -784,12 +794,12 @@ k_balance(MocBalanceArgs a) {
         }
         return;
     }
-    Lds L;
+    LdsT<R> L;
     L.S2 = S2;
     L.X = lds;
-    L.alpha = L.X + a.nX;
+    L.alpha = reinterpret_cast<R*>(L.X + a.nX);
     L.cI1 = L.alpha + a.nX;
-    L.out = L.cI1 + a.nX;
+    L.out = reinterpret_cast<double*>(L.cI1 + a.nX);
     L.cool = L.out + a.nX;
     L.diO = L.cool + (a.nNlv > 0 ? a.nNlv : 1);
     L.hot = (int*)(L.diO + MOC_MAXS + 1);
@@ -804,7 +814,7 @@ k_balance(MocBalanceArgs a) {
     const int nG = PE ? a.nGS * a.nGA : 0;
     GrainLds G{};
     if constexpr (PE) {
-        size_t off = balance_lds_bytes(a.nX, a.nNlv);
+        size_t off = balance_lds_bytes(a.nX, a.nNlv, sizeof(R));
         if constexpr (ST) off += balance_stage_bytes(a);
         G.q = lds + off / sizeof(double);
         G.X0 = G.q + a.B;
@@ -950,7 +960,7 @@ int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a0, const double* J, 
                        int ncol, double* R, int chunk) {
     MocBalanceArgs a = a0;
     const int C1 = a.nCells + 1;
-    const size_t base = balance_lds_bytes(a.nX, a.nNlv);
+    const size_t base = balance_lds_bytes(a.nX, a.nNlv, a.fitF32 ? sizeof(float) : sizeof(double));
     const size_t staged = base + balance_stage_bytes(a);
     const bool st = staged <= kMaxBalanceLds;
     const bool pe = a.nGS > 0 && a.nGA > 0;
