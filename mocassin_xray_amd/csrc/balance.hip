@@ -40,18 +40,26 @@ constexpr int RK = 32;  // bins per LDS stage
 constexpr int RN = 32;  // columns per tile
 
 // cells cBase .. cBase + nc - 1 of the J planes (row stride ldJ) into
-// R[local cell][col]
+// R[local cell][col].  Band form (perm != nullptr): W's columns are stored
+// sorted by the bins where they are nonzero, column n of W being column
+// perm[n] of R, and column tile t is contracted over its bins bLo[t] ..
+// bHi[t] only (every column of the tile is zero outside them, so the sums
+// are those of the full contraction).
 __global__ void __launch_bounds__(256) k_rates(const double* __restrict__ J, int ldJ, int cBase,
                                                int nc, int B, const double* __restrict__ W,
-                                               int ncol, double* __restrict__ R) {
+                                               int ncol, double* __restrict__ R,
+                                               const int* __restrict__ perm,
+                                               const int* __restrict__ bLo,
+                                               const int* __restrict__ bHi) {
     __shared__ double sJ[RK][RC];
     __shared__ double sW[RK][RN];
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 4 waves x 8 columns
     const int c0 = blockIdx.x * RC, n0 = blockIdx.y * RN;
+    const int bFirst = perm ? bLo[blockIdx.y] : 0, bEnd = perm ? bHi[blockIdx.y] + 1 : B;
     double acc[RN / 4];
 #pragma unroll
     for (int j = 0; j < RN / 4; ++j) acc[j] = 0.0;
-    for (int b0 = 0; b0 < B; b0 += RK) {
+    for (int b0 = bFirst; b0 < bEnd; b0 += RK) {
         for (int k = threadIdx.x; k < RK * RC; k += 256) {
             const int kb = k / RC, kc = k % RC, b = b0 + kb, c = c0 + kc;
             sJ[kb][kc] = (b < B && c < nc) ? J[(size_t)b * ldJ + cBase + c] : 0.0;
@@ -76,7 +84,7 @@ __global__ void __launch_bounds__(256) k_rates(const double* __restrict__ J, int
     __syncthreads();
     for (int k = threadIdx.x; k < RC * RN; k += 256) {
         const int c = c0 + k / RN, n = n0 + k % RN;
-        if (c < nc && n < ncol) R[(size_t)c * ncol + n] = sOut[k];
+        if (c < nc && n < ncol) R[(size_t)c * ncol + (perm ? perm[n] : n)] = sOut[k];
     }
 }
 
@@ -947,14 +955,18 @@ void launch_balance(moc_ctx* ctx, const MocBalanceArgs& a, bool st, dim3 grid, d
 }  // namespace
 
 void launch_rates(moc_ctx* ctx, const double* J, int ldJ, int c0, int nc, const double* W, int ncol,
-                  double* R) {
+                  double* R, const int* perm = nullptr, const int* bLo = nullptr, const int* bHi = nullptr) {
     dim3 g1((unsigned)((nc + RC - 1) / RC), (unsigned)((ncol + RN - 1) / RN));
-    hipLaunchKernelGGL(k_rates, g1, dim3(256), 0, ctx->stream, J, ldJ, c0, nc, ctx->nbins, W, ncol, R);
+    hipLaunchKernelGGL(k_rates, g1, dim3(256), 0, ctx->stream, J, ldJ, c0, nc, ctx->nbins, W, ncol, R,
+                       perm, bLo, bHi);
 }
 
 void moc_launch_rates(moc_ctx* ctx, const double* J, int C1, const double* W, int ncol, double* R) {
     launch_rates(ctx, J, C1, 0, C1, W, ncol, R);
 }
+
+int moc_rate_tile_cols() { return RN; }
+int moc_rate_tile_bins() { return RK; }
 
 int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a0, const double* J, const double* W,
                        int ncol, double* R, int chunk) {
@@ -974,7 +986,7 @@ int moc_launch_balance(moc_ctx* ctx, const MocBalanceArgs& a0, const double* J, 
     a.R = R;
     for (int c0 = 1; c0 <= a.nCells; c0 += chunk) {
         const int n = a.nCells - c0 + 1 < chunk ? a.nCells - c0 + 1 : chunk;
-        launch_rates(ctx, J, C1, c0, n, W, ncol, R);
+        launch_rates(ctx, J, C1, c0, n, W, ncol, R, a.ratePerm, a.rateLo, a.rateHi);
         a.cell0 = c0;
         a.nLaunch = n;
         const dim3 grid((unsigned)n), block(64);

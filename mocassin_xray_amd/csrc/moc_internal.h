@@ -199,6 +199,12 @@ struct MocBalanceArgs {
     double* collD;            // [nGS*nGA][C1] out: gas-grain collision heating of the grains
     double* peD;              // [nGS*nGA][C1] out: photoelectric cooling of the grains
     const double* R;          // [nLaunch][ncol] rates of cells cell0 .. cell0 + nLaunch - 1
+    // band form of the rate weights: W's columns sorted by their nonzero bins
+    // (column n is rate column ratePerm[n]); column tile t spans bins
+    // rateLo[t] .. rateHi[t]
+    const int* ratePerm;
+    const int* rateLo;
+    const int* rateHi;
     int cell0, nLaunch;       // the cells of one launch (a chunk of the grid)
     double* Te;
     double* Ne;
@@ -485,6 +491,8 @@ int moc_launch_dust_temp(moc_ctx* ctx, const MocDustArgs& a, const double* J, co
                          double* R);
 // R[col][cell] = sum_nu J[nu][cell] W[nu][col] (k_rates, balance.hip)
 void moc_launch_rates(moc_ctx* ctx, const double* J, int C1, const double* W, int ncol, double* R);
+int moc_rate_tile_cols();  // columns per k_rates tile
+int moc_rate_tile_bins();  // bins per k_rates stage
 int moc_launch_volavg(moc_ctx* ctx, const double* ionDen, const double* Te, const double* Ne,
                       const double* H, const double* vol, const int* black, int C1, int nPairs,
                       int nChunks, double* part);

@@ -1124,7 +1124,53 @@ int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* t) {
         a.atomRec = nullptr;
         a.atomRecValid = nullptr;
     }
-    PUT(p_d, t->W, (size_t)B * b.ncol, "W");
+    {
+        // the rate weights in band form: columns sorted by the bins where they
+        // are nonzero (by first bin, last bin or both, whichever leaves the
+        // least work), each column tile contracted over its own bins only
+        const int ncol = b.ncol, TC = moc_rate_tile_cols();
+        std::vector<int> lo((size_t)ncol, B), hi((size_t)ncol, -1);
+        for (int bb = 0; bb < B; ++bb)
+            for (int c = 0; c < ncol; ++c)
+                if (t->W[(size_t)bb * ncol + c] != 0.0) {
+                    if (lo[c] > bb) lo[c] = bb;
+                    hi[c] = bb;
+                }
+        std::vector<int> best, bestLo, bestHi;
+        long long bestWork = -1;
+        for (int key = 0; key < 3; ++key) {
+            std::vector<int> perm((size_t)ncol);
+            for (int c = 0; c < ncol; ++c) perm[c] = c;
+            std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) {
+                if (key == 0) return lo[x] < lo[y];
+                if (key == 1) return hi[x] < hi[y];
+                return lo[x] != lo[y] ? lo[x] < lo[y] : hi[x] < hi[y];
+            });
+            const int nt = (ncol + TC - 1) / TC;
+            std::vector<int> tlo((size_t)nt, B), thi((size_t)nt, -1);
+            long long work = 0;
+            for (int tt = 0; tt < nt; ++tt) {
+                for (int n = tt * TC; n < std::min(ncol, (tt + 1) * TC); ++n) {
+                    tlo[tt] = std::min(tlo[tt], lo[perm[n]]);
+                    thi[tt] = std::max(thi[tt], hi[perm[n]]);
+                }
+                work += std::max(0, thi[tt] + 1 - tlo[tt]);
+            }
+            if (bestWork < 0 || work < bestWork) {
+                bestWork = work;
+                best.swap(perm);
+                bestLo.swap(tlo);
+                bestHi.swap(thi);
+            }
+        }
+        std::vector<double> Wp((size_t)B * ncol);
+        for (int bb = 0; bb < B; ++bb)
+            for (int n = 0; n < ncol; ++n) Wp[(size_t)bb * ncol + n] = t->W[(size_t)bb * ncol + best[n]];
+        PUT(p_d, Wp.data(), Wp.size(), "W");
+        PUT(a.ratePerm, best.data(), best.size(), "rate column order");
+        PUT(a.rateLo, bestLo.data(), bestLo.size(), "rate tile bins");
+        PUT(a.rateHi, bestHi.data(), bestHi.size(), "rate tile bins");
+    }
     b.d_W = nullptr;  // W lives in b.owned
 #undef PUT
     (void)p_i; (void)p_l;
