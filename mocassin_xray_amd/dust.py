@@ -6,12 +6,12 @@ Restates the set-up that MOCASSIN's dust runs do once per model:
   per-grain cross-sections          makeDustXsec  ph_mod.f90:1502-1524
   Kirchhoff emission integrals      dustEmissionInt  dust_mod.f90:146-184
   grain number density from MdMg    setMotherGrid  grid_mod.f90:1690-1727
-The efficiencies Q_abs, Q_sca come from Mie theory for homogeneous spheres,
-evaluated with the Bohren & Huffman (1983, appendix A) recurrences.  The
-reference's getQs switches to Rayleigh-Gans and geometric optics in their
-regimes (Laor & Draine 1993); those switches are not restated, so the
-cross-sections here are an input of the same shape and magnitude, not the
-reference's numbers.  The device path consumes whatever tables it is given.
+The efficiencies Q_abs, Q_sca follow getQs (ph_mod.f90:1604-1667): Mie
+theory for homogeneous spheres (the Bohren & Huffman (1983, appendix A)
+recurrences) while |m| x < 1000, with x capped at 100; the Rayleigh-Gans
+forms where |m| x >= 1000 and |m - 1| x < 0.001; the reference's geometric
+optics otherwise (geometricOptics, :1669-1735).  The device path consumes
+whatever tables it is given.
 """
 from __future__ import annotations
 
@@ -115,6 +115,75 @@ def _mie_q(x: np.ndarray, m: np.ndarray):
     return qext * f, qsca * f
 
 
+PI_GETQS = 3.14159265        # the pi of getQs's size parameter (ph_mod.f90:1628)
+
+
+def geometric_optics(x: float, a: float, mre: float, mim: float, lam: float):
+    """geometricOptics (ph_mod.f90:1669-1735) as the reference computes it:
+    45 incidence angles of 1 degree for Q_abs, then Q_sca from the
+    Rayleigh-Gans Q saturated to 2 minus Q_abs.  Kept from the reference:
+    1/eps's imaginary part taken as -Re(eps)/|eps|^2, and the refraction angle
+    psi of the previous angle in the exit-face terms p21, q21 (psi starts at
+    0; the reference reads it uninitialised at the first angle)."""
+    absmm1sq = (mre - 1.) ** 2 + mim ** 2
+    epsre = mre ** 2 - mim ** 2
+    epsim = mre * mim
+    epsrem1 = epsre / (epsre ** 2 + epsim ** 2)
+    epsimm1 = -(epsre / (epsre ** 2 + epsim ** 2))
+    qa = 0.
+    psi = 0.
+    for i in range(1, 46):
+        th = i * PI_REF / 180.
+        s2 = np.sin(th) ** 2
+        p20 = 0.5 * (s2 - epsre + ((2 * epsim) ** 2 + (epsre - s2)) ** 0.5)
+        q20 = 0.5 * (-s2 + epsre + ((2 * epsim) ** 2 + (epsre - s2)) ** 0.5)
+        sp2 = np.sin(psi) ** 2
+        p21 = 0.5 * (sp2 - epsrem1 + ((2 * epsimm1) ** 2 + (epsrem1 - sp2)) ** 0.5)
+        q21 = 0.5 * (-sp2 + epsrem1 + ((2 * epsimm1) ** 2 + (epsrem1 - sp2)) ** 0.5)
+        rperp0 = ((np.sqrt(q20) - np.cos(th)) ** 2 + p20) / ((np.sqrt(q20) + np.cos(th)) ** 2 + p20)
+        rpara0 = rperp0 * ((np.sqrt(q20) - np.sin(th) * np.tan(th)) ** 2 + p20) / \
+            ((np.sqrt(q20) + np.sin(th) * np.tan(th)) ** 2 + p20)
+        psi = np.arcsin(np.sin(th) / (q20 + s2) ** 2)
+        rperp1 = ((np.sqrt(q21) - np.cos(psi)) ** 2 + p21) / ((np.sqrt(q21) + np.cos(psi)) ** 2 + p21)
+        rpara1 = rperp1 * ((np.sqrt(q21) - np.sin(psi) * np.tan(psi)) ** 2 + p21) / \
+            ((np.sqrt(q21) + np.sin(psi) * np.tan(psi)) ** 2 + p21)
+        at = np.exp(-8. * PI_REF * a * mim * np.cos(psi) / lam)
+        qa += np.sin(th) * np.cos(th) * (PI_REF / 180.) * (1 - at) * \
+            ((1. - rperp0) / (1. - at * rperp1) + (1. - rpara0) / (1. - at * rpara1))
+    qrg = (32. * absmm1sq * x ** 4) / (27. + 16 * x ** 2) + (8. / 3.) * mim * x
+    qs = qrg / np.sqrt(1. + 0.25 * qrg ** 2) - qa
+    return qa, qs
+
+
+def get_qs(n_re: np.ndarray, n_im: np.ndarray, a: float, lam_um: np.ndarray,
+           scattering: bool = True):
+    """Q_abs, Q_sca of grains of radius a [um] over the bins (getQs,
+    ph_mod.f90:1604-1667): the regime of each bin from |m| x and |m - 1| x."""
+    n_re = np.asarray(n_re, dtype=np.float64)
+    n_im = np.asarray(n_im, dtype=np.float64)
+    x = 2.0 * PI_GETQS * a / np.asarray(lam_um, dtype=np.float64)
+    absm = np.sqrt(n_re ** 2 + n_im ** 2)
+    absm1 = np.sqrt((n_re - 1.) ** 2 + n_im ** 2)
+    qabs = np.zeros_like(x)
+    qsca = np.zeros_like(x)
+    mie = absm * x < 1000.
+    rg = ~mie & (absm1 * x < 0.001)
+    geo = ~mie & ~rg
+    if mie.any():
+        xm = np.minimum(x[mie], 100.)              # geometrical-optics cap of the series
+        qe, qs = mie_q(xm, n_re[mie] + 1j * n_im[mie])
+        qabs[mie] = qe - qs
+        qsca[mie] = qs
+    if rg.any():
+        qabs[rg] = (8. / 3.) * n_im[rg] * x[rg]
+        qsca[rg] = 32. * absm1[rg] ** 2 * x[rg] ** 4 / (27. + 16. * x[rg] ** 2)
+    for i in np.flatnonzero(geo):
+        qabs[i], qsca[i] = geometric_optics(x[i], a, n_re[i], n_im[i], lam_um[i])
+    if not scattering:
+        qsca[:] = 0.
+    return qabs, qsca
+
+
 def size_weights(radius: np.ndarray, weight: np.ndarray) -> np.ndarray:
     """grainWeight (ph_mod.f90:1056-1081): weight x da, normalised; 1 for one size."""
     n = radius.size
@@ -170,7 +239,7 @@ def emission_integrals(Cabs: np.ndarray, nu: np.ndarray, widflx: np.ndarray,
 
 
 def build_dust(nu: np.ndarray, widflx: np.ndarray, species: list, abundances,
-               radius: np.ndarray, weights: np.ndarray) -> DustTables:
+               radius: np.ndarray, weights: np.ndarray, scattering: bool = True) -> DustTables:
     """Grain tables of a species mixture and size distribution on the mesh."""
     radius = np.asarray(radius, dtype=np.float64)
     gw = size_weights(radius, np.asarray(weights, dtype=np.float64))
@@ -186,12 +255,10 @@ def build_dust(nu: np.ndarray, widflx: np.ndarray, species: list, abundances,
         n_re, n_im = refractive_index_on_mesh(name, nu)
         hdr = optical_constants(name)[3]
         Tsub[s], rho[s], vn[s], ms[s] = hdr[0], hdr[1], hdr[2], hdr[3]
-        m = n_re + 1j * n_im
         for a_i, a in enumerate(radius):
-            x = 2.0 * PI_REF * a / lam_um
-            qe, qs = mie_q(x, m)
+            qa, qs = get_qs(n_re, n_im, a, lam_um, scattering)
             geo = PI_REF * a * a * 1.e-8
-            Cabs[s, a_i] = (qe - qs) * geo
+            Cabs[s, a_i] = qa * geo
             Csca[s, a_i] = qs * geo
     emi = emission_integrals(Cabs, nu, widflx)
     return DustTables(list(species), np.asarray(abundances, dtype=np.float64), radius, gw,

@@ -542,3 +542,61 @@ def test_gpu_dusty_lognormal_nebula_matches_oracle_chain(oracle):
                                  collD=up["collD"], peD=up["peD"])[0]
     np.testing.assert_allclose(got["Tdust"][1:], To[1:], rtol=1e-10)
     assert st["convPercent"] == pytest.approx(100.0 * up["lgConverged"][1:].mean())
+
+
+def test_getqs_rayleigh_gans_branch_is_the_mie_limit():
+    """Where |m| x >= 1000 and |m - 1| x < 0.001 getQs takes the Rayleigh-Gans
+    forms Q_abs = 8/3 Im(m) x and Q_sca = 32 |m-1|^2 x^4 / (27 + 16 x^2)
+    (ph_mod.f90:1640-1645): the weak-contrast limit that the Mie series itself
+    reaches there (2 x |m - 1| << 1)."""
+    from mocassin_xray_amd import dust as D
+    x = 1000.0
+    m = 1.0 + 1e-7 + 1e-7j
+    lam = 1.0
+    a = x * lam / (2.0 * D.PI_GETQS)
+    qa, qs = D.get_qs(np.array([m.real]), np.array([m.imag]), a, np.array([lam]))
+    assert qa[0] == pytest.approx(8.0 / 3.0 * 1e-7 * x, rel=1e-9)
+    qe_m, qs_m = D.mie_q(np.array([x]), np.array([m]))
+    assert qa[0] == pytest.approx(qe_m[0] - qs_m[0], rel=0.02)
+    assert qs[0] == pytest.approx(qs_m[0], rel=0.05)
+
+
+def test_getqs_mie_branch_caps_the_size_parameter():
+    """In the Mie regime the series runs at min(x, 100)."""
+    from mocassin_xray_amd import dust as D
+    m = 1.7 + 0.05j
+    lam = np.array([1.0, 1.0])
+    a150 = 150.0 / (2.0 * D.PI_GETQS)
+    qa, qs = D.get_qs(np.array([m.real] * 2), np.array([m.imag] * 2), a150, lam)
+    qe, qss = D.mie_q(np.array([100.0]), np.array([m]))
+    assert qa[0] == pytest.approx(qe[0] - qss[0], rel=1e-12)
+    assert qs[0] == pytest.approx(qss[0], rel=1e-12)
+
+
+def test_geometric_optics_saturates_extinction():
+    """Large, strongly absorbing grains (|m| x >= 1000): Q_sca is the
+    saturated Rayleigh-Gans Q minus Q_abs, so Q_abs + Q_sca -> 2 (the
+    extinction paradox), and Q_abs lies in (0, 1)."""
+    from mocassin_xray_amd import dust as D
+    lam = 0.01
+    for mre, mim in ((1.5, 0.3), (2.0, 1.0), (1.2, 0.02)):
+        a = 3000.0 * lam / (2.0 * D.PI_GETQS)
+        qa, qs = D.get_qs(np.array([mre]), np.array([mim]), a, np.array([lam]))
+        assert 0.0 < qa[0] < 1.0
+        assert qa[0] + qs[0] == pytest.approx(2.0, abs=1e-6)
+
+
+def test_xray_grain_tables_switch_regimes():
+    """0.25 um silicate grains on an X-ray mesh: the short wavelengths leave
+    the Mie regime; every cross-section stays finite and positive."""
+    from mocassin_xray_amd import dust as D
+    from mocassin_xray_amd import tables as T
+    nu = np.geomspace(0.01, 700.0, 300)
+    wid = np.gradient(nu)
+    n_re, n_im = D.refractive_index_on_mesh("sil-dlee", nu)
+    lam = D.C_LIGHT / (nu * D.FR1RYD) * 1.e4
+    x = 2.0 * D.PI_GETQS * 0.25 / lam
+    assert np.any(np.sqrt(n_re ** 2 + n_im ** 2) * x >= 1000.0)
+    dt = D.build_dust(nu, wid, ["sil-dlee"], [1.0], np.array([0.01, 0.25]), np.array([1.0, 1.0]))
+    assert np.all(np.isfinite(dt.Cabs)) and np.all(dt.Cabs > 0.0)
+    assert np.all(np.isfinite(dt.Csca)) and np.all(dt.Csca >= 0.0)
