@@ -122,6 +122,13 @@ typedef struct moc_grid_desc {
     int32_t nCells;
     int32_t lgSymmetricXYZ;   /* octant model with mirror walls at the lower faces */
     double R_out;             /* outer radius [cm]; <= 0 disables the test */
+    /* plane-parallel illumination (planeIonization, set_input_mod.f90:256-261):
+     * stellar packets start on the y = 0 plane at a uniform (x, z), moving
+     * along +y (photon_mod.f90:796-859); packets leave through the y faces
+     * and are mirrored at the x and z faces (:3249-3606); the end cells of
+     * every axis are half-width (:2166-2223); 5000 segments per flight
+     * (:1873-1878).  Exclusive with lgSymmetricXYZ; nx, nz >= 3. */
+    int32_t lgPlaneIonization;
 } moc_grid_desc;
 
 /* Counters of one moc_transport call (photon_mod.f90 Qphot, trapped, ...). */

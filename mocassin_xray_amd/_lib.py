@@ -60,6 +60,7 @@ class MocGridDesc(C.Structure):
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
         ("xAxis", _dp), ("yAxis", _dp), ("zAxis", _dp), ("active", _ip),
         ("nCells", C.c_int32), ("lgSymmetricXYZ", C.c_int32), ("R_out", C.c_double),
+        ("lgPlaneIonization", C.c_int32),
     ]
 
 

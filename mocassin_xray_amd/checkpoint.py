@@ -435,7 +435,8 @@ def read_grid(outdir, elems: list) -> Checkpoint:
             dust["Ndust"][c], = RD.read("f")
             for ai in range(p.nSizes + 1):
                 dust["Tdust"][c, ai, :] = RD.read(*("f",) * (p.nSpecies + 1))
-    grid = CartesianGrid(ax, ay, az, act, nCells, bool(p.lgSymmetricXYZ), float(R_out))
+    grid = CartesianGrid(ax, ay, az, act, nCells, bool(p.lgSymmetricXYZ), float(R_out),
+                         lgPlaneIonization=bool(p.lgPlaneIonization))
     sources = read_sources(d / "photoSource.out", (ax[-1], ay[-1], az[-1]))
     return Checkpoint(grid, st, p, sources, dust, motherP)
 
@@ -449,6 +450,7 @@ def checkpoint_of(model, g, params: RunParams | None = None, TStellar=None,
     p.nbins = model.tables.nbins
     p.nstages = model.nstages
     p.lgSymmetricXYZ = bool(model.grid.lgSymmetricXYZ)
+    p.lgPlaneIonization = bool(model.grid.lgPlaneIonization)
     p.lgCompton = bool(model.tables.lgCompton)
     dust = None
     st = {}

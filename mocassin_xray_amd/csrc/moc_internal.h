@@ -11,6 +11,7 @@
 
 #define MOC_RECURSION_LIMIT 100000   // re-emissions per packet (constants_mod.f90:70)
 #define MOC_SAFE_LIMIT 500000        // segments per pathSegment call (photon_mod.f90:1873-1878)
+#define MOC_SAFE_LIMIT_PLANE 5000    // the same, plane-parallel illumination
 
 // Device counters of one transport launch (doubles accumulated as u64).
 enum {
@@ -35,7 +36,7 @@ struct MocGridView {
     const T *ax, *ay, *az, *vx, *vy, *vz;
     const int* active;
     const T* sourceCDF;   // [B] of the grid's star
-    int nx, ny, nz, base, sym;
+    int nx, ny, nz, base, sym, plane;
     int starIdx[3];       // 0-based
     T R_out, deltaE;
     T star[3];
@@ -46,6 +47,7 @@ template <typename T>
 struct MocTransportArgs {
     // grid
     int nx, ny, nz, nCells, sym;
+    int plane;          // plane-parallel illumination (lgPlaneIonization)
     const T* ax;        // [nx] axes
     const T* ay;
     const T* az;
@@ -358,7 +360,7 @@ __host__ __device__ inline double moc_planck_h(double nu, double T) {
 
 struct MocGrid {
     bool attached = false;
-    int nx = 0, ny = 0, nz = 0, nCells = 0, sym = 0;
+    int nx = 0, ny = 0, nz = 0, nCells = 0, sym = 0, plane = 0;
     // a batch of independent grids (nGrids > 1): per-grid sizes, first cell,
     // offsets into the concatenated axes and active maps; grid k is lit by
     // star k

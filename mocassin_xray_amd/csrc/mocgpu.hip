@@ -239,16 +239,17 @@ int ensure_ionden(moc_ctx* ctx) {
 
 // cell widths of the volume element (photon_mod.f90:2169-2223), in 1e15 cm:
 // interior cells span the two half-gaps; the first cell is half-width in a
-// symmetric (octant) grid, the last cell full-width, as the reference has it.
-std::vector<double> widths(const std::vector<double>& a, bool sym) {
+// symmetric (octant) or plane-parallel grid, the last cell in a plane-parallel
+// grid only (full-width otherwise), as the reference has it.
+std::vector<double> widths(const std::vector<double>& a, bool sym, bool plane) {
     const int n = (int)a.size();
     std::vector<double> w(n, 0.0);
     for (int i = 0; i < n; ++i) {
         double d;
         if (n == 1) d = 0.0;
         else if (i > 0 && i < n - 1) d = fabs(a[i + 1] - a[i - 1]) / 2.0;
-        else if (i == 0) d = sym ? fabs(a[1] - a[0]) / 2.0 : fabs(a[1] - a[0]);
-        else d = fabs(a[i] - a[i - 1]);
+        else if (i == 0) d = (sym || plane) ? fabs(a[1] - a[0]) / 2.0 : fabs(a[1] - a[0]);
+        else d = plane ? fabs(a[i] - a[i - 1]) / 2.0 : fabs(a[i] - a[i - 1]);
         w[i] = d / 1.e15;
     }
     return w;
@@ -446,8 +447,19 @@ int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who
             ctx->err = w + ": invalid grid " + std::to_string(k) + " (n>=2 per axis, nCells>=1)";
             return MOC_ERR_ARG;
         }
-        if ((gd->lgSymmetricXYZ ? 1 : 0) != (gds->lgSymmetricXYZ ? 1 : 0)) {
-            ctx->err = w + ": the grids of a batch share lgSymmetricXYZ";
+        if ((gd->lgSymmetricXYZ ? 1 : 0) != (gds->lgSymmetricXYZ ? 1 : 0) ||
+            (gd->lgPlaneIonization ? 1 : 0) != (gds->lgPlaneIonization ? 1 : 0)) {
+            ctx->err = w + ": the grids of a batch share lgSymmetricXYZ and lgPlaneIonization";
+            return MOC_ERR_ARG;
+        }
+        // set_input_mod.f90:581-584 and :622-625 (the reference drops the
+        // symmetry with a warning; here the caller chooses)
+        if (gd->lgPlaneIonization && gd->lgSymmetricXYZ) {
+            ctx->err = w + ": lgPlaneIonization and lgSymmetricXYZ are exclusive";
+            return MOC_ERR_ARG;
+        }
+        if (gd->lgPlaneIonization && (gd->nx < 3 || gd->nz < 3)) {
+            ctx->err = w + ": a plane-parallel grid is at least 3 cells deep in x and z";
             return MOC_ERR_ARG;
         }
     }
@@ -466,6 +478,7 @@ int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who
     g.nx = gds->nx; g.ny = gds->ny; g.nz = gds->nz;
     g.nCells = (int)total;
     g.sym = gds->lgSymmetricXYZ ? 1 : 0;
+    g.plane = gds->lgPlaneIonization ? 1 : 0;
     g.R_out = gds->R_out;
     std::vector<double> vx, vy, vz;
     std::vector<double> vol((size_t)total + 1, 0.0);  // getVolume of every cell
@@ -483,7 +496,8 @@ int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who
         g.ax.insert(g.ax.end(), ax.begin(), ax.end());
         g.ay.insert(g.ay.end(), ay.begin(), ay.end());
         g.az.insert(g.az.end(), az.begin(), az.end());
-        const std::vector<double> wx = widths(ax, g.sym), wy = widths(ay, g.sym), wz = widths(az, g.sym);
+        const std::vector<double> wx = widths(ax, g.sym, g.plane), wy = widths(ay, g.sym, g.plane),
+                                  wz = widths(az, g.sym, g.plane);
         vx.insert(vx.end(), wx.begin(), wx.end());
         vy.insert(vy.end(), wy.begin(), wy.end());
         vz.insert(vz.end(), wz.begin(), wz.end());
@@ -845,7 +859,7 @@ MocTransportArgs<double> transport_args_f64(moc_ctx* ctx, int s, int64_t n, uint
                                             uint32_t ctr_base, double deltaE) {
     MocGrid& g = ctx->grid;
     MocTransportArgs<double> a;
-    a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nCells = g.nCells; a.sym = g.sym;
+    a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nCells = g.nCells; a.sym = g.sym; a.plane = g.plane;
     a.ax = g.d_ax; a.ay = g.d_ay; a.az = g.d_az; a.vx = g.d_vx; a.vy = g.d_vy; a.vz = g.d_vz;
     a.active = g.d_active; a.dense = g.dense ? 1 : 0; a.R_out = g.R_out;
     a.nbins = ctx->nbins; a.nAngles = ctx->nAngles; a.lgCompton = ctx->lgCompton;
@@ -867,7 +881,7 @@ MocTransportArgs<float> transport_args_f32(moc_ctx* ctx, int s, int64_t n, uint6
                                            uint32_t ctr_base, double deltaE) {
     MocGrid& g = ctx->grid;
     MocTransportArgs<float> a;
-    a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nCells = g.nCells; a.sym = g.sym;
+    a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nCells = g.nCells; a.sym = g.sym; a.plane = g.plane;
     a.ax = g.f_ax; a.ay = g.f_ay; a.az = g.f_az; a.vx = g.f_vx; a.vy = g.f_vy; a.vz = g.f_vz;
     a.active = g.d_active; a.dense = g.dense ? 1 : 0; a.R_out = (float)g.R_out;
     a.nbins = ctx->nbins; a.nAngles = ctx->nAngles; a.lgCompton = ctx->lgCompton;
@@ -920,6 +934,7 @@ int upload_batch_views(moc_ctx* ctx, int iStar, int64_t* n) {
             a.nx = b.nx = g.gnx[k]; a.ny = b.ny = g.gny[k]; a.nz = b.nz = g.gnz[k];
             a.base = b.base = g.gbase[k];
             a.sym = b.sym = g.sym;
+            a.plane = b.plane = g.plane;
             for (int d = 0; d < 3; ++d) {
                 a.starIdx[d] = b.starIdx[d] = ctx->starIdx[3 * k + d] - 1;
                 a.star[d] = ctx->starPos[3 * k + d];

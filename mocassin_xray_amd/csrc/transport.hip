@@ -110,11 +110,14 @@ __device__ __forceinline__ void unit_vector(MocRng& rng, T& u, T& v, T& w) {
 
 // distance to the next wall along one axis; walls sit at axis midpoints and
 // the outer faces at the first/last axis point (photon_mod.f90:1990-2094).
-// Returns false when the packet snaps onto the outer face and is dropped.
+// Returns false when the packet snaps onto the upper outer face and is
+// dropped, which a plane-parallel slab does not do on its x and z axes
+// (keep = true there: the face mirrors the packet after the step).
 // One division per axis: the target wall is selected first (the midpoint sum
 // is commutative, so both directions form it as ax[idx] + ax[j]).
 template <typename T>
-__device__ __forceinline__ bool wall_dist(T& pos, T dir, int& idx, const T* ax, int n, T& dS) {
+__device__ __forceinline__ bool wall_dist(T& pos, T dir, int& idx, const T* ax, int n, T& dS,
+                                          bool keep = false) {
     const bool up = dir > T(0);
     if (!up && !(dir < T(0))) {
         dS = ax[n - 1];
@@ -127,7 +130,7 @@ __device__ __forceinline__ bool wall_dist(T& pos, T dir, int& idx, const T* ax, 
     if (tabs(dS) < T(1.e-5)) {
         pos = t;
         if (inner) idx = j;
-        else if (up) return false;
+        else if (up && !keep) return false;
     }
     return true;
 }
@@ -170,6 +173,15 @@ __device__ __forceinline__ void tally_escape(const MocTransportArgs<T>& a, const
     ++c.esc;
 }
 
+// cell index of a birth coordinate on one axis (photon_mod.f90:814-819):
+// locate, then the next index from the midpoint on
+template <typename T>
+__device__ __forceinline__ int plane_index(const T* ax, int n, T x) {
+    int k = locate_floor(ax, n, x);
+    if (k < n - 1 && x >= (ax[k] + ax[k + 1]) / T(2)) ++k;
+    return k;
+}
+
 // newPhotonPacket + initPhotonPacket + the escape/line branches of
 // energyPacketRun, then the start of pathSegment (tau = -ln(1-R)).
 template <typename T>
@@ -182,6 +194,19 @@ __device__ __forceinline__ void emit(const MocTransportArgs<T>& a, const View<T>
         p.nu0 = get_nu2(v.sourceCDF, a.nbins, rng, &bad);
         const T nu = a.nuArray[p.nu0];
         if (nu >= T(1)) c.qphot += (double)v.deltaE / (kRyd2ergQ * (double)nu);
+        if (v.plane) {
+            // plane-parallel illumination (photon_mod.f90:796-859): born on
+            // the y = 0 plane at a uniform (x, z) of the slab's face
+            const int nx = v.nx, nz = v.nz;
+            T r = T(1) - rng.uniform<T>();
+            p.x = v.ax[0] + r * (v.ax[nx - 1] - v.ax[0]);
+            p.ix = plane_index(v.ax, nx, p.x);
+            p.y = T(0);
+            p.iy = 0;
+            r = T(1) - rng.uniform<T>();
+            p.z = v.az[0] + r * (v.az[nz - 1] - v.az[0]);
+            p.iz = plane_index(v.az, nz, p.z);
+        }
     } else {
         const int cell = cell_of(v, p.ix, p.iy, p.iz);
         if (cell <= 0) { ++c.err; p.state = ST_DONE; return; }
@@ -196,7 +221,13 @@ __device__ __forceinline__ void emit(const MocTransportArgs<T>& a, const View<T>
         p.nu0 = get_nu2(a.recPDF + (size_t)cell * (size_t)a.nbins, a.nbins, rng, &bad);
     }
     if (bad) ++c.err;
-    unit_vector(rng, p.u, p.v, p.w);
+    if (v.plane && p.stellar) {  // along +y (:851-854)
+        p.u = T(0);
+        p.v = T(1);
+        p.w = T(0);
+    } else {
+        unit_vector(rng, p.u, p.v, p.w);
+    }
     if (v.sym && p.stellar) {
         p.u = tabs(p.u);
         p.v = tabs(p.v);
@@ -241,7 +272,7 @@ __device__ __forceinline__ void compton(const MocTransportArgs<T>& a, Pk<T>& p, 
 template <typename T>
 __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const View<T>& g, Pk<T>& p,
                                         MocRng& rng, Cnt& c) {
-    if (p.segs >= MOC_SAFE_LIMIT) {
+    if (p.segs >= (g.plane ? MOC_SAFE_LIMIT_PLANE : MOC_SAFE_LIMIT)) {
         ++c.lost;
         p.state = ST_DONE;
         return;
@@ -271,8 +302,9 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const View
     T kap0 = p.kap;
     if (cell0 != p.kcell) kap0 = a.opacity[row + (size_t)cell0];
     T dSx = T(0), dSy = T(0), dSz = T(0);
-    if (!wall_dist(p.x, p.u, p.ix, g.ax, nx, dSx) || !wall_dist(p.y, p.v, p.iy, g.ay, ny, dSy) ||
-        !wall_dist(p.z, p.w, p.iz, g.az, nz, dSz)) {
+    const bool keep = g.plane != 0;
+    if (!wall_dist(p.x, p.u, p.ix, g.ax, nx, dSx, keep) || !wall_dist(p.y, p.v, p.iy, g.ay, ny, dSy) ||
+        !wall_dist(p.z, p.w, p.iz, g.az, nz, dSz, keep)) {
         ++c.lost;
         p.state = ST_DONE;
         return;
@@ -393,7 +425,17 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const View
     else if (dS == dSz && p.w < T(0)) --p.iz;
 
     bool esc = false;
-    if (!g.sym) {
+    if (g.plane) {
+        // plane-parallel slab (:3249-3606): out through the y faces, mirrored
+        // at the x and z faces (the lower z face puts the packet at yAxis(1),
+        // as the reference does)
+        if (p.y < g.ay[0] || p.iy < 0) { p.iy = 0; esc = true; }
+        if (p.y > g.ay[ny - 1] || p.iy > ny - 1) { p.iy = ny - 1; esc = true; }
+        if (p.x <= g.ax[0] || p.ix < 0) { p.ix = 0; p.x = g.ax[0]; p.u = -p.u; }
+        if (p.x >= g.ax[nx - 1] || p.ix > nx - 1) { p.ix = nx - 1; p.x = g.ax[nx - 1]; p.u = -p.u; }
+        if (p.z <= g.az[0] || p.iz < 0) { p.iz = 0; p.z = g.ay[0]; p.w = -p.w; }
+        if (p.z >= g.az[nz - 1] || p.iz > nz - 1) { p.iz = nz - 1; p.z = g.az[nz - 1]; p.w = -p.w; }
+    } else if (!g.sym) {
         if (p.y < g.ay[0] || p.iy < 0) { p.iy = 0; esc = true; }
         if (p.y > g.ay[ny - 1] || p.iy > ny - 1) { p.iy = ny - 1; esc = true; }
         if (p.x <= g.ax[0] || p.ix < 0) { p.ix = 0; esc = true; }
@@ -401,9 +443,10 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const View
         if (p.z <= g.az[0] || p.iz < 0) { p.iz = 0; esc = true; }
         if (p.z >= g.az[nz - 1] || p.iz > nz - 1) { p.iz = nz - 1; esc = true; }
     }
-    if (!esc && ((!g.sym && (p.x <= g.ax[0] || p.y <= g.ay[0] || p.z <= g.az[0])) ||
-                 p.x >= g.ax[nx - 1] || p.y >= g.ay[ny - 1] || p.z >= g.az[nz - 1] ||
-                 p.ix > nx - 1 || p.iy > ny - 1 || p.iz > nz - 1))
+    if (!esc && !g.plane &&
+        ((!g.sym && (p.x <= g.ax[0] || p.y <= g.ay[0] || p.z <= g.az[0])) ||
+         p.x >= g.ax[nx - 1] || p.y >= g.ay[ny - 1] || p.z >= g.az[nz - 1] ||
+         p.ix > nx - 1 || p.iy > ny - 1 || p.iz > nz - 1))
         esc = true;
     if (!esc) {
         if (g.sym) {
@@ -464,7 +507,7 @@ __global__ void __launch_bounds__(256) k_transport(MocTransportArgs<T> a) {
         g.vx = sg + nt; g.vy = sg + nt + nx; g.vz = sg + nt + nx + ny;
         g.active = a.dense ? nullptr : a.active;
         g.sourceCDF = a.sourceCDF;
-        g.nx = nx; g.ny = ny; g.nz = nz; g.base = 0; g.sym = a.sym;
+        g.nx = nx; g.ny = ny; g.nz = nz; g.base = 0; g.sym = a.sym; g.plane = a.plane;
         g.R_out = a.R_out;
         g.deltaE = a.deltaE;
     }

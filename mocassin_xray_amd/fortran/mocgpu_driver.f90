@@ -250,6 +250,7 @@ program mocgpu_driver
   gd%nCells = ival("grid.nCells")
   gd%lgSymmetricXYZ = ival("grid.lgSymmetricXYZ")
   gd%R_out = dval("grid.R_out")
+  gd%lgPlaneIonization = ival("grid.lgPlaneIonization")
   call moc_check(ctx, moc_grid_attach(ctx, 1, gd), "moc_grid_attach")
 
   ! elements and cross-sections

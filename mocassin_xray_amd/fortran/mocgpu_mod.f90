@@ -46,6 +46,7 @@ module mocgpu_mod
      type(c_ptr) :: xAxis, yAxis, zAxis, active
      integer(c_int32_t) :: nCells, lgSymmetricXYZ
      real(c_double) :: R_out
+     integer(c_int32_t) :: lgPlaneIonization
   end type moc_grid_desc
 
   type, bind(C), public :: moc_atomic

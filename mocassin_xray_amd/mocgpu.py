@@ -63,6 +63,7 @@ class CartesianGrid:
     nCells: int
     lgSymmetricXYZ: bool
     R_out: float = 0.0
+    lgPlaneIonization: bool = False
     extra: dict = field(default_factory=dict)
 
 
@@ -115,7 +116,7 @@ def grid_arrays(g: CartesianGrid) -> dict:
     xa, ya, za = _c(g.xAxis), _c(g.yAxis), _c(g.zAxis)
     return {"nx": xa.size, "ny": ya.size, "nz": za.size, "xAxis": xa, "yAxis": ya, "zAxis": za,
             "active": act, "nCells": int(g.nCells), "lgSymmetricXYZ": int(bool(g.lgSymmetricXYZ)),
-            "R_out": float(g.R_out)}
+            "R_out": float(g.R_out), "lgPlaneIonization": int(bool(g.lgPlaneIonization))}
 
 
 def atomic_arrays(elem_z, abund, nstages: int, xsec) -> dict:

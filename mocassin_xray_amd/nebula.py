@@ -51,7 +51,8 @@ def source_tables(nu, T_star, L36, n_packets, sym, nx, ny, nz, lgCompton=True,
 def frozen_fields(grid: CartesianGrid, nu: np.ndarray, hden_cell: np.ndarray, R_s: float,
                   x_in: float = 1.e-3, T_e: float = 1.e4, lines: float = 0.3,
                   kn_sigma: np.ndarray | None = None, rng_seed: int = 7) -> dict:
-    """Per-cell opacity / Ne / recPDF / totalLines of a Stromgren-like sphere.
+    """Per-cell opacity / Ne / recPDF / totalLines of a Stromgren-like sphere
+    (or, on a plane-parallel grid, a slab ionised to the depth R_s along y).
 
     Neutral fraction x_in inside R_s and 1 outside (smoothed over a shell);
     kappa = n_H0 * A0 (nu/nu0)^-3 above the H edge (+ n_e sigma_KN);
@@ -62,7 +63,7 @@ def frozen_fields(grid: CartesianGrid, nu: np.ndarray, hden_cell: np.ndarray, R_
     B = nu.size
     xa, ya, za = grid.xAxis, grid.yAxis, grid.zAxis
     X, Y, Z = np.meshgrid(xa, ya, za, indexing="ij")
-    r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    r = Y if grid.lgPlaneIonization else np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
     on = grid.active > 0
     rcell = np.zeros(C1)
     rcell[grid.active[on]] = r[on]
@@ -123,6 +124,73 @@ def frozen_nebula(n: int = 32, nbins: int = 1000, n_packets: int = 10_000_000,
     f = frozen_fields(grid, nu, hcell, R_s, kn_sigma=tabs.KNsigmaT)
     f["Hden"] = hcell
     return FrozenModel("nebula%d" % n, tabs, grid, f, n_packets)
+
+
+# ------------------------------------------------ plane-parallel (slab) models
+def slab_grid(shape: tuple, side: float, depth: float, hden: float) -> tuple:
+    """A plane-parallel grid: x and z over [0, side], y (the depth, along
+    which the field enters at y = 0) over [0, depth], every cell at hden and
+    active (no radius cut, grid_mod.f90:1258-1266); R_out at the far corner.
+    Returns (grid, hden per cell)."""
+    nx, ny, nz = shape
+    xa = np.linspace(0.0, side, nx)
+    ya = np.linspace(0.0, depth, ny)
+    za = np.linspace(0.0, side, nz)
+    act, ncells, hcell = T.active_map(xa, ya, za, np.full((nx, ny, nz), hden))
+    grid = CartesianGrid(xa, ya, za, act, ncells, False, T.plane_R_out(xa, ya, za),
+                         lgPlaneIonization=True)
+    return grid, hcell
+
+
+def slab_source_tables(nu, T_star, meanField, nu0, nu0Add, grid, n_packets, lgCompton=True,
+                       ion_edge1=H_EDGE, flux=None) -> SourceTables:
+    """Mesh and source tables of a slab lit by a blackbody field of photon
+    flux meanField [phot/cm^2/s] above nu0 (planeIonization meanField nu0
+    nu0Add); the source position is unused (packets are born on the face)."""
+    wid = T.widths_of(nu)
+    f = T.blackbody_flux(nu, T_star) if flux is None else flux
+    L36, _, cdf = T.plane_source(nu, wid, f, meanField, nu0, nu0Add, grid.xAxis, grid.zAxis)
+    kn_t, kn_cdf, pc = T.compton_tables(nu)
+    return SourceTables(nuArray=nu, widFlx=wid, ionEdge1=ion_edge1, KNsigmaT=kn_t, KNcdf=kn_cdf,
+                        Pcomp=pc, sourceCDF=cdf[None, :], Lstar=np.array([L36]),
+                        nPhotons=np.array([float(n_packets)]), starPosition=np.zeros((1, 3)),
+                        starIndex=np.array([[1, 1, 1]], dtype=np.int32), lgCompton=lgCompton)
+
+
+def frozen_slab(shape: tuple = (5, 16, 5), nbins: int = 200, n_packets: int = 100_000,
+                side: float = 2.e18, depth: float = 6.e18, hden: float = 100.,
+                T_star: float = 4.e4, phi: float = 1.e10, y_s: float = 3.8e18,
+                lgCompton: bool = True) -> FrozenModel:
+    """Pure-hydrogen slab: 5 x 16 x 5 cells, 2e18 cm square face, 6e18 cm
+    deep, n_H = 100, 40 kK blackbody field of 1e10 ionising photons/cm^2/s,
+    ionised to ~3.8e18 cm (the Stromgren depth phi / (n_H^2 alpha_B))."""
+    grid, hcell = slab_grid(shape, side, depth, hden)
+    nu = T.log_mesh(nbins, 1.0, 25.0)
+    tabs = slab_source_tables(nu, T_star, phi, 1.0, 0.0, grid, n_packets, lgCompton)
+    f = frozen_fields(grid, nu, hcell, y_s, kn_sigma=tabs.KNsigmaT if lgCompton else None)
+    f["Hden"] = hcell
+    return FrozenModel("slab%dx%dx%d" % shape, tabs, grid, f, n_packets)
+
+
+def gas_slab(shape: tuple = (5, 16, 5), nbins: int = 400, n_packets: int = 1_000_000,
+             side: float = 2.e18, depth: float = 6.e18, hden: float = 100.,
+             elems: list = (1, 2, 8), log_abund: list = (0.0, -1.0, -3.3), nstages: int = 4,
+             T_star: float = 4.e4, phi: float = 1.e10, maxLev: int = 27,
+             lgCompton: bool = True) -> GasModel:
+    """Gas slab lit on its y = 0 face: 5 x 16 x 5 cells, 2e18 cm square face,
+    6e18 cm deep, n_H = 100, H He O, 40 kK blackbody field of 1e10 ionising
+    photons/cm^2/s above 1 Ryd, 400 points over 1-25 Ryd."""
+    from . import atomic, balance, emission
+    grid, hcell = slab_grid(shape, side, depth, hden)
+    nu = T.reference_gas_mesh(nbins, 1.0, 25.0, atomic.ion_edges(list(elems), nstages, 25.0))
+    tabs = slab_source_tables(nu, T_star, phi, 1.0, 0.0, grid, n_packets, lgCompton)
+    xs = atomic.build_xsec(nu, list(elems), nstages)
+    bt = balance.build_balance(xs, maxLev=maxLev)
+    et = emission.build_emission(xs, tabs.widFlx)
+    st = initial_state(list(elems), nstages, hcell)
+    return GasModel("slab%dx%dx%d" % tuple(shape), tabs, grid, list(elems),
+                    10.0 ** np.asarray(log_abund, dtype=np.float64), nstages, xs, bt, et, st,
+                    n_packets)
 
 
 # ------------------------------------------------------------- full gas models

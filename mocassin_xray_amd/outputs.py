@@ -19,6 +19,8 @@ from pathlib import Path
 
 import numpy as np
 
+from . import tables as T
+
 N_ELEMENTS = 30
 
 
@@ -127,10 +129,12 @@ def lineflux_rows(lines: np.ndarray, ions: list, Hbeta: float, bt, nForLevels: i
 
 
 def write_lineflux(outdir, lines: np.ndarray, ions: list, Hbeta: float, bt, symmetric: bool,
-                   nElemOn: int, nComponents: int = 1) -> Path:
+                   nElemOn: int, nComponents: int = 1, planeArea: float | None = None) -> Path:
     """lineFlux.out (:810-1060) with Hbeta [1e36 erg/s] (the sum of
     hydroLines(1,4,2) H dV times 1e-16, and 8 for an octant model, :795-805)
-    and the heavy-ion forbidden lines relative to Hbeta."""
+    and the heavy-ion forbidden lines relative to Hbeta.  A plane-parallel
+    model (planeArea = getPlaneArea [1e36 cm^2]) adds Hbeta per unit area of
+    the lit face [erg/cm^2/s] (:837-840)."""
     hb36 = Hbeta * 1.e-16 * (8. if symmetric else 1.)
     rows = lineflux_rows(lines, ions, Hbeta, bt, nElemOn=nElemOn)
     out = [" Totals integrated over all nebular components: \n"]
@@ -139,7 +143,10 @@ def write_lineflux(outdir, lines: np.ndarray, ions: list, Hbeta: float, bt, symm
         if iAb > 0:
             out.append(_rec("Component: ", iAb))
         out += ["\n", " Line Ratios (Hbeta=1.) :               Analytical\n",
-                _rec("Hbeta [E36 erg/s]:     ", float(hb36)), "\n",
+                _rec("Hbeta [E36 erg/s]:     ", float(hb36))]
+        if planeArea is not None:
+            out.append(_rec("Hbeta [erg/cm^2/s]:     ", float(hb36) / planeArea, float(planeArea)))
+        out += ["\n",
                 " Heavy ions forbidden lines\n", "\n",
                 "  Z    Ion    lowlev  highlev   lambda[A] I/I(Hbeta)  index\n"]
         for z, st, lo, hi, wav, rel, idx in rows:
@@ -155,7 +162,10 @@ def write_output_gas(outdir, g, model) -> dict:
     Path(outdir).mkdir(parents=True, exist_ok=True)
     TeVol, XVol = g.volume_averages()
     lines, ions, hb = g.line_luminosities()
+    gr = model.grid
+    area = T.plane_area(gr.xAxis, gr.zAxis) if gr.lgPlaneIonization else None
     return {"temperature": write_temperature(outdir, TeVol, model.elems, model.nstages),
             "ionratio": write_ionratio(outdir, XVol, model.elems, model.nstages),
             "lineFlux": write_lineflux(outdir, lines, ions, hb, model.bt,
-                                       bool(model.grid.lgSymmetricXYZ), len(model.elems))}
+                                       bool(gr.lgSymmetricXYZ), len(model.elems),
+                                       planeArea=area)}

@@ -104,10 +104,13 @@ def powerlaw_flux(nu: np.ndarray, index: float, nu_cut_lo: float = 0.0,
     return f
 
 
-def source_cdf(flux: np.ndarray, widflx: np.ndarray) -> np.ndarray:
+def source_cdf(flux: np.ndarray, widflx: np.ndarray, start: int = 1) -> np.ndarray:
     """inSpectrumProbDen (setProbDen, continuum_mod.f90:415-504): cumulative
-    flux*width, normalised, entries at the maximum set to exactly 1."""
-    cdf = np.cumsum(flux * widflx)
+    flux*width from the 1-based bin `start` on (0 below it), normalised,
+    entries at the maximum set to exactly 1."""
+    s = max(int(start), 1) - 1
+    cdf = np.zeros(np.asarray(flux).size)
+    cdf[s:] = np.cumsum((flux * widflx)[s:])
     cdf = cdf / cdf[-1]
     cdf[cdf >= cdf.max()] = 1.0
     return cdf
@@ -169,14 +172,16 @@ def active_map(xa, ya, za, hden: np.ndarray, R_in: float = 0.0, R_out: float = 0
     return act, ncells, hcell
 
 
-def cell_volumes(xa, ya, za, active, sym: bool) -> np.ndarray:
-    """dV per active cell [1e45 cm^3] as pathSegment uses it (photon_mod.f90:2169-2223)."""
+def cell_volumes(xa, ya, za, active, sym: bool, plane: bool = False) -> np.ndarray:
+    """dV per active cell [1e45 cm^3] as pathSegment uses it (photon_mod.f90:2169-2223):
+    the first cell of an axis is half-width in an octant or plane-parallel
+    grid, the last one in a plane-parallel grid only."""
     def w(a):
         n = len(a)
         d = np.empty(n)
         d[1:-1] = np.abs(a[2:] - a[:-2]) / 2.0
-        d[0] = np.abs(a[1] - a[0]) / (2.0 if sym else 1.0)
-        d[-1] = np.abs(a[-1] - a[-2])
+        d[0] = np.abs(a[1] - a[0]) / (2.0 if (sym or plane) else 1.0)
+        d[-1] = np.abs(a[-1] - a[-2]) / (2.0 if plane else 1.0)
         return d / 1.e15
     V = w(xa)[:, None, None] * w(ya)[None, :, None] * w(za)[None, None, :]
     ncells = int(active.max())
@@ -184,3 +189,76 @@ def cell_volumes(xa, ya, za, active, sym: bool) -> np.ndarray:
     on = active > 0
     out[active[on]] = V[on]
     return out
+
+
+# ------------------------------------------------ plane-parallel illumination
+def _locate(xa: np.ndarray, x: float) -> int:
+    """locate (interpolation_mod.f90:49-106): 1-based bracket index, 0 below
+    the first point, n at or above the last."""
+    xa = np.asarray(xa)
+    if x > xa[-1]:
+        return xa.size
+    if x < xa[0]:
+        return 0
+    return int(np.searchsorted(xa, x, side="right"))
+
+
+def _locate_mid(xa: np.ndarray, x: float) -> int:
+    """locate, then the next point when x lies above the midpoint
+    (grid_mod.f90:1771-1784)."""
+    p = _locate(xa, x)
+    if 1 <= p < len(xa) and x > (xa[p - 1] + xa[p]) / 2.0:
+        p += 1
+    return p
+
+
+def plane_R_out(xa, ya, za) -> float:
+    """R_out of a plane-parallel model: the distance of the far corner, so the
+    radius test never cuts the slab (grid_mod.f90:852-858)."""
+    return float(1.e10 * np.sqrt((xa[-1] / 1.e10) ** 2 + (ya[-1] / 1.e10) ** 2 +
+                                 (za[-1] / 1.e10) ** 2))
+
+
+def plane_area(xa, za) -> float:
+    """getPlaneArea (grid_mod.f90:2989-3047): the illuminated face [1e36 cm^2],
+    summed over (x, z) cells with half-width end cells."""
+    def w(a):
+        a = np.asarray(a, dtype=np.float64)
+        d = np.empty(a.size)
+        d[1:-1] = np.abs(a[2:] - a[:-2]) / 2.0
+        d[0] = np.abs(a[1] - a[0]) / 2.0
+        d[-1] = np.abs(a[-1] - a[-2]) / 2.0
+        return d / 1.e18
+    return float(np.sum(w(xa)[:, None] * w(za)[None, :]))
+
+
+def plane_source(nu: np.ndarray, widflx: np.ndarray, flux_erg: np.ndarray, meanFieldin: float,
+                 nu0: float, nu0Add: float, xa, za):
+    """The source of a plane-parallel model (planeIonization meanField nu0
+    nu0Add).  Returns (Lstar [1e36 erg/s], meanField [erg/cm^2/s], sourceCDF).
+
+    With nu0 > 0, meanFieldin is the photon flux above nu0 [phot/cm^2/s]: the
+    spectrum is scaled to it and meanField becomes the energy flux from nu0Add
+    on (setMotherGrid, grid_mod.f90:1764-1815); with nu0 = 0 it is the energy
+    flux itself.  Lstar = meanField/1e36 times the face (x_n - x_1)(z_n - z_1)
+    (:1818-1820).  The CDF starts at locate(nu, nu0Add) when nu0Add > 0
+    (setProbDen, continuum_mod.f90:472-482, which runs before setMotherGrid
+    and so sees the unclipped spectrum)."""
+    nu = np.asarray(nu, dtype=np.float64)
+    wid = np.asarray(widflx, dtype=np.float64)
+    erg = np.asarray(flux_erg, dtype=np.float64)
+    nb = nu.size
+    meanField = float(meanFieldin)
+    if nu0 > 0.0:
+        nu0P = _locate_mid(nu, nu0)
+        nu0AddP = _locate_mid(nu, nu0Add) if nu0Add != nu0 else nu0P
+        if nu0P >= nb or nu0P < 1:
+            raise ValueError("plane_source: nu0 outside the frequency mesh (grid_mod.f90:1786-1790)")
+        phot = erg / (nu * HCRYD)
+        scale = meanField / float(np.sum(phot[nu0P - 1:] * wid[nu0P - 1:]))
+        clip = erg.copy()
+        clip[:max(nu0AddP, 1) - 1] = 0.0
+        meanField = float(np.sum(clip * wid)) * scale
+    Lstar = meanField / 1.e36 * (xa[-1] - xa[0]) * (za[-1] - za[0])
+    start = _locate(nu, nu0Add) if nu0Add > 0.0 else 1
+    return float(Lstar), meanField, source_cdf(erg, wid, start)

@@ -10,8 +10,9 @@
  *   pathSegment :1758-4085, comptonScatter :6805-6844;
  *   vector_mod.f90:303-314 (randomUnitVector);
  *   interpolation_mod.f90:49-106 (locate).
- * Scope: one Cartesian grid, no sub-grids, no plane-parallel or 1D mode, no
- * dust, no fluorescence, nAngleBins = 0.  The clock-seeded random_number of
+ * Scope: one Cartesian grid (whole, octant or plane-parallel slab), grains
+ * as scatterers and absorbers, no sub-grids, no 1D mode, no fluorescence,
+ * nAngleBins = 0.  The clock-seeded random_number of
  * the reference (photon_mod.f90:97-114) is replaced by Philox4x32-10 streams
  * keyed by (seed, packet, draw, ctr_base), shared in definition (not in code)
  * with the GPU path, so single packet histories can be compared.
@@ -27,6 +28,7 @@
 
 #define ORC_RECURSION_LIMIT 100000 /* constants_mod.f90:70 */
 #define ORC_SAFE_LIMIT 500000      /* photon_mod.f90:1873-1878 */
+#define ORC_SAFE_LIMIT_PLANE 5000  /* the same lines, plane-parallel illumination */
 #define ORC_PI 3.141592654         /* constants_mod.f90:45 */
 
 /* ---- Philox4x32-10, Salmon et al. SC'11 -------------------------------- */
@@ -188,9 +190,11 @@ static int path_segment(const oracle_transport_in* in, oracle_transport_out* out
     double absTau = 0., passProb, dSx = 0., dSy = 0., dSz = 0., dS, tauCell, dV, dx, dy, dz;
     int i, cell, bad = 0;
 
+    const int safeLimit = in->plane ? ORC_SAFE_LIMIT_PLANE : ORC_SAFE_LIMIT;
+
     passProb = -log(1. - orng_u(g));
 
-    for (i = 1; i <= ORC_SAFE_LIMIT; i++) {
+    for (i = 1; i <= safeLimit; i++) {
         cnt->c[ORC_SEGMENTS] += 1;
         /* 2 % nudge, signed midpoint in the ratio (:1882-1919) */
         if (xP < nx && vx > 0. && fabs(rx - (xa[xP - 1] + xa[xP]) / 2.) / ((xa[xP - 1] + xa[xP]) / 2.) < 2.e-2) xP = xP + 1;
@@ -216,7 +220,10 @@ static int path_segment(const oracle_transport_in* in, oracle_transport_out* out
                 if (fabs(dSx) < 1.e-5) { rx = (xa[xP] + xa[xP - 1]) / 2.; xP = xP + 1; }
             } else {
                 dSx = (xa[nx - 1] - rx) / vx;
-                if (fabs(dSx) < 1.e-5) { rx = xa[nx - 1]; cnt->c[ORC_LOST] += 1; return 0; }
+                if (fabs(dSx) < 1.e-5) {
+                    rx = xa[nx - 1];
+                    if (!in->plane) { cnt->c[ORC_LOST] += 1; return 0; }
+                }
             }
         } else if (vx < 0.) {
             if (xP > 1) {
@@ -254,7 +261,10 @@ static int path_segment(const oracle_transport_in* in, oracle_transport_out* out
                 if (fabs(dSz) < 1.e-5) { rz = (za[zP] + za[zP - 1]) / 2.; zP = zP + 1; }
             } else {
                 dSz = (za[nz - 1] - rz) / vz;
-                if (fabs(dSz) < 1.e-5) { rz = za[nz - 1]; cnt->c[ORC_LOST] += 1; return 0; }
+                if (fabs(dSz) < 1.e-5) {
+                    rz = za[nz - 1];
+                    if (!in->plane) { cnt->c[ORC_LOST] += 1; return 0; }
+                }
             }
         } else if (vz < 0.) {
             if (zP > 1) {
@@ -287,16 +297,20 @@ static int path_segment(const oracle_transport_in* in, oracle_transport_out* out
         cell = ACT(in, xP, yP, zP);
         tauCell = dS * in->opacity[PLANE(in, pk->nuP, cell)];
 
-        /* volume element in 1e45 cm^3 (:2166-2223) */
-        if (xP > 1 && xP < nx) dx = fabs(xa[xP] - xa[xP - 2]) / 2.;
-        else if (xP == 1) dx = in->sym ? fabs(xa[1] - xa[0]) / 2. : fabs(xa[1] - xa[0]);
-        else dx = fabs(xa[xP - 1] - xa[xP - 2]);
-        if (yP > 1 && yP < ny) dy = fabs(ya[yP] - ya[yP - 2]) / 2.;
-        else if (yP == 1) dy = in->sym ? fabs(ya[1] - ya[0]) / 2. : fabs(ya[1] - ya[0]);
-        else dy = fabs(ya[yP - 1] - ya[yP - 2]);
-        if (zP > 1 && zP < nz) dz = fabs(za[zP] - za[zP - 2]) / 2.;
-        else if (zP == 1) dz = in->sym ? fabs(za[1] - za[0]) / 2. : fabs(za[1] - za[0]);
-        else dz = fabs(za[zP - 1] - za[zP - 2]);
+        /* volume element in 1e45 cm^3 (:2166-2223); a plane-parallel grid
+         * halves the first and the last cell of every axis */
+        {
+            const int h0 = in->sym || in->plane, h1 = in->plane;
+            if (xP > 1 && xP < nx) dx = fabs(xa[xP] - xa[xP - 2]) / 2.;
+            else if (xP == 1) dx = h0 ? fabs(xa[1] - xa[0]) / 2. : fabs(xa[1] - xa[0]);
+            else dx = h1 ? fabs(xa[xP - 1] - xa[xP - 2]) / 2. : fabs(xa[xP - 1] - xa[xP - 2]);
+            if (yP > 1 && yP < ny) dy = fabs(ya[yP] - ya[yP - 2]) / 2.;
+            else if (yP == 1) dy = h0 ? fabs(ya[1] - ya[0]) / 2. : fabs(ya[1] - ya[0]);
+            else dy = h1 ? fabs(ya[yP - 1] - ya[yP - 2]) / 2. : fabs(ya[yP - 1] - ya[yP - 2]);
+            if (zP > 1 && zP < nz) dz = fabs(za[zP] - za[zP - 2]) / 2.;
+            else if (zP == 1) dz = h0 ? fabs(za[1] - za[0]) / 2. : fabs(za[1] - za[0]);
+            else dz = h1 ? fabs(za[zP - 1] - za[zP - 2]) / 2. : fabs(za[zP - 1] - za[zP - 2]);
+        }
         dx = dx / 1.e15;
         dy = dy / 1.e15;
         dz = dz / 1.e15;
@@ -370,6 +384,20 @@ static int path_segment(const oracle_transport_in* in, oracle_transport_out* out
         else if (dS == dSz && vz > 0.) zP = zP + 1;
         else if (dS == dSz && vz < 0.) zP = zP - 1;
 
+        if (in->plane) {
+            /* plane-parallel slab (:3249-3606): the packet leaves through the
+             * lower or upper y face and is mirrored at the x and z faces (the
+             * lower z face puts it at yAxis(1), as the reference does) */
+            int ret = 0;
+            if (ry < ya[0] || yP < 1) { yP = 1; ret = 1; }
+            if (ry > ya[ny - 1] || yP > ny) { yP = ny; ret = 1; }
+            if (rx <= xa[0] || xP < 1) { xP = 1; rx = xa[0]; vx = -vx; }
+            if (rx >= xa[nx - 1] || xP > nx) { xP = nx; rx = xa[nx - 1]; vx = -vx; }
+            if (rz <= za[0] || zP < 1) { zP = 1; rz = ya[0]; vz = -vz; }
+            if (rz >= za[nz - 1] || zP > nz) { zP = nz; rz = za[nz - 1]; vz = -vz; }
+            if (ret) { escape_tally(in, out, pk, cnt); return 0; }
+            continue;
+        }
         if (!in->sym) { /* :2884-3244 */
             int ret = 0;
             if (ry < ya[0] || yP < 1) { yP = 1; ret = 1; }
@@ -396,6 +424,28 @@ static int path_segment(const oracle_transport_in* in, oracle_transport_out* out
     return 0;
 }
 
+/* cell index of a birth coordinate on one axis (photon_mod.f90:814-819):
+ * locate, then the next index from the midpoint on, at least 1 */
+static int plane_index(const double* a, int n, double x) {
+    int p = oracle_locate(a, n, x);
+    if (p >= 1 && p < n && x >= (a[p - 1] + a[p]) / 2.) p = p + 1;
+    return p < 1 ? 1 : p;
+}
+
+/* a stellar packet of a plane-parallel model (photon_mod.f90:796-859): born
+ * on the y = 0 plane at a uniform (x, z) of the slab's face, x drawn first */
+static void plane_birth(const oracle_transport_in* in, opacket_t* pk, orng_t* g) {
+    const double *xa = in->xAxis, *za = in->zAxis;
+    double r = 1. - orng_u(g);
+    pk->x = xa[0] + r * (xa[in->nx - 1] - xa[0]);
+    pk->xP = plane_index(xa, in->nx, pk->x);
+    pk->y = 0.;
+    pk->yP = 1;
+    r = 1. - orng_u(g);
+    pk->z = za[0] + r * (za[in->nz - 1] - za[0]);
+    pk->zP = plane_index(za, in->nz, pk->z);
+}
+
 /* newPhotonPacket + initPhotonPacket + energyPacketRun; returns reRun */
 static int energy_packet_run(const oracle_transport_in* in, oracle_transport_out* out,
                              opacket_t* pk, orng_t* g, ocount_t* cnt) {
@@ -406,6 +456,7 @@ static int energy_packet_run(const oracle_transport_in* in, oracle_transport_out
         pk->nuP = getnu2_draw(in->sourceCDF, in->nbins, g, &bad);
         if (in->nuArray[pk->nuP - 1] >= 1.)
             cnt->qphot += in->deltaE / (2.1799153e-11 * in->nuArray[pk->nuP - 1]);
+        if (in->plane) plane_birth(in, pk, g);
     } else {
         int cell = ACT(in, pk->xP, pk->yP, pk->zP);
         double r;
@@ -420,7 +471,13 @@ static int energy_packet_run(const oracle_transport_in* in, oracle_transport_out
         pk->nuP = getnu2_draw(in->recPDF + (size_t)cell * in->nbins, in->nbins, g, &bad);
     }
     if (bad) cnt->c[ORC_ERRORS] += 1;
-    random_unit_vector(g, &pk->u, &pk->v, &pk->w);
+    if (in->plane && pk->lgStellar) {
+        pk->u = 0.; /* along +y (:851-854) */
+        pk->v = 1.;
+        pk->w = 0.;
+    } else {
+        random_unit_vector(g, &pk->u, &pk->v, &pk->w);
+    }
     if (in->sym && pk->lgStellar) {
         pk->u = fabs(pk->u);
         pk->v = fabs(pk->v);

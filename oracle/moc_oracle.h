@@ -51,6 +51,7 @@ typedef struct oracle_transport_in {
     uint32_t ctr_base;
     int nthreads;             /* OpenMP threads (<=1: serial) */
     uint64_t id_base;         /* RNG packet id of packet i: id_base + i (a grid of a batch) */
+    int plane;                /* plane-parallel illumination (lgPlaneIonization) */
 } oracle_transport_in;
 
 /* counters[] slots */

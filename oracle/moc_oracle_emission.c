@@ -600,9 +600,9 @@ int oracle_line_luminosities(const oracle_emission_tables* t, const oracle_emiss
                 NeGrid = s->Ne[cell];
                 Ne = NeGrid; /* outputGas uses the grid's n_e unfloored (output_mod.f90:321) */
                 H = g->Hden[cell];
-                dV = oracle_cell_width(g->xAxis, g->nx, i, g->sym) *
-                     oracle_cell_width(g->yAxis, g->ny, j, g->sym) *
-                     oracle_cell_width(g->zAxis, g->nz, k, g->sym);
+                dV = oracle_cell_width(g->xAxis, g->nx, i, g->sym, g->plane) *
+                     oracle_cell_width(g->yAxis, g->ny, j, g->sym, g->plane) *
+                     oracle_cell_width(g->zAxis, g->nz, k, g->sym, g->plane);
                 (void)rec_lines(t, &c, Te, Ne, NeGrid, log10(NeGrid), &hb, 1);
                 if (hb * H * dV > 1.e-35) *hbeta += hb * H * dV;
                 for (z = 3; z <= 30; z++) {

@@ -20,11 +20,11 @@
 
 #include "moc_oracle_output.h"
 
-double oracle_cell_width(const double* a, int n, int i, int sym) {
+double oracle_cell_width(const double* a, int n, int i, int sym, int plane) {
     double d;
     if (i > 0 && i < n - 1) d = fabs(a[i + 1] - a[i - 1]) / 2.;
-    else if (i == 0) d = sym ? fabs(a[1] - a[0]) / 2. : fabs(a[1] - a[0]);
-    else d = fabs(a[i] - a[i - 1]);
+    else if (i == 0) d = (sym || plane) ? fabs(a[1] - a[0]) / 2. : fabs(a[1] - a[0]);
+    else d = plane ? fabs(a[i] - a[i - 1]) / 2. : fabs(a[i] - a[i - 1]);
     return d / 1.e15;
 }
 
@@ -42,9 +42,9 @@ int oracle_volume_averages(const oracle_output_in* in, double* TeVol, double* io
                 double dV, f1, f2;
                 if (c <= 0) continue;
                 if (in->lgBlack && in->lgBlack[c] >= 1) continue;
-                dV = oracle_cell_width(in->xAxis, in->nx, i, in->sym) *
-                     oracle_cell_width(in->yAxis, in->ny, j, in->sym) *
-                     oracle_cell_width(in->zAxis, in->nz, k, in->sym);
+                dV = oracle_cell_width(in->xAxis, in->nx, i, in->sym, in->plane) *
+                     oracle_cell_width(in->yAxis, in->ny, j, in->sym, in->plane) *
+                     oracle_cell_width(in->zAxis, in->nz, k, in->sym, in->plane);
                 f1 = in->Te[c] * in->Ne[c] * in->Hden[c] * dV;
                 f2 = in->Ne[c] * in->Hden[c] * dV;
                 for (e = 0; e < NE; e++)

@@ -17,12 +17,15 @@ typedef struct oracle_output_in {
     const double* Hden;                  /* [C+1] */
     const double* ionDen;                /* [NS][NE][C+1] */
     const int32_t* lgBlack;              /* [C+1] or NULL */
+    int plane;                           /* plane-parallel grid: both end cells half-width */
 } oracle_output_in;
 
 /* TeVol, ionDenVol: [NS][NE], element fastest */
 int oracle_volume_averages(const oracle_output_in* in, double* TeVol, double* ionDenVol);
-/* width of cell i along one axis / 1e15 (getVolume, grid_mod.f90:2926-2985) */
-double oracle_cell_width(const double* a, int n, int i, int sym);
+/* width of cell i along one axis / 1e15 (getVolume, grid_mod.f90:2926-2985):
+ * the first cell is half-width in an octant or plane-parallel grid, the last
+ * one in a plane-parallel grid only */
+double oracle_cell_width(const double* a, int n, int i, int sym, int plane);
 /* sed: [nbins]; escaped: [nbins][C+1]; returns the total */
 double oracle_sed(const double* escaped, int nCells, int nbins, int sym, double* sed);
 

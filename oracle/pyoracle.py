@@ -28,7 +28,7 @@ class TransportIn(C.Structure):
         ("scaOpac", _dp), ("lgDust", C.c_int), ("dustOnly", C.c_int),
         ("star", C.c_double * 3), ("starIdx", C.c_int32 * 3), ("deltaE", C.c_double),
         ("n", C.c_int64), ("seed", C.c_uint64), ("ctr_base", C.c_uint32), ("nthreads", C.c_int),
-        ("id_base", C.c_uint64),
+        ("id_base", C.c_uint64), ("plane", C.c_int),
     ]
 
 
@@ -112,7 +112,7 @@ def transport(model_tables, grid, fields, n, seed, ctr_base=0, nthreads=1, istar
         scaOpac=d("sca", np.asarray(fields["scaOpac"]).T) if "scaOpac" in fields else None,
         lgDust=int("scaOpac" in fields), dustOnly=int(bool(fields.get("dustOnly", False))),
         deltaE=deltaE, n=int(n), seed=int(seed), ctr_base=int(ctr_base), nthreads=int(nthreads),
-        id_base=int(id_base))
+        id_base=int(id_base), plane=int(bool(getattr(grid, "lgPlaneIonization", False))))
     pos = np.asarray(t.starPosition).reshape(-1, 3)[istar]
     idx = np.asarray(t.starIndex).reshape(-1, 3)[istar]
     for k in range(3):
@@ -602,7 +602,7 @@ class OutputIn(C.Structure):
         ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nCells", C.c_int), ("sym", C.c_int),
         ("xAxis", _dp), ("yAxis", _dp), ("zAxis", _dp), ("active", _ip), ("NE", C.c_int),
         ("NS", C.c_int), ("elemZ", _ip), ("Te", _dp), ("Ne", _dp), ("Hden", _dp), ("ionDen", _dp),
-        ("lgBlack", _ip),
+        ("lgBlack", _ip), ("plane", C.c_int),
     ]
 
 
@@ -622,7 +622,8 @@ def output_in(grid, elem_z, Te, Ne, Hden, ionDen, lgBlack=None):
                    active=d("a", np.asarray(grid.active).reshape(-1, order="F"), np.int32, _ip),
                    NE=NE, NS=NS, elemZ=d("ez", elem_z, np.int32, _ip), Te=d("te", Te),
                    Ne=d("ne", Ne), Hden=d("h", Hden), ionDen=d("X", X.transpose(2, 1, 0)),
-                   lgBlack=None if lgBlack is None else d("bl", lgBlack, np.int32, _ip))
+                   lgBlack=None if lgBlack is None else d("bl", lgBlack, np.int32, _ip),
+                   plane=int(bool(getattr(grid, "lgPlaneIonization", False))))
     return inp, keep
 
 
