@@ -151,28 +151,22 @@ constexpr size_t kMaxGrainLds = 65536;
 
 // dielectronic recombination of the ions with more than 11 electrons
 // (Aldrovandi & Pequignot above 6e4 K, Nussbaumer & Storey below;
-// dielectronic :3204-3305): the last matching row wins
+// dielectronic :3204-3305): the last matching row of the data wins, which the
+// host finds once per (Z, stage) and temperature side of 2e4 K
 template <class R>
 __device__ R direc(const MocBalanceArgs& a, int z, int ion, R T) {
     const R tt = T > (R)1000. ? T / (R)10000. : (R)0.1;
-    R v = 0;
     if (T > (R)60000.) {
-        for (int i = 0; i < a.nAldro; ++i) {
-            const double* r = a.aldro + 6 * i;
-            if ((int)r[0] == z && (int)r[0] - (int)r[1] + 1 == ion)
-                v = (R)r[2] * pow(T, (R)-1.5) * exp(-(R)r[4] / T) * ((R)1. + (R)r[3] * exp(-(R)r[5] / T));
-        }
-        return v;
+        const int i = a.aldroIdx[z * 32 + ion];
+        if (i < 0) return 0;
+        const double* r = a.aldro + 6 * i;
+        return (R)r[2] * pow(T, (R)-1.5) * exp(-(R)r[4] / T) * ((R)1. + (R)r[3] * exp(-(R)r[5] / T));
     }
-    for (int i = 0; i < a.nDiel; ++i) {
-        const double* r = a.diel + 8 * i;
-        const int g = (int)r[7];
-        if ((int)r[0] != z || (int)r[0] + 1 - (int)r[1] != ion) continue;
-        if (g == 0 || (g == 1 && T < (R)20000.) || (g == 2 && T >= (R)20000.))
-            v = (R)1.e-12 * ((R)r[2] / tt + (R)r[3] + (R)r[4] * tt + (R)r[5] * tt * tt) * pow(tt, (R)-1.5) *
-                exp(-(R)r[6] / tt);
-    }
-    return v;
+    const int i = a.dielIdx[(z * 32 + ion) * 2 + (T < (R)20000. ? 0 : 1)];
+    if (i < 0) return 0;
+    const double* r = a.diel + 8 * i;
+    return (R)1.e-12 * ((R)r[2] / tt + (R)r[3] + (R)r[4] * tt + (R)r[5] * tt * tt) * pow(tt, (R)-1.5) *
+           exp(-(R)r[6] / tt);
 }
 
 // recombination coefficients of every ion (calcAlpha :2298-2413), one
@@ -349,48 +343,93 @@ __device__ void hhe_chain(const MocBalanceArgs& a, const RT& r, const LdsT<R>& L
     }
 }
 
-// bidiagonal chain of one heavy element with Auger spreading (:1826-1929)
+// The heavy-element chains (ionBalance :1826-1929) in three lane-parallel
+// steps.  (1) The rates out of each stage: task t of the host's gather lists
+// sums, on one lane, the terms the sequential walk over (stage, shell, Auger
+// electrons) adds to out(stage) -- collisional ionisation, the shells'
+// photoionisation (the N I / O I outer shell with the frozen reverse charge
+// exchange) and the Auger terms of lower stages at the ratio X(ion)/X(stage)
+// -- in the walk's order, so that each sum is the sequential one.
 template <class R, class RT>
-__device__ void heavy_chain(const MocBalanceArgs& a, const RT& r, const LdsT<R>& L, int e, double Te,
-                            double Ne, double H, int nIterX, const double revFrozen[2]) {
-    const int NS = a.NS, z = a.elemZ[e];
-    double* out = L.out + L.off[e];  // rates out of each stage, then the chain products
-    for (int i = 0; i < L.len(e); ++i) out[i] = 0.;
-    const int top = imin(NS, z + 1);
-    for (int ion = 1; ion <= imin(z, NS - 1); ++ion) {
-        out[ion] += L.c1(e, ion) * Ne;
-        const int spec = r.specOf[e * 32 + ion];
-        if (spec < 0) continue;
-        for (int sh = r.specShell[spec]; sh < r.specShell[spec + 1]; ++sh) {
-            // the outer shell of N I / O I takes the reverse charge exchange
-            const int info = r.shInfo[sh], cx = info >> 8, na = info & 0xff;
-            double g = r.gamma(sh);
-            if (cx) g += revFrozen[cx - 1];
-            out[ion] += g;
-            const double* aug = a.auger + r.shAug[sh];
-            for (int ne = 2; ne <= na; ++ne) {
-                const int mx = imin(ion + ne - 1, imin(NS - 1, z));
-                const double ratio = L.x(e, mx) > 1e-30 ? L.x(e, ion) / L.x(e, mx) : 1.;
-                out[mx] += g * aug[ne] * ratio;
+__device__ void heavy_out(const MocBalanceArgs& a, const RT& r, const LdsT<R>& L, double Ne,
+                          const double revFrozen[2], int tid) {
+    constexpr int U = 4;  // terms whose loads are issued together
+    for (int t = tid; t < a.nHTask; t += 64) {
+        const int e = a.htSlot[t], mx = a.htMx[t];
+        const int q1 = a.htBeg[t + 1];
+        const double xm = L.x(e, mx), c1Ne = L.c1(e, mx) * Ne;
+        double o = 0.;
+        for (int q0 = a.htBeg[t]; q0 < q1; q0 += U) {
+            int code[U];
+            double y[U], g[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                code[u] = q0 + u < q1 ? a.htCode[q0 + u] : 3;  // kind 3: past the task's end
+                y[u] = q0 + u < q1 ? a.htAug[q0 + u] : 0.;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int sh = code[u] >> 9;
+                g[u] = (code[u] & 3) == 1 || (code[u] & 3) == 2 ? r.gamma(sh) : 0.;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int kind = code[u] & 3, cx = (code[u] >> 2) & 3, ion = (code[u] >> 4) & 31;
+                if (kind == 3) break;
+                if (kind == 0) {
+                    o += c1Ne;
+                    continue;
+                }
+                double gg = g[u];
+                if (cx) gg += revFrozen[cx - 1];
+                if (kind == 1) {
+                    o += gg;
+                } else {
+                    const double ratio = xm > 1e-30 ? L.x(e, ion) / xm : 1.;
+                    o += gg * y[u] * ratio;
+                }
             }
         }
+        L.out[L.off[e] + mx] = o;
     }
-    // mat(ion) = mat(ion-1) out(ion-1) / in(ion-1) with mat(1) = 1 and
-    // in = Ne alpha + chex H0 H; mat(ion-1) replaces out(ion-1) once used
-    const int eH = a.elemSlot[1];
+}
+
+// (2) The chain products of element e on one lane: mat(ion) = mat(ion-1)
+// out(ion-1) / in(ion-1) with mat(1) = 1 and in = Ne alpha + chex H0 H (the
+// coefficient unscaled); mat replaces out in place and 1 / sum(mat) goes to
+// the unused stage-0 entry.
+template <class R>
+__device__ void heavy_products(const MocBalanceArgs& a, const LdsT<R>& L, int e, double Ne, double H) {
+    const int z = a.elemZ[e], top = imin(a.NS, z + 1);
+    double* out = L.out + L.off[e];
+    const double xH = L.x(a.elemSlot[1], 1);
     double m = 1., sum = 1.;
     for (int ion = 2; ion <= top; ++ion) {
-        const double in = Ne * L.al(e, ion - 1) + chex<R>(a, z, ion - 1, Te, false) * H * L.x(eH, 1);
+        const double in = Ne * L.al(e, ion - 1) + a.chexU[z * 32 + ion - 1] * H * xH;
         const double mn = m * out[ion - 1] / in;
         out[ion - 1] = m;
         m = mn;
         sum += mn;
     }
     out[top] = m;
-    const double* mat = out;
-    sum = 1. / sum;
-    L.x(e, top) = mat[top] * sum;
-    for (int ion = 1; ion <= imin(z, NS - 1); ++ion) {
+    out[0] = 1. / sum;
+}
+
+// (3) The new fractions, one (element, stage) per lane: X = mat / sum, on the
+// first pass the geometric mean with the old value, clamped to
+// [1e-20, 0.99999] below the top stage.
+template <class R>
+__device__ void heavy_update(const MocBalanceArgs& a, const LdsT<R>& L, int nIterX, int tid) {
+    for (int k = tid; k < a.NE * L.S2; k += 64) {
+        const int e = k / L.S2, ion = k % L.S2;
+        const int z = a.elemZ[e], top = imin(a.NS, z + 1);
+        if (z < 3 || ion < 1 || ion > top) continue;
+        const double* mat = L.out + L.off[e];
+        const double sum = mat[0];
+        if (ion == top) {
+            L.x(e, top) = mat[top] * sum;
+            continue;
+        }
         double v;
         if (nIterX < 2 && mat[ion] * sum > 0.) {
             const double old = L.x(e, ion) <= 0. ? 1.e-5 : L.x(e, ion);
@@ -424,9 +463,12 @@ __device__ bool ion_balance(const MocBalanceArgs& a, const RT& r, const LdsT<R>&
         __syncthreads();
         if (tid == 0) hhe_chain<R>(a, r, L, 1, Te, Ne, H, H0, collIonH);
         if (tid == 1 && eHe >= 0) hhe_chain<R>(a, r, L, 2, Te, Ne, H, H0, collIonH);
+        heavy_out<R>(a, r, L, Ne, revFrozen, tid);
         __syncthreads();
         for (int e = tid; e < NE; e += 64)
-            if (a.elemZ[e] >= 3) heavy_chain<R>(a, r, L, e, Te, Ne, H, nIterX, revFrozen);
+            if (a.elemZ[e] >= 3) heavy_products<R>(a, L, e, Ne, H);
+        __syncthreads();
+        heavy_update<R>(a, L, nIterX, tid);
         __syncthreads();
         // free electrons: the stages of each element over the lanes, the
         // per-lane partial sums then reduced over the wave

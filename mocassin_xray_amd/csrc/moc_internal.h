@@ -158,6 +158,19 @@ struct MocBalanceArgs {
     const double* auger;   // [31][32][11][11]
     const int* chexIdx;    // [31*32] -> row of chexTab or -1
     const double* chexTab; // [n][4]
+    const double* chexU;   // [31*32] unscaled charge-exchange coefficient (0 without a row)
+    const int* dielIdx;    // [31*32][2] dielectronic row in force below / at or above 2e4 K, or -1
+    const int* aldroIdx;   // [31*32] Aldrovandi & Pequignot row or -1
+    // heavy-element chains as gather lists: task t sums the terms
+    // htBeg[t] .. htBeg[t+1]-1 into out(htSlot[t], htMx[t]) in the sequential
+    // walk's order (term code: kind | rev. charge-exchange slot << 2 | source
+    // stage << 4 | shell << 9; htAug: Auger yield of kind-2 terms)
+    int nHTask;
+    const int* htSlot;
+    const int* htMx;
+    const int* htBeg;
+    const int* htCode;
+    const double* htAug;
     const int* atomIdx;    // [31*32]
     const int64_t* atomMeta;  // [nAtoms][8]
     const double* atomAtp;

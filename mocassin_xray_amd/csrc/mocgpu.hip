@@ -1084,6 +1084,90 @@ int moc_set_balance(moc_ctx* ctx, const moc_balance_tables* t) {
         }
         PUT(a.shInfo, info.data(), info.size(), "shInfo");
         PUT(a.shAug, aug.data(), aug.size(), "shAug");
+        // The rates out of each stage of the heavy-element chains (ionBalance
+        // :1826-1929) as gather lists: the sequential walk over (stage, shell,
+        // Auger electrons) adds its terms to out(target stage); here each
+        // target stage (a task) keeps its own terms in the walk's order, so
+        // that a lane per task forms the same sums, addition for addition.
+        // Term code: kind (0 collisional c1 Ne, 1 the shell's rate, 2 an
+        // Auger term) | reverse charge-exchange slot << 2 | source stage << 4
+        // | shell << 9; the Auger yield of kind-2 terms in htAug.  Tasks are
+        // sorted by length (longest first) so that the lanes share the work.
+        const int NS = at.nstages;
+        std::vector<std::vector<int>> code;
+        std::vector<std::vector<double>> yld;
+        std::vector<int> tSlot, tMx;
+        for (int e = 0; e < at.nElemUsed; ++e) {
+            const int z = at.elemZ[e];
+            if (z < 3) continue;
+            const int nIon = std::min(z, NS - 1), base = (int)code.size();
+            for (int mx = 1; mx <= nIon; ++mx) {
+                tSlot.push_back(e);
+                tMx.push_back(mx);
+                code.emplace_back();
+                yld.emplace_back();
+            }
+            for (int ion = 1; ion <= nIon; ++ion) {
+                code[base + ion - 1].push_back(0 | (ion << 4));
+                yld[base + ion - 1].push_back(0.);
+                const int spec = specOf[(size_t)e * 32 + ion];
+                if (spec < 0) continue;
+                for (int sh = t->speciesShell[spec]; sh < t->speciesShell[spec + 1]; ++sh) {
+                    const int na = info[sh] & 0xff, cx = info[sh] >> 8;
+                    code[base + ion - 1].push_back(1 | (cx << 2) | (ion << 4) | (sh << 9));
+                    yld[base + ion - 1].push_back(0.);
+                    for (int ne = 2; ne <= na; ++ne) {
+                        const int mx = std::min(ion + ne - 1, nIon);
+                        code[base + mx - 1].push_back(2 | (cx << 2) | (ion << 4) | (sh << 9));
+                        yld[base + mx - 1].push_back(t->auger[aug[sh] + ne]);
+                    }
+                }
+            }
+        }
+        std::vector<int> order(code.size());
+        for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return code[x].size() > code[y].size(); });
+        std::vector<int> hs, hm, hb{0}, hc;
+        std::vector<double> hy;
+        for (int k : order) {
+            hs.push_back(tSlot[k]);
+            hm.push_back(tMx[k]);
+            hc.insert(hc.end(), code[k].begin(), code[k].end());
+            hy.insert(hy.end(), yld[k].begin(), yld[k].end());
+            hb.push_back((int)hc.size());
+        }
+        a.nHTask = (int)hs.size();
+        PUT(a.htSlot, hs.data(), hs.size(), "chain tasks");
+        PUT(a.htMx, hm.data(), hm.size(), "chain tasks");
+        PUT(a.htBeg, hb.data(), hb.size(), "chain tasks");
+        PUT(a.htCode, hc.data(), hc.size(), "chain terms");
+        PUT(a.htAug, hy.data(), hy.size(), "chain terms");
+        // the charge-exchange coefficient of the heavy chains, unscaled (ionBalance
+        // leaves it at its 1e4 K value): chexFit(0) 1e-9 of the (Z, stage) row
+        std::vector<double> cu(31 * 32, 0.);
+        for (int k = 0; k < 31 * 32; ++k)
+            if (chexIdx[k] >= 0) cu[k] = t->chexFit[4 * (size_t)chexIdx[k]] * 1.e-9;
+        PUT(a.chexU, cu.data(), cu.size(), "chexU");
+        // dielectronic rows (dielectronic :3204-3305, the last matching row
+        // wins): per (Z, stage) the Nussbaumer & Storey row in force below and
+        // at or above 2e4 K, and the Aldrovandi & Pequignot row
+        std::vector<int> di(31 * 32 * 2, -1), ai(31 * 32, -1);
+        for (int i = 0; i < t->nDiel; ++i) {
+            const double* r = t->diel + 8 * (size_t)i;
+            const int z = (int)r[0], ion = (int)r[0] + 1 - (int)r[1], g = (int)r[7];
+            if (z < 0 || z > 30 || ion < 0 || ion > 31) continue;
+            if (g == 0 || g == 1) di[(z * 32 + ion) * 2 + 0] = i;
+            if (g == 0 || g == 2) di[(z * 32 + ion) * 2 + 1] = i;
+        }
+        for (int i = 0; i < t->nAldro; ++i) {
+            const double* r = t->aldro + 6 * (size_t)i;
+            const int z = (int)r[0], ion = (int)r[0] - (int)r[1] + 1;
+            if (z < 0 || z > 30 || ion < 0 || ion > 31) continue;
+            ai[z * 32 + ion] = i;
+        }
+        PUT(a.dielIdx, di.data(), di.size(), "dielIdx");
+        PUT(a.aldroIdx, ai.data(), ai.size(), "aldroIdx");
     }
     PUT(a.chexIdx, chexIdx.data(), 31 * 32, "chexIdx");
     PUT(a.chexTab, t->chexFit, 4 * (size_t)t->nChex, "chexFit");
