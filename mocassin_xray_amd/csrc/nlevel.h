@@ -13,6 +13,71 @@
 
 namespace {
 
+// Lane I of this lane's team of W = 8, 16 or 32 consecutive lanes, read by
+// every lane of the team without LDS: DPP row_newbcast broadcasts lane I of
+// each 16-lane row to its row; teams of 8 pick their half's lane, and teams
+// of 32 take the row that holds lane I through v_permlane16_swap (which
+// exchanges the odd rows of one operand with the even rows of the other).
+// Every lane of the wave must be active, and I a compile-time constant.
+template <int I>
+__device__ __forceinline__ int row_bcast(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x150 + I, 0xF, 0xF, false);
+}
+
+template <int W, int I>
+__device__ __forceinline__ int team_bcast_i(int v) {
+    static_assert(W == 8 || W == 16 || W == 32, "team width");
+    static_assert(I >= 0 && I < W, "team lane");
+    if constexpr (W == 16) {
+        return row_bcast<I>(v);
+    } else if constexpr (W == 8) {
+        const int lo = row_bcast<I>(v), hi = row_bcast<8 + I>(v);
+        return (threadIdx.x & 8) ? hi : lo;
+    } else {
+        const int t = row_bcast<I % 16>(v);
+        const auto p = __builtin_amdgcn_permlane16_swap(t, t, false, false);
+        return I < 16 ? (int)p[0] : (int)p[1];
+    }
+}
+
+template <int W, int I>
+__device__ __forceinline__ double team_bcast_d(double v) {
+    const int lo = team_bcast_i<W, I>(__double2loint(v));
+    const int hi = team_bcast_i<W, I>(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// the same with the lane a constant of the unrolled loops that call it; teams
+// of 32 lanes solve ions of 17-27 levels, whose loops are not unrolled, and
+// read the lane by a shuffle
+template <int W>
+__device__ __forceinline__ double team_bcast(double v, int i) {
+    if constexpr (W > 16) return __shfl(v, i, W);
+#define MOC_TB(I)                                                  \
+    case I:                                                         \
+        if constexpr (I < W) return team_bcast_d<W, (I < W ? I : 0)>(v); \
+        break;
+    switch (i) {
+        MOC_TB(0) MOC_TB(1) MOC_TB(2) MOC_TB(3) MOC_TB(4) MOC_TB(5) MOC_TB(6) MOC_TB(7)
+        MOC_TB(8) MOC_TB(9) MOC_TB(10) MOC_TB(11) MOC_TB(12) MOC_TB(13) MOC_TB(14) MOC_TB(15)
+        MOC_TB(16) MOC_TB(17) MOC_TB(18) MOC_TB(19) MOC_TB(20) MOC_TB(21) MOC_TB(22) MOC_TB(23)
+        MOC_TB(24) MOC_TB(25) MOC_TB(26) MOC_TB(27) MOC_TB(28) MOC_TB(29) MOC_TB(30) MOC_TB(31)
+        default: break;
+    }
+#undef MOC_TB
+    return __shfl(v, i, W);
+}
+
+// A table pointer as a global-memory pointer: the solves run in functions of
+// their own, where the compiler no longer sees that the tables are in global
+// memory and would read them with flat loads.
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> as_global(const T* p) {
+    return (gptr<T>)p;
+}
+
 // Level count of n-level ion `atom` as solved: its file's levels, cut at maxLev.
 __device__ __forceinline__ int nlevel_count(const MocBalanceArgs& a, int atom) {
     const int nf = (int)a.atomMeta[8 * atom];
@@ -28,24 +93,24 @@ __device__ __forceinline__ int nlevel_count(const MocBalanceArgs& a, int atom) {
 template <class R>
 struct NlevIon {
     int nf, n, iup2p;
-    const double *g, *en, *A, *q, *q2;
+    gptr<double> g, en, A, q, q2;
     double atp;
     size_t lo, hi;      // offsets of the bracketing tables
     R wa, wb, ca, cb;   // weights of the bracketing values and second derivatives
     R lT, Te, cs, kT;   // log10 T, T, 8.63e-6 / sqrt(T), 1.4388463 / T
 
     __device__ NlevIon(const MocBalanceArgs& a, int atom, double Te_) {
-        const int64_t* m = a.atomMeta + 8 * atom;
+        const gptr<int64_t> m = as_global(a.atomMeta + 8 * atom);
         nf = (int)m[0];
         const int nT = (int)m[1];
-        const double* lt = a.atomLogT + m[2];
-        g = a.atomG + m[3];
-        en = a.atomE + m[3];
-        A = a.atomA + m[4];
-        q = a.atomQom + m[5];
-        q2 = a.atomQ2 + m[5];
+        const gptr<double> lt = as_global(a.atomLogT + m[2]);
+        g = as_global(a.atomG + m[3]);
+        en = as_global(a.atomE + m[3]);
+        A = as_global(a.atomA + m[4]);
+        q = as_global(a.atomQom + m[5]);
+        q2 = as_global(a.atomQ2 + m[5]);
         iup2p = (int)m[6];
-        atp = a.atomAtp[atom];
+        atp = as_global(a.atomAtp)[atom];
         n = nf < a.maxLev ? nf : a.maxLev;
         Te = (R)Te_;
         lT = log10(Te);
@@ -218,7 +283,7 @@ __device__ double nlevel_fixed(const MocBalanceArgs& a, int atom, double Te, dou
 // The same sum for ions of up to NL = 8, 16 or 27 levels, solved by a team of
 // W = 8, 16 or 32 lanes of one wave with the level matrix in registers: team
 // lane k < n owns row k (its entries and its right-hand side).  The pivot
-// row's entries travel by shuffles (one division per lane and step); the back
+// row's entries travel by DPP broadcasts (one division per lane and step); the back
 // substitution is column-oriented (row k's solution broadcast once, every row
 // above it updated in parallel); the populations and the line sums are
 // gathered on every lane of the team.  Every lane of the team calls it with
@@ -267,29 +332,29 @@ __device__ __attribute__((noinline)) double nlevel_team(const MocBalanceArgs& a,
     // elimination without pivoting: step i subtracts pivot row i from the rows below
 #pragma unroll
     for (int i = 0; i < NL - 1; ++i) {
-        const double piv = __shfl(x[i], i, W);
+        const double piv = team_bcast<W>(x[i], i);
         const double f = x[i] / piv;
         const bool below = k > i;
 #pragma unroll
         for (int j = i + 1; j < NL; ++j) {
-            const double xij = __shfl(x[j], i, W);
+            const double xij = team_bcast<W>(x[j], i);
             if (below) x[j] -= xij * f;
         }
-        const double yi = __shfl(y, i, W);
+        const double yi = team_bcast<W>(y, i);
         if (below) y -= yi * f;
     }
     // back substitution, column-oriented: the last row first
 #pragma unroll
     for (int j = NL - 1; j >= 0; --j) {
         if (k == j) y = y / x[j];
-        const double yj = __shfl(y, j, W);
+        const double yj = team_bcast<W>(y, j);
         if (k < j && j < n) y -= yj * x[j];
     }
     // populations: the sum over the levels on every lane, in level order
     double sumN = 0.;
 #pragma unroll
     for (int j = 0; j < NL; ++j) {
-        const double yj = __shfl(y, j, W);
+        const double yj = team_bcast<W>(y, j);
         if (j < n) sumN += yj;
     }
     // lines out of level k (the upper level), summed on its lane, then over the team
