@@ -485,6 +485,241 @@ k_emission(MocEmissionArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// The continuum of the mixed-precision form: the per-bin energies in FP32 (the
+// fits, exponentials, Gaunt factors and sums over ions), the CDF scan and its
+// normalisation in FP64, the row written in FP32 (the transport's copy).  The
+// constant factors of each term are folded so that no intermediate product
+// leaves the FP32 range while its term can still matter: the Milne terms take
+// 4 pi kMilne hcRyd 1e40 (times 2 or 0.5) as one factor, and the heavy-ion
+// weights carry 1e25 that their common factor takes back.  Per-cell set-up
+// (the gamma fits at T_e, the Gaunt coefficients, the two-photon and He line
+// factors) stays in FP64 and is rounded once.
+constexpr double kScaleFb = 1.e25;
+
+// hydro2phot in FP32; hPlanck 1e27 folded into 6.6262
+__device__ float hydro2phot_f32(float nu, int Z, float aEffH, float QH, float aEffHeII, float Ne) {
+    const float nu0 = Z == 1 ? 0.7496f : 3.00f;
+    if (nu >= nu0) return 0.f;
+    const float y = nu / nu0;
+    const float shape = y * (1.f - y) * (1.f - powf(4.f * y * (1.f - y), 0.8f)) +
+                        0.88f * powf(y * (1.f - y), 1.53f) * powf(4.f * y * (1.f - y), 0.8f);
+    if (Z == 1) {
+        const float Ay = 202.0f * shape;
+        const float gNu = 6.6262f * y * Ay / 8.2249f;
+        return aEffH * gNu / (1.f + (Ne * QH) / 8.23f);
+    }
+    const float Ay = 64.f * 0.9994667f * 202.0f * shape;
+    const float gNu = 6.6262f * y * Ay / (8.226f * 64.f);
+    return aEffHeII * gNu;
+}
+
+struct CellCtxF {
+    float Te, Ne, sqrTe, kT;      // kT: hcRyd/k / Te
+    float abHXH2, abHeXHe2, abHeXHe3;
+    bool heOn;
+    float aEffHeI;
+    GauntF g1, g2;
+    float aEffH, QH, aEffHeII;
+    float corr, ly[4];
+};
+
+// bin_energy in FP32 (gas only: no grain term)
+__device__ float bin_energy_f32(const MocEmissionArgs& a, const CellCtxF& c, const float (*sLoc)[MOC_EM_MAXLIM],
+                                const float (*sFit)[MOC_EM_MAXLIM / 2], const double (*sX)[MOC_MAXS + 1],
+                                int i, int b0, float ePass, const float* sF) {
+    constexpr float kFourPi = (float)(4. * kPi);
+    constexpr float kM2 = (float)(4. * kPi * 2. * kHcRyd * kMilne * 1.e40);
+    constexpr float kM05 = (float)(4. * kPi * 0.5 * kHcRyd * kMilne * 1.e40);
+    constexpr float kMH = (float)(4. * kPi * kMilne * kHcRyd * 1.e40 / kScaleFb);
+    constexpr float kFF = (float)(4. * kPi * 54.43);
+    const double nuD = a.nu[i - 1];  // frequency differences are taken in FP64 (no cancellation)
+    const float nu = (float)nuD;
+    const float Ne = c.Ne;
+    float gam[3] = {0.f, 0.f, 0.f};
+    for (int k = 0; k < 3; ++k) {
+        const int p = a.gpair[k * a.B + i - 1];
+        if (p > 0) {
+            const int iup = a.edge[k][2 * p - 1];
+            gam[k] = exp10f(sLoc[k][2 * p - 1] + sFit[k][p - 1] * (float)(nuD - a.nu[iup - 1]));
+        }
+    }
+    const float xlf = (float)a.log10nu[i - 1];
+    const float ex0 = expf(-nu * c.kT);
+    const float ff1 = kFF * 1.f * gaunt_eval_f32(c.g1, xlf) * ex0 / c.sqrTe;
+    const float ff2 = kFF * 4.f * gaunt_eval_f32(c.g2, xlf) * ex0 / c.sqrTe;
+    const float factor = nu * nu * nu / (c.Te * c.sqrTe);
+    if (i >= a.HIP) {
+        const float ex = expf((float)(a.nu[a.HIP - 1] - nuD) * c.kT);
+        gam[0] = kM2 * (float)a.xsHI[i - a.HIP] * factor * ex + ff1;
+        if (c.heOn && i >= a.edge[1][a.nlim[1] - 1]) {
+            const float sI = i >= a.HeIP ? (float)a.xsHeI[i - a.HeIP] : 0.f;
+            const float exI = expf((float)(a.nu[a.HeIP - 1] - nuD) * c.kT);
+            gam[1] = kM05 * sI * factor * exI + ff1;
+            if (i >= a.edge[2][a.nlim[2] - 1]) {
+                const float sII = i >= a.HeIIP ? (float)a.xsHeII[i - a.HeIIP] : 0.f;
+                const float exII = expf((float)(a.nu[a.HeIIP - 1] - nuD) * c.kT);
+                gam[2] = kM2 * sII * factor * exII + ff2;
+            }
+        }
+    }
+    float gFac = 0.f, gDir = 0.f;
+    for (int n = 0; n < a.nFb; ++n) {
+        const int lo = a.fbLo[n];
+        if (i < lo || i >= a.fbHi[n]) continue;
+        const float xs = (float)a.fbXs[a.fbOff[n] + (i - lo)];
+        if (lo - 1 <= b0) {
+            gFac += xs * sF[n];
+        } else {
+            const int s = a.fbSlot[n];
+            const float ex = expf((float)(a.nu[lo - 1] - nuD) * c.kT);
+            gDir += ex * xs * (float)(a.fbG[n] * sX[s][a.fbIon[n] + 1] * a.abund[s] * kScaleFb);
+        }
+    }
+    const float gHv = (ePass * gFac + gDir) * kMH * factor;
+    const float t2HI = hydro2phot_f32(nu, 1, c.aEffH, c.QH, c.aEffHeII, Ne);
+    const float t2HeII = hydro2phot_f32(nu, 2, c.aEffH, c.QH, c.aEffHeII, Ne);
+    float t2HeI = 0.f;
+    const int j = a.he2Row[i - 1];
+    if (j > 0) {
+        const double* yd = a.hei2phot;
+        const float y = nu / 1.514f;
+        const float f1 = (float)((yd[2 * j + 1] - yd[2 * (j - 1) + 1]) / (yd[2 * j] - yd[2 * (j - 1)]));
+        const float f2 = (float)yd[2 * (j - 1) + 1] - (float)yd[2 * (j - 1)] * f1;
+        float Ay = f1 * y + f2;
+        if (Ay < 9.2163086E-03f) Ay = 0.f;
+        t2HeI = c.aEffHeI * Ay * 0.66262f * y / 51.3f;
+    }
+    float eHI = (gam[0] + t2HI) * c.abHXH2 * Ne + gHv * Ne;
+    float eHeI = c.heOn ? (gam[1] + t2HeI) * c.abHeXHe2 * Ne : 0.f;
+    float eHeII = c.heOn ? (gam[2] + t2HeII) * c.abHeXHe3 * Ne : 0.f;
+    if (eHI < 1.e-15f) eHI = 0.f;
+    if (eHeI < 1.e-15f) eHeI = 0.f;
+    if (eHeII < 1.e-15f) eHeII = 0.f;
+    const float w = (float)(a.widFlx[i - 1] * (kCRyd / 1.e15));
+    const float sHI = w * eHI;
+    float sHeI = w * eHeI;
+    float sHeII = w * eHeII;
+    if (c.heOn) {
+        if (i == a.j2TsP) sHeI += c.corr;
+        for (int k = 0; k < 4; ++k)
+            if (i == a.lymP[k]) sHeII += c.ly[k];
+    }
+    (void)kFourPi;
+    return sHI + sHeI + sHeII;
+}
+
+// The continuum and its CDF in the mixed form (gas runs), one workgroup of
+// four waves per cell, as k_emission; the row goes to the FP32 recPDF.
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_emission_f32(MocEmissionArgs a) {
+    __shared__ double sX[MOC_MAXE][MOC_MAXS + 1];
+    __shared__ double sLocD[3][MOC_EM_MAXLIM];
+    __shared__ float sLoc[3][MOC_EM_MAXLIM];
+    __shared__ float sFit[3][MOC_EM_MAXLIM / 2];
+    __shared__ double sWave[kThreads / 64];
+    __shared__ float sF[MOC_EM_MAXFB];
+    const int cell = blockIdx.x + 1;
+    if (moc_unchanged(a.TeOld, a.NeOld, a.Te, a.Ne, cell)) return;  // uniform per workgroup
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int C1 = a.C1, B = a.B;
+    const double NeGrid = a.Ne[cell], Te = a.Te[cell];
+    const double Ne = NeGrid < 1.e-5 ? 1.e-5 : NeGrid;
+    for (int k = tid; k < a.NE * a.NS; k += kThreads) {
+        const int e = k % a.NE, ion = k / a.NE + 1;
+        sX[e][ion] = a.ionDen[(size_t)k * C1 + cell];
+    }
+    for (int k = 0; k < 3; ++k) {  // gamma fits at Te (fb_ff :389-432), in FP64
+        const int nl = a.nlim[k];
+        const double* lg = a.lg[k];
+        for (int i = tid; i < nl; i += kThreads) {
+            double v;
+            if (Te <= a.tk[0]) v = lg[i];
+            else if (Te >= a.tk[a.ntk - 1]) v = lg[(size_t)(a.ntk - 1) * nl + i];
+            else {
+                int lo = 0, hi = a.ntk - 1;
+                while (hi - lo > 1) {
+                    const int m = (lo + hi) >> 1;
+                    if (a.tk[m] <= Te) lo = m; else hi = m;
+                }
+                const double factor = log10(Te / a.tk[lo]);
+                const double s = (lg[(size_t)hi * nl + i] - lg[(size_t)lo * nl + i]) / log10(a.tk[hi] / a.tk[lo]);
+                v = lg[(size_t)lo * nl + i] + s * factor;
+            }
+            sLocD[k][i] = v;
+            sLoc[k][i] = (float)v;
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < 3; ++k)
+        for (int p = tid + 1; p <= a.nlim[k] / 2; p += kThreads) {
+            const int ilow = a.edge[k][2 * p - 2], iup = a.edge[k][2 * p - 1];
+            sFit[k][p - 1] = (float)((sLocD[k][2 * p - 1] - sLocD[k][2 * p - 2]) / (a.nu[iup - 1] - a.nu[ilow - 1]));
+        }
+    CellCtxF c;
+    c.Te = (float)Te;
+    c.Ne = (float)Ne;
+    c.sqrTe = (float)sqrt(Te);
+    c.kT = (float)(kHcRydK / Te);
+    c.abHXH2 = (float)(a.abund[a.hSlot] * sX[a.hSlot][2]);
+    c.heOn = a.heSlot >= 0 && a.xsHeI && a.xsHeII;
+    const double abHe = c.heOn ? a.abund[a.heSlot] : 0.;
+    const double XHe2 = c.heOn ? sX[a.heSlot][2] : 0., XHe3 = c.heOn ? sX[a.heSlot][3] : 0.;
+    c.abHeXHe2 = (float)(abHe * XHe2);
+    c.abHeXHe3 = (float)(abHe * XHe3);
+    c.aEffHeI = (float)(6.23 * pow(Te / 10000., -0.827));
+    c.g1 = gaunt_f32(gaunt_setup(0., log10(Te)));
+    c.g2 = gaunt_f32(gaunt_setup(0.30103, log10(Te)));
+    const TwoPhotT tp = two_phot_setup(Te);
+    c.aEffH = (float)tp.aEffH;
+    c.QH = (float)tp.QH;
+    c.aEffHeII = (float)tp.aEffHeII;
+    c.corr = 0.f;
+    for (int k = 0; k < 4; ++k) c.ly[k] = 0.f;
+    if (c.heOn) {
+        const double T4 = Te * 1.e-4;
+        c.corr = (float)(27.2 * pow(T4, -0.678) * Ne * abHe * XHe2 * 1.45673 * kHcRyd * 1e11);
+        const double ly3 = pow(10., -0.792 * log10(Te) + 6.01) * Ne * XHe3 * abHe;
+        c.ly[3] = (float)ly3;
+        for (int k = 0; k < 3; ++k) c.ly[k] = (float)(a.heiiLyman[2 * k] * ly3);
+    }
+    float* row = a.recPDF32 + (size_t)cell * B;
+    double carry = 0.;
+    for (int b0 = 0; b0 < B; b0 += kThreads) {
+        const int b = b0 + tid;
+        const double nuA = a.nu[b0];
+        for (int n = tid; n < a.nFb; n += kThreads) {
+            const int lo = a.fbLo[n], sl = a.fbSlot[n];
+            sF[n] = lo - 1 <= b0 ? (float)(exp((-nuA + a.nu[lo - 1]) * kHcRydK / Te) * a.fbG[n] *
+                                           sX[sl][a.fbIon[n] + 1] * a.abund[sl] * kScaleFb)
+                                 : 0.f;
+        }
+        __syncthreads();
+        const float ePass = b < B ? expf((float)((-a.nu[b] + nuA) * kHcRydK / Te)) : 0.f;
+        const double v = b < B ? (double)bin_energy_f32(a, c, sLoc, sFit, sX, b + 1, b0, ePass, sF) : 0.;
+        double sc = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const double t = __shfl_up(sc, off, 64);
+            if (lane >= off) sc += t;
+        }
+        if (lane == 63) sWave[wv] = sc;
+        __syncthreads();
+        double pre = carry;
+        for (int k = 0; k < wv; ++k) pre += sWave[k];
+        if (b < B) row[b] = (float)(pre + sc);
+        double next = carry;
+        for (int k = 0; k < kThreads / 64; ++k) next += sWave[k];
+        __syncthreads();
+        carry = next;
+    }
+    const double total = carry;
+    for (int b = tid; b < B; b += kThreads) row[b] = b == B - 1 ? 1.f : (float)((double)row[b] / total);
+    if (tid == 0) {
+        const double L = a.lineSum[cell];
+        a.totalLines[cell] = L / (total + L);
+    }
+}
+
 // n-level line emission of the ions of team class c (nlevel_team_class), P =
 // 64 / W ions at a time, one per team of W lanes; with lineAcc (the line
 // luminosities) only the ions k0 .. k0 + nk - 1 of the launch's group add
@@ -664,7 +899,8 @@ int moc_launch_emission(moc_ctx* ctx, const MocEmissionArgs& a) {
     const dim3 grid((unsigned)a.nCells);
     if (maxLev <= kNlevLane) hipLaunchKernelGGL(k_em_lines<false>, grid, dim3(64), 0, ctx->stream, a);
     else hipLaunchKernelGGL(k_em_lines<true>, grid, dim3(64), 0, ctx->stream, a);
-    hipLaunchKernelGGL(k_emission, grid, dim3(kThreads), 0, ctx->stream, a);
+    if (a.recPDF32) hipLaunchKernelGGL(k_emission_f32, grid, dim3(kThreads), 0, ctx->stream, a);
+    else hipLaunchKernelGGL(k_emission, grid, dim3(kThreads), 0, ctx->stream, a);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         ctx->err = std::string("emission launch: ") + hipGetErrorString(e);

@@ -92,4 +92,51 @@ __device__ double gaunt_eval(const GauntT& g, double xlf) {
     return 0.0;
 }
 
+// The same fit in FP32 for the mixed-precision form: coefficients from the
+// FP64 set-up, rounded once; the same branches.
+struct GauntF {
+    float c[9];
+    float con, gamma2, lg2;
+};
+
+__device__ GauntF gaunt_f32(const GauntT& g) {
+    GauntF f;
+    for (int j = 0; j < 9; ++j) f.c[j] = (float)g.c[j];
+    f.con = (float)g.con;
+    f.gamma2 = (float)g.gamma2;
+    f.lg2 = (float)g.lg2;
+    return f;
+}
+
+// gaunt_eval in FP32 (the same branches)
+__device__ float gaunt_eval_f32(const GauntF& g, float xlf) {
+    const float txu = 0.72727273f * xlf + g.con;
+    if (fabsf(txu) <= 2.0f) {
+        float b[9];
+        b[8] = g.c[8];
+        b[7] = txu * b[8] + g.c[7];
+        int ir = 6;
+        for (int j = 1; j <= 6; ++j) {
+            b[ir] = txu * b[ir + 1] - b[ir + 2] + g.c[ir];
+            --ir;
+        }
+        return b[1] - b[3];
+    }
+    const float u = exp10f((txu - .90909091f) / .72727273f);
+    if (txu > 2.0f) {
+        if (g.lg2 < -3.f) return sqrtf(0.9549297f / u);
+        if (g.lg2 < 0.0f) {
+            const float slope = (sqrtf(12.f * g.gamma2 / u) - sqrtf(0.9549297f / u)) / 3.0f;
+            return sqrtf(12.f * g.gamma2 / u) + slope * g.lg2;
+        }
+        const float v = sqrtf(12.f * g.gamma2 / u);
+        return v > 1.05f ? 1.05f : v;
+    }
+    if (log10f(u) < -4.0f) {
+        if (g.gamma2 < 0.3f) return 0.551329f * logf(2.24593f / u);
+        return 0.551329f * logf(0.944931f / u / sqrtf(g.gamma2));
+    }
+    return 0.0f;
+}
+
 }  // namespace

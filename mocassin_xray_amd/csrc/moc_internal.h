@@ -113,6 +113,7 @@ struct MocOpacityArgs {
     const double* Te;
     const double* Hden;
     double* opacity;       // [B][C1]
+    float* opacity32;      // [B][C1]: mixed form (gas runs), the FP32 kernel writes this plane instead
     // cells whose Te and Ne moved by < 0.5% since the last balance keep their
     // opacity (iterateMC, iteration_mod.f90:126-153); NULL: every cell
     const double* TeOld;
@@ -304,6 +305,7 @@ struct MocEmissionArgs {
     const double* Ne;
     const double* ionDen;    // [NS][NE][C1]
     double* recPDF;          // [C1][B]
+    float* recPDF32;         // [C1][B]: mixed form (gas runs), the continuum in FP32 writes this row instead
     double* totalLines;      // [C1]
     double* lineSum;         // [C1] line emissivity of each cell (lines kernel -> CDF kernel)
     const double* TeOld;     // skip test as for the opacity (iteration_mod.f90:318-349); NULL: none
@@ -412,6 +414,10 @@ struct MocGrid {
     float* f_recPDF = nullptr;
     float* f_totalLines = nullptr;
     bool f32_dirty = true;
+    // the mixed form's FP32 kernels write the FP32 planes directly: while set,
+    // the FP32 plane holds the current opacity / diffuse CDF and the FP64 one
+    // is brought up to date before the host reads or writes it
+    bool op32_master = false, pdf32_master = false;
     // grains (moc_set_dust)
     double* d_Ndust = nullptr;       // [C+1]
     double* d_Tdust = nullptr;       // [C+1][nSizes+1][nSpecies+1] (species fastest)
@@ -467,6 +473,8 @@ struct moc_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
     uint64_t graphSeed = 0;
+    // planes the captured iteration writes in FP32 (their FP64 copies go stale at each replay)
+    bool graphOp32 = false, graphPdf32 = false;
     uint32_t* d_ctr = nullptr;
     uint32_t ctrHost = 0;  // source of the d_ctr upload, alive until the replay is waited for
     int precision = MOC_PREC_FP64;
@@ -495,6 +503,7 @@ struct moc_ctx {
 int moc_launch_transport_f64(moc_ctx* ctx, const MocTransportArgs<double>& a, float* ms);
 int moc_launch_transport_f32(moc_ctx* ctx, const MocTransportArgs<float>& a, float* ms);
 void moc_convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);
+void moc_convert_f32_to_f64(const float* src, double* dst, int64_t n, hipStream_t s);
 void moc_scale_f64(double* p, int64_t n, double f, hipStream_t s);
 void moc_fill_zero_u64(unsigned long long* p, int64_t n, hipStream_t s);
 void moc_copy_f64(const double* src, double* dst, int64_t n, hipStream_t s);

@@ -302,13 +302,31 @@ int refresh_f32(moc_ctx* ctx) {
     MocGrid& g = ctx->grid;
     if (!g.f32_dirty) return MOC_OK;
     const int64_t C1 = g.nCells + 1, B = ctx->nbins;
-    moc_convert_f64_to_f32(g.d_opacity, g.f_opacity, C1 * B, ctx->stream);
-    moc_convert_f64_to_f32(g.d_recPDF, g.f_recPDF, C1 * B, ctx->stream);
+    if (!g.op32_master) moc_convert_f64_to_f32(g.d_opacity, g.f_opacity, C1 * B, ctx->stream);
+    if (!g.pdf32_master) moc_convert_f64_to_f32(g.d_recPDF, g.f_recPDF, C1 * B, ctx->stream);
     moc_convert_f64_to_f32(g.d_Ne, g.f_Ne, C1, ctx->stream);
     moc_convert_f64_to_f32(g.d_totalLines, g.f_totalLines, C1, ctx->stream);
     if (g.d_scaOpac) moc_convert_f64_to_f32(g.d_scaOpac, g.f_scaOpac, C1 * B, ctx->stream);
     MOC_TRY(hipGetLastError(), "f32 refresh");
     g.f32_dirty = false;
+    return MOC_OK;
+}
+
+// Bring the FP64 plane of field f up to date where the mixed form's FP32
+// kernels wrote its FP32 plane (the widening is exact, so both then agree and
+// the FP64 plane is the master again).
+int sync_f64_master(moc_ctx* ctx, int f) {
+    MocGrid& g = ctx->grid;
+    const int64_t n = ((int64_t)g.nCells + 1) * ctx->nbins;
+    if (f == MOC_F_OPACITY && g.op32_master) {
+        moc_convert_f32_to_f64(g.f_opacity, g.d_opacity, n, ctx->stream);
+        g.op32_master = false;
+    }
+    if (f == MOC_F_RECPDF && g.pdf32_master) {
+        moc_convert_f32_to_f64(g.f_recPDF, g.d_recPDF, n, ctx->stream);
+        g.pdf32_master = false;
+    }
+    MOC_TRY(hipGetLastError(), "FP64 plane refresh");
     return MOC_OK;
 }
 
@@ -715,6 +733,8 @@ MocOpacityArgs opacity_args(moc_ctx* ctx, bool skipUnchanged) {
     // is not added twice
     const bool dust = ctx->dust.set && ctx->dust.lgGas;
     a.opacity = dust ? g.d_gasOpac : g.d_opacity;
+    // the mixed form's opacity in FP32 (gas runs): the plane the transport reads
+    a.opacity32 = ctx->precision == MOC_PREC_MIXED && !ctx->dust.set ? g.f_opacity : nullptr;
     a.TeOld = skipUnchanged ? g.d_TeOld : nullptr;
     a.NeOld = skipUnchanged ? g.d_NeOld : nullptr;
     return a;
@@ -724,9 +744,12 @@ int run_opacity(moc_ctx* ctx, bool skipUnchanged, float* ms) {
     MocGrid& g = ctx->grid;
     const bool dust = ctx->dust.set && ctx->dust.lgGas;
     const MocOpacityArgs a = opacity_args(ctx, skipUnchanged);
+    int rc;
+    // the FP64 kernel keeps the rows of unchanged cells: bring them up to date
+    if (!a.opacity32 && (rc = sync_f64_master(ctx, MOC_F_OPACITY))) return rc;
     MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
-    int rc = moc_launch_opacity(ctx, a);
-    if (rc) return rc;
+    if ((rc = moc_launch_opacity(ctx, a))) return rc;
+    if (a.opacity32) g.op32_master = true;
     if (dust && (rc = launch_dust_opacity(ctx))) return rc;
     MOC_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
     MOC_TRY(hipEventSynchronize(ctx->ev1), "opacity");
@@ -740,6 +763,7 @@ int moc_upload(moc_ctx* ctx, int field, int ig, const void* host, int64_t count,
     if (!ctx) return MOC_ERR_ARG;
     MocGrid& g = ctx->grid;
     if (!g.attached || ig != 1) { ctx->err = "moc_upload: no grid attached"; return MOC_ERR_STATE; }
+    if (int rcs = sync_f64_master(ctx, field)) return rcs;
     double* d = field_ptr(ctx, field);
     if (!d || !host) { ctx->err = "moc_upload: unknown field"; return MOC_ERR_ARG; }
     const int64_t C1 = g.nCells + 1, B = ctx->nbins;
@@ -787,6 +811,7 @@ int moc_download(moc_ctx* ctx, int field, int ig, void* host, int64_t count, int
         }
         return MOC_OK;
     }
+    if (int rcs = sync_f64_master(ctx, field)) return rcs;
     double* d = field_ptr(ctx, field);
     if (!d || !host) { ctx->err = "moc_download: unknown field"; return MOC_ERR_ARG; }
     const int64_t C1 = g.nCells + 1, B = ctx->nbins;
@@ -818,6 +843,7 @@ int moc_device_ptr(moc_ctx* ctx, int field, int ig, void** dptr, int64_t* count,
     if (!ctx || !dptr) return MOC_ERR_ARG;
     MocGrid& g = ctx->grid;
     if (!g.attached || ig != 1) { ctx->err = "moc_device_ptr: no grid attached"; return MOC_ERR_STATE; }
+    if (int rcs = sync_f64_master(ctx, field)) return rcs;
     double* d = field_ptr(ctx, field);
     if (!d) { ctx->err = "moc_device_ptr: unknown field"; return MOC_ERR_ARG; }
     *dptr = d;
@@ -1559,6 +1585,8 @@ MocEmissionArgs emission_args(moc_ctx* ctx, bool skipUnchanged) {
     a.Ne = g.d_Ne;
     a.ionDen = g.d_ionDen;
     a.recPDF = g.d_recPDF;
+    // the mixed form's continuum in FP32 (gas runs): the row goes to the FP32 plane
+    a.recPDF32 = ctx->precision == MOC_PREC_MIXED && !ctx->dust.set ? g.f_recPDF : nullptr;
     a.totalLines = g.d_totalLines;
     a.lineSum = g.d_lineSum;
     a.TeOld = skipUnchanged ? g.d_TeOld : nullptr;
@@ -1576,9 +1604,12 @@ MocEmissionArgs emission_args(moc_ctx* ctx, bool skipUnchanged) {
 int run_emission(moc_ctx* ctx, bool skipUnchanged, float* kernel_ms) {
     MocGrid& g = ctx->grid;
     const MocEmissionArgs a = emission_args(ctx, skipUnchanged);
+    int rc;
+    if (!a.recPDF32 && (rc = sync_f64_master(ctx, MOC_F_RECPDF))) return rc;
     MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
-    int rc = moc_launch_emission(ctx, a);
+    rc = moc_launch_emission(ctx, a);
     if (rc) return rc;
+    if (a.recPDF32) g.pdf32_master = true;
     MOC_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
     MOC_TRY(hipEventSynchronize(ctx->ev1), "emission kernel");
     float ms = 0.f;
@@ -1616,9 +1647,15 @@ int enqueue_iteration(moc_ctx* ctx, uint64_t seed, int64_t nBatch) {
             (rc = moc_launch_dust_pdf(ctx, dust_args(ctx))))
             return rc;
     } else {
-        if ((rc = moc_launch_opacity(ctx, opacity_args(ctx, true)))) return rc;
+        const MocOpacityArgs oa = opacity_args(ctx, true);
+        if (!oa.opacity32 && (rc = sync_f64_master(ctx, MOC_F_OPACITY))) return rc;
+        if ((rc = moc_launch_opacity(ctx, oa))) return rc;
+        if (oa.opacity32) g.op32_master = true;
         if (ctx->dust.set && (rc = moc_launch_dust_opacity(ctx, dust_args(ctx)))) return rc;
-        if ((rc = moc_launch_emission(ctx, emission_args(ctx, true)))) return rc;
+        const MocEmissionArgs ea = emission_args(ctx, true);
+        if (!ea.recPDF32 && (rc = sync_f64_master(ctx, MOC_F_RECPDF))) return rc;
+        if ((rc = moc_launch_emission(ctx, ea))) return rc;
+        if (ea.recPDF32) g.pdf32_master = true;
     }
     // the tallies (moc_reset_tallies) and the transport counters
     moc_fill_zero_u64(reinterpret_cast<unsigned long long*>(g.d_J), n, ctx->stream);
@@ -1702,6 +1739,8 @@ int capture_iteration(moc_ctx* ctx, uint64_t seed) {
     ctx->graph = graph;
     ctx->graphExec = exec;
     ctx->graphSeed = seed;
+    ctx->graphOp32 = g.op32_master;
+    ctx->graphPdf32 = g.pdf32_master;
     return MOC_OK;
 }
 
@@ -1719,6 +1758,9 @@ int iterate_graph(moc_ctx* ctx, int nIterateMC, uint64_t seed, moc_iteration_sta
             "iteration counter");
     MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
     MOC_TRY(hipGraphLaunch(ctx->graphExec, ctx->stream), "graph launch");
+    // the replay wrote these planes in FP32 again
+    g.op32_master = g.op32_master || ctx->graphOp32;
+    g.pdf32_master = g.pdf32_master || ctx->graphPdf32;
     MOC_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
     MOC_TRY(hipEventSynchronize(ctx->ev1), "iteration graph");
     float ms = 0.f;

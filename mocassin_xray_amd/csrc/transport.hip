@@ -601,6 +601,12 @@ __global__ void k_f64_to_f32(const double* __restrict__ s, float* __restrict__ d
     for (int64_t k = i; k < n; k += st) d[k] = (float)s[k];
 }
 
+__global__ void k_f32_to_f64(const float* __restrict__ s, double* __restrict__ d, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = i; k < n; k += st) d[k] = (double)s[k];
+}
+
 __global__ void k_scale(double* __restrict__ p, int64_t n, double f) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t st = (int64_t)gridDim.x * blockDim.x;
@@ -682,6 +688,12 @@ void moc_convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_
     int64_t blocks = (n + 255) / 256;
     if (blocks > 65535) blocks = 65535;
     hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n);
+}
+void moc_convert_f32_to_f64(const float* src, double* dst, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65535) blocks = 65535;
+    hipLaunchKernelGGL(k_f32_to_f64, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n);
 }
 void moc_scale_f64(double* p, int64_t n, double f, hipStream_t s) {
     if (n <= 0) return;
