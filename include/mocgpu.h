@@ -451,6 +451,10 @@ typedef struct moc_tuning {
     int32_t transportWaves; /* resident waves of one transport launch, 64..2^20 (4096:
                                four per SIMD) */
     int32_t graph;          /* moc_get_tuning: 1 when moc_set_graph is on */
+    int32_t hotJ;           /* mixed form: 1 sums the J tallies of the cells around the
+                               star (times the bins that carry most path there) as FP32
+                               partials in LDS, workgroups of 1024 lanes; 0 adds every
+                               tally with a global atomic (1) */
 } moc_tuning;
 int moc_set_tuning(moc_ctx* ctx, const moc_tuning* t);
 int moc_get_tuning(moc_ctx* ctx, moc_tuning* t);

@@ -134,7 +134,7 @@ class MocIterationStats(C.Structure):
 
 
 class MocTuning(C.Structure):
-    _fields_ = [(n, C.c_int32) for n in ("emitBatch", "segsPerTrip", "transportWaves", "graph")]
+    _fields_ = [(n, C.c_int32) for n in ("emitBatch", "segsPerTrip", "transportWaves", "graph", "hotJ")]
 
     def as_dict(self) -> dict:
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -1,15 +1,17 @@
 """Launch-parameter tuning of the packet transport on a live model.
 
-The transport kernel has three launch parameters (moc_set_tuning): how many
+The transport kernel has four launch parameters (moc_set_tuning): how many
 diffuse packets a wave waits for before it emits them together, how many
-path segments a flying lane steps per trip through the wave's loop, and how
-many waves one launch keeps resident.  None of them changes which packets
-are drawn (every packet keeps its own Philox stream), so the choice is pure
-speed and depends on the model: optically thick, line-rich grids re-emit
-often and favour larger emission batches; thin grids favour longer trips.
+path segments a flying lane steps per trip through the wave's loop, how
+many waves one launch keeps resident, and (mixed form) whether the J
+tallies of the cells around the star are summed in LDS first.  None of them
+changes which packets are drawn (every packet keeps its own Philox stream),
+so the choice is pure speed and depends on the model: optically thick,
+line-rich grids re-emit often and favour larger emission batches; thin grids
+favour longer trips; the LDS sums pay where many tallies land near the star.
 
-tune_transport times candidates by coordinate descent (waves, then emission
-batch, then segments per trip), each with the same packets of star 1, and
+tune_transport times candidates by coordinate descent (the LDS sums, waves,
+emission batch, then segments per trip), each with the same packets of star 1, and
 leaves the fastest set on the context.  The transport adds to the J and
 escape tallies, so the tuner zeroes them when it is done: run it before an
 iteration (moc_iterate starts by zeroing them too), not between an iteration
@@ -24,6 +26,7 @@ from pathlib import Path
 WAVES = (1024, 2048, 4096, 8192, 16384)
 EMIT_BATCH = (8, 16, 24, 32, 48, 64)
 SEGS_PER_TRIP = (4, 8, 16, 24, 32, 48, 64)
+HOT_J = (0, 1)  # the mixed form's LDS sums of the hot J block off / on
 
 
 def _time(g, n: int, seed: int, repeats: int) -> float:
@@ -47,7 +50,7 @@ def tune_transport(g, packets: int | None = None, seed: int = 77, repeats: int =
     if n < 1:
         raise ValueError("tune_transport: no packets to time")
     start = g.get_tuning()
-    cur = {k: start[k] for k in ("emitBatch", "segsPerTrip", "transportWaves")}
+    cur = {k: start[k] for k in ("emitBatch", "segsPerTrip", "transportWaves", "hotJ")}
     trials = []
     t0 = time.perf_counter()
 
@@ -59,7 +62,7 @@ def tune_transport(g, packets: int | None = None, seed: int = 77, repeats: int =
 
     default_ms = trial(cur)
     best_ms = default_ms
-    for key, values in (("transportWaves", WAVES), ("emitBatch", EMIT_BATCH),
+    for key, values in (("hotJ", HOT_J), ("transportWaves", WAVES), ("emitBatch", EMIT_BATCH),
                         ("segsPerTrip", SEGS_PER_TRIP)):
         for v in values:
             if v == cur[key]:

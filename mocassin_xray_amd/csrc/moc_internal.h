@@ -94,6 +94,18 @@ struct MocTransportArgs {
     int nGrids;
     const MocGridView<T>* grids;  // [nGrids]
     const int64_t* gridFirst;     // [nGrids + 1]
+    // J tallies of a hot block of (bin, cell) keys -- a box of cells around
+    // the star times the bins that carried the most path there -- summed in
+    // LDS per workgroup as FP32 partials, then added to J in FP64 (the mixed
+    // form on a single grid; nHotC = 0: every tally is a global atomic)
+    int nHotC, nHotB;
+    int hotLo[3], hotN[3];   // the box: cell indices lo .. lo + n - 1 per axis (nHotC = product)
+    const int* hotBin;       // [B] slot of a bin, or -1
+    double* hotS;            // [B] work: J of each bin summed over the hot cells
+    const int* hotCellList;  // [nHotC] cell of each slot
+    const int* hotBinList;   // [nHotB] bin of each slot, or -1
+    float* hotPart;          // [blocks][nHotB * nHotC] the workgroups' partial sums
+    int hotBlocks;           // rows of hotPart
 };
 
 // Opacity kernel view (opacity.hip)
@@ -418,6 +430,14 @@ struct MocGrid {
     // the FP32 plane holds the current opacity / diffuse CDF and the FP64 one
     // is brought up to date before the host reads or writes it
     bool op32_master = false, pdf32_master = false;
+    // hot block of the J tallies (MocTransportArgs::nHotC): the maps and the
+    // per-workgroup partial sums, set up at the first mixed transport
+    int nHotC = 0, nHotB = 0, hotBlocks = 0;
+    bool hotTried = false;
+    int hotLo[3] = {0, 0, 0}, hotN[3] = {0, 0, 0};
+    int *d_hotBin = nullptr, *d_hotCellList = nullptr, *d_hotBinList = nullptr;
+    float* d_hotPart = nullptr;
+    double* d_hotS = nullptr;
     // grains (moc_set_dust)
     double* d_Ndust = nullptr;       // [C+1]
     double* d_Tdust = nullptr;       // [C+1][nSizes+1][nSpecies+1] (species fastest)
@@ -461,6 +481,7 @@ struct MocTuning {
     int emitBatch = 32;         // transport: lanes that wait to emit together
     int segsPerTrip = 32;       // transport: segments per flying lane and loop trip
     int transportWaves = 4096;  // transport: resident waves (the launch's lanes / 64)
+    int hotJ = 1;               // mixed transport: the hot block of J tallies in LDS
 };
 
 struct moc_ctx {
@@ -502,6 +523,9 @@ struct moc_ctx {
 // launchers (transport.hip)
 int moc_launch_transport_f64(moc_ctx* ctx, const MocTransportArgs<double>& a, float* ms);
 int moc_launch_transport_f32(moc_ctx* ctx, const MocTransportArgs<float>& a, float* ms);
+// workgroup size and LDS of the mixed transport with a hot J block of nHot keys
+int moc_transport_hot_block();
+size_t moc_transport_hot_lds(int nx, int ny, int nz, int nHot);
 void moc_convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);
 void moc_convert_f32_to_f64(const float* src, double* dst, int64_t n, hipStream_t s);
 void moc_scale_f64(double* p, int64_t n, double f, hipStream_t s);

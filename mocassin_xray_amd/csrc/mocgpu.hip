@@ -104,6 +104,8 @@ void free_grid(MocGrid& g) {
     dfree(g.d_conv); dfree(g.d_black); dfree(g.d_nIterT); dfree(g.d_rates);
     dfree(g.d_Ndust); dfree(g.d_Tdust); dfree(g.d_absOpac); dfree(g.d_scaOpac);
     dfree(g.d_gasOpac); dfree(g.f_scaOpac); dfree(g.d_collD); dfree(g.d_peD);
+    dfree(g.d_hotBin); dfree(g.d_hotCellList); dfree(g.d_hotBinList);
+    dfree(g.d_hotPart); dfree(g.d_hotS);
     g = MocGrid();
 }
 
@@ -903,6 +905,81 @@ MocTransportArgs<double> transport_args_f64(moc_ctx* ctx, int s, int64_t n, uint
     return a;
 }
 
+// The hot block of the mixed transport's J tallies (single grid): a box of
+// h x h x h cells around star 1 -- the cells every stellar packet crosses,
+// whose slot the kernel computes from the cell indices -- times as many bins
+// as the LDS of a workgroup of moc_transport_hot_block() lanes then holds (at
+// most kHotBins); the bins are chosen on the device after each launch
+// (k_hot_binsum, k_hot_select).  The partial buffer has a row per workgroup of
+// the launch at the current wave count.  Allocates, so it is called before
+// any graph capture.
+constexpr int kHotBins = 256;
+int ensure_hot(moc_ctx* ctx) {
+    MocGrid& g = ctx->grid;
+    const int B = ctx->nbins;
+    if (ctx->precision != MOC_PREC_MIXED || !ctx->tune.hotJ || g.nGrids > 1 || g.plane || B > 8192 ||
+        g.nCells < 1)
+        return MOC_OK;
+    const int blk = moc_transport_hot_block();
+    const int blocksNeed = (int)((64LL * ctx->tune.transportWaves + blk - 1) / blk);
+    if (g.hotTried && g.nHotC == 0) return MOC_OK;  // no hot block fits this grid
+    int rc;
+    if (!g.hotTried) {
+        g.hotTried = true;
+        int dev = 0, maxLds = 0;
+        MOC_TRY(hipGetDevice(&dev), "device");
+        MOC_TRY(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "LDS size");
+        const size_t cap = std::min<size_t>((size_t)maxLds, 160 * 1024);
+        const size_t geo = moc_transport_hot_lds(g.nx, g.ny, g.nz, 0) + 1024;
+        if (cap <= geo) return MOC_OK;
+        const int slots = (int)((cap - geo) / sizeof(float));
+        // the largest cube that leaves at least 128 bins (or all of them)
+        const int nbMin = std::min(B, 128);
+        int h = 1;
+        while ((h + 1) * (h + 1) * (h + 1) * nbMin <= slots) ++h;
+        const int n[3] = {g.nx, g.ny, g.nz};
+        int lo[3], hn[3];
+        for (int k = 0; k < 3; ++k) {
+            const int si = ctx->starIdx[k] - 1;  // 0-based index of the star's cell
+            hn[k] = std::min(h, n[k]);
+            lo[k] = g.sym ? 0 : std::max(0, std::min(si - hn[k] / 2, n[k] - hn[k]));
+        }
+        const int nc = hn[0] * hn[1] * hn[2];
+        const int nb = std::min({B, kHotBins, slots / nc});
+        if (nc < 8 || nb < 1) return MOC_OK;
+        std::vector<int> cellList((size_t)nc, 0);
+        for (int x = 0; x < hn[0]; ++x)
+            for (int y = 0; y < hn[1]; ++y)
+                for (int z = 0; z < hn[2]; ++z) {
+                    const int ix = lo[0] + x, iy = lo[1] + y, iz = lo[2] + z;
+                    cellList[(size_t)(x * hn[1] + y) * hn[2] + z] =
+                        g.dense ? 1 + iz + g.nz * (iy + g.ny * ix)
+                                : g.active[(size_t)ix + (size_t)g.nx * ((size_t)iy + (size_t)g.ny * iz)];
+                }
+        std::vector<int> binMap((size_t)B, -1), binList((size_t)nb, -1);
+        if ((rc = dalloc_n(ctx, &g.d_hotCellList, (int64_t)nc, "hot cells")) ||
+            (rc = dalloc_n(ctx, &g.d_hotBin, (int64_t)B, "hot bins")) ||
+            (rc = dalloc_n(ctx, &g.d_hotBinList, (int64_t)nb, "hot bins")) ||
+            (rc = dalloc_n(ctx, &g.d_hotS, (int64_t)B, "hot bin sums")))
+            return rc;
+        if ((rc = h2d(ctx, g.d_hotCellList, cellList.data(), cellList.size() * sizeof(int), "hot cells")) ||
+            (rc = h2d(ctx, g.d_hotBin, binMap.data(), binMap.size() * sizeof(int), "hot bins")) ||
+            (rc = h2d(ctx, g.d_hotBinList, binList.data(), binList.size() * sizeof(int), "hot bins")))
+            return rc;
+        for (int k = 0; k < 3; ++k) { g.hotLo[k] = lo[k]; g.hotN[k] = hn[k]; }
+        g.nHotC = nc;
+        g.nHotB = nb;
+    }
+    if (g.hotBlocks < blocksNeed) {
+        dfree(g.d_hotPart);
+        g.hotBlocks = 0;
+        if ((rc = dalloc_n(ctx, &g.d_hotPart, (int64_t)blocksNeed * g.nHotC * g.nHotB, "hot partials")))
+            return rc;
+        g.hotBlocks = blocksNeed;
+    }
+    return MOC_OK;
+}
+
 MocTransportArgs<float> transport_args_f32(moc_ctx* ctx, int s, int64_t n, uint64_t seed,
                                            uint32_t ctr_base, double deltaE) {
     MocGrid& g = ctx->grid;
@@ -922,6 +999,11 @@ MocTransportArgs<float> transport_args_f32(moc_ctx* ctx, int s, int64_t n, uint6
     a.deltaE = (float)deltaE; a.n = n; a.seed = seed; a.ctr_base = ctr_base; a.ctrDev = nullptr;
     a.emitBatch = ctx->tune.emitBatch; a.segsPerTrip = ctx->tune.segsPerTrip;
     a.nGrids = g.nGrids > 1 ? g.nGrids : 0; a.grids = g.d_view32; a.gridFirst = g.d_gridFirst;
+    const bool hot = ctx->tune.hotJ && a.nGrids == 0 && g.nHotC > 0 && g.hotBlocks > 0;
+    a.nHotC = hot ? g.nHotC : 0; a.nHotB = hot ? g.nHotB : 0;
+    for (int k = 0; k < 3; ++k) { a.hotLo[k] = g.hotLo[k]; a.hotN[k] = g.hotN[k]; }
+    a.hotBin = g.d_hotBin; a.hotCellList = g.d_hotCellList; a.hotS = g.d_hotS;
+    a.hotBinList = g.d_hotBinList; a.hotPart = g.d_hotPart; a.hotBlocks = g.hotBlocks;
     return a;
 }
 
@@ -993,7 +1075,7 @@ int run_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ct
         MocTransportArgs<double> a = transport_args_f64(ctx, s, n, seed, ctr_base, deltaE);
         rc = moc_launch_transport_f64(ctx, a, &ms);
     } else {
-        if ((rc = refresh_f32(ctx))) return rc;
+        if ((rc = refresh_f32(ctx)) || (rc = ensure_hot(ctx))) return rc;
         MocTransportArgs<float> a = transport_args_f32(ctx, s, n, seed, ctr_base, deltaE);
         rc = moc_launch_transport_f32(ctx, a, &ms);
     }
@@ -1714,6 +1796,7 @@ int capture_iteration(moc_ctx* ctx, uint64_t seed) {
     // what allocates or uploads from the host happens before the capture
     if (!ctx->d_ctr && (rc = dalloc_n(ctx, &ctx->d_ctr, 1, "iteration counter"))) return rc;
     if (!dustOnly && (rc = ensure_balance_grid(ctx))) return rc;
+    if ((rc = ensure_hot(ctx))) return rc;
     int64_t nBatch = 0;
     if (g.nGrids > 1 && (rc = upload_batch_views(ctx, 0, &nBatch))) return rc;
     MOC_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "graph capture");
@@ -1817,14 +1900,16 @@ int moc_set_graph(moc_ctx* ctx, int enable) {
 int moc_set_tuning(moc_ctx* ctx, const moc_tuning* t) {
     if (!ctx) return MOC_ERR_ARG;
     if (!t || t->emitBatch < 1 || t->emitBatch > 64 || t->segsPerTrip < 1 || t->segsPerTrip > 64 ||
-        t->transportWaves < 64 || t->transportWaves > (1 << 20)) {
-        ctx->err = "moc_set_tuning: emitBatch and segsPerTrip in 1..64, transportWaves in 64..2^20";
+        t->transportWaves < 64 || t->transportWaves > (1 << 20) || t->hotJ < 0 || t->hotJ > 1) {
+        ctx->err = "moc_set_tuning: emitBatch and segsPerTrip in 1..64, transportWaves in 64..2^20, "
+                   "hotJ 0 or 1";
         return MOC_ERR_ARG;
     }
     drop_graph(ctx);
     ctx->tune.emitBatch = t->emitBatch;
     ctx->tune.segsPerTrip = t->segsPerTrip;
     ctx->tune.transportWaves = t->transportWaves;
+    ctx->tune.hotJ = t->hotJ;
     return MOC_OK;
 }
 
@@ -1834,6 +1919,7 @@ int moc_get_tuning(moc_ctx* ctx, moc_tuning* t) {
     t->segsPerTrip = ctx->tune.segsPerTrip;
     t->transportWaves = ctx->tune.transportWaves;
     t->graph = ctx->graphOn ? 1 : 0;
+    t->hotJ = ctx->tune.hotJ;
     return MOC_OK;
 }
 

@@ -23,6 +23,8 @@
 // (same bin) in one plane.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "moc_internal.h"
 #include "philox.h"
 
@@ -48,7 +50,32 @@ struct Pk {
     int segs;        // segments of the current pathSegment call
     int kcell;       // cell whose opacity in bin nu0 is held in kap (-1: none)
     T kap;
+    int hb;          // hot-block slot of bin nu0 (-1: none)
 };
+
+// The J tally of one step: into the workgroup's LDS partial of the hot block
+// when both the bin and the cell have a slot there, else a global atomic.
+template <typename T>
+__device__ __forceinline__ void add_j(const MocTransportArgs<T>& a, float* hot, double* Jc, double dJ,
+                                      int hb, int hc) {
+    if (hot && hb >= 0 && hc >= 0) atomicAdd(&hot[hb * a.nHotC + hc], (float)dJ);
+    else atomicAdd(Jc, dJ);
+}
+
+template <typename T>
+__device__ __forceinline__ int hot_bin(const MocTransportArgs<T>& a, const float* hot, int nu0) {
+    return hot ? a.hotBin[nu0] : -1;
+}
+// slot of cell (ix, iy, iz) in the hot box of cells around the star, or -1
+template <typename T>
+__device__ __forceinline__ int hot_cell(const MocTransportArgs<T>& a, const float* hot, int ix, int iy, int iz) {
+    if (!hot) return -1;
+    const int x = ix - a.hotLo[0], y = iy - a.hotLo[1], z = iz - a.hotLo[2];
+    if ((unsigned)x >= (unsigned)a.hotN[0] || (unsigned)y >= (unsigned)a.hotN[1] ||
+        (unsigned)z >= (unsigned)a.hotN[2])
+        return -1;
+    return (x * a.hotN[1] + y) * a.hotN[2] + z;
+}
 
 // The grid a packet moves in (MocGridView): for a single grid it is built
 // once per lane from the launch's (uniform) arguments, with the axes and
@@ -186,7 +213,7 @@ __device__ __forceinline__ int plane_index(const T* ax, int n, T x) {
 // energyPacketRun, then the start of pathSegment (tau = -ln(1-R)).
 template <typename T>
 __device__ __forceinline__ void emit(const MocTransportArgs<T>& a, const View<T>& v, Pk<T>& p,
-                                     MocRng& rng, Cnt& c) {
+                                     MocRng& rng, Cnt& c, const float* hot) {
     ++c.emit;
     ++p.emits;
     int bad = 0;
@@ -246,6 +273,7 @@ __device__ __forceinline__ void emit(const MocTransportArgs<T>& a, const View<T>
     p.state = ST_FLY;
     p.kcell = p.origin;
     p.kap = a.opacity[(size_t)p.nu0 * (size_t)(a.nCells + 1) + (size_t)p.origin];
+    p.hb = hot_bin(a, hot, p.nu0);
 }
 
 // Compton scattering (photon_mod.f90:6805-6844): the KN angle is used as the
@@ -271,7 +299,7 @@ __device__ __forceinline__ void compton(const MocTransportArgs<T>& a, Pk<T>& p, 
 // step's wait for that opacity does not include the atomic's round trip.
 template <typename T>
 __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const View<T>& g, Pk<T>& p,
-                                        MocRng& rng, Cnt& c) {
+                                        MocRng& rng, Cnt& c, float* hot) {
     if (p.segs >= (g.plane ? MOC_SAFE_LIMIT_PLANE : MOC_SAFE_LIMIT)) {
         ++c.lost;
         p.state = ST_DONE;
@@ -329,6 +357,7 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const View
     const int cell = cell_of(g, p.ix, p.iy, p.iz);
     T kap = kap0;
     if (cell != cell0) kap = a.opacity[row + (size_t)cell];
+    const int hc = hot_cell(a, hot, p.ix, p.iy, p.iz);  // the hot slot of this cell
     const T tauCell = dS * kap;
     const T dV = g.vx[p.ix] * g.vy[p.iy] * g.vz[p.iz];
     double* const Jc = a.J + row + (size_t)cell;
@@ -350,7 +379,7 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const View
             const T sx = p.x / T(1.e10), sy = p.y / T(1.e10), sz = p.z / T(1.e10);
             const T rr = sqrt(sx * sx + sy * sy + sz * sz) * T(1.e10);
             if (rr >= g.R_out) {
-                atomicAdd(Jc, dJ);
+                add_j(a, hot, Jc, dJ, p.hb, hc);
                 tally_escape(a, g, p, c);
                 p.state = ST_DONE;
                 return;
@@ -370,10 +399,10 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const View
                 p.passProb = -log(T(1) - rng.uniform<T>());
                 p.kcell = cell;
                 p.kap = kap;
-                atomicAdd(Jc, dJ);
+                add_j(a, hot, Jc, dJ, p.hb, hc);
                 return;
             }
-            atomicAdd(Jc, dJ);
+            add_j(a, hot, Jc, dJ, p.hb, hc);
             ++c.absb;
             if (p.emits >= MOC_RECURSION_LIMIT) {
                 ++c.trap;
@@ -395,10 +424,11 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const View
             p.passProb = -log(T(1) - rng.uniform<T>());
             p.kcell = cell;
             p.kap = a.opacity[(size_t)p.nu0 * (size_t)(a.nCells + 1) + (size_t)cell];
-            atomicAdd(Jc, dJ);
+            add_j(a, hot, Jc, dJ, p.hb, hc);  // the step was taken in the old bin
+            p.hb = hot_bin(a, hot, p.nu0);
             return;
         }
-        atomicAdd(Jc, dJ);
+        add_j(a, hot, Jc, dJ, p.hb, hc);
         // absorbed: re-emitted as a diffuse packet from this position
         ++c.absb;
         if (p.emits >= MOC_RECURSION_LIMIT) {
@@ -458,7 +488,7 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const View
         p.kcell = cell_of(g, p.ix, p.iy, p.iz);
         p.kap = a.opacity[row + (size_t)p.kcell];
     }
-    if (cell > 0) atomicAdd(Jc, dJ);
+    if (cell > 0) add_j(a, hot, Jc, dJ, p.hb, hc);
     if (esc) {
         tally_escape(a, g, p, c);
         p.state = ST_DONE;
@@ -488,10 +518,14 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 extern __shared__ __align__(16) unsigned char transport_lds[];
 
-// BATCH: the packets of a batch of grids (a.nGrids > 0), each in its own grid
-template <typename T, bool BATCH>
-__global__ void __launch_bounds__(256) k_transport(MocTransportArgs<T> a) {
+// BATCH: the packets of a batch of grids (a.nGrids > 0), each in its own grid.
+// HOT: workgroups of 1024 lanes (one per CU) sum the tallies of the hot block
+// of J keys (a.nHotC x a.nHotB) as FP32 partials in LDS, after the staged
+// geometry, and write them to their row of a.hotPart at the end.
+template <typename T, bool BATCH, bool HOT>
+__global__ void __launch_bounds__(HOT ? 1024 : 256) k_transport(MocTransportArgs<T> a) {
     View<T> g;
+    float* hot = nullptr;
     if constexpr (!BATCH) {
         // stage the axes and volume factors: [ax ay az vx vy vz]
         T* const sg = reinterpret_cast<T*>(transport_lds);
@@ -501,6 +535,10 @@ __global__ void __launch_bounds__(256) k_transport(MocTransportArgs<T> a) {
             const int k = bx ? i : (by ? i - nx : i - nx - ny);
             sg[i] = bx ? a.ax[k] : (by ? a.ay[k] : a.az[k]);
             sg[nt + i] = bx ? a.vx[k] : (by ? a.vy[k] : a.vz[k]);
+        }
+        if constexpr (HOT) {
+            hot = reinterpret_cast<float*>(sg + 2 * nt);
+            for (int i = threadIdx.x; i < a.nHotB * a.nHotC; i += blockDim.x) hot[i] = 0.f;
         }
         __syncthreads();
         g.ax = sg; g.ay = sg + nx; g.az = sg + nx + ny;
@@ -515,6 +553,7 @@ __global__ void __launch_bounds__(256) k_transport(MocTransportArgs<T> a) {
     Pk<T> p;
     p.state = ST_DONE;
     p.kcell = -1;
+    p.hb = -1;
     MocRng rng;
     bool more = true;
     const int lane = threadIdx.x & 63;
@@ -568,11 +607,16 @@ __global__ void __launch_bounds__(256) k_transport(MocTransportArgs<T> a) {
         const unsigned long long flyers = __ballot(p.state == ST_FLY);
         if (!emitters && !flyers && !__ballot(more)) break;
         if (emitters && (__popcll(emitters) >= emitBatch || !flyers || !__ballot(more))) {
-            if (p.state == ST_EMIT) emit(a, g, p, rng, c);
+            if (p.state == ST_EMIT) emit(a, g, p, rng, c, hot);
         }
 #pragma unroll 1
         for (int k = 0; k < segsPerTrip; ++k)
-            if (p.state == ST_FLY) segment(a, g, p, rng, c);
+            if (p.state == ST_FLY) segment(a, g, p, rng, c, hot);
+    }
+    if constexpr (HOT) {  // every wave of the workgroup is done: its partials out
+        __syncthreads();
+        float* out = a.hotPart + (size_t)blockIdx.x * (size_t)(a.nHotB * a.nHotC);
+        for (int i = threadIdx.x; i < a.nHotB * a.nHotC; i += blockDim.x) out[i] = hot[i];
     }
     // all lanes of the wave have left the loop: reduce the counters once
     unsigned long long* cnt = a.counters;
@@ -627,8 +671,66 @@ __global__ void k_copy_f64(const double* __restrict__ src, double* __restrict__ 
     for (int64_t k = i; k < n; k += st) dst[k] = src[k];
 }
 
+// The hot block's partial sums of every workgroup added to J: one key per
+// lane, the rows summed in FP64 in workgroup order (deterministic for given
+// partials); each key belongs to one slot, so the read-modify-write of J is
+// plain.
+__global__ void k_hot_reduce(const float* __restrict__ part, int rows, int nHotC, int nHotB,
+                             const int* __restrict__ cellList, const int* __restrict__ binList,
+                             double* __restrict__ J, int64_t C1) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nHot = nHotC * nHotB;
+    if (k >= nHot) return;
+    const int bin = binList[k / nHotC], cell = cellList[k % nHotC];
+    if (bin < 0 || cell <= 0) return;
+    double sum = 0.;
+    for (int r = 0; r < rows; ++r) sum += (double)part[(size_t)r * nHot + k];
+    if (sum != 0.) J[(size_t)bin * C1 + cell] += sum;
+}
+
+// The hot bins of the next launch: the nHotB bins with the most J (path per
+// volume) summed over the hot cells, ranked with ties to the lower bin.
+// k_hot_binsum: one workgroup per bin, the cells over the lanes (summed in a
+// fixed order); k_hot_select: one workgroup ranks the B sums, B <= kHotMaxBins.
+constexpr int kHotMaxBins = 8192;
+__global__ void __launch_bounds__(256) k_hot_binsum(const double* __restrict__ J, int64_t C1,
+                                                    const int* __restrict__ cellList, int nHotC,
+                                                    double* __restrict__ S) {
+    __shared__ double w[256];
+    const int b = blockIdx.x;
+    double v = 0.;
+    for (int k = threadIdx.x; k < nHotC; k += 256) {
+        const int cell = cellList[k];
+        if (cell > 0) v += J[(size_t)b * C1 + cell];
+    }
+    w[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) w[threadIdx.x] += w[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) S[b] = w[0];
+}
+
+__global__ void __launch_bounds__(1024) k_hot_select(const double* __restrict__ Sg, int B, int nHotB,
+                                                     int* __restrict__ binMap, int* __restrict__ binList) {
+    __shared__ double S[kHotMaxBins];
+    for (int b = threadIdx.x; b < B; b += blockDim.x) S[b] = Sg[b];
+    __syncthreads();
+    for (int b = threadIdx.x; b < B; b += blockDim.x) {
+        const double v = S[b];
+        int rank = 0;
+        for (int q = 0; q < B; ++q) rank += (S[q] > v || (S[q] == v && q < b)) ? 1 : 0;
+        const bool in = rank < nHotB;
+        binMap[b] = in ? rank : -1;
+        if (in) binList[rank] = b;
+    }
+}
+
 // LDS for the staged geometry of one workgroup (2 x (nx+ny+nz) values)
 constexpr size_t kMaxGeoLds = 32768;
+// workgroup of the transport with a hot J block: 16 waves, one workgroup per CU
+constexpr int kHotBlock = 1024;
 
 // ms == nullptr: enqueue only (no events, no wait), as inside a graph capture
 template <typename T>
@@ -641,24 +743,44 @@ int launch(moc_ctx* ctx, const MocTransportArgs<T>& a, float* ms) {
         ctx->err = "transport: tuning parameters out of range (emit batch 1..64, segments per trip 1..64)";
         return MOC_ERR_ARG;
     }
+    constexpr bool kF32 = std::is_same<T, float>::value;
+    const bool hot = kF32 && a.nGrids == 0 && a.nHotC > 0 && a.nHotB > 0;
     // the resident wave count (a tuning parameter; the default fills the 256
     // CUs several times over); lanes beyond the packet count exit at their
     // first fetch
     const int64_t lanes_max = 64LL * (int64_t)ctx->tune.transportWaves;
     int64_t lanes = a.n < lanes_max ? a.n : lanes_max;
-    const int block = 256;
+    const int block = hot ? kHotBlock : 256;
     const int blocks = (int)((lanes + block - 1) / block);
-    const size_t lds = a.nGrids > 0 ? 0 : 2 * (size_t)(a.nx + a.ny + a.nz) * sizeof(T);
-    if (lds > kMaxGeoLds) {
+    const size_t geo = a.nGrids > 0 ? 0 : 2 * (size_t)(a.nx + a.ny + a.nz) * sizeof(T);
+    if (geo > kMaxGeoLds) {
         ctx->err = "transport: the grid axes (" + std::to_string(a.nx + a.ny + a.nz) +
                    " points) exceed the " + std::to_string(kMaxGeoLds) + " B LDS staging area";
         return MOC_ERR_ARG;
     }
+    const size_t lds = hot ? geo + (size_t)a.nHotC * a.nHotB * sizeof(float) : geo;
+    if (hot && (blocks > a.hotBlocks || a.nHotB > kHotMaxBins || a.nbins > kHotMaxBins)) {
+        ctx->err = "transport: the hot J block's partial buffer does not fit this launch";
+        return MOC_ERR_STATE;
+    }
     if (ms) (void)hipEventRecord(ctx->ev0, ctx->stream);
     if (a.nGrids > 0)
-        hipLaunchKernelGGL((k_transport<T, true>), dim3(blocks), dim3(block), lds, ctx->stream, a);
+        hipLaunchKernelGGL((k_transport<T, true, false>), dim3(blocks), dim3(block), lds, ctx->stream, a);
+    else if (hot)
+        hipLaunchKernelGGL((k_transport<T, false, kF32>), dim3(blocks), dim3(block), lds, ctx->stream, a);
     else
-        hipLaunchKernelGGL((k_transport<T, false>), dim3(blocks), dim3(block), lds, ctx->stream, a);
+        hipLaunchKernelGGL((k_transport<T, false, false>), dim3(blocks), dim3(block), lds, ctx->stream, a);
+    if (hot) {
+        // the partials into J, then the hot bins of the next launch from this J
+        const int nHot = a.nHotC * a.nHotB;
+        hipLaunchKernelGGL(k_hot_reduce, dim3((unsigned)((nHot + 255) / 256)), dim3(256), 0, ctx->stream,
+                           a.hotPart, blocks, a.nHotC, a.nHotB, a.hotCellList, a.hotBinList, a.J,
+                           (int64_t)a.nCells + 1);
+        hipLaunchKernelGGL(k_hot_binsum, dim3((unsigned)a.nbins), dim3(256), 0, ctx->stream, a.J,
+                           (int64_t)a.nCells + 1, a.hotCellList, a.nHotC, a.hotS);
+        hipLaunchKernelGGL(k_hot_select, dim3(1), dim3(1024), 0, ctx->stream, a.hotS, a.nbins, a.nHotB,
+                           const_cast<int*>(a.hotBin), const_cast<int*>(a.hotBinList));
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         ctx->err = std::string("transport launch: ") + hipGetErrorString(e);
@@ -682,6 +804,10 @@ int moc_launch_transport_f64(moc_ctx* ctx, const MocTransportArgs<double>& a, fl
 }
 int moc_launch_transport_f32(moc_ctx* ctx, const MocTransportArgs<float>& a, float* ms) {
     return launch<float>(ctx, a, ms);
+}
+int moc_transport_hot_block() { return kHotBlock; }
+size_t moc_transport_hot_lds(int nx, int ny, int nz, int nHot) {
+    return 2 * (size_t)(nx + ny + nz) * sizeof(float) + (size_t)nHot * sizeof(float);
 }
 void moc_convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t s) {
     if (n <= 0) return;

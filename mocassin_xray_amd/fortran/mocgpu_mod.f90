@@ -116,7 +116,7 @@ module mocgpu_mod
 
   ! launch parameters of the transport (moc_set_tuning); graph is set by moc_get_tuning
   type, bind(C), public :: moc_tuning
-     integer(c_int32_t) :: emitBatch, segsPerTrip, transportWaves, graph
+     integer(c_int32_t) :: emitBatch, segsPerTrip, transportWaves, graph, hotJ
   end type moc_tuning
 
   interface

@@ -482,14 +482,14 @@ class MocGPU:
         return t.as_dict()
 
     def set_tuning(self, emitBatch: int | None = None, segsPerTrip: int | None = None,
-                   transportWaves: int | None = None) -> None:
+                   transportWaves: int | None = None, hotJ: int | None = None) -> None:
         """Set some of the transport's launch parameters (moc_set_tuning); the
         others keep their values.  They change no packet's random stream."""
         cur = self.get_tuning()
         t = MocTuning(int(emitBatch if emitBatch is not None else cur["emitBatch"]),
                       int(segsPerTrip if segsPerTrip is not None else cur["segsPerTrip"]),
                       int(transportWaves if transportWaves is not None else cur["transportWaves"]),
-                      0)
+                      0, int(hotJ if hotJ is not None else cur["hotJ"]))
         self._check(self._L.moc_set_tuning(self._ctx, C.byref(t)), "moc_set_tuning")
 
     def MCIterationDriver(self, maxIterateMC: int, minConvergence: float = 100.0,
