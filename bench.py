@@ -176,6 +176,17 @@ def state_error(m, a: dict, b: dict) -> dict:
     return out
 
 
+def atomic_calibration(nbytes: int = 1 << 28) -> float | None:
+    """The device's rate of scattered no-return FP64 atomic additions
+    (moc_calibrate_atomics) into a buffer the size of a J plane."""
+    import ctypes as C
+    from mocassin_xray_amd import _lib
+    rate = C.c_double(0.0)
+    if _lib.lib().moc_calibrate_atomics(nbytes, C.byref(rate)) != 0:
+        return None
+    return rate.value
+
+
 def copy_calibration(nbytes: int = 1 << 30, reps: int = 3):
     """Plain device copy of known size: FETCH_SIZE/WRITE_SIZE of this kernel
     calibrate the counters (bytes per KiB reported)."""
@@ -404,6 +415,11 @@ def main():
     bytes_mix = transport_bytes(mean(st_mix, "segments"), mean(st_mix, "emissions"),
                                 mean(st_mix, "escaped"), m.tables.nbins, 4, dense)
     achieved = bytes_mix / (kms * 1e-3) / 1e9
+    # the transport's other bound: one J tally per path segment, each a
+    # scattered FP64 atomic addition unless the LDS hot block takes it; the
+    # device's rate of such additions is measured here, live
+    atomic_rate = atomic_calibration()
+    tallies_per_s = mean(st_mix, "segments") / (kms * 1e-3)
     out = {
         "metric": "packets/s of the Monte Carlo iteration (mixed precision) + balance cells/s",
         "value": args.steps * n / dt_mix,
@@ -433,7 +449,7 @@ def main():
         "output_ms": s_mix.get("output_ms"),
         "graph_replay": args.graph,
         "transport_launch": {k: s_mix["tuning"][k] for k in ("emitBatch", "segsPerTrip",
-                                                             "transportWaves")},
+                                                             "transportWaves", "hotJ")},
         "mixed_over_fp64": dt_64 / dt_mix,
         "mixed_vs_fp64_error": state_error(m, s_mix, s_64),
         "mc_noise_band": band,
@@ -444,7 +460,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profiled(args.workload, "traffic_bytes_per_launch"),
                      "l2_hit_rate": profiled(args.workload, "l2_hit_rate"),
-                     "kernel": "k_transport<float>", "kernel_ms": kms},
+                     "kernel": "k_transport<float>", "kernel_ms": kms,
+                     "tally_bound": {"tallies_per_s": tallies_per_s,
+                                     "device_atomic_adds_per_s": atomic_rate,
+                                     "frac": tallies_per_s / atomic_rate if atomic_rate else None}},
     }
     if band is not None:
         err = out["mixed_vs_fp64_error"]

@@ -286,6 +286,18 @@ typedef struct moc_dust_tables {
     const double* ipRyd;         /* [31*32] (Z, stage >= 2): the energy [Ryd] released when the
                                     stage recombines, nu of the first bin of the outer shell of
                                     stage-1 (setDustGasCollHeatCool :796-806) */
+    /* quantum heating (qHeat, emission_mod.f90:1993-2504): grains smaller than minaQHeat
+     * emit from their temperature distribution once more than minConvQHeat per cent of
+     * the cells have converged, from the second iteration on (:1701-1702, :1946-1947);
+     * needs grainRadius */
+    int32_t lgQHeat;
+    int32_t nTbins;              /* temperature bins of a distribution, >= 11 (300) */
+    double minaQHeat;            /* um (1e-3) */
+    double minConvQHeat;         /* per cent (99) */
+    double Tmax;                 /* K, the top bin (700) */
+    const int32_t* grainKind;    /* [nSpecies] the first letter of the species' label:
+                                    1 'S' (silicate), 2 'C' (carbon), 0 other (no
+                                    distribution: the grain then emits nothing) */
 } moc_dust_tables;
 
 /* Create a context: uploads the mesh and source tables. */
@@ -484,6 +496,21 @@ int moc_device_ptr(moc_ctx* ctx, int field, int ig, void** dptr, int64_t* count,
 
 /* Library self-description: "gfx950 mocgpu <version>". */
 const char* moc_version(void);
+
+/* Calibration of the transport's bound (no counterpart in the reference: it
+ * measures the device): the rate [additions/s] of no-return FP64 atomic adds
+ * at scattered addresses of a buffer of `bytes` (a power of two, 1 MiB ..
+ * 4 GiB) -- the operation of every J tally.  Allocates and frees the buffer;
+ * needs no context. */
+int moc_calibrate_atomics(uint64_t bytes, double* rate);
+
+/* The iteration state that switches quantum heating on (lgQHeat): the
+ * emission of the next moc_dust_emission / moc_emission uses the grains'
+ * temperature distributions when nIterateMC > 1 and convPercent (of the last
+ * iteration) > minConvQHeat (emission_mod.f90:1701-1702).  moc_iterate sets
+ * it itself; this is for callers that run the phases one by one.  Replaces
+ * the module variables nIterateMC and convPercent that qHeat's callers read. */
+int moc_set_qheat_state(moc_ctx* ctx, int nIterateMC, double convPercent);
 
 #ifdef __cplusplus
 }

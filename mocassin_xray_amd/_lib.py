@@ -38,6 +38,7 @@ EXPORTS = [
     "moc_set_dust", "moc_dust_opacity", "moc_dust_emission", "moc_dust_temperature",
     "moc_grid_attach_batch", "moc_volume_averages", "moc_sed", "moc_line_luminosities",
     "moc_volume_averages_noalloc", "moc_set_graph", "moc_set_tuning", "moc_get_tuning",
+    "moc_calibrate_atomics", "moc_set_qheat_state",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -111,7 +112,9 @@ class MocDustTables(C.Structure):
         ("lgGas", C.c_int32), ("lgPhotoelectric", C.c_int32), ("grainAbun", _dp),
         ("grainWeight", _dp), ("TdustSublime", _dp), ("Cabs", _dp), ("Csca", _dp),
         ("emIntegral", _dp), ("XHILimit", C.c_double), ("grainRadius", _dp), ("grainVn", _dp),
-        ("MsurfAtom", _dp), ("ipRyd", _dp),
+        ("MsurfAtom", _dp), ("ipRyd", _dp), ("lgQHeat", C.c_int32), ("nTbins", C.c_int32),
+        ("minaQHeat", C.c_double), ("minConvQHeat", C.c_double), ("Tmax", C.c_double),
+        ("grainKind", _ip),
     ]
 
 
@@ -207,6 +210,10 @@ def lib() -> C.CDLL:
     L.moc_get_tuning.restype = C.c_int
     L.moc_version.argtypes = []
     L.moc_version.restype = C.c_char_p
+    L.moc_calibrate_atomics.argtypes = [C.c_uint64, _dp]
+    L.moc_calibrate_atomics.restype = C.c_int
+    L.moc_set_qheat_state.argtypes = [vp, C.c_int, C.c_double]
+    L.moc_set_qheat_state.restype = C.c_int
     _LIB = L
     return L
 

@@ -19,7 +19,7 @@ LIB = PKG / "libmocgpu.so"
 FORTRAN = PKG / "fortran"
 
 HIP_SOURCES = ["mocgpu.hip", "transport.hip", "opacity.hip", "balance.hip", "emission.hip", "dust.hip",
-               "output.hip"]
+               "output.hip", "calib.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -241,6 +241,7 @@ __device__ double hei_line(const MocEmissionArgs& a, int i, double Te, double Ne
 }
 
 struct CellCtx {
+    int cell;
     double Te, Ne, sqrTe;
     double abH, XH2, abHe, XHe2, XHe3;
     bool heOn;
@@ -343,8 +344,16 @@ __device__ double bin_energy(const MocEmissionArgs& a, const CellCtx& c, const d
             for (int s = 0; s < a.nDS; ++s) {
                 const double T = sTd[s * a.nDA + ai];
                 if (!(T < a.dTsub[s])) continue;
-                acc = acc + a.dCabs[((size_t)s * a.nDA + ai) * a.B + (i - 1)] * moc_planck_h(nu, T) * wf *
-                                a.dWeight[ai] * a.dAbun[s];
+                const double cab = a.dCabs[((size_t)s * a.nDA + ai) * a.B + (i - 1)];
+                const int q = a.qActive ? a.qIdx[s * a.nDA + ai] : -1;
+                if (q >= 0) {  // quantum heating: the grain's temperature distribution
+                    const double* qt = a.qT + ((size_t)c.cell * a.nQ + q) * a.nTb;
+                    const double* qp = a.qP + ((size_t)c.cell * a.nQ + q) * a.nTb;
+                    for (int iT = 0; iT < a.nTb; ++iT)
+                        acc = acc + cab * moc_planck_h(nu, qt[iT]) * wf * a.dWeight[ai] * a.dAbun[s] * qp[iT];
+                } else {
+                    acc = acc + cab * moc_planck_h(nu, T) * wf * a.dWeight[ai] * a.dAbun[s];
+                }
             }
         sDust = acc * c.dustScale;
     }
@@ -430,6 +439,7 @@ k_emission(MocEmissionArgs a) {
     c.g1 = gaunt_setup(0., log10(Te));
     c.g2 = gaunt_setup(0.30103, log10(Te));
     c.tp = two_phot_setup(Te);
+    c.cell = cell;
     c.dustScale = 0.;
     if (a.nDS > 0) {
         for (int k = tid; k < a.nDS * a.nDA; k += kThreads) {

@@ -276,6 +276,11 @@ struct MocEmissionArgs {
     const double* Tdust;     // [C+1][nDA+1][nDS+1]
     const double* Ndust;     // [C+1]
     const double* Hden;      // [C+1]
+    // quantum heating of the grains (qActive: qIdx, qT, qP as in MocDustArgs)
+    int qActive, nQ, nTb;
+    const int* qIdx;
+    const double* qT;
+    const double* qP;
     const double* lg[3];     // [ntk][nlim] log10 gamma
     int HIP, HeIP, HeIIP;    // threshold bins (He: 0 when off)
     const double* xsHI;
@@ -345,6 +350,13 @@ struct MocDust {
     double* d_W = nullptr;      // [B][nS*nA+1] Cabs/pi and the hit column
     double* d_R = nullptr;      // [C+1][nS*nA+1] absorption integrals, a row per cell
     unsigned long long* d_counters = nullptr;  // [2]
+    // quantum heating (qHeat): the grains below minaQHeat, their distributions
+    int lgQHeat = 0, nTb = 0, nQ = 0;
+    double minaQHeat = 0., minConvQHeat = 0., Tmax = 0.;
+    bool qActive = false;       // this iteration's emission uses the distributions
+    int* d_kind = nullptr;      // [nS] 1 silicate, 2 carbon, 0 other
+    int* d_qIdx = nullptr;      // [nS*nA] slot of a grain among the nQ, or -1
+    int* d_qG = nullptr;        // [nQ] grain (s * nA + ai, 0-based) of each slot
 };
 
 // Dust kernel view (dust.hip)
@@ -372,6 +384,19 @@ struct MocDustArgs {
     int* lgBlack;
     unsigned long long* counters;  // noHit, converged
     double XHILimit;
+    // quantum heating: with qActive, grain g with qIdx[g] = q >= 0 emits from
+    // its temperature distribution qT/qP [C+1][nQ][nTb] instead of its Tdust
+    int qActive, nQ, nTb;
+    const int* qIdx;        // [nS*nA]
+    const int* qG;          // [nQ]
+    const int* kind;        // [nS]
+    const double* radius;   // [nA] um
+    double Tmax;
+    const double* J;        // [B][C+1] normalised J
+    double* qT;             // [C+1][nQ][nTb]
+    double* qP;             // [C+1][nQ][nTb]
+    double* qWork;          // [qBlocks][nTb][nTb]
+    int qBlocks;
 };
 
 // getFlux('blackbody') (continuum_mod.f90:358-400): the Planck function over
@@ -430,6 +455,9 @@ struct MocGrid {
     // the FP32 plane holds the current opacity / diffuse CDF and the FP64 one
     // is brought up to date before the host reads or writes it
     bool op32_master = false, pdf32_master = false;
+    // quantum heating distributions and the solver's per-workgroup work rows
+    double *d_qT = nullptr, *d_qP = nullptr, *d_qWork = nullptr;
+    int qBlocks = 0;
     // hot block of the J tallies (MocTransportArgs::nHotC): the maps and the
     // per-workgroup partial sums, set up at the first mixed transport
     int nHotC = 0, nHotB = 0, hotBlocks = 0;
@@ -494,6 +522,7 @@ struct moc_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
     uint64_t graphSeed = 0;
+    double lastConvPercent = 0.;  // converged cells of the last moc_iterate [%] (quantum heating)
     // planes the captured iteration writes in FP32 (their FP64 copies go stale at each replay)
     bool graphOp32 = false, graphPdf32 = false;
     uint32_t* d_ctr = nullptr;
@@ -535,6 +564,8 @@ int moc_launch_opacity(moc_ctx* ctx, const MocOpacityArgs& a);
 int moc_launch_emission(moc_ctx* ctx, const MocEmissionArgs& a);
 int moc_launch_dust_opacity(moc_ctx* ctx, const MocDustArgs& a);
 int moc_launch_dust_pdf(moc_ctx* ctx, const MocDustArgs& a);
+int moc_launch_qheat(moc_ctx* ctx, const MocDustArgs& a);
+size_t moc_qheat_lds(int B, int nTb);
 int moc_launch_dust_temp(moc_ctx* ctx, const MocDustArgs& a, const double* J, const double* W, int ncol,
                          double* R);
 // R[col][cell] = sum_nu J[nu][cell] W[nu][col] (k_rates, balance.hip)

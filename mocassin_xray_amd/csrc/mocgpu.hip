@@ -106,6 +106,7 @@ void free_grid(MocGrid& g) {
     dfree(g.d_gasOpac); dfree(g.f_scaOpac); dfree(g.d_collD); dfree(g.d_peD);
     dfree(g.d_hotBin); dfree(g.d_hotCellList); dfree(g.d_hotBinList);
     dfree(g.d_hotPart); dfree(g.d_hotS);
+    dfree(g.d_qT); dfree(g.d_qP); dfree(g.d_qWork);
     g = MocGrid();
 }
 
@@ -701,7 +702,19 @@ MocDustArgs dust_args(moc_ctx* ctx) {
     a.lgConverged = g.d_conv; a.lgBlack = g.d_black;
     a.counters = d.d_counters;
     a.XHILimit = d.XHILimit;
+    a.qActive = d.lgQHeat && d.qActive && d.nQ > 0 ? 1 : 0;
+    a.nQ = d.nQ; a.nTb = d.nTb; a.qIdx = d.d_qIdx; a.qG = d.d_qG; a.kind = d.d_kind;
+    a.radius = d.d_radius; a.Tmax = d.Tmax; a.J = g.d_J;
+    a.qT = g.d_qT; a.qP = g.d_qP; a.qWork = g.d_qWork; a.qBlocks = g.qBlocks;
     return a;
+}
+
+// the grains' temperature distributions for this iteration's emission, when
+// quantum heating is on for it
+int launch_qheat(moc_ctx* ctx) {
+    const MocDustArgs a = dust_args(ctx);
+    if (!a.qActive) return MOC_OK;
+    return moc_launch_qheat(ctx, a);
 }
 
 int launch_dust_opacity(moc_ctx* ctx) {
@@ -914,6 +927,8 @@ MocTransportArgs<double> transport_args_f64(moc_ctx* ctx, int s, int64_t n, uint
 // the launch at the current wave count.  Allocates, so it is called before
 // any graph capture.
 constexpr int kHotBins = 256;
+// workgroups (and work rows of nTbins^2 doubles) of the quantum-heating solver
+constexpr int kQHeatBlocks = 512;
 int ensure_hot(moc_ctx* ctx) {
     MocGrid& g = ctx->grid;
     const int B = ctx->nbins;
@@ -1679,6 +1694,8 @@ MocEmissionArgs emission_args(moc_ctx* ctx, bool skipUnchanged) {
         a.nDS = d.nS; a.nDA = d.nA;
         a.dAbun = d.d_abun; a.dWeight = d.d_weight; a.dTsub = d.d_tsub; a.dCabs = d.d_Cabs;
         a.Tdust = g.d_Tdust; a.Ndust = g.d_Ndust; a.Hden = g.d_Hden;
+        a.qActive = d.lgQHeat && d.qActive && d.nQ > 0 ? 1 : 0;
+        a.nQ = d.nQ; a.nTb = d.nTb; a.qIdx = d.d_qIdx; a.qT = g.d_qT; a.qP = g.d_qP;
     }
     return a;
 }
@@ -1688,6 +1705,7 @@ int run_emission(moc_ctx* ctx, bool skipUnchanged, float* kernel_ms) {
     const MocEmissionArgs a = emission_args(ctx, skipUnchanged);
     int rc;
     if (!a.recPDF32 && (rc = sync_f64_master(ctx, MOC_F_RECPDF))) return rc;
+    if (ctx->dust.set && ctx->dust.lgGas && (rc = launch_qheat(ctx))) return rc;
     MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
     rc = moc_launch_emission(ctx, a);
     if (rc) return rc;
@@ -1942,7 +1960,12 @@ int moc_iterate(moc_ctx* ctx, int ig, int nIterateMC, uint64_t seed, moc_iterati
         return MOC_ERR_STATE;
     }
     if (nIterateMC < 1 || nIterateMC > (1 << 23)) { ctx->err = "moc_iterate: nIterateMC"; return MOC_ERR_ARG; }
-    if (ctx->graphOn && nIterateMC > 1) return iterate_graph(ctx, nIterateMC, seed, stats);
+    // quantum heating from the second iteration on, once enough cells have
+    // converged (setDiffusePDF :1701-1702, setDustPDF :1946-1947); those
+    // iterations run eagerly (the switch is a host decision per iteration)
+    const bool qheat = ctx->dust.set && ctx->dust.lgQHeat;
+    if (qheat) ctx->dust.qActive = nIterateMC > 1 && ctx->lastConvPercent > ctx->dust.minConvQHeat;
+    if (ctx->graphOn && nIterateMC > 1 && !qheat) return iterate_graph(ctx, nIterateMC, seed, stats);
     const bool skip = nIterateMC > 1;
     float tOp = 0.f, tEm = 0.f;
     int rc;
@@ -1954,7 +1977,7 @@ int moc_iterate(moc_ctx* ctx, int ig, int nIterateMC, uint64_t seed, moc_iterati
         MOC_TRY(hipEventSynchronize(ctx->ev1), "dust opacity");
         (void)hipEventElapsedTime(&tOp, ctx->ev0, ctx->ev1);
         MOC_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
-        if ((rc = moc_launch_dust_pdf(ctx, dust_args(ctx)))) return rc;
+        if ((rc = launch_qheat(ctx)) || (rc = moc_launch_dust_pdf(ctx, dust_args(ctx)))) return rc;
         MOC_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
         MOC_TRY(hipEventSynchronize(ctx->ev1), "dust emission");
         (void)hipEventElapsedTime(&tEm, ctx->ev0, ctx->ev1);
@@ -1989,6 +2012,7 @@ int moc_iterate(moc_ctx* ctx, int ig, int nIterateMC, uint64_t seed, moc_iterati
     if ((rc = moc_normalize_j(ctx, ig))) return rc;
     moc_update_stats us{};
     if ((rc = moc_update(ctx, ig, &us))) return rc;
+    ctx->lastConvPercent = 100.0 * us.converged / (g.nCells > 0 ? (double)g.nCells : 1.0);
     if (stats) {
         const double nc = g.nCells > 0 ? (double)g.nCells : 1.0;
         stats->convPercent = 100.0 * us.converged / nc;
@@ -2020,6 +2044,12 @@ int moc_set_dust(moc_ctx* ctx, const moc_dust_tables* t) {
     if (!t || t->nSpecies < 1 || t->nSizes < 1 || t->nTemps < 2 || !t->grainAbun || !t->grainWeight ||
         !t->TdustSublime || !t->Cabs || !t->Csca || !t->emIntegral || (t->lgGas != 0 && t->lgGas != 1)) {
         ctx->err = "moc_set_dust: invalid grain tables";
+        return MOC_ERR_ARG;
+    }
+    if (t->lgQHeat && (!t->grainRadius || !t->grainKind || t->nTbins < 11 || t->nTbins > 2048 ||
+                       !(t->Tmax > 1.) || !(t->minaQHeat >= 0.))) {
+        ctx->err = "moc_set_dust: quantum heating needs the grain radii and kinds, 11..2048 "
+                   "temperature bins and Tmax > 1 K";
         return MOC_ERR_ARG;
     }
     if (t->lgPhotoelectric && (!t->lgGas || !ctx->balance.set || !t->grainRadius || !t->grainVn ||
@@ -2056,9 +2086,43 @@ int moc_set_dust(moc_ctx* ctx, const moc_dust_tables* t) {
         (rc = dust_put(ctx, (const double*)nullptr, (size_t)(nG + 1) * C1, &d.d_R, "dust R")) ||
         (rc = dust_put(ctx, (const unsigned long long*)nullptr, 2, &d.d_counters, "dust counters")))
         return rc;
+    d.lgQHeat = t->lgQHeat ? 1 : 0;
+    if (d.lgQHeat) {
+        // the grains below minaQHeat, species outer and sizes inner
+        std::vector<int> qIdx((size_t)nG, -1), qG;
+        for (int s = 0; s < nS; ++s)
+            for (int ai = 0; ai < nA; ++ai)
+                if (t->grainRadius[ai] < t->minaQHeat) {
+                    qIdx[(size_t)s * nA + ai] = (int)qG.size();
+                    qG.push_back(s * nA + ai);
+                }
+        d.nQ = (int)qG.size();
+        d.nTb = t->nTbins;
+        d.minaQHeat = t->minaQHeat;
+        d.minConvQHeat = t->minConvQHeat;
+        d.Tmax = t->Tmax;
+        if ((rc = dust_put(ctx, qIdx.data(), qIdx.size(), &d.d_qIdx, "qHeat grains")) ||
+            (rc = dust_put(ctx, qG.data(), qG.size(), &d.d_qG, "qHeat grains")) ||
+            (rc = dust_put(ctx, t->grainKind, (size_t)nS, &d.d_kind, "grainKind")) ||
+            (rc = dust_put(ctx, t->grainRadius, (size_t)nA, &d.d_radius, "grainRadius")))
+            return rc;
+        // the distributions of every cell, and the solver's work rows
+        dfree(g.d_qT); dfree(g.d_qP); dfree(g.d_qWork);
+        g.qBlocks = 0;
+        if (d.nQ > 0) {
+            const int blocks = kQHeatBlocks;
+            if ((rc = dalloc_n(ctx, &g.d_qT, C1 * d.nQ * d.nTb, "qHeat T")) ||
+                (rc = dalloc_n(ctx, &g.d_qP, C1 * d.nQ * d.nTb, "qHeat P")) ||
+                (rc = dalloc_n(ctx, &g.d_qWork, (size_t)blocks * d.nTb * d.nTb, "qHeat work")))
+                return rc;
+            MOC_TRY(hipMemsetAsync(g.d_qT, 0, C1 * d.nQ * d.nTb * 8, ctx->stream), "zero");
+            MOC_TRY(hipMemsetAsync(g.d_qP, 0, C1 * d.nQ * d.nTb * 8, ctx->stream), "zero");
+            g.qBlocks = blocks;
+        }
+    }
     d.lgPE = t->lgPhotoelectric ? 1 : 0;
     if (d.lgPE &&
-        ((rc = dust_put(ctx, t->grainRadius, (size_t)nA, &d.d_radius, "grainRadius")) ||
+        ((!d.d_radius && (rc = dust_put(ctx, t->grainRadius, (size_t)nA, &d.d_radius, "grainRadius"))) ||
          (rc = dust_put(ctx, t->grainVn, (size_t)nS, &d.d_vn, "grainVn")) ||
          (rc = dust_put(ctx, t->MsurfAtom, (size_t)nS, &d.d_msurf, "MsurfAtom")) ||
          (rc = dust_put(ctx, t->ipRyd, (size_t)31 * 32, &d.d_ip, "ipRyd"))))
@@ -2116,10 +2180,21 @@ int moc_dust_emission(moc_ctx* ctx, int ig) {
         ctx->err = "moc_dust_emission: needs an attached grid and moc_set_dust with lgGas = 0";
         return MOC_ERR_STATE;
     }
-    int rc = moc_launch_dust_pdf(ctx, dust_args(ctx));
-    if (rc) return rc;
+    int rc;
+    if ((rc = launch_qheat(ctx)) || (rc = moc_launch_dust_pdf(ctx, dust_args(ctx)))) return rc;
     MOC_TRY(hipStreamSynchronize(ctx->stream), "dust emission");
     ctx->grid.f32_dirty = true;
+    return MOC_OK;
+}
+
+int moc_set_qheat_state(moc_ctx* ctx, int nIterateMC, double convPercent) {
+    if (!ctx) return MOC_ERR_ARG;
+    MocDust& d = ctx->dust;
+    if (!d.set || !d.lgQHeat) {
+        ctx->err = "moc_set_qheat_state: moc_set_dust with lgQHeat first";
+        return MOC_ERR_STATE;
+    }
+    d.qActive = nIterateMC > 1 && convPercent > d.minConvQHeat;
     return MOC_OK;
 }
 

@@ -7,8 +7,10 @@ makeDustXsec for an 'nk' file (ph_mod.f90:1108-1213):
   line 2   label, sublimation temperature [K], bulk density [g/cm^3],
            work function grainVn [Ryd], surface atom mass MsurfAtom
   then     wavelength [um], Re(m), Im(m) per line, wavelength ascending
-For species <label> the archive holds <label>_wav, <label>_n, <label>_k and
-<label>_hdr = [Tsub, rho, Vn, Msurf].  Also packed: nuDustRyd.dat, the dust
+For species <label> the archive holds <label>_wav, <label>_n, <label>_k,
+<label>_hdr = [Tsub, rho, Vn, Msurf] and <label>_name, the file's own label
+(grainLabel; its first letter is the species class qHeat goes by, 'S'
+silicate and 'C' carbon, emission_mod.f90:2070-2096).  Also packed: nuDustRyd.dat, the dust
 frequency points that merge into the mesh of gas+dust runs
 (grid_mod.f90:341-362), as nu_dust_ryd.
 Run once where the data directory exists; the .npz travels with the package.
@@ -54,6 +56,7 @@ def main(src: str = "/root/reference/dustData", out: str | None = None):
         arrays[f"{key}_wav"] = wav
         arrays[f"{key}_n"] = n
         arrays[f"{key}_k"] = k
+        arrays[f"{key}_name"] = np.array(label)
     nu = [float(ln.split()[0]) for ln in (src / "nuDustRyd.dat").read_text().splitlines() if ln.strip()]
     arrays["nu_dust_ryd"] = np.array(nu)
     arrays["species"] = np.array(labels)

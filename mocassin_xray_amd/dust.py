@@ -48,6 +48,14 @@ def optical_constants(species: str):
     return d[f"{species}_wav"], d[f"{species}_n"], d[f"{species}_k"], d[f"{species}_hdr"]
 
 
+def grain_kind(species: str) -> int:
+    """qHeat's class of a species from the first letter of its file label
+    (emission_mod.f90:2070-2096): 1 silicate ('S'), 2 carbon ('C'), 0 any
+    other (no temperature distribution; e.g. the graphite files' 'g')."""
+    name = str(_data()[f"{species}_name"])
+    return {"S": 1, "C": 2}.get(name[:1], 0)
+
+
 def linear_map(y: np.ndarray, x: np.ndarray, x_new: np.ndarray) -> np.ndarray:
     """linearMap (interpolation_mod.f90:111-131): clamp outside [x1, xn]."""
     out = np.empty_like(x_new)
@@ -216,6 +224,7 @@ class DustTables:
     emIntegral: np.ndarray   # [nS][nSizes][N_TEMPS] dustEmIntegral
     Vn: np.ndarray = None    # [nS] potential of a neutral grain [Ryd] (grainVn)
     Msurf: np.ndarray = None # [nS] surface atom mass [amu] (MsurfAtom)
+    kind: np.ndarray = None  # [nS] qHeat class: 1 silicate, 2 carbon, 0 other (grain_kind)
 
     @property
     def nSpecies(self) -> int:
@@ -261,8 +270,9 @@ def build_dust(nu: np.ndarray, widflx: np.ndarray, species: list, abundances,
             Cabs[s, a_i] = qa * geo
             Csca[s, a_i] = qs * geo
     emi = emission_integrals(Cabs, nu, widflx)
+    kind = np.array([grain_kind(s) for s in species], dtype=np.int32)
     return DustTables(list(species), np.asarray(abundances, dtype=np.float64), radius, gw,
-                      Tsub, rho, Cabs, Csca, emi, vn, ms)
+                      Tsub, rho, Cabs, Csca, emi, vn, ms, kind)
 
 
 def ndust_from_mdmg(dt: DustTables, mdmg: float, hden: np.ndarray, elem_z, abund) -> np.ndarray:

@@ -97,6 +97,10 @@ module mocgpu_mod
      type(c_ptr) :: grainAbun, grainWeight, TdustSublime, Cabs, Csca, emIntegral
      real(c_double) :: XHILimit
      type(c_ptr) :: grainRadius, grainVn, MsurfAtom, ipRyd
+     ! quantum heating (qHeat): lgQHeat, nTbins, minaQHeat, minConvQHeat, Tmax, grainKind
+     integer(c_int32_t) :: lgQHeat = 0, nTbins = 0
+     real(c_double) :: minaQHeat = 0d0, minConvQHeat = 0d0, Tmax = 0d0
+     type(c_ptr) :: grainKind = c_null_ptr
   end type moc_dust_tables
 
   type, bind(C), public :: moc_transport_stats
@@ -295,9 +299,18 @@ module mocgpu_mod
        type(c_ptr), value :: ctx
        type(moc_tuning), intent(out) :: t
      end function moc_get_tuning
+     ! the iteration state that switches quantum heating on (nIterateMC, convPercent
+     ! of the reference's module variables)
+     integer(c_int) function moc_set_qheat_state(ctx, nIterateMC, convPercent) &
+          bind(C, name="moc_set_qheat_state")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nIterateMC
+       real(c_double), value :: convPercent
+     end function moc_set_qheat_state
   end interface
 
-  public :: moc_set_graph, moc_set_tuning, moc_get_tuning
+  public :: moc_set_graph, moc_set_tuning, moc_get_tuning, moc_set_qheat_state
   public :: moc_create, moc_destroy, moc_last_error, moc_version, moc_grid_attach, moc_upload, &
        moc_download, moc_set_atomic, moc_opacity, moc_set_balance, moc_set_emission, &
        moc_emission, moc_set_packets, moc_iterate, moc_update, moc_reset_tallies, moc_transport, &

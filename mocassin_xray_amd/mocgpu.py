@@ -175,18 +175,27 @@ def emission_arrays(et) -> dict:
     }
 
 
-def dust_arrays(dt, lgGas: bool, XHILimit: float, xsec=None) -> dict:
+def dust_arrays(dt, lgGas: bool, XHILimit: float, xsec=None, qheat: dict | None = None) -> dict:
     """xsec (the gas cross-section stack) switches on the grain photoelectric
-    and gas-grain collision terms (lgPhotoelectric)."""
+    and gas-grain collision terms (lgPhotoelectric); qheat (keys minaQHeat,
+    minConvQHeat, Tmax, nTbins; the reference's defaults 1e-3 um, 99 %,
+    700 K, 300 where left out) switches on quantum heating."""
     from .dust import ip_ryd
     pe = xsec is not None
+    q = None if qheat is None else {"minaQHeat": 1.e-3, "minConvQHeat": 99.0, "Tmax": 700.0,
+                                    "nTbins": 300, **qheat}
+    kind = getattr(dt, "kind", None)
     return {
+        "lgQHeat": 1 if q else 0, "nTbins": int(q["nTbins"]) if q else 0,
+        "minaQHeat": float(q["minaQHeat"]) if q else 0.0,
+        "minConvQHeat": float(q["minConvQHeat"]) if q else 0.0, "Tmax": float(q["Tmax"]) if q else 0.0,
+        "grainKind": _c(kind, np.int32) if q else None,
         "nSpecies": dt.nSpecies, "nSizes": dt.nSizes, "nTemps": int(dt.emIntegral.shape[2]),
         "lgGas": 1 if lgGas else 0, "lgPhotoelectric": 1 if pe else 0,
         "grainAbun": _c(dt.grainAbun), "grainWeight": _c(dt.grainWeight),
         "TdustSublime": _c(dt.Tsub), "Cabs": _c(dt.Cabs), "Csca": _c(dt.Csca),
         "emIntegral": _c(dt.emIntegral), "XHILimit": float(XHILimit),
-        "grainRadius": _c(dt.radius) if pe else None, "grainVn": _c(dt.Vn) if pe else None,
+        "grainRadius": _c(dt.radius) if (pe or q) else None, "grainVn": _c(dt.Vn) if pe else None,
         "MsurfAtom": _c(dt.Msurf) if pe else None, "ipRyd": _c(ip_ryd(xsec)) if pe else None,
     }
 
@@ -367,16 +376,24 @@ class MocGPU:
         return out, tot.value
 
     # ------------------------------------------------------------------- dust
-    def set_dust(self, dt, lgGas: bool, XHILimit: float = 0.05, photoelectric_xsec=None) -> None:
+    def set_dust(self, dt, lgGas: bool, XHILimit: float = 0.05, photoelectric_xsec=None,
+                 qheat: dict | None = None) -> None:
         """Grain tables (a dust.DustTables built on the same mesh); after
         grid_attach.  Tdust starts at 50 K, Ndust at 0.  With gas, passing the
         cross-section stack as photoelectric_xsec switches on the grain
         charge, photoelectric heating and gas-grain collisions (after
-        set_balance)."""
-        self._dust_keep = dust_arrays(dt, lgGas, XHILimit, photoelectric_xsec)
+        set_balance).  qheat (a dict, see dust_arrays) switches on the quantum
+        heating of the grains smaller than its minaQHeat."""
+        self._dust_keep = dust_arrays(dt, lgGas, XHILimit, photoelectric_xsec, qheat)
         t = to_struct(MocDustTables, self._dust_keep)
         self._check(self._L.moc_set_dust(self._ctx, C.byref(t)), "moc_set_dust")
         self.dust_shape = (dt.nSpecies, dt.nSizes)
+
+    def set_qheat_state(self, nIterateMC: int, convPercent: float) -> None:
+        """The iteration state that switches quantum heating on for the next
+        emission (moc_set_qheat_state); moc_iterate sets it itself."""
+        self._check(self._L.moc_set_qheat_state(self._ctx, int(nIterateMC), float(convPercent)),
+                    "moc_set_qheat_state")
 
     def dustOpacity(self):
         """absOpac, scaOpac and the total opacity of every cell (the dust

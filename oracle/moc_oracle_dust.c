@@ -9,6 +9,8 @@
  *   oracle_dust_pdf         setDustPDF               emission_mod.f90:1921-1986
  *   oracle_dust_temperature getDustT                 update_mod.f90:3346-3453
  *                           (+ dust-only convergence :240-267)
+ *   oracle_qheat            qHeat, getTmin, enthalpy, bbody, clrate, cheat
+ *                                                    emission_mod.f90:1993-2504
  *
  * Pins: the Planck function against its closed-form limits (Wien and
  * Rayleigh-Jeans) and the Stefan-Boltzmann integral; the emission integral
@@ -16,8 +18,11 @@
  * getDustT against the inverse of the emission-integral table on a field
  * of known T (a dilute blackbody for which Kirchhoff's law fixes T); the
  * opacity and PDF sums against their definitions on hand-made inputs, and
- * against the sublimation switch.  The quantum-heating branches (lgQHeat)
- * are not restated.
+ * against the sublimation switch.  qHeat: the distributions sum to 1, a
+ * large grain in a thermal field sits at the field's temperature, the energy
+ * absorbed equals the energy emitted by the distribution (to the few per
+ * cent of the binning) in thermal and dilute hot fields, and the mean
+ * temperature rises with the field (tests/test_qheat.py).
  */
 #include <math.h>
 #include <stdint.h>
@@ -87,7 +92,18 @@ void oracle_dust_pdf(const oracle_dust_in* in, const double* Tdust, double* pdf)
             for (ai = 1; ai <= in->nA; ai++) {
                 const double T = Tdust[tdust_at(in, cell, s, ai)];
                 const double* c = in->Cabs + ((size_t)(s - 1) * in->nA + (ai - 1)) * B;
+                const int q = in->qIdx ? in->qIdx[(s - 1) * in->nA + ai - 1] : -1;
                 if (!(T > 0. && T < in->tsub[s - 1])) continue;
+                if (q >= 0) {  /* quantum heating (:1946-1960) */
+                    const double* qt = in->qT + ((size_t)cell * in->nQ + q) * in->nTb;
+                    const double* qp = in->qP + ((size_t)cell * in->nQ + q) * in->nTb;
+                    int iT;
+                    for (i = 0; i < B; i++)
+                        for (iT = 0; iT < in->nTb; iT++)
+                            row[i] = row[i] + c[i] * oracle_planck_h(in->nu[i], qt[iT]) * in->widFlx[i] *
+                                                  in->weight[ai - 1] * in->abun[s - 1] * qp[iT];
+                    continue;
+                }
                 for (i = 0; i < B; i++)
                     row[i] = row[i] + c[i] * oracle_planck_h(in->nu[i], T) * in->widFlx[i] *
                                           in->weight[ai - 1] * in->abun[s - 1];
@@ -161,4 +177,201 @@ int oracle_dust_temperature(const oracle_dust_in* in, const double* J, double* T
         }
     }
     return 0;
+}
+
+/* ---------------------------------------------------------------- qHeat
+ * Quantum heating of a small grain (qHeat, emission_mod.f90:1993-2264, after
+ * Guhathakurta & Draine 1989), with getTmin (:2267-2320), enthalpy
+ * (:2326-2375), bbody (:2381-2396), clrate (:2398-2433) and cheat
+ * (:2436-2500).  All arithmetic in FP64 with the reference's constants.
+ * Kept from the reference: getTmin zeroes the field below its 9.11e-5 Ryd
+ * cutoff before it integrates over those same bins, so the lowest
+ * temperature is always the emission table's first, 1 K; the last bin's
+ * probability is left 0; the 1e250 rescaling of the recurrence.  Guarded
+ * where the reference reads past its arrays: the wavelength searches clamp to
+ * the last mesh point, and the heating of bin if+2 from the cooling of bin
+ * if is skipped when if+2 lies past the last bin. */
+#define QH_C 2.9979250e10
+#define QH_CUT 0.1
+
+static double qh_enthalpy(double tg, int natom, double radius, int kind) {
+    const double enth1 = (1.4e3 / 3.) * pow(tg, 3.);
+    const double enth1_50 = (1.4e3 / 3.) * pow(50., 3.);
+    const double enth2 = (2.1647e4 / 2.3) * (pow(tg, 2.3) - pow(50., 2.3));
+    const double enth2_150 = (2.1647e4 / 2.3) * (pow(150., 2.3) - pow(50., 2.3));
+    const double enth3 = (4.8375e5 / 1.68) * (pow(tg, 1.68) - pow(150., 1.68));
+    const double enth3_500 = (4.8375e5 / 1.68) * (pow(500., 1.68) - pow(150., 1.68));
+    const double enth4 = 3.3107e7 * (tg - 500.);
+    const double vg = 4.18878 * 1.e-12 * radius * radius * radius;
+    double e;
+    if (kind == 1) {
+        if (tg <= 50.) e = enth1;
+        else if (tg <= 150.) e = enth1_50 + enth2;
+        else if (tg <= 500.) e = enth1_50 + enth2_150 + enth3;
+        else e = enth1_50 + enth2_150 + enth3_500 + enth4;
+        return (1. - (2. / (double)natom)) * vg * e;
+    }
+    e = (double)natom * (4.15e-22 * pow(tg, 3.3));
+    e = e / (1. + 6.51e-3 * tg + 1.5e-6 * tg * tg + 8.3e-7 * pow(tg, 2.3));
+    return (1. - (2. / (double)natom)) * e;
+}
+
+static double qh_bbody(double lam, double T) {
+    const double C1 = 3.74185e-5, C2 = 1.43883;
+    const double x = C2 / (lam * T);
+    if (x > 400.) return 0.;
+    return C1 * (1. / pow(lam, 5.)) / (exp(x) - 1.);
+}
+
+/* last 0-based index with lam < x, or -1 */
+static int qh_below(const double* lam, int B, double x) {
+    int k = B - 1;
+    while (k >= 0 && !(lam[k] < x)) --k;
+    return k;
+}
+
+static double qh_clrate(double tem, const double* lam, const double* fl, const double* sQ, int B) {
+    double engout = 0., prev = 0.;
+    int k, cut = qh_below(lam, B, QH_CUT);
+    if (cut < 0) cut = 0;
+    for (k = 0; k < B; k++) {
+        const double e = sQ[k] * qh_bbody(lam[k], tem);
+        if (k >= 1) engout = engout + 4. * ((e + prev) / 2.) * (lam[k] - lam[k - 1]);
+        prev = e;
+    }
+    return engout - 0.25 * fl[cut] * sQ[cut] * QH_CUT;
+}
+
+/* it: 1-based bin */
+static double qh_cheat(int it, int n, const double* lam, const double* fl, const double* sQ, int B,
+                       const double* U, double hc) {
+    double einmax, shwav, engin = 0.;
+    int k, nshort, nlong;
+    if (it >= n) return 0.;
+    einmax = 0.5 * (U[it] - U[it - 1]);
+    shwav = hc / einmax;
+    if (shwav >= QH_CUT) return 0.;
+    nshort = qh_below(lam, B, shwav) + 2;  /* 1-based, the first with lam >= shwav */
+    nlong = qh_below(lam, B, QH_CUT) + 1;  /* 1-based, the last with lam < cutoff */
+    if (nshort >= nlong) return 0.;
+    for (k = nshort; k <= nlong; k++)
+        if (k >= 2) engin = engin + ((sQ[k - 1] * fl[k - 1] + sQ[k - 2] * fl[k - 2]) / 2.) * (lam[k - 1] - lam[k - 2]);
+    return engin;
+}
+
+int oracle_qheat(const oracle_qheat_in* q, double tg, double* temp, double* pss, double* work) {
+    const int B = q->B, n = q->nTbins;
+    const double hc = 6.6262e-27 * QH_C;
+    double *lam = work, *rf = lam + B, *sQ = rf + B, *U = sQ + B, *dU = U + n, *x = dU + n;
+    double *A = x + n, *Bm = A + (size_t)n * n;  /* A[f][i], Bm[f][i], 0-based */
+    int natom, i, f, ii, j;
+    double sumx = 0.;
+#define AA(r, c) A[(size_t)(r - 1) * n + (c - 1)]
+#define BB(r, c) Bm[(size_t)(r - 1) * n + (c - 1)]
+    for (i = 0; i < n; i++) temp[i] = pss[i] = 0.;
+    /* the field as F_lambda, the mesh reversed to increasing wavelength */
+    for (i = 0; i < B; i++) {
+        const int k = B - 1 - i;
+        const double r = q->J[k] / (q->widFlx[k] * 3.28984e15);
+        rf[i] = r * pow(q->nu[k] * 3.28984e15, 2.) / QH_C;
+        lam[i] = QH_C / (q->nu[k] * 3.28984e15);
+        sQ[i] = q->Cabs[k];
+    }
+    if (q->kind == 1) natom = (int)(0.44 * 1.e12 * pow(q->radius, 3.));
+    else if (q->kind == 2) natom = (int)(0.454 * 3. * 1.e12 * pow(q->radius, 3.));
+    else return 0;  /* neither silicate nor carbon: no distribution */
+    temp[0] = 1.;   /* getTmin */
+    U[0] = qh_enthalpy(temp[0], natom, q->radius, q->kind);
+    for (i = 2; i <= 10 && i <= n; i++) {
+        temp[i - 1] = temp[0] + (double)(i - 1) / 9. * (tg - temp[0]);
+        U[i - 1] = qh_enthalpy(temp[i - 1], natom, q->radius, q->kind);
+    }
+    for (i = 11; i <= n; i++) {
+        temp[i - 1] = tg + (double)(i - 10) / (double)(n - 10) * (q->Tmax - tg);
+        U[i - 1] = qh_enthalpy(temp[i - 1], natom, q->radius, q->kind);
+    }
+    dU[0] = 0.5 * (U[1] - U[0]);
+    for (i = 2; i <= n - 1; i++) dU[i - 1] = 0.5 * (U[i] - U[i - 2]);
+    dU[n - 1] = 0.5 * (U[n - 1] - U[n - 2]);
+    /* heating: from bin ii to bin f by the photons of hc / (U(f) - U(ii)) */
+    for (ii = 1; ii <= n - 1; ii++) {
+        double UiTrun, wlim, qint = 0.;
+        for (f = ii + 1; f <= n; f++) {
+            const double wl = hc / (U[f - 1] - U[ii - 1]);
+            int wlp = qh_below(lam, B, wl) + 2;
+            if (wlp > B) wlp = B;
+            if (dU[f - 1] >= 0.1 * (U[f - 1] - U[ii - 1]) && f < n) {
+                double wls = hc / ((U[f - 1] + U[f]) / 2. - U[ii - 1]);
+                double wll = hc / ((U[f - 1] + U[f - 2]) / 2. - U[ii - 1]);
+                if (wl >= 0.1 || wl < lam[0]) {
+                    AA(f, ii) = 0.;
+                } else {
+                    int wllp, wlsp;
+                    if (wll > 0.1) wll = .1;
+                    if (wls < lam[0]) wls = lam[0];
+                    wllp = qh_below(lam, B, wll) + 2;
+                    if (wllp > B) wllp = B;
+                    wlsp = qh_below(lam, B, wls) + 2;
+                    if (wlsp > B) wlsp = B;
+                    AA(f, ii) = 0.5 * (sQ[wlsp - 1] * rf[wlsp - 1] * (wls * wls * wls) + sQ[wlp - 1] * rf[wlp - 1] * (wl * wl * wl)) *
+                                (U[f] - U[f - 1]) / 2.;
+                    AA(f, ii) = AA(f, ii) + 0.5 * (sQ[wlp - 1] * rf[wlp - 1] * (wl * wl * wl) + sQ[wllp - 1] * rf[wllp - 1] * (wll * wll * wll)) *
+                                                (U[f - 1] - U[f - 2]) / 2.;
+                    AA(f, ii) = AA(f, ii) / pow(hc, 2.);
+                }
+            } else {
+                if (wl >= 0.1 || wl < lam[0]) AA(f, ii) = 0.;
+                else AA(f, ii) = sQ[wlp - 1] * rf[wlp - 1] * dU[f - 1] * (wl * wl * wl) / pow(hc, 2.);
+            }
+        }
+        /* heating beyond the last bin (G&D p232) */
+        UiTrun = U[n - 1] + dU[n - 1] / 2. - U[ii - 1];
+        wlim = hc / UiTrun;
+        for (j = 1; j <= B - 1; j++) {
+            const double q1 = lam[j - 1] * sQ[j - 1] * rf[j - 1], q2 = lam[j] * sQ[j] * rf[j];
+            const double del = lam[j] - lam[j - 1];
+            if (lam[j] < wlim) qint = qint + 0.5 * (q1 + q2) * del;
+            else if (lam[j - 1] < wlim) qint = qint + (wlim - lam[j - 1]) * (q1 + 0.5 * (wlim - lam[j - 1]) * (q2 - q1) / del);
+        }
+        qint = qint / hc;
+        AA(n, ii) = AA(n, ii) + qint;
+    }
+    /* cooling: from bin if+1 to bin if */
+    for (f = 1; f <= n - 1; f++) {
+        const double mdUdt = qh_clrate(temp[f], lam, rf, sQ, B);
+        const double ch = qh_cheat(f + 1, n, lam, rf, sQ, B, U, hc);
+        AA(f, f + 1) = mdUdt / dU[f];
+        if (mdUdt <= ch) {
+            AA(f, f + 1) = AA(f, f + 1) * 1.e-20;
+            if (f + 2 <= n) AA(f + 2, f + 1) = AA(f + 2, f + 1) + (ch - mdUdt) / dU[f];
+        } else {
+            AA(f, f + 1) = AA(f, f + 1) - ch / dU[f];
+        }
+    }
+    if (AA(2, 1) == 0.) AA(2, 1) = 2. * AA(1, 2);
+    /* B(f, ii) = sum of A(f', ii) over f' >= f (G&D p233) */
+    for (ii = 1; ii <= n; ii++) {
+        if (ii < n) BB(n, ii) = AA(n, ii);
+        for (f = n - 1; f >= ii + 1; f--) BB(f, ii) = BB(f + 1, ii) + AA(f, ii);
+    }
+    x[0] = 1.e-100;
+    for (f = 2; f <= n; f++) {
+        double s = 0.;
+        for (j = 1; j <= f - 1; j++) s = s + BB(f, j) * x[j - 1];
+        x[f - 1] = s / AA(f - 1, f);
+        if (x[f - 1] > 1.e250)
+            for (i = 1; i <= f; i++) x[i - 1] = x[i - 1] < 1. ? 0. : x[i - 1] * 1.e-250;
+    }
+    for (j = 1; j <= n; j++) sumx = sumx + x[j - 1];
+    for (i = 1; i <= n - 1; i++) {
+        if (x[i - 1] > 0. && log10(x[i - 1]) - log10(sumx) <= -250.) x[i - 1] = 0.;
+        pss[i - 1] = x[i - 1] / sumx;
+    }
+#undef AA
+#undef BB
+    return 1;
+}
+
+size_t oracle_qheat_work(int B, int nTbins) {
+    return (size_t)3 * B + (size_t)3 * nTbins + (size_t)2 * nTbins * nTbins;
 }

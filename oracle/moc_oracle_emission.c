@@ -512,12 +512,27 @@ static void emission_one(const oracle_emission_tables* t, const prep_t* p, oracl
             for (sp = 1; sp <= nS; sp++) {
                 const double T = s->Tdust[((size_t)cell * (nA + 1) + ai) * (nS + 1) + sp];
                 const double* c = s->dustCabs + ((size_t)(sp - 1) * nA + (ai - 1)) * B;
+                const int q = s->qIdx ? s->qIdx[(sp - 1) * nA + ai - 1] : -1;
                 if (!(T < s->dustTsub[sp - 1])) continue;
                 for (i = 0; i < B; i++) {
-                    const double v = c[i] * oracle_planck_h(t->bal->nuArray[i], T) * t->widFlx[i] *
-                                     s->dustWeight[ai - 1] * s->dustAbun[sp - 1];
-                    sDust[i] = sDust[i] + v;
-                    normDust = normDust + v;
+                    if (q >= 0) {  /* quantum heating (:1701-1730) */
+                        const double* qt = s->qT + ((size_t)cell * s->nQ + q) * s->nTb;
+                        const double* qp = s->qP + ((size_t)cell * s->nQ + q) * s->nTb;
+                        int iT;
+                        for (iT = 0; iT < s->nTb; iT++) {
+                            const double v = c[i] * oracle_planck_h(t->bal->nuArray[i], qt[iT]) * t->widFlx[i] *
+                                             s->dustWeight[ai - 1] * s->dustAbun[sp - 1] * qp[iT];
+                            sDust[i] = sDust[i] + v;
+                            normDust = normDust + v;
+                        }
+                        continue;
+                    }
+                    {
+                        const double v = c[i] * oracle_planck_h(t->bal->nuArray[i], T) * t->widFlx[i] *
+                                         s->dustWeight[ai - 1] * s->dustAbun[sp - 1];
+                        sDust[i] = sDust[i] + v;
+                        normDust = normDust + v;
+                    }
                 }
             }
         for (i = 0; i < B; i++) sDust[i] = sDust[i] * cst * s->Ndust[cell] / s->Hden[cell];

@@ -65,6 +65,12 @@ typedef struct oracle_emission_state {
     const double* Tdust;      /* [C+1][dustA+1][dustS+1] */
     const double* Ndust;      /* [C+1] */
     const double* Hden;       /* [C+1] */
+    /* quantum heating (qIdx NULL: off): grain g = (species-1) dustA + size-1 with
+     * qIdx[g] = q >= 0 emits from qT/qP [C+1][nQ][nTb] */
+    int nQ, nTb;
+    const int* qIdx;
+    const double* qT;
+    const double* qP;
 } oracle_emission_state;
 
 /* returns 0, or -1 when two gamma edges share a bin (fb_ff stops there) */

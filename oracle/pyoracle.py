@@ -392,6 +392,7 @@ class EmissionStateC(C.Structure):
         ("totalLines", _dp), ("normCont", _dp), ("normRec", _dp), ("normFor", _dp),
         ("dustS", C.c_int), ("dustA", C.c_int), ("dustAbun", _dp), ("dustWeight", _dp),
         ("dustTsub", _dp), ("dustCabs", _dp), ("Tdust", _dp), ("Ndust", _dp), ("Hden", _dp),
+        ("nQ", C.c_int), ("nTb", C.c_int), ("qIdx", _ip), ("qT", _dp), ("qP", _dp),
     ]
 
 
@@ -435,8 +436,8 @@ def emission_tables(et, bt, elem_z, abund, nstages):
 def emission(et, bt, elem_z, abund, nstages, Te, Ne, ionDen, nthreads=1, dust=None):
     """Oracle emissionDriver over all cells.  ionDen is (C+1, nElemUsed,
     nstages).  Returns recPDF (C+1, B), totalLines and the per-cell sums.
-    dust = (DustTables, Tdust (C+1, nA+1, nS+1), Ndust, Hden) adds the grains
-    of a gas + dust run."""
+    dust = (DustTables, Tdust (C+1, nA+1, nS+1), Ndust, Hden[, (qIdx, qT, qP)])
+    adds the grains of a gas + dust run (the last item: quantum heating)."""
     L = lib()
     L.oracle_emission.argtypes = [C.POINTER(EmissionTablesC), C.POINTER(EmissionStateC), C.c_int]
     L.oracle_emission.restype = C.c_int
@@ -452,7 +453,15 @@ def emission(et, bt, elem_z, abund, nstages, Te, Ne, ionDen, nthreads=1, dust=No
     st = EmissionStateC(nCells=C1 - 1, Te=dp("Te"), Ne=dp("Ne"), ionDen=dp("X"), recPDF=dp("pdf"),
                         totalLines=dp("tl"), normCont=dp("nc"), normRec=dp("nr"), normFor=dp("nf"))
     if dust is not None:
-        dt, Td, Nd, Hd = dust
+        dt, Td, Nd, Hd = dust[:4]
+        if len(dust) > 4 and dust[4] is not None:
+            qi, qT, qP = dust[4]
+            s["qi"] = np.ascontiguousarray(qi, dtype=np.int32)
+            s["qT"] = np.ascontiguousarray(qT, dtype=np.float64)
+            s["qP"] = np.ascontiguousarray(qP, dtype=np.float64)
+            st.nQ, st.nTb = int(s["qT"].shape[1]), int(s["qT"].shape[2])
+            st.qIdx = s["qi"].ctypes.data_as(_ip)
+            st.qT, st.qP = dp("qT"), dp("qP")
         for name, a in (("da", dt.grainAbun), ("dw", dt.grainWeight), ("dt", dt.Tsub),
                         ("dc", dt.Cabs), ("td", Td), ("nd", Nd), ("hd", Hd)):
             s[name] = np.ascontiguousarray(a, dtype=np.float64)
@@ -512,6 +521,7 @@ class DustIn(C.Structure):
         ("dustOnly", C.c_int), ("abun", _dp), ("weight", _dp), ("tsub", _dp), ("Cabs", _dp),
         ("Csca", _dp), ("emInt", _dp), ("nu", _dp), ("widFlx", _dp), ("Ndust", _dp),
         ("XHILimit", C.c_double), ("collD", _dp), ("peD", _dp),
+        ("nQ", C.c_int), ("nTb", C.c_int), ("qIdx", _ip), ("qT", _dp), ("qP", _dp),
     ]
 
 
@@ -569,9 +579,24 @@ def dust_opacity(dt, nu, widflx, Ndust, Tdust, gasOpac=None):
     return f(op), f(ab), f(sc)
 
 
-def dust_pdf(dt, nu, widflx, Ndust, Tdust):
-    """setDustPDF rows (C+1, B)."""
+def _q(inp, keep, qheat):
+    """qheat = (qIdx [nS*nA], qT, qP [C+1][nQ][nTb]) into an oracle input."""
+    if qheat is None:
+        return
+    qi, qT, qP = qheat
+    keep["qi"] = np.ascontiguousarray(qi, dtype=np.int32)
+    keep["qT"] = np.ascontiguousarray(qT, dtype=np.float64)
+    keep["qP"] = np.ascontiguousarray(qP, dtype=np.float64)
+    inp.nQ, inp.nTb = int(keep["qT"].shape[1]), int(keep["qT"].shape[2])
+    inp.qIdx = keep["qi"].ctypes.data_as(_ip)
+    inp.qT, inp.qP = keep["qT"].ctypes.data_as(_dp), keep["qP"].ctypes.data_as(_dp)
+
+
+def dust_pdf(dt, nu, widflx, Ndust, Tdust, qheat=None):
+    """setDustPDF rows (C+1, B); qheat = (qIdx, qT, qP) for the grains with
+    temperature distributions."""
     inp, keep = _dust_in(dt, nu, widflx, Ndust, True)
+    _q(inp, keep, qheat)
     C1, B = np.asarray(Ndust).size, np.asarray(nu).size
     td = np.ascontiguousarray(Tdust, dtype=np.float64)
     pdf = np.zeros(C1 * B)
@@ -594,6 +619,29 @@ def dust_temperature(dt, nu, widflx, Ndust, Tdust, J, dustOnly=True, XHILimit=0.
     lib().oracle_dust_temperature(C.byref(inp), j.ctypes.data_as(_dp), td.ctypes.data_as(_dp),
                                   conv.ctypes.data_as(_ip), black.ctypes.data_as(_ip), cnt)
     return td, conv, black, cnt[0], cnt[1]
+
+
+class QHeatIn(C.Structure):
+    _fields_ = [("B", C.c_int), ("nTbins", C.c_int), ("Tmax", C.c_double), ("nu", _dp), ("widFlx", _dp),
+                ("J", _dp), ("Cabs", _dp), ("radius", C.c_double), ("kind", C.c_int)]
+
+
+def qheat(nu, widflx, J, Cabs, radius, kind, tg, nTbins=300, Tmax=700.0):
+    """qHeat of one grain in one cell: (temp, pss) over nTbins bins; J (B,) the
+    cell's normalised J, Cabs (B,) the grain's cross-section, kind 1 silicate
+    or 2 carbon."""
+    keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (nu, widflx, J, Cabs)]
+    B = keep[0].size
+    q = QHeatIn(B=B, nTbins=int(nTbins), Tmax=float(Tmax), nu=keep[0].ctypes.data_as(_dp),
+                widFlx=keep[1].ctypes.data_as(_dp), J=keep[2].ctypes.data_as(_dp),
+                Cabs=keep[3].ctypes.data_as(_dp), radius=float(radius), kind=int(kind))
+    L = lib()
+    L.oracle_qheat_work.restype = C.c_size_t
+    work = np.zeros(int(L.oracle_qheat_work(C.c_int(B), C.c_int(int(nTbins)))))
+    temp, pss = np.zeros(nTbins), np.zeros(nTbins)
+    L.oracle_qheat(C.byref(q), C.c_double(float(tg)), temp.ctypes.data_as(_dp), pss.ctypes.data_as(_dp),
+                   work.ctypes.data_as(_dp))
+    return temp, pss
 
 
 # ---- output reductions (outputGas volume averages, writeSED) ----------------
