@@ -1,4 +1,4 @@
-"""ctypes binding of libmocgpu (include/mocgpu.h).
+"""ctypes binding of libmocgpu (include/mocgpu.h, include/mocmodel.h).
 
 The library is loaded from the package directory; a missing or unloadable
 library raises at once -- there is no CPU fallback.
@@ -40,9 +40,28 @@ EXPORTS = [
     "moc_volume_averages_noalloc", "moc_set_graph", "moc_set_tuning", "moc_get_tuning",
     "moc_calibrate_atomics", "moc_set_qheat_state",
 ]
+# every symbol include/mocmodel.h declares
+MODEL_EXPORTS = [
+    "moc_model_build", "moc_model_free", "moc_model_last_error", "moc_model_get",
+    "moc_model_state", "moc_model_load", "moc_model_write",
+]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
+
+
+class MocModelSpec(C.Structure):
+    _fields_ = [
+        ("dataDir", C.c_char_p), ("precision", C.c_int32), ("nElem", C.c_int32),
+        ("elemZ", C.POINTER(C.c_int32)), ("abund", C.POINTER(C.c_double)),
+        ("nstages", C.c_int32), ("maxLev", C.c_int32), ("XHILimit", C.c_double),
+        ("nbins", C.c_int32), ("nuMin", C.c_double), ("nuMax", C.c_double),
+        ("strictMesh", C.c_int32), ("lgCompton", C.c_int32), ("Tstellar", C.c_double),
+        ("LStar", C.c_double), ("nPhotons", C.c_double), ("plFraction", C.c_double),
+        ("plLoKeV", C.c_double), ("plHiKeV", C.c_double), ("nx", C.c_int32),
+        ("Rout", C.c_double), ("Hdens", C.c_double), ("hden", C.POINTER(C.c_double)),
+        ("TeStart", C.c_double), ("H0in", C.c_double),
+    ]
 
 
 class MocTables(C.Structure):
@@ -214,6 +233,22 @@ def lib() -> C.CDLL:
     L.moc_calibrate_atomics.restype = C.c_int
     L.moc_set_qheat_state.argtypes = [vp, C.c_int, C.c_double]
     L.moc_set_qheat_state.restype = C.c_int
+    L.moc_model_build.argtypes = [C.POINTER(MocModelSpec), C.POINTER(vp)]
+    L.moc_model_build.restype = C.c_int
+    L.moc_model_free.argtypes = [vp]
+    L.moc_model_free.restype = None
+    L.moc_model_last_error.argtypes = []
+    L.moc_model_last_error.restype = C.c_char_p
+    L.moc_model_get.argtypes = [vp, C.POINTER(MocTables), C.POINTER(MocGridDesc),
+                                C.POINTER(MocAtomic), C.POINTER(MocBalanceTables),
+                                C.POINTER(MocEmissionTables)]
+    L.moc_model_get.restype = C.c_int
+    L.moc_model_state.argtypes = [vp, C.POINTER(C.c_int64)] + [C.POINTER(_dp)] * 4
+    L.moc_model_state.restype = C.c_int
+    L.moc_model_load.argtypes = [C.POINTER(vp), vp]
+    L.moc_model_load.restype = C.c_int
+    L.moc_model_write.argtypes = [vp, C.c_char_p]
+    L.moc_model_write.restype = C.c_int
     _LIB = L
     return L
 

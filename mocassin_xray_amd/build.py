@@ -20,6 +20,8 @@ FORTRAN = PKG / "fortran"
 
 HIP_SOURCES = ["mocgpu.hip", "transport.hip", "opacity.hip", "balance.hip", "emission.hip", "dust.hip",
                "output.hip", "calib.hip"]
+# host C++ of the same library: the native model set-up (mocmodel.h)
+CPP_SOURCES = ["model.cpp", "model_data.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -45,7 +47,9 @@ def build_library(verbose: bool = False) -> Path:
     """Compile every HIP source of the package into libmocgpu.so."""
     srcs = [str(CSRC / s) for s in HIP_SOURCES]
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(ROOT / "include"), "-o", str(tmp), *srcs]
+    # the host sources go in as C++ (-x c++), the HIP sources after -x hip
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(ROOT / "include"), "-o", str(tmp),
+           "-x", "c++", *[str(CSRC / s) for s in CPP_SOURCES], "-x", "hip", *srcs]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + " ".join(cmd) + "\n" + res.stderr[-4000:])

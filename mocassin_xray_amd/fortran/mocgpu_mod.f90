@@ -10,6 +10,10 @@
 !                                                  percentage, minConvergence,
 !                                                  maxIterateMC, autoPackets)
 !
+! The model set-up of include/mocmodel.h (moc_model_build and friends) reads
+! the reference's data/ files in the library: a Fortran host then needs no
+! other program to build the tables (mocgpu_driver --namelist).
+!
 ! The grid arrays stay in the reference's layout on the Fortran side:
 ! (0:nCells) per cell, (0:nCells, 1:nbins) per cell and frequency (cell index
 ! fastest) and (0:nCells, nElementsUsed, nstages) for ionDen; put_field /
@@ -122,6 +126,66 @@ module mocgpu_mod
   type, bind(C), public :: moc_tuning
      integer(c_int32_t) :: emitBatch, segsPerTrip, transportWaves, graph, hotJ
   end type moc_tuning
+
+  ! one gas model for moc_model_build (include/mocmodel.h)
+  type, bind(C), public :: moc_model_spec
+     type(c_ptr) :: dataDir
+     integer(c_int32_t) :: precision, nElem
+     type(c_ptr) :: elemZ, abund
+     integer(c_int32_t) :: nstages, maxLev
+     real(c_double) :: XHILimit
+     integer(c_int32_t) :: nbins
+     real(c_double) :: nuMin, nuMax
+     integer(c_int32_t) :: strictMesh, lgCompton
+     real(c_double) :: Tstellar, LStar, nPhotons, plFraction, plLoKeV, plHiKeV
+     integer(c_int32_t) :: nx
+     real(c_double) :: Rout, Hdens
+     type(c_ptr) :: hden
+     real(c_double) :: TeStart, H0in
+  end type moc_model_spec
+
+  interface
+     integer(c_int) function moc_model_build(spec, model) bind(C, name="moc_model_build")
+       import :: c_int, c_ptr, moc_model_spec
+       type(moc_model_spec), intent(in) :: spec
+       type(c_ptr), intent(out) :: model
+     end function moc_model_build
+     subroutine moc_model_free(model) bind(C, name="moc_model_free")
+       import :: c_ptr
+       type(c_ptr), value :: model
+     end subroutine moc_model_free
+     type(c_ptr) function moc_model_last_error() bind(C, name="moc_model_last_error")
+       import :: c_ptr
+     end function moc_model_last_error
+     integer(c_int) function moc_model_get(model, tables, grid, atomic, balance, emission) &
+          bind(C, name="moc_model_get")
+       import :: c_int, c_ptr, moc_tables, moc_grid_desc, moc_atomic, moc_balance_tables, &
+            moc_emission_tables
+       type(c_ptr), value :: model
+       type(moc_tables), intent(out) :: tables
+       type(moc_grid_desc), intent(out) :: grid
+       type(moc_atomic), intent(out) :: atomic
+       type(moc_balance_tables), intent(out) :: balance
+       type(moc_emission_tables), intent(out) :: emission
+     end function moc_model_get
+     integer(c_int) function moc_model_state(model, nCells, Te, Ne, Hden, ionDen) &
+          bind(C, name="moc_model_state")
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: model
+       integer(c_int64_t), intent(out) :: nCells
+       type(c_ptr), intent(out) :: Te, Ne, Hden, ionDen
+     end function moc_model_state
+     integer(c_int) function moc_model_load(ctx, model) bind(C, name="moc_model_load")
+       import :: c_int, c_ptr
+       type(c_ptr), intent(out) :: ctx
+       type(c_ptr), value :: model
+     end function moc_model_load
+     integer(c_int) function moc_model_write(model, path) bind(C, name="moc_model_write")
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: model
+       character(kind=c_char), intent(in) :: path(*)
+     end function moc_model_write
+  end interface
 
   interface
      integer(c_int) function moc_create(ctx, tables) bind(C, name="moc_create")
@@ -311,6 +375,8 @@ module mocgpu_mod
   end interface
 
   public :: moc_set_graph, moc_set_tuning, moc_get_tuning, moc_set_qheat_state
+  public :: moc_model_build, moc_model_free, moc_model_last_error, moc_model_get, &
+       moc_model_state, moc_model_load, moc_model_write
   public :: moc_create, moc_destroy, moc_last_error, moc_version, moc_grid_attach, moc_upload, &
        moc_download, moc_set_atomic, moc_opacity, moc_set_balance, moc_set_emission, &
        moc_emission, moc_set_packets, moc_iterate, moc_update, moc_reset_tallies, moc_transport, &
