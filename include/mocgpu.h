@@ -365,6 +365,23 @@ int moc_sed(moc_ctx* ctx, int ig, double* sed, double* totalE);
  * star iStar's packets in grid iStar-1.  Grains are not supported in a batch. */
 int moc_grid_attach_batch(moc_ctx* ctx, int nGrids, const moc_grid_desc* grids);
 
+/* Nested sub-grids (multiGrids: setSubGrids grid_mod.f90:1834-2100, fillGrid
+ * :907-960, the sub-grid branches of pathSegment photon_mod.f90:1931-1963,
+ * 2169-2223, 2884-3948): grids[0] is the mother grid, grids[1..nGrids-1]
+ * its sub-grids, each inside the mother's box.  The mother's active map
+ * holds -g for the cells that sub-grid g (1-based grid number, 2..nGrids)
+ * covers; a sub-grid's own map numbers its cells 1..nCells (0 inactive) and
+ * its lgSymmetricXYZ / lgPlaneIonization are ignored (it takes the mother's
+ * symmetry; plane-parallel mothers are refused).  Cells are numbered as in a
+ * batch (cell c of grid k is cellBase[k] + c); the single source of
+ * moc_tables starts in the grid that holds its mother cell.  Packets cross
+ * into a sub-grid where they reach a covered mother cell, walk its cells
+ * (end cells half-width on every face, no loss at its faces), and go back to
+ * the mother cell that holds their position when they leave its box.
+ * Opacity, emission, balance, normalisation and the output reductions run
+ * over every grid's cells; grains are not supported. */
+int moc_grid_attach_nested(moc_ctx* ctx, int nGrids, const moc_grid_desc* grids);
+
 /* Copy a per-cell field between host and device.  `count` is the number of
  * elements of the host buffer; (cell, bin) fields take `layout`. */
 int moc_upload(moc_ctx* ctx, int field, int ig, const void* host, int64_t count,

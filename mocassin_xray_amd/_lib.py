@@ -38,7 +38,7 @@ EXPORTS = [
     "moc_set_dust", "moc_dust_opacity", "moc_dust_emission", "moc_dust_temperature",
     "moc_grid_attach_batch", "moc_volume_averages", "moc_sed", "moc_line_luminosities",
     "moc_volume_averages_noalloc", "moc_set_graph", "moc_set_tuning", "moc_get_tuning",
-    "moc_calibrate_atomics", "moc_set_qheat_state",
+    "moc_calibrate_atomics", "moc_set_qheat_state", "moc_grid_attach_nested",
 ]
 # every symbol include/mocmodel.h declares
 MODEL_EXPORTS = [
@@ -233,6 +233,8 @@ def lib() -> C.CDLL:
     L.moc_calibrate_atomics.restype = C.c_int
     L.moc_set_qheat_state.argtypes = [vp, C.c_int, C.c_double]
     L.moc_set_qheat_state.restype = C.c_int
+    L.moc_grid_attach_nested.argtypes = [vp, C.c_int, C.POINTER(MocGridDesc)]
+    L.moc_grid_attach_nested.restype = C.c_int
     L.moc_model_build.argtypes = [C.POINTER(MocModelSpec), C.POINTER(vp)]
     L.moc_model_build.restype = C.c_int
     L.moc_model_free.argtypes = [vp]

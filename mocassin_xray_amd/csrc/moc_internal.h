@@ -37,6 +37,7 @@ struct MocGridView {
     const int* active;
     const T* sourceCDF;   // [B] of the grid's star
     int nx, ny, nz, base, sym, plane;
+    int sub;              // a nested sub-grid (its mother is view 0)
     int starIdx[3];       // 0-based
     T R_out, deltaE;
     T star[3];
@@ -94,6 +95,11 @@ struct MocTransportArgs {
     int nGrids;
     const MocGridView<T>* grids;  // [nGrids]
     const int64_t* gridFirst;     // [nGrids + 1]
+    // nested sub-grids (nested = 1): view 0 is the mother grid, whose active
+    // map holds -g for the cells sub-grid g (view g - 1) covers; the packets
+    // start in view nestStart; mir = the mother's lower faces (mirror planes)
+    int nested, nestStart;
+    T mir[3];
     // J tallies of a hot block of (bin, cell) keys -- a box of cells around
     // the star times the bins that carried the most path there -- summed in
     // LDS per workgroup as FP32 partials, then added to J in FP64 (the mixed
@@ -417,6 +423,12 @@ struct MocGrid {
     // offsets into the concatenated axes and active maps; grid k is lit by
     // star k
     int nGrids = 1;
+    // nested sub-grids (moc_grid_attach_nested): grid 0 the mother, whose
+    // active map holds -g for the cells that grid g (1-based) covers; the
+    // source starts in grid nestStart (0-based) at cell nestStartIdx (0-based)
+    int nested = 0, nestStart = 0;
+    int nestStartIdx[3] = {0, 0, 0};
+    bool batch() const { return nGrids > 1 && !nested; }
     std::vector<int> gnx, gny, gnz, gbase, gdense;
     std::vector<size_t> goffx, goffy, goffz, goffa;
     std::vector<double> gR_out;

@@ -333,9 +333,50 @@ int sync_f64_master(moc_ctx* ctx, int f) {
     return MOC_OK;
 }
 
-int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who);
+int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who, bool nested = false);
+int upload_nested_views(moc_ctx* ctx, int s);
 MocBalanceArgs balance_args(moc_ctx* ctx, const double** W);
 void drop_graph(moc_ctx* ctx);  // a captured iteration holds the arguments of its time
+// nested sub-grids: the view of every grid for source s (0-based): the
+// source's deltaE, the mother's symmetry and R_out in every view, the start
+// cell in the source's grid
+int upload_nested_views(moc_ctx* ctx, int s) {
+    MocGrid& g = ctx->grid;
+    const int nG = g.nGrids;
+    const double dE = ctx->Lstar[s] / ctx->nPhotons[s];  // photon_mod.f90:88
+    if (!(dE >= 0.0)) { ctx->err = "moc_transport: deltaEUsed < 0"; return MOC_ERR_ARG; }
+    std::vector<MocGridView<double>> v64((size_t)nG);
+    std::vector<MocGridView<float>> v32((size_t)nG);
+    for (int k = 0; k < nG; ++k) {
+        MocGridView<double>& a = v64[k];
+        MocGridView<float>& b = v32[k];
+        a.ax = g.d_ax + g.goffx[k]; a.ay = g.d_ay + g.goffy[k]; a.az = g.d_az + g.goffz[k];
+        a.vx = g.d_vx + g.goffx[k]; a.vy = g.d_vy + g.goffy[k]; a.vz = g.d_vz + g.goffz[k];
+        b.ax = g.f_ax + g.goffx[k]; b.ay = g.f_ay + g.goffy[k]; b.az = g.f_az + g.goffz[k];
+        b.vx = g.f_vx + g.goffx[k]; b.vy = g.f_vy + g.goffy[k]; b.vz = g.f_vz + g.goffz[k];
+        a.active = b.active = (k > 0 && g.gdense[k]) ? nullptr : g.d_active + g.goffa[k];
+        a.sourceCDF = ctx->d_sourceCDF + (size_t)s * ctx->nbins;
+        b.sourceCDF = ctx->f_sourceCDF + (size_t)s * ctx->nbins;
+        a.nx = b.nx = g.gnx[k]; a.ny = b.ny = g.gny[k]; a.nz = b.nz = g.gnz[k];
+        a.base = b.base = g.gbase[k];
+        a.sym = b.sym = g.sym;
+        a.plane = b.plane = 0;
+        a.sub = b.sub = k > 0 ? 1 : 0;
+        for (int d = 0; d < 3; ++d) {
+            a.starIdx[d] = b.starIdx[d] = g.nestStartIdx[d];
+            a.star[d] = ctx->starPos[3 * s + d];
+            b.star[d] = (float)ctx->starPos[3 * s + d];
+        }
+        a.R_out = g.gR_out[0]; b.R_out = (float)g.gR_out[0];
+        a.deltaE = dE; b.deltaE = (float)dE;
+    }
+    int rc;
+    if ((rc = h2d(ctx, g.d_view64, v64.data(), v64.size() * sizeof(MocGridView<double>), "grid views")) ||
+        (rc = h2d(ctx, g.d_view32, v32.data(), v32.size() * sizeof(MocGridView<float>), "grid views")))
+        return rc;
+    return MOC_OK;
+}
+
 int run_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ctr_base,
                   moc_transport_stats* stats);
 
@@ -451,6 +492,15 @@ int moc_grid_attach_batch(moc_ctx* ctx, int nGrids, const moc_grid_desc* grids) 
     return attach_grids(ctx, nGrids, grids, "moc_grid_attach_batch");
 }
 
+int moc_grid_attach_nested(moc_ctx* ctx, int nGrids, const moc_grid_desc* grids) {
+    if (!ctx) return MOC_ERR_ARG;
+    if (nGrids < 2 || !grids) {
+        ctx->err = "moc_grid_attach_nested: a mother grid and at least one sub-grid";
+        return MOC_ERR_ARG;
+    }
+    return attach_grids(ctx, nGrids, grids, "moc_grid_attach_nested", true);
+}
+
 }  // extern "C"
 
 namespace {
@@ -458,7 +508,7 @@ namespace {
 // Attach nG grids; more than one is a batch of independent grids whose cells
 // are numbered one grid after the other and whose axes and active maps are
 // concatenated (active ids become global).
-int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who) {
+int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who, bool nested) {
     drop_graph(ctx);
     const std::string w(who);
     for (int k = 0; k < nG; ++k) {
@@ -467,6 +517,21 @@ int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who
             !gd->active || gd->nCells < 1) {
             ctx->err = w + ": invalid grid " + std::to_string(k) + " (n>=2 per axis, nCells>=1)";
             return MOC_ERR_ARG;
+        }
+        if (nested && (gds->lgPlaneIonization || gd->lgPlaneIonization)) {
+            ctx->err = w + ": nested sub-grids of a plane-parallel slab are not supported";
+            return MOC_ERR_ARG;
+        }
+        if (nested && k > 0) {
+            // a sub-grid lies inside its mother's box (setSubGrids, grid_mod.f90:1834-2100)
+            const moc_grid_desc* m = gds;
+            if (gd->xAxis[0] < m->xAxis[0] || gd->xAxis[gd->nx - 1] > m->xAxis[m->nx - 1] ||
+                gd->yAxis[0] < m->yAxis[0] || gd->yAxis[gd->ny - 1] > m->yAxis[m->ny - 1] ||
+                gd->zAxis[0] < m->zAxis[0] || gd->zAxis[gd->nz - 1] > m->zAxis[m->nz - 1]) {
+                ctx->err = w + ": sub-grid " + std::to_string(k + 1) + " reaches outside the mother grid";
+                return MOC_ERR_ARG;
+            }
+            continue;  // a sub-grid takes the mother's symmetry
         }
         if ((gd->lgSymmetricXYZ ? 1 : 0) != (gds->lgSymmetricXYZ ? 1 : 0) ||
             (gd->lgPlaneIonization ? 1 : 0) != (gds->lgPlaneIonization ? 1 : 0)) {
@@ -484,7 +549,11 @@ int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who
             return MOC_ERR_ARG;
         }
     }
-    if (nG > 1 && ctx->nStars != nG) {
+    if (nested && ctx->nStars != 1) {
+        ctx->err = w + ": nested sub-grids take a single source";
+        return MOC_ERR_ARG;
+    }
+    if (nG > 1 && !nested && ctx->nStars != nG) {
         ctx->err = w + ": a batch of " + std::to_string(nG) + " grids needs one star per grid (" +
                    std::to_string(ctx->nStars) + " stars)";
         return MOC_ERR_ARG;
@@ -496,6 +565,7 @@ int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who
     free_dust(ctx->dust);  // its per-cell arrays belonged to the old grid
     MocGrid& g = ctx->grid;
     g.nGrids = nG;
+    g.nested = nested ? 1 : 0;
     g.nx = gds->nx; g.ny = gds->ny; g.nz = gds->nz;
     g.nCells = (int)total;
     g.sym = gds->lgSymmetricXYZ ? 1 : 0;
@@ -517,8 +587,11 @@ int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who
         g.ax.insert(g.ax.end(), ax.begin(), ax.end());
         g.ay.insert(g.ay.end(), ay.begin(), ay.end());
         g.az.insert(g.az.end(), az.begin(), az.end());
-        const std::vector<double> wx = widths(ax, g.sym, g.plane), wy = widths(ay, g.sym, g.plane),
-                                  wz = widths(az, g.sym, g.plane);
+        // a sub-grid's end cells are half-width on every face (photon_mod.f90:2169-2223, gP > 1)
+        const bool half = nested && k > 0;
+        const std::vector<double> wx = widths(ax, g.sym || half, g.plane || half),
+                                  wy = widths(ay, g.sym || half, g.plane || half),
+                                  wz = widths(az, g.sym || half, g.plane || half);
         vx.insert(vx.end(), wx.begin(), wx.end());
         vy.insert(vy.end(), wy.begin(), wy.end());
         vz.insert(vz.end(), wz.begin(), wz.end());
@@ -528,18 +601,21 @@ int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who
         bool dense = true;
         for (size_t i = 0; i < n3; ++i) {
             const int id = gd->active[i];
-            if (id < 0 || id > gd->nCells) {
-                ctx->err = w + ": active ids must lie in [0, nCells] (no sub-grids)";
+            // the mother of nested grids marks the cells of sub-grid g as -g (fillGrid, grid_mod.f90:907-960)
+            const bool subCell = nested && k == 0 && id <= -2 && id >= -nG;
+            if ((id < 0 && !subCell) || id > gd->nCells) {
+                ctx->err = w + (nested && k == 0 ? ": mother active ids must lie in [0, nCells] or be -g for sub-grid g in 2..nGrids"
+                                                 : ": active ids must lie in [0, nCells] (sub-grids: moc_grid_attach_nested)");
                 free_grid(g);
                 return MOC_ERR_ARG;
             }
             const size_t ix = i % nx, iy = (i / nx) % ny, iz = i / ((size_t)nx * ny);
             if (id != 1 + (int)(iz + nz * (iy + ny * ix))) dense = false;
-            g.active.push_back(id > 0 ? base + id : 0);
+            g.active.push_back(id > 0 ? base + id : (subCell ? id : 0));
             if (id > 0) vol[(size_t)base + id] = wx[ix] * wy[iy] * wz[iz];
         }
         g.gdense.push_back(dense ? 1 : 0);
-        if (nG > 1) {  // star k lights grid k
+        if (nG > 1 && !nested) {  // star k lights grid k
             const int* si = &ctx->starIdx[3 * k];
             if (si[0] < 1 || si[0] > nx || si[1] < 1 || si[1] > ny || si[2] < 1 || si[2] > nz) {
                 ctx->err = w + ": star cell index outside its grid " + std::to_string(k);
@@ -550,6 +626,33 @@ int attach_grids(moc_ctx* ctx, int nG, const moc_grid_desc* gds, const char* who
         base += gd->nCells;
     }
     g.dense = nG == 1 && g.gdense[0];
+    if (nested) {
+        // the source's grid: the mother, or the sub-grid that covers its cell
+        // (setStarPosition, grid_mod.f90:3691-3788)
+        const int* si = ctx->starIdx.data();
+        if (si[0] < 1 || si[0] > g.nx || si[1] < 1 || si[1] > g.ny || si[2] < 1 || si[2] > g.nz) {
+            ctx->err = w + ": star cell index outside the mother grid";
+            free_grid(g);
+            return MOC_ERR_ARG;
+        }
+        const int id = g.active[(size_t)(si[0] - 1) + (size_t)g.nx * ((size_t)(si[1] - 1) + (size_t)g.ny * (si[2] - 1))];
+        g.nestStart = 0;
+        for (int d = 0; d < 3; ++d) g.nestStartIdx[d] = si[d] - 1;
+        if (id < 0) {
+            const int k = -id - 1;
+            g.nestStart = k;
+            const double* axes[3] = {g.ax.data() + g.goffx[k], g.ay.data() + g.goffy[k], g.az.data() + g.goffz[k]};
+            const int nn[3] = {g.gnx[k], g.gny[k], g.gnz[k]};
+            for (int d = 0; d < 3; ++d) {  // locate, then the next cell above the midpoint
+                const double x = ctx->starPos[d];
+                const double* a = axes[d];
+                int p = 0;
+                while (p + 1 < nn[d] && a[p + 1] <= x) ++p;
+                if (p < nn[d] - 1 && x > (a[p] + a[p + 1]) / 2.0) ++p;
+                g.nestStartIdx[d] = p;
+            }
+        }
+    }
     if (nG == 1)
         for (int s = 0; s < ctx->nStars; ++s) {
             const int* si = &ctx->starIdx[3 * s];
@@ -915,6 +1018,9 @@ MocTransportArgs<double> transport_args_f64(moc_ctx* ctx, int s, int64_t n, uint
     a.deltaE = deltaE; a.n = n; a.seed = seed; a.ctr_base = ctr_base; a.ctrDev = nullptr;
     a.emitBatch = ctx->tune.emitBatch; a.segsPerTrip = ctx->tune.segsPerTrip;
     a.nGrids = g.nGrids > 1 ? g.nGrids : 0; a.grids = g.d_view64; a.gridFirst = g.d_gridFirst;
+    a.nested = g.nested; a.nestStart = g.nestStart;
+    a.mir[0] = g.ax.empty() ? 0.0 : g.ax[0]; a.mir[1] = g.ay.empty() ? 0.0 : g.ay[0];
+    a.mir[2] = g.az.empty() ? 0.0 : g.az[0];
     return a;
 }
 
@@ -1014,6 +1120,9 @@ MocTransportArgs<float> transport_args_f32(moc_ctx* ctx, int s, int64_t n, uint6
     a.deltaE = (float)deltaE; a.n = n; a.seed = seed; a.ctr_base = ctr_base; a.ctrDev = nullptr;
     a.emitBatch = ctx->tune.emitBatch; a.segsPerTrip = ctx->tune.segsPerTrip;
     a.nGrids = g.nGrids > 1 ? g.nGrids : 0; a.grids = g.d_view32; a.gridFirst = g.d_gridFirst;
+    a.nested = g.nested; a.nestStart = g.nestStart;
+    a.mir[0] = g.ax.empty() ? 0.f : (float)g.ax[0]; a.mir[1] = g.ay.empty() ? 0.f : (float)g.ay[0];
+    a.mir[2] = g.az.empty() ? 0.f : (float)g.az[0];
     const bool hot = ctx->tune.hotJ && a.nGrids == 0 && g.nHotC > 0 && g.hotBlocks > 0;
     a.nHotC = hot ? g.nHotC : 0; a.nHotB = hot ? g.nHotB : 0;
     for (int k = 0; k < 3; ++k) { a.hotLo[k] = g.hotLo[k]; a.hotN[k] = g.hotN[k]; }
@@ -1082,7 +1191,8 @@ int run_transport(moc_ctx* ctx, int iStar, int64_t n, uint64_t seed, uint32_t ct
     const double deltaE = ctx->Lstar[s] / ctx->nPhotons[s];  // photon_mod.f90:88
     if (!(deltaE >= 0.0)) { ctx->err = "moc_transport: deltaEUsed < 0"; return MOC_ERR_ARG; }
     int rc;
-    if (g.nGrids > 1 && (rc = upload_batch_views(ctx, iStar, &n))) return rc;
+    if (g.nested && (rc = upload_nested_views(ctx, s))) return rc;
+    if (g.batch() && (rc = upload_batch_views(ctx, iStar, &n))) return rc;
     MOC_TRY(hipMemsetAsync(ctx->d_counters, 0, MOC_CNT_N * sizeof(unsigned long long), ctx->stream), "counters");
     MOC_TRY(hipMemsetAsync(ctx->d_Qphot, 0, sizeof(double), ctx->stream), "Qphot");
     float ms = 0.f;
@@ -1766,8 +1876,8 @@ int enqueue_iteration(moc_ctx* ctx, uint64_t seed, int64_t nBatch) {
         g.f32_dirty = true;
         if ((rc = refresh_f32(ctx))) return rc;
     }
-    for (int s = 0; s < (g.nGrids > 1 ? 1 : ctx->nStars); ++s) {
-        const int64_t np = g.nGrids > 1 ? nBatch : (int64_t)ctx->nPhotons[s];
+    for (int s = 0; s < (g.batch() ? 1 : ctx->nStars); ++s) {
+        const int64_t np = g.batch() ? nBatch : (int64_t)ctx->nPhotons[s];
         const double deltaE = ctx->Lstar[s] / ctx->nPhotons[s];  // photon_mod.f90:88
         if (!(deltaE >= 0.0)) { ctx->err = "moc_iterate: deltaEUsed < 0"; return MOC_ERR_ARG; }
         const uint32_t cb = (uint32_t)s + 1u;  // (nIterateMC * 256 from d_ctr) + iStar
@@ -1816,7 +1926,8 @@ int capture_iteration(moc_ctx* ctx, uint64_t seed) {
     if (!dustOnly && (rc = ensure_balance_grid(ctx))) return rc;
     if ((rc = ensure_hot(ctx))) return rc;
     int64_t nBatch = 0;
-    if (g.nGrids > 1 && (rc = upload_batch_views(ctx, 0, &nBatch))) return rc;
+    if (g.batch() && (rc = upload_batch_views(ctx, 0, &nBatch))) return rc;
+    if (g.nested && (rc = upload_nested_views(ctx, 0))) return rc;
     MOC_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "graph capture");
     rc = enqueue_iteration(ctx, seed, nBatch);
     hipGraph_t graph = nullptr;
@@ -1988,7 +2099,7 @@ int moc_iterate(moc_ctx* ctx, int ig, int nIterateMC, uint64_t seed, moc_iterati
     }
     if ((rc = moc_reset_tallies(ctx, ig))) return rc;
     double packets = 0.0, tTr = 0.0, seg = 0.0, emi = 0.0, esc = 0.0;
-    if (g.nGrids > 1) {  // every grid's packets in one launch (RNG packet id (k << 32) + local)
+    if (g.batch()) {  // every grid's packets in one launch (RNG packet id (k << 32) + local)
         moc_transport_stats ts{};
         const uint32_t ctr = (uint32_t)nIterateMC * 256u + 1u;
         if ((rc = run_transport(ctx, 0, 0, seed, ctr, &ts))) return rc;
@@ -1998,7 +2109,7 @@ int moc_iterate(moc_ctx* ctx, int ig, int nIterateMC, uint64_t seed, moc_iterati
         emi = ts.emissions;
         esc = ts.escaped;
     }
-    for (int s = 1; g.nGrids == 1 && s <= ctx->nStars; ++s) {
+    for (int s = 1; !g.batch() && s <= ctx->nStars; ++s) {
         const int64_t n = (int64_t)ctx->nPhotons[s - 1];
         moc_transport_stats ts{};
         const uint32_t ctr = (uint32_t)nIterateMC * 256u + (uint32_t)s;
@@ -2036,7 +2147,7 @@ int moc_set_dust(moc_ctx* ctx, const moc_dust_tables* t) {
     if (!ctx) return MOC_ERR_ARG;
     drop_graph(ctx);
     if (ctx->grid.attached && ctx->grid.nGrids > 1) {
-        ctx->err = "moc_set_dust: grains are not supported in a batch of grids";
+        ctx->err = "moc_set_dust: grains are not supported in a batch of grids or with sub-grids";
         return MOC_ERR_STATE;
     }
     MocGrid& g = ctx->grid;
@@ -2266,7 +2377,7 @@ int moc_volume_averages(moc_ctx* ctx, int ig, double* TeVol, double* ionDenVol) 
         ctx->err = "moc_volume_averages: needs an attached grid and moc_set_atomic";
         return MOC_ERR_STATE;
     }
-    if (g.nGrids > 1) {  // one nebula's output: sums over a batch would mix its grids
+    if (g.batch()) {  // one nebula's output: sums over a batch would mix its grids
         ctx->err = "moc_volume_averages: the context holds a batch of grids; reduce one nebula per context";
         return MOC_ERR_STATE;
     }
@@ -2309,7 +2420,7 @@ int moc_volume_averages_noalloc(moc_ctx* ctx, int ig, double* TeVol, double* ion
         ctx->err = "moc_volume_averages_noalloc: needs an attached grid and moc_set_atomic";
         return MOC_ERR_STATE;
     }
-    if (g.nGrids > 1) {
+    if (g.batch()) {
         ctx->err = "moc_volume_averages_noalloc: the context holds a batch of grids; reduce one nebula per context";
         return MOC_ERR_STATE;
     }
@@ -2350,7 +2461,7 @@ int moc_line_luminosities(moc_ctx* ctx, int ig, int* nIons, int* nLev, int* ionZ
         ctx->err = "moc_line_luminosities: needs an attached grid, moc_set_balance and moc_set_emission";
         return MOC_ERR_STATE;
     }
-    if (g.nGrids > 1) {  // one nebula's output: sums over a batch would mix its grids
+    if (g.batch()) {  // one nebula's output: sums over a batch would mix its grids
         ctx->err = "moc_line_luminosities: the context holds a batch of grids; reduce one nebula per context";
         return MOC_ERR_STATE;
     }
@@ -2420,7 +2531,7 @@ int moc_sed(moc_ctx* ctx, int ig, double* sed, double* totalE) {
     if (!ctx) return MOC_ERR_ARG;
     MocGrid& g = ctx->grid;
     if (!g.attached || ig != 1) { ctx->err = "moc_sed: no grid attached"; return MOC_ERR_STATE; }
-    if (g.nGrids > 1) {  // one nebula's output: sums over a batch would mix its grids
+    if (g.batch()) {  // one nebula's output: sums over a batch would mix its grids
         ctx->err = "moc_sed: the context holds a batch of grids; reduce one nebula per context";
         return MOC_ERR_STATE;
     }

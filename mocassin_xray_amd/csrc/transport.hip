@@ -51,6 +51,7 @@ struct Pk {
     int kcell;       // cell whose opacity in bin nu0 is held in kap (-1: none)
     T kap;
     int hb;          // hot-block slot of bin nu0 (-1: none)
+    int gi;          // view of the grid the packet is in (nested sub-grids; 0 otherwise)
 };
 
 // The J tally of one step: into the workgroup's LDS partial of the hot block
@@ -209,6 +210,61 @@ __device__ __forceinline__ int plane_index(const T* ax, int n, T x) {
     return k;
 }
 
+// a position's cell on one axis of a sub-grid it enters (photon_mod.f90:
+// 1937-1960): locate, at least the first cell, the next one above the midpoint
+template <typename T>
+__device__ __forceinline__ int nest_index(const T* ax, int n, T x) {
+    int k = locate_floor(ax, n, x);
+    if (k < n - 1 && x > (ax[k] + ax[k + 1]) / T(2)) ++k;
+    return k;
+}
+
+// the packet enters the sub-grid of view vi at its position (a mother cell
+// that the sub-grid covers, photon_mod.f90:1931-1963)
+template <typename T>
+__device__ __forceinline__ void nest_enter(const MocTransportArgs<T>& a, View<T>& g, Pk<T>& p, int vi) {
+    g = a.grids[vi];
+    p.gi = vi;
+    p.ix = nest_index(g.ax, g.nx, p.x);
+    p.iy = nest_index(g.ay, g.ny, p.y);
+    p.iz = nest_index(g.az, g.nz, p.z);
+}
+
+// back to the mother grid, in the mother cell that holds the position
+// (photon_mod.f90:3894-3948: locate, at least 1, the next cell from the
+// midpoint on).  Departures from the reference: it writes that midpoint as
+// xAxis(i) + xAxis(i+1)/2, the parenthesis misplaced; and a packet leaving
+// through a sub-grid face, which lies on a mother wall, sits on that wall up
+// to rounding, so the position alone can put it back into a covered cell,
+// where it re-enters the sub-grid and slides along the face.  Here the
+// midpoint itself, on it the cell ahead of the packet, and a return that
+// lands in a covered cell steps on along the axis the packet left by.
+template <typename T>
+__device__ __forceinline__ int leave_index(const T* ax, int n, T x, T dir) {
+    int k = locate_floor(ax, n, x);
+    if (k < n - 1) {
+        const T m = (ax[k] + ax[k + 1]) / T(2);
+        if (x > m || (x == m && dir > T(0))) ++k;
+    }
+    return k;
+}
+template <typename T>
+__device__ __forceinline__ void nest_leave(const MocTransportArgs<T>& a, View<T>& g, Pk<T>& p, int axis) {
+    g = a.grids[0];
+    p.gi = 0;
+    p.ix = leave_index(g.ax, g.nx, p.x, p.u);
+    p.iy = leave_index(g.ay, g.ny, p.y, p.v);
+    p.iz = leave_index(g.az, g.nz, p.z, p.w);
+    int* idx = axis == 0 ? &p.ix : (axis == 1 ? &p.iy : &p.iz);
+    const int n = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
+    const int step = (axis == 0 ? p.u : (axis == 1 ? p.v : p.w)) < T(0) ? -1 : 1;
+    for (int t = 0; t < n && cell_of(g, p.ix, p.iy, p.iz) < 0; ++t) {
+        const int k = *idx + step;
+        if (k < 0 || k > n - 1) break;
+        *idx = k;
+    }
+}
+
 // newPhotonPacket + initPhotonPacket + the escape/line branches of
 // energyPacketRun, then the start of pathSegment (tau = -ln(1-R)).
 template <typename T>
@@ -293,12 +349,197 @@ __device__ __forceinline__ void compton(const MocTransportArgs<T>& a, Pk<T>& p, 
     p.nu = nn;
 }
 
+// the packet interacts inside the cell (photon_mod.f90:2244-2700): it moves
+// to the interaction point (mirrored at the octant's walls in the mother
+// grid), leaves beyond R_out, scatters off grains or free electrons, or is
+// absorbed and re-emitted; the step's J is added in every case
+template <typename T>
+__device__ __forceinline__ void interact(const MocTransportArgs<T>& a, const View<T>& g, Pk<T>& p,
+                                         MocRng& rng, Cnt& c, float* hot, int cell, T kap, size_t row,
+                                         double* Jc, T dV, int hc, bool mirror) {
+    // the packet interacts inside this cell
+    const T dl = (p.passProb - p.absTau) / kap;
+    const double dJ = (double)(dl * g.deltaE / dV);
+    p.x = p.x + dl * p.u;
+    p.y = p.y + dl * p.v;
+    p.z = p.z + dl * p.w;
+    if (mirror) {
+        if (p.x <= g.ax[0]) { if (p.u < T(0)) p.u = -p.u; p.x = g.ax[0]; }
+        if (p.y <= g.ay[0]) { if (p.v < T(0)) p.v = -p.v; p.y = g.ay[0]; }
+        if (p.z <= g.az[0]) { if (p.w < T(0)) p.w = -p.w; p.z = g.az[0]; }
+    }
+    if (g.R_out > T(0)) {
+        // radius with the 1e10 scaling of photon_mod.f90:2294 (no FP32 overflow)
+        const T sx = p.x / T(1.e10), sy = p.y / T(1.e10), sz = p.z / T(1.e10);
+        const T rr = sqrt(sx * sx + sy * sy + sz * sz) * T(1.e10);
+        if (rr >= g.R_out) {
+            add_j(a, hot, Jc, dJ, p.hb, hc);
+            tally_escape(a, g, p, c);
+            p.state = ST_DONE;
+            return;
+        }
+    }
+    if (a.lgDust) {
+        // grains (:2385-2500): scattered with probability scaOpac/kappa,
+        // isotropically; otherwise absorbed and re-emitted (dustEmi
+        // without gas, diffuse with it); no Compton scattering
+        const T pSca = a.scaOpac[row + (size_t)cell] / kap;
+        const T rd = T(1) - rng.uniform<T>();
+        if (rd <= pSca) {
+            ++c.sca;
+            unit_vector(rng, p.u, p.v, p.w);
+            p.origin = cell;
+            p.absTau = T(0);
+            p.passProb = -log(T(1) - rng.uniform<T>());
+            p.kcell = cell;
+            p.kap = kap;
+            add_j(a, hot, Jc, dJ, p.hb, hc);
+            return;
+        }
+        add_j(a, hot, Jc, dJ, p.hb, hc);
+        ++c.absb;
+        if (p.emits >= MOC_RECURSION_LIMIT) {
+            ++c.trap;
+            p.state = ST_DONE;
+            return;
+        }
+        p.stellar = 0;
+        p.state = ST_EMIT;
+        return;
+    }
+    T probSca = T(0);
+    if (a.lgCompton) probSca = a.KNsigmaT[p.nu0] * a.Ne[cell] / kap;
+    const T r = T(1) - rng.uniform<T>();
+    if (r <= probSca) {
+        ++c.sca;
+        compton(a, p, rng);
+        p.origin = cell;
+        p.absTau = T(0);
+        p.passProb = -log(T(1) - rng.uniform<T>());
+        p.kcell = cell;
+        p.kap = a.opacity[(size_t)p.nu0 * (size_t)(a.nCells + 1) + (size_t)cell];
+        add_j(a, hot, Jc, dJ, p.hb, hc);  // the step was taken in the old bin
+        p.hb = hot_bin(a, hot, p.nu0);
+        return;
+    }
+    add_j(a, hot, Jc, dJ, p.hb, hc);
+    // absorbed: re-emitted as a diffuse packet from this position
+    ++c.absb;
+    if (p.emits >= MOC_RECURSION_LIMIT) {
+        ++c.trap;
+        p.state = ST_DONE;
+        return;
+    }
+    p.stellar = 0;
+    p.state = ST_EMIT;
+    return;
+}
+
+// one trip of pathSegment inside a nested sub-grid (gP > 1 in photon_mod.f90:
+// 1758-4038): the sub-grid's walls, none of whose faces drops a packet; the
+// mother's mirror planes in an octant model (the interaction point is not
+// mirrored in a sub-grid, :2254); cell volumes with half-width end cells on
+// every face (:2169-2223); the packet goes back to the mother grid when it
+// leaves the sub-grid's box (:2884-3244, :3612-3948).  No hot J block.
+template <typename T>
+__device__ void segment_sub(const MocTransportArgs<T>& a, View<T>& g, Pk<T>& p, MocRng& rng, Cnt& c) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    if (a.sym) {  // the mother's mirror planes (:1966-1979)
+        if (p.x <= a.mir[0]) { if (p.u < T(0)) p.u = -p.u; p.x = a.mir[0]; }
+        if (p.y <= a.mir[1]) { if (p.v < T(0)) p.v = -p.v; p.y = a.mir[1]; }
+        if (p.z <= a.mir[2]) { if (p.w < T(0)) p.w = -p.w; p.z = a.mir[2]; }
+    }
+    T dSx = T(0), dSy = T(0), dSz = T(0);
+    wall_dist(p.x, p.u, p.ix, g.ax, nx, dSx, true);
+    wall_dist(p.y, p.v, p.iy, g.ay, ny, dSy, true);
+    wall_dist(p.z, p.w, p.iz, g.az, nz, dSz, true);
+    if (tabs(dSx) < T(1.e-5)) dSx = g.ax[nx - 1];
+    if (tabs(dSy) < T(1.e-5)) dSy = g.ay[ny - 1];
+    if (tabs(dSz) < T(1.e-5)) dSz = g.az[nz - 1];
+    dSx = tabs(dSx);
+    dSy = tabs(dSy);
+    dSz = tabs(dSz);
+    T dS;
+    if (dSx <= T(0)) dS = fmin(dSy, dSz);
+    else if (dSy <= T(0)) dS = fmin(dSx, dSz);
+    else if (dSz <= T(0)) dS = fmin(dSx, dSy);
+    else dS = fmin(fmin(dSx, dSy), dSz);
+    if (!(dS > T(0))) {
+        ++c.err;
+        p.state = ST_DONE;
+        return;
+    }
+    const size_t row = (size_t)p.nu0 * (size_t)(a.nCells + 1);
+    const int cell = cell_of(g, p.ix, p.iy, p.iz);  // >= 0: a sub-grid holds no sub-grids
+    const T kap = cell == p.kcell ? p.kap : a.opacity[row + (size_t)cell];
+    const T tauCell = dS * kap;
+    const T dV = g.vx[p.ix] * g.vy[p.iy] * g.vz[p.iz];
+    double* const Jc = a.J + row + (size_t)cell;
+    if (p.absTau + tauCell > p.passProb && cell > 0) {
+        interact(a, g, p, rng, c, (float*)nullptr, cell, kap, row, Jc, dV, -1, false);
+        return;
+    }
+    // the packet crosses the cell
+    const double dJ = (double)(dS * g.deltaE / dV);
+    p.absTau = p.absTau + tauCell;
+    p.x = p.x + dS * p.u;
+    p.y = p.y + dS * p.v;
+    p.z = p.z + dS * p.w;
+    if (dS == dSx && p.u > T(0)) ++p.ix;
+    else if (dS == dSx && p.u < T(0)) --p.ix;
+    else if (dS == dSy && p.v > T(0)) ++p.iy;
+    else if (dS == dSy && p.v < T(0)) --p.iy;
+    else if (dS == dSz && p.w > T(0)) ++p.iz;
+    else if (dS == dSz && p.w < T(0)) --p.iz;
+    const T* ax = g.ax;
+    const T* ay = g.ay;
+    const T* az = g.az;
+    // out through a y face of a full (non-octant) model (:2884-3244)
+    bool out = !a.sym && (p.y < ay[0] || p.iy < 0 || p.y > ay[ny - 1] || p.iy > ny - 1);
+    // beyond an upper face, or at the mother's lower faces (:3612-3624)
+    out = out || (!a.sym && (p.x <= a.mir[0] || p.y <= a.mir[1] || p.z <= a.mir[2])) ||
+          p.x >= ax[nx - 1] || p.y >= ay[ny - 1] || p.z >= az[nz - 1] || p.ix > nx - 1 ||
+          p.iy > ny - 1 || p.iz > nz - 1;
+    if (!out && a.sym) {  // the mother's mirror planes (:3866-3885)
+        if (p.x <= a.mir[0]) { if (p.u < T(0)) p.u = -p.u; p.ix = 0; p.x = ax[0]; }
+        if (p.y <= a.mir[1]) { if (p.v < T(0)) p.v = -p.v; p.iy = 0; p.y = ay[0]; }
+        if (p.z <= a.mir[2]) { if (p.w < T(0)) p.w = -p.w; p.iz = 0; p.z = a.mir[2]; }
+    }
+    // out through any face, moving outwards (:3894-3948)
+    out = out || ((p.x <= ax[0] || p.ix < 0) && p.u <= T(0)) ||
+          ((p.y <= ay[0] || p.iy < 0) && p.v <= T(0)) || ((p.z <= az[0] || p.iz < 0) && p.w <= T(0)) ||
+          ((p.x >= ax[nx - 1] || p.ix > nx - 1) && p.u >= T(0)) ||
+          ((p.y >= ay[ny - 1] || p.iy > ny - 1) && p.v >= T(0)) ||
+          ((p.z >= az[nz - 1] || p.iz > nz - 1) && p.w >= T(0));
+    if (out) {
+        // the axis the packet left by: the index that stepped out of the box,
+        // else the face it sits on moving outwards
+        int axis = 0;
+        if (p.iy < 0 || p.iy > ny - 1) axis = 1;
+        else if (p.iz < 0 || p.iz > nz - 1) axis = 2;
+        else if (p.ix >= 0 && p.ix <= nx - 1) {
+            if (((p.y <= ay[0] && p.v <= T(0)) || (p.y >= ay[ny - 1] && p.v >= T(0)))) axis = 1;
+            else if (((p.z <= az[0] && p.w <= T(0)) || (p.z >= az[nz - 1] && p.w >= T(0)))) axis = 2;
+        }
+        nest_leave(a, g, p, axis);
+        if (g.sym) {  // the mother's own walls after the return (:3866-3885)
+            if (p.x <= g.ax[0]) { if (p.u < T(0)) p.u = -p.u; p.ix = 0; p.x = g.ax[0]; }
+            if (p.y <= g.ay[0]) { if (p.v < T(0)) p.v = -p.v; p.iy = 0; p.y = g.ay[0]; }
+            if (p.z <= g.az[0]) { if (p.w < T(0)) p.w = -p.w; p.iz = 0; p.z = g.az[0]; }
+        }
+    }
+    // the next cell's opacity (none for a mother cell that a sub-grid covers)
+    p.kcell = cell_of(g, p.ix, p.iy, p.iz);
+    if (p.kcell >= 0) p.kap = a.opacity[row + (size_t)p.kcell];
+    if (cell > 0) add_j(a, (float*)nullptr, Jc, dJ, p.hb, -1);
+}
+
 // one trip of the pathSegment loop (photon_mod.f90:1880-4038).  The J
 // contribution of the step is added last, after the opacity of the next cell
 // has been requested: the vector memory counter retires in order, so the next
 // step's wait for that opacity does not include the atomic's round trip.
-template <typename T>
-__device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const View<T>& g, Pk<T>& p,
+template <typename T, bool NEST>
+__device__ __forceinline__ void segment(const MocTransportArgs<T>& a, View<T>& g, Pk<T>& p,
                                         MocRng& rng, Cnt& c, float* hot) {
     if (p.segs >= (g.plane ? MOC_SAFE_LIMIT_PLANE : MOC_SAFE_LIMIT)) {
         ++c.lost;
@@ -307,6 +548,21 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const View
     }
     ++p.segs;
     ++c.seg;
+    if constexpr (NEST) {
+        if (p.gi > 0) {
+            const int nx = g.nx, ny = g.ny, nz = g.nz;
+            nudge(p.x, p.u, p.ix, g.ax, nx);
+            nudge(p.y, p.v, p.iy, g.ay, ny);
+            nudge(p.z, p.w, p.iz, g.az, nz);
+            if (p.ix < 0 || p.ix >= nx || p.iy < 0 || p.iy >= ny || p.iz < 0 || p.iz >= nz) {
+                ++c.err;
+                p.state = ST_DONE;
+                return;
+            }
+            segment_sub(a, g, p, rng, c);
+            return;
+        }
+    }
     const int nx = g.nx, ny = g.ny, nz = g.nz;
 
     nudge(p.x, p.u, p.ix, g.ax, nx);
@@ -316,6 +572,15 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const View
         ++c.err;
         p.state = ST_DONE;
         return;
+    }
+    if constexpr (NEST) {
+        // a mother cell that a sub-grid covers: the step is taken in the sub-grid
+        const int cm = cell_of(g, p.ix, p.iy, p.iz);
+        if (cm < 0) {
+            nest_enter(a, g, p, -cm - 1);
+            segment_sub(a, g, p, rng, c);
+            return;
+        }
     }
     if (g.sym) {  // mirror walls of the octant model
         if (p.x <= g.ax[0]) { if (p.u < T(0)) p.u = -p.u; p.x = g.ax[0]; }
@@ -355,6 +620,13 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const View
     }
 
     const int cell = cell_of(g, p.ix, p.iy, p.iz);
+    if constexpr (NEST) {
+        if (cell < 0) {  // a wall snap moved the packet into a sub-grid's cells (:2083)
+            nest_enter(a, g, p, -cell - 1);
+            segment_sub(a, g, p, rng, c);
+            return;
+        }
+    }
     T kap = kap0;
     if (cell != cell0) kap = a.opacity[row + (size_t)cell];
     const int hc = hot_cell(a, hot, p.ix, p.iy, p.iz);  // the hot slot of this cell
@@ -363,81 +635,7 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const View
     double* const Jc = a.J + row + (size_t)cell;
 
     if (p.absTau + tauCell > p.passProb && cell > 0) {
-        // the packet interacts inside this cell
-        const T dl = (p.passProb - p.absTau) / kap;
-        const double dJ = (double)(dl * g.deltaE / dV);
-        p.x = p.x + dl * p.u;
-        p.y = p.y + dl * p.v;
-        p.z = p.z + dl * p.w;
-        if (g.sym) {
-            if (p.x <= g.ax[0]) { if (p.u < T(0)) p.u = -p.u; p.x = g.ax[0]; }
-            if (p.y <= g.ay[0]) { if (p.v < T(0)) p.v = -p.v; p.y = g.ay[0]; }
-            if (p.z <= g.az[0]) { if (p.w < T(0)) p.w = -p.w; p.z = g.az[0]; }
-        }
-        if (g.R_out > T(0)) {
-            // radius with the 1e10 scaling of photon_mod.f90:2294 (no FP32 overflow)
-            const T sx = p.x / T(1.e10), sy = p.y / T(1.e10), sz = p.z / T(1.e10);
-            const T rr = sqrt(sx * sx + sy * sy + sz * sz) * T(1.e10);
-            if (rr >= g.R_out) {
-                add_j(a, hot, Jc, dJ, p.hb, hc);
-                tally_escape(a, g, p, c);
-                p.state = ST_DONE;
-                return;
-            }
-        }
-        if (a.lgDust) {
-            // grains (:2385-2500): scattered with probability scaOpac/kappa,
-            // isotropically; otherwise absorbed and re-emitted (dustEmi
-            // without gas, diffuse with it); no Compton scattering
-            const T pSca = a.scaOpac[row + (size_t)cell] / kap;
-            const T rd = T(1) - rng.uniform<T>();
-            if (rd <= pSca) {
-                ++c.sca;
-                unit_vector(rng, p.u, p.v, p.w);
-                p.origin = cell;
-                p.absTau = T(0);
-                p.passProb = -log(T(1) - rng.uniform<T>());
-                p.kcell = cell;
-                p.kap = kap;
-                add_j(a, hot, Jc, dJ, p.hb, hc);
-                return;
-            }
-            add_j(a, hot, Jc, dJ, p.hb, hc);
-            ++c.absb;
-            if (p.emits >= MOC_RECURSION_LIMIT) {
-                ++c.trap;
-                p.state = ST_DONE;
-                return;
-            }
-            p.stellar = 0;
-            p.state = ST_EMIT;
-            return;
-        }
-        T probSca = T(0);
-        if (a.lgCompton) probSca = a.KNsigmaT[p.nu0] * a.Ne[cell] / kap;
-        const T r = T(1) - rng.uniform<T>();
-        if (r <= probSca) {
-            ++c.sca;
-            compton(a, p, rng);
-            p.origin = cell;
-            p.absTau = T(0);
-            p.passProb = -log(T(1) - rng.uniform<T>());
-            p.kcell = cell;
-            p.kap = a.opacity[(size_t)p.nu0 * (size_t)(a.nCells + 1) + (size_t)cell];
-            add_j(a, hot, Jc, dJ, p.hb, hc);  // the step was taken in the old bin
-            p.hb = hot_bin(a, hot, p.nu0);
-            return;
-        }
-        add_j(a, hot, Jc, dJ, p.hb, hc);
-        // absorbed: re-emitted as a diffuse packet from this position
-        ++c.absb;
-        if (p.emits >= MOC_RECURSION_LIMIT) {
-            ++c.trap;
-            p.state = ST_DONE;
-            return;
-        }
-        p.stellar = 0;
-        p.state = ST_EMIT;
+        interact(a, g, p, rng, c, hot, cell, kap, row, Jc, dV, hc, g.sym != 0);
         return;
     }
 
@@ -485,8 +683,9 @@ __device__ __forceinline__ void segment(const MocTransportArgs<T>& a, const View
             if (p.z <= g.az[0] || p.iz < 0) { if (p.w < T(0)) p.w = -p.w; p.iz = 0; p.z = g.az[0]; }
         }
         // every index is inside the grid here: request the next cell's opacity
+        // (none for a cell that a sub-grid covers)
         p.kcell = cell_of(g, p.ix, p.iy, p.iz);
-        p.kap = a.opacity[row + (size_t)p.kcell];
+        if (!NEST || p.kcell >= 0) p.kap = a.opacity[row + (size_t)p.kcell];
     }
     if (cell > 0) add_j(a, hot, Jc, dJ, p.hb, hc);
     if (esc) {
@@ -519,10 +718,12 @@ __device__ __forceinline__ double wave_sum(double v) {
 extern __shared__ __align__(16) unsigned char transport_lds[];
 
 // BATCH: the packets of a batch of grids (a.nGrids > 0), each in its own grid.
+// NEST (with BATCH): one source in a mother grid with nested sub-grids
+// (a.nested), the packets moving between the grids' views.
 // HOT: workgroups of 1024 lanes (one per CU) sum the tallies of the hot block
 // of J keys (a.nHotC x a.nHotB) as FP32 partials in LDS, after the staged
 // geometry, and write them to their row of a.hotPart at the end.
-template <typename T, bool BATCH, bool HOT>
+template <typename T, bool BATCH, bool HOT, bool NEST>
 __global__ void __launch_bounds__(HOT ? 1024 : 256) k_transport(MocTransportArgs<T> a) {
     View<T> g;
     float* hot = nullptr;
@@ -554,6 +755,7 @@ __global__ void __launch_bounds__(HOT ? 1024 : 256) k_transport(MocTransportArgs
     p.state = ST_DONE;
     p.kcell = -1;
     p.hb = -1;
+    p.gi = 0;
     MocRng rng;
     bool more = true;
     const int lane = threadIdx.x & 63;
@@ -570,7 +772,18 @@ __global__ void __launch_bounds__(HOT ? 1024 : 256) k_transport(MocTransportArgs
             if (p.state == ST_DONE && more) {
                 const unsigned long long id = base + (unsigned long long)__popcll(want & below);
                 if ((int64_t)id < a.n) {
-                    if constexpr (BATCH) {
+                    if constexpr (NEST) {
+                        // the source's grid: the mother or the sub-grid around it
+                        rng.init(a.seed, id, ctrBase);
+                        g = a.grids[a.nestStart];
+                        p.gi = a.nestStart;
+                        p.x = g.star[0];
+                        p.y = g.star[1];
+                        p.z = g.star[2];
+                        p.ix = g.starIdx[0];
+                        p.iy = g.starIdx[1];
+                        p.iz = g.starIdx[2];
+                    } else if constexpr (BATCH) {
                         // the grid of this packet: last k with gridFirst[k] <= id
                         int lo = 0, hi = a.nGrids;
                         while (hi - lo > 1) {
@@ -611,7 +824,7 @@ __global__ void __launch_bounds__(HOT ? 1024 : 256) k_transport(MocTransportArgs
         }
 #pragma unroll 1
         for (int k = 0; k < segsPerTrip; ++k)
-            if (p.state == ST_FLY) segment(a, g, p, rng, c, hot);
+            if (p.state == ST_FLY) segment<T, NEST>(a, g, p, rng, c, hot);
     }
     if constexpr (HOT) {  // every wave of the workgroup is done: its partials out
         __syncthreads();
@@ -764,12 +977,14 @@ int launch(moc_ctx* ctx, const MocTransportArgs<T>& a, float* ms) {
         return MOC_ERR_STATE;
     }
     if (ms) (void)hipEventRecord(ctx->ev0, ctx->stream);
-    if (a.nGrids > 0)
-        hipLaunchKernelGGL((k_transport<T, true, false>), dim3(blocks), dim3(block), lds, ctx->stream, a);
+    if (a.nGrids > 0 && a.nested)
+        hipLaunchKernelGGL((k_transport<T, true, false, true>), dim3(blocks), dim3(block), lds, ctx->stream, a);
+    else if (a.nGrids > 0)
+        hipLaunchKernelGGL((k_transport<T, true, false, false>), dim3(blocks), dim3(block), lds, ctx->stream, a);
     else if (hot)
-        hipLaunchKernelGGL((k_transport<T, false, kF32>), dim3(blocks), dim3(block), lds, ctx->stream, a);
+        hipLaunchKernelGGL((k_transport<T, false, kF32, false>), dim3(blocks), dim3(block), lds, ctx->stream, a);
     else
-        hipLaunchKernelGGL((k_transport<T, false, false>), dim3(blocks), dim3(block), lds, ctx->stream, a);
+        hipLaunchKernelGGL((k_transport<T, false, false, false>), dim3(blocks), dim3(block), lds, ctx->stream, a);
     if (hot) {
         // the partials into J, then the hot bins of the next launch from this J
         const int nHot = a.nHotC * a.nHotB;

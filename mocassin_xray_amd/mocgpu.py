@@ -277,6 +277,16 @@ class MocGPU:
                     "moc_grid_attach_batch")
         self.grid = BatchGrid(list(grids))
 
+    def grid_attach_nested(self, grids: list) -> None:
+        """A mother grid (grids[0], whose active map holds -g for the cells
+        that grid g covers) and its nested sub-grids (moc_grid_attach_nested);
+        the cells are numbered one grid after the other."""
+        self._grid_keep = [grid_arrays(g) for g in grids]
+        arr = (MocGridDesc * len(grids))(*[to_struct(MocGridDesc, d) for d in self._grid_keep])
+        self._check(self._L.moc_grid_attach_nested(self._ctx, len(grids), arr),
+                    "moc_grid_attach_nested")
+        self.grid = BatchGrid(list(grids))
+
     def _count(self, name: str) -> int:
         c1 = self.grid.nCells + 1
         if name == "ionDen":

@@ -90,6 +90,77 @@ def frozen_fields(grid: CartesianGrid, nu: np.ndarray, hden_cell: np.ndarray, R_
     return {"opacity": kap, "Ne": ne, "recPDF": rec, "totalLines": tl}
 
 
+def nested_grids(mother: tuple, hden_mother, subs: list, sym: bool, R_out: float = 0.0):
+    """A mother grid and nested sub-grids (multiGrids).
+
+    mother = (xa, ya, za); hden_mother [nx, ny, nz] or a scalar; subs =
+    [((xa, ya, za), hden)] with each sub-grid inside the mother's box.  A
+    mother cell is covered by sub-grid g (g = 2, 3, ...) when its centre lies
+    inside that sub-grid's box -- strictly, except that in an octant model a
+    centre on a lower face at 0 counts (fillGrid, grid_mod.f90:907-960); the
+    mother's active map holds -g there, and its other cells with n_H > 0 are
+    numbered as setMotherGrid does.  Returns (grids, hcell): the
+    CartesianGrids, mother first, and n_H per cell of all grids in their
+    global numbering (row 0 the dummy cell)."""
+    xa, ya, za = (np.asarray(a, dtype=np.float64) for a in mother)
+    X, Y, Z = np.meshgrid(xa, ya, za, indexing="ij")
+    cover = np.zeros(X.shape, dtype=np.int32)
+    for jg, ((sx, sy, sz), _) in enumerate(subs, start=2):
+        def inside(V, a):
+            lo = (V > a[0]) | ((V >= a[0]) & (a[0] == 0.0)) if sym else (V > a[0])
+            return lo & (V < a[-1])
+        cover[inside(X, sx) & inside(Y, sy) & inside(Z, sz)] = -jg
+    hd = np.broadcast_to(np.asarray(hden_mother, dtype=np.float64), X.shape)
+    act, nc, hm = T.active_map(xa, ya, za, np.where(cover < 0, 0.0, hd))
+    act = np.where(cover < 0, cover, act).astype(np.int32)
+    grids = [CartesianGrid(xa, ya, za, act, nc, sym, R_out)]
+    hcell = [hm]
+    for (sx, sy, sz), hs in subs:
+        sx, sy, sz = (np.asarray(a, dtype=np.float64) for a in (sx, sy, sz))
+        hsub = np.broadcast_to(np.asarray(hs, dtype=np.float64), (sx.size, sy.size, sz.size))
+        a, n, h = T.active_map(sx, sy, sz, np.ascontiguousarray(hsub))
+        grids.append(CartesianGrid(sx, sy, sz, a, n, sym, R_out))
+        hcell.append(h[1:])
+    return grids, np.concatenate(hcell)
+
+
+@dataclass
+class NestedFrozenModel:
+    name: str
+    tables: SourceTables
+    grids: list            # the mother first
+    fields: dict           # over every grid's cells, global numbering
+    n_packets: int
+
+
+def frozen_nested(n: int = 9, sub_n: int = 9, sub_cells: int = 4, nbins: int = 200,
+                  n_packets: int = 20000, R: float = 3.e18, hden: float = 100.,
+                  T_star: float = 4.e4, L36: float = 10.0, R_s: float = 1.6e18,
+                  sym: bool = True, lgCompton: bool = True) -> NestedFrozenModel:
+    """A frozen pure-H model on an octant (or, sym=False, a whole cube of odd
+    n) of n^3 cells with a sub-grid of sub_n^3 points around the star: its box
+    ends on the mother's wall after sub_cells cells from the centre (the star
+    at the origin sits in it), so the sub-grid exactly replaces the mother
+    cells it covers; the Stromgren-like fields of frozen_fields per cell."""
+    if sym:
+        ax = T.symmetric_axes(n, R)
+        hi = (ax[sub_cells - 1] + ax[sub_cells]) / 2.0
+        sax = np.linspace(0.0, hi, sub_n)
+    else:
+        ax = T.full_axes(n, R)
+        c = n // 2
+        hi = (ax[c + sub_cells - 1] + ax[c + sub_cells]) / 2.0
+        sax = np.linspace(-hi, hi, sub_n)
+    grids, hcell = nested_grids((ax, ax, ax), hden, [((sax, sax, sax), hden)], sym)
+    nu = T.log_mesh(nbins, 1.0, 25.0)
+    tabs = source_tables(nu, T_star, L36, n_packets, sym, n, n, n, lgCompton)
+    parts = []
+    for g, h in zip(grids, np.split(hcell[1:], np.cumsum([g.nCells for g in grids])[:-1])):
+        parts.append(frozen_fields(g, nu, np.concatenate([[0.0], h]), R_s, kn_sigma=tabs.KNsigmaT))
+    fields = {k: np.concatenate([parts[0][k][:1]] + [p[k][1:] for p in parts]) for k in parts[0]}
+    return NestedFrozenModel("nested%d+%d" % (n, sub_n), tabs, grids, fields, n_packets)
+
+
 def frozen_hsphere(n: int = 16, nbins: int = 400, n_packets: int = 1_000_000,
                    mesh: str = "reference", R: float = 3.e18, hden: float = 100.,
                    T_star: float = 4.e4, L36: float = 10.0, R_s: float = 2.4e18,

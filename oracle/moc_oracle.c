@@ -118,6 +118,7 @@ typedef struct {
     int xP, yP, zP;    /* 1-based cell indices */
     int origin;        /* active id of the emitting cell */
     int lgStellar, lgLine;
+    int gP;            /* grid: 0 the mother, k sub-grid k (subs[k-1]) */
 } opacket_t;
 
 typedef struct {
@@ -128,6 +129,49 @@ typedef struct {
 #define ACT(in, i, j, k) \
     ((in)->active[((i)-1) + (size_t)(in)->nx * (((j)-1) + (size_t)(in)->ny * ((k)-1))])
 #define PLANE(in, nuP, cell) ((size_t)((nuP)-1) * (size_t)((in)->nCells + 1) + (size_t)(cell))
+
+/* the grid a packet is in: the mother (gP = 0) or a nested sub-grid */
+typedef struct {
+    int nx, ny, nz;
+    const double *xa, *ya, *za;
+    const int32_t* act;
+} ogrid_t;
+
+static ogrid_t grid_of(const oracle_transport_in* in, int gP) {
+    ogrid_t G;
+    if (gP == 0) {
+        G.nx = in->nx; G.ny = in->ny; G.nz = in->nz;
+        G.xa = in->xAxis; G.ya = in->yAxis; G.za = in->zAxis;
+        G.act = in->active;
+    } else {
+        const oracle_subgrid* s = &in->subs[gP - 1];
+        G.nx = s->nx; G.ny = s->ny; G.nz = s->nz;
+        G.xa = s->xAxis; G.ya = s->yAxis; G.za = s->zAxis;
+        G.act = s->active;
+    }
+    return G;
+}
+
+#define GACT(G, i, j, k) \
+    ((G).act[((i)-1) + (size_t)(G).nx * (((j)-1) + (size_t)(G).ny * ((k)-1))])
+
+/* locate in a grid's axis, at least 1, then the next cell above the midpoint:
+ * the cell of a position entering a sub-grid (photon_mod.f90:1937-1960), or
+ * returning to the mother (:3894-3948), where a position on the midpoint
+ * goes to the cell ahead of the packet (dir: its direction on the axis).  The
+ * reference writes the return's midpoint xAxis(i) + xAxis(i+1)/2, its
+ * parenthesis misplaced; and a packet leaving through a sub-grid face sits on
+ * a mother wall up to rounding, so the position alone can put it back into a
+ * covered cell (path_segment then steps on along the axis it left by). */
+static int nest_cell(const double* a, int n, double x, int entering, double dir) {
+    int p = oracle_locate(a, n, x);
+    if (p < 1) p = 1;
+    if (p < n) {
+        const double m = (a[p - 1] + a[p]) / 2.;
+        if (entering ? x > m : (x > m || (x == m && dir > 0.))) p = p + 1;
+    }
+    return p;
+}
 
 static void add_to(double* p, double v) {
 #ifdef _OPENMP
@@ -182,8 +226,12 @@ static void compton_scatter(const oracle_transport_in* in, opacket_t* pk, orng_t
  * absorbed (re-emission requested, reRun=1), 0 otherwise. */
 static int path_segment(const oracle_transport_in* in, oracle_transport_out* out, opacket_t* pk,
                         orng_t* g, ocount_t* cnt) {
-    const int nx = in->nx, ny = in->ny, nz = in->nz;
-    const double *xa = in->xAxis, *ya = in->yAxis, *za = in->zAxis;
+    int gP = pk->gP;
+    ogrid_t G = grid_of(in, gP);
+    int nx = G.nx, ny = G.ny, nz = G.nz;
+    const double *xa = G.xa, *ya = G.ya, *za = G.za;
+#define SETG(k) do { gP = (k); G = grid_of(in, gP); nx = G.nx; ny = G.ny; nz = G.nz; \
+                     xa = G.xa; ya = G.ya; za = G.za; } while (0)
     double rx = pk->x, ry = pk->y, rz = pk->z;
     double vx = pk->u, vy = pk->v, vz = pk->w;
     int xP = pk->xP, yP = pk->yP, zP = pk->zP;
@@ -208,12 +256,20 @@ static int path_segment(const oracle_transport_in* in, oracle_transport_out* out
             cnt->c[ORC_ERRORS] += 1; /* "pathSegment: insanity" stop */
             return 0;
         }
-        if (in->sym) {
-            if (rx <= xa[0]) { if (vx < 0.) vx = -vx; rx = xa[0]; }
-            if (ry <= ya[0]) { if (vy < 0.) vy = -vy; ry = ya[0]; }
-            if (rz <= za[0]) { if (vz < 0.) vz = -vz; rz = za[0]; }
+        for (;;) {
+        /* a mother cell that a sub-grid covers: into the sub-grid (:1931-1963) */
+        if (gP == 0 && in->nSub > 0 && GACT(G, xP, yP, zP) < 0) {
+            SETG(-GACT(G, xP, yP, zP) - 1);
+            xP = nest_cell(xa, nx, rx, 1, vx);
+            yP = nest_cell(ya, ny, ry, 1, vy);
+            zP = nest_cell(za, nz, rz, 1, vz);
         }
-        /* distances from the walls (:1990-2094) */
+        if (in->sym) { /* the mother's mirror planes (:1966-1979) */
+            if (rx <= in->xAxis[0]) { if (vx < 0.) vx = -vx; rx = in->xAxis[0]; }
+            if (ry <= in->yAxis[0]) { if (vy < 0.) vy = -vy; ry = in->yAxis[0]; }
+            if (rz <= in->zAxis[0]) { if (vz < 0.) vz = -vz; rz = in->zAxis[0]; }
+        }
+        /* distances from the walls (:1990-2094); a sub-grid's faces drop no packet */
         if (vx > 0.) {
             if (xP < nx) {
                 dSx = ((xa[xP] + xa[xP - 1]) / 2. - rx) / vx;
@@ -222,7 +278,7 @@ static int path_segment(const oracle_transport_in* in, oracle_transport_out* out
                 dSx = (xa[nx - 1] - rx) / vx;
                 if (fabs(dSx) < 1.e-5) {
                     rx = xa[nx - 1];
-                    if (!in->plane) { cnt->c[ORC_LOST] += 1; return 0; }
+                    if (!in->plane && gP == 0) { cnt->c[ORC_LOST] += 1; return 0; }
                 }
             }
         } else if (vx < 0.) {
@@ -242,7 +298,10 @@ static int path_segment(const oracle_transport_in* in, oracle_transport_out* out
                 if (fabs(dSy) < 1.e-5) { ry = (ya[yP] + ya[yP - 1]) / 2.; yP = yP + 1; }
             } else {
                 dSy = (ya[ny - 1] - ry) / vy;
-                if (fabs(dSy) < 1.e-5) { ry = ya[ny - 1]; cnt->c[ORC_LOST] += 1; return 0; }
+                if (fabs(dSy) < 1.e-5) {
+                    ry = ya[ny - 1];
+                    if (gP == 0) { cnt->c[ORC_LOST] += 1; return 0; }
+                }
             }
         } else if (vy < 0.) {
             if (yP > 1) {
@@ -263,7 +322,7 @@ static int path_segment(const oracle_transport_in* in, oracle_transport_out* out
                 dSz = (za[nz - 1] - rz) / vz;
                 if (fabs(dSz) < 1.e-5) {
                     rz = za[nz - 1];
-                    if (!in->plane) { cnt->c[ORC_LOST] += 1; return 0; }
+                    if (!in->plane && gP == 0) { cnt->c[ORC_LOST] += 1; return 0; }
                 }
             }
         } else if (vz < 0.) {
@@ -281,6 +340,10 @@ static int path_segment(const oracle_transport_in* in, oracle_transport_out* out
             cnt->c[ORC_ERRORS] += 1;
             return 0;
         }
+        /* a wall snap into a sub-grid's cells: again from its walls (:2083) */
+        if (gP == 0 && in->nSub > 0 && GACT(G, xP, yP, zP) < 0) continue;
+        break;
+        }
         /* cells on a wall (:2096-2102), nearest wall (:2104-2137) */
         if (fabs(dSx) < 1.e-5) dSx = xa[nx - 1];
         if (fabs(dSy) < 1.e-5) dSy = ya[ny - 1];
@@ -294,13 +357,13 @@ static int path_segment(const oracle_transport_in* in, oracle_transport_out* out
         else { dS = fmin(dSx, dSy); dS = fmin(dS, dSz); }
         if (!(dS > 0.)) { cnt->c[ORC_ERRORS] += 1; return 0; }
 
-        cell = ACT(in, xP, yP, zP);
+        cell = GACT(G, xP, yP, zP);
         tauCell = dS * in->opacity[PLANE(in, pk->nuP, cell)];
 
         /* volume element in 1e45 cm^3 (:2166-2223); a plane-parallel grid
          * halves the first and the last cell of every axis */
         {
-            const int h0 = in->sym || in->plane, h1 = in->plane;
+            const int h0 = in->sym || in->plane || gP > 0, h1 = in->plane || gP > 0;
             if (xP > 1 && xP < nx) dx = fabs(xa[xP] - xa[xP - 2]) / 2.;
             else if (xP == 1) dx = h0 ? fabs(xa[1] - xa[0]) / 2. : fabs(xa[1] - xa[0]);
             else dx = h1 ? fabs(xa[xP - 1] - xa[xP - 2]) / 2. : fabs(xa[xP - 1] - xa[xP - 2]);
@@ -323,7 +386,7 @@ static int path_segment(const oracle_transport_in* in, oracle_transport_out* out
             rx = rx + dlLoc * vx;
             ry = ry + dlLoc * vy;
             rz = rz + dlLoc * vz;
-            if (in->sym) {
+            if (in->sym && gP == 0) {
                 if (rx <= xa[0]) { if (vx < 0.) vx = -vx; rx = xa[0]; }
                 if (ry <= ya[0]) { if (vy < 0.) vy = -vy; ry = ya[0]; }
                 if (rz <= za[0]) { if (vz < 0.) vz = -vz; rz = za[0]; }
@@ -349,7 +412,7 @@ static int path_segment(const oracle_transport_in* in, oracle_transport_out* out
                 }
                 cnt->c[ORC_ABSORBED] += 1;
                 pk->x = rx; pk->y = ry; pk->z = rz;
-                pk->xP = xP; pk->yP = yP; pk->zP = zP;
+                pk->xP = xP; pk->yP = yP; pk->zP = zP; pk->gP = gP;
                 return 1; /* "dustEmi" without gas, "diffuse" with it */
             }
             probSca = in->lgCompton ? in->KNsigmaT[pk->nuP - 1] * in->Ne[cell] / kap : 0.;
@@ -367,7 +430,7 @@ static int path_segment(const oracle_transport_in* in, oracle_transport_out* out
             }
             cnt->c[ORC_ABSORBED] += 1;
             pk->x = rx; pk->y = ry; pk->z = rz;
-            pk->xP = xP; pk->yP = yP; pk->zP = zP;
+            pk->xP = xP; pk->yP = yP; pk->zP = zP; pk->gP = gP;
             return 1; /* packetType = "diffuse", reRun = 1 (:4076-4082) */
         }
 
@@ -384,6 +447,55 @@ static int path_segment(const oracle_transport_in* in, oracle_transport_out* out
         else if (dS == dSz && vz > 0.) zP = zP + 1;
         else if (dS == dSz && vz < 0.) zP = zP - 1;
 
+        if (gP > 0) { /* in a sub-grid: back to the mother when it leaves the box */
+            const double mx = in->xAxis[0], my = in->yAxis[0], mz = in->zAxis[0];
+            int ret = 0;
+            /* out through a y face of a full model (:2884-3244) */
+            if (!in->sym && (ry < ya[0] || yP < 1 || ry > ya[ny - 1] || yP > ny)) ret = 1;
+            /* beyond an upper face, or at the mother's lower faces (:3612-3624) */
+            if ((!in->sym && (rx <= mx || ry <= my || rz <= mz)) || rx >= xa[nx - 1] ||
+                ry >= ya[ny - 1] || rz >= za[nz - 1] || xP > nx || yP > ny || zP > nz)
+                ret = 1;
+            if (!ret && in->sym) { /* :3866-3885 */
+                if (rx <= mx) { if (vx < 0.) vx = -vx; xP = 1; rx = xa[0]; }
+                if (ry <= my) { if (vy < 0.) vy = -vy; yP = 1; ry = ya[0]; }
+                if (rz <= mz) { if (vz < 0.) vz = -vz; zP = 1; rz = mz; }
+            }
+            /* out through any face, moving outwards (:3894-3948) */
+            if (((rx <= xa[0] || xP < 1) && vx <= 0.) || ((ry <= ya[0] || yP < 1) && vy <= 0.) ||
+                ((rz <= za[0] || zP < 1) && vz <= 0.) || ((rx >= xa[nx - 1] || xP > nx) && vx >= 0.) ||
+                ((ry >= ya[ny - 1] || yP > ny) && vy >= 0.) || ((rz >= za[nz - 1] || zP > nz) && vz >= 0.))
+                ret = 1;
+            if (ret) {
+                /* the axis left by: the index out of the box, else the face
+                 * it sits on moving outwards */
+                int axis = 0, t, *idx, n, step;
+                if (yP < 1 || yP > ny) axis = 1;
+                else if (zP < 1 || zP > nz) axis = 2;
+                else if (xP >= 1 && xP <= nx) {
+                    if ((ry <= ya[0] && vy <= 0.) || (ry >= ya[ny - 1] && vy >= 0.)) axis = 1;
+                    else if ((rz <= za[0] && vz <= 0.) || (rz >= za[nz - 1] && vz >= 0.)) axis = 2;
+                }
+                SETG(0);
+                xP = nest_cell(xa, nx, rx, 0, vx);
+                yP = nest_cell(ya, ny, ry, 0, vy);
+                zP = nest_cell(za, nz, rz, 0, vz);
+                /* a return into a covered cell steps on along that axis */
+                idx = axis == 0 ? &xP : (axis == 1 ? &yP : &zP);
+                n = axis == 0 ? nx : (axis == 1 ? ny : nz);
+                step = (axis == 0 ? vx : (axis == 1 ? vy : vz)) < 0. ? -1 : 1;
+                for (t = 0; t < n && GACT(G, xP, yP, zP) < 0; t++) {
+                    if (*idx + step < 1 || *idx + step > n) break;
+                    *idx += step;
+                }
+                if (in->sym) {
+                    if (rx <= xa[0]) { if (vx < 0.) vx = -vx; xP = 1; rx = xa[0]; }
+                    if (ry <= ya[0]) { if (vy < 0.) vy = -vy; yP = 1; ry = ya[0]; }
+                    if (rz <= za[0]) { if (vz < 0.) vz = -vz; zP = 1; rz = za[0]; }
+                }
+            }
+            continue;
+        }
         if (in->plane) {
             /* plane-parallel slab (:3249-3606): the packet leaves through the
              * lower or upper y face and is mirrored at the x and z faces (the
@@ -458,7 +570,7 @@ static int energy_packet_run(const oracle_transport_in* in, oracle_transport_out
             cnt->qphot += in->deltaE / (2.1799153e-11 * in->nuArray[pk->nuP - 1]);
         if (in->plane) plane_birth(in, pk, g);
     } else {
-        int cell = ACT(in, pk->xP, pk->yP, pk->zP);
+        int cell = GACT(grid_of(in, pk->gP), pk->xP, pk->yP, pk->zP);
         double r;
         if (cell <= 0) { cnt->c[ORC_ERRORS] += 1; return 0; }
         if (!in->dustOnly) { /* dustEmi packets (:1673-1741) are never lines */
@@ -483,7 +595,7 @@ static int energy_packet_run(const oracle_transport_in* in, oracle_transport_out
         pk->v = fabs(pk->v);
         pk->w = fabs(pk->w);
     }
-    pk->origin = ACT(in, pk->xP, pk->yP, pk->zP);
+    pk->origin = GACT(grid_of(in, pk->gP), pk->xP, pk->yP, pk->zP);
     pk->nu = in->nuArray[pk->nuP - 1];
     if (!in->lgDust && pk->nu < in->ionEdge1) { /* :634-719 */
         escape_tally(in, out, pk, cnt);
@@ -520,6 +632,7 @@ int oracle_transport(const oracle_transport_in* in, oracle_transport_out* out) {
             pk.xP = in->starIdx[0];
             pk.yP = in->starIdx[1];
             pk.zP = in->starIdx[2];
+            pk.gP = in->nSub > 0 ? in->startGrid : 0;
             pk.lgStellar = 1;
             for (e = 1; e <= ORC_RECURSION_LIMIT; e++) {
                 reRun = energy_packet_run(in, out, &pk, &g, &cnt);

@@ -52,7 +52,22 @@ typedef struct oracle_transport_in {
     int nthreads;             /* OpenMP threads (<=1: serial) */
     uint64_t id_base;         /* RNG packet id of packet i: id_base + i (a grid of a batch) */
     int plane;                /* plane-parallel illumination (lgPlaneIonization) */
+    /* nested sub-grids (nSub > 0; setSubGrids grid_mod.f90:1834, pathSegment
+     * photon_mod.f90:1931-1963, 2169-2223, 2884-3948): the grid above is the
+     * mother, whose active map holds -g for the cells of sub-grid g
+     * (g = 2..nSub+1 is subs[g-2]); packets start in grid startGrid (0: the
+     * mother, k: subs[k-1]) at starIdx of that grid */
+    int nSub;
+    const struct oracle_subgrid* subs;
+    int startGrid;
 } oracle_transport_in;
+
+/* one nested sub-grid: axes [cm] and active map (x fastest) of global cell ids */
+typedef struct oracle_subgrid {
+    int nx, ny, nz;
+    const double *xAxis, *yAxis, *zAxis;
+    const int32_t* active;
+} oracle_subgrid;
 
 /* counters[] slots */
 enum {

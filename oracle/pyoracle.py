@@ -18,6 +18,11 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
 
+class SubGrid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("xAxis", _dp), ("yAxis", _dp),
+                ("zAxis", _dp), ("active", _ip)]
+
+
 class TransportIn(C.Structure):
     _fields_ = [
         ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nCells", C.c_int), ("sym", C.c_int),
@@ -29,6 +34,7 @@ class TransportIn(C.Structure):
         ("star", C.c_double * 3), ("starIdx", C.c_int32 * 3), ("deltaE", C.c_double),
         ("n", C.c_int64), ("seed", C.c_uint64), ("ctr_base", C.c_uint32), ("nthreads", C.c_int),
         ("id_base", C.c_uint64), ("plane", C.c_int),
+        ("nSub", C.c_int), ("subs", C.POINTER(SubGrid)), ("startGrid", C.c_int),
     ]
 
 
@@ -83,13 +89,37 @@ def locate(xa, x) -> int:
     return lib().oracle_locate(a.ctypes.data_as(_dp), a.size, float(x))
 
 
-def transport(model_tables, grid, fields, n, seed, ctr_base=0, nthreads=1, istar=0, id_base=0):
+def nest_start(grids, star, idx):
+    """The grid (0: the mother) and 1-based cell of a source in nested grids:
+    a mother cell that a sub-grid covers puts it in that sub-grid, at the cell
+    that locate and the midpoint rule give (setStarPosition)."""
+    m = grids[0]
+    cm = int(np.asarray(m.active)[idx[0] - 1, idx[1] - 1, idx[2] - 1])
+    if cm >= 0:
+        return 0, [int(v) for v in idx]
+    g = grids[-cm - 1]
+    out = []
+    for a, x in ((g.xAxis, star[0]), (g.yAxis, star[1]), (g.zAxis, star[2])):
+        a = np.asarray(a, dtype=np.float64)
+        p = max(locate(a, float(x)), 1)
+        if p < a.size and x > (a[p - 1] + a[p]) / 2.0:
+            p += 1
+        out.append(p)
+    return -cm - 1, out
+
+
+def transport(model_tables, grid, fields, n, seed, ctr_base=0, nthreads=1, istar=0, id_base=0,
+              nested=None):
     """Run the oracle transport.  Returns (J[C+1,B], escaped[C+1,B], counters, Qphot).
     With grains, fields holds scaOpac (C+1, B) and dustOnly (bool).  Packet i
-    draws from the RNG stream of packet id id_base + i."""
+    draws from the RNG stream of packet id id_base + i.  nested: a list of
+    grids, the mother first (whose active map holds -g for the cells of grid
+    g), the cells numbered one grid after the other; grid is then ignored."""
     t = model_tables
     B = t.nbins
-    C1 = grid.nCells + 1
+    if nested is not None:
+        grid = nested[0]
+    C1 = (sum(g.nCells for g in nested) if nested is not None else grid.nCells) + 1
     keep = {}
 
     def d(name, a):
@@ -101,7 +131,7 @@ def transport(model_tables, grid, fields, n, seed, ctr_base=0, nthreads=1, istar
     src = np.asarray(t.sourceCDF).reshape(-1, B)[istar]
     deltaE = float(np.atleast_1d(t.Lstar)[istar] / np.atleast_1d(t.nPhotons)[istar])
     inp = TransportIn(
-        nx=len(grid.xAxis), ny=len(grid.yAxis), nz=len(grid.zAxis), nCells=grid.nCells,
+        nx=len(grid.xAxis), ny=len(grid.yAxis), nz=len(grid.zAxis), nCells=C1 - 1,
         sym=int(bool(grid.lgSymmetricXYZ)), xAxis=d("x", grid.xAxis), yAxis=d("y", grid.yAxis),
         zAxis=d("z", grid.zAxis), active=act.ctypes.data_as(_ip), R_out=float(grid.R_out),
         nbins=B, nAngles=np.asarray(t.KNcdf).shape[1], lgCompton=int(bool(t.lgCompton)),
@@ -115,6 +145,21 @@ def transport(model_tables, grid, fields, n, seed, ctr_base=0, nthreads=1, istar
         id_base=int(id_base), plane=int(bool(getattr(grid, "lgPlaneIonization", False))))
     pos = np.asarray(t.starPosition).reshape(-1, 3)[istar]
     idx = np.asarray(t.starIndex).reshape(-1, 3)[istar]
+    if nested is not None:
+        subs = (SubGrid * (len(nested) - 1))()
+        base = nested[0].nCells
+        for k, g in enumerate(nested[1:]):
+            a = np.asarray(g.active, dtype=np.int32).reshape(-1, order="F")
+            ga = np.ascontiguousarray(np.where(a > 0, a + base, 0), dtype=np.int32)
+            keep[f"sub{k}"] = ga
+            subs[k] = SubGrid(nx=len(g.xAxis), ny=len(g.yAxis), nz=len(g.zAxis),
+                              xAxis=d(f"sx{k}", g.xAxis), yAxis=d(f"sy{k}", g.yAxis),
+                              zAxis=d(f"sz{k}", g.zAxis), active=ga.ctypes.data_as(_ip))
+            base += g.nCells
+        keep["subs"] = subs
+        inp.nSub = len(nested) - 1
+        inp.subs = C.cast(subs, C.POINTER(SubGrid))
+        inp.startGrid, idx = nest_start(nested, pos, idx)
     for k in range(3):
         inp.star[k] = float(pos[k])
         inp.starIdx[k] = int(idx[k])
