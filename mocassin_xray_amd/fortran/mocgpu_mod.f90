@@ -219,6 +219,14 @@ module mocgpu_mod
        integer(c_int), value :: nGrids
        type(moc_grid_desc), intent(in) :: grids(*)
      end function moc_grid_attach_batch
+     ! a mother grid and its nested sub-grids (multiGrids)
+     integer(c_int) function moc_grid_attach_nested(ctx, nGrids, grids) &
+          bind(C, name="moc_grid_attach_nested")
+       import :: c_int, c_ptr, moc_grid_desc
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nGrids
+       type(moc_grid_desc), intent(in) :: grids(*)
+     end function moc_grid_attach_nested
      integer(c_int) function moc_upload(ctx, field, ig, host, count, dtype, layout) &
           bind(C, name="moc_upload")
        import :: c_int, c_ptr, c_int64_t
@@ -381,7 +389,7 @@ module mocgpu_mod
        moc_download, moc_set_atomic, moc_opacity, moc_set_balance, moc_set_emission, &
        moc_emission, moc_set_packets, moc_iterate, moc_update, moc_reset_tallies, moc_transport, &
        moc_normalize_j, moc_set_dust, moc_dust_opacity, moc_dust_emission, moc_dust_temperature, &
-       moc_grid_attach_batch, moc_volume_averages, moc_line_luminosities, moc_sed
+       moc_grid_attach_batch, moc_grid_attach_nested, moc_volume_averages, moc_line_luminosities, moc_sed
   public :: moc_check, c_string, put_field, get_field, ionizationDriver, emissionDriver, &
        energyPacketDriver, updateCell, MCIterationDriver, dustDriver, getDustT
 
