@@ -83,19 +83,35 @@ def profiled(workload: str, key: str):
 
 
 def run_path(m, prec: str, steps: int, warmup: int, seed: int = SEED, outputs: bool = True,
-             graph: bool = False):
+             graph: bool = False, tune: bool = True):
     """W warmup and K timed iterations; with graph, iterations from the second
-    on replay one captured graph (captured in the warmup when W >= 2).  After
-    the state is read, one more iteration runs eagerly for the phase times."""
+    on replay one captured graph (captured in the warmup when W >= 2).  With
+    tune, the transport's launch parameters come from the package's
+    autotuner, run on the model after the last warmup iteration (the first
+    with graph; outside the timed region; the parameters change no packet).  After the state is read,
+    one more iteration runs eagerly for the phase times."""
     import torch
     from mocassin_xray_amd import MocGPU
     with MocGPU(m.tables, prec) as g:
         m.load(g)
         g.set_graph(graph)
         tuning = g.get_tuning()
+        tuned = None
         for w in range(1, warmup + 1):
             g.iterateMC(w, seed)
             print(f"bench: {prec} warmup iteration {w} done", file=sys.stderr, flush=True)
+            # after the last warmup iteration (the state the timed ones start
+            # from); with graph replay after the first, so that the capture in
+            # the second keeps the tuned parameters
+            if tune and w == (1 if graph else warmup):
+                from mocassin_xray_amd.autotune import tune_transport
+                npk = int(min(float(m.tables.nPhotons[0]), 2e7))
+                r = tune_transport(g, packets=npk, repeats=2)
+                tuning = g.get_tuning()
+                tuned = {"default_ms": r["default_ms"], "tuned_ms": r["ms"], "packets": npk,
+                         "seconds": r["seconds"]}
+                print(f"bench: {prec} transport tuned {r['best']} ({r['default_ms']:.2f} -> "
+                      f"{r['ms']:.2f} ms per {npk} packets)", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
         stats = []
         t0 = time.perf_counter()
@@ -110,6 +126,7 @@ def run_path(m, prec: str, steps: int, warmup: int, seed: int = SEED, outputs: b
             keys = ["Te", "Ne", "ionDen"]
         state = {k: g.download(k) for k in keys}
         state["tuning"] = tuning
+        state["autotune"] = tuned
         if graph:  # the phases of one more iteration, eager (a graph replay times only the whole)
             g.set_graph(False)
             state["eager_stats"] = g.iterateMC(warmup + steps + 1, seed)
@@ -372,6 +389,8 @@ def main():
                     help="time one path only (for profiler runs)")
     ap.add_argument("--graph", action="store_true",
                     help="replay iterations 2.. as one captured HIP graph (moc_set_graph)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default transport launch parameters (no autotuner run)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -394,7 +413,8 @@ def main():
     for prec in ("mixed", "fp64"):
         if args.only not in (None, prec):
             continue
-        dt, st, state = run_path(m, prec, args.steps, args.warmup, graph=args.graph)
+        dt, st, state = run_path(m, prec, args.steps, args.warmup, graph=args.graph,
+                                 tune=not args.no_tune)
         res[prec] = (dt, st, state)
         print(f"bench: {prec} {args.steps * n / dt:.3e} packets/s, "
               f"{dt / args.steps * 1e3:.1f} ms/iteration", file=sys.stderr, flush=True)
@@ -407,7 +427,7 @@ def main():
         # Monte Carlo noise band: the FP64 path again with another packet
         # stream (seed), the same iterations from the same start
         _, _, s_64b = run_path(m, "fp64", args.steps, args.warmup, seed=SEED + 1, outputs=False,
-                               graph=args.graph)
+                               graph=args.graph, tune=not args.no_tune)
         print("bench: fp64 second-seed run done (noise band)", file=sys.stderr, flush=True)
         band = state_error(m, s_64b, s_64)
     kms = mean(st_mix, "transport_ms")
@@ -450,6 +470,7 @@ def main():
         "graph_replay": args.graph,
         "transport_launch": {k: s_mix["tuning"][k] for k in ("emitBatch", "segsPerTrip",
                                                              "transportWaves", "hotJ")},
+        "transport_autotune": s_mix.get("autotune"),
         "mixed_over_fp64": dt_64 / dt_mix,
         "mixed_vs_fp64_error": state_error(m, s_mix, s_64),
         "mc_noise_band": band,
