@@ -474,9 +474,9 @@ int moc_set_graph(moc_ctx* ctx, int enable);
  * settings differ only in the order of the atomic additions to J. */
 typedef struct moc_tuning {
     int32_t emitBatch;      /* diffuse packets a wave waits for before it emits them
-                               together, 1..64 (32) */
+                               together, 1..64 (16) */
     int32_t segsPerTrip;    /* path segments a flying lane steps per trip through the
-                               wave's loop, 1..64 (32) */
+                               wave's loop, 1..64 (16) */
     int32_t transportWaves; /* resident waves of one transport launch, 64..2^20 (4096:
                                four per SIMD) */
     int32_t graph;          /* moc_get_tuning: 1 when moc_set_graph is on */

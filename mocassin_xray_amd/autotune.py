@@ -11,8 +11,10 @@ line-rich grids re-emit often and favour larger emission batches; thin grids
 favour longer trips; the LDS sums pay where many tallies land near the star.
 
 tune_transport times candidates by coordinate descent (the LDS sums, waves,
-emission batch, then segments per trip), each with the same packets of star 1, and
-leaves the fastest set on the context.  The transport adds to the J and
+emission batch, then segments per trip, in two passes: the parameters
+interact -- on the 32^3 nebula the LDS sums pay only with short trips --
+so the second pass revisits each one at the others' new values), each with
+the same packets of star 1, and leaves the fastest set on the context.  The transport adds to the J and
 escape tallies, so the tuner zeroes them when it is done: run it before an
 iteration (moc_iterate starts by zeroing them too), not between an iteration
 and the reading of its J.
@@ -38,7 +40,7 @@ def _time(g, n: int, seed: int, repeats: int) -> float:
 
 
 def tune_transport(g, packets: int | None = None, seed: int = 77, repeats: int = 2,
-                   store: str | Path | None = None) -> dict:
+                   store: str | Path | None = None, passes: int = 2) -> dict:
     """Tune the transport launch of MocGPU context g on its attached model.
 
     packets: packets of star 1 per timed launch (default: its nPhotons, at
@@ -62,15 +64,19 @@ def tune_transport(g, packets: int | None = None, seed: int = 77, repeats: int =
 
     default_ms = trial(cur)
     best_ms = default_ms
-    for key, values in (("hotJ", HOT_J), ("transportWaves", WAVES), ("emitBatch", EMIT_BATCH),
-                        ("segsPerTrip", SEGS_PER_TRIP)):
-        for v in values:
-            if v == cur[key]:
-                continue
-            cand = {**cur, key: v}
-            ms = trial(cand)
-            if ms < best_ms:
-                best_ms, cur = ms, cand
+    seen = {tuple(sorted(cur.items())): default_ms}
+    for _ in range(max(1, passes)):
+        for key, values in (("hotJ", HOT_J), ("transportWaves", WAVES), ("emitBatch", EMIT_BATCH),
+                            ("segsPerTrip", SEGS_PER_TRIP)):
+            for v in values:
+                if v == cur[key]:
+                    continue
+                cand = {**cur, key: v}
+                sig = tuple(sorted(cand.items()))
+                ms = seen[sig] if sig in seen else trial(cand)
+                seen[sig] = ms
+                if ms < best_ms:
+                    best_ms, cur = ms, cand
     g.set_tuning(**cur)
     g.reset_tallies()
     out = {"best": cur, "ms": best_ms, "default_ms": default_ms, "packets": n,

@@ -518,8 +518,8 @@ struct MocAtomic {
 
 // Runtime tuning parameters (moc_set_tuning)
 struct MocTuning {
-    int emitBatch = 32;         // transport: lanes that wait to emit together
-    int segsPerTrip = 32;       // transport: segments per flying lane and loop trip
+    int emitBatch = 16;         // transport: lanes that wait to emit together
+    int segsPerTrip = 16;       // transport: segments per flying lane and loop trip
     int transportWaves = 4096;  // transport: resident waves (the launch's lanes / 64)
     int hotJ = 1;               // mixed transport: the hot block of J tallies in LDS
 };
