@@ -298,6 +298,13 @@ typedef struct moc_dust_tables {
     const int32_t* grainKind;    /* [nSpecies] the first letter of the species' label:
                                     1 'S' (silicate), 2 'C' (carbon), 0 other (no
                                     distribution: the grain then emits nothing) */
+    /* lgEquivalentTau (iteration_mod.f90:200-203): the first iteration runs
+     * without the grains' opacity, so its escaped SED is the unextinguished
+     * one, and later iterations with it; the equivalent optical depth per bin
+     * is ln(SED_1 / SED) (output_mod.f90:3020-3035).  moc_iterate knows the
+     * iteration; a host that calls the phases one by one says it with
+     * moc_set_qheat_state. */
+    int32_t lgEquivalentTau;
 } moc_dust_tables;
 
 /* Create a context: uploads the mesh and source tables. */

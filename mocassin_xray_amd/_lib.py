@@ -133,7 +133,7 @@ class MocDustTables(C.Structure):
         ("emIntegral", _dp), ("XHILimit", C.c_double), ("grainRadius", _dp), ("grainVn", _dp),
         ("MsurfAtom", _dp), ("ipRyd", _dp), ("lgQHeat", C.c_int32), ("nTbins", C.c_int32),
         ("minaQHeat", C.c_double), ("minConvQHeat", C.c_double), ("Tmax", C.c_double),
-        ("grainKind", _ip),
+        ("grainKind", _ip), ("lgEquivalentTau", C.c_int32),
     ]
 
 

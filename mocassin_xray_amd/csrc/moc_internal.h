@@ -349,6 +349,12 @@ struct MocDust {
     double XHILimit = 0.;
     std::vector<void*> owned;   // device allocations of this table set
     double *d_abun = nullptr, *d_weight = nullptr, *d_tsub = nullptr;
+    // lgEquivalentTau: the first iteration runs without dust opacity (its
+    // escaped SED is the unextinguished one); d_tsub0 holds zero
+    // sublimation temperatures, which leave no grain in the opacity
+    int equivTau = 0;
+    bool firstIter = false;
+    double* d_tsub0 = nullptr;
     double *d_Cabs = nullptr, *d_Csca = nullptr, *d_emInt = nullptr;
     double* d_widFlx = nullptr; // [B]
     int lgPE = 0;               // grain charge, photoelectric heating, gas-grain collisions

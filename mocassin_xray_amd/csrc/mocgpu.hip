@@ -820,8 +820,16 @@ int launch_qheat(moc_ctx* ctx) {
     return moc_launch_qheat(ctx, a);
 }
 
+// the arguments of the dust opacity: with lgEquivalentTau in the first
+// iteration, no grain counts (zero sublimation temperatures)
+MocDustArgs dust_opacity_args(moc_ctx* ctx) {
+    MocDustArgs a = dust_args(ctx);
+    if (ctx->dust.equivTau && ctx->dust.firstIter && ctx->dust.d_tsub0) a.tsub = ctx->dust.d_tsub0;
+    return a;
+}
+
 int launch_dust_opacity(moc_ctx* ctx) {
-    const int rc = moc_launch_dust_opacity(ctx, dust_args(ctx));
+    const int rc = moc_launch_dust_opacity(ctx, dust_opacity_args(ctx));
     ctx->grid.f32_dirty = true;
     return rc;
 }
@@ -1853,7 +1861,7 @@ int enqueue_iteration(moc_ctx* ctx, uint64_t seed, int64_t nBatch) {
     const int64_t C1 = (int64_t)g.nCells + 1, n = C1 * ctx->nbins;
     int rc;
     if (dustOnly) {  // kappa and setDustPDF of every cell
-        if ((rc = moc_launch_dust_opacity(ctx, dust_args(ctx))) ||
+        if ((rc = moc_launch_dust_opacity(ctx, dust_opacity_args(ctx))) ||
             (rc = moc_launch_dust_pdf(ctx, dust_args(ctx))))
             return rc;
     } else {
@@ -1861,7 +1869,7 @@ int enqueue_iteration(moc_ctx* ctx, uint64_t seed, int64_t nBatch) {
         if (!oa.opacity32 && (rc = sync_f64_master(ctx, MOC_F_OPACITY))) return rc;
         if ((rc = moc_launch_opacity(ctx, oa))) return rc;
         if (oa.opacity32) g.op32_master = true;
-        if (ctx->dust.set && (rc = moc_launch_dust_opacity(ctx, dust_args(ctx)))) return rc;
+        if (ctx->dust.set && (rc = moc_launch_dust_opacity(ctx, dust_opacity_args(ctx)))) return rc;
         const MocEmissionArgs ea = emission_args(ctx, true);
         if (!ea.recPDF32 && (rc = sync_f64_master(ctx, MOC_F_RECPDF))) return rc;
         if ((rc = moc_launch_emission(ctx, ea))) return rc;
@@ -2064,6 +2072,7 @@ int moc_iterate(moc_ctx* ctx, int ig, int nIterateMC, uint64_t seed, moc_iterati
     if (!ctx) return MOC_ERR_ARG;
     MocGrid& g = ctx->grid;
     const bool dustOnly = ctx->dust.set && !ctx->dust.lgGas;
+    ctx->dust.firstIter = nIterateMC == 1;  // lgEquivalentTau: no dust opacity in iteration 1
     if (!g.attached || ig != 1 ||
         (!dustOnly && (!ctx->emission.set || !ctx->balance.set || !g.d_ionDen))) {
         ctx->err = "moc_iterate: needs an attached grid and either moc_set_atomic, moc_set_balance and "
@@ -2197,7 +2206,14 @@ int moc_set_dust(moc_ctx* ctx, const moc_dust_tables* t) {
         (rc = dust_put(ctx, (const double*)nullptr, (size_t)(nG + 1) * C1, &d.d_R, "dust R")) ||
         (rc = dust_put(ctx, (const unsigned long long*)nullptr, 2, &d.d_counters, "dust counters")))
         return rc;
+    if (t->lgEquivalentTau) {
+        const std::vector<double> zero((size_t)nS, 0.0);
+        if ((rc = dust_put(ctx, zero.data(), (size_t)nS, &d.d_tsub0, "zero sublimation temperatures")))
+            return rc;
+    }
     d.lgQHeat = t->lgQHeat ? 1 : 0;
+    d.equivTau = t->lgEquivalentTau ? 1 : 0;
+    d.firstIter = false;
     if (d.lgQHeat) {
         // the grains below minaQHeat, species outer and sizes inner
         std::vector<int> qIdx((size_t)nG, -1), qG;
@@ -2301,11 +2317,12 @@ int moc_dust_emission(moc_ctx* ctx, int ig) {
 int moc_set_qheat_state(moc_ctx* ctx, int nIterateMC, double convPercent) {
     if (!ctx) return MOC_ERR_ARG;
     MocDust& d = ctx->dust;
-    if (!d.set || !d.lgQHeat) {
-        ctx->err = "moc_set_qheat_state: moc_set_dust with lgQHeat first";
+    if (!d.set || (!d.lgQHeat && !d.equivTau)) {
+        ctx->err = "moc_set_qheat_state: moc_set_dust with lgQHeat or lgEquivalentTau first";
         return MOC_ERR_STATE;
     }
     d.qActive = nIterateMC > 1 && convPercent > d.minConvQHeat;
+    d.firstIter = nIterateMC == 1;
     return MOC_OK;
 }
 

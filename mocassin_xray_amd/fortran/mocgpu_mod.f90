@@ -105,6 +105,8 @@ module mocgpu_mod
      integer(c_int32_t) :: lgQHeat = 0, nTbins = 0
      real(c_double) :: minaQHeat = 0d0, minConvQHeat = 0d0, Tmax = 0d0
      type(c_ptr) :: grainKind = c_null_ptr
+     ! lgEquivalentTau: iteration 1 without the grains' opacity
+     integer(c_int32_t) :: lgEquivalentTau = 0
   end type moc_dust_tables
 
   type, bind(C), public :: moc_transport_stats

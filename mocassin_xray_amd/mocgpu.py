@@ -175,7 +175,8 @@ def emission_arrays(et) -> dict:
     }
 
 
-def dust_arrays(dt, lgGas: bool, XHILimit: float, xsec=None, qheat: dict | None = None) -> dict:
+def dust_arrays(dt, lgGas: bool, XHILimit: float, xsec=None, qheat: dict | None = None,
+                equivalent_tau: bool = False) -> dict:
     """xsec (the gas cross-section stack) switches on the grain photoelectric
     and gas-grain collision terms (lgPhotoelectric); qheat (keys minaQHeat,
     minConvQHeat, Tmax, nTbins; the reference's defaults 1e-3 um, 99 %,
@@ -190,6 +191,7 @@ def dust_arrays(dt, lgGas: bool, XHILimit: float, xsec=None, qheat: dict | None 
         "minaQHeat": float(q["minaQHeat"]) if q else 0.0,
         "minConvQHeat": float(q["minConvQHeat"]) if q else 0.0, "Tmax": float(q["Tmax"]) if q else 0.0,
         "grainKind": _c(kind, np.int32) if q else None,
+        "lgEquivalentTau": 1 if equivalent_tau else 0,
         "nSpecies": dt.nSpecies, "nSizes": dt.nSizes, "nTemps": int(dt.emIntegral.shape[2]),
         "lgGas": 1 if lgGas else 0, "lgPhotoelectric": 1 if pe else 0,
         "grainAbun": _c(dt.grainAbun), "grainWeight": _c(dt.grainWeight),
@@ -387,14 +389,17 @@ class MocGPU:
 
     # ------------------------------------------------------------------- dust
     def set_dust(self, dt, lgGas: bool, XHILimit: float = 0.05, photoelectric_xsec=None,
-                 qheat: dict | None = None) -> None:
+                 qheat: dict | None = None, equivalent_tau: bool = False) -> None:
         """Grain tables (a dust.DustTables built on the same mesh); after
         grid_attach.  Tdust starts at 50 K, Ndust at 0.  With gas, passing the
         cross-section stack as photoelectric_xsec switches on the grain
         charge, photoelectric heating and gas-grain collisions (after
         set_balance).  qheat (a dict, see dust_arrays) switches on the quantum
-        heating of the grains smaller than its minaQHeat."""
-        self._dust_keep = dust_arrays(dt, lgGas, XHILimit, photoelectric_xsec, qheat)
+        heating of the grains smaller than its minaQHeat; equivalent_tau
+        (lgEquivalentTau) runs the first iteration without the grains'
+        opacity, whose escaped SED is then the unextinguished one."""
+        self._dust_keep = dust_arrays(dt, lgGas, XHILimit, photoelectric_xsec, qheat,
+                                      equivalent_tau)
         t = to_struct(MocDustTables, self._dust_keep)
         self._check(self._L.moc_set_dust(self._ctx, C.byref(t)), "moc_set_dust")
         self.dust_shape = (dt.nSpecies, dt.nSizes)

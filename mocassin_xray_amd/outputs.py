@@ -169,3 +169,26 @@ def write_output_gas(outdir, g, model) -> dict:
             "lineFlux": write_lineflux(outdir, lines, ions, hb, model.bt,
                                        bool(gr.lgSymmetricXYZ), len(model.elems),
                                        planeArea=area)}
+
+
+def equivalent_tau(sed_noext, sed):
+    """The equivalent optical depth per bin (output_mod.f90:3020-3035):
+    ln(SEDnoExt / SED) where both are positive, else -1; SEDnoExt is the
+    escaped SED of the first iteration of an lgEquivalentTau run (no dust
+    opacity), SED that of the current one."""
+    a = np.asarray(sed_noext, dtype=np.float64)
+    b = np.asarray(sed, dtype=np.float64)
+    ok = (a > 0.0) & (b > 0.0)
+    out = np.full(a.shape, -1.0)
+    out[ok] = np.log(a[ok] / b[ok])
+    return out
+
+
+def write_equivalent_tau(path, nu, sed_noext, sed) -> None:
+    """equivalentTau.out: nu [Ryd], lambda [um], the equivalent tau and SEDnoExt
+    per bin, as the reference writes it (output_mod.f90:3023-3035)."""
+    c, fr1ryd = 2.9979250e10, 3.28984e15  # constants_mod.f90:12-13
+    tau = equivalent_tau(sed_noext, sed)
+    with open(path, "w") as f:
+        for v, t, s0 in zip(np.asarray(nu, dtype=np.float64), tau, np.asarray(sed_noext)):
+            f.write(f" {v:.8E} {c / (v * fr1ryd) * 1.e4:.8E} {t:.8E} {s0:.8E}\n")

@@ -600,3 +600,48 @@ def test_xray_grain_tables_switch_regimes():
     dt = D.build_dust(nu, wid, ["sil-dlee"], [1.0], np.array([0.01, 0.25]), np.array([1.0, 1.0]))
     assert np.all(np.isfinite(dt.Cabs)) and np.all(dt.Cabs > 0.0)
     assert np.all(np.isfinite(dt.Csca)) and np.all(dt.Csca >= 0.0)
+
+
+# ---------------------------------------------------------------- equivalent tau
+def test_equivalent_tau_formula(tmp_path):
+    """ln(SEDnoExt / SED) where both are positive, -1 elsewhere
+    (output_mod.f90:3026-3031); the file's wavelength column is c / (nu fr1Ryd)
+    in um: 1 Ryd is 0.0911 um."""
+    from mocassin_xray_amd import outputs
+    a = np.array([2.0, 1.0, 0.0, 3.0])
+    b = np.array([1.0, 1.0, 1.0, 0.0])
+    t = outputs.equivalent_tau(a, b)
+    assert np.allclose(t, [np.log(2.0), 0.0, -1.0, -1.0])
+    outputs.write_equivalent_tau(tmp_path / "equivalentTau.out", np.array([1.0, 2.0, 3.0, 4.0]), a, b)
+    rows = np.loadtxt(tmp_path / "equivalentTau.out")
+    assert rows.shape == (4, 4)
+    assert abs(rows[0, 1] - 0.0911267) < 1e-6 and np.allclose(rows[:, 2], t)
+
+
+@pytest.mark.gpu
+def test_gpu_equivalent_tau_first_iteration_without_dust_opacity():
+    """lgEquivalentTau (iteration_mod.f90:200-203): the first iteration of a
+    dust shell sees no grain opacity -- every packet escapes, the SED is the
+    source's -- and the second sees the grains, so the equivalent tau of the
+    bins the grains absorb is positive."""
+    from mocassin_xray_amd import MocGPU, outputs
+    m = nebula.dust_shell(n=8, nbins=120, n_packets=20000)
+    print("case equivalent tau: dust shell 8^3, 120 bins, 20000 packets", flush=True)
+    with MocGPU(m.tables, "fp64") as g:
+        g.grid_attach(m.grid)
+        g.set_dust(m.dust, lgGas=False, XHILimit=m.XHILimit, equivalent_tau=True)
+        g.upload("Ndust", m.Ndust)
+        s1 = g.iterateMC(1, 5)
+        sed1, tot1 = g.sed()
+        s2 = g.iterateMC(2, 5)
+        sed2, tot2 = g.sed()
+    assert s1["escaped"] == s1["emissions"] and s1["escaped"] > 0
+    L = float(np.atleast_1d(m.tables.Lstar)[0])
+    assert abs(tot1 - L) <= 1e-9 * L               # every packet's energy escapes unabsorbed
+    assert abs(tot2 - tot1) <= 1e-3 * tot1         # grains re-emit what they absorb
+    assert s2["escaped"] < s2["emissions"]         # grains absorb and re-emit
+    tau = outputs.equivalent_tau(sed1, sed2)
+    # the source's strongest bins lose energy to the grains (which re-emit it
+    # at lower frequencies, where tau turns negative)
+    top = np.argsort(sed1)[-10:]
+    assert (tau[top] > 0).all(), tau[top]
