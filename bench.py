@@ -395,7 +395,7 @@ def main():
                     help="run a plain 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the transport shards by packets "
+        print("bench.py: this benchmark measures one GPU only (the transport shards by packets "
               "and the balance by cells, but no multi-GPU path is built); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)

@@ -1,7 +1,7 @@
 /*
  * moc_oracle.h -- CPU restatement (plain C, FP64) of MOCASSIN's Monte Carlo
  * iteration, used ONLY as test infrastructure: the parity tests, the smoke
- * check and the cpu_baseline leg of bench.py call it as the checker.  The
+ * check and the cpu_baseline leg of bench.py call it to check results.  The
  * product (libmocgpu) never links or calls it.
  */
 #ifndef MOC_ORACLE_H

@@ -1,6 +1,6 @@
 /*
  * ORACLE -- test infrastructure only.  Plain-C FP64 restatement of MOCASSIN's
- * grain routines, used by tests/ as the checker of the device path; nothing
+ * grain routines, used by tests/ to check the device path; nothing
  * in the product links it.
  *
  *   oracle_planck_h         getFlux('blackbody')     continuum_mod.f90:358-400

@@ -1,6 +1,6 @@
 /*
  * ORACLE -- test infrastructure only.  Plain-C FP64 restatement of MOCASSIN's
- * output reductions, used by tests/ as the checker of the device path;
+ * output reductions, used by tests/ to check the device path;
  * nothing in the product links it.
  *
  *   oracle_volume_averages  outputGas volume averages   output_mod.f90:642-730

@@ -1,7 +1,7 @@
 """ctypes front end of the ORACLE (liboracle.so) -- test infrastructure only.
 
 Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
-where it is the checker; the product never imports it.
+where it checks the device path; the product never imports it.
 """
 from __future__ import annotations
 
